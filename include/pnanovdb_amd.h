@@ -1,0 +1,347 @@
+/*
+ * pnanovdb_amd.h -- C ABI of libvdbraster.so, the MI355X (gfx950) Gaussian -> NanoVDB voxelizer.
+ *
+ * Two layers:
+ *   1. The reference's function-table ABI for the raster3d path (pnanovdb_get_raster,
+ *      pnanovdb_get_compute, pnanovdb_get_parallel_primitives, pnanovdb_get_grid_build),
+ *      with struct layouts identical to openvdb/nanovdb-editor so existing callers (the
+ *      ctypes pymodule, C++ hosts copying the tables by value) bind without change.
+ *   2. Direct entry points (vdbr_*) on device pointers, used by the Python package on torch
+ *      tensors, by the tests and by the benchmark.
+ * No torch or HIP types appear in any signature; streams travel as void*.
+ */
+#ifndef PNANOVDB_AMD_H
+#define PNANOVDB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- basic types
+ * nanovdb_editor/putil/Reflect.h:30-61, PNanoVDB.h scalar typedefs */
+#ifndef PNANOVDB_ABI
+#define PNANOVDB_ABI
+#endif
+#ifndef PNANOVDB_API
+#ifdef __cplusplus
+#define PNANOVDB_API extern "C" __attribute__((visibility("default")))
+#else
+#define PNANOVDB_API __attribute__((visibility("default")))
+#endif
+#endif
+
+typedef uint32_t pnanovdb_uint32_t;
+typedef int32_t pnanovdb_int32_t;
+typedef uint64_t pnanovdb_uint64_t;
+typedef int64_t pnanovdb_int64_t;
+typedef int32_t pnanovdb_bool_t; /* 0 / 1 */
+#define PNANOVDB_FALSE 0
+#define PNANOVDB_TRUE 1
+
+typedef struct pnanovdb_reflect_data_type_t pnanovdb_reflect_data_type_t;
+
+/* nanovdb_editor/putil/Compute.h:765-771 */
+typedef struct pnanovdb_compute_array_t
+{
+    void* data;
+    pnanovdb_uint64_t element_size;
+    pnanovdb_uint64_t element_count;
+    const char* filepath;
+} pnanovdb_compute_array_t;
+
+/* Opaque handles of the reference ABI. In this library a pnanovdb_compute_buffer_t* is a
+ * vdbr_buffer_t* (device allocation owned by the caller); queues and contexts are opaque. */
+typedef struct pnanovdb_compute_queue_t pnanovdb_compute_queue_t;
+typedef struct pnanovdb_compute_buffer_t pnanovdb_compute_buffer_t;
+typedef struct pnanovdb_compute_texture_t pnanovdb_compute_texture_t;
+typedef struct pnanovdb_compiler_t pnanovdb_compiler_t;
+typedef struct pnanovdb_raster_context_t pnanovdb_raster_context_t;
+typedef struct pnanovdb_raster_gaussian_data_t pnanovdb_raster_gaussian_data_t;
+typedef struct pnanovdb_grid_build_context_t pnanovdb_grid_build_context_t;
+typedef struct pnanovdb_parallel_primitives_context_t pnanovdb_parallel_primitives_context_t;
+typedef struct pnanovdb_camera_mat_t pnanovdb_camera_mat_t;
+typedef struct pnanovdb_raster_shader_params_t pnanovdb_raster_shader_params_t;
+typedef struct pnanovdb_compute_profiler_entry_t pnanovdb_compute_profiler_entry_t;
+typedef void(PNANOVDB_ABI* pnanovdb_profiler_report_t)(void* userdata, pnanovdb_uint64_t capture_id,
+                                                       pnanovdb_uint32_t num_entries,
+                                                       pnanovdb_compute_profiler_entry_t* entries);
+typedef void(PNANOVDB_ABI* pnanovdb_compute_log_print_t)(pnanovdb_uint32_t level, const char* format, ...);
+
+/* nanovdb_editor/putil/Compute.h:734-747 (4 pointers) */
+typedef struct pnanovdb_compute_shader_interface_t
+{
+    const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
+    void* create_shader;
+    void* map_shader_build;
+    void* destroy_shader;
+} pnanovdb_compute_shader_interface_t;
+
+/* nanovdb_editor/putil/Compute.h:566-664 (1 + 33 pointers). This library keeps every slot
+ * NULL: there is no Vulkan device manager; the HIP device is chosen by vdbr_context_create. */
+typedef struct pnanovdb_compute_device_interface_t
+{
+    const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
+    void* create_device_manager;
+    void* destroy_device_manager;
+    void* enumerate_devices;
+    void* create_device;
+    void* destroy_device;
+    void* get_memory_stats;
+    void* create_semaphore;
+    void* destroy_semaphore;
+    void* get_semaphore_external_handle;
+    void* close_semaphore_external_handle;
+    void* get_device_queue;
+    void* get_compute_queue;
+    void* flush;
+    void* get_frame_global_completed;
+    void* wait_for_frame;
+    void* wait_idle;
+    void* get_compute_interface;
+    void* get_compute_context;
+    void* create_swapchain;
+    void* destroy_swapchain;
+    void* resize_swapchain;
+    void* present_swapchain;
+    void* get_swapchain_front_texture;
+    void* create_encoder;
+    void* destroy_encoder;
+    void* present_encoder;
+    void* get_encoder_front_texture;
+    void* map_encoder_data;
+    void* unmap_encoder_data;
+    void* enable_profiler;
+    void* disable_profiler;
+    void* set_resource_min_lifetime;
+    void* get_device_index;
+} pnanovdb_compute_device_interface_t;
+
+typedef struct pnanovdb_compute_t pnanovdb_compute_t;
+
+/* nanovdb_editor/putil/Compute.h:784-853. Implemented: create_array, destroy_array,
+ * map_array, unmap_array, duplicate_array, compute_array_print_range (Compute.cpp:707-760
+ * semantics: zero-filled host storage of max(size, 64 KiB)). Shader and Vulkan entries are NULL. */
+struct pnanovdb_compute_t
+{
+    const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
+    void* module;
+    const pnanovdb_compiler_t* compiler;
+    pnanovdb_compute_shader_interface_t shader_interface;
+    pnanovdb_compute_device_interface_t device_interface;
+    void* load_nanovdb;
+    void* save_nanovdb;
+    void* create_shader_context;
+    void* destroy_shader_context;
+    void* init_shader;
+    void* destroy_shader;
+    void* dispatch_shader;
+    void* dispatch_shader_on_array;
+    void* dispatch_shader_on_nanovdb_array;
+    pnanovdb_compute_array_t*(PNANOVDB_ABI* create_array)(size_t element_size, pnanovdb_uint64_t element_count,
+                                                          const void* data);
+    void(PNANOVDB_ABI* destroy_array)(pnanovdb_compute_array_t* array);
+    void*(PNANOVDB_ABI* map_array)(pnanovdb_compute_array_t* array);
+    void(PNANOVDB_ABI* unmap_array)(pnanovdb_compute_array_t* array);
+    void(PNANOVDB_ABI* compute_array_print_range)(const pnanovdb_compute_t* compute,
+                                                  pnanovdb_compute_log_print_t log_print, const char* name,
+                                                  pnanovdb_compute_array_t* arr, pnanovdb_uint32_t channel_count);
+    void* nanovdb_from_image_rgba8;
+    pnanovdb_compute_array_t*(PNANOVDB_ABI* duplicate_array)(pnanovdb_compute_array_t* array);
+};
+
+/* nanovdb_editor/putil/Raster.h:58-174 (field order is the ABI). raster_gaussian_2d,
+ * raster_file and create_gaussian_data_from_desc are NULL (outside the raster3d path). */
+typedef struct pnanovdb_raster_t pnanovdb_raster_t;
+struct pnanovdb_raster_t
+{
+    const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
+    const pnanovdb_compute_t* compute;
+    pnanovdb_raster_context_t*(PNANOVDB_ABI* create_context)(const pnanovdb_compute_t* compute,
+                                                             pnanovdb_compute_queue_t* queue);
+    void(PNANOVDB_ABI* destroy_context)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                        pnanovdb_raster_context_t* context);
+    pnanovdb_raster_gaussian_data_t*(PNANOVDB_ABI* create_gaussian_data)(
+        const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue, pnanovdb_raster_context_t* context,
+        pnanovdb_compute_array_t* means, pnanovdb_compute_array_t* quaternions, pnanovdb_compute_array_t* scales,
+        pnanovdb_compute_array_t* colors, pnanovdb_compute_array_t* sh_0, pnanovdb_compute_array_t* sh_n,
+        pnanovdb_compute_array_t* opacities, pnanovdb_compute_array_t** shader_params_arrays,
+        pnanovdb_raster_shader_params_t* raster_params);
+    void(PNANOVDB_ABI* upload_gaussian_data)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                             pnanovdb_raster_context_t* context, pnanovdb_raster_gaussian_data_t* data);
+    void(PNANOVDB_ABI* destroy_gaussian_data)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                              pnanovdb_raster_gaussian_data_t* data);
+    void* raster_gaussian_2d;
+    void(PNANOVDB_ABI* raster_gaussian_3d)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                           pnanovdb_raster_context_t* context, float voxel_size,
+                                           pnanovdb_raster_gaussian_data_t* data, pnanovdb_compute_buffer_t* nanovdb_out,
+                                           pnanovdb_uint64_t nanovdb_word_count, void* userdata);
+    pnanovdb_compute_array_t*(PNANOVDB_ABI* raster_to_nanovdb)(
+        const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue, float voxel_size,
+        pnanovdb_compute_array_t* means, pnanovdb_compute_array_t* quaternions, pnanovdb_compute_array_t* scales,
+        pnanovdb_compute_array_t* colors, pnanovdb_compute_array_t* sh_0, pnanovdb_compute_array_t* sh_n,
+        pnanovdb_compute_array_t* opacities, pnanovdb_compute_array_t** shader_params_arrays,
+        pnanovdb_profiler_report_t profiler_report, void* userdata);
+    void* raster_file;
+    pnanovdb_bool_t(PNANOVDB_ABI* raster_to_nanovdb_from_arrays)(pnanovdb_raster_t* raster,
+                                                                 const pnanovdb_compute_t* compute,
+                                                                 pnanovdb_compute_queue_t* queue, float voxel_size,
+                                                                 pnanovdb_compute_array_t** arrays_gaussian,
+                                                                 pnanovdb_uint32_t array_count,
+                                                                 pnanovdb_compute_array_t** out_nanovdb_arr);
+    pnanovdb_bool_t(PNANOVDB_ABI* create_gaussian_data_from_arrays)(
+        pnanovdb_raster_t* raster, const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+        pnanovdb_compute_array_t** arrays_gaussian, pnanovdb_uint32_t array_count,
+        pnanovdb_raster_gaussian_data_t** gaussian_data, pnanovdb_raster_shader_params_t* raster_params,
+        pnanovdb_raster_context_t** raster_context);
+    void* create_gaussian_data_from_desc;
+};
+
+/* nanovdb_editor/putil/ParallelPrimitives.h:22-88. Buffers are vdbr_buffer_t*. Scans are
+ * inclusive; sorts are stable, ascending, LSD over the low key_bit_count bits; key_count 0 is
+ * a no-op. radix_sort_dual_key is NULL (its only caller is the out-of-scope Raster2D path). */
+typedef struct pnanovdb_parallel_primitives_t
+{
+    const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
+    pnanovdb_parallel_primitives_context_t*(PNANOVDB_ABI* create_context)(const pnanovdb_compute_t* compute,
+                                                                          pnanovdb_compute_queue_t* queue);
+    void(PNANOVDB_ABI* destroy_context)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                        pnanovdb_parallel_primitives_context_t* context);
+    void(PNANOVDB_ABI* global_scan)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                    pnanovdb_parallel_primitives_context_t* context, pnanovdb_compute_buffer_t* val_in,
+                                    pnanovdb_compute_buffer_t* val_out, pnanovdb_uint64_t val_count,
+                                    pnanovdb_uint32_t dispatch_count);
+    void(PNANOVDB_ABI* global_scan_uint64)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                           pnanovdb_parallel_primitives_context_t* context,
+                                           pnanovdb_compute_buffer_t* val_in, pnanovdb_compute_buffer_t* val_out,
+                                           pnanovdb_uint64_t val_count, pnanovdb_uint32_t dispatch_count);
+    void(PNANOVDB_ABI* global_scan_max)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                        pnanovdb_parallel_primitives_context_t* context,
+                                        pnanovdb_compute_buffer_t* val_in, pnanovdb_compute_buffer_t* val_out,
+                                        pnanovdb_uint64_t val_count, pnanovdb_uint32_t dispatch_count);
+    void(PNANOVDB_ABI* radix_sort)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                   pnanovdb_parallel_primitives_context_t* context_in,
+                                   pnanovdb_compute_buffer_t* key_inout, pnanovdb_compute_buffer_t* val_inout,
+                                   pnanovdb_uint64_t key_count, pnanovdb_uint64_t buffer_key_count,
+                                   pnanovdb_uint32_t key_bit_count);
+    void* radix_sort_dual_key;
+    void(PNANOVDB_ABI* radix_sort_key64)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                         pnanovdb_parallel_primitives_context_t* context_in,
+                                         pnanovdb_compute_buffer_t* key_inout, pnanovdb_compute_buffer_t* val_inout,
+                                         pnanovdb_uint64_t key_count, pnanovdb_uint64_t buffer_key_count,
+                                         pnanovdb_uint32_t key_bit_count);
+    const pnanovdb_compute_t* compute;
+} pnanovdb_parallel_primitives_t;
+
+/* Getters: nanovdb_editor/putil/Raster.h:197, Compute.h:883, ParallelPrimitives.h:107.
+ * Each returns a writable static table that lives as long as the library. */
+PNANOVDB_API pnanovdb_raster_t* pnanovdb_get_raster(void);
+PNANOVDB_API pnanovdb_compute_t* pnanovdb_get_compute(void);
+PNANOVDB_API pnanovdb_parallel_primitives_t* pnanovdb_get_parallel_primitives(void);
+
+/* ================================================================ direct device API
+ * Return codes: 0 = success, negative = error (message from vdbr_last_error). Calls on one
+ * context are synchronous with respect to the host unless stated otherwise, and run on the
+ * context's stream. One context per host thread at a time. */
+typedef struct vdbr_context_t vdbr_context_t;
+
+/* device buffer handle; also the concrete type behind pnanovdb_compute_buffer_t */
+typedef struct vdbr_buffer_t
+{
+    void* device_ptr;
+    uint64_t size_bytes;
+} vdbr_buffer_t;
+
+PNANOVDB_API vdbr_context_t* vdbr_context_create(int device);
+PNANOVDB_API void vdbr_context_destroy(vdbr_context_t* ctx);
+PNANOVDB_API const char* vdbr_last_error(vdbr_context_t* ctx);
+/* use an external stream (hipStream_t passed as void*); NULL restores the context's own */
+PNANOVDB_API int vdbr_context_set_stream(vdbr_context_t* ctx, void* stream);
+PNANOVDB_API void* vdbr_context_get_stream(vdbr_context_t* ctx);
+PNANOVDB_API int vdbr_synchronize(vdbr_context_t* ctx);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+PNANOVDB_API int vdbr_copy(vdbr_context_t* ctx, void* dst, const void* src, uint64_t bytes, int kind);
+/* bytes of device memory currently held by the context's workspace */
+PNANOVDB_API uint64_t vdbr_workspace_bytes(vdbr_context_t* ctx);
+PNANOVDB_API int vdbr_release_workspace(vdbr_context_t* ctx);
+
+/* Gaussian arrays on the device, float32, reference layouts: means[3N], opacities[N],
+ * quats[4N] (w,x,y,z), scales[3N], colors[3N]. When VDBR_FLAG_RAW_INPUTS is set the arrays
+ * hold opacity logits, unnormalised quaternions, log-scales and sh_0, and are preprocessed in
+ * place exactly as RasterUtils.cpp:31-136 (colors receives 0.28209479*sh0 + 0.5; sh0 may
+ * alias colors). */
+typedef struct vdbr_gaussians_t
+{
+    float* means;
+    float* opacities;
+    float* quats;
+    float* scales;
+    float* colors;
+    const float* sh0;
+    uint64_t count;
+} vdbr_gaussians_t;
+
+#define VDBR_VALUE_RGBA8 0u   /* gaussian_frag_color.slang semantics (packed RGBA8, max) */
+#define VDBR_VALUE_DENSITY 1u /* sum_g o_g exp(-m_g/2), ascending gaussian id, float32 */
+
+#define VDBR_FLAG_RAW_INPUTS 1u
+#define VDBR_FLAG_SPATIAL_ORDER 2u /* nodes in BFS tree order instead of the windowed order */
+
+typedef struct vdbr_raster_params_t
+{
+    float voxel_size;
+    float cull_scale;           /* 0 -> 0.05 (gaussian_frag_color.slang:205-208) */
+    uint32_t value_mode;        /* VDBR_VALUE_* */
+    uint32_t flags;             /* VDBR_FLAG_* */
+    uint32_t prim_batch_size;   /* node-order model, 0 -> 262144 (Raster.cpp:443) */
+    uint32_t ijk_batch_size;    /* 0 -> 8388608 (Raster.cpp:444) */
+    uint32_t max_node_count;    /* 0 -> 2097152 (GridBuild.cpp:222) */
+    uint32_t reserved;
+} vdbr_raster_params_t;
+
+typedef struct vdbr_raster_stats_t
+{
+    uint64_t gaussian_count;
+    uint64_t fragment_count;   /* AABB candidates */
+    uint64_t pair_count;       /* surviving (voxel, gaussian) pairs */
+    uint64_t voxel_count;      /* active voxels */
+    uint64_t grid_size;        /* bytes */
+    uint32_t upper_count;
+    uint32_t lower_count;
+    uint32_t leaf_count;
+    uint32_t window_count;     /* fragment windows of the node-order model */
+    uint32_t key_bits;         /* bits of the sorted pair keys */
+    uint32_t emit_attempts;    /* 1, or 2 when the pair buffer had to grow */
+    float ms_setup, ms_emit, ms_sort, ms_reduce, ms_build, ms_total;
+} vdbr_raster_stats_t;
+
+/* Full raster3d pipeline on device inputs. The node2 grid is written to a context-owned
+ * device buffer (*grid_dev, valid until the next raster call on the context); *grid_size
+ * receives its size in bytes. */
+PNANOVDB_API int vdbr_raster_node2(vdbr_context_t* ctx, const vdbr_gaussians_t* gaussians,
+                                   const vdbr_raster_params_t* params, void** grid_dev, uint64_t* grid_size,
+                                   vdbr_raster_stats_t* stats);
+
+/* The surviving pair list of the last vdbr_raster_node2 call, sorted by voxel then gaussian
+ * id: voxel ijk (int32 x3) and gaussian id per pair, copied to device buffers of the caller.
+ * Returns the pair count through *count (buffers may be NULL to query it). */
+PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap,
+                                 uint64_t* count);
+
+/* Stable LSD radix sort of 64-bit keys over bits [bit_begin, bit_end), optional 32-bit
+ * payload, in place (ParallelPrimitives.cpp:685-869 semantics). */
+PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32_t* vals, uint64_t count,
+                                     uint32_t bit_begin, uint32_t bit_end);
+/* Stable LSD radix sort of 32-bit keys with optional 32-bit payload, in place. */
+PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32_t* vals, uint64_t count,
+                                     uint32_t key_bit_count);
+/* Inclusive scans (ParallelPrimitives.cpp:151-310): op 0 = add u32, 1 = add u64, 2 = max u32. */
+PNANOVDB_API int vdbr_inclusive_scan(vdbr_context_t* ctx, const void* in, void* out, uint64_t count, int op);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,898 @@
+// vdb_abi.hip -- context/workspace management and the exported C ABI of libvdbraster.so:
+// the reference function tables (pnanovdb_get_raster / _compute / _parallel_primitives) and
+// the direct vdbr_* device entry points. No C++ exception crosses the ABI.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "vdb_internal.h"
+
+namespace vdbr
+{
+
+static const size_t PINNED_BYTES = 1u << 20;
+
+void* Context::ws(WsSlot slot, size_t bytes)
+{
+    if (bytes == 0)
+        bytes = 8;
+    if (bytes <= ws_size[slot] && ws_ptr[slot])
+        return ws_ptr[slot];
+    if (ws_ptr[slot])
+    {
+        // earlier kernels on the stream may still read the old allocation
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(ws_ptr[slot]);
+        ws_ptr[slot] = nullptr;
+        ws_size[slot] = 0;
+    }
+    size_t alloc = bytes + bytes / 8; // headroom for growing workloads
+    alloc = (alloc + 255) & ~(size_t)255;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, alloc);
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        e = hipMalloc(&p, (bytes + 255) & ~(size_t)255);
+        alloc = (bytes + 255) & ~(size_t)255;
+    }
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        fail("device allocation of " + std::to_string(bytes) + " bytes failed");
+        return nullptr;
+    }
+    ws_ptr[slot] = p;
+    ws_size[slot] = alloc;
+    if (slot == WS_SORT_STATUS)
+    {
+        (void)hipMemsetAsync(p, 0, alloc, stream);
+        sort_epoch = 0;
+    }
+    return p;
+}
+
+int Context::sort_workspace(uint64_t n, bool with_vals, uint64_t tiles, bool aux, SortWorkspace* out)
+{
+    out->keys_alt = (uint64_t*)ws(aux ? WS_AUX_ALT : WS_SORT_ALT, n * sizeof(uint64_t));
+    out->vals_alt = with_vals ? (uint32_t*)ws(aux ? WS_AUX_VALS_ALT : WS_SORT_VALS_ALT, n * sizeof(uint32_t)) : nullptr;
+    out->status = (uint64_t*)ws(WS_SORT_STATUS, tiles * 256 * sizeof(uint64_t));
+    uint8_t* small = (uint8_t*)ws(WS_SORT_SMALL, 8 * 256 * 8 * 2 + 64);
+    if (!out->keys_alt || (with_vals && !out->vals_alt) || !out->status || !small)
+        return -1;
+    out->hist = (uint64_t*)small;
+    out->starts = (uint64_t*)(small + 8 * 256 * 8);
+    out->tile_counters = (uint32_t*)(small + 8 * 256 * 8 * 2);
+    sort_status_tiles = ws_size[WS_SORT_STATUS] / (256 * sizeof(uint64_t));
+    return 0;
+}
+
+uint32_t Context::next_sort_epoch(uint64_t tiles)
+{
+    (void)tiles;
+    sort_epoch++;
+    if (sort_epoch >= 65535u)
+    {
+        (void)hipMemsetAsync(ws_ptr[WS_SORT_STATUS], 0, ws_size[WS_SORT_STATUS], stream);
+        sort_epoch = 1u;
+    }
+    return sort_epoch;
+}
+
+uint64_t Context::workspace_bytes() const
+{
+    uint64_t s = 0;
+    for (int i = 0; i < WS_COUNT; i++)
+        s += ws_size[i];
+    return s;
+}
+
+void Context::release()
+{
+    (void)hipStreamSynchronize(stream);
+    for (int i = 0; i < WS_COUNT; i++)
+    {
+        if (ws_ptr[i])
+            (void)hipFree(ws_ptr[i]);
+        ws_ptr[i] = nullptr;
+        ws_size[i] = 0;
+    }
+    last_pairs = nullptr;
+    last_pair_count = 0;
+}
+
+static Context* create_context(int device)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (device < 0 || device >= count)
+        return nullptr;
+    if (hipSetDevice(device) != hipSuccess)
+        return nullptr;
+    Context* ctx = new (std::nothrow) Context();
+    if (!ctx)
+        return nullptr;
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess)
+    {
+        delete ctx;
+        return nullptr;
+    }
+    ctx->stream = ctx->own_stream;
+    if (hipHostMalloc((void**)&ctx->pinned, PINNED_BYTES, hipHostMallocDefault) != hipSuccess)
+    {
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return nullptr;
+    }
+    return ctx;
+}
+
+static void destroy_context(Context* ctx)
+{
+    if (!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    ctx->release();
+    if (ctx->pinned)
+        (void)hipHostFree(ctx->pinned);
+    if (ctx->own_stream)
+        (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+// ----------------------------------------------------------------------------- default context
+// The fn-table entry points carry no HIP state of their own: they run on one library context
+// (device 0), serialised by a mutex (the reference raster entry points are synchronous too).
+static std::mutex g_default_mutex;
+static Context* g_default = nullptr;
+
+static Context* default_context()
+{
+    if (!g_default)
+        g_default = create_context(0);
+    return g_default;
+}
+
+} // namespace vdbr
+
+using namespace vdbr;
+
+// ============================================================================= compute arrays
+// Compute.cpp:707-760 semantics
+static pnanovdb_compute_array_t* PNANOVDB_ABI arr_create(size_t element_size, pnanovdb_uint64_t element_count,
+                                                         const void* data)
+{
+    pnanovdb_compute_array_t* a = new (std::nothrow) pnanovdb_compute_array_t();
+    if (!a)
+        return nullptr;
+    a->element_size = element_size;
+    a->element_count = element_count;
+    a->filepath = nullptr;
+    size_t bytes = element_size * element_count;
+    size_t alloc = bytes < 65536u ? 65536u : bytes;
+    a->data = new (std::nothrow) char[alloc];
+    if (!a->data)
+    {
+        delete a;
+        return nullptr;
+    }
+    memset(a->data, 0, 65536u < alloc ? 65536u : alloc);
+    if (data)
+        memcpy(a->data, data, bytes);
+    else
+        memset(a->data, 0, bytes);
+    return a;
+}
+
+static void PNANOVDB_ABI arr_destroy(pnanovdb_compute_array_t* a)
+{
+    if (!a)
+        return;
+    delete[](char*) a->data;
+    a->data = nullptr;
+    delete a;
+}
+
+static void* PNANOVDB_ABI arr_map(pnanovdb_compute_array_t* a)
+{
+    return a ? a->data : nullptr;
+}
+
+static void PNANOVDB_ABI arr_unmap(pnanovdb_compute_array_t* a)
+{
+    (void)a;
+}
+
+static pnanovdb_compute_array_t* PNANOVDB_ABI arr_duplicate(pnanovdb_compute_array_t* a)
+{
+    return a ? arr_create(a->element_size, a->element_count, a->data) : nullptr;
+}
+
+static void PNANOVDB_ABI arr_print_range(const pnanovdb_compute_t* compute, pnanovdb_compute_log_print_t log_print,
+                                         const char* name, pnanovdb_compute_array_t* arr,
+                                         pnanovdb_uint32_t channel_count)
+{
+    (void)compute;
+    if (!arr || !log_print || channel_count == 0)
+        return;
+    if (channel_count > 4u)
+        channel_count = 4u;
+    float mn[4] = { INFINITY, INFINITY, INFINITY, INFINITY }, mx[4] = { -INFINITY, -INFINITY, -INFINITY, -INFINITY };
+    const float* v = (const float*)arr->data;
+    for (uint64_t i = 0; i < arr->element_count; i++)
+    {
+        uint32_t c = (uint32_t)(i % channel_count);
+        mn[c] = fminf(mn[c], v[i]);
+        mx[c] = fmaxf(mx[c], v[i]);
+    }
+    for (uint32_t c = 0; c < channel_count; c++)
+        log_print(1u, "%s[%u] min(%f) max(%f)", name ? name : "array", c, mn[c], mx[c]);
+}
+
+static pnanovdb_compute_t g_compute;
+static bool g_compute_init = false;
+
+static const pnanovdb_compute_t* compute_table()
+{
+    if (!g_compute_init)
+    {
+        memset(&g_compute, 0, sizeof(g_compute));
+        g_compute.create_array = arr_create;
+        g_compute.destroy_array = arr_destroy;
+        g_compute.map_array = arr_map;
+        g_compute.unmap_array = arr_unmap;
+        g_compute.compute_array_print_range = arr_print_range;
+        g_compute.duplicate_array = arr_duplicate;
+        g_compute_init = true;
+    }
+    return &g_compute;
+}
+
+// ============================================================================= raster table
+struct RasterContextImpl
+{
+    int device;
+};
+
+struct GaussianDataImpl
+{
+    uint64_t count;
+    std::vector<float> means, quats, scales, colors, opacities;
+    float cull_scale;
+};
+
+static pnanovdb_raster_context_t* PNANOVDB_ABI r_create_context(const pnanovdb_compute_t* compute,
+                                                                pnanovdb_compute_queue_t* queue)
+{
+    (void)compute;
+    (void)queue;
+    RasterContextImpl* c = new (std::nothrow) RasterContextImpl();
+    if (c)
+        c->device = 0;
+    return (pnanovdb_raster_context_t*)c;
+}
+
+static void PNANOVDB_ABI r_destroy_context(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                           pnanovdb_raster_context_t* context)
+{
+    (void)compute;
+    (void)queue;
+    delete (RasterContextImpl*)context;
+}
+
+static bool copy_floats(std::vector<float>& dst, pnanovdb_compute_array_t* a)
+{
+    if (!a)
+    {
+        dst.clear();
+        return false;
+    }
+    dst.resize(a->element_count);
+    if (a->element_count)
+        memcpy(dst.data(), a->data, a->element_count * sizeof(float));
+    return true;
+}
+
+// shader_params_arrays[gaussian_frag_color] holds {float cull_scale; bool use_default; ...}
+// (gaussian_frag_color.slang:21-28); index 1 in the reference shader enum (Raster.h:212-214)
+static float cull_scale_of(pnanovdb_compute_array_t** shader_params_arrays)
+{
+    if (!shader_params_arrays || !shader_params_arrays[1] || !shader_params_arrays[1]->data)
+        return 0.f;
+    const uint8_t* p = (const uint8_t*)shader_params_arrays[1]->data;
+    float cs;
+    memcpy(&cs, p, 4);
+    uint32_t use_default;
+    memcpy(&use_default, p + 4, 4);
+    return use_default ? 0.f : cs;
+}
+
+static pnanovdb_raster_gaussian_data_t* PNANOVDB_ABI r_create_gaussian_data(
+    const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue, pnanovdb_raster_context_t* context,
+    pnanovdb_compute_array_t* means, pnanovdb_compute_array_t* quaternions, pnanovdb_compute_array_t* scales,
+    pnanovdb_compute_array_t* colors, pnanovdb_compute_array_t* sh_0, pnanovdb_compute_array_t* sh_n,
+    pnanovdb_compute_array_t* opacities, pnanovdb_compute_array_t** shader_params_arrays,
+    pnanovdb_raster_shader_params_t* raster_params)
+{
+    (void)compute;
+    (void)queue;
+    (void)context;
+    (void)sh_0;
+    (void)sh_n;
+    (void)raster_params;
+    if (!means || !quaternions || !scales || !colors || !opacities)
+        return nullptr;
+    GaussianDataImpl* d = new (std::nothrow) GaussianDataImpl();
+    if (!d)
+        return nullptr;
+    d->count = means->element_count / 3u;
+    copy_floats(d->means, means);
+    copy_floats(d->quats, quaternions);
+    copy_floats(d->scales, scales);
+    copy_floats(d->colors, colors);
+    copy_floats(d->opacities, opacities);
+    d->cull_scale = cull_scale_of(shader_params_arrays);
+    if (d->quats.size() < 4 * d->count || d->scales.size() < 3 * d->count || d->colors.size() < 3 * d->count ||
+        d->opacities.size() < d->count)
+    {
+        delete d;
+        return nullptr;
+    }
+    return (pnanovdb_raster_gaussian_data_t*)d;
+}
+
+static void PNANOVDB_ABI r_upload_gaussian_data(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                                pnanovdb_raster_context_t* context, pnanovdb_raster_gaussian_data_t* data)
+{
+    // host copies are uploaded by each raster call
+    (void)compute;
+    (void)queue;
+    (void)context;
+    (void)data;
+}
+
+static void PNANOVDB_ABI r_destroy_gaussian_data(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                                 pnanovdb_raster_gaussian_data_t* data)
+{
+    (void)compute;
+    (void)queue;
+    delete (GaussianDataImpl*)data;
+}
+
+// uploads host arrays (preprocessed or raw) into one device region and runs the pipeline;
+// returns the grid on the host in a vector
+static int run_host(Context* ctx, uint64_t n, const float* means, const float* quats, const float* scales,
+                    const float* colors_or_sh0, const float* opacities, float voxel_size, float cull_scale,
+                    uint32_t flags, std::vector<uint8_t>* grid_out, float* quats_back, float* scales_back,
+                    float* opac_back)
+{
+    uint64_t nn = n > 0 ? n : 1;
+    size_t bytes = nn * (3 + 4 + 3 + 3 + 1 + 3) * sizeof(float);
+    float* dev = (float*)ctx->ws(WS_RAW_COPY, bytes);
+    if (!dev)
+        return -1;
+    float* d_means = dev;
+    float* d_quats = d_means + 3 * nn;
+    float* d_scales = d_quats + 4 * nn;
+    float* d_colors = d_scales + 3 * nn;
+    float* d_opac = d_colors + 3 * nn;
+    float* d_sh0 = d_opac + nn;
+    if (n > 0)
+    {
+        VDBR_CHECK(ctx, hipMemcpyAsync(d_means, means, 3 * n * 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(d_quats, quats, 4 * n * 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(d_scales, scales, 3 * n * 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync((flags & VDBR_FLAG_RAW_INPUTS) ? d_sh0 : d_colors, colors_or_sh0, 3 * n * 4,
+                                       hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(d_opac, opacities, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    vdbr_gaussians_t g;
+    g.means = d_means;
+    g.opacities = d_opac;
+    g.quats = d_quats;
+    g.scales = d_scales;
+    g.colors = d_colors;
+    g.sh0 = d_sh0;
+    g.count = n;
+    vdbr_raster_params_t p;
+    memset(&p, 0, sizeof(p));
+    p.voxel_size = voxel_size;
+    p.cull_scale = cull_scale;
+    p.value_mode = VDBR_VALUE_RGBA8;
+    p.flags = flags;
+    void* grid = nullptr;
+    uint64_t size = 0;
+    if (raster_node2(ctx, &g, &p, &grid, &size, nullptr))
+        return -1;
+    grid_out->resize(size);
+    VDBR_CHECK(ctx, hipMemcpyAsync(grid_out->data(), grid, size, hipMemcpyDeviceToHost, ctx->stream));
+    if (n > 0 && quats_back)
+    {
+        VDBR_CHECK(ctx, hipMemcpyAsync(quats_back, d_quats, 4 * n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(scales_back, d_scales, 3 * n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(opac_back, d_opac, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+static pnanovdb_compute_array_t* make_output(const pnanovdb_compute_t* compute, const std::vector<uint8_t>& grid)
+{
+    const pnanovdb_compute_t* c = (compute && compute->create_array) ? compute : compute_table();
+    uint64_t words = (grid.size() + 3) / 4;
+    pnanovdb_compute_array_t* out = c->create_array(4u, words, nullptr);
+    if (out && !grid.empty())
+        memcpy(out->data, grid.data(), grid.size());
+    return out;
+}
+
+// Raster.cpp:560-672: preprocessed arrays -> node2 grid array, trimmed to the grid size
+static pnanovdb_compute_array_t* PNANOVDB_ABI r_raster_to_nanovdb(
+    const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue, float voxel_size,
+    pnanovdb_compute_array_t* means, pnanovdb_compute_array_t* quaternions, pnanovdb_compute_array_t* scales,
+    pnanovdb_compute_array_t* colors, pnanovdb_compute_array_t* sh_0, pnanovdb_compute_array_t* sh_n,
+    pnanovdb_compute_array_t* opacities, pnanovdb_compute_array_t** shader_params_arrays,
+    pnanovdb_profiler_report_t profiler_report, void* userdata)
+{
+    (void)queue;
+    (void)sh_0;
+    (void)sh_n;
+    (void)profiler_report;
+    (void)userdata;
+    if (!means || !quaternions || !scales || !colors || !opacities)
+        return nullptr;
+    uint64_t n = means->element_count / 3u;
+    if (quaternions->element_count < 4 * n || scales->element_count < 3 * n || colors->element_count < 3 * n ||
+        opacities->element_count < n)
+        return nullptr;
+    std::lock_guard<std::mutex> lock(g_default_mutex);
+    Context* ctx = default_context();
+    if (!ctx)
+        return nullptr;
+    std::vector<uint8_t> grid;
+    if (run_host(ctx, n, (const float*)means->data, (const float*)quaternions->data, (const float*)scales->data,
+                 (const float*)colors->data, (const float*)opacities->data, voxel_size,
+                 cull_scale_of(shader_params_arrays), 0u, &grid, nullptr, nullptr, nullptr))
+    {
+        fprintf(stderr, "libvdbraster: %s\n", ctx->error.c_str());
+        return nullptr;
+    }
+    return make_output(compute, grid);
+}
+
+static void PNANOVDB_ABI r_raster_gaussian_3d(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                              pnanovdb_raster_context_t* context, float voxel_size,
+                                              pnanovdb_raster_gaussian_data_t* data, pnanovdb_compute_buffer_t* nanovdb_out,
+                                              pnanovdb_uint64_t nanovdb_word_count, void* userdata)
+{
+    // nanovdb_out is a vdbr_buffer_t* (device memory of the caller)
+    (void)compute;
+    (void)queue;
+    (void)context;
+    (void)userdata;
+    GaussianDataImpl* d = (GaussianDataImpl*)data;
+    vdbr_buffer_t* out = (vdbr_buffer_t*)nanovdb_out;
+    if (!d || !out || !out->device_ptr)
+        return;
+    std::lock_guard<std::mutex> lock(g_default_mutex);
+    Context* ctx = default_context();
+    if (!ctx)
+        return;
+    std::vector<uint8_t> grid;
+    if (run_host(ctx, d->count, d->means.data(), d->quats.data(), d->scales.data(), d->colors.data(),
+                 d->opacities.data(), voxel_size, d->cull_scale, 0u, &grid, nullptr, nullptr, nullptr))
+    {
+        fprintf(stderr, "libvdbraster: %s\n", ctx->error.c_str());
+        return;
+    }
+    uint64_t cap = nanovdb_word_count * 4u;
+    if (out->size_bytes < cap)
+        cap = out->size_bytes;
+    uint64_t copy = grid.size() < cap ? grid.size() : cap;
+    (void)hipMemset(out->device_ptr, 0, cap);
+    (void)hipMemcpy(out->device_ptr, grid.data(), copy, hipMemcpyHostToDevice);
+}
+
+// RasterUtils.cpp:143-207, 400-415: raw arrays (means, opacities, quats, scales, sh_0, sh_n);
+// the caller's opacity/quaternion/scale arrays receive their preprocessed values in place
+static pnanovdb_bool_t PNANOVDB_ABI r_raster_to_nanovdb_from_arrays(pnanovdb_raster_t* raster,
+                                                                    const pnanovdb_compute_t* compute,
+                                                                    pnanovdb_compute_queue_t* queue, float voxel_size,
+                                                                    pnanovdb_compute_array_t** arrays,
+                                                                    pnanovdb_uint32_t array_count,
+                                                                    pnanovdb_compute_array_t** out)
+{
+    (void)raster;
+    (void)queue;
+    if (array_count < 5u || !arrays || !out)
+        return PNANOVDB_FALSE;
+    for (int i = 0; i < 5; i++)
+        if (!arrays[i] || (!arrays[i]->data && arrays[i]->element_count))
+            return PNANOVDB_FALSE;
+    uint64_t n = arrays[0]->element_count / 3u;
+    if (arrays[1]->element_count < n || arrays[2]->element_count < 4 * n || arrays[3]->element_count < 3 * n ||
+        arrays[4]->element_count < 3 * n)
+        return PNANOVDB_FALSE;
+    std::lock_guard<std::mutex> lock(g_default_mutex);
+    Context* ctx = default_context();
+    if (!ctx)
+        return PNANOVDB_FALSE;
+    std::vector<uint8_t> grid;
+    if (run_host(ctx, n, (const float*)arrays[0]->data, (const float*)arrays[2]->data, (const float*)arrays[3]->data,
+                 (const float*)arrays[4]->data, (const float*)arrays[1]->data, voxel_size, 0.f, VDBR_FLAG_RAW_INPUTS,
+                 &grid, (float*)arrays[2]->data, (float*)arrays[3]->data, (float*)arrays[1]->data))
+    {
+        fprintf(stderr, "libvdbraster: %s\n", ctx->error.c_str());
+        return PNANOVDB_FALSE;
+    }
+    *out = make_output(compute, grid);
+    return *out ? PNANOVDB_TRUE : PNANOVDB_FALSE;
+}
+
+static pnanovdb_bool_t PNANOVDB_ABI r_create_gaussian_data_from_arrays(
+    pnanovdb_raster_t* raster, const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+    pnanovdb_compute_array_t** arrays, pnanovdb_uint32_t array_count, pnanovdb_raster_gaussian_data_t** gaussian_data,
+    pnanovdb_raster_shader_params_t* raster_params, pnanovdb_raster_context_t** raster_context)
+{
+    (void)raster;
+    (void)compute;
+    (void)queue;
+    (void)raster_params;
+    if (array_count < 5u || !arrays || !gaussian_data)
+        return PNANOVDB_FALSE;
+    uint64_t n = arrays[0]->element_count / 3u;
+    GaussianDataImpl* d = new (std::nothrow) GaussianDataImpl();
+    if (!d)
+        return PNANOVDB_FALSE;
+    d->count = n;
+    d->cull_scale = 0.f;
+    copy_floats(d->means, arrays[0]);
+    copy_floats(d->opacities, arrays[1]);
+    copy_floats(d->quats, arrays[2]);
+    copy_floats(d->scales, arrays[3]);
+    d->colors.resize(3 * n);
+    // RasterUtils.cpp:31-136 on the host copies, same arithmetic as the device preprocessing
+    {
+        std::lock_guard<std::mutex> lock(g_default_mutex);
+        Context* ctx = default_context();
+        if (!ctx)
+        {
+            delete d;
+            return PNANOVDB_FALSE;
+        }
+        std::vector<uint8_t> unused;
+        // run the device preprocessing by rasterizing nothing: upload, preprocess, download
+        uint64_t nn = n > 0 ? n : 1;
+        float* dev = (float*)ctx->ws(WS_RAW_COPY, nn * 17 * sizeof(float));
+        if (!dev)
+        {
+            delete d;
+            return PNANOVDB_FALSE;
+        }
+        (void)unused;
+        float* d_quats = dev + 3 * nn;
+        float* d_scales = d_quats + 4 * nn;
+        float* d_colors = d_scales + 3 * nn;
+        float* d_opac = d_colors + 3 * nn;
+        float* d_sh0 = d_opac + nn;
+        if (n > 0)
+        {
+            (void)hipMemcpy(d_quats, d->quats.data(), 4 * n * 4, hipMemcpyHostToDevice);
+            (void)hipMemcpy(d_scales, d->scales.data(), 3 * n * 4, hipMemcpyHostToDevice);
+            (void)hipMemcpy(d_opac, d->opacities.data(), n * 4, hipMemcpyHostToDevice);
+            (void)hipMemcpy(d_sh0, arrays[4]->data, 3 * n * 4, hipMemcpyHostToDevice);
+            if (preprocess_device(ctx, n, d_opac, d_quats, d_scales, d_sh0, d_colors))
+            {
+                delete d;
+                return PNANOVDB_FALSE;
+            }
+            (void)hipMemcpy(d->quats.data(), d_quats, 4 * n * 4, hipMemcpyDeviceToHost);
+            (void)hipMemcpy(d->scales.data(), d_scales, 3 * n * 4, hipMemcpyDeviceToHost);
+            (void)hipMemcpy(d->opacities.data(), d_opac, n * 4, hipMemcpyDeviceToHost);
+            (void)hipMemcpy(d->colors.data(), d_colors, 3 * n * 4, hipMemcpyDeviceToHost);
+        }
+    }
+    *gaussian_data = (pnanovdb_raster_gaussian_data_t*)d;
+    if (raster_context && !*raster_context)
+        *raster_context = r_create_context(compute, queue);
+    return PNANOVDB_TRUE;
+}
+
+static pnanovdb_raster_t g_raster;
+static bool g_raster_init = false;
+
+// ============================================================================= parallel primitives
+static pnanovdb_parallel_primitives_context_t* PNANOVDB_ABI pp_create_context(const pnanovdb_compute_t* compute,
+                                                                             pnanovdb_compute_queue_t* queue)
+{
+    (void)compute;
+    (void)queue;
+    return (pnanovdb_parallel_primitives_context_t*)create_context(0);
+}
+
+static void PNANOVDB_ABI pp_destroy_context(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                            pnanovdb_parallel_primitives_context_t* context)
+{
+    (void)compute;
+    (void)queue;
+    destroy_context((Context*)context);
+}
+
+static void* buf_ptr(pnanovdb_compute_buffer_t* b)
+{
+    return b ? ((vdbr_buffer_t*)b)->device_ptr : nullptr;
+}
+
+static void PNANOVDB_ABI pp_global_scan(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                        pnanovdb_parallel_primitives_context_t* context, pnanovdb_compute_buffer_t* val_in,
+                                        pnanovdb_compute_buffer_t* val_out, pnanovdb_uint64_t val_count,
+                                        pnanovdb_uint32_t dispatch_count)
+{
+    (void)compute;
+    (void)queue;
+    (void)dispatch_count;
+    Context* ctx = (Context*)context;
+    if (!ctx || val_count == 0)
+        return;
+    scan_u32(ctx, (const uint32_t*)buf_ptr(val_in), (uint32_t*)buf_ptr(val_out), val_count, true, nullptr);
+    (void)hipStreamSynchronize(ctx->stream);
+}
+
+static void PNANOVDB_ABI pp_global_scan_uint64(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                               pnanovdb_parallel_primitives_context_t* context,
+                                               pnanovdb_compute_buffer_t* val_in, pnanovdb_compute_buffer_t* val_out,
+                                               pnanovdb_uint64_t val_count, pnanovdb_uint32_t dispatch_count)
+{
+    (void)compute;
+    (void)queue;
+    (void)dispatch_count;
+    Context* ctx = (Context*)context;
+    if (!ctx || val_count == 0)
+        return;
+    scan_u64(ctx, (const uint64_t*)buf_ptr(val_in), (uint64_t*)buf_ptr(val_out), val_count, true, nullptr);
+    (void)hipStreamSynchronize(ctx->stream);
+}
+
+static void PNANOVDB_ABI pp_global_scan_max(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                            pnanovdb_parallel_primitives_context_t* context,
+                                            pnanovdb_compute_buffer_t* val_in, pnanovdb_compute_buffer_t* val_out,
+                                            pnanovdb_uint64_t val_count, pnanovdb_uint32_t dispatch_count)
+{
+    (void)compute;
+    (void)queue;
+    (void)dispatch_count;
+    Context* ctx = (Context*)context;
+    if (!ctx || val_count == 0)
+        return;
+    scan_max_u32(ctx, (const uint32_t*)buf_ptr(val_in), (uint32_t*)buf_ptr(val_out), val_count, true);
+    (void)hipStreamSynchronize(ctx->stream);
+}
+
+static void PNANOVDB_ABI pp_radix_sort(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                       pnanovdb_parallel_primitives_context_t* context,
+                                       pnanovdb_compute_buffer_t* key_inout, pnanovdb_compute_buffer_t* val_inout,
+                                       pnanovdb_uint64_t key_count, pnanovdb_uint64_t buffer_key_count,
+                                       pnanovdb_uint32_t key_bit_count)
+{
+    (void)compute;
+    (void)queue;
+    (void)buffer_key_count;
+    if (!context || key_count == 0)
+        return;
+    Context* ctx = (Context*)context;
+    radix_sort_u32(ctx, (uint32_t*)buf_ptr(key_inout), (uint32_t*)buf_ptr(val_inout), key_count,
+                   key_bit_count > 32 ? 32 : key_bit_count);
+    (void)hipStreamSynchronize(ctx->stream);
+}
+
+static void PNANOVDB_ABI pp_radix_sort_key64(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                             pnanovdb_parallel_primitives_context_t* context,
+                                             pnanovdb_compute_buffer_t* key_inout, pnanovdb_compute_buffer_t* val_inout,
+                                             pnanovdb_uint64_t key_count, pnanovdb_uint64_t buffer_key_count,
+                                             pnanovdb_uint32_t key_bit_count)
+{
+    (void)compute;
+    (void)queue;
+    (void)buffer_key_count;
+    Context* ctx = (Context*)context;
+    if (!ctx || key_count == 0)
+        return;
+    radix_sort_u64(ctx, (uint64_t*)buf_ptr(key_inout), (uint32_t*)buf_ptr(val_inout), key_count, 0,
+                   key_bit_count > 64 ? 64 : key_bit_count, nullptr);
+    (void)hipStreamSynchronize(ctx->stream);
+}
+
+static pnanovdb_parallel_primitives_t g_pp;
+static bool g_pp_init = false;
+
+// ============================================================================= exports
+extern "C" {
+
+PNANOVDB_API pnanovdb_compute_t* pnanovdb_get_compute(void)
+{
+    return (pnanovdb_compute_t*)compute_table();
+}
+
+PNANOVDB_API pnanovdb_raster_t* pnanovdb_get_raster(void)
+{
+    if (!g_raster_init)
+    {
+        memset(&g_raster, 0, sizeof(g_raster));
+        g_raster.compute = compute_table();
+        g_raster.create_context = r_create_context;
+        g_raster.destroy_context = r_destroy_context;
+        g_raster.create_gaussian_data = r_create_gaussian_data;
+        g_raster.upload_gaussian_data = r_upload_gaussian_data;
+        g_raster.destroy_gaussian_data = r_destroy_gaussian_data;
+        g_raster.raster_gaussian_3d = r_raster_gaussian_3d;
+        g_raster.raster_to_nanovdb = r_raster_to_nanovdb;
+        g_raster.raster_to_nanovdb_from_arrays = r_raster_to_nanovdb_from_arrays;
+        g_raster.create_gaussian_data_from_arrays = r_create_gaussian_data_from_arrays;
+        g_raster_init = true;
+    }
+    return &g_raster;
+}
+
+PNANOVDB_API pnanovdb_parallel_primitives_t* pnanovdb_get_parallel_primitives(void)
+{
+    if (!g_pp_init)
+    {
+        memset(&g_pp, 0, sizeof(g_pp));
+        g_pp.create_context = pp_create_context;
+        g_pp.destroy_context = pp_destroy_context;
+        g_pp.global_scan = pp_global_scan;
+        g_pp.global_scan_uint64 = pp_global_scan_uint64;
+        g_pp.global_scan_max = pp_global_scan_max;
+        g_pp.radix_sort = pp_radix_sort;
+        g_pp.radix_sort_key64 = pp_radix_sort_key64;
+        g_pp.compute = compute_table();
+        g_pp_init = true;
+    }
+    return &g_pp;
+}
+
+PNANOVDB_API vdbr_context_t* vdbr_context_create(int device)
+{
+    return (vdbr_context_t*)create_context(device);
+}
+
+PNANOVDB_API void vdbr_context_destroy(vdbr_context_t* ctx)
+{
+    destroy_context((Context*)ctx);
+}
+
+PNANOVDB_API const char* vdbr_last_error(vdbr_context_t* ctx)
+{
+    return ctx ? ((Context*)ctx)->error.c_str() : "null context";
+}
+
+PNANOVDB_API int vdbr_context_set_stream(vdbr_context_t* ctx, void* stream)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    (void)hipStreamSynchronize(c->stream);
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return 0;
+}
+
+PNANOVDB_API void* vdbr_context_get_stream(vdbr_context_t* ctx)
+{
+    return ctx ? (void*)((Context*)ctx)->stream : nullptr;
+}
+
+PNANOVDB_API int vdbr_synchronize(vdbr_context_t* ctx)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    VDBR_CHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+PNANOVDB_API int vdbr_copy(vdbr_context_t* ctx, void* dst, const void* src, uint64_t bytes, int kind)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    if (bytes == 0)
+        return 0;
+    hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    VDBR_CHECK(c, hipMemcpyAsync(dst, src, bytes, k, c->stream));
+    VDBR_CHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+PNANOVDB_API uint64_t vdbr_workspace_bytes(vdbr_context_t* ctx)
+{
+    return ctx ? ((Context*)ctx)->workspace_bytes() : 0;
+}
+
+PNANOVDB_API int vdbr_release_workspace(vdbr_context_t* ctx)
+{
+    if (!ctx)
+        return -1;
+    ((Context*)ctx)->release();
+    return 0;
+}
+
+PNANOVDB_API int vdbr_raster_node2(vdbr_context_t* ctx, const vdbr_gaussians_t* gaussians,
+                                   const vdbr_raster_params_t* params, void** grid_dev, uint64_t* grid_size,
+                                   vdbr_raster_stats_t* stats)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    (void)hipSetDevice(c->device);
+    return raster_node2(c, gaussians, params, grid_dev, grid_size, stats);
+}
+
+PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap,
+                                 uint64_t* count)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    return last_pairs(c, ijk_dev, gid_dev, cap, count);
+}
+
+PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32_t* vals, uint64_t count,
+                                     uint32_t bit_begin, uint32_t bit_end)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    if (bit_end > 64 || bit_begin > bit_end)
+        return c->fail("radix sort: bit range must satisfy bit_begin <= bit_end <= 64");
+    if (radix_sort_u64(c, keys, vals, count, bit_begin, bit_end, nullptr))
+        return -1;
+    VDBR_CHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32_t* vals, uint64_t count,
+                                     uint32_t key_bit_count)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    if (key_bit_count > 32)
+        return c->fail("radix sort: key_bit_count must be <= 32");
+    if (radix_sort_u32(c, keys, vals, count, key_bit_count))
+        return -1;
+    VDBR_CHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+PNANOVDB_API int vdbr_inclusive_scan(vdbr_context_t* ctx, const void* in, void* out, uint64_t count, int op)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    int r;
+    if (op == 0)
+        r = scan_u32(c, (const uint32_t*)in, (uint32_t*)out, count, true, nullptr);
+    else if (op == 1)
+        r = scan_u64(c, (const uint64_t*)in, (uint64_t*)out, count, true, nullptr);
+    else if (op == 2)
+        r = scan_max_u32(c, (const uint32_t*)in, (uint32_t*)out, count, true);
+    else
+        return c->fail("scan: unknown op");
+    if (r)
+        return r;
+    VDBR_CHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,229 @@
+// vdb_device.h -- device-side records and arithmetic of the raster3d pipeline (gfx950, wave64).
+//
+// Every FP32 expression that decides topology or the RGBA8 value is written in the operation
+// order of the reference shaders (gaussian_prim.slang, gaussian_frag_alloc.slang,
+// gaussian_frag_color.slang); the library is compiled with -ffp-contract=off, divisions and
+// square roots are IEEE correctly rounded, and exp/log/pow come from vdb_fpmath.h.
+#ifndef VDB_DEVICE_H
+#define VDB_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vdb_fpmath.h"
+#include "vdb_layout.h"
+
+#define VDB_DEV __device__ __forceinline__
+
+// ----------------------------------------------------------------------------- per-Gaussian
+// Predicate frame of one Gaussian (gaussian_frag_alloc.slang:62-121): position, the rows of the
+// inverse rotation, 1/r and 1/s (s floored to voxel_size/8).
+struct __align__(16) PrimFrame
+{
+    float mu[3];
+    float inv_r;
+    float rinv[9]; // rot0_inv, rot1_inv, rot2_inv (x, y, z each)
+    float inv_s[3];
+};
+
+// Voxel AABB of one Gaussian (gaussian_prim.slang:70-139); rng = max + 1 - min, 0 when dropped.
+struct __align__(16) PrimBox
+{
+    int32_t mn[3];
+    int32_t rng_x;
+    int32_t rng_y;
+    int32_t rng_z;
+    uint32_t pad0;
+    uint32_t pad1;
+};
+
+// Value-pass constants of one Gaussian (gaussian_frag_color.slang:184-212).
+struct __align__(16) PrimVal
+{
+    float opacity;
+    float local_scale; // r
+    float pow_exp;     // 1 / voxelRelativeScale
+    uint32_t rgb_cull; // r8 | g8 << 8 | b8 << 16 | culled << 24
+};
+
+// Compact voxel key: leaf coordinates relative to the global leaf minimum, packed
+// lexicographically (x, y, z) with per-axis bit widths, followed by the 9-bit in-leaf index
+// n = (i&7)<<6 | (j&7)<<3 | (k&7). Voxels of one leaf are contiguous and in leaf order.
+struct KeyLayout
+{
+    int32_t lmin[3];
+    uint32_t bits[3];
+    uint32_t gid_bits;
+    uint32_t voxel_bits; // bits[0] + bits[1] + bits[2] + 9
+};
+
+// Correctly rounded FP32 division and square root, independent of compiler flags: the exact
+// result of a float operation rounded to double (53 bits) and then to float (24 bits) equals the
+// direct float rounding because 53 >= 2*24 + 2, and f64 division / square root are IEEE
+// correctly rounded on gfx950.
+VDB_DEV float fdiv_rn(float a, float b)
+{
+    return (float)((double)a / (double)b);
+}
+VDB_DEV float fsqrt_rn(float a)
+{
+    return (float)__builtin_sqrt((double)a);
+}
+
+VDB_DEV float dot3(float ax, float ay, float az, float bx, float by, float bz)
+{
+    float s = ax * bx;
+    s = s + ay * by;
+    s = s + az * bz;
+    return s;
+}
+
+VDB_DEV uint64_t encode_voxel(const KeyLayout& kl, int32_t i, int32_t j, int32_t k)
+{
+    uint64_t lx = (uint32_t)((i >> 3) - kl.lmin[0]);
+    uint64_t ly = (uint32_t)((j >> 3) - kl.lmin[1]);
+    uint64_t lz = (uint32_t)((k >> 3) - kl.lmin[2]);
+    uint64_t leaf = (lx << (kl.bits[1] + kl.bits[2])) | (ly << kl.bits[2]) | lz;
+    uint32_t n = ((uint32_t)(i & 7) << 6) | ((uint32_t)(j & 7) << 3) | (uint32_t)(k & 7);
+    return (leaf << 9) | n;
+}
+
+VDB_DEV void decode_voxel(const KeyLayout& kl, uint64_t v, int32_t* i, int32_t* j, int32_t* k)
+{
+    uint32_t n = (uint32_t)(v & 511u);
+    uint64_t leaf = v >> 9;
+    uint32_t lz = (uint32_t)(leaf & ((1ull << kl.bits[2]) - 1ull));
+    uint32_t ly = (uint32_t)((leaf >> kl.bits[2]) & ((1ull << kl.bits[1]) - 1ull));
+    uint32_t lx = (uint32_t)(leaf >> (kl.bits[1] + kl.bits[2]));
+    *i = ((int32_t)lx + kl.lmin[0]) * 8 + (int32_t)(n >> 6);
+    *j = ((int32_t)ly + kl.lmin[1]) * 8 + (int32_t)((n >> 3) & 7u);
+    *k = ((int32_t)lz + kl.lmin[2]) * 8 + (int32_t)(n & 7u);
+}
+
+// gaussian_frag_alloc.slang:79-121 -- AABB of the voxel's 8 corners in the Gaussian frame,
+// each corner computed independently in the reference operation order.
+VDB_DEV void voxel_box(const PrimFrame& f, int32_t i, int32_t j, int32_t k, float vs, float vmin[3], float vmax[3])
+{
+#pragma unroll
+    for (uint32_t v = 0; v < 8u; v++)
+    {
+        float px = (float)i, py = (float)j, pz = (float)k;
+        if (v & 1u)
+            px += 1.f;
+        if (v & 2u)
+            py += 1.f;
+        if (v & 4u)
+            pz += 1.f;
+        float lx = px * vs;
+        float ly = py * vs;
+        float lz = pz * vs;
+        lx = lx - f.mu[0];
+        ly = ly - f.mu[1];
+        lz = lz - f.mu[2];
+        float q0 = dot3(lx, ly, lz, f.rinv[0], f.rinv[1], f.rinv[2]);
+        float q1 = dot3(lx, ly, lz, f.rinv[3], f.rinv[4], f.rinv[5]);
+        float q2 = dot3(lx, ly, lz, f.rinv[6], f.rinv[7], f.rinv[8]);
+        q0 = q0 * f.inv_r;
+        q1 = q1 * f.inv_r;
+        q2 = q2 * f.inv_r;
+        q0 = q0 * f.inv_s[0];
+        q1 = q1 * f.inv_s[1];
+        q2 = q2 * f.inv_s[2];
+        if (v == 0u)
+        {
+            vmin[0] = vmax[0] = q0;
+            vmin[1] = vmax[1] = q1;
+            vmin[2] = vmax[2] = q2;
+        }
+        else
+        {
+            vmin[0] = fminf(vmin[0], q0);
+            vmin[1] = fminf(vmin[1], q1);
+            vmin[2] = fminf(vmin[2], q2);
+            vmax[0] = fmaxf(vmax[0], q0);
+            vmax[1] = fmaxf(vmax[1], q1);
+            vmax[2] = fmaxf(vmax[2], q2);
+        }
+    }
+}
+
+// gaussian_frag_alloc.slang:139-162 -- voxel kept iff !(dist2 <= 0)
+VDB_DEV bool box_kept(const float vmin[3], const float vmax[3])
+{
+    float d2 = 1.f;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+    {
+        if (0.f < vmin[a])
+            d2 = d2 - vmin[a] * vmin[a];
+        else if (0.f > vmax[a])
+            d2 = d2 - vmax[a] * vmax[a];
+    }
+    return !(d2 <= 0.f);
+}
+
+VDB_DEV uint32_t u8_of(float v)
+{
+    return (uint32_t)(255.f * fmaxf(0.f, fminf(1.f, v)));
+}
+
+// gaussian_frag_color.slang:184-212 -- packed RGBA8 of one surviving pair. The 64-sample
+// maximum of o*exp(-(r*|p|^2)) equals o*exp(-(r*min|p|^2)) because every step is monotone in
+// |p|^2 (correctly rounded exp, multiplication by positive r and o); the minimum of the
+// separable sum over the 4x4x4 lattice is the sum of per-axis minima for the same reason
+// (rounded addition is monotone in each operand).
+VDB_DEV uint32_t pair_rgba(const PrimVal& pv, const float vmin[3], const float vmax[3])
+{
+    const float third = 1.f / 3.f;
+    float msq[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        float best = 0.f;
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+        {
+            float w = third * (float)s;
+            float ow = 1.f - w;
+            float lp = w * vmax[c] + ow * vmin[c];
+            float sq = lp * lp;
+            best = (s == 0) ? sq : fminf(best, sq);
+        }
+        msq[c] = best;
+    }
+    float d = msq[0] + msq[1];
+    d = d + msq[2];
+    float e = pv.opacity * vdb_expf(-(pv.local_scale * d));
+    float a = fmaxf(0.f, e);
+    a = 1.f - vdb_powf(1.f - a, pv.pow_exp);
+    if (pv.rgb_cull >> 24)
+        a = 0.f;
+    return (pv.rgb_cull & 0x00FFFFFFu) | (u8_of(a) << 24);
+}
+
+// density contribution o * exp(-m/2) at the voxel centre. The Mahalanobis term is formed in
+// double precision: for elongated Gaussians the short-axis coordinate is a difference of much
+// larger products, which loses ~1e-5 relative accuracy in float.
+VDB_DEV float pair_density(const PrimFrame& f, float opacity, int32_t i, int32_t j, int32_t k, double vs)
+{
+    const double d0 = ((double)i + 0.5) * vs - (double)f.mu[0];
+    const double d1 = ((double)j + 0.5) * vs - (double)f.mu[1];
+    const double d2 = ((double)k + 0.5) * vs - (double)f.mu[2];
+    double m = 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+    {
+        double u = (double)f.rinv[3 * r] * d0 + (double)f.rinv[3 * r + 1] * d1 + (double)f.rinv[3 * r + 2] * d2;
+        double q = u * (double)f.inv_s[r];
+        m += q * q;
+    }
+    return opacity * expf((float)(-0.5 * m));
+}
+
+// ----------------------------------------------------------------------------- wave helpers
+VDB_DEV uint32_t lane_id()
+{
+    return __lane_id();
+}
+
+#endif

@@ -1,0 +1,136 @@
+// vdb_internal.h -- host-side context, workspace and internal entry points of libvdbraster.
+#ifndef VDB_INTERNAL_H
+#define VDB_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/pnanovdb_amd.h"
+
+namespace vdbr
+{
+
+// workspace slots: each slot is one growable device allocation reused across calls
+enum WsSlot
+{
+    WS_FRAMES,
+    WS_BOXES,
+    WS_VALS,
+    WS_FOOT,
+    WS_FOOT_SCAN,
+    WS_ROWS,
+    WS_ROW_SCAN,
+    WS_MISC,
+    WS_PAIRS,
+    WS_SORT_ALT,
+    WS_SORT_VALS_ALT,
+    WS_SORT_STATUS,
+    WS_SORT_SMALL,
+    WS_SCAN_PARTIALS,
+    WS_SEG,
+    WS_SEG_TILES,
+    WS_VOX_KEY,
+    WS_VOX_VAL,
+    WS_VOX_T,
+    WS_LEAF_FIRST,
+    WS_LEAF_INFO,
+    WS_LEAF_MASK,
+    WS_LEAF_SORT_KEYS,
+    WS_LEAF_SORT_VALS,
+    WS_LOWER,
+    WS_LOWER_MASK,
+    WS_UPPER,
+    WS_UPPER_MASK,
+    WS_NODE_KEYS,
+    WS_NODE_VALS,
+    WS_NODE_BYIDX,
+    WS_NODE_SCAN,
+    WS_HIST,
+    WS_GRID,
+    WS_BATCH,
+    WS_RAW_COPY,
+    WS_AUX_ALT,
+    WS_AUX_VALS_ALT,
+    WS_COUNT
+};
+
+struct SortWorkspace
+{
+    uint64_t* keys_alt;
+    uint32_t* vals_alt;
+    uint64_t* status;
+    uint64_t* hist;
+    uint64_t* starts;
+    uint32_t* tile_counters;
+};
+
+struct SortTiming
+{
+    float ms;
+};
+
+struct Context
+{
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string error;
+    void* ws_ptr[WS_COUNT] = {};
+    size_t ws_size[WS_COUNT] = {};
+    uint64_t* pinned = nullptr; // small host readback area (pinned)
+    uint32_t sort_epoch = 0;
+    uint64_t sort_status_tiles = 0;
+
+    // state of the last raster call (for vdbr_last_pairs)
+    uint64_t last_pair_count = 0;
+    uint64_t* last_pairs = nullptr;
+    uint32_t last_gid_bits = 0;
+    int32_t last_lmin[3] = { 0, 0, 0 };
+    uint32_t last_bits[3] = { 0, 0, 0 };
+
+    int fail(const std::string& msg)
+    {
+        error = msg;
+        return -1;
+    }
+    void* ws(WsSlot slot, size_t bytes);
+    void* scratch_scan(size_t bytes)
+    {
+        return ws(WS_SCAN_PARTIALS, bytes);
+    }
+    int sort_workspace(uint64_t n, bool with_vals, uint64_t tiles, bool aux, SortWorkspace* out);
+    uint32_t next_sort_epoch(uint64_t tiles);
+    uint64_t workspace_bytes() const;
+    void release();
+};
+
+#define VDBR_CHECK(ctx, expr)                                                                                          \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t _e = (expr);                                                                                        \
+        if (_e != hipSuccess)                                                                                          \
+            return (ctx)->fail(std::string(#expr) + ": " + hipGetErrorString(_e));                                     \
+    } while (0)
+
+// vdb_prims.hip
+int scan_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, uint32_t* total_dev);
+int scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, uint64_t n, bool inclusive, uint64_t* total_dev);
+int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive);
+int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+                   SortTiming* timing);
+int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+                          uint64_t** result_keys);
+
+// vdb_raster.hip
+int raster_node2(Context* ctx, const vdbr_gaussians_t* g, const vdbr_raster_params_t* p, void** grid_dev,
+                 uint64_t* grid_size, vdbr_raster_stats_t* stats);
+int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, uint64_t* count);
+int preprocess_device(Context* ctx, uint64_t n, float* opacities, float* quats, float* scales, const float* sh0,
+                      float* colors);
+int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits);
+
+} // namespace vdbr
+
+#endif

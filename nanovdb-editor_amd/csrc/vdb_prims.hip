@@ -1,0 +1,565 @@
+// vdb_prims.hip -- device-wide scans and the onesweep LSD radix sort (gfx950, wave64).
+//
+// Scans (ParallelPrimitives.cpp:151-310 semantics: inclusive, u32 add / u64 add / u32 max):
+// reduce-then-scan over 2048-element tiles (tile reduce, one-block scan of tile partials,
+// tile scan + offset).
+//
+// Radix sort (ParallelPrimitives.cpp:685-869 semantics: stable, ascending, LSD over a bit
+// range): one histogram pass over the keys builds the digit counts of every 8-bit pass; each
+// pass is then a single onesweep kernel over 4096-key tiles:
+//   - wave-local stable ranking with ballot peer masks (8 ballots per digit bit set),
+//   - per-tile digit counts published to a status array and resolved by decoupled look-back
+//     (epoch-tagged 64-bit status words, so the array is never cleared between passes),
+//   - keys staged in LDS in tile-sorted order and written out in digit runs.
+// Tiles take their index from an atomic counter, so every tile a block waits on has already
+// been claimed by a running block.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vdb_internal.h"
+
+namespace vdbr
+{
+
+// ============================================================================ scans
+template <typename T, int OP>
+struct ScanOp;
+template <>
+struct ScanOp<uint32_t, 0>
+{
+    static __device__ __forceinline__ uint32_t apply(uint32_t a, uint32_t b)
+    {
+        return a + b;
+    }
+    static __device__ __forceinline__ uint32_t identity()
+    {
+        return 0u;
+    }
+};
+template <>
+struct ScanOp<uint64_t, 1>
+{
+    static __device__ __forceinline__ uint64_t apply(uint64_t a, uint64_t b)
+    {
+        return a + b;
+    }
+    static __device__ __forceinline__ uint64_t identity()
+    {
+        return 0ull;
+    }
+};
+template <>
+struct ScanOp<uint32_t, 2>
+{
+    static __device__ __forceinline__ uint32_t apply(uint32_t a, uint32_t b)
+    {
+        return a > b ? a : b;
+    }
+    static __device__ __forceinline__ uint32_t identity()
+    {
+        return 0u;
+    }
+};
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+// inclusive scan of one value per thread across a 256-thread block; returns the inclusive
+// prefix, *total receives the block aggregate
+template <typename T, int OP>
+__device__ __forceinline__ T block_scan_inclusive(T v, T* smem_waves, T* total)
+{
+    using Op = ScanOp<T, OP>;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        T o = __shfl_up(v, off, 64);
+        if (lane >= off)
+            v = Op::apply(o, v);
+    }
+    if (lane == 63)
+        smem_waves[wave] = v;
+    __syncthreads();
+    T prefix = Op::identity();
+    for (int w = 0; w < wave; w++)
+        prefix = Op::apply(prefix, smem_waves[w]);
+    T tot = Op::identity();
+    for (int w = 0; w < (SCAN_THREADS >> 6); w++)
+        tot = Op::apply(tot, smem_waves[w]);
+    *total = tot;
+    __syncthreads();
+    return Op::apply(prefix, v);
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(SCAN_THREADS) void k_tile_reduce(const T* __restrict__ in, uint64_t n, T* __restrict__ partials)
+{
+    using Op = ScanOp<T, OP>;
+    __shared__ T smem[SCAN_THREADS / 64];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    T acc = Op::identity();
+#pragma unroll
+    for (int m = 0; m < SCAN_ITEMS; m++)
+    {
+        uint64_t idx = base + (uint64_t)m * SCAN_THREADS + threadIdx.x;
+        if (idx < n)
+            acc = Op::apply(acc, in[idx]);
+    }
+    T total;
+    block_scan_inclusive<T, OP>(acc, smem, &total);
+    if (threadIdx.x == 0)
+        partials[blockIdx.x] = total;
+}
+
+// exclusive scan of the tile partials in place, one block
+template <typename T, int OP>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(T* __restrict__ partials, uint64_t count, T* __restrict__ grand_total)
+{
+    using Op = ScanOp<T, OP>;
+    __shared__ T smem[SCAN_THREADS / 64];
+    T carry = Op::identity();
+    for (uint64_t base = 0; base < count; base += SCAN_THREADS)
+    {
+        uint64_t idx = base + threadIdx.x;
+        T v = idx < count ? partials[idx] : Op::identity();
+        T total;
+        T incl = block_scan_inclusive<T, OP>(v, smem, &total);
+        T excl_local = Op::identity();
+        // exclusive value: inclusive of the previous lane (shift through shared memory)
+        __shared__ T sh[SCAN_THREADS];
+        sh[threadIdx.x] = incl;
+        __syncthreads();
+        excl_local = threadIdx.x == 0 ? Op::identity() : sh[threadIdx.x - 1];
+        __syncthreads();
+        if (idx < count)
+            partials[idx] = Op::apply(carry, excl_local);
+        carry = Op::apply(carry, total);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && grand_total)
+        *grand_total = carry;
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const T* __restrict__ in, T* __restrict__ out, uint64_t n,
+                                                         const T* __restrict__ partials, int inclusive)
+{
+    using Op = ScanOp<T, OP>;
+    __shared__ T tile[SCAN_TILE];
+    __shared__ T smem[SCAN_THREADS / 64];
+    uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int m = 0; m < SCAN_ITEMS; m++)
+    {
+        uint64_t idx = base + (uint64_t)m * SCAN_THREADS + threadIdx.x;
+        tile[m * SCAN_THREADS + threadIdx.x] = idx < n ? in[idx] : Op::identity();
+    }
+    __syncthreads();
+    T items[SCAN_ITEMS];
+    T acc = Op::identity();
+#pragma unroll
+    for (int m = 0; m < SCAN_ITEMS; m++)
+    {
+        items[m] = tile[threadIdx.x * SCAN_ITEMS + m];
+        acc = Op::apply(acc, items[m]);
+    }
+    T total;
+    T incl = block_scan_inclusive<T, OP>(acc, smem, &total);
+    // exclusive prefix of this thread = inclusive prefix of the previous thread
+    __shared__ T sh[SCAN_THREADS];
+    sh[threadIdx.x] = incl;
+    __syncthreads();
+    T prefix = Op::apply(partials[blockIdx.x], threadIdx.x == 0 ? Op::identity() : sh[threadIdx.x - 1]);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < SCAN_ITEMS; m++)
+    {
+        T before = prefix;
+        prefix = Op::apply(prefix, items[m]);
+        tile[threadIdx.x * SCAN_ITEMS + m] = inclusive ? prefix : before;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < SCAN_ITEMS; m++)
+    {
+        uint64_t idx = base + (uint64_t)m * SCAN_THREADS + threadIdx.x;
+        if (idx < n)
+            out[idx] = tile[m * SCAN_THREADS + threadIdx.x];
+    }
+}
+
+template <typename T, int OP>
+int scan_impl(Context* ctx, const T* in, T* out, uint64_t n, bool inclusive, T* total_dev)
+{
+    if (n == 0)
+    {
+        if (total_dev)
+            VDBR_CHECK(ctx, hipMemsetAsync(total_dev, 0, sizeof(T), ctx->stream));
+        return 0;
+    }
+    uint64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    T* partials = (T*)ctx->scratch_scan(tiles * sizeof(T));
+    hipLaunchKernelGGL((k_tile_reduce<T, OP>), dim3((uint32_t)tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, n, partials);
+    hipLaunchKernelGGL((k_scan_partials<T, OP>), dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, partials, tiles, total_dev);
+    hipLaunchKernelGGL((k_tile_scan<T, OP>), dim3((uint32_t)tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
+                       (const T*)partials, inclusive ? 1 : 0);
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+int scan_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, uint32_t* total_dev)
+{
+    return scan_impl<uint32_t, 0>(ctx, in, out, n, inclusive, total_dev);
+}
+int scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, uint64_t n, bool inclusive, uint64_t* total_dev)
+{
+    return scan_impl<uint64_t, 1>(ctx, in, out, n, inclusive, total_dev);
+}
+int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive)
+{
+    return scan_impl<uint32_t, 2>(ctx, in, out, n, inclusive, nullptr);
+}
+
+// ============================================================================ radix sort
+constexpr int SORT_THREADS = 256;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_WAVE_TILE = 64 * SORT_ITEMS;  // 1024
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS; // 4096
+constexpr int SORT_MAX_PASSES = 8;
+
+// status word: [63:48] epoch, [47:46] flag (1 = tile aggregate, 2 = inclusive prefix), [45:0] count
+#define ST_AGG 1ull
+#define ST_INCL 2ull
+__device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint64_t flag, uint64_t value)
+{
+    return ((uint64_t)epoch << 48) | (flag << 46) | (value & ((1ull << 46) - 1ull));
+}
+
+// digit of pass p covers key bits [bit_begin + 8p, min(bit_end, bit_begin + 8p + 8)); the same
+// mask is applied here and in k_sort_pass so the per-pass counts always agree
+__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, uint64_t n, uint32_t bit_begin,
+                                                   uint32_t bit_count, unsigned long long* __restrict__ hist)
+{
+    const uint32_t passes = (bit_count + 7u) / 8u;
+    __shared__ uint32_t h[SORT_MAX_PASSES][256];
+    for (int p = 0; p < SORT_MAX_PASSES; p++)
+        h[p][threadIdx.x] = 0u;
+    __syncthreads();
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        uint64_t k = keys[i] >> bit_begin;
+        for (uint32_t p = 0; p < passes; p++)
+        {
+            uint32_t dbits = (bit_count - 8u * p) < 8u ? (bit_count - 8u * p) : 8u;
+            atomicAdd(&h[p][(uint32_t)(k & ((1u << dbits) - 1u))], 1u);
+            k >>= 8;
+        }
+    }
+    __syncthreads();
+    for (uint32_t p = 0; p < passes; p++)
+    {
+        uint32_t c = h[p][threadIdx.x];
+        if (c)
+            atomicAdd(&hist[p * 256 + threadIdx.x], (unsigned long long)c);
+    }
+}
+
+// exclusive scan of each pass's 256 digit counts (one block of 256 threads)
+__global__ __launch_bounds__(256) void k_sort_digit_starts(const unsigned long long* __restrict__ hist, uint32_t passes,
+                                                           uint64_t* __restrict__ starts)
+{
+    __shared__ uint64_t smem[4];
+    for (uint32_t p = 0; p < passes; p++)
+    {
+        uint64_t c = hist[p * 256 + threadIdx.x];
+        uint64_t total;
+        uint64_t incl = block_scan_inclusive<uint64_t, 1>(c, smem, &total);
+        starts[p * 256 + threadIdx.x] = incl - c;
+        __syncthreads();
+    }
+}
+
+template <bool HAS_VALS>
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
+                                                            const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
+                                                            uint64_t n, uint32_t shift, uint32_t digit_bits,
+                                                            const uint64_t* __restrict__ digit_start,
+                                                            uint64_t* __restrict__ status, uint32_t* __restrict__ tile_counter,
+                                                            uint32_t epoch)
+{
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t s_hist_excl[256];
+    __shared__ uint64_t s_base[256];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint64_t s_keys[SORT_TILE];
+    __shared__ uint32_t s_vals[HAS_VALS ? SORT_TILE : 1];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t wave = tid >> 6;
+    const uint32_t lane = tid & 63u;
+    if (tid == 0)
+        s_tile = atomicAdd(tile_counter, 1u);
+#pragma unroll
+    for (int w = 0; w < 4; w++)
+        wcnt[w][tid] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * SORT_TILE + (uint64_t)wave * SORT_WAVE_TILE;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const uint32_t dmask = (1u << digit_bits) - 1u;
+
+    uint64_t keys[SORT_ITEMS];
+    uint32_t vals[SORT_ITEMS];
+    uint32_t rank[SORT_ITEMS];
+#pragma unroll
+    for (int m = 0; m < SORT_ITEMS; m++)
+    {
+        uint64_t idx = base + (uint64_t)m * 64u + lane;
+        bool valid = idx < n;
+        keys[m] = valid ? kin[idx] : ~0ull;
+        if (HAS_VALS)
+            vals[m] = valid ? vin[idx] : 0u;
+    }
+#pragma unroll
+    for (int m = 0; m < SORT_ITEMS; m++)
+    {
+        uint64_t idx = base + (uint64_t)m * 64u + lane;
+        bool valid = idx < n;
+        uint32_t d = (uint32_t)(keys[m] >> shift) & dmask;
+        uint64_t peers = __ballot(valid);
+        for (uint32_t b = 0; b < digit_bits; b++)
+        {
+            uint64_t bal = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        uint32_t old = valid ? wcnt[wave][d] : 0u;
+        rank[m] = old + (uint32_t)__popcll(peers & lt_mask);
+        if (valid && (peers & lt_mask) == 0ull)
+            wcnt[wave][d] = old + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // per digit (thread tid == digit): wave offsets and tile count
+    const uint32_t d = tid;
+    uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
+    uint32_t total = c0 + c1 + c2 + c3;
+    // publish the tile aggregate first (look-back chains never wait on a later tile)
+    uint64_t* my_status = status + (uint64_t)tile * 256u + d;
+    if (tile == 0)
+        __hip_atomic_store(my_status, st_pack(epoch, ST_INCL, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        __hip_atomic_store(my_status, st_pack(epoch, ST_AGG, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    wcnt[0][d] = 0u;
+    wcnt[1][d] = c0;
+    wcnt[2][d] = c0 + c1;
+    wcnt[3][d] = c0 + c1 + c2;
+    // exclusive scan of the tile digit counts
+    {
+        uint32_t v = total;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            uint32_t o = __shfl_up(v, off, 64);
+            if (lane >= (uint32_t)off)
+                v += o;
+        }
+        if (lane == 63u)
+            s_wsum[wave] = v;
+        __syncthreads();
+        uint32_t pre = 0u;
+        for (uint32_t w = 0; w < wave; w++)
+            pre += s_wsum[w];
+        s_hist_excl[d] = pre + v - total;
+    }
+    // decoupled look-back for this digit
+    uint64_t excl = 0ull;
+    if (tile > 0)
+    {
+        int64_t t = (int64_t)tile - 1;
+        while (true)
+        {
+            uint64_t s = __hip_atomic_load(status + (uint64_t)t * 256u + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t ep = (uint32_t)(s >> 48);
+            uint64_t flag = (s >> 46) & 3ull;
+            if (ep != epoch || flag == 0ull)
+            {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += s & ((1ull << 46) - 1ull);
+            if (flag == ST_INCL)
+                break;
+            t--;
+        }
+        __hip_atomic_store(my_status, st_pack(epoch, ST_INCL, excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    s_base[d] = digit_start[d] + excl - (uint64_t)s_hist_excl[d];
+#pragma unroll
+    for (int m = 0; m < SORT_ITEMS; m++)
+    {
+        uint64_t idx = base + (uint64_t)m * 64u + lane;
+        if (idx < n)
+        {
+            uint32_t dd = (uint32_t)(keys[m] >> shift) & dmask;
+            uint32_t pos = s_hist_excl[dd] + wcnt[wave][dd] + rank[m];
+            s_keys[pos] = keys[m];
+            if (HAS_VALS)
+                s_vals[pos] = vals[m];
+        }
+    }
+    __syncthreads();
+    uint64_t tile_begin = (uint64_t)tile * SORT_TILE;
+    uint32_t tile_n = (uint32_t)((n - tile_begin) < (uint64_t)SORT_TILE ? (n - tile_begin) : (uint64_t)SORT_TILE);
+    for (uint32_t p = tid; p < tile_n; p += SORT_THREADS)
+    {
+        uint64_t k = s_keys[p];
+        uint32_t dd = (uint32_t)(k >> shift) & dmask;
+        uint64_t g = s_base[dd] + p;
+        kout[g] = k;
+        if (HAS_VALS)
+            vout[g] = s_vals[p];
+    }
+}
+
+int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+                   SortTiming* timing)
+{
+    if (n <= 1 || bit_end <= bit_begin)
+        return 0;
+    uint32_t bits = bit_end - bit_begin;
+    uint32_t passes = (bits + 7u) / 8u;
+    if (passes > SORT_MAX_PASSES)
+        return ctx->fail("radix sort: more than 64 key bits");
+    uint64_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    if (tiles > 0xFFFFFFFFull)
+        return ctx->fail("radix sort: too many tiles");
+    SortWorkspace ws;
+    if (ctx->sort_workspace(n, vals != nullptr, tiles, true, &ws))
+        return -1;
+    VDBR_CHECK(ctx, hipMemsetAsync(ws.hist, 0, SORT_MAX_PASSES * 256 * sizeof(uint64_t), ctx->stream));
+    VDBR_CHECK(ctx, hipMemsetAsync(ws.tile_counters, 0, SORT_MAX_PASSES * sizeof(uint32_t), ctx->stream));
+    uint32_t hist_blocks = (uint32_t)((n + 255) / 256);
+    if (hist_blocks > 4096u)
+        hist_blocks = 4096u;
+    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits,
+                       (unsigned long long*)ws.hist);
+    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
+                       passes, ws.starts);
+    uint64_t* kin = keys;
+    uint64_t* kout = ws.keys_alt;
+    uint32_t* vin = vals;
+    uint32_t* vout = ws.vals_alt;
+    for (uint32_t p = 0; p < passes; p++)
+    {
+        uint32_t shift = bit_begin + 8u * p;
+        uint32_t dbits = (bits - 8u * p) < 8u ? (bits - 8u * p) : 8u;
+        uint32_t epoch = ctx->next_sort_epoch(tiles);
+        if (vals)
+            hipLaunchKernelGGL(k_sort_pass<true>, dim3((uint32_t)tiles), dim3(SORT_THREADS), 0, ctx->stream, kin, kout,
+                               vin, vout, n, shift, dbits, ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
+        else
+            hipLaunchKernelGGL(k_sort_pass<false>, dim3((uint32_t)tiles), dim3(SORT_THREADS), 0, ctx->stream, kin, kout,
+                               nullptr, nullptr, n, shift, dbits, ws.starts + p * 256, ws.status, ws.tile_counters + p,
+                               epoch);
+        uint64_t* tk = kin;
+        kin = kout;
+        kout = tk;
+        uint32_t* tv = vin;
+        vin = vout;
+        vout = tv;
+    }
+    if (kin != keys)
+    {
+        VDBR_CHECK(ctx, hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+        if (vals)
+            VDBR_CHECK(ctx, hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    (void)timing;
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+__global__ void k_widen_u32(const uint32_t* __restrict__ in, uint64_t n, uint64_t* __restrict__ out)
+{
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n)
+        out[x] = in[x];
+}
+
+__global__ void k_narrow_u64(const uint64_t* __restrict__ in, uint64_t n, uint32_t* __restrict__ out)
+{
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < n)
+        out[x] = (uint32_t)in[x];
+}
+
+// ParallelPrimitives.cpp:347-525 (u32 keys, u32 payload): keys are widened to 64 bits
+int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits)
+{
+    if (n <= 1 || key_bits == 0)
+        return 0;
+    uint64_t* wide = (uint64_t*)ctx->ws(WS_LEAF_SORT_KEYS, n * sizeof(uint64_t));
+    if (!wide)
+        return -1;
+    uint32_t blocks = (uint32_t)((n + 255) / 256);
+    hipLaunchKernelGGL(k_widen_u32, dim3(blocks), dim3(256), 0, ctx->stream, keys, n, wide);
+    if (radix_sort_u64(ctx, wide, vals, n, 0, key_bits, nullptr))
+        return -1;
+    hipLaunchKernelGGL(k_narrow_u64, dim3(blocks), dim3(256), 0, ctx->stream, wide, n, keys);
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+// sort returning the buffer that holds the result (no copy back): *result_keys is either
+// keys or the workspace alternate
+int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+                          uint64_t** result_keys)
+{
+    *result_keys = keys;
+    if (n <= 1 || bit_end <= bit_begin)
+        return 0;
+    uint32_t bits = bit_end - bit_begin;
+    uint32_t passes = (bits + 7u) / 8u;
+    if (passes > SORT_MAX_PASSES)
+        return ctx->fail("radix sort: more than 64 key bits");
+    uint64_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    SortWorkspace ws;
+    if (ctx->sort_workspace(n, false, tiles, false, &ws))
+        return -1;
+    VDBR_CHECK(ctx, hipMemsetAsync(ws.hist, 0, SORT_MAX_PASSES * 256 * sizeof(uint64_t), ctx->stream));
+    VDBR_CHECK(ctx, hipMemsetAsync(ws.tile_counters, 0, SORT_MAX_PASSES * sizeof(uint32_t), ctx->stream));
+    uint32_t hist_blocks = (uint32_t)((n + 255) / 256);
+    if (hist_blocks > 4096u)
+        hist_blocks = 4096u;
+    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits,
+                       (unsigned long long*)ws.hist);
+    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
+                       passes, ws.starts);
+    uint64_t* kin = keys;
+    uint64_t* kout = ws.keys_alt;
+    for (uint32_t p = 0; p < passes; p++)
+    {
+        uint32_t shift = bit_begin + 8u * p;
+        uint32_t dbits = (bits - 8u * p) < 8u ? (bits - 8u * p) : 8u;
+        uint32_t epoch = ctx->next_sort_epoch(tiles);
+        hipLaunchKernelGGL(k_sort_pass<false>, dim3((uint32_t)tiles), dim3(SORT_THREADS), 0, ctx->stream, kin, kout,
+                           nullptr, nullptr, n, shift, dbits, ws.starts + p * 256, ws.status, ws.tile_counters + p,
+                           epoch);
+        uint64_t* tk = kin;
+        kin = kout;
+        kout = tk;
+    }
+    *result_keys = kin;
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+} // namespace vdbr

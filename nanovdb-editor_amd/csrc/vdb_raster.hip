@@ -1,0 +1,1639 @@
+// vdb_raster.hip -- the raster3d pipeline on gfx950: Gaussian arrays -> node2 NanoVDB grid.
+//
+//   preprocess  (RasterUtils.cpp:31-136)          N threads, in place
+//   setup       (gaussian_prim.slang:25-145)      per-Gaussian frame/AABB/footprint/row count
+//   emit        (gaussian_frag_alloc.slang:28-177) one thread per AABB row (i, j); the row's
+//               8-corner transforms are shared along k (each lattice point computed once, in
+//               the reference operation order), survivors compacted with wave ballots into an
+//               LDS stage and flushed in 4 KB bursts -> 64-bit keys (voxel << gid_bits | gid)
+//   sort        stable onesweep radix sort of the keys (vdb_prims.hip)
+//   reduce      one thread per voxel run: RGBA8 max (gaussian_frag_color.slang:184-247) or
+//               float density sum in ascending gaussian id; no float atomics
+//   build       leaves/lowers/uppers from the sorted voxel keys (ballot-free: runs of key
+//               prefixes), node order of the reference's windowed allocation
+//               (grid_build_scatter_new.slang:110-174, Raster.cpp:441-535), packed prefix
+//               sums, node2 write (GridBuild.cpp:118-187, grid_build_blindmetadata.slang,
+//               grid_build_compute_bbox.slang)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "vdb_device.h"
+#include "vdb_internal.h"
+
+namespace vdbr
+{
+
+// ============================================================================ preprocess
+// RasterUtils.cpp:59-117 (quaternion normalisation, exp scales, SH0 colour, sigmoid)
+__global__ __launch_bounds__(256) void k_preprocess(uint64_t n, float* __restrict__ opac, float* __restrict__ quats,
+                                                    float* __restrict__ scales, const float* sh0, float* colors)
+{
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    float x = quats[4 * g + 1], y = quats[4 * g + 2], z = quats[4 * g + 3], w = quats[4 * g + 0];
+    float sum = x * x;
+    sum = sum + y * y;
+    sum = sum + z * z;
+    sum = sum + w * w;
+    float magn_inv = fdiv_rn(1.f, fsqrt_rn(sum));
+    quats[4 * g + 1] = x * magn_inv;
+    quats[4 * g + 2] = y * magn_inv;
+    quats[4 * g + 3] = z * magn_inv;
+    quats[4 * g + 0] = w * magn_inv;
+    for (int a = 0; a < 3; a++)
+        scales[3 * g + a] = vdb_expf(scales[3 * g + a]);
+    const float c0 = 0.28209479177387814f;
+    for (int a = 0; a < 3; a++)
+    {
+        float s = sh0[3 * g + a];
+        float c = c0 * s;
+        colors[3 * g + a] = c + 0.5f;
+    }
+    float o = opac[g];
+    opac[g] = fdiv_rn(1.f, 1.f + vdb_expf(-o));
+}
+
+// ============================================================================ setup
+struct SetupOut
+{
+    PrimFrame* frames;
+    PrimBox* boxes;
+    PrimVal* vals;
+    uint64_t* foot;
+    uint64_t* rows;
+    int32_t* bounds;             // [6] min xyz, max xyz over valid Gaussian AABBs
+    unsigned long long* est;     // pair-count bound accumulator
+};
+
+__global__ __launch_bounds__(256) void k_setup(uint64_t n, const float* __restrict__ means, const float* __restrict__ quats,
+                                               const float* __restrict__ scales, const float* __restrict__ opac,
+                                               const float* __restrict__ colors, float vs, float vs_inv,
+                                               float cull_scale, SetupOut o)
+{
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    float pos[3] = { means[3 * g + 0], means[3 * g + 1], means[3 * g + 2] };
+    float x = quats[4 * g + 1], y = quats[4 * g + 2], z = quats[4 * g + 3], w = quats[4 * g + 0];
+    float opacity = opac[g];
+    // gaussian_prim.slang:65-68
+    float local_scale = fsqrt_rn(2.f * vdb_logf(fdiv_rn(opacity, 0.01f)));
+    const float min_scale = 1.f / 8.f;
+    float floor_s = min_scale * vs;
+    float s[3];
+    for (int a = 0; a < 3; a++)
+    {
+        float v = scales[3 * g + a];
+        s[a] = fmaxf(v, floor_s);
+    }
+    // drop Gaussians whose radius is not a positive finite number (opacity <= 0.01)
+    bool valid = (local_scale > 0.f) && (local_scale <= 3.402823466e38f);
+
+    int32_t mn[3] = { 0, 0, 0 }, mx[3] = { -1, -1, -1 };
+    if (valid)
+    {
+        // gaussian_prim.slang:70-111
+        float r0x = 1.f - 2.f * (y * y + z * z), r0y = 2.f * (x * y - z * w), r0z = 2.f * (x * z + y * w);
+        float r1x = 2.f * (x * y + z * w), r1y = 1.f - 2.f * (x * x + z * z), r1z = 2.f * (y * z - x * w);
+        float r2x = 2.f * (x * z - y * w), r2y = 2.f * (y * z + x * w), r2z = 1.f - 2.f * (x * x + y * y);
+        for (uint32_t v = 0; v < 8u; v++)
+        {
+            float c[3] = { (v & 1u) ? 1.f : -1.f, (v & 2u) ? 1.f : -1.f, (v & 4u) ? 1.f : -1.f };
+            for (int a = 0; a < 3; a++)
+            {
+                c[a] = c[a] * s[a];
+                c[a] = c[a] * local_scale;
+            }
+            float q[3] = { dot3(c[0], c[1], c[2], r0x, r0y, r0z), dot3(c[0], c[1], c[2], r1x, r1y, r1z),
+                           dot3(c[0], c[1], c[2], r2x, r2y, r2z) };
+            for (int a = 0; a < 3; a++)
+            {
+                q[a] = q[a] + pos[a];
+                int32_t ijk = (int32_t)floorf(q[a] * vs_inv);
+                if (v == 0u)
+                {
+                    mn[a] = ijk;
+                    mx[a] = ijk;
+                }
+                else
+                {
+                    mn[a] = min(mn[a], ijk);
+                    mx[a] = max(mx[a], ijk);
+                }
+            }
+        }
+    }
+    PrimBox box;
+    box.mn[0] = mn[0];
+    box.mn[1] = mn[1];
+    box.mn[2] = mn[2];
+    box.rng_x = valid ? mx[0] + 1 - mn[0] : 0;
+    box.rng_y = valid ? mx[1] + 1 - mn[1] : 0;
+    box.rng_z = valid ? mx[2] + 1 - mn[2] : 0;
+    box.pad0 = 0u;
+    box.pad1 = 0u;
+    o.boxes[g] = box;
+    uint64_t rows = valid ? (uint64_t)box.rng_x * (uint64_t)box.rng_y : 0ull;
+    o.rows[g] = rows;
+    o.foot[g] = rows * (uint64_t)(valid ? box.rng_z : 0);
+
+    // gaussian_frag_alloc.slang:103-120 quantities
+    PrimFrame f;
+    f.mu[0] = pos[0];
+    f.mu[1] = pos[1];
+    f.mu[2] = pos[2];
+    f.inv_r = valid ? fdiv_rn(1.f, local_scale) : 0.f;
+    f.rinv[0] = 1.f - 2.f * (y * y + z * z);
+    f.rinv[1] = 2.f * (x * y + z * w);
+    f.rinv[2] = 2.f * (x * z - y * w);
+    f.rinv[3] = 2.f * (x * y - z * w);
+    f.rinv[4] = 1.f - 2.f * (x * x + z * z);
+    f.rinv[5] = 2.f * (y * z + x * w);
+    f.rinv[6] = 2.f * (x * z + y * w);
+    f.rinv[7] = 2.f * (y * z - x * w);
+    f.rinv[8] = 1.f - 2.f * (x * x + y * y);
+    for (int a = 0; a < 3; a++)
+        f.inv_s[a] = fdiv_rn(1.f, s[a]);
+    o.frames[g] = f;
+
+    // gaussian_frag_color.slang:184-212 per-Gaussian constants
+    PrimVal pv;
+    pv.opacity = opacity;
+    pv.local_scale = local_scale;
+    float scale1d = fsqrt_rn(dot3(s[0], s[1], s[2], s[0], s[1], s[2]));
+    float vrs = fdiv_rn(0.1f * scale1d, vs);
+    pv.pow_exp = fdiv_rn(1.f, vrs);
+    const float default_cull = 0.05f;
+    float cs = cull_scale == 0.f ? default_cull : cull_scale;
+    uint32_t culled = (scale1d > cs) ? 1u : 0u;
+    pv.rgb_cull = u8_of(colors[3 * g + 0]) | (u8_of(colors[3 * g + 1]) << 8) | (u8_of(colors[3 * g + 2]) << 16) |
+                  (culled << 24);
+    o.vals[g] = pv;
+
+    if (valid)
+    {
+        atomicMin(&o.bounds[0], mn[0]);
+        atomicMin(&o.bounds[1], mn[1]);
+        atomicMin(&o.bounds[2], mn[2]);
+        atomicMax(&o.bounds[3], mx[0]);
+        atomicMax(&o.bounds[4], mx[1]);
+        atomicMax(&o.bounds[5], mx[2]);
+        // bound on survivors: voxel centres of kept voxels satisfy |A(c - mu)| <= 1 + |h|, h the
+        // local half extent of a voxel; volume (4/3) pi (1+|h|)^3 r^3 s_x s_y s_z / vs^3
+        float h2 = 0.f;
+        for (int a = 0; a < 3; a++)
+        {
+            float ha = 0.5f * vs * (fabsf(f.rinv[3 * a]) + fabsf(f.rinv[3 * a + 1]) + fabsf(f.rinv[3 * a + 2])) * f.inv_r *
+                       f.inv_s[a];
+            h2 += ha * ha;
+        }
+        double rad = 1.0 + sqrt((double)h2);
+        double vol = 4.18879020478639 * rad * rad * rad * (double)local_scale * local_scale * local_scale *
+                     ((double)s[0] / vs) * ((double)s[1] / vs) * ((double)s[2] / vs);
+        double est = 1.25 * vol + 16.0;
+        double foot = (double)o.foot[g];
+        if (est > foot)
+            est = foot;
+        atomicAdd(o.est, (unsigned long long)est);
+    }
+}
+
+// ============================================================================ emit
+constexpr int EMIT_THREADS = 256;
+constexpr int EMIT_STAGE = 4096;
+
+struct EmitArgs
+{
+    const PrimFrame* frames;
+    const PrimBox* boxes;
+    const uint64_t* row_scan; // exclusive scan of rows, n + 1 entries
+    uint64_t n;
+    uint64_t total_rows;
+    float vs;
+    KeyLayout kl;
+    uint64_t* out;
+    uint64_t cap;
+    unsigned long long* counter;
+};
+
+__device__ __forceinline__ uint64_t find_prim(const uint64_t* __restrict__ row_scan, uint64_t lo, uint64_t hi, uint64_t row)
+{
+    // largest g in [lo, hi] with row_scan[g] <= row
+    while (lo < hi)
+    {
+        uint64_t mid = (lo + hi + 1) >> 1;
+        if (row_scan[mid] <= row)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void emit_flush(const EmitArgs& a, uint64_t* stage, uint32_t* stage_count,
+                                           unsigned long long* flush_base)
+{
+    uint32_t cnt = *stage_count;
+    if (threadIdx.x == 0)
+        *flush_base = atomicAdd(a.counter, (unsigned long long)cnt);
+    __syncthreads();
+    uint64_t base = *flush_base;
+    for (uint32_t i = threadIdx.x; i < cnt; i += EMIT_THREADS)
+    {
+        uint64_t pos = base + i;
+        if (pos < a.cap)
+            a.out[pos] = stage[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *stage_count = 0u;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(EMIT_THREADS) void k_emit(EmitArgs a)
+{
+    __shared__ uint64_t stage[EMIT_STAGE];
+    __shared__ uint32_t stage_count;
+    __shared__ unsigned long long flush_base;
+    __shared__ int32_t max_len;
+    __shared__ uint64_t g_range[2];
+
+    const uint64_t row0 = (uint64_t)blockIdx.x * EMIT_THREADS;
+    const uint64_t row = row0 + threadIdx.x;
+    if (threadIdx.x == 0)
+    {
+        stage_count = 0u;
+        max_len = 0;
+        g_range[0] = find_prim(a.row_scan, 0, a.n - 1, row0);
+        uint64_t last = row0 + EMIT_THREADS - 1;
+        if (last >= a.total_rows)
+            last = a.total_rows - 1;
+        g_range[1] = find_prim(a.row_scan, 0, a.n - 1, last);
+    }
+    __syncthreads();
+    const bool active = row < a.total_rows;
+    uint64_t g = 0;
+    int32_t len = 0, i = 0, j = 0, k0 = 0;
+    PrimFrame f;
+    if (active)
+    {
+        g = find_prim(a.row_scan, g_range[0], g_range[1], row);
+        PrimBox b = a.boxes[g];
+        f = a.frames[g];
+        uint64_t lr = row - a.row_scan[g];
+        i = b.mn[0] + (int32_t)(lr / (uint64_t)b.rng_y);
+        j = b.mn[1] + (int32_t)(lr % (uint64_t)b.rng_y);
+        k0 = b.mn[2];
+        len = b.rng_z;
+        atomicMax(&max_len, len);
+    }
+    __syncthreads();
+    const int32_t steps = max_len;
+    const float vs = a.vs;
+
+    // row-constant partial dot products of the 4 corner columns (dx, dy)
+    float part[3][4];
+    if (active)
+    {
+#pragma unroll
+        for (int col = 0; col < 4; col++)
+        {
+            float px = (float)i;
+            float py = (float)j;
+            if (col & 1)
+                px += 1.f;
+            if (col & 2)
+                py += 1.f;
+            float lx = px * vs;
+            float ly = py * vs;
+            lx = lx - f.mu[0];
+            ly = ly - f.mu[1];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+                float t = lx * f.rinv[3 * c + 0];
+                t = t + ly * f.rinv[3 * c + 1];
+                part[c][col] = t;
+            }
+        }
+    }
+    // lattice plane z -> per-axis min/max over the 4 columns
+    auto plane = [&](int32_t kz, float pmin[3], float pmax[3]) {
+        float pz = (float)kz;
+        float lz = pz * vs;
+        lz = lz - f.mu[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+        {
+            float tz = lz * f.rinv[3 * c + 2];
+#pragma unroll
+            for (int col = 0; col < 4; col++)
+            {
+                float q = part[c][col] + tz;
+                q = q * f.inv_r;
+                q = q * f.inv_s[c];
+                if (col == 0)
+                {
+                    pmin[c] = q;
+                    pmax[c] = q;
+                }
+                else
+                {
+                    pmin[c] = fminf(pmin[c], q);
+                    pmax[c] = fmaxf(pmax[c], q);
+                }
+            }
+        }
+    };
+    float lo_min[3] = { 0.f, 0.f, 0.f }, lo_max[3] = { 0.f, 0.f, 0.f };
+    if (active)
+        plane(k0, lo_min, lo_max);
+
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    for (int32_t st = 0; st < steps; st++)
+    {
+        bool keep = false;
+        uint64_t key = 0ull;
+        if (st < len)
+        {
+            float hi_min[3], hi_max[3];
+            plane(k0 + st + 1, hi_min, hi_max);
+            float vmin[3], vmax[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+                vmin[c] = fminf(lo_min[c], hi_min[c]);
+                vmax[c] = fmaxf(lo_max[c], hi_max[c]);
+                lo_min[c] = hi_min[c];
+                lo_max[c] = hi_max[c];
+            }
+            keep = box_kept(vmin, vmax);
+            if (keep)
+                key = (encode_voxel(a.kl, i, j, k0 + st) << a.kl.gid_bits) | g;
+        }
+        uint64_t m = __ballot(keep);
+        if (m)
+        {
+            uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
+            uint32_t base = 0u;
+            if (lane == leader)
+                base = atomicAdd(&stage_count, (uint32_t)__popcll(m));
+            base = __shfl(base, (int)leader, 64);
+            if (keep)
+                stage[base + (uint32_t)__popcll(m & lt_mask)] = key;
+        }
+        __syncthreads();
+        if (stage_count > (uint32_t)(EMIT_STAGE - EMIT_THREADS))
+            emit_flush(a, stage, &stage_count, &flush_base);
+    }
+    emit_flush(a, stage, &stage_count, &flush_base);
+}
+
+// ============================================================================ run heads
+constexpr int HEAD_THREADS = 256;
+constexpr int HEAD_ITEMS = 16;
+constexpr int HEAD_TILE = HEAD_THREADS * HEAD_ITEMS;
+
+// counts positions i where (key[i] >> shift) differs from (key[i-1] >> shift)
+__global__ __launch_bounds__(HEAD_THREADS) void k_head_count(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
+                                                             uint32_t* __restrict__ tile_counts)
+{
+    __shared__ uint32_t cnt;
+    if (threadIdx.x == 0)
+        cnt = 0u;
+    __syncthreads();
+    uint64_t base = (uint64_t)blockIdx.x * HEAD_TILE;
+    uint32_t local = 0u;
+    for (int m = 0; m < HEAD_ITEMS; m++)
+    {
+        uint64_t idx = base + (uint64_t)m * HEAD_THREADS + threadIdx.x;
+        if (idx < n)
+        {
+            uint64_t v = keys[idx] >> shift;
+            bool head = idx == 0 || (keys[idx - 1] >> shift) != v;
+            local += head ? 1u : 0u;
+        }
+    }
+    atomicAdd(&cnt, local);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        tile_counts[blockIdx.x] = cnt;
+}
+
+// writes run starts in order: starts[rank] = index of the run head
+__global__ __launch_bounds__(HEAD_THREADS) void k_head_write(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
+                                                             const uint32_t* __restrict__ tile_offsets,
+                                                             uint32_t* __restrict__ starts)
+{
+    __shared__ uint32_t wave_tot[HEAD_THREADS / 64];
+    __shared__ uint32_t run_base;
+    uint64_t base = (uint64_t)blockIdx.x * HEAD_TILE;
+    if (threadIdx.x == 0)
+        run_base = tile_offsets[blockIdx.x];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    for (int m = 0; m < HEAD_ITEMS; m++)
+    {
+        uint64_t idx = base + (uint64_t)m * HEAD_THREADS + threadIdx.x;
+        bool head = false;
+        if (idx < n)
+        {
+            uint64_t v = keys[idx] >> shift;
+            head = idx == 0 || (keys[idx - 1] >> shift) != v;
+        }
+        uint64_t bal = __ballot(head);
+        if (lane == 0)
+            wave_tot[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t pre = run_base;
+        for (uint32_t w = 0; w < wave; w++)
+            pre += wave_tot[w];
+        if (head)
+            starts[pre + (uint32_t)__popcll(bal & lt_mask)] = (uint32_t)idx;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            uint32_t t = 0u;
+            for (uint32_t w = 0; w < HEAD_THREADS / 64; w++)
+                t += wave_tot[w];
+            run_base += t;
+        }
+        __syncthreads();
+    }
+}
+
+// ============================================================================ reduce
+struct ReduceArgs
+{
+    const uint64_t* keys;
+    const uint32_t* seg; // V + 1 entries
+    uint64_t nvox;
+    KeyLayout kl;
+    const PrimFrame* frames;
+    const PrimVal* vals;
+    const PrimBox* boxes;
+    const uint64_t* foot_scan; // exclusive, n + 1
+    const uint32_t* win_base;  // per prim batch
+    uint32_t prim_batch;
+    uint32_t ijk_batch;
+    uint32_t value_mode;
+    float vs;
+    uint64_t* vox_key;
+    uint32_t* vox_val;
+    uint32_t* vox_t;
+    int32_t* bounds; // [6]
+};
+
+__global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = v < a.nvox;
+    int32_t mn[3] = { 2147483647, 2147483647, 2147483647 }, mx[3] = { -2147483647, -2147483647, -2147483647 };
+    if (active)
+    {
+        uint32_t b = a.seg[v], e = a.seg[v + 1];
+        const uint64_t gmask = (1ull << a.kl.gid_bits) - 1ull;
+        uint64_t k0 = a.keys[b];
+        uint64_t vkey = k0 >> a.kl.gid_bits;
+        int32_t i, j, k;
+        decode_voxel(a.kl, vkey, &i, &j, &k);
+        uint32_t value = 0u;
+        float dens = 0.f;
+        for (uint32_t p = b; p < e; p++)
+        {
+            uint32_t gid = (uint32_t)(a.keys[p] & gmask);
+            const PrimFrame f = a.frames[gid];
+            if (a.value_mode == VDBR_VALUE_RGBA8)
+            {
+                float vmin[3], vmax[3];
+                voxel_box(f, i, j, k, a.vs, vmin, vmax);
+                uint32_t rgba = pair_rgba(a.vals[gid], vmin, vmax);
+                value = rgba > value ? rgba : value;
+            }
+            else
+            {
+                dens += pair_density(f, a.vals[gid].opacity, i, j, k, (double)a.vs);
+            }
+        }
+        if (a.value_mode != VDBR_VALUE_RGBA8)
+            value = __float_as_uint(dens);
+        // window of the first pair (lowest gaussian id): fragment slot within its prim batch
+        uint32_t g0 = (uint32_t)(k0 & gmask);
+        PrimBox bx = a.boxes[g0];
+        uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
+                      (uint64_t)(k - bx.mn[2]);
+        uint32_t batch = g0 / a.prim_batch;
+        uint64_t slot = a.foot_scan[g0] - a.foot_scan[(uint64_t)batch * a.prim_batch] + ri;
+        uint32_t t = a.win_base[batch] + (uint32_t)(slot / a.ijk_batch);
+        a.vox_key[v] = vkey;
+        a.vox_val[v] = value;
+        a.vox_t[v] = t;
+        mn[0] = mx[0] = i;
+        mn[1] = mx[1] = j;
+        mn[2] = mx[2] = k;
+    }
+    // wave-reduced voxel bounds, one atomic per wave and axis
+    for (int off = 32; off > 0; off >>= 1)
+        for (int c = 0; c < 3; c++)
+        {
+            mn[c] = min(mn[c], __shfl_xor(mn[c], off, 64));
+            mx[c] = max(mx[c], __shfl_xor(mx[c], off, 64));
+        }
+    if ((threadIdx.x & 63u) == 0u && mn[0] <= mx[0])
+    {
+        atomicMin(&a.bounds[0], mn[0]);
+        atomicMin(&a.bounds[1], mn[1]);
+        atomicMin(&a.bounds[2], mn[2]);
+        atomicMax(&a.bounds[3], mx[0]);
+        atomicMax(&a.bounds[4], mx[1]);
+        atomicMax(&a.bounds[5], mx[2]);
+    }
+}
+
+// ============================================================================ leaves
+// per leaf (run of voxels sharing key >> 9): value mask, first-touch window, tree key
+__global__ __launch_bounds__(256) void k_leaf_info(const uint64_t* __restrict__ vox_key, const uint32_t* __restrict__ vox_t,
+                                                   const uint32_t* __restrict__ leaf_first, uint64_t nleaf, KeyLayout kl,
+                                                   uint64_t* __restrict__ leaf_mask, uint32_t* __restrict__ leaf_t,
+                                                   uint64_t* __restrict__ sort_keys, uint32_t* __restrict__ sort_vals)
+{
+    uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf)
+        return;
+    uint32_t b = leaf_first[l], e = leaf_first[l + 1];
+    uint64_t m[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+    uint32_t t = 0xFFFFFFFFu;
+    for (uint32_t v = b; v < e; v++)
+    {
+        uint32_t n = (uint32_t)(vox_key[v] & 511u);
+        m[n >> 6] |= 1ull << (n & 63u);
+        t = min(t, vox_t[v]);
+    }
+    for (int w = 0; w < 8; w++)
+        leaf_mask[8 * l + w] = m[w];
+    leaf_t[l] = t;
+    int32_t i, j, k;
+    decode_voxel(kl, vox_key[b] & ~511ull, &i, &j, &k);
+    uint32_t ui = (uint32_t)i & 0x1FFFFu, uj = (uint32_t)j & 0x1FFFFu, uk = (uint32_t)k & 0x1FFFFu;
+    uint64_t root_n = ((uint64_t)(ui >> 12) << 10) | ((uint64_t)(uj >> 12) << 5) | (uint64_t)(uk >> 12);
+    uint64_t upper_n = ((uint64_t)((ui & 0xFFFu) >> 7) << 10) | ((uint64_t)((uj & 0xFFFu) >> 7) << 5) |
+                       (uint64_t)((uk & 0xFFFu) >> 7);
+    uint64_t lower_n = ((uint64_t)((ui & 0x7Fu) >> 3) << 8) | ((uint64_t)((uj & 0x7Fu) >> 3) << 4) |
+                       (uint64_t)((uk & 0x7Fu) >> 3);
+    sort_keys[l] = (root_n << 27) | (upper_n << 12) | lower_n;
+    sort_vals[l] = (uint32_t)l;
+}
+
+// gather per tree-ordered leaf: key, t; and inverse permutation
+__global__ __launch_bounds__(256) void k_leaf_gather(const uint64_t* __restrict__ sorted_keys,
+                                                     const uint32_t* __restrict__ sorted_idx, uint64_t nleaf,
+                                                     const uint32_t* __restrict__ leaf_t, uint32_t* __restrict__ tree_t,
+                                                     uint32_t* __restrict__ leaf_pos)
+{
+    uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf)
+        return;
+    uint32_t src = sorted_idx[l];
+    tree_t[l] = leaf_t[src];
+    leaf_pos[src] = (uint32_t)l;
+}
+
+// min of a u32 array over runs [first[r], first[r+1])
+__global__ __launch_bounds__(256) void k_run_min(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ first,
+                                                 uint64_t nrun, uint32_t* __restrict__ out)
+{
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrun)
+        return;
+    uint32_t t = 0xFFFFFFFFu;
+    for (uint32_t x = first[r]; x < first[r + 1]; x++)
+        t = min(t, vals[x]);
+    out[r] = t;
+}
+
+// key of each run head: out[r] = keys[first[r]] >> shift
+__global__ __launch_bounds__(256) void k_run_keys(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ first,
+                                                  uint64_t nrun, uint32_t shift, uint64_t* __restrict__ out)
+{
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrun)
+        return;
+    out[r] = keys[first[r]] >> shift;
+}
+
+// ============================================================================ node order
+// window histogram per level: hist[level * T + t]
+__global__ __launch_bounds__(256) void k_window_hist(const uint32_t* __restrict__ t, uint64_t count, uint32_t T,
+                                                     uint32_t* __restrict__ hist_level)
+{
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= count)
+        return;
+    atomicAdd(&hist_level[t[x]], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_window_totals(const uint32_t* __restrict__ hist, uint32_t T,
+                                                       uint32_t* __restrict__ tot)
+{
+    uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T)
+        return;
+    tot[t] = hist[t] + hist[T + t] + hist[2 * T + t];
+}
+
+// sort key of a node: (t << tshift) | (parent node idx << nbits) | slot
+__global__ __launch_bounds__(256) void k_node_keys(const uint32_t* __restrict__ t, const uint32_t* __restrict__ parent_idx,
+                                                   const uint64_t* __restrict__ slot_key, uint64_t count, uint32_t slot_bits,
+                                                   uint32_t parent_bits, uint64_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals)
+{
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= count)
+        return;
+    uint64_t slot = slot_key[x] & ((1ull << slot_bits) - 1ull);
+    uint64_t par = parent_idx ? (uint64_t)parent_idx[x] : 0ull;
+    keys[x] = ((uint64_t)t[x] << (slot_bits + parent_bits)) | (par << slot_bits) | slot;
+    vals[x] = (uint32_t)x;
+}
+
+// node_idx from the sorted order: 1 + tot_before[t] + level_offset_in_t + (pos - level_before[t])
+__global__ __launch_bounds__(256) void k_node_assign(const uint64_t* __restrict__ sorted_keys,
+                                                     const uint32_t* __restrict__ sorted_vals, uint64_t count,
+                                                     uint32_t tshift, const uint32_t* __restrict__ tot_before,
+                                                     const uint32_t* __restrict__ level_before,
+                                                     const uint32_t* __restrict__ hist, uint32_t T, uint32_t level,
+                                                     uint32_t* __restrict__ node_idx)
+{
+    uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= count)
+        return;
+    uint32_t t = (uint32_t)(sorted_keys[pos] >> tshift);
+    uint32_t within = 0u;
+    for (uint32_t l = 0; l < level; l++)
+        within += hist[l * T + t];
+    node_idx[sorted_vals[pos]] = 1u + tot_before[t] + within + (uint32_t)(pos - level_before[t]);
+}
+
+__global__ __launch_bounds__(256) void k_node_bfs(uint64_t count, uint32_t base, uint32_t* __restrict__ node_idx)
+{
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= count)
+        return;
+    node_idx[x] = base + (uint32_t)x;
+}
+
+// parent node idx of each child run member: child c in [first[p], first[p+1]) -> parent_idx[p]
+__global__ __launch_bounds__(256) void k_parent_idx(const uint32_t* __restrict__ first, uint64_t nparent,
+                                                    const uint32_t* __restrict__ parent_node_idx,
+                                                    uint32_t* __restrict__ child_parent)
+{
+    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nparent)
+        return;
+    uint32_t idx = parent_node_idx ? parent_node_idx[p] : 0u;
+    for (uint32_t c = first[p]; c < first[p + 1]; c++)
+        child_parent[c] = idx;
+}
+
+// per node idx: byte size and value count
+__global__ __launch_bounds__(256) void k_node_sizes(const uint32_t* __restrict__ node_idx, uint64_t count, uint32_t nmax,
+                                                    uint64_t size,
+                                                    const uint64_t* __restrict__ leaf_mask,
+                                                    const uint32_t* __restrict__ tree_to_compact,
+                                                    uint64_t* __restrict__ size_by_idx, uint64_t* __restrict__ vals_by_idx)
+{
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= count)
+        return;
+    uint32_t ni = node_idx[x];
+    if (ni == 0u || ni >= nmax)
+        return; // unreachable for consistent orders; keeps a faulty order from writing out of bounds
+    size_by_idx[ni] = size;
+    uint64_t pc = 0;
+    if (leaf_mask)
+    {
+        uint32_t l = tree_to_compact[x];
+        for (int w = 0; w < 8; w++)
+            pc += (uint64_t)__popcll(leaf_mask[8 * (uint64_t)l + w]);
+    }
+    vals_by_idx[ni] = pc;
+}
+
+// ============================================================================ node2 write
+struct GridInfo
+{
+    uint64_t root_addr;
+    uint64_t nodes_begin; // byte offset of node idx 1
+    uint64_t end;         // end of the node segment
+    uint64_t meta0;
+    uint64_t meta1;
+    uint64_t bboxes;
+    uint64_t values;
+    uint64_t grid_size;
+    uint64_t value_count;
+    uint32_t upper_count;
+    uint32_t lower_count;
+    uint32_t leaf_count;
+    uint32_t value_type;
+    uint32_t node_count;
+    float voxel_size;
+    float voxel_size_inv;
+};
+
+__device__ __forceinline__ void put_u32(uint8_t* g, uint64_t off, uint32_t v)
+{
+    *(uint32_t*)(g + off) = v;
+}
+__device__ __forceinline__ void put_u64(uint8_t* g, uint64_t off, uint64_t v)
+{
+    *(uint64_t*)(g + off) = v;
+}
+__device__ __forceinline__ void put_f32(uint8_t* g, uint64_t off, float v)
+{
+    *(float*)(g + off) = v;
+}
+__device__ __forceinline__ void put_f64(uint8_t* g, uint64_t off, double v)
+{
+    *(double*)(g + off) = v;
+}
+
+// node byte address of node idx ni (ni >= 1): nodes_begin + exclusive size scan
+__device__ __forceinline__ uint64_t node_addr(const GridInfo& gi, const uint64_t* size_scan, uint32_t ni)
+{
+    // indices outside [1, node_count) cannot come from a consistent order; they map to the root
+    // so that no write leaves the grid
+    return (ni == 0u || ni >= gi.node_count) ? gi.root_addr : gi.nodes_begin + size_scan[ni];
+}
+
+__global__ void k_write_header(uint8_t* __restrict__ g, GridInfo gi, const int32_t* __restrict__ bounds)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    // GridBuild.cpp:126-187
+    put_u64(g, VDB_GRID_OFF_MAGIC, VDB_MAGIC_GRID);
+    put_u32(g, VDB_GRID_OFF_VERSION, VDB_VERSION);
+    put_u32(g, VDB_GRID_OFF_FLAGS, 0u);
+    put_u32(g, VDB_GRID_OFF_GRID_INDEX, 0u);
+    put_u32(g, VDB_GRID_OFF_GRID_COUNT, 1u);
+    put_u64(g, VDB_GRID_OFF_GRID_SIZE, gi.grid_size);
+    put_u32(g, VDB_GRID_OFF_NAME + 0, 0x65646f6eu); // "node2"
+    put_u32(g, VDB_GRID_OFF_NAME + 4, 0x00000032u);
+    for (int a = 0; a < 3; a++)
+        put_f64(g, VDB_GRID_OFF_VOXEL_SIZE + 8 * a, (double)gi.voxel_size);
+    put_u32(g, VDB_GRID_OFF_CLASS, 0u);
+    put_u32(g, VDB_GRID_OFF_TYPE, VDB_GRID_TYPE_NODE2);
+    const uint64_t map = VDB_GRID_OFF_MAP;
+    for (int a = 0; a < 3; a++)
+    {
+        put_f32(g, map + VDB_MAP_OFF_MATF + 4 * (4 * a), gi.voxel_size);
+        put_f32(g, map + VDB_MAP_OFF_INVMATF + 4 * (4 * a), gi.voxel_size_inv);
+        put_f64(g, map + VDB_MAP_OFF_MATD + 8 * (4 * a), (double)gi.voxel_size);
+        put_f64(g, map + VDB_MAP_OFF_INVMATD + 8 * (4 * a), (double)gi.voxel_size_inv);
+    }
+    put_u64(g, VDB_GRID_OFF_BLIND_OFFSET, gi.meta0);
+    put_u32(g, VDB_GRID_OFF_BLIND_COUNT, 2u);
+    // tree
+    put_u64(g, VDB_TREE_ADDR + VDB_TREE_OFF_ROOT, gi.root_addr - VDB_TREE_ADDR);
+    put_u32(g, VDB_TREE_ADDR + VDB_TREE_OFF_COUNT_LEAF, gi.leaf_count);
+    put_u32(g, VDB_TREE_ADDR + VDB_TREE_OFF_COUNT_LOWER, gi.lower_count);
+    put_u32(g, VDB_TREE_ADDR + VDB_TREE_OFF_COUNT_UPPER, gi.upper_count);
+    // blind metadata (grid_build_blindmetadata.slang:85-110)
+    put_u64(g, gi.meta0 + VDB_BLIND_OFF_DATA, gi.bboxes - gi.meta0);
+    put_u64(g, gi.meta0 + VDB_BLIND_OFF_COUNT, 2u);
+    put_u32(g, gi.meta0 + VDB_BLIND_OFF_SIZE, 12u);
+    put_u32(g, gi.meta0 + VDB_BLIND_OFF_TYPE, VDB_GRID_TYPE_VEC3F);
+    put_u64(g, gi.meta1 + VDB_BLIND_OFF_DATA, gi.values - gi.meta1);
+    put_u64(g, gi.meta1 + VDB_BLIND_OFF_COUNT, gi.value_count);
+    put_u32(g, gi.meta1 + VDB_BLIND_OFF_SIZE, 4u);
+    put_u32(g, gi.meta1 + VDB_BLIND_OFF_TYPE, gi.value_type);
+    // root bbox (grid_build_compute_bbox.slang:150-158): min over voxels, or the empty sentinel
+    for (int a = 0; a < 3; a++)
+    {
+        put_u32(g, gi.bboxes + 4 * a, (uint32_t)bounds[a]);
+        put_u32(g, gi.bboxes + 12 + 4 * a, (uint32_t)bounds[3 + a]);
+    }
+    // root node idx 0, value idx 2
+    put_u64(g, gi.root_addr + 8ull * VDB_OFF_VALUE_IDX, 2ull);
+}
+
+// packed prefix lanes of a mask held in LDS (count words, count % 4 == 0) -> out words
+__device__ void write_prefix_words(const uint64_t* mask_lds, uint32_t count, uint32_t* pc_lds, uint64_t* out)
+{
+    // popcounts and exclusive scan (single thread scan is enough for <= 512 words per node,
+    // done by wave 0 with a shuffle scan over 8-word chunks)
+    for (uint32_t w = threadIdx.x; w < count; w += blockDim.x)
+        pc_lds[w] = (uint32_t)__popcll(mask_lds[w]);
+    __syncthreads();
+    if (threadIdx.x < 64)
+    {
+        uint32_t lane = threadIdx.x;
+        uint32_t per = (count + 63) / 64; // words per lane
+        uint32_t s = 0;
+        for (uint32_t q = 0; q < per; q++)
+        {
+            uint32_t w = lane * per + q;
+            if (w < count)
+                s += pc_lds[w];
+        }
+        uint32_t incl = s;
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            uint32_t o = __shfl_up(incl, off, 64);
+            if (lane >= (uint32_t)off)
+                incl += o;
+        }
+        uint32_t run = incl - s;
+        for (uint32_t q = 0; q < per; q++)
+        {
+            uint32_t w = lane * per + q;
+            if (w < count)
+            {
+                uint32_t c = pc_lds[w];
+                pc_lds[w] = run; // exclusive
+                run += c;
+            }
+        }
+        if (lane == 63)
+            pc_lds[count] = incl; // total
+    }
+    __syncthreads();
+    uint32_t total = pc_lds[count];
+    for (uint32_t q = threadIdx.x; q < count / 4; q += blockDim.x)
+    {
+        uint64_t l0 = (q == 0) ? total : pc_lds[4 * q];
+        uint64_t v = (l0 & 0xFFFFull) | ((uint64_t)(pc_lds[4 * q + 1] & 0xFFFFu) << 16) |
+                     ((uint64_t)(pc_lds[4 * q + 2] & 0xFFFFu) << 32) | ((uint64_t)(pc_lds[4 * q + 3] & 0xFFFFu) << 48);
+        out[q] = v;
+    }
+}
+
+// root: child mask + key mask over upper root slots, prefixes, children offsets and keys
+__global__ __launch_bounds__(1024) void k_write_root(uint8_t* __restrict__ g, GridInfo gi,
+                                                     const uint64_t* __restrict__ upper_key, uint64_t nupper,
+                                                     const uint32_t* __restrict__ upper_node_idx,
+                                                     const uint64_t* __restrict__ size_scan)
+{
+    __shared__ uint64_t mask[512];
+    __shared__ uint32_t pc[513];
+    for (uint32_t w = threadIdx.x; w < 512; w += blockDim.x)
+        mask[w] = 0ull;
+    __syncthreads();
+    for (uint64_t u = threadIdx.x; u < nupper; u += blockDim.x)
+    {
+        uint32_t n = (uint32_t)upper_key[u];
+        atomicOr((unsigned long long*)&mask[n >> 6], 1ull << (n & 63u));
+    }
+    __syncthreads();
+    uint64_t* root = (uint64_t*)(g + gi.root_addr);
+    for (uint32_t w = threadIdx.x; w < 512; w += blockDim.x)
+    {
+        root[VDB_OFF_CM_ROOT + w] = mask[w];
+        root[VDB_OFF_KM_ROOT + w] = mask[w];
+    }
+    write_prefix_words(mask, 512, pc, root + VDB_OFF_CMP_ROOT);
+    __syncthreads();
+    write_prefix_words(mask, 512, pc, root + VDB_OFF_KMP_ROOT);
+    // children (rank order == root slot order) then keys, rest END_KEY
+    for (uint64_t x = threadIdx.x; x < 65536ull; x += blockDim.x)
+    {
+        uint64_t v;
+        if (x < nupper)
+        {
+            v = node_addr(gi, size_scan, upper_node_idx[x]) - gi.root_addr;
+        }
+        else if (x < 2 * nupper)
+        {
+            uint32_t n = (uint32_t)upper_key[x - nupper];
+            int32_t ci = (int32_t)(((n >> 10) & 0x1Fu) << 12), cj = (int32_t)(((n >> 5) & 0x1Fu) << 12),
+                    ck = (int32_t)((n & 0x1Fu) << 12);
+            const int32_t half = 65536;
+            ci = ci >= half ? ci - half - half : ci;
+            cj = cj >= half ? cj - half - half : cj;
+            ck = ck >= half ? ck - half - half : ck;
+            v = (uint64_t)((uint32_t)ck >> 12) | ((uint64_t)((uint32_t)cj >> 12) << 21) |
+                ((uint64_t)((uint32_t)ci >> 12) << 42);
+        }
+        else
+        {
+            v = VDB_END_KEY;
+        }
+        root[VDB_OFF_CHILDREN_ROOT + x] = v;
+    }
+}
+
+// internal node (upper or lower): one block per node
+template <int WORDS>
+__global__ __launch_bounds__(256) void k_write_internal(uint8_t* __restrict__ g, GridInfo gi,
+                                                        const uint32_t* __restrict__ node_first, // children runs
+                                                        const uint64_t* __restrict__ child_slot_key,
+                                                        uint32_t slot_bits,
+                                                        const uint32_t* __restrict__ node_idx,
+                                                        const uint32_t* __restrict__ child_node_idx,
+                                                        const uint64_t* __restrict__ size_scan,
+                                                        const uint64_t* __restrict__ vals_scan,
+                                                        uint32_t off_cmp, uint32_t off_cm, uint32_t off_children)
+{
+    __shared__ uint64_t mask[WORDS];
+    __shared__ uint32_t pc[WORDS + 1];
+    const uint32_t node = blockIdx.x;
+    for (uint32_t w = threadIdx.x; w < WORDS; w += blockDim.x)
+        mask[w] = 0ull;
+    __syncthreads();
+    uint32_t b = node_first[node], e = node_first[node + 1];
+    const uint64_t smask = (1ull << slot_bits) - 1ull;
+    for (uint32_t c = b + threadIdx.x; c < e; c += blockDim.x)
+    {
+        uint32_t n = (uint32_t)(child_slot_key[c] & smask);
+        atomicOr((unsigned long long*)&mask[n >> 6], 1ull << (n & 63u));
+    }
+    __syncthreads();
+    uint32_t ni = node_idx[node];
+    uint64_t addr = node_addr(gi, size_scan, ni);
+    uint64_t* nd = (uint64_t*)(g + addr);
+    for (uint32_t w = threadIdx.x; w < WORDS; w += blockDim.x)
+        nd[off_cm + w] = mask[w];
+    write_prefix_words(mask, WORDS, pc, nd + off_cmp);
+    for (uint32_t c = b + threadIdx.x; c < e; c += blockDim.x)
+        nd[off_children + (c - b)] = node_addr(gi, size_scan, child_node_idx[c]) - addr;
+    if (threadIdx.x == 0)
+    {
+        nd[VDB_OFF_NODE_IDX] = ni;
+        nd[VDB_OFF_VALUE_IDX] = 2ull + vals_scan[ni];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_write_leaves(uint8_t* __restrict__ g, GridInfo gi, uint64_t nleaf,
+                                                      const uint32_t* __restrict__ tree_to_compact,
+                                                      const uint64_t* __restrict__ leaf_mask,
+                                                      const uint32_t* __restrict__ leaf_node_idx,
+                                                      const uint64_t* __restrict__ size_scan,
+                                                      const uint64_t* __restrict__ vals_scan)
+{
+    uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf)
+        return;
+    uint32_t src = tree_to_compact[l];
+    uint32_t ni = leaf_node_idx[l];
+    uint64_t* nd = (uint64_t*)(g + node_addr(gi, size_scan, ni));
+    uint64_t m[8];
+    uint32_t ex[9];
+    ex[0] = 0;
+    for (int w = 0; w < 8; w++)
+    {
+        m[w] = leaf_mask[8 * (uint64_t)src + w];
+        ex[w + 1] = ex[w] + (uint32_t)__popcll(m[w]);
+    }
+    nd[VDB_OFF_NODE_IDX] = ni;
+    nd[VDB_OFF_VALUE_IDX] = 2ull + vals_scan[ni];
+    for (int w = 0; w < 8; w++)
+        nd[VDB_OFF_VM_LEAF + w] = m[w];
+    nd[VDB_OFF_VMP_LEAF + 0] = (uint64_t)(ex[8] & 0xFFFFu) | ((uint64_t)ex[1] << 16) | ((uint64_t)ex[2] << 32) |
+                               ((uint64_t)ex[3] << 48);
+    nd[VDB_OFF_VMP_LEAF + 1] = (uint64_t)ex[4] | ((uint64_t)ex[5] << 16) | ((uint64_t)ex[6] << 32) |
+                               ((uint64_t)ex[7] << 48);
+}
+
+__global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, GridInfo gi, uint64_t nvox,
+                                                      const uint32_t* __restrict__ vox_val,
+                                                      const uint64_t* __restrict__ vox_key,
+                                                      const uint32_t* __restrict__ leaf_first,
+                                                      const uint32_t* __restrict__ leaf_of_run_start, // unused
+                                                      const uint32_t* __restrict__ leaf_pos,
+                                                      const uint32_t* __restrict__ leaf_node_idx,
+                                                      const uint64_t* __restrict__ vals_scan,
+                                                      const uint32_t* __restrict__ vox_leaf)
+{
+    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox)
+        return;
+    uint32_t l = vox_leaf[v];
+    uint32_t rank = (uint32_t)v - leaf_first[l];
+    uint32_t ni = leaf_node_idx[leaf_pos[l]];
+    if (ni >= gi.node_count)
+        return;
+    uint64_t idx = 2ull + vals_scan[ni] + rank;
+    if (idx < gi.value_count)
+        *(uint32_t*)(g + gi.values + 4ull * idx) = vox_val[v];
+    (void)vox_key;
+    (void)leaf_of_run_start;
+}
+
+// voxel -> leaf run index
+__global__ __launch_bounds__(256) void k_run_member(const uint32_t* __restrict__ first, uint64_t nrun,
+                                                    uint32_t* __restrict__ member_of)
+{
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrun)
+        return;
+    for (uint32_t x = first[r]; x < first[r + 1]; x++)
+        member_of[x] = (uint32_t)r;
+}
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ src, uint64_t stride, uint64_t count,
+                             uint64_t* __restrict__ dst)
+{
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= count)
+        return;
+    dst[x] = src[x * stride];
+}
+
+__global__ void k_init_bounds(int32_t* b)
+{
+    if (threadIdx.x < 3)
+        b[threadIdx.x] = 2147483647;
+    else if (threadIdx.x < 6)
+        b[threadIdx.x] = -2147483647;
+}
+
+__global__ void k_pairs_decode(const uint64_t* __restrict__ keys, uint64_t n, KeyLayout kl, int32_t* __restrict__ ijk,
+                               uint32_t* __restrict__ gid)
+{
+    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n)
+        return;
+    uint64_t k = keys[x];
+    if (gid)
+        gid[x] = (uint32_t)(k & ((1ull << kl.gid_bits) - 1ull));
+    if (ijk)
+    {
+        int32_t i, j, kk;
+        decode_voxel(kl, k >> kl.gid_bits, &i, &j, &kk);
+        ijk[3 * x + 0] = i;
+        ijk[3 * x + 1] = j;
+        ijk[3 * x + 2] = kk;
+    }
+}
+
+// ============================================================================ host side
+static inline uint32_t blocks_for(uint64_t n, uint32_t t = 256)
+{
+    return (uint32_t)((n + t - 1) / t);
+}
+
+static inline uint32_t bits_for(uint64_t range) // bits to represent values in [0, range)
+{
+    uint32_t b = 0;
+    while (b < 64 && (1ull << b) < range)
+        b++;
+    return b;
+}
+
+// run heads of keys >> shift: writes starts[0..runs) and starts[runs] = n; returns runs
+static int run_heads(Context* ctx, const uint64_t* keys, uint64_t n, uint32_t shift, uint32_t* starts,
+                     uint64_t* runs_out)
+{
+    if (n == 0)
+    {
+        *runs_out = 0;
+        VDBR_CHECK(ctx, hipMemsetAsync(starts, 0, 4, ctx->stream));
+        return 0;
+    }
+    if (n >= 0xFFFFFFFFull)
+        return ctx->fail("run heads: more than 2^32-1 elements");
+    uint64_t tiles = (n + HEAD_TILE - 1) / HEAD_TILE;
+    uint32_t* tile_counts = (uint32_t*)ctx->ws(WS_SEG_TILES, (tiles + 1) * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_head_count, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift, tile_counts);
+    if (scan_u32(ctx, tile_counts, tile_counts, tiles, false, tile_counts + tiles))
+        return -1;
+    hipLaunchKernelGGL(k_head_write, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
+                       (const uint32_t*)tile_counts, starts);
+    uint32_t runs = 0;
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, tile_counts + tiles, 4, hipMemcpyDeviceToHost, ctx->stream));
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    runs = *(uint32_t*)ctx->pinned;
+    ((uint32_t*)ctx->pinned)[1] = (uint32_t)n;
+    VDBR_CHECK(ctx, hipMemcpyAsync(starts + runs, (uint32_t*)ctx->pinned + 1, 4, hipMemcpyHostToDevice, ctx->stream));
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *runs_out = runs;
+    return 0;
+}
+
+struct Timer
+{
+    hipEvent_t ev[8];
+    int n = 0;
+};
+
+int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_params_t* prm, void** grid_dev,
+                 uint64_t* grid_size_out, vdbr_raster_stats_t* stats)
+{
+    if (!gs || !prm || !grid_dev || !grid_size_out)
+        return ctx->fail("raster_node2: null argument");
+    const uint64_t n = gs->count;
+    const float vs = prm->voxel_size;
+    if (!(vs > 0.f) || !(vs < 3.0e38f))
+        return ctx->fail("raster_node2: voxel_size must be a positive finite number");
+    if (n >= (1ull << 32))
+        return ctx->fail("raster_node2: more than 2^32-1 Gaussians");
+    if (n > 0 && (!gs->means || !gs->opacities || !gs->quats || !gs->scales || !gs->colors))
+        return ctx->fail("raster_node2: null Gaussian array");
+    if (prm->value_mode != VDBR_VALUE_RGBA8 && prm->value_mode != VDBR_VALUE_DENSITY)
+        return ctx->fail("raster_node2: unknown value_mode");
+    const uint32_t PB = prm->prim_batch_size ? prm->prim_batch_size : 262144u;
+    const uint32_t W = prm->ijk_batch_size ? prm->ijk_batch_size : 8388608u;
+    const uint32_t max_nodes = prm->max_node_count ? prm->max_node_count : 2097152u;
+    const float vs_inv = 1.f / vs;
+    hipStream_t st = ctx->stream;
+
+    hipEvent_t ev[7];
+    for (int e = 0; e < 7; e++)
+        VDBR_CHECK(ctx, hipEventCreate(&ev[e]));
+    struct EvGuard
+    {
+        hipEvent_t* e;
+        ~EvGuard()
+        {
+            for (int i = 0; i < 7; i++)
+                (void)hipEventDestroy(e[i]);
+        }
+    } evg{ ev };
+    VDBR_CHECK(ctx, hipEventRecord(ev[0], st));
+
+    // ---------------------------------------------------------------- preprocess + setup
+    if ((prm->flags & VDBR_FLAG_RAW_INPUTS) && n > 0)
+    {
+        if (!gs->sh0)
+            return ctx->fail("raster_node2: raw inputs need sh0");
+        hipLaunchKernelGGL(k_preprocess, dim3(blocks_for(n)), dim3(256), 0, st, n, gs->opacities, gs->quats, gs->scales,
+                           gs->sh0, gs->colors);
+    }
+    const uint64_t nn = n > 0 ? n : 1;
+    PrimFrame* frames = (PrimFrame*)ctx->ws(WS_FRAMES, nn * sizeof(PrimFrame));
+    PrimBox* boxes = (PrimBox*)ctx->ws(WS_BOXES, nn * sizeof(PrimBox));
+    PrimVal* pvals = (PrimVal*)ctx->ws(WS_VALS, nn * sizeof(PrimVal));
+    uint64_t* foot = (uint64_t*)ctx->ws(WS_FOOT, nn * sizeof(uint64_t));
+    uint64_t* foot_scan = (uint64_t*)ctx->ws(WS_FOOT_SCAN, (nn + 1) * sizeof(uint64_t));
+    uint64_t* rows = (uint64_t*)ctx->ws(WS_ROWS, nn * sizeof(uint64_t));
+    uint64_t* row_scan = (uint64_t*)ctx->ws(WS_ROW_SCAN, (nn + 1) * sizeof(uint64_t));
+    uint8_t* misc = (uint8_t*)ctx->ws(WS_MISC, 4096);
+    int32_t* bounds = (int32_t*)misc;                               // 6 ints: AABB bounds
+    int32_t* vbounds = (int32_t*)(misc + 32);                      // 6 ints: voxel bounds
+    unsigned long long* est = (unsigned long long*)(misc + 64);    // pair estimate
+    unsigned long long* counter = (unsigned long long*)(misc + 72); // emitted pairs
+    if (!frames || !boxes || !pvals || !foot || !foot_scan || !rows || !row_scan || !misc)
+        return -1;
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, bounds);
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, vbounds);
+    VDBR_CHECK(ctx, hipMemsetAsync(est, 0, 16, st));
+    if (n > 0)
+    {
+        SetupOut so{ frames, boxes, pvals, foot, rows, bounds, est };
+        hipLaunchKernelGGL(k_setup, dim3(blocks_for(n)), dim3(256), 0, st, n, gs->means, gs->quats, gs->scales,
+                           gs->opacities, gs->colors, vs, vs_inv, prm->cull_scale, so);
+        if (scan_u64(ctx, foot, foot_scan, n, false, foot_scan + n))
+            return -1;
+        if (scan_u64(ctx, rows, row_scan, n, false, row_scan + n))
+            return -1;
+    }
+    else
+    {
+        VDBR_CHECK(ctx, hipMemsetAsync(foot_scan, 0, 8, st));
+        VDBR_CHECK(ctx, hipMemsetAsync(row_scan, 0, 8, st));
+    }
+    // batch starts of the footprint scan (window model)
+    const uint64_t nbatch = n > 0 ? (n + PB - 1) / PB : 0;
+    std::vector<uint64_t> batch_start(nbatch + 1, 0);
+    uint64_t* bs_dev = nullptr;
+    {
+        uint64_t* host = ctx->pinned;
+        VDBR_CHECK(ctx, hipMemcpyAsync(host, bounds, 24, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipMemcpyAsync(host + 3, est, 8, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipMemcpyAsync(host + 4, foot_scan + n, 8, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipMemcpyAsync(host + 5, row_scan + n, 8, hipMemcpyDeviceToHost, st));
+        bs_dev = (uint64_t*)ctx->ws(WS_BATCH, (nbatch + 2) * sizeof(uint64_t) + (nbatch + 2) * sizeof(uint32_t));
+        if (!bs_dev)
+            return -1;
+        if (nbatch > 0)
+        {
+            hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nbatch)), dim3(256), 0, st, foot_scan, (uint64_t)PB, nbatch,
+                               bs_dev);
+        }
+        VDBR_CHECK(ctx, hipStreamSynchronize(st));
+        std::vector<uint64_t> tmp(nbatch);
+        if (nbatch > 0)
+            VDBR_CHECK(ctx, hipMemcpy(tmp.data(), bs_dev, nbatch * 8, hipMemcpyDeviceToHost));
+        for (uint64_t b = 0; b < nbatch; b++)
+            batch_start[b] = tmp[b];
+        batch_start[nbatch] = host[4];
+    }
+    int32_t hb[6];
+    memcpy(hb, ctx->pinned, 24);
+    const uint64_t est_pairs = ctx->pinned[3];
+    const uint64_t frag_total = ctx->pinned[4];
+    const uint64_t total_rows = ctx->pinned[5];
+    // window bases per prim batch
+    std::vector<uint32_t> win_base(nbatch + 1, 0);
+    uint64_t wtot = 0;
+    for (uint64_t b = 0; b < nbatch; b++)
+    {
+        win_base[b] = (uint32_t)wtot;
+        uint64_t tb = batch_start[b + 1] - batch_start[b];
+        wtot += (tb + W - 1) / W;
+    }
+    if (wtot >= (1ull << 20))
+        return ctx->fail("raster_node2: more than 2^20 fragment windows in the node-order model");
+    const uint32_t T = (uint32_t)(wtot > 0 ? wtot : 1);
+    uint32_t* win_base_dev = (uint32_t*)(bs_dev + (nbatch + 2));
+    if (nbatch > 0)
+        VDBR_CHECK(ctx, hipMemcpy(win_base_dev, win_base.data(), nbatch * 4, hipMemcpyHostToDevice));
+    VDBR_CHECK(ctx, hipEventRecord(ev[1], st));
+
+    // ---------------------------------------------------------------- key layout
+    KeyLayout kl;
+    memset(&kl, 0, sizeof(kl));
+    bool any = frag_total > 0;
+    if (any)
+    {
+        for (int a = 0; a < 3; a++)
+        {
+            if (hb[a] < -65536 || hb[3 + a] > 65535)
+                return ctx->fail("raster_node2: voxel coordinates outside [-65536, 65535] (node2 root range)");
+            kl.lmin[a] = hb[a] >> 3;
+            int32_t lmax = hb[3 + a] >> 3;
+            kl.bits[a] = bits_for((uint64_t)(lmax - kl.lmin[a] + 1));
+        }
+    }
+    kl.gid_bits = bits_for(n > 1 ? n : 2);
+    kl.voxel_bits = kl.bits[0] + kl.bits[1] + kl.bits[2] + 9u;
+    const uint32_t key_bits = kl.voxel_bits + kl.gid_bits;
+    if (key_bits > 64)
+        return ctx->fail("raster_node2: pair keys need more than 64 bits");
+
+    // ---------------------------------------------------------------- emit
+    uint64_t pair_count = 0;
+    uint64_t* pairs = nullptr;
+    uint32_t attempts = 0;
+    if (any)
+    {
+        uint64_t cap = est_pairs + est_pairs / 4 + 1024;
+        if (cap > frag_total)
+            cap = frag_total;
+        for (;;)
+        {
+            attempts++;
+            pairs = (uint64_t*)ctx->ws(WS_PAIRS, cap * sizeof(uint64_t));
+            if (!pairs)
+                return -1;
+            VDBR_CHECK(ctx, hipMemsetAsync(counter, 0, 8, st));
+            EmitArgs ea;
+            ea.frames = frames;
+            ea.boxes = boxes;
+            ea.row_scan = row_scan;
+            ea.n = n;
+            ea.total_rows = total_rows;
+            ea.vs = vs;
+            ea.kl = kl;
+            ea.out = pairs;
+            ea.cap = cap;
+            ea.counter = counter;
+            uint64_t blocks = (total_rows + EMIT_THREADS - 1) / EMIT_THREADS;
+            if (blocks > 0x7FFFFFFFull)
+                return ctx->fail("raster_node2: too many AABB rows for one launch");
+            hipLaunchKernelGGL(k_emit, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, ea);
+            VDBR_CHECK(ctx, hipGetLastError());
+            VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, counter, 8, hipMemcpyDeviceToHost, st));
+            VDBR_CHECK(ctx, hipStreamSynchronize(st));
+            pair_count = ctx->pinned[0];
+            if (pair_count <= cap)
+                break;
+            if (attempts >= 2)
+                return ctx->fail("raster_node2: pair buffer overflow after resize");
+            cap = pair_count;
+        }
+    }
+    VDBR_CHECK(ctx, hipEventRecord(ev[2], st));
+
+    // ---------------------------------------------------------------- sort
+    uint64_t* sorted = pairs;
+    if (pair_count > 1)
+    {
+        if (radix_sort_u64_noswap(ctx, pairs, pair_count, 0, key_bits, &sorted))
+            return -1;
+    }
+    VDBR_CHECK(ctx, hipEventRecord(ev[3], st));
+    ctx->last_pairs = sorted;
+    ctx->last_pair_count = pair_count;
+    ctx->last_gid_bits = kl.gid_bits;
+    for (int a = 0; a < 3; a++)
+    {
+        ctx->last_lmin[a] = kl.lmin[a];
+        ctx->last_bits[a] = kl.bits[a];
+    }
+
+    // ---------------------------------------------------------------- reduce per voxel
+    if (pair_count >= 0xFFFFFFFFull)
+        return ctx->fail("raster_node2: more than 2^32-1 pairs in one build");
+    uint32_t* seg = (uint32_t*)ctx->ws(WS_SEG, (pair_count + 2) * sizeof(uint32_t));
+    uint64_t nvox = 0;
+    if (run_heads(ctx, sorted, pair_count, kl.gid_bits, seg, &nvox))
+        return -1;
+    const uint64_t nv1 = nvox > 0 ? nvox : 1;
+    uint64_t* vox_key = (uint64_t*)ctx->ws(WS_VOX_KEY, nv1 * sizeof(uint64_t));
+    uint32_t* vox_val = (uint32_t*)ctx->ws(WS_VOX_VAL, nv1 * sizeof(uint32_t));
+    uint32_t* vox_t = (uint32_t*)ctx->ws(WS_VOX_T, 2 * nv1 * sizeof(uint32_t));
+    uint32_t* vox_leaf = vox_t + nv1;
+    if (nvox > 0)
+    {
+        ReduceArgs ra;
+        ra.keys = sorted;
+        ra.seg = seg;
+        ra.nvox = nvox;
+        ra.kl = kl;
+        ra.frames = frames;
+        ra.vals = pvals;
+        ra.boxes = boxes;
+        ra.foot_scan = foot_scan;
+        ra.win_base = win_base_dev;
+        ra.prim_batch = PB;
+        ra.ijk_batch = W;
+        ra.value_mode = prm->value_mode;
+        ra.vs = vs;
+        ra.vox_key = vox_key;
+        ra.vox_val = vox_val;
+        ra.vox_t = vox_t;
+        ra.bounds = vbounds;
+        hipLaunchKernelGGL(k_reduce, dim3(blocks_for(nvox)), dim3(256), 0, st, ra);
+        VDBR_CHECK(ctx, hipGetLastError());
+    }
+    VDBR_CHECK(ctx, hipEventRecord(ev[4], st));
+
+    // ---------------------------------------------------------------- leaves / lowers / uppers
+    uint32_t* leaf_first = (uint32_t*)ctx->ws(WS_LEAF_FIRST, (nv1 + 2) * sizeof(uint32_t));
+    uint64_t nleaf = 0;
+    if (run_heads(ctx, vox_key, nvox, 9, leaf_first, &nleaf))
+        return -1;
+    const uint64_t nl1 = nleaf > 0 ? nleaf : 1;
+    uint64_t* leaf_mask = (uint64_t*)ctx->ws(WS_LEAF_MASK, nl1 * 8 * sizeof(uint64_t));
+    uint32_t* leaf_info = (uint32_t*)ctx->ws(WS_LEAF_INFO, nl1 * 6 * sizeof(uint32_t));
+    uint32_t* leaf_t = leaf_info;               // compact order
+    uint32_t* tree_t = leaf_info + nl1;         // tree order
+    uint32_t* leaf_pos = leaf_info + 2 * nl1;   // compact -> tree position
+    uint32_t* leaf_node_idx = leaf_info + 3 * nl1;
+    uint32_t* leaf_parent = leaf_info + 4 * nl1;
+    uint64_t* lkeys = (uint64_t*)ctx->ws(WS_LEAF_SORT_KEYS, nl1 * sizeof(uint64_t));
+    uint32_t* lvals = (uint32_t*)ctx->ws(WS_LEAF_SORT_VALS, nl1 * sizeof(uint32_t));
+    if (!leaf_mask || !leaf_info || !lkeys || !lvals)
+        return -1;
+    if (nleaf > 0)
+    {
+        hipLaunchKernelGGL(k_leaf_info, dim3(blocks_for(nleaf)), dim3(256), 0, st, vox_key, vox_t, leaf_first, nleaf, kl,
+                           leaf_mask, leaf_t, lkeys, lvals);
+        hipLaunchKernelGGL(k_run_member, dim3(blocks_for(nleaf)), dim3(256), 0, st, leaf_first, nleaf, vox_leaf);
+        if (radix_sort_u64(ctx, lkeys, lvals, nleaf, 0, 42, nullptr))
+            return -1;
+        hipLaunchKernelGGL(k_leaf_gather, dim3(blocks_for(nleaf)), dim3(256), 0, st, lkeys, lvals, nleaf, leaf_t, tree_t,
+                           leaf_pos);
+    }
+    // lowers: runs of tree key >> 12 over the tree-ordered leaves
+    uint32_t* lower_first = (uint32_t*)ctx->ws(WS_LOWER, (nl1 + 2) * sizeof(uint32_t) * 4);
+    uint64_t nlower = 0;
+    if (run_heads(ctx, lkeys, nleaf, 12, lower_first, &nlower))
+        return -1;
+    const uint64_t nlo1 = nlower > 0 ? nlower : 1;
+    uint64_t* lower_key = (uint64_t*)ctx->ws(WS_LOWER_MASK, nlo1 * sizeof(uint64_t) * 2);
+    uint32_t* lower_t = lower_first + (nl1 + 2);
+    uint32_t* lower_node_idx = lower_t + nlo1;
+    uint32_t* lower_parent = lower_node_idx + nlo1;
+    if (nlower > 0)
+    {
+        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nlower)), dim3(256), 0, st, lkeys, lower_first, nlower, 12u,
+                           lower_key);
+        hipLaunchKernelGGL(k_run_min, dim3(blocks_for(nlower)), dim3(256), 0, st, tree_t, lower_first, nlower, lower_t);
+    }
+    uint32_t* upper_first = (uint32_t*)ctx->ws(WS_UPPER, (nlo1 + 2) * sizeof(uint32_t) * 4);
+    uint64_t nupper = 0;
+    if (run_heads(ctx, lower_key, nlower, 15, upper_first, &nupper))
+        return -1;
+    const uint64_t nu1 = nupper > 0 ? nupper : 1;
+    uint64_t* upper_key = (uint64_t*)ctx->ws(WS_UPPER_MASK, nu1 * sizeof(uint64_t));
+    uint32_t* upper_t = upper_first + (nlo1 + 2);
+    uint32_t* upper_node_idx = upper_t + nu1;
+    if (nupper > 0)
+    {
+        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nupper)), dim3(256), 0, st, lower_key, upper_first, nupper, 15u,
+                           upper_key);
+        hipLaunchKernelGGL(k_run_min, dim3(blocks_for(nupper)), dim3(256), 0, st, lower_t, upper_first, nupper, upper_t);
+    }
+    const uint64_t node_count = 1 + nupper + nlower + nleaf;
+    if (node_count > max_nodes)
+        return ctx->fail("raster_node2: node count exceeds max_node_count");
+
+    // ---------------------------------------------------------------- node order
+    bool spatial = (prm->flags & VDBR_FLAG_SPATIAL_ORDER) != 0 || T <= 1;
+    if (!spatial && node_count > 1)
+    {
+        // histogram of first-touch windows per level
+        uint32_t* hist = (uint32_t*)ctx->ws(WS_HIST, (size_t)(3 * T + 4 * (T + 1)) * sizeof(uint32_t));
+        uint32_t* tot = hist + 3 * T;
+        uint32_t* tot_before = tot + (T + 1);
+        uint32_t* lev_before = tot_before + (T + 1); // 2 arrays of T+1 used per level in turn
+        VDBR_CHECK(ctx, hipMemsetAsync(hist, 0, (size_t)3 * T * sizeof(uint32_t), st));
+        if (nupper)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_t, nupper, T, hist);
+        if (nlower)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_t, nlower, T, hist + T);
+        if (nleaf)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nleaf)), dim3(256), 0, st, tree_t, nleaf, T, hist + 2 * T);
+        hipLaunchKernelGGL(k_window_totals, dim3(blocks_for(T)), dim3(256), 0, st, hist, T, tot);
+        if (scan_u32(ctx, tot, tot_before, T, false, nullptr))
+            return -1;
+        uint64_t* nkeys = (uint64_t*)ctx->ws(WS_NODE_KEYS, nl1 * sizeof(uint64_t));
+        uint32_t* nvals = (uint32_t*)ctx->ws(WS_NODE_VALS, nl1 * sizeof(uint32_t));
+        struct Level
+        {
+            uint64_t count;
+            const uint32_t* t;
+            const uint32_t* parent;
+            const uint64_t* slot_key;
+            uint32_t slot_bits;
+            uint32_t* node_idx;
+        };
+        // parents: upper -> root (0); lower -> its upper; leaf -> its lower
+        Level levels[3] = { { nupper, upper_t, nullptr, upper_key, 15u, upper_node_idx },
+                            { nlower, lower_t, lower_parent, lower_key, 15u, lower_node_idx },
+                            { nleaf, tree_t, leaf_parent, lkeys, 12u, leaf_node_idx } };
+        const uint32_t parent_bits = bits_for(node_count + 1);
+        if (20 + parent_bits + 15 > 64)
+            return ctx->fail("raster_node2: node order keys exceed 64 bits");
+        for (uint32_t lv = 0; lv < 3; lv++)
+        {
+            Level& L = levels[lv];
+            if (L.count == 0)
+                continue;
+            if (lv == 1)
+                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_first, nupper,
+                                   upper_node_idx, lower_parent);
+            if (lv == 2)
+                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_first, nlower,
+                                   lower_node_idx, leaf_parent);
+            if (scan_u32(ctx, hist + lv * T, lev_before, T, false, nullptr))
+                return -1;
+            hipLaunchKernelGGL(k_node_keys, dim3(blocks_for(L.count)), dim3(256), 0, st, L.t, lv == 0 ? nullptr : L.parent,
+                               L.slot_key, L.count, L.slot_bits, parent_bits, nkeys, nvals);
+            if (radix_sort_u64(ctx, nkeys, nvals, L.count, 0, 20 + parent_bits + L.slot_bits, nullptr))
+                return -1;
+            hipLaunchKernelGGL(k_node_assign, dim3(blocks_for(L.count)), dim3(256), 0, st, nkeys, nvals, L.count,
+                               parent_bits + L.slot_bits, tot_before, lev_before, hist, T, lv, L.node_idx);
+        }
+    }
+    else
+    {
+        if (nupper)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nupper)), dim3(256), 0, st, nupper, 1u, upper_node_idx);
+        if (nlower)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nlower)), dim3(256), 0, st, nlower, (uint32_t)(1 + nupper),
+                               lower_node_idx);
+        if (nleaf)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nleaf)), dim3(256), 0, st, nleaf,
+                               (uint32_t)(1 + nupper + nlower), leaf_node_idx);
+    }
+    // sizes and value counts by node idx -> exclusive scans
+    uint64_t* byidx = (uint64_t*)ctx->ws(WS_NODE_BYIDX, (node_count + 1) * 2 * sizeof(uint64_t));
+    uint64_t* size_by_idx = byidx;
+    uint64_t* vals_by_idx = byidx + (node_count + 1);
+    uint64_t* nscan = (uint64_t*)ctx->ws(WS_NODE_SCAN, (node_count + 1) * 2 * sizeof(uint64_t));
+    uint64_t* size_scan = nscan;
+    uint64_t* vals_scan = nscan + (node_count + 1);
+    VDBR_CHECK(ctx, hipMemsetAsync(byidx, 0, (node_count + 1) * 2 * sizeof(uint64_t), st));
+    if (nupper)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_node_idx, nupper, (uint32_t)node_count, VDB_SIZE_UPPER,
+                           nullptr, nullptr, size_by_idx, vals_by_idx);
+    if (nlower)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_node_idx, nlower, (uint32_t)node_count, VDB_SIZE_LOWER,
+                           nullptr, nullptr, size_by_idx, vals_by_idx);
+    if (nleaf)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nleaf)), dim3(256), 0, st, leaf_node_idx, nleaf, (uint32_t)node_count, VDB_SIZE_LEAF,
+                           leaf_mask, lvals, size_by_idx, vals_by_idx);
+    // node idx 0 is the root: its slot holds size 0 so the scan at idx 1 starts at 0
+    if (scan_u64(ctx, size_by_idx, size_scan, node_count, false, size_scan + node_count))
+        return -1;
+    if (scan_u64(ctx, vals_by_idx, vals_scan, node_count, false, vals_scan + node_count))
+        return -1;
+
+    // ---------------------------------------------------------------- grid layout
+    GridInfo gi;
+    memset(&gi, 0, sizeof(gi));
+    gi.root_addr = VDB_ROOT_ADDR;
+    gi.nodes_begin = VDB_ROOT_ADDR + VDB_SIZE_ROOT;
+    gi.end = gi.nodes_begin + nupper * VDB_SIZE_UPPER + nlower * VDB_SIZE_LOWER + nleaf * VDB_SIZE_LEAF;
+    gi.meta0 = gi.end;
+    gi.meta1 = gi.end + VDB_BLINDMETADATA_SIZE;
+    gi.bboxes = gi.meta1 + VDB_BLINDMETADATA_SIZE;
+    gi.values = gi.bboxes + 32;
+    gi.value_count = 2 + nvox;
+    gi.grid_size = gi.values + 4 * gi.value_count;
+    gi.upper_count = (uint32_t)nupper;
+    gi.lower_count = (uint32_t)nlower;
+    gi.leaf_count = (uint32_t)nleaf;
+    gi.value_type = prm->value_mode == VDBR_VALUE_RGBA8 ? VDB_GRID_TYPE_RGBA8 : VDB_GRID_TYPE_FLOAT;
+    gi.node_count = (uint32_t)node_count;
+    gi.voxel_size = vs;
+    gi.voxel_size_inv = vs_inv;
+    uint8_t* grid = (uint8_t*)ctx->ws(WS_GRID, (gi.grid_size + 7) & ~7ull);
+    if (!grid)
+        return -1;
+    VDBR_CHECK(ctx, hipMemsetAsync(grid, 0, (gi.grid_size + 7) & ~7ull, st));
+    hipLaunchKernelGGL(k_write_header, dim3(1), dim3(64), 0, st, grid, gi, (const int32_t*)vbounds);
+    hipLaunchKernelGGL(k_write_root, dim3(1), dim3(1024), 0, st, grid, gi, (const uint64_t*)upper_key, nupper,
+                       (const uint32_t*)upper_node_idx, (const uint64_t*)size_scan);
+    if (nupper)
+        hipLaunchKernelGGL((k_write_internal<512>), dim3((uint32_t)nupper), dim3(256), 0, st, grid, gi,
+                           (const uint32_t*)upper_first, (const uint64_t*)lower_key, 15u, (const uint32_t*)upper_node_idx,
+                           (const uint32_t*)lower_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan,
+                           VDB_OFF_CMP_UPPER, VDB_OFF_CM_UPPER, VDB_OFF_CHILDREN_UPPER);
+    if (nlower)
+        hipLaunchKernelGGL((k_write_internal<64>), dim3((uint32_t)nlower), dim3(256), 0, st, grid, gi,
+                           (const uint32_t*)lower_first, (const uint64_t*)lkeys, 12u, (const uint32_t*)lower_node_idx,
+                           (const uint32_t*)leaf_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan,
+                           VDB_OFF_CMP_LOWER, VDB_OFF_CM_LOWER, VDB_OFF_CHILDREN_LOWER);
+    if (nleaf)
+        hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nleaf)), dim3(256), 0, st, grid, gi, nleaf,
+                           (const uint32_t*)lvals, (const uint64_t*)leaf_mask, (const uint32_t*)leaf_node_idx,
+                           (const uint64_t*)size_scan, (const uint64_t*)vals_scan);
+    if (nvox)
+        hipLaunchKernelGGL(k_write_values, dim3(blocks_for(nvox)), dim3(256), 0, st, grid, gi, nvox,
+                           (const uint32_t*)vox_val, (const uint64_t*)vox_key, (const uint32_t*)leaf_first,
+                           (const uint32_t*)nullptr, (const uint32_t*)leaf_pos, (const uint32_t*)leaf_node_idx,
+                           (const uint64_t*)vals_scan, (const uint32_t*)vox_leaf);
+    VDBR_CHECK(ctx, hipGetLastError());
+    VDBR_CHECK(ctx, hipEventRecord(ev[5], st));
+    VDBR_CHECK(ctx, hipEventSynchronize(ev[5]));
+
+    *grid_dev = grid;
+    *grid_size_out = gi.grid_size;
+    if (stats)
+    {
+        memset(stats, 0, sizeof(*stats));
+        stats->gaussian_count = n;
+        stats->fragment_count = frag_total;
+        stats->pair_count = pair_count;
+        stats->voxel_count = nvox;
+        stats->grid_size = gi.grid_size;
+        stats->upper_count = (uint32_t)nupper;
+        stats->lower_count = (uint32_t)nlower;
+        stats->leaf_count = (uint32_t)nleaf;
+        stats->window_count = (uint32_t)wtot;
+        stats->key_bits = key_bits;
+        stats->emit_attempts = attempts;
+        float ms;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        stats->ms_setup = ms;
+        (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
+        stats->ms_emit = ms;
+        (void)hipEventElapsedTime(&ms, ev[2], ev[3]);
+        stats->ms_sort = ms;
+        (void)hipEventElapsedTime(&ms, ev[3], ev[4]);
+        stats->ms_reduce = ms;
+        (void)hipEventElapsedTime(&ms, ev[4], ev[5]);
+        stats->ms_build = ms;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[5]);
+        stats->ms_total = ms;
+    }
+    return 0;
+}
+
+int preprocess_device(Context* ctx, uint64_t n, float* opacities, float* quats, float* scales, const float* sh0,
+                      float* colors)
+{
+    if (n == 0)
+        return 0;
+    hipLaunchKernelGGL(k_preprocess, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, n, opacities, quats, scales, sh0,
+                       colors);
+    VDBR_CHECK(ctx, hipGetLastError());
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, uint64_t* count)
+{
+    if (count)
+        *count = ctx->last_pair_count;
+    if (!ijk_dev && !gid_dev)
+        return 0;
+    uint64_t n = ctx->last_pair_count < cap ? ctx->last_pair_count : cap;
+    if (n == 0)
+        return 0;
+    KeyLayout kl;
+    memset(&kl, 0, sizeof(kl));
+    for (int a = 0; a < 3; a++)
+    {
+        kl.lmin[a] = ctx->last_lmin[a];
+        kl.bits[a] = ctx->last_bits[a];
+    }
+    kl.gid_bits = ctx->last_gid_bits;
+    hipLaunchKernelGGL(k_pairs_decode, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, ctx->last_pairs, n, kl, ijk_dev,
+                       gid_dev);
+    VDBR_CHECK(ctx, hipGetLastError());
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+} // namespace vdbr

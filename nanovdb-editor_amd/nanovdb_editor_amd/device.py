@@ -1,0 +1,146 @@
+"""Device-resident entry points on torch tensors (HIP path of libvdbraster).
+
+``Voxelizer.raster_node2`` runs the whole raster3d pipeline on Gaussian arrays that already
+live in GPU memory and returns the node2 NanoVDB grid as a uint8 tensor on the same device.
+"""
+import ctypes
+
+import torch
+
+from ._lib import VdbrGaussians, VdbrRasterParams, VdbrRasterStats, lib
+
+VALUE_RGBA8 = 0
+VALUE_DENSITY = 1
+FLAG_RAW_INPUTS = 1
+FLAG_SPATIAL_ORDER = 2
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p()
+
+
+def _check_f32(name, t, numel):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch.Tensor")
+    if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous float32 GPU tensor")
+    if t.numel() != numel:
+        raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
+
+
+class Voxelizer:
+    """One library context (HIP device, stream, reusable workspace)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = lib()
+        self._ctx = self._lib.vdbr_context_create(int(device))
+        if not self._ctx:
+            raise RuntimeError(f"vdbr_context_create({device}) failed (no HIP device?)")
+        self.device = torch.device("cuda", int(device))
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.vdbr_context_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _err(self, what):
+        msg = self._lib.vdbr_last_error(self._ctx)
+        raise RuntimeError(f"{what}: {msg.decode() if msg else 'unknown error'}")
+
+    def _bind_stream(self):
+        s = torch.cuda.current_stream(self.device)
+        self._lib.vdbr_context_set_stream(self._ctx, ctypes.c_void_p(s.cuda_stream))
+
+    def raster_node2(self, means, quats, scales, colors, opacities, voxel_size, *, value_mode=VALUE_RGBA8,
+                     cull_scale=0.0, raw=False, sh0=None, prim_batch_size=0, ijk_batch_size=0, max_node_count=0,
+                     spatial_order=False, copy_out=True):
+        """Gaussians (float32 GPU tensors, reference layouts) -> node2 grid.
+
+        With ``raw=True`` the arrays hold opacity logits, unnormalised (w,x,y,z) quaternions and
+        log-scales and are preprocessed in place (colors receives 0.28209479*sh0 + 0.5 from
+        ``sh0``). Returns ``(grid_uint8_tensor or None, stats_dict)``.
+        """
+        n = means.numel() // 3
+        _check_f32("means", means, 3 * n)
+        _check_f32("quats", quats, 4 * n)
+        _check_f32("scales", scales, 3 * n)
+        _check_f32("colors", colors, 3 * n)
+        _check_f32("opacities", opacities, n)
+        if raw:
+            if sh0 is None:
+                raise ValueError("raw inputs need sh0")
+            _check_f32("sh0", sh0, 3 * n)
+        g = VdbrGaussians(_ptr(means), _ptr(opacities), _ptr(quats), _ptr(scales), _ptr(colors),
+                          _ptr(sh0), n)
+        p = VdbrRasterParams(float(voxel_size), float(cull_scale), int(value_mode),
+                             (FLAG_RAW_INPUTS if raw else 0) | (FLAG_SPATIAL_ORDER if spatial_order else 0),
+                             int(prim_batch_size), int(ijk_batch_size), int(max_node_count), 0)
+        grid_dev = ctypes.c_void_p()
+        size = ctypes.c_uint64()
+        st = VdbrRasterStats()
+        self._bind_stream()
+        rc = self._lib.vdbr_raster_node2(self._ctx, ctypes.byref(g), ctypes.byref(p), ctypes.byref(grid_dev),
+                                         ctypes.byref(size), ctypes.byref(st))
+        if rc != 0:
+            self._err("vdbr_raster_node2")
+        out = None
+        if copy_out:
+            out = torch.empty(size.value, dtype=torch.uint8, device=self.device)
+            if size.value and self._lib.vdbr_copy(self._ctx, _ptr(out), grid_dev, size.value, 2) != 0:
+                self._err("vdbr_copy")
+        return out, st.as_dict()
+
+    def last_pairs(self):
+        """Sorted pair list of the last raster call: (ijk int32 [P,3], gid int32 [P]) on the GPU."""
+        cnt = ctypes.c_uint64()
+        self._lib.vdbr_last_pairs(self._ctx, None, None, 0, ctypes.byref(cnt))
+        p = cnt.value
+        ijk = torch.empty((p, 3), dtype=torch.int32, device=self.device)
+        gid = torch.empty(p, dtype=torch.int32, device=self.device)
+        if p and self._lib.vdbr_last_pairs(self._ctx, _ptr(ijk), _ptr(gid), p, ctypes.byref(cnt)) != 0:
+            self._err("vdbr_last_pairs")
+        return ijk, gid
+
+    def radix_sort_u64(self, keys, vals=None, bit_begin=0, bit_end=64):
+        """Stable in-place sort of int64 keys (as unsigned) over [bit_begin, bit_end)."""
+        if keys.dtype != torch.int64 or not keys.is_cuda or not keys.is_contiguous():
+            raise ValueError("keys must be a contiguous int64 GPU tensor")
+        if vals is not None and (vals.dtype != torch.int32 or vals.numel() != keys.numel()):
+            raise ValueError("vals must be an int32 tensor with one value per key")
+        self._bind_stream()
+        if self._lib.vdbr_radix_sort_u64(self._ctx, _ptr(keys), _ptr(vals), keys.numel(), bit_begin, bit_end) != 0:
+            self._err("vdbr_radix_sort_u64")
+
+    def radix_sort_u32(self, keys, vals=None, key_bit_count=32):
+        if keys.dtype != torch.int32 or not keys.is_cuda or not keys.is_contiguous():
+            raise ValueError("keys must be a contiguous int32 GPU tensor")
+        self._bind_stream()
+        if self._lib.vdbr_radix_sort_u32(self._ctx, _ptr(keys), _ptr(vals), keys.numel(), key_bit_count) != 0:
+            self._err("vdbr_radix_sort_u32")
+
+    def inclusive_scan(self, values, op="add"):
+        """Inclusive scan: op 'add' (int32 as u32 or int64 as u64) or 'max' (int32 as u32)."""
+        if not values.is_cuda or not values.is_contiguous():
+            raise ValueError("values must be a contiguous GPU tensor")
+        out = torch.empty_like(values)
+        if op == "add" and values.dtype == torch.int32:
+            code = 0
+        elif op == "add" and values.dtype == torch.int64:
+            code = 1
+        elif op == "max" and values.dtype == torch.int32:
+            code = 2
+        else:
+            raise ValueError("unsupported scan op / dtype")
+        self._bind_stream()
+        if self._lib.vdbr_inclusive_scan(self._ctx, _ptr(values), _ptr(out), values.numel(), code) != 0:
+            self._err("vdbr_inclusive_scan")
+        return out
+
+    def workspace_bytes(self):
+        return int(self._lib.vdbr_workspace_bytes(self._ctx))
