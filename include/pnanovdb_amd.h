@@ -315,6 +315,10 @@ typedef struct vdbr_raster_stats_t
     uint32_t key_bits;         /* bits of the sorted pair keys */
     uint32_t emit_attempts;    /* 1, or 2 when the pair buffer had to grow */
     float ms_setup, ms_emit, ms_sort, ms_reduce, ms_build, ms_total;
+    uint32_t sort_passes;      /* onesweep passes of the pair sort */
+    float ms_sort_passes;      /* time of the onesweep pass kernels only (HIP events) */
+    float ms_emit_kernel;      /* time of the last emission kernel launch (HIP events) */
+    float ms_reduce_kernel;    /* time of the per-voxel reduction kernel (HIP events) */
 } vdbr_raster_stats_t;
 
 /* Full raster3d pipeline on device inputs. The node2 grid is written to a context-owned

@@ -121,7 +121,7 @@ int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bo
 int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
                    SortTiming* timing);
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
-                          uint64_t** result_keys);
+                          uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out);
 
 // vdb_raster.hip
 int raster_node2(Context* ctx, const vdbr_gaussians_t* g, const vdbr_raster_params_t* p, void** grid_dev,

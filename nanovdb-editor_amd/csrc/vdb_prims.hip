@@ -521,9 +521,11 @@ int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, uin
 // sort returning the buffer that holds the result (no copy back): *result_keys is either
 // keys or the workspace alternate
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
-                          uint64_t** result_keys)
+                          uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out)
 {
     *result_keys = keys;
+    if (passes_out)
+        *passes_out = 0;
     if (n <= 1 || bit_end <= bit_begin)
         return 0;
     uint32_t bits = bit_end - bit_begin;
@@ -545,6 +547,8 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
                        passes, ws.starts);
     uint64_t* kin = keys;
     uint64_t* kout = ws.keys_alt;
+    if (passes_begin)
+        VDBR_CHECK(ctx, hipEventRecord(passes_begin, ctx->stream));
     for (uint32_t p = 0; p < passes; p++)
     {
         uint32_t shift = bit_begin + 8u * p;
@@ -557,6 +561,10 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
         kin = kout;
         kout = tk;
     }
+    if (passes_end)
+        VDBR_CHECK(ctx, hipEventRecord(passes_end, ctx->stream));
+    if (passes_out)
+        *passes_out = passes;
     *result_keys = kin;
     VDBR_CHECK(ctx, hipGetLastError());
     return 0;

@@ -1143,18 +1143,20 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     const float vs_inv = 1.f / vs;
     hipStream_t st = ctx->stream;
 
-    hipEvent_t ev[7];
-    for (int e = 0; e < 7; e++)
+    const int NEV = 12;
+    hipEvent_t ev[NEV];
+    for (int e = 0; e < NEV; e++)
         VDBR_CHECK(ctx, hipEventCreate(&ev[e]));
     struct EvGuard
     {
         hipEvent_t* e;
         ~EvGuard()
         {
-            for (int i = 0; i < 7; i++)
+            for (int i = 0; i < 12; i++)
                 (void)hipEventDestroy(e[i]);
         }
     } evg{ ev };
+    uint32_t sort_passes = 0;
     VDBR_CHECK(ctx, hipEventRecord(ev[0], st));
 
     // ---------------------------------------------------------------- preprocess + setup
@@ -1297,8 +1299,10 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             uint64_t blocks = (total_rows + EMIT_THREADS - 1) / EMIT_THREADS;
             if (blocks > 0x7FFFFFFFull)
                 return ctx->fail("raster_node2: too many AABB rows for one launch");
+            VDBR_CHECK(ctx, hipEventRecord(ev[8], st));
             hipLaunchKernelGGL(k_emit, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, ea);
             VDBR_CHECK(ctx, hipGetLastError());
+            VDBR_CHECK(ctx, hipEventRecord(ev[9], st));
             VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, counter, 8, hipMemcpyDeviceToHost, st));
             VDBR_CHECK(ctx, hipStreamSynchronize(st));
             pair_count = ctx->pinned[0];
@@ -1315,7 +1319,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     uint64_t* sorted = pairs;
     if (pair_count > 1)
     {
-        if (radix_sort_u64_noswap(ctx, pairs, pair_count, 0, key_bits, &sorted))
+        if (radix_sort_u64_noswap(ctx, pairs, pair_count, 0, key_bits, &sorted, ev[6], ev[7], &sort_passes))
             return -1;
     }
     VDBR_CHECK(ctx, hipEventRecord(ev[3], st));
@@ -1360,8 +1364,10 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         ra.vox_val = vox_val;
         ra.vox_t = vox_t;
         ra.bounds = vbounds;
+        VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
         hipLaunchKernelGGL(k_reduce, dim3(blocks_for(nvox)), dim3(256), 0, st, ra);
         VDBR_CHECK(ctx, hipGetLastError());
+        VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
     }
     VDBR_CHECK(ctx, hipEventRecord(ev[4], st));
 
@@ -1596,6 +1602,14 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         stats->ms_build = ms;
         (void)hipEventElapsedTime(&ms, ev[0], ev[5]);
         stats->ms_total = ms;
+        stats->sort_passes = sort_passes;
+        if (sort_passes && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess)
+            stats->ms_sort_passes = ms;
+        if (pair_count && hipEventElapsedTime(&ms, ev[8], ev[9]) == hipSuccess)
+            stats->ms_emit_kernel = ms;
+        if (nvox && hipEventElapsedTime(&ms, ev[10], ev[11]) == hipSuccess)
+            stats->ms_reduce_kernel = ms;
+        (void)hipGetLastError();
     }
     return 0;
 }

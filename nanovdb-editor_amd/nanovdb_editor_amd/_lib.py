@@ -28,7 +28,8 @@ class VdbrRasterStats(ctypes.Structure):
     _fields_ = [("gaussian_count", u64), ("fragment_count", u64), ("pair_count", u64), ("voxel_count", u64),
                 ("grid_size", u64), ("upper_count", u32), ("lower_count", u32), ("leaf_count", u32),
                 ("window_count", u32), ("key_bits", u32), ("emit_attempts", u32), ("ms_setup", f32),
-                ("ms_emit", f32), ("ms_sort", f32), ("ms_reduce", f32), ("ms_build", f32), ("ms_total", f32)]
+                ("ms_emit", f32), ("ms_sort", f32), ("ms_reduce", f32), ("ms_build", f32), ("ms_total", f32),
+                ("sort_passes", u32), ("ms_sort_passes", f32), ("ms_emit_kernel", f32), ("ms_reduce_kernel", f32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
