@@ -328,9 +328,11 @@ PNANOVDB_API int vdbr_raster_node2(vdbr_context_t* ctx, const vdbr_gaussians_t* 
                                    const vdbr_raster_params_t* params, void** grid_dev, uint64_t* grid_size,
                                    vdbr_raster_stats_t* stats);
 
-/* The surviving pair list of the last vdbr_raster_node2 call, sorted by voxel then gaussian
- * id: voxel ijk (int32 x3) and gaussian id per pair, copied to device buffers of the caller.
- * Returns the pair count through *count (buffers may be NULL to query it). */
+/* The surviving pair list of the last vdbr_raster_node2 call, grouped by voxel (voxels in
+ * leaf-major key order; within a voxel in ascending gaussian id for VDBR_VALUE_DENSITY, in
+ * emission order for VDBR_VALUE_RGBA8): voxel ijk (int32 x3) and gaussian id per pair, copied
+ * to device buffers of the caller. Returns the pair count through *count (buffers may be NULL
+ * to query it). */
 PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap,
                                  uint64_t* count);
 

@@ -49,12 +49,16 @@ struct __align__(16) PrimVal
 // Compact voxel key: leaf coordinates relative to the global leaf minimum, packed
 // lexicographically (x, y, z) with per-axis bit widths, followed by the 9-bit in-leaf index
 // n = (i&7)<<6 | (j&7)<<3 | (k&7). Voxels of one leaf are contiguous and in leaf order.
+// Pair key = voxel << (gid_bits + alpha_bits) | gid << alpha_bits | alpha8, alpha_bits 8 when
+// the RGBA8 alpha of the pair travels in the key (0 otherwise).
 struct KeyLayout
 {
     int32_t lmin[3];
     uint32_t bits[3];
     uint32_t gid_bits;
     uint32_t voxel_bits; // bits[0] + bits[1] + bits[2] + 9
+    uint32_t alpha_bits; // 8 or 0
+    uint32_t pad;
 };
 
 // Correctly rounded FP32 division and square root, independent of compiler flags: the exact
@@ -167,12 +171,13 @@ VDB_DEV uint32_t u8_of(float v)
     return (uint32_t)(255.f * fmaxf(0.f, fminf(1.f, v)));
 }
 
-// gaussian_frag_color.slang:184-212 -- packed RGBA8 of one surviving pair. The 64-sample
+// gaussian_frag_color.slang:184-212 -- 8-bit alpha of one surviving pair. The 64-sample
 // maximum of o*exp(-(r*|p|^2)) equals o*exp(-(r*min|p|^2)) because every step is monotone in
 // |p|^2 (correctly rounded exp, multiplication by positive r and o); the minimum of the
 // separable sum over the 4x4x4 lattice is the sum of per-axis minima for the same reason
 // (rounded addition is monotone in each operand).
-VDB_DEV uint32_t pair_rgba(const PrimVal& pv, const float vmin[3], const float vmax[3])
+// minimum over the 4x4x4 sample lattice of |lerp(vmin, vmax, w)|^2 (gaussian_frag_color.slang:192-199)
+VDB_DEV float min_sample_dist2(const float vmin[3], const float vmax[3])
 {
     const float third = 1.f / 3.f;
     float msq[3];
@@ -193,12 +198,29 @@ VDB_DEV uint32_t pair_rgba(const PrimVal& pv, const float vmin[3], const float v
     }
     float d = msq[0] + msq[1];
     d = d + msq[2];
+    return d;
+}
+
+// alpha from the minimum sample distance (gaussian_frag_color.slang:197-212)
+VDB_DEV uint32_t alpha8_from_dist2(const PrimVal& pv, float d)
+{
+    if (pv.rgb_cull >> 24)
+        return 0u; // culled: alpha forced to 0 (the colour channels are per Gaussian)
     float e = pv.opacity * vdb_expf(-(pv.local_scale * d));
     float a = fmaxf(0.f, e);
     a = 1.f - vdb_powf(1.f - a, pv.pow_exp);
-    if (pv.rgb_cull >> 24)
-        a = 0.f;
-    return (pv.rgb_cull & 0x00FFFFFFu) | (u8_of(a) << 24);
+    return u8_of(a);
+}
+
+VDB_DEV uint32_t pair_alpha8(const PrimVal& pv, const float vmin[3], const float vmax[3])
+{
+    return alpha8_from_dist2(pv, min_sample_dist2(vmin, vmax));
+}
+
+// packed RGBA8 of one surviving pair (r8 | g8 << 8 | b8 << 16 | a8 << 24)
+VDB_DEV uint32_t pair_rgba(const PrimVal& pv, const float vmin[3], const float vmax[3])
+{
+    return (pv.rgb_cull & 0x00FFFFFFu) | (pair_alpha8(pv, vmin, vmax) << 24);
 }
 
 // density contribution o * exp(-m/2) at the voxel centre. The Mahalanobis term is formed in

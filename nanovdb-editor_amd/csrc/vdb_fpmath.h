@@ -259,16 +259,8 @@ VDB_HD float vdb_logf(float x)
 
 // pow for x >= 0 and finite y (the raster3d value pass only raises values in [0,1] to a
 // positive finite power)
-VDB_HD float vdb_powf(float x, float y)
+VDB_HD float vdb_powf_slow(float x, float y)
 {
-    if (x != x || y != y)
-        return vdb_bits_to_float(0x7FC00000u);
-    if (x == 1.0f || y == 0.0f)
-        return 1.0f;
-    if (x == 0.0f)
-        return y > 0.0f ? 0.0f : vdb_bits_to_float(0x7F800000u);
-    if (x < 0.0f)
-        return vdb_bits_to_float(0x7FC00000u);
     double lh, ll;
     vdb_log_dd((double)x, &lh, &ll);
     double yd = (double)y;
@@ -281,6 +273,34 @@ VDB_HD float vdb_powf(float x, float y)
     double eh, el;
     vdb_exp_dd(zh, zl, &eh, &el);
     return vdb_dd_to_float(eh, el);
+}
+
+// pow for x >= 0 and finite y (the raster3d value pass only raises values in [0,1] to a
+// positive finite power). Fast path: p = exp(y * log(x)) in double, relative error below
+// (|y log x| + 4) * 5e-16; its float rounding is returned when p is farther than that bound
+// from both rounding midpoints around it, otherwise the double-double evaluation decides.
+VDB_HD float vdb_powf(float x, float y)
+{
+    if (x != x || y != y)
+        return vdb_bits_to_float(0x7FC00000u);
+    if (x == 1.0f || y == 0.0f)
+        return 1.0f;
+    if (x == 0.0f)
+        return y > 0.0f ? 0.0f : vdb_bits_to_float(0x7F800000u);
+    if (x < 0.0f)
+        return vdb_bits_to_float(0x7FC00000u);
+    double z = (double)y * vdb_log_d((double)x);
+    if (z > 88.0 || z < -86.0)
+        return vdb_powf_slow(x, y); // overflow / subnormal territory
+    double p = vdb_exp_d(z);
+    float f = (float)p;
+    uint32_t fb = vdb_float_to_bits(f);
+    double lo = 0.5 * ((double)f + (double)vdb_bits_to_float(fb - 1u));
+    double hi = 0.5 * ((double)f + (double)vdb_bits_to_float(fb + 1u));
+    double eps = (fabs(z) + 4.0) * 5e-16 * p;
+    if (p - lo > eps && hi - p > eps)
+        return f;
+    return vdb_powf_slow(x, y);
 }
 
 #endif

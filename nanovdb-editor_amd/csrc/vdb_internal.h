@@ -87,6 +87,7 @@ struct Context
     uint64_t last_pair_count = 0;
     uint64_t* last_pairs = nullptr;
     uint32_t last_gid_bits = 0;
+    uint32_t last_alpha_bits = 0;
     int32_t last_lmin[3] = { 0, 0, 0 };
     uint32_t last_bits[3] = { 0, 0, 0 };
 

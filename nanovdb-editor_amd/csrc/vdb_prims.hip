@@ -239,7 +239,10 @@ __device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint64_t flag, uint6
 }
 
 // digit of pass p covers key bits [bit_begin + 8p, min(bit_end, bit_begin + 8p + 8)); the same
-// mask is applied here and in k_sort_pass so the per-pass counts always agree
+// mask is applied here and in k_sort_pass so the per-pass counts always agree.
+// Each thread counts 16 consecutive keys and adds runs of equal digits at once: the leading
+// digits of neighbouring pair keys are mostly equal, so per-key LDS atomics would serialise.
+constexpr int HIST_RUN = 16;
 __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, uint64_t n, uint32_t bit_begin,
                                                    uint32_t bit_count, unsigned long long* __restrict__ hist)
 {
@@ -248,15 +251,58 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
     for (int p = 0; p < SORT_MAX_PASSES; p++)
         h[p][threadIdx.x] = 0u;
     __syncthreads();
-    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    uint32_t dmask[SORT_MAX_PASSES];
+#pragma unroll
+    for (int p = 0; p < SORT_MAX_PASSES; p++)
     {
-        uint64_t k = keys[i] >> bit_begin;
-        for (uint32_t p = 0; p < passes; p++)
+        uint32_t rem = bit_count > 8u * p ? bit_count - 8u * p : 0u;
+        dmask[p] = rem >= 8u ? 255u : ((1u << rem) - 1u);
+    }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * HIST_RUN;
+    for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * HIST_RUN; c < n; c += stride)
+    {
+        uint64_t k[HIST_RUN];
+        uint32_t cnt = (n - c) < (uint64_t)HIST_RUN ? (uint32_t)(n - c) : (uint32_t)HIST_RUN;
+        if (cnt == HIST_RUN)
         {
-            uint32_t dbits = (bit_count - 8u * p) < 8u ? (bit_count - 8u * p) : 8u;
-            atomicAdd(&h[p][(uint32_t)(k & ((1u << dbits) - 1u))], 1u);
-            k >>= 8;
+            const ulonglong2* src = (const ulonglong2*)(keys + c);
+#pragma unroll
+            for (int r = 0; r < HIST_RUN / 2; r++)
+            {
+                ulonglong2 v = src[r];
+                k[2 * r] = v.x;
+                k[2 * r + 1] = v.y;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int r = 0; r < HIST_RUN; r++)
+                k[r] = (uint32_t)r < cnt ? keys[c + r] : 0ull;
+        }
+#pragma unroll
+        for (int p = 0; p < SORT_MAX_PASSES; p++)
+        {
+            if ((uint32_t)p >= passes)
+                break;
+            const uint32_t shift = bit_begin + 8u * p;
+            uint32_t cur = (uint32_t)(k[0] >> shift) & dmask[p];
+            uint32_t run = 1u;
+#pragma unroll
+            for (int r = 1; r < HIST_RUN; r++)
+            {
+                if ((uint32_t)r >= cnt)
+                    break;
+                uint32_t d = (uint32_t)(k[r] >> shift) & dmask[p];
+                if (d != cur)
+                {
+                    atomicAdd(&h[p][cur], run);
+                    cur = d;
+                    run = 0u;
+                }
+                run++;
+            }
+            atomicAdd(&h[p][cur], run);
         }
     }
     __syncthreads();
@@ -446,9 +492,9 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
         return -1;
     VDBR_CHECK(ctx, hipMemsetAsync(ws.hist, 0, SORT_MAX_PASSES * 256 * sizeof(uint64_t), ctx->stream));
     VDBR_CHECK(ctx, hipMemsetAsync(ws.tile_counters, 0, SORT_MAX_PASSES * sizeof(uint32_t), ctx->stream));
-    uint32_t hist_blocks = (uint32_t)((n + 255) / 256);
-    if (hist_blocks > 4096u)
-        hist_blocks = 4096u;
+    uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
+    if (hist_blocks > 2048u)
+        hist_blocks = 2048u;
     hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits,
                        (unsigned long long*)ws.hist);
     hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
@@ -538,9 +584,9 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
         return -1;
     VDBR_CHECK(ctx, hipMemsetAsync(ws.hist, 0, SORT_MAX_PASSES * 256 * sizeof(uint64_t), ctx->stream));
     VDBR_CHECK(ctx, hipMemsetAsync(ws.tile_counters, 0, SORT_MAX_PASSES * sizeof(uint32_t), ctx->stream));
-    uint32_t hist_blocks = (uint32_t)((n + 255) / 256);
-    if (hist_blocks > 4096u)
-        hist_blocks = 4096u;
+    uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
+    if (hist_blocks > 2048u)
+        hist_blocks = 2048u;
     hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits,
                        (unsigned long long*)ws.hist);
     hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,

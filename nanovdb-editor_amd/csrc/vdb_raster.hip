@@ -27,6 +27,66 @@
 namespace vdbr
 {
 
+// ============================================================================ block reductions
+// min/max of 6 ints (xyz min, xyz max) over a 256-thread block, committed with one set of
+// atomics per block: per-wave atomics on one cache line serialise at the L2.
+__device__ __forceinline__ void commit_bounds(int32_t mn[3], int32_t mx[3], int32_t* bounds)
+{
+    __shared__ int32_t sb[4][6];
+    for (int off = 32; off > 0; off >>= 1)
+        for (int c = 0; c < 3; c++)
+        {
+            mn[c] = min(mn[c], __shfl_xor(mn[c], off, 64));
+            mx[c] = max(mx[c], __shfl_xor(mx[c], off, 64));
+        }
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0u)
+        for (int c = 0; c < 3; c++)
+        {
+            sb[wave][c] = mn[c];
+            sb[wave][3 + c] = mx[c];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (uint32_t w = 1; w < (blockDim.x >> 6); w++)
+            for (int c = 0; c < 3; c++)
+            {
+                sb[0][c] = min(sb[0][c], sb[w][c]);
+                sb[0][3 + c] = max(sb[0][3 + c], sb[w][3 + c]);
+            }
+        if (sb[0][0] <= sb[0][3])
+        {
+            for (int c = 0; c < 3; c++)
+            {
+                atomicMin(&bounds[c], sb[0][c]);
+                atomicMax(&bounds[3 + c], sb[0][3 + c]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void commit_sum(unsigned long long v, unsigned long long* target)
+{
+    __shared__ unsigned long long ss[4];
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (lane == 0u)
+        ss[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned long long t = 0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); w++)
+            t += ss[w];
+        if (t)
+            atomicAdd(target, t);
+    }
+    __syncthreads();
+}
+
 // ============================================================================ preprocess
 // RasterUtils.cpp:59-117 (quaternion normalisation, exp scales, SH0 colour, sigmoid)
 __global__ __launch_bounds__(256) void k_preprocess(uint64_t n, float* __restrict__ opac, float* __restrict__ quats,
@@ -70,14 +130,12 @@ struct SetupOut
     unsigned long long* est;     // pair-count bound accumulator
 };
 
-__global__ __launch_bounds__(256) void k_setup(uint64_t n, const float* __restrict__ means, const float* __restrict__ quats,
-                                               const float* __restrict__ scales, const float* __restrict__ opac,
-                                               const float* __restrict__ colors, float vs, float vs_inv,
-                                               float cull_scale, SetupOut o)
+__device__ __forceinline__ void setup_one(uint64_t g, const float* __restrict__ means, const float* __restrict__ quats,
+                                          const float* __restrict__ scales, const float* __restrict__ opac,
+                                          const float* __restrict__ colors, float vs, float vs_inv, float cull_scale,
+                                          const SetupOut& o, int32_t bmn[3], int32_t bmx[3],
+                                          unsigned long long* est_sum)
 {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n)
-        return;
     float pos[3] = { means[3 * g + 0], means[3 * g + 1], means[3 * g + 2] };
     float x = quats[4 * g + 1], y = quats[4 * g + 2], z = quats[4 * g + 3], w = quats[4 * g + 0];
     float opacity = opac[g];
@@ -177,12 +235,11 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t n, const float* __restri
 
     if (valid)
     {
-        atomicMin(&o.bounds[0], mn[0]);
-        atomicMin(&o.bounds[1], mn[1]);
-        atomicMin(&o.bounds[2], mn[2]);
-        atomicMax(&o.bounds[3], mx[0]);
-        atomicMax(&o.bounds[4], mx[1]);
-        atomicMax(&o.bounds[5], mx[2]);
+        for (int c = 0; c < 3; c++)
+        {
+            bmn[c] = mn[c];
+            bmx[c] = mx[c];
+        }
         // bound on survivors: voxel centres of kept voxels satisfy |A(c - mu)| <= 1 + |h|, h the
         // local half extent of a voxel; volume (4/3) pi (1+|h|)^3 r^3 s_x s_y s_z / vs^3
         float h2 = 0.f;
@@ -199,18 +256,34 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t n, const float* __restri
         double foot = (double)o.foot[g];
         if (est > foot)
             est = foot;
-        atomicAdd(o.est, (unsigned long long)est);
+        *est_sum = (unsigned long long)est;
     }
+}
+
+__global__ __launch_bounds__(256) void k_setup(uint64_t n, const float* __restrict__ means, const float* __restrict__ quats,
+                                               const float* __restrict__ scales, const float* __restrict__ opac,
+                                               const float* __restrict__ colors, float vs, float vs_inv,
+                                               float cull_scale, SetupOut o)
+{
+    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t bmn[3] = { 2147483647, 2147483647, 2147483647 }, bmx[3] = { -2147483647, -2147483647, -2147483647 };
+    unsigned long long est_sum = 0ull;
+    if (g < n)
+        setup_one(g, means, quats, scales, opac, colors, vs, vs_inv, cull_scale, o, bmn, bmx, &est_sum);
+    commit_bounds(bmn, bmx, o.bounds);
+    commit_sum(est_sum, o.est);
 }
 
 // ============================================================================ emit
 constexpr int EMIT_THREADS = 256;
-constexpr int EMIT_STAGE = 4096;
+constexpr int EMIT_ROWS = 1024;       // rows per block
+constexpr int EMIT_WAVE_STAGE = 1024; // staged survivors per wave
 
 struct EmitArgs
 {
     const PrimFrame* frames;
     const PrimBox* boxes;
+    const PrimVal* vals; // alpha of each pair computed here when kl.alpha_bits == 8
     const uint64_t* row_scan; // exclusive scan of rows, n + 1 entries
     uint64_t n;
     uint64_t total_rows;
@@ -235,164 +308,275 @@ __device__ __forceinline__ uint64_t find_prim(const uint64_t* __restrict__ row_s
     return lo;
 }
 
-__device__ __forceinline__ void emit_flush(const EmitArgs& a, uint64_t* stage, uint32_t* stage_count,
-                                           unsigned long long* flush_base)
+// Writes a wave's staged pairs at an atomically reserved position. With the alpha in the key,
+// the stage holds (key without alpha, minimum sample distance) and the alpha is evaluated here
+// on the compacted pairs, so the double-precision exp/pow run with every lane busy.
+__device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, uint64_t* stage, float* stage_d, uint32_t cnt,
+                                                uint32_t lane)
 {
-    uint32_t cnt = *stage_count;
-    if (threadIdx.x == 0)
-        *flush_base = atomicAdd(a.counter, (unsigned long long)cnt);
-    __syncthreads();
-    uint64_t base = *flush_base;
-    for (uint32_t i = threadIdx.x; i < cnt; i += EMIT_THREADS)
+    if (cnt == 0u)
+        return;
+    unsigned long long base = 0ull;
+    if (lane == 0u)
+        base = atomicAdd(a.counter, (unsigned long long)cnt);
+    base = __shfl(base, 0, 64);
+    const uint64_t gmask = (1ull << a.kl.gid_bits) - 1ull;
+    for (uint32_t e = lane; e < cnt; e += 64u)
     {
-        uint64_t pos = base + i;
+        uint64_t pos = base + e;
+        uint64_t key = stage[e];
+        if (a.kl.alpha_bits)
+        {
+            uint32_t gid = (uint32_t)((key >> a.kl.alpha_bits) & gmask);
+            key |= (uint64_t)alpha8_from_dist2(a.vals[gid], stage_d[e]);
+        }
         if (pos < a.cap)
-            a.out[pos] = stage[i];
+            a.out[pos] = key;
     }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        *stage_count = 0u;
-    __syncthreads();
 }
 
+// Conservative candidate interval of one AABB row (i, j): in exact arithmetic the local-frame
+// corners of voxel k form a parallelepiped centred at C(k) = C0 + k B with AABB half extent H,
+// so the voxel's box is farther than |C(k)| - |H| from the origin. The float corner transforms
+// differ from the exact ones by at most delta (operation-count bound, doubled), so every voxel
+// with |C(k)| > 1 + |H| + delta is culled by the exact FP32 predicate. Solving
+// |C0 + k B|^2 <= R^2 in double gives [ka, kb]; the interval is widened by one voxel each side
+// and clipped to the row.
+__device__ __forceinline__ void row_interval(const PrimFrame& f, const PrimBox& b, int32_t i, int32_t j, float vs,
+                                             int32_t* k0, int32_t* len)
+{
+    const double vsd = (double)vs;
+    double A[3][3];
+    double H2 = 0.0, hmax = 0.0;
+    double w = (double)f.inv_r;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        double sc = w * (double)f.inv_s[c];
+        double hs = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < 3; bb++)
+        {
+            A[c][bb] = (double)f.rinv[3 * c + bb] * sc;
+            hs += fabs(A[c][bb]);
+        }
+        double h = 0.5 * vsd * hs;
+        H2 += h * h;
+        hmax = fmax(hmax, sc);
+    }
+    // magnitude of the world-space operands of the corner transforms
+    double M = fmax(fmax(fabs((double)f.mu[0]), fabs((double)f.mu[1])), fabs((double)f.mu[2])) +
+               vsd * (double)max(max(b.rng_x, b.rng_y), b.rng_z) + vsd;
+    const double u = 5.9604644775390625e-08; // 2^-24
+    double H = sqrt(H2);
+    double delta = 2.0 * (hmax * 10.4 * u * M + 2.0 * u * (1.0 + 2.0 * H)) * 1.7320508075688772 + 1e-6;
+    double R = 1.0 + H + delta;
+    double dx = ((double)i + 0.5) * vsd - (double)f.mu[0];
+    double dy = ((double)j + 0.5) * vsd - (double)f.mu[1];
+    double dz = 0.5 * vsd - (double)f.mu[2];
+    double qa = 0.0, qb = 0.0, qc = -R * R;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        double c0 = A[c][0] * dx + A[c][1] * dy + A[c][2] * dz;
+        double bc = vsd * A[c][2];
+        qa += bc * bc;
+        qb += 2.0 * c0 * bc;
+        qc += c0 * c0;
+    }
+    int32_t lo = *k0, hi = *k0 + *len - 1;
+    if (qa > 0.0)
+    {
+        double disc = qb * qb - 4.0 * qa * qc;
+        if (disc < 0.0)
+        {
+            *len = 0;
+            return;
+        }
+        double sq = sqrt(disc);
+        double r1 = (-qb - sq) / (2.0 * qa);
+        double r2 = (-qb + sq) / (2.0 * qa);
+        if (r1 > (double)hi + 1.0 || r2 < (double)lo - 1.0)
+        {
+            *len = 0;
+            return;
+        }
+        double fl = floor(r1) - 1.0, ce = ceil(r2) + 1.0;
+        if (fl > (double)lo)
+            lo = (int32_t)fl;
+        if (ce < (double)hi)
+            hi = (int32_t)ce;
+    }
+    else if (qc > 0.0)
+    {
+        *len = 0;
+        return;
+    }
+    *k0 = lo;
+    *len = hi >= lo ? hi - lo + 1 : 0;
+}
+
+// One block covers EMIT_ROWS consecutive AABB rows; each wave takes 64-row chunks (one row per
+// lane) and walks the rows' candidate intervals in lock step. Survivors are compacted into a
+// wave-private LDS stage with ballots and flushed by the wave alone (no block barriers).
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit(EmitArgs a)
 {
-    __shared__ uint64_t stage[EMIT_STAGE];
-    __shared__ uint32_t stage_count;
-    __shared__ unsigned long long flush_base;
-    __shared__ int32_t max_len;
+    __shared__ uint64_t stage_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
+    __shared__ float stage_d_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
     __shared__ uint64_t g_range[2];
 
-    const uint64_t row0 = (uint64_t)blockIdx.x * EMIT_THREADS;
-    const uint64_t row = row0 + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t* stage = stage_all[wave];
+    float* stage_d = stage_d_all[wave];
+    const uint64_t row0 = (uint64_t)blockIdx.x * EMIT_ROWS;
     if (threadIdx.x == 0)
     {
-        stage_count = 0u;
-        max_len = 0;
         g_range[0] = find_prim(a.row_scan, 0, a.n - 1, row0);
-        uint64_t last = row0 + EMIT_THREADS - 1;
+        uint64_t last = row0 + EMIT_ROWS - 1;
         if (last >= a.total_rows)
             last = a.total_rows - 1;
         g_range[1] = find_prim(a.row_scan, 0, a.n - 1, last);
     }
     __syncthreads();
-    const bool active = row < a.total_rows;
-    uint64_t g = 0;
-    int32_t len = 0, i = 0, j = 0, k0 = 0;
-    PrimFrame f;
-    if (active)
-    {
-        g = find_prim(a.row_scan, g_range[0], g_range[1], row);
-        PrimBox b = a.boxes[g];
-        f = a.frames[g];
-        uint64_t lr = row - a.row_scan[g];
-        i = b.mn[0] + (int32_t)(lr / (uint64_t)b.rng_y);
-        j = b.mn[1] + (int32_t)(lr % (uint64_t)b.rng_y);
-        k0 = b.mn[2];
-        len = b.rng_z;
-        atomicMax(&max_len, len);
-    }
-    __syncthreads();
-    const int32_t steps = max_len;
+    const uint64_t glo = g_range[0], ghi = g_range[1];
     const float vs = a.vs;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    uint32_t count = 0u; // wave-uniform stage fill
 
-    // row-constant partial dot products of the 4 corner columns (dx, dy)
-    float part[3][4];
-    if (active)
+    for (uint32_t chunk = wave; chunk < EMIT_ROWS / 64u; chunk += EMIT_THREADS / 64u)
     {
-#pragma unroll
-        for (int col = 0; col < 4; col++)
+        const uint64_t row = row0 + (uint64_t)chunk * 64u + lane;
+        if (row0 + (uint64_t)chunk * 64u >= a.total_rows)
+            break;
+        const bool active = row < a.total_rows;
+        uint64_t g = 0;
+        int32_t len = 0, i = 0, j = 0, k0 = 0;
+        PrimFrame f;
+        if (active)
         {
-            float px = (float)i;
-            float py = (float)j;
-            if (col & 1)
-                px += 1.f;
-            if (col & 2)
-                py += 1.f;
-            float lx = px * vs;
-            float ly = py * vs;
-            lx = lx - f.mu[0];
-            ly = ly - f.mu[1];
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-            {
-                float t = lx * f.rinv[3 * c + 0];
-                t = t + ly * f.rinv[3 * c + 1];
-                part[c][col] = t;
-            }
+            g = find_prim(a.row_scan, glo, ghi, row);
+            PrimBox b = a.boxes[g];
+            f = a.frames[g];
+            uint64_t lr = row - a.row_scan[g];
+            i = b.mn[0] + (int32_t)(lr / (uint64_t)b.rng_y);
+            j = b.mn[1] + (int32_t)(lr % (uint64_t)b.rng_y);
+            k0 = b.mn[2];
+            len = b.rng_z;
+            row_interval(f, b, i, j, vs, &k0, &len);
         }
-    }
-    // lattice plane z -> per-axis min/max over the 4 columns
-    auto plane = [&](int32_t kz, float pmin[3], float pmax[3]) {
-        float pz = (float)kz;
-        float lz = pz * vs;
-        lz = lz - f.mu[2];
-#pragma unroll
-        for (int c = 0; c < 3; c++)
+        int32_t steps = len;
+        for (int off = 32; off > 0; off >>= 1)
+            steps = max(steps, __shfl_xor(steps, off, 64));
+
+        // row-constant partial dot products of the 4 corner columns (dx, dy)
+        float part[3][4];
+        if (len > 0)
         {
-            float tz = lz * f.rinv[3 * c + 2];
 #pragma unroll
             for (int col = 0; col < 4; col++)
             {
-                float q = part[c][col] + tz;
-                q = q * f.inv_r;
-                q = q * f.inv_s[c];
-                if (col == 0)
+                float px = (float)i;
+                float py = (float)j;
+                if (col & 1)
+                    px += 1.f;
+                if (col & 2)
+                    py += 1.f;
+                float lx = px * vs;
+                float ly = py * vs;
+                lx = lx - f.mu[0];
+                ly = ly - f.mu[1];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
                 {
-                    pmin[c] = q;
-                    pmax[c] = q;
-                }
-                else
-                {
-                    pmin[c] = fminf(pmin[c], q);
-                    pmax[c] = fmaxf(pmax[c], q);
+                    float t = lx * f.rinv[3 * c + 0];
+                    t = t + ly * f.rinv[3 * c + 1];
+                    part[c][col] = t;
                 }
             }
         }
-    };
-    float lo_min[3] = { 0.f, 0.f, 0.f }, lo_max[3] = { 0.f, 0.f, 0.f };
-    if (active)
-        plane(k0, lo_min, lo_max);
-
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
-    for (int32_t st = 0; st < steps; st++)
-    {
-        bool keep = false;
-        uint64_t key = 0ull;
-        if (st < len)
-        {
-            float hi_min[3], hi_max[3];
-            plane(k0 + st + 1, hi_min, hi_max);
-            float vmin[3], vmax[3];
+        // lattice plane z -> per-axis min/max over the 4 columns
+        auto plane = [&](int32_t kz, float pmin[3], float pmax[3]) {
+            float pz = (float)kz;
+            float lz = pz * vs;
+            lz = lz - f.mu[2];
 #pragma unroll
             for (int c = 0; c < 3; c++)
             {
-                vmin[c] = fminf(lo_min[c], hi_min[c]);
-                vmax[c] = fmaxf(lo_max[c], hi_max[c]);
-                lo_min[c] = hi_min[c];
-                lo_max[c] = hi_max[c];
+                float tz = lz * f.rinv[3 * c + 2];
+#pragma unroll
+                for (int col = 0; col < 4; col++)
+                {
+                    float q = part[c][col] + tz;
+                    q = q * f.inv_r;
+                    q = q * f.inv_s[c];
+                    if (col == 0)
+                    {
+                        pmin[c] = q;
+                        pmax[c] = q;
+                    }
+                    else
+                    {
+                        pmin[c] = fminf(pmin[c], q);
+                        pmax[c] = fmaxf(pmax[c], q);
+                    }
+                }
             }
-            keep = box_kept(vmin, vmax);
-            if (keep)
-                key = (encode_voxel(a.kl, i, j, k0 + st) << a.kl.gid_bits) | g;
-        }
-        uint64_t m = __ballot(keep);
-        if (m)
+        };
+        float lo_min[3] = { 0.f, 0.f, 0.f }, lo_max[3] = { 0.f, 0.f, 0.f };
+        if (len > 0)
+            plane(k0, lo_min, lo_max);
+        for (int32_t st = 0; st < steps; st++)
         {
-            uint32_t leader = (uint32_t)(__ffsll((long long)m) - 1);
-            uint32_t base = 0u;
-            if (lane == leader)
-                base = atomicAdd(&stage_count, (uint32_t)__popcll(m));
-            base = __shfl(base, (int)leader, 64);
+            bool keep = false;
+            uint64_t key = 0ull;
+            float dist = 0.f;
+            if (st < len)
+            {
+                float hi_min[3], hi_max[3];
+                plane(k0 + st + 1, hi_min, hi_max);
+                float vmin[3], vmax[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                {
+                    vmin[c] = fminf(lo_min[c], hi_min[c]);
+                    vmax[c] = fmaxf(lo_max[c], hi_max[c]);
+                    lo_min[c] = hi_min[c];
+                    lo_max[c] = hi_max[c];
+                }
+                keep = box_kept(vmin, vmax);
+                if (keep)
+                {
+                    key = (encode_voxel(a.kl, i, j, k0 + st) << (a.kl.gid_bits + a.kl.alpha_bits)) |
+                          (g << a.kl.alpha_bits);
+                    if (a.kl.alpha_bits)
+                        dist = min_sample_dist2(vmin, vmax);
+                }
+            }
+            const uint64_t m = __ballot(keep);
             if (keep)
-                stage[base + (uint32_t)__popcll(m & lt_mask)] = key;
+            {
+                uint32_t slot = count + (uint32_t)__popcll(m & lt_mask);
+                stage[slot] = key;
+                stage_d[slot] = dist;
+            }
+            count += (uint32_t)__popcll(m);
+            if (count > (uint32_t)(EMIT_WAVE_STAGE - 64))
+            {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                emit_flush_wave(a, stage, stage_d, count, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                count = 0u;
+            }
         }
-        __syncthreads();
-        if (stage_count > (uint32_t)(EMIT_STAGE - EMIT_THREADS))
-            emit_flush(a, stage, &stage_count, &flush_base);
     }
-    emit_flush(a, stage, &stage_count, &flush_base);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    emit_flush_wave(a, stage, stage_d, count, lane);
 }
 
 // ============================================================================ run heads
@@ -499,33 +683,50 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
     if (active)
     {
         uint32_t b = a.seg[v], e = a.seg[v + 1];
+        const uint32_t ab = a.kl.alpha_bits;
         const uint64_t gmask = (1ull << a.kl.gid_bits) - 1ull;
         uint64_t k0 = a.keys[b];
-        uint64_t vkey = k0 >> a.kl.gid_bits;
+        uint64_t vkey = k0 >> (a.kl.gid_bits + ab);
         int32_t i, j, k;
         decode_voxel(a.kl, vkey, &i, &j, &k);
         uint32_t value = 0u;
+        uint32_t g0 = 0xFFFFFFFFu;
         float dens = 0.f;
-        for (uint32_t p = b; p < e; p++)
+        if (ab)
         {
-            uint32_t gid = (uint32_t)(a.keys[p] & gmask);
-            const PrimFrame f = a.frames[gid];
-            if (a.value_mode == VDBR_VALUE_RGBA8)
+            // RGBA8 with the pair alpha in the key: max of (rgb | alpha << 24), any pair order
+            for (uint32_t p = b; p < e; p++)
             {
-                float vmin[3], vmax[3];
-                voxel_box(f, i, j, k, a.vs, vmin, vmax);
-                uint32_t rgba = pair_rgba(a.vals[gid], vmin, vmax);
+                uint64_t key = a.keys[p];
+                uint32_t gid = (uint32_t)((key >> ab) & gmask);
+                uint32_t rgba = (a.vals[gid].rgb_cull & 0x00FFFFFFu) | ((uint32_t)(key & 255u) << 24);
                 value = rgba > value ? rgba : value;
+                g0 = gid < g0 ? gid : g0;
             }
-            else
+        }
+        else
+        {
+            for (uint32_t p = b; p < e; p++)
             {
-                dens += pair_density(f, a.vals[gid].opacity, i, j, k, (double)a.vs);
+                uint32_t gid = (uint32_t)(a.keys[p] & gmask);
+                g0 = gid < g0 ? gid : g0;
+                const PrimFrame f = a.frames[gid];
+                if (a.value_mode == VDBR_VALUE_RGBA8)
+                {
+                    float vmin[3], vmax[3];
+                    voxel_box(f, i, j, k, a.vs, vmin, vmax);
+                    uint32_t rgba = pair_rgba(a.vals[gid], vmin, vmax);
+                    value = rgba > value ? rgba : value;
+                }
+                else
+                {
+                    dens += pair_density(f, a.vals[gid].opacity, i, j, k, (double)a.vs);
+                }
             }
         }
         if (a.value_mode != VDBR_VALUE_RGBA8)
             value = __float_as_uint(dens);
-        // window of the first pair (lowest gaussian id): fragment slot within its prim batch
-        uint32_t g0 = (uint32_t)(k0 & gmask);
+        // window of the pair with the lowest gaussian id: fragment slot within its prim batch
         PrimBox bx = a.boxes[g0];
         uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
                       (uint64_t)(k - bx.mn[2]);
@@ -539,22 +740,115 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
         mn[1] = mx[1] = j;
         mn[2] = mx[2] = k;
     }
-    // wave-reduced voxel bounds, one atomic per wave and axis
-    for (int off = 32; off > 0; off >>= 1)
-        for (int c = 0; c < 3; c++)
-        {
-            mn[c] = min(mn[c], __shfl_xor(mn[c], off, 64));
-            mx[c] = max(mx[c], __shfl_xor(mx[c], off, 64));
-        }
-    if ((threadIdx.x & 63u) == 0u && mn[0] <= mx[0])
+    commit_bounds(mn, mx, a.bounds);
+}
+
+// RGBA8 with the alpha in the key: coalesced segmented reduction. Each block covers one
+// 4096-pair head tile (the tile's first voxel index comes from the head-count scan); each wave
+// reduces 64 consecutive pairs with a segmented shuffle scan (max of rgba, min of gaussian id)
+// and the last lane of every segment piece commits with one atomicMax / atomicMin (both are
+// order independent, so the result is deterministic).
+__global__ __launch_bounds__(HEAD_THREADS) void k_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
+                                                              uint32_t vshift, uint32_t gid_bits,
+                                                              const uint32_t* __restrict__ tile_offsets,
+                                                              const PrimVal* __restrict__ vals,
+                                                              uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
+                                                              uint32_t* __restrict__ vox_g0)
+{
+    __shared__ uint32_t wave_heads[HEAD_THREADS / 64];
+    __shared__ uint32_t running;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t le_mask = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
+    const uint64_t gmask = (1ull << gid_bits) - 1ull;
+    const uint64_t base = (uint64_t)blockIdx.x * HEAD_TILE;
+    if (threadIdx.x == 0)
+        running = tile_offsets[blockIdx.x];
+    __syncthreads();
+    for (int m = 0; m < HEAD_ITEMS; m++)
     {
-        atomicMin(&a.bounds[0], mn[0]);
-        atomicMin(&a.bounds[1], mn[1]);
-        atomicMin(&a.bounds[2], mn[2]);
-        atomicMax(&a.bounds[3], mx[0]);
-        atomicMax(&a.bounds[4], mx[1]);
-        atomicMax(&a.bounds[5], mx[2]);
+        const uint64_t p = base + (uint64_t)m * HEAD_THREADS + wave * 64u + lane;
+        const bool valid = p < n;
+        const uint64_t key = valid ? keys[p] : 0ull;
+        const uint64_t vk = key >> vshift;
+        uint64_t prev = __shfl_up(vk, 1, 64);
+        if (lane == 0u)
+            prev = (valid && p > 0) ? (keys[p - 1] >> vshift) : ~vk;
+        const bool head = valid && (p == 0 || vk != prev);
+        const uint64_t hmask = __ballot(head);
+        const uint64_t vmask = __ballot(valid);
+        if (lane == 0u)
+            wave_heads[wave] = (uint32_t)__popcll(hmask);
+        __syncthreads();
+        uint32_t wave_base = running;
+        for (uint32_t w = 0; w < wave; w++)
+            wave_base += wave_heads[w];
+        const uint64_t le = hmask & le_mask;
+        const uint32_t v = wave_base + (uint32_t)__popcll(le) - 1u;
+        const uint32_t seg_start = le ? (63u - (uint32_t)__clzll(le)) : 0u;
+        const uint32_t gid = (uint32_t)((key >> 8) & gmask);
+        uint32_t val = valid ? ((vals[gid].rgb_cull & 0x00FFFFFFu) | ((uint32_t)(key & 255u) << 24)) : 0u;
+        uint32_t g = valid ? gid : 0xFFFFFFFFu;
+#pragma unroll
+        for (uint32_t off = 1; off < 64; off <<= 1)
+        {
+            uint32_t ov = __shfl_up(val, off, 64);
+            uint32_t og = __shfl_up(g, off, 64);
+            if (lane >= off && lane - off >= seg_start)
+            {
+                val = ov > val ? ov : val;
+                g = og < g ? og : g;
+            }
+        }
+        const bool next_head = lane < 63u && ((hmask >> (lane + 1u)) & 1ull);
+        const bool next_valid = lane < 63u && ((vmask >> (lane + 1u)) & 1ull);
+        if (valid && (lane == 63u || next_head || !next_valid))
+        {
+            atomicMax(&vox_val[v], val);
+            atomicMin(&vox_g0[v], g);
+        }
+        if (head)
+            vox_key[v] = vk;
+        __syncthreads();
+        if (threadIdx.x == 0)
+        {
+            uint32_t t = 0;
+            for (uint32_t w = 0; w < HEAD_THREADS / 64; w++)
+                t += wave_heads[w];
+            running += t;
+        }
+        __syncthreads();
     }
+}
+
+// per voxel: first-touch window from the lowest gaussian id, and voxel bounds
+__global__ __launch_bounds__(256) void k_voxel_finish(uint64_t nvox, KeyLayout kl, const uint64_t* __restrict__ vox_key,
+                                                      const uint32_t* __restrict__ vox_g0, const PrimBox* __restrict__ boxes,
+                                                      const uint64_t* __restrict__ foot_scan,
+                                                      const uint32_t* __restrict__ win_base, uint32_t prim_batch,
+                                                      uint32_t ijk_batch, uint32_t* __restrict__ vox_t,
+                                                      int32_t* __restrict__ bounds)
+{
+    int32_t mn[3] = { 2147483647, 2147483647, 2147483647 }, mx[3] = { -2147483647, -2147483647, -2147483647 };
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += stride)
+    {
+        int32_t i, j, k;
+        decode_voxel(kl, vox_key[v], &i, &j, &k);
+        uint32_t g0 = vox_g0[v];
+        PrimBox bx = boxes[g0];
+        uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
+                      (uint64_t)(k - bx.mn[2]);
+        uint32_t batch = g0 / prim_batch;
+        uint64_t slot = foot_scan[g0] - foot_scan[(uint64_t)batch * prim_batch] + ri;
+        vox_t[v] = win_base[batch] + (uint32_t)(slot / ijk_batch);
+        mn[0] = min(mn[0], i);
+        mn[1] = min(mn[1], j);
+        mn[2] = min(mn[2], k);
+        mx[0] = max(mx[0], i);
+        mx[1] = max(mx[1], j);
+        mx[2] = max(mx[2], k);
+    }
+    commit_bounds(mn, mx, bounds);
 }
 
 // ============================================================================ leaves
@@ -1061,11 +1355,11 @@ __global__ void k_pairs_decode(const uint64_t* __restrict__ keys, uint64_t n, Ke
         return;
     uint64_t k = keys[x];
     if (gid)
-        gid[x] = (uint32_t)(k & ((1ull << kl.gid_bits) - 1ull));
+        gid[x] = (uint32_t)((k >> kl.alpha_bits) & ((1ull << kl.gid_bits) - 1ull));
     if (ijk)
     {
         int32_t i, j, kk;
-        decode_voxel(kl, k >> kl.gid_bits, &i, &j, &kk);
+        decode_voxel(kl, k >> (kl.gid_bits + kl.alpha_bits), &i, &j, &kk);
         ijk[3 * x + 0] = i;
         ijk[3 * x + 1] = j;
         ijk[3 * x + 2] = kk;
@@ -1086,33 +1380,43 @@ static inline uint32_t bits_for(uint64_t range) // bits to represent values in [
     return b;
 }
 
-// run heads of keys >> shift: writes starts[0..runs) and starts[runs] = n; returns runs
+// run heads of keys >> shift: writes starts[0..runs) and starts[runs] = n when starts is not
+// null; returns the run count and, through tile_offsets_out, the exclusive head counts of the
+// HEAD_TILE-sized tiles (valid until the next run_heads call)
 static int run_heads(Context* ctx, const uint64_t* keys, uint64_t n, uint32_t shift, uint32_t* starts,
-                     uint64_t* runs_out)
+                     uint64_t* runs_out, const uint32_t** tile_offsets_out = nullptr)
 {
     if (n == 0)
     {
         *runs_out = 0;
-        VDBR_CHECK(ctx, hipMemsetAsync(starts, 0, 4, ctx->stream));
+        if (starts)
+            VDBR_CHECK(ctx, hipMemsetAsync(starts, 0, 4, ctx->stream));
         return 0;
     }
     if (n >= 0xFFFFFFFFull)
         return ctx->fail("run heads: more than 2^32-1 elements");
     uint64_t tiles = (n + HEAD_TILE - 1) / HEAD_TILE;
     uint32_t* tile_counts = (uint32_t*)ctx->ws(WS_SEG_TILES, (tiles + 1) * sizeof(uint32_t));
+    if (!tile_counts)
+        return -1;
     hipLaunchKernelGGL(k_head_count, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift, tile_counts);
     if (scan_u32(ctx, tile_counts, tile_counts, tiles, false, tile_counts + tiles))
         return -1;
-    hipLaunchKernelGGL(k_head_write, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
-                       (const uint32_t*)tile_counts, starts);
-    uint32_t runs = 0;
+    if (starts)
+        hipLaunchKernelGGL(k_head_write, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
+                           (const uint32_t*)tile_counts, starts);
     VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, tile_counts + tiles, 4, hipMemcpyDeviceToHost, ctx->stream));
     VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    runs = *(uint32_t*)ctx->pinned;
-    ((uint32_t*)ctx->pinned)[1] = (uint32_t)n;
-    VDBR_CHECK(ctx, hipMemcpyAsync(starts + runs, (uint32_t*)ctx->pinned + 1, 4, hipMemcpyHostToDevice, ctx->stream));
-    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    uint32_t runs = *(uint32_t*)ctx->pinned;
+    if (starts)
+    {
+        ((uint32_t*)ctx->pinned)[1] = (uint32_t)n;
+        VDBR_CHECK(ctx, hipMemcpyAsync(starts + runs, (uint32_t*)ctx->pinned + 1, 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
     *runs_out = runs;
+    if (tile_offsets_out)
+        *tile_offsets_out = tile_counts;
     return 0;
 }
 
@@ -1265,9 +1569,15 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     }
     kl.gid_bits = bits_for(n > 1 ? n : 2);
     kl.voxel_bits = kl.bits[0] + kl.bits[1] + kl.bits[2] + 9u;
-    const uint32_t key_bits = kl.voxel_bits + kl.gid_bits;
-    if (key_bits > 64)
+    if (kl.voxel_bits + kl.gid_bits > 64)
         return ctx->fail("raster_node2: pair keys need more than 64 bits");
+    // RGBA8: the pair alpha rides in the key and only the voxel bits are sorted (the max and
+    // the lowest gaussian id per voxel do not depend on the pair order). Density: voxel and
+    // gaussian id bits are sorted so each voxel sums its pairs in ascending gaussian id.
+    const bool alpha_in_key = prm->value_mode == VDBR_VALUE_RGBA8 && kl.voxel_bits + kl.gid_bits + 8u <= 64u;
+    kl.alpha_bits = alpha_in_key ? 8u : 0u;
+    const uint32_t key_bits = kl.voxel_bits + kl.gid_bits + kl.alpha_bits;
+    const uint32_t sort_begin = alpha_in_key ? kl.gid_bits + 8u : 0u;
 
     // ---------------------------------------------------------------- emit
     uint64_t pair_count = 0;
@@ -1288,6 +1598,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             EmitArgs ea;
             ea.frames = frames;
             ea.boxes = boxes;
+            ea.vals = pvals;
             ea.row_scan = row_scan;
             ea.n = n;
             ea.total_rows = total_rows;
@@ -1296,7 +1607,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             ea.out = pairs;
             ea.cap = cap;
             ea.counter = counter;
-            uint64_t blocks = (total_rows + EMIT_THREADS - 1) / EMIT_THREADS;
+            uint64_t blocks = (total_rows + EMIT_ROWS - 1) / EMIT_ROWS;
             if (blocks > 0x7FFFFFFFull)
                 return ctx->fail("raster_node2: too many AABB rows for one launch");
             VDBR_CHECK(ctx, hipEventRecord(ev[8], st));
@@ -1319,13 +1630,14 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     uint64_t* sorted = pairs;
     if (pair_count > 1)
     {
-        if (radix_sort_u64_noswap(ctx, pairs, pair_count, 0, key_bits, &sorted, ev[6], ev[7], &sort_passes))
+        if (radix_sort_u64_noswap(ctx, pairs, pair_count, sort_begin, key_bits, &sorted, ev[6], ev[7], &sort_passes))
             return -1;
     }
     VDBR_CHECK(ctx, hipEventRecord(ev[3], st));
     ctx->last_pairs = sorted;
     ctx->last_pair_count = pair_count;
     ctx->last_gid_bits = kl.gid_bits;
+    ctx->last_alpha_bits = kl.alpha_bits;
     for (int a = 0; a < 3; a++)
     {
         ctx->last_lmin[a] = kl.lmin[a];
@@ -1335,16 +1647,37 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     // ---------------------------------------------------------------- reduce per voxel
     if (pair_count >= 0xFFFFFFFFull)
         return ctx->fail("raster_node2: more than 2^32-1 pairs in one build");
-    uint32_t* seg = (uint32_t*)ctx->ws(WS_SEG, (pair_count + 2) * sizeof(uint32_t));
+    const bool rgba_wave = kl.alpha_bits != 0u;
+    uint32_t* seg = rgba_wave ? nullptr : (uint32_t*)ctx->ws(WS_SEG, (pair_count + 2) * sizeof(uint32_t));
     uint64_t nvox = 0;
-    if (run_heads(ctx, sorted, pair_count, kl.gid_bits, seg, &nvox))
+    const uint32_t* tile_offsets = nullptr;
+    if (run_heads(ctx, sorted, pair_count, kl.gid_bits + kl.alpha_bits, seg, &nvox, &tile_offsets))
         return -1;
     const uint64_t nv1 = nvox > 0 ? nvox : 1;
     uint64_t* vox_key = (uint64_t*)ctx->ws(WS_VOX_KEY, nv1 * sizeof(uint64_t));
     uint32_t* vox_val = (uint32_t*)ctx->ws(WS_VOX_VAL, nv1 * sizeof(uint32_t));
-    uint32_t* vox_t = (uint32_t*)ctx->ws(WS_VOX_T, 2 * nv1 * sizeof(uint32_t));
+    uint32_t* vox_t = (uint32_t*)ctx->ws(WS_VOX_T, 3 * nv1 * sizeof(uint32_t));
     uint32_t* vox_leaf = vox_t + nv1;
-    if (nvox > 0)
+    uint32_t* vox_g0 = vox_t + 2 * nv1;
+    if (!vox_key || !vox_val || !vox_t)
+        return -1;
+    if (nvox > 0 && rgba_wave)
+    {
+        VDBR_CHECK(ctx, hipMemsetAsync(vox_val, 0, nvox * sizeof(uint32_t), st));
+        VDBR_CHECK(ctx, hipMemsetAsync(vox_g0, 0xFF, nvox * sizeof(uint32_t), st));
+        uint64_t tiles = (pair_count + HEAD_TILE - 1) / HEAD_TILE;
+        VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
+        hipLaunchKernelGGL(k_reduce_rgba, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, st, (const uint64_t*)sorted,
+                           pair_count, kl.gid_bits + kl.alpha_bits, kl.gid_bits, tile_offsets, (const PrimVal*)pvals,
+                           vox_key, vox_val, vox_g0);
+        VDBR_CHECK(ctx, hipGetLastError());
+        VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
+        hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(nvox), 2048u)), dim3(256), 0, st, nvox, kl, (const uint64_t*)vox_key,
+                           (const uint32_t*)vox_g0, (const PrimBox*)boxes, (const uint64_t*)foot_scan,
+                           (const uint32_t*)win_base_dev, PB, W, vox_t, vbounds);
+        VDBR_CHECK(ctx, hipGetLastError());
+    }
+    else if (nvox > 0)
     {
         ReduceArgs ra;
         ra.keys = sorted;
@@ -1643,6 +1976,7 @@ int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, 
         kl.bits[a] = ctx->last_bits[a];
     }
     kl.gid_bits = ctx->last_gid_bits;
+    kl.alpha_bits = ctx->last_alpha_bits;
     hipLaunchKernelGGL(k_pairs_decode, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, ctx->last_pairs, n, kl, ijk_dev,
                        gid_dev);
     VDBR_CHECK(ctx, hipGetLastError());

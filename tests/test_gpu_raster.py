@@ -76,10 +76,11 @@ def test_preprocessing_in_place_matches_oracle(vox):
         assert np.array_equal(got.ravel().view(np.uint32), ref.ravel().view(np.uint32))
 
 
-def test_pairs_match_oracle(vox):
+@pytest.mark.parametrize("value_mode", [0, 1])
+def test_pairs_match_oracle(vox, value_mode):
     c = small_uniform(1000, 0.01, 13)
     m, op, q, s, col = preprocessed(c)
-    _run_gpu(vox, c)
+    _run_gpu(vox, c, value_mode=value_mode)
     ijk, gid = vox.last_pairs()
     ijk = ijk.cpu().numpy()
     gid = gid.cpu().numpy().astype(np.uint32)
@@ -87,9 +88,11 @@ def test_pairs_match_oracle(vox):
     a_ijk, a_gid = sort_pairs(ijk, gid)
     b_ijk, b_gid = sort_pairs(rijk, rg)
     assert np.array_equal(a_ijk, b_ijk) and np.array_equal(a_gid, b_gid)
-    # the device list is already in (voxel, gid) order of its compact key; gid ascending per voxel
+    # pairs of a voxel are contiguous; in density mode they are in ascending gaussian id
     same_voxel = np.all(ijk[1:] == ijk[:-1], axis=1)
-    assert np.all(gid[1:][same_voxel] > gid[:-1][same_voxel])
+    assert np.unique(ijk, axis=0).shape[0] == ijk.shape[0] - same_voxel.sum()
+    if value_mode == 1:
+        assert np.all(gid[1:][same_voxel] > gid[:-1][same_voxel])
 
 
 def test_density_values_within_tolerance(vox):
@@ -136,3 +139,19 @@ def test_spatial_order_same_topology(vox):
     a1, v1 = oracle.node2_active_voxels(g1)
     a2, v2 = oracle.node2_active_voxels(g2)
     assert np.array_equal(a1, a2)
+
+
+@pytest.mark.parametrize("seed", [51, 52])
+def test_extreme_shapes_pairs(vox, seed):
+    # strongly anisotropic, tiny and large Gaussians, opacities near the 0.01 cut: the row
+    # candidate interval must never drop a pair the exact predicate keeps
+    c = small_uniform(700, 0.01, seed, log_scale_mean=-4.5, log_scale_std=1.4)
+    rng = np.random.default_rng(seed)
+    c.opacity_logits[: 200] = rng.uniform(-4.59, -4.0, 200).astype(np.float32)
+    c.opacity_logits[200: 300] = rng.uniform(4.0, 9.0, 100).astype(np.float32)
+    m, op, q, s, col = preprocessed(c)
+    ref, rst = oracle.raster_node2(m, q, s, col, op, c.voxel_size)
+    got, st, _ = _run_gpu(vox, c)
+    print("extreme", rst["pair_count"], rst["fragment_count"], st)
+    assert st["pair_count"] == rst["pair_count"]
+    assert np.array_equal(got, ref)
