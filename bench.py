@@ -33,7 +33,43 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=6_000,
                     help="Gaussians in the CPU-oracle baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (reads 1 GiB, writes 1 GiB per launch): the "
+                         "reference kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    dev = torch.device("cuda", 0)
+    x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(3):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        y = x + 0.0
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    nbytes = x.numel() * 4
+    print(json.dumps({"copy_calibration": True, "read_bytes_per_launch": nbytes, "write_bytes_per_launch": nbytes,
+                      "ms": times, "gbs": 2 * nbytes / (min(times) * 1e-3) / 1e9, "check": float(y[-1])}))
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` as measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
+    default workload (the 'bench-traffic' line of the committed profile summary), or None."""
+    path = os.path.join(ROOT, "profiles", "raster_clustered_2m_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "bench-traffic" and parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(cloud, sample, threads):
@@ -60,6 +96,9 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     import torch
     from nanovdb_editor_amd import synthetic as syn
@@ -104,6 +143,8 @@ def main():
     pass_ms = avg("ms_sort_passes") / passes
     pass_bytes = P * 16
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
+    default_workload = args.gaussians == 2_000_000 and args.value_mode == "rgba8"
+    traffic = profiled_traffic("k_sort_pass") if default_workload else None
     emit_bytes = st["gaussian_count"] * 112 + P * 8  # frame+box reads per Gaussian + key writes
     out = {
         "metric": METRIC,
@@ -125,7 +166,7 @@ def main():
                    "leaves": st["leaf_count"], "key_bits": st["key_bits"], "grid_bytes": st["grid_size"]},
         "roofline": {"bound": "hbm", "kernel": "k_sort_pass (onesweep radix pass over 64-bit pair keys)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_pass_ms": pass_ms, "algorithmic_bytes_per_launch": pass_bytes},
+                     "traffic": traffic, "avg_pass_ms": pass_ms, "algorithmic_bytes_per_launch": pass_bytes},
         "pairs_per_s": P / (ms_per_step * 1e-3),
         "stage_ms": {k: avg(k) for k in ("ms_setup", "ms_emit", "ms_sort", "ms_reduce", "ms_build", "ms_total",
                                         "ms_sort_passes", "ms_emit_kernel", "ms_reduce_kernel")},

@@ -6,9 +6,10 @@
 //
 // Radix sort (ParallelPrimitives.cpp:685-869 semantics: stable, ascending, LSD over a bit
 // range): one histogram pass over the keys builds the digit counts of every 8-bit pass; each
-// pass is then a single onesweep kernel over 4096-key tiles:
+// pass is then a single onesweep kernel over 8192-key tiles (512 threads x 16 keys):
 //   - wave-local stable ranking with ballot peer masks (8 ballots per digit bit set),
 //   - per-tile digit counts published to a status array and resolved by decoupled look-back
+//     that inspects four predecessor tiles per step
 //     (epoch-tagged 64-bit status words, so the array is never cleared between passes),
 //   - keys staged in LDS in tile-sorted order and written out in digit runs.
 // Tiles take their index from an atomic counter, so every tile a block waits on has already
@@ -224,10 +225,10 @@ int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bo
 }
 
 // ============================================================================ radix sort
-constexpr int SORT_THREADS = 256;
+constexpr int SORT_THREADS = 512;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_WAVE_TILE = 64 * SORT_ITEMS;  // 1024
-constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS; // 4096
+constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS; // 8192
 constexpr int SORT_MAX_PASSES = 8;
 
 // status word: [63:48] epoch, [47:46] flag (1 = tile aggregate, 2 = inclusive prefix), [45:0] count
@@ -337,8 +338,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
                                                             uint64_t* __restrict__ status, uint32_t* __restrict__ tile_counter,
                                                             uint32_t epoch)
 {
+    constexpr int WAVES = SORT_THREADS / 64;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t wcnt[WAVES][256];
     __shared__ uint32_t s_hist_excl[256];
     __shared__ uint64_t s_base[256];
     __shared__ uint32_t s_wsum[4];
@@ -350,9 +352,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
     const uint32_t lane = tid & 63u;
     if (tid == 0)
         s_tile = atomicAdd(tile_counter, 1u);
-#pragma unroll
-    for (int w = 0; w < 4; w++)
-        wcnt[w][tid] = 0u;
+    for (uint32_t x = tid; x < WAVES * 256u; x += SORT_THREADS)
+        (&wcnt[0][0])[x] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint64_t base = (uint64_t)tile * SORT_TILE + (uint64_t)wave * SORT_WAVE_TILE;
@@ -391,22 +392,29 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
     }
     __syncthreads();
 
-    // per digit (thread tid == digit): wave offsets and tile count
+    // per digit (thread tid < 256 handles digit tid): wave offsets, tile count, look-back
+    uint64_t excl = 0ull;
+    uint32_t total = 0u;
     const uint32_t d = tid;
-    uint32_t c0 = wcnt[0][d], c1 = wcnt[1][d], c2 = wcnt[2][d], c3 = wcnt[3][d];
-    uint32_t total = c0 + c1 + c2 + c3;
-    // publish the tile aggregate first (look-back chains never wait on a later tile)
-    uint64_t* my_status = status + (uint64_t)tile * 256u + d;
-    if (tile == 0)
-        __hip_atomic_store(my_status, st_pack(epoch, ST_INCL, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else
-        __hip_atomic_store(my_status, st_pack(epoch, ST_AGG, total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    wcnt[0][d] = 0u;
-    wcnt[1][d] = c0;
-    wcnt[2][d] = c0 + c1;
-    wcnt[3][d] = c0 + c1 + c2;
-    // exclusive scan of the tile digit counts
+    if (tid < 256u)
+    {
+        uint32_t c[WAVES];
+#pragma unroll
+        for (int w = 0; w < WAVES; w++)
+            c[w] = wcnt[w][d];
+#pragma unroll
+        for (int w = 0; w < WAVES; w++)
+        {
+            wcnt[w][d] = total;
+            total += c[w];
+        }
+        uint64_t* my_status = status + (uint64_t)tile * 256u + d;
+        // publish the tile aggregate first (look-back chains never wait on a later tile)
+        __hip_atomic_store(my_status, st_pack(epoch, tile == 0 ? ST_INCL : ST_AGG, total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // exclusive scan of the tile digit counts (threads 0..255)
+    if (tid < 256u)
     {
         uint32_t v = total;
 #pragma unroll
@@ -418,36 +426,64 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
         }
         if (lane == 63u)
             s_wsum[wave] = v;
-        __syncthreads();
+    }
+    __syncthreads();
+    if (tid < 256u)
+    {
         uint32_t pre = 0u;
         for (uint32_t w = 0; w < wave; w++)
             pre += s_wsum[w];
-        s_hist_excl[d] = pre + v - total;
-    }
-    // decoupled look-back for this digit
-    uint64_t excl = 0ull;
-    if (tile > 0)
-    {
-        int64_t t = (int64_t)tile - 1;
-        while (true)
+        uint32_t v = total;
+        // recompute the in-wave inclusive prefix (cheap) to avoid keeping it live across the barrier
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
         {
-            uint64_t s = __hip_atomic_load(status + (uint64_t)t * 256u + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            uint32_t ep = (uint32_t)(s >> 48);
-            uint64_t flag = (s >> 46) & 3ull;
-            if (ep != epoch || flag == 0ull)
-            {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += s & ((1ull << 46) - 1ull);
-            if (flag == ST_INCL)
-                break;
-            t--;
+            uint32_t o = __shfl_up(v, off, 64);
+            if (lane >= (uint32_t)off)
+                v += o;
         }
-        __hip_atomic_store(my_status, st_pack(epoch, ST_INCL, excl + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_hist_excl[d] = pre + v - total;
+        // decoupled look-back, four predecessors per step
+        if (tile > 0)
+        {
+            int64_t t = (int64_t)tile - 1;
+            while (true)
+            {
+                uint64_t st4[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    st4[q] = (t - q >= 0) ? __hip_atomic_load(status + (uint64_t)(t - q) * 256u + d, __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : 0ull;
+                int consumed = 0;
+                bool done = false;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                {
+                    if (done || consumed != q || t - q < 0)
+                        continue;
+                    uint64_t sq = st4[q];
+                    uint32_t ep = (uint32_t)(sq >> 48);
+                    uint64_t flag = (sq >> 46) & 3ull;
+                    if (ep != epoch || flag == 0ull)
+                        continue; // not ready: stop consuming at this predecessor
+                    excl += sq & ((1ull << 46) - 1ull);
+                    consumed = q + 1;
+                    if (flag == ST_INCL)
+                        done = true;
+                }
+                if (done)
+                    break;
+                t -= consumed;
+                if (consumed == 0)
+                    __builtin_amdgcn_s_sleep(1);
+            }
+            __hip_atomic_store(status + (uint64_t)tile * 256u + d, st_pack(epoch, ST_INCL, excl + total),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_base[d] = digit_start[d] + excl - (uint64_t)s_hist_excl[d];
     }
     __syncthreads();
-    s_base[d] = digit_start[d] + excl - (uint64_t)s_hist_excl[d];
 #pragma unroll
     for (int m = 0; m < SORT_ITEMS; m++)
     {

@@ -39,6 +39,13 @@ class VdbrBuffer(ctypes.Structure):
     _fields_ = [("device_ptr", P), ("size_bytes", u64)]
 
 
+class PnanovdbParallelPrimitives(ctypes.Structure):
+    """Field order of pnanovdb_parallel_primitives_t (nanovdb_editor/putil/ParallelPrimitives.h:22-88)."""
+    _fields_ = [(name, P) for name in (
+        "interface_pnanovdb_reflect_data_type", "create_context", "destroy_context", "global_scan",
+        "global_scan_uint64", "global_scan_max", "radix_sort", "radix_sort_dual_key", "radix_sort_key64", "compute")]
+
+
 # exported symbols and their signatures (include/pnanovdb_amd.h)
 SIGNATURES = {
     "pnanovdb_get_raster": ([], P),
