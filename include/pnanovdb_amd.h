@@ -344,6 +344,15 @@ PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32
 PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32_t* vals, uint64_t count,
                                      uint32_t key_bit_count);
 /* Inclusive scans (ParallelPrimitives.cpp:151-310): op 0 = add u32, 1 = add u64, 2 = max u32. */
+/* putil/Convert.h:40-313 (pnanovdb_node2_convert_to_grid_type with PNANOVDB_GRID_TYPE_ONINDEX, as
+ * called by RasterUtils.cpp:187-204): converts a node2 grid in device memory into a NanoVDB
+ * OnIndex grid whose only blind data is the node2 value array. *onindex_dev points into the
+ * context workspace (valid until the next call on ctx); *onindex_size is the OnIndex layout size
+ * and *buffer_size (optional) the zero-padded length that also covers the grid size field the
+ * OnIndex header copies from the node2 header. Returns 0 on success. */
+PNANOVDB_API int vdbr_node2_to_onindex(vdbr_context_t* ctx, const void* node2_dev, uint64_t node2_size,
+                                       void** onindex_dev, uint64_t* onindex_size, uint64_t* buffer_size);
+
 PNANOVDB_API int vdbr_inclusive_scan(vdbr_context_t* ctx, const void* in, void* out, uint64_t count, int op);
 
 #ifdef __cplusplus

@@ -375,7 +375,7 @@ static void PNANOVDB_ABI r_destroy_gaussian_data(const pnanovdb_compute_t* compu
 static int run_host(Context* ctx, uint64_t n, const float* means, const float* quats, const float* scales,
                     const float* colors_or_sh0, const float* opacities, float voxel_size, float cull_scale,
                     uint32_t flags, std::vector<uint8_t>* grid_out, float* quats_back, float* scales_back,
-                    float* opac_back)
+                    float* opac_back, bool onindex = false)
 {
     uint64_t nn = n > 0 ? n : 1;
     size_t bytes = nn * (3 + 4 + 3 + 3 + 1 + 3) * sizeof(float);
@@ -415,6 +415,16 @@ static int run_host(Context* ctx, uint64_t n, const float* means, const float* q
     uint64_t size = 0;
     if (raster_node2(ctx, &g, &p, &grid, &size, nullptr))
         return -1;
+    if (onindex)
+    {
+        // RasterUtils.cpp:187-204: the node2 grid converted to OnIndex (putil/Convert.h)
+        void* oni = nullptr;
+        uint64_t oni_size = 0, oni_alloc = 0;
+        if (node2_to_onindex(ctx, grid, size, &oni, &oni_size, &oni_alloc))
+            return -1;
+        grid = oni;
+        size = oni_alloc;
+    }
     grid_out->resize(size);
     VDBR_CHECK(ctx, hipMemcpyAsync(grid_out->data(), grid, size, hipMemcpyDeviceToHost, ctx->stream));
     if (n > 0 && quats_back)
@@ -531,7 +541,7 @@ static pnanovdb_bool_t PNANOVDB_ABI r_raster_to_nanovdb_from_arrays(pnanovdb_ras
     std::vector<uint8_t> grid;
     if (run_host(ctx, n, (const float*)arrays[0]->data, (const float*)arrays[2]->data, (const float*)arrays[3]->data,
                  (const float*)arrays[4]->data, (const float*)arrays[1]->data, voxel_size, 0.f, VDBR_FLAG_RAW_INPUTS,
-                 &grid, (float*)arrays[2]->data, (float*)arrays[3]->data, (float*)arrays[1]->data))
+                 &grid, (float*)arrays[2]->data, (float*)arrays[3]->data, (float*)arrays[1]->data, true))
     {
         fprintf(stderr, "libvdbraster: %s\n", ctx->error.c_str());
         return PNANOVDB_FALSE;
@@ -873,6 +883,19 @@ PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32
         return -1;
     VDBR_CHECK(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+PNANOVDB_API int vdbr_node2_to_onindex(vdbr_context_t* ctx, const void* node2_dev, uint64_t node2_size,
+                                       void** onindex_dev, uint64_t* onindex_size, uint64_t* buffer_size)
+{
+    Context* c = (Context*)ctx;
+    if (!c || !onindex_dev || !onindex_size)
+        return -1;
+    uint64_t alloc = 0;
+    int r = node2_to_onindex(c, node2_dev, node2_size, onindex_dev, onindex_size, &alloc);
+    if (buffer_size)
+        *buffer_size = alloc;
+    return r;
 }
 
 PNANOVDB_API int vdbr_inclusive_scan(vdbr_context_t* ctx, const void* in, void* out, uint64_t count, int op)

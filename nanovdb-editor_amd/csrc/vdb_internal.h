@@ -53,6 +53,13 @@ enum WsSlot
     WS_RAW_COPY,
     WS_AUX_ALT,
     WS_AUX_VALS_ALT,
+    WS_CONV_ROOT,
+    WS_CONV_UPPERS,
+    WS_CONV_FLAGS,
+    WS_CONV_LOWERS,
+    WS_CONV_FLAGS2,
+    WS_CONV_LEAVES,
+    WS_ONINDEX,
     WS_COUNT
 };
 
@@ -131,6 +138,12 @@ int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, 
 int preprocess_device(Context* ctx, uint64_t n, float* opacities, float* quats, float* scales, const float* sh0,
                       float* colors);
 int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits);
+
+// vdb_convert.hip: node2 -> NanoVDB OnIndex (putil/Convert.h:40-313). *out_size is the OnIndex
+// layout size; *alloc_size (>= out_size and >= the grid size field copied from the node2 header,
+// a multiple of 8) is the zero-padded buffer length. The result lives in the context workspace.
+int node2_to_onindex(Context* ctx, const void* node2, uint64_t node2_size, void** out_dev, uint64_t* out_size,
+                     uint64_t* alloc_size);
 
 } // namespace vdbr
 

@@ -64,6 +64,8 @@ SIGNATURES = {
                            ctypes.POINTER(P), ctypes.POINTER(u64), ctypes.POINTER(VdbrRasterStats)], ctypes.c_int),
     "vdbr_last_pairs": ([P, P, P, u64, ctypes.POINTER(u64)], ctypes.c_int),
     "vdbr_radix_sort_u64": ([P, P, P, u64, u32, u32], ctypes.c_int),
+    "vdbr_node2_to_onindex": ([P, P, u64, ctypes.POINTER(P), ctypes.POINTER(u64), ctypes.POINTER(u64)],
+                              ctypes.c_int),
     "vdbr_radix_sort_u32": ([P, P, P, u64, u32], ctypes.c_int),
     "vdbr_inclusive_scan": ([P, P, P, u64, ctypes.c_int], ctypes.c_int),
 }

@@ -96,6 +96,23 @@ class Voxelizer:
                 self._err("vdbr_copy")
         return out, st.as_dict()
 
+    def node2_to_onindex(self, grid):
+        """NanoVDB OnIndex grid (uint8 GPU tensor, zero padded to cover the header's grid size) from a
+        node2 grid held in a uint8 GPU tensor (putil/Convert.h:40-313)."""
+        if grid.dtype != torch.uint8 or not grid.is_cuda or not grid.is_contiguous():
+            raise ValueError("grid must be a contiguous uint8 GPU tensor")
+        self._bind_stream()
+        dev = ctypes.c_void_p()
+        size = ctypes.c_uint64()
+        buf = ctypes.c_uint64()
+        if self._lib.vdbr_node2_to_onindex(self._ctx, _ptr(grid), grid.numel(), ctypes.byref(dev), ctypes.byref(size),
+                                           ctypes.byref(buf)) != 0:
+            self._err("vdbr_node2_to_onindex")
+        out = torch.empty(buf.value, dtype=torch.uint8, device=self.device)
+        if buf.value and self._lib.vdbr_copy(self._ctx, _ptr(out), dev, buf.value, 2) != 0:
+            self._err("vdbr_copy")
+        return out, size.value
+
     def last_pairs(self):
         """Sorted pair list of the last raster call: (ijk int32 [P,3], gid int32 [P]) on the GPU."""
         cnt = ctypes.c_uint64()

@@ -64,6 +64,8 @@ def lib():
         L.vdbo_density_grad_f64.restype = None
         L.vdbo_node2_active_voxels.argtypes = [P, u64, P, P, u64]
         L.vdbo_node2_active_voxels.restype = u64
+        L.vdbo_node2_to_onindex.argtypes = [P, u64, P, u64]
+        L.vdbo_node2_to_onindex.restype = u64
         _LIB = L
     return _LIB
 
@@ -169,3 +171,13 @@ def node2_active_voxels(grid):
     vidx = np.zeros(max(cnt, 1), np.uint64)
     L.vdbo_node2_active_voxels(_p(grid), grid.size, _p(ijk), _p(vidx), cnt)
     return ijk[: 3 * cnt].reshape(cnt, 3), vidx[:cnt]
+
+
+def node2_to_onindex(grid):
+    """putil/Convert.h:40-313: node2 grid bytes -> NanoVDB OnIndex grid bytes."""
+    grid = np.ascontiguousarray(grid, np.uint8)
+    L = lib()
+    size = L.vdbo_node2_to_onindex(_p(grid), grid.size, None, 0)
+    out = np.zeros(size, np.uint8)
+    L.vdbo_node2_to_onindex(_p(grid), grid.size, _p(out), size)
+    return out

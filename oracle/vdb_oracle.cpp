@@ -1355,4 +1355,203 @@ uint64_t vdbo_node2_active_voxels(const uint8_t* grid, uint64_t grid_size, int32
     return count;
 }
 
+
+// putil/Convert.h:40-313 (pnanovdb_node2_convert_to_grid_type, dst type ONINDEX) -- node2 grid to a
+// NanoVDB OnIndex grid. Five passes over the node2 tree in slot order (root tiles, uppers,
+// lowers, leaves, values), then the value blind data. The OnIndex node layout (struct offsets of
+// RootData/Tile/InternalData/LeafData<ValueOnIndex>) comes from the public NanoVDB format, which
+// PNanoVDB.h encodes and which is not part of the reference tree.
+uint64_t vdbo_node2_to_onindex(const uint8_t* grid, uint64_t grid_size, uint8_t* out, uint64_t cap)
+{
+    const uint32_t ROOT_SZ = 96u, TILE_SZ = 32u, UPPER_SZ = 270400u, LOWER_SZ = 33856u, LEAF_SZ = 96u;
+    const uint32_t UPPER_VM = 32u, UPPER_CM = 4128u, UPPER_TABLE = 8256u;
+    const uint32_t LOWER_VM = 32u, LOWER_CM = 544u, LOWER_TABLE = 1088u;
+    const uint32_t LEAF_VM = 16u, LEAF_TABLE = 80u;
+    const uint32_t GRID_TYPE_ONINDEX = 20u;
+    Buf b;
+    b.w.assign((grid_size + 7) / 8, 0ull);
+    memcpy(b.w.data(), grid, grid_size);
+    Buf d;
+    const uint32_t root = ROOT_IDX64;
+    auto get_key = [&](uint32_t n) -> uint64_t {
+        bool act = false;
+        uint32_t ki = mask_n_to_idx(b, root + OFF_KMP[ROOT], ROOT, n, &act);
+        if (!act)
+            return END_KEY;
+        uint32_t child_count = (uint32_t)(b.r(root + OFF_CMP[ROOT]) & 0xFFFFu);
+        return b.r(root + OFF_CHILDREN[ROOT] + child_count + ki);
+    };
+    auto get_child = [&](uint32_t node, uint32_t t, uint32_t n) -> uint32_t {
+        bool act = false;
+        uint32_t ci = mask_n_to_idx(b, node + OFF_CMP[t], t, n, &act);
+        if (!act)
+            return 0u;
+        return node + (uint32_t)(b.r(node + OFF_CHILDREN[t] + ci) >> 3u);
+    };
+    auto value_bit = [&](uint32_t node, uint32_t t, uint32_t n) -> bool {
+        return (b.r(node + OFF_VM[t] + (n >> 6u)) >> (n & 63u)) & 1ull;
+    };
+    auto copy_bytes = [&](uint64_t dst, uint64_t src_byte, uint32_t n) {
+        std::vector<uint8_t> tmp(n);
+        b.get(src_byte, tmp.data(), n);
+        d.put(dst, tmp.data(), n);
+    };
+    auto clear = [&](uint64_t dst, uint64_t n) { d.ensure_bytes(dst + n); };
+    const uint64_t dst_root = GRID_SIZE + TREE_SIZE;
+    auto find_tile = [&](uint64_t key) -> uint64_t {
+        uint32_t count = d.gu32(dst_root + 24u);
+        for (uint32_t t = 0; t < count; t++)
+        {
+            uint64_t tile = dst_root + ROOT_SZ + (uint64_t)t * TILE_SZ;
+            if (d.gu64(tile) == key)
+                return tile;
+        }
+        return 0ull;
+    };
+    uint64_t end = 0;
+    // grid header copied with the type changed; tree header cleared
+    copy_bytes(0, 0, GRID_SIZE);
+    d.u32(636, GRID_TYPE_ONINDEX);
+    clear(GRID_SIZE, TREE_SIZE);
+    end = GRID_SIZE + TREE_SIZE;
+    for (uint32_t pass = 0; pass < 5u; pass++)
+    {
+        if (pass == 0u)
+        {
+            clear(end, ROOT_SZ);
+            end += ROOT_SZ;
+            for (uint32_t rn = 0; rn < 32768u; rn++)
+            {
+                uint64_t key = get_key(rn);
+                if (key == END_KEY)
+                    continue;
+                uint64_t tile = end;
+                clear(end, TILE_SZ);
+                end += TILE_SZ;
+                d.u64(tile + 0u, key);
+                d.u32(tile + 16u, value_bit(root, ROOT, rn) ? 1u : 0u);
+                d.u32(dst_root + 24u, d.gu32(dst_root + 24u) + 1u);
+            }
+            // bbox from blind data [0]
+            uint64_t meta0 = b.gu64(640);
+            uint64_t bboxes = meta0 + b.gu64(meta0);
+            for (uint32_t a = 0; a < 6u; a++)
+                d.u32(dst_root + 4u * a, b.gu32(bboxes + 4u * a));
+            d.u64(GRID_SIZE + 24u, dst_root - GRID_SIZE);
+            continue;
+        }
+        uint64_t root_value_idx = b.r(root + OFF_VALUE_IDX);
+        for (uint32_t rn = 0; rn < 32768u; rn++)
+        {
+            uint64_t key = get_key(rn);
+            uint64_t tile = key != END_KEY ? find_tile(key) : 0ull;
+            if (pass == 4u)
+            {
+                bool act = false;
+                uint32_t local = mask_n_to_idx(b, root + OFF_VMP[ROOT], ROOT, rn, &act);
+                if (act)
+                    d.u64(tile + 24u, root_value_idx + local);
+            }
+            uint32_t up = get_child(root, ROOT, rn);
+            if (!up)
+                continue;
+            if (pass == 1u)
+            {
+                uint64_t du = end;
+                clear(end, UPPER_SZ);
+                end += UPPER_SZ;
+                copy_bytes(du + UPPER_CM, 8ull * (up + OFF_CM[UPPER]), 32768u / 8u);
+                copy_bytes(du + UPPER_VM, 8ull * (up + OFF_VM[UPPER]), 32768u / 8u);
+                d.u64(tile + 8u, du - dst_root);
+                continue;
+            }
+            uint64_t upper_value_idx = b.r(up + OFF_VALUE_IDX);
+            uint64_t du = dst_root + d.gu64(tile + 8u);
+            for (uint32_t un = 0; un < 32768u; un++)
+            {
+                if (pass == 4u)
+                {
+                    bool act = false;
+                    uint32_t local = mask_n_to_idx(b, up + OFF_VMP[UPPER], UPPER, un, &act);
+                    if (act)
+                        d.u64(du + UPPER_TABLE + 8ull * un, upper_value_idx + local);
+                }
+                uint32_t lo = get_child(up, UPPER, un);
+                if (!lo)
+                    continue;
+                if (pass == 2u)
+                {
+                    uint64_t dl = end;
+                    clear(end, LOWER_SZ);
+                    end += LOWER_SZ;
+                    copy_bytes(dl + LOWER_CM, 8ull * (lo + OFF_CM[LOWER]), 4096u / 8u);
+                    copy_bytes(dl + LOWER_VM, 8ull * (lo + OFF_VM[LOWER]), 4096u / 8u);
+                    d.u64(du + UPPER_TABLE + 8ull * un, dl - du);
+                    continue;
+                }
+                uint64_t lower_value_idx = b.r(lo + OFF_VALUE_IDX);
+                uint64_t dl = du + d.gu64(du + UPPER_TABLE + 8ull * un);
+                for (uint32_t ln = 0; ln < 4096u; ln++)
+                {
+                    if (pass == 4u)
+                    {
+                        bool act = false;
+                        uint32_t local = mask_n_to_idx(b, lo + OFF_VMP[LOWER], LOWER, ln, &act);
+                        if (act)
+                            d.u64(dl + LOWER_TABLE + 8ull * ln, lower_value_idx + local);
+                    }
+                    uint32_t lf = get_child(lo, LOWER, ln);
+                    if (!lf)
+                        continue;
+                    if (pass == 3u)
+                    {
+                        uint64_t dfl = end;
+                        clear(end, LEAF_SZ);
+                        end += LEAF_SZ;
+                        copy_bytes(dfl + LEAF_VM, 8ull * (lf + OFF_VM[LEAF]), 512u / 8u);
+                        d.u64(dl + LOWER_TABLE + 8ull * ln, dfl - dl);
+                        continue;
+                    }
+                    if (pass == 4u)
+                    {
+                        uint64_t dfl = dl + d.gu64(dl + LOWER_TABLE + 8ull * ln);
+                        d.u64(dfl + LEAF_TABLE, b.r(lf + OFF_VALUE_IDX));
+                        uint64_t sum = 0ull;
+                        uint32_t accum = 0u;
+                        for (uint32_t w = 0; w < 7u; w++)
+                        {
+                            accum += (uint32_t)__builtin_popcountll(d.gu64(dfl + LEAF_VM + 8u * w));
+                            sum |= (uint64_t)accum << (9u * w);
+                        }
+                        d.u64(dfl + LEAF_TABLE + 8u, sum);
+                    }
+                }
+            }
+        }
+    }
+    // value blind data (entry [1] of the node2 grid) as the only blind data
+    uint64_t meta1 = b.gu64(640) + BLINDMETADATA_SIZE;
+    uint64_t src_data = meta1 + b.gu64(meta1);
+    uint64_t dmeta = end;
+    end += BLINDMETADATA_SIZE;
+    copy_bytes(dmeta, meta1, BLINDMETADATA_SIZE);
+    uint64_t count = b.gu64(meta1 + 8u);
+    uint32_t vsize = b.gu32(meta1 + 16u);
+    uint64_t ddata = end;
+    uint64_t nbytes = count * vsize;
+    end += nbytes;
+    {
+        std::vector<uint8_t> tmp(nbytes);
+        b.get(src_data, tmp.data(), (uint32_t)nbytes);
+        d.put(ddata, tmp.data(), (uint32_t)nbytes);
+    }
+    d.u64(dmeta, ddata - dmeta);
+    d.u64(640, dmeta);
+    d.u32(648, 1u);
+    d.ensure_bytes(end);
+    if (out && cap >= end)
+        memcpy(out, d.w.data(), end);
+    return end;
+}
+
 } // extern "C"

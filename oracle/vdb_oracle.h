@@ -77,6 +77,10 @@ void vdbo_density_grad_f64(uint64_t n, uint64_t pair_count, const uint32_t* gid,
 uint64_t vdbo_node2_active_voxels(const uint8_t* grid, uint64_t grid_size, int32_t* ijk_out,
                                   uint64_t* value_idx_out, uint64_t cap);
 
+/* putil/Convert.h:40-313 -- node2 grid to NanoVDB OnIndex (value blind data carried over).
+ * Returns the OnIndex size in bytes; writes it to out when cap >= size. */
+uint64_t vdbo_node2_to_onindex(const uint8_t* grid, uint64_t grid_size, uint8_t* out, uint64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

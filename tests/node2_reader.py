@@ -212,3 +212,77 @@ class Node2Grid:
             frontier = nxt
         assert len(order) == len(by_addr)
         return order
+
+
+# ----------------------------------------------------------------------------- NanoVDB OnIndex
+# Node layouts of a NanoVDB OnIndex grid (RootData / Tile / InternalData / LeafData<ValueOnIndex>
+# with 32-byte data alignment): byte offsets within each node.
+ONI = dict(root_size=96, tile_size=32, upper_vm=32, upper_cm=4128, upper_table=8256, upper_size=270400,
+           lower_vm=32, lower_cm=544, lower_table=1088, lower_size=33856, leaf_vm=16, leaf_table=80, leaf_size=96)
+GRID_TYPE_ONINDEX = 20
+
+
+class OnIndexGrid:
+    """Reader of a NanoVDB OnIndex grid: root tiles by key, internal tables of child offsets, leaf
+    value offsets with the 9-bit packed prefix sums of the leaf value mask."""
+
+    def __init__(self, data):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        pad = (-data.size) % 8
+        if pad:
+            data = np.concatenate([data, np.zeros(pad, np.uint8)])
+        self.b = data
+        assert int(data[:8].view("<u8")[0]) == MAGIC
+        assert int(data[636:640].view("<u4")[0]) == GRID_TYPE_ONINDEX
+        self.root = 672 + int(data[672 + 24:672 + 32].view("<u8")[0])
+
+    def u64(self, off):
+        return int(self.b[off:off + 8].view("<u8")[0])
+
+    def i64(self, off):
+        return int(self.b[off:off + 8].view("<i8")[0])
+
+    def u32(self, off):
+        return int(self.b[off:off + 4].view("<u4")[0])
+
+    def bbox(self):
+        return self.b[self.root:self.root + 24].view("<i4").reshape(2, 3)
+
+    def blind(self):
+        meta = self.u64(640)
+        assert self.u32(648) == 1
+        off = meta + self.u64(meta)
+        return off, self.u64(meta + 8), self.u32(meta + 16)
+
+    def voxels(self):
+        """(M, 4) int64 array of (i, j, k, value index) over all active leaf voxels."""
+        out = []
+        r = self.root
+        tiles = self.u32(r + 24)
+        for t in range(tiles):
+            tile = r + ONI["root_size"] + ONI["tile_size"] * t
+            key = self.u64(tile)
+            child = self.i64(tile + 8)
+            if child == 0:
+                continue
+            hi = [(key >> 42) & 0x1FFFFF, (key >> 21) & 0x1FFFFF, key & 0x1FFFFF]
+            origin = [int(np.int32(np.uint32((h << 12) & 0xFFFFFFFF))) for h in hi]
+            up = r + child
+            cm = self.b[up + ONI["upper_cm"]:up + ONI["upper_cm"] + 4096].view("<u8")
+            for un in _bits(cm):
+                lo = up + self.i64(up + ONI["upper_table"] + 8 * int(un))
+                uo = (origin[0] + (((un >> 10) & 31) << 7), origin[1] + (((un >> 5) & 31) << 7), origin[2] + ((un & 31) << 7))
+                lcm = self.b[lo + ONI["lower_cm"]:lo + ONI["lower_cm"] + 512].view("<u8")
+                for ln in _bits(lcm):
+                    lf = lo + self.i64(lo + ONI["lower_table"] + 8 * int(ln))
+                    lo_o = (uo[0] + (((ln >> 8) & 15) << 3), uo[1] + (((ln >> 4) & 15) << 3), uo[2] + ((ln & 15) << 3))
+                    vm = self.b[lf + ONI["leaf_vm"]:lf + ONI["leaf_vm"] + 64].view("<u8")
+                    first = self.u64(lf + ONI["leaf_table"])
+                    psum = self.u64(lf + ONI["leaf_table"] + 8)
+                    for n in _bits(vm):
+                        w = int(n) >> 6
+                        below = int(vm[w]) & ((1 << (int(n) & 63)) - 1)
+                        before = ((psum >> (9 * (w - 1))) & 511) if w > 0 else 0
+                        idx = first + before + bin(below).count("1")
+                        out.append((lo_o[0] + ((n >> 6) & 7), lo_o[1] + ((n >> 3) & 7), lo_o[2] + (n & 7), idx))
+        return np.array(out, dtype=np.int64).reshape(-1, 4)

@@ -353,3 +353,31 @@ def test_node2_clustered_multi_leaf_levels():
     g, nodes, vox = _voxel_values(grid)
     assert st["upper_count"] >= 1 and st["lower_count"] > 1 and st["leaf_count"] > 10
     assert vox.shape[0] == st["value_count"] - 2
+
+
+# ----------------------------------------------------------------------------- node2 -> OnIndex
+@pytest.mark.parametrize("seed,n,vs", [(30, 200, 0.01), (31, 500, 0.004)])
+def test_onindex_round_trip(seed, n, vs):
+    from node2_reader import OnIndexGrid
+    cloud = syn.clustered_cloud(n, 3, 0.4, vs, -5.0, 0.6, seed=seed)
+    means, opac, quats, scales, col = preprocessed(cloud)
+    grid, st = oracle.raster_node2(means, quats, scales, col, opac, float(np.float32(vs)))
+    g = Node2Grid(grid)
+    _, v2 = g.check_invariants()
+    oni = oracle.node2_to_onindex(grid)
+    o = OnIndexGrid(oni)
+    v1 = o.voxels()
+    # same voxels, same value index per voxel (the Convert.h validation invariant)
+    k1 = np.lexsort((v1[:, 2], v1[:, 1], v1[:, 0]))
+    k2 = np.lexsort((v2[:, 2], v2[:, 1], v2[:, 0]))
+    assert np.array_equal(v1[k1], v2[k2])
+    assert np.array_equal(o.bbox(), g.bbox())
+    off, count, size = o.blind()
+    goff, gcount, gsize, _ = g.blind(1)
+    assert (count, size) == (gcount, gsize)
+    assert np.array_equal(oni[off:off + 4 * count], grid[goff:goff + 4 * gcount])
+    # layout size: header, root + tiles, uppers, lowers, leaves, one blind metadata, values
+    tiles = int(oni[o.root + 24:o.root + 28].view("<u4")[0])
+    want = 736 + 96 + 32 * tiles + 270400 * st["upper_count"] + 33856 * st["lower_count"] + 96 * st["leaf_count"] \
+        + 288 + 4 * count
+    assert oni.size == want
