@@ -257,7 +257,10 @@ typedef struct vdbr_buffer_t
 PNANOVDB_API vdbr_context_t* vdbr_context_create(int device);
 PNANOVDB_API void vdbr_context_destroy(vdbr_context_t* ctx);
 PNANOVDB_API const char* vdbr_last_error(vdbr_context_t* ctx);
-/* use an external stream (hipStream_t passed as void*); NULL restores the context's own */
+/* Run the context's work on a caller stream (hipStream_t passed as void*). NULL selects the
+ * device's null stream, which orders with torch's default stream; a new context starts on a
+ * non-blocking stream of its own (vdbr_context_get_stream returns it). Work already queued on
+ * the previous stream is completed first. */
 PNANOVDB_API int vdbr_context_set_stream(vdbr_context_t* ctx, void* stream);
 PNANOVDB_API void* vdbr_context_get_stream(vdbr_context_t* ctx);
 PNANOVDB_API int vdbr_synchronize(vdbr_context_t* ctx);
@@ -282,6 +285,16 @@ typedef struct vdbr_gaussians_t
     const float* sh0;
     uint64_t count;
 } vdbr_gaussians_t;
+
+/* Gradient outputs of vdbr_density_backward (device, float32, the layouts of vdbr_gaussians_t):
+ * d/d means[3N], d/d opacity logits[N], d/d unnormalised quats[4N], d/d log-scales[3N]. */
+typedef struct vdbr_gaussian_grads_t
+{
+    float* means;
+    float* opacities;
+    float* quats;
+    float* scales;
+} vdbr_gaussian_grads_t;
 
 #define VDBR_VALUE_RGBA8 0u   /* gaussian_frag_color.slang semantics (packed RGBA8, max) */
 #define VDBR_VALUE_DENSITY 1u /* sum_g o_g exp(-m_g/2), ascending gaussian id, float32 */
@@ -327,6 +340,18 @@ typedef struct vdbr_raster_stats_t
 PNANOVDB_API int vdbr_raster_node2(vdbr_context_t* ctx, const vdbr_gaussians_t* gaussians,
                                    const vdbr_raster_params_t* params, void** grid_dev, uint64_t* grid_size,
                                    vdbr_raster_stats_t* stats);
+
+/* Adjoint of the density value mode (no counterpart in the reference, which has no backward
+ * pass; the forward it differentiates is vdbr_raster_node2 with VDBR_VALUE_DENSITY):
+ * rho(v) = sum_g o_g exp(-|diag(1/s_g) R_g^T (x_v - mu_g)|^2 / 2) over the pairs of voxel v.
+ * raw holds the raw parameters of that forward call (means, opacity logits, unnormalised
+ * quaternions, log-scales; colors and sh0 unused), grid_dev/grid_size its node2 grid, and
+ * grad_values_dev[value_count] the gradient of the loss with respect to the grid's value array
+ * (indexed by value index; slots 0 and 1 are the background). Gradients are computed in double
+ * precision, summed in a fixed order (bitwise reproducible) and written as float32. */
+PNANOVDB_API int vdbr_density_backward(vdbr_context_t* ctx, const vdbr_gaussians_t* raw, float voxel_size,
+                                       const void* grid_dev, uint64_t grid_size, const float* grad_values_dev,
+                                       uint64_t value_count, const vdbr_gaussian_grads_t* grads);
 
 /* The surviving pair list of the last vdbr_raster_node2 call, grouped by voxel (voxels in
  * leaf-major key order; within a voxel in ascending gaussian id for VDBR_VALUE_DENSITY, in

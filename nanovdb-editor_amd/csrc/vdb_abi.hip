@@ -793,7 +793,7 @@ PNANOVDB_API int vdbr_context_set_stream(vdbr_context_t* ctx, void* stream)
     if (!c)
         return -1;
     (void)hipStreamSynchronize(c->stream);
-    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    c->stream = (hipStream_t)stream;
     return 0;
 }
 
@@ -883,6 +883,16 @@ PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32
         return -1;
     VDBR_CHECK(c, hipStreamSynchronize(c->stream));
     return 0;
+}
+
+PNANOVDB_API int vdbr_density_backward(vdbr_context_t* ctx, const vdbr_gaussians_t* raw, float voxel_size,
+                                       const void* grid_dev, uint64_t grid_size, const float* grad_values_dev,
+                                       uint64_t value_count, const vdbr_gaussian_grads_t* grads)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    return density_backward(c, raw, voxel_size, grid_dev, grid_size, grad_values_dev, value_count, grads, nullptr);
 }
 
 PNANOVDB_API int vdbr_node2_to_onindex(vdbr_context_t* ctx, const void* node2_dev, uint64_t node2_size,

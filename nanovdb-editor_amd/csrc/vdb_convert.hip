@@ -21,6 +21,7 @@
 
 #include "vdb_internal.h"
 #include "vdb_layout.h"
+#include "vdb_node2.h"
 
 namespace vdbr
 {
@@ -36,17 +37,6 @@ constexpr uint32_t ONI_LEAF_VM = 16u, ONI_LEAF_TABLE = 80u, ONI_LEAF_SIZE = 96u;
 constexpr uint32_t GRID_TYPE_ONINDEX = 20u;
 
 constexpr int CONV_THREADS = 256;
-
-// node2 accessors on 64-bit words (PNanoVDBExt.h:94-143)
-__device__ __forceinline__ uint32_t n2_mask_n_to_idx(const uint64_t* w, uint64_t prefix_idx64, uint32_t mask_off_words,
-                                                     uint32_t n, bool* active)
-{
-    uint64_t mask = w[prefix_idx64 + mask_off_words + (n >> 6u)];
-    uint32_t prefix = (uint32_t)(w[prefix_idx64 + (n >> 8u)] >> ((n >> 2u) & 0x30u)) & 0xFFFFu;
-    uint64_t cmp = 1ull << (n & 63u);
-    *active = (mask & cmp) != 0ull;
-    return (uint32_t)__popcll(mask & (cmp - 1ull)) + (n >= 64u ? prefix : 0u);
-}
 
 struct ConvLevel
 {

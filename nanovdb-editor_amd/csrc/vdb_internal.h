@@ -60,6 +60,9 @@ enum WsSlot
     WS_CONV_FLAGS2,
     WS_CONV_LEAVES,
     WS_ONINDEX,
+    WS_BWD_ACC,
+    WS_BWD_REC,
+    WS_BWD_RECGID,
     WS_COUNT
 };
 
@@ -138,6 +141,13 @@ int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, 
 int preprocess_device(Context* ctx, uint64_t n, float* opacities, float* quats, float* scales, const float* sh0,
                       float* colors);
 int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, uint32_t key_bits);
+
+// density adjoint (vdb_raster.hip): gradients of sum_v grad_values[value_idx(v)] * rho(v) with
+// respect to the raw parameters; raw->scales are log-scales, raw->opacities logits, raw->quats
+// unnormalised. grid is the node2 density grid of the forward pass on the same parameters.
+int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
+                     const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,
+                     float* ms_out);
 
 // vdb_convert.hip: node2 -> NanoVDB OnIndex (putil/Convert.h:40-313). *out_size is the OnIndex
 // layout size; *alloc_size (>= out_size and >= the grid size field copied from the node2 header,

@@ -23,6 +23,7 @@
 
 #include "vdb_device.h"
 #include "vdb_internal.h"
+#include "vdb_node2.h"
 
 namespace vdbr
 {
@@ -416,6 +417,124 @@ __device__ __forceinline__ void row_interval(const PrimFrame& f, const PrimBox& 
     *len = hi >= lo ? hi - lo + 1 : 0;
 }
 
+// The candidate voxels (i, j, k0 + st), st < len, of one AABB row, walked in lock step by the
+// wave: steps is the wave maximum of len, and on_step(st, keep, vmin, vmax) runs on every lane
+// for every st < steps (keep false past the lane's own interval). Adjacent voxels share the
+// lattice plane between them: each step transforms one new plane of 4 corner columns, and the
+// per-corner operation order is that of gaussian_frag_alloc.slang:79-121.
+template <typename OnStep>
+__device__ __forceinline__ void walk_row(const PrimFrame& f, int32_t i, int32_t j, int32_t k0, int32_t len,
+                                         int32_t steps, float vs, OnStep&& on_step)
+{
+    // row-constant partial dot products of the 4 corner columns (dx, dy)
+    float part[3][4];
+    if (len > 0)
+    {
+#pragma unroll
+        for (int col = 0; col < 4; col++)
+        {
+            float px = (float)i;
+            float py = (float)j;
+            if (col & 1)
+                px += 1.f;
+            if (col & 2)
+                py += 1.f;
+            float lx = px * vs;
+            float ly = py * vs;
+            lx = lx - f.mu[0];
+            ly = ly - f.mu[1];
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+                float t = lx * f.rinv[3 * c + 0];
+                t = t + ly * f.rinv[3 * c + 1];
+                part[c][col] = t;
+            }
+        }
+    }
+    // lattice plane z -> per-axis min/max over the 4 columns
+    auto plane = [&](int32_t kz, float pmin[3], float pmax[3]) {
+        float pz = (float)kz;
+        float lz = pz * vs;
+        lz = lz - f.mu[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+        {
+            float tz = lz * f.rinv[3 * c + 2];
+#pragma unroll
+            for (int col = 0; col < 4; col++)
+            {
+                float q = part[c][col] + tz;
+                q = q * f.inv_r;
+                q = q * f.inv_s[c];
+                if (col == 0)
+                {
+                    pmin[c] = q;
+                    pmax[c] = q;
+                }
+                else
+                {
+                    pmin[c] = fminf(pmin[c], q);
+                    pmax[c] = fmaxf(pmax[c], q);
+                }
+            }
+        }
+    };
+    float lo_min[3] = { 0.f, 0.f, 0.f }, lo_max[3] = { 0.f, 0.f, 0.f };
+    if (len > 0)
+        plane(k0, lo_min, lo_max);
+    for (int32_t st = 0; st < steps; st++)
+    {
+        bool keep = false;
+        float vmin[3] = { 0.f, 0.f, 0.f }, vmax[3] = { 0.f, 0.f, 0.f };
+        if (st < len)
+        {
+            float hi_min[3], hi_max[3];
+            plane(k0 + st + 1, hi_min, hi_max);
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+                vmin[c] = fminf(lo_min[c], hi_min[c]);
+                vmax[c] = fmaxf(lo_max[c], hi_max[c]);
+                lo_min[c] = hi_min[c];
+                lo_max[c] = hi_max[c];
+            }
+            keep = box_kept(vmin, vmax);
+        }
+        on_step(st, keep, vmin, vmax);
+    }
+}
+
+// Row -> (Gaussian, i, j, candidate interval): the Gaussian owning the row (binary search of the
+// row scan between the block's first and last Gaussian), its frame, and the pruned k interval.
+__device__ __forceinline__ bool row_candidates(const uint64_t* __restrict__ row_scan, const PrimFrame* __restrict__ frames,
+                                               const PrimBox* __restrict__ boxes, uint64_t row, uint64_t total_rows,
+                                               uint64_t glo, uint64_t ghi, float vs, uint64_t* g, PrimFrame* f,
+                                               int32_t* i, int32_t* j, int32_t* k0, int32_t* len)
+{
+    *g = 0;
+    *i = *j = *k0 = *len = 0;
+    if (row >= total_rows)
+        return false;
+    *g = find_prim(row_scan, glo, ghi, row);
+    PrimBox b = boxes[*g];
+    *f = frames[*g];
+    uint64_t lr = row - row_scan[*g];
+    *i = b.mn[0] + (int32_t)(lr / (uint64_t)b.rng_y);
+    *j = b.mn[1] + (int32_t)(lr % (uint64_t)b.rng_y);
+    *k0 = b.mn[2];
+    *len = b.rng_z;
+    row_interval(*f, b, *i, *j, vs, k0, len);
+    return true;
+}
+
+__device__ __forceinline__ int32_t wave_max(int32_t v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 // One block covers EMIT_ROWS consecutive AABB rows; each wave takes 64-row chunks (one row per
 // lane) and walks the rows' candidate intervals in lock step. Survivors are compacted into a
 // wave-private LDS stage with ballots and flushed by the wave alone (no block barriers).
@@ -448,109 +567,19 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit(EmitArgs a)
         const uint64_t row = row0 + (uint64_t)chunk * 64u + lane;
         if (row0 + (uint64_t)chunk * 64u >= a.total_rows)
             break;
-        const bool active = row < a.total_rows;
-        uint64_t g = 0;
-        int32_t len = 0, i = 0, j = 0, k0 = 0;
+        uint64_t g;
+        int32_t i, j, k0, len;
         PrimFrame f;
-        if (active)
-        {
-            g = find_prim(a.row_scan, glo, ghi, row);
-            PrimBox b = a.boxes[g];
-            f = a.frames[g];
-            uint64_t lr = row - a.row_scan[g];
-            i = b.mn[0] + (int32_t)(lr / (uint64_t)b.rng_y);
-            j = b.mn[1] + (int32_t)(lr % (uint64_t)b.rng_y);
-            k0 = b.mn[2];
-            len = b.rng_z;
-            row_interval(f, b, i, j, vs, &k0, &len);
-        }
-        int32_t steps = len;
-        for (int off = 32; off > 0; off >>= 1)
-            steps = max(steps, __shfl_xor(steps, off, 64));
-
-        // row-constant partial dot products of the 4 corner columns (dx, dy)
-        float part[3][4];
-        if (len > 0)
-        {
-#pragma unroll
-            for (int col = 0; col < 4; col++)
-            {
-                float px = (float)i;
-                float py = (float)j;
-                if (col & 1)
-                    px += 1.f;
-                if (col & 2)
-                    py += 1.f;
-                float lx = px * vs;
-                float ly = py * vs;
-                lx = lx - f.mu[0];
-                ly = ly - f.mu[1];
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                {
-                    float t = lx * f.rinv[3 * c + 0];
-                    t = t + ly * f.rinv[3 * c + 1];
-                    part[c][col] = t;
-                }
-            }
-        }
-        // lattice plane z -> per-axis min/max over the 4 columns
-        auto plane = [&](int32_t kz, float pmin[3], float pmax[3]) {
-            float pz = (float)kz;
-            float lz = pz * vs;
-            lz = lz - f.mu[2];
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-            {
-                float tz = lz * f.rinv[3 * c + 2];
-#pragma unroll
-                for (int col = 0; col < 4; col++)
-                {
-                    float q = part[c][col] + tz;
-                    q = q * f.inv_r;
-                    q = q * f.inv_s[c];
-                    if (col == 0)
-                    {
-                        pmin[c] = q;
-                        pmax[c] = q;
-                    }
-                    else
-                    {
-                        pmin[c] = fminf(pmin[c], q);
-                        pmax[c] = fmaxf(pmax[c], q);
-                    }
-                }
-            }
-        };
-        float lo_min[3] = { 0.f, 0.f, 0.f }, lo_max[3] = { 0.f, 0.f, 0.f };
-        if (len > 0)
-            plane(k0, lo_min, lo_max);
-        for (int32_t st = 0; st < steps; st++)
-        {
-            bool keep = false;
+        row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, vs, &g, &f, &i, &j, &k0, &len);
+        const int32_t steps = wave_max(len);
+        walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float* vmin, const float* vmax) {
             uint64_t key = 0ull;
             float dist = 0.f;
-            if (st < len)
+            if (keep)
             {
-                float hi_min[3], hi_max[3];
-                plane(k0 + st + 1, hi_min, hi_max);
-                float vmin[3], vmax[3];
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                {
-                    vmin[c] = fminf(lo_min[c], hi_min[c]);
-                    vmax[c] = fmaxf(lo_max[c], hi_max[c]);
-                    lo_min[c] = hi_min[c];
-                    lo_max[c] = hi_max[c];
-                }
-                keep = box_kept(vmin, vmax);
-                if (keep)
-                {
-                    key = (encode_voxel(a.kl, i, j, k0 + st) << (a.kl.gid_bits + a.kl.alpha_bits)) |
-                          (g << a.kl.alpha_bits);
-                    if (a.kl.alpha_bits)
-                        dist = min_sample_dist2(vmin, vmax);
-                }
+                key = (encode_voxel(a.kl, i, j, k0 + st) << (a.kl.gid_bits + a.kl.alpha_bits)) | (g << a.kl.alpha_bits);
+                if (a.kl.alpha_bits)
+                    dist = min_sample_dist2(vmin, vmax);
             }
             const uint64_t m = __ballot(keep);
             if (keep)
@@ -571,12 +600,293 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit(EmitArgs a)
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 count = 0u;
             }
-        }
+        });
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     emit_flush_wave(a, stage, stage_d, count, lane);
+}
+
+// ============================================================================ density backward
+// Adjoint of the density value mode: rho(v) = sum over pairs (v, g) of o_g exp(-m_g(v) / 2), with
+// m = |diag(1/s) R^T (x_v - mu)|^2 at the voxel centre, differentiated with respect to the raw
+// parameters (means, log-scales, unnormalised quaternions (w, x, y, z), opacity logits) in
+// double precision. The pairs are enumerated again exactly as the forward emits them (same row
+// intervals and predicate), each kept voxel finds dL/drho through the node2 tree, and every row
+// accumulates 16 sums of its Gaussian:
+//   E = sum r e,  Q1_a = sum W u_a,  Q2_a = sum W u_a^2,  D_ab = sum W u_a d_b
+// with r = dL/drho(v), e = exp(-m/2), W = r o e, d = x_v - mu, u = R^T d. Rows are reduced per
+// Gaussian inside the wave (segmented scan over the lanes, whose Gaussian ids are non-decreasing);
+// a Gaussian whose rows cross a 64-row chunk boundary leaves one partial per chunk, summed in
+// chunk order by the finalize pass. The result does not depend on scheduling.
+constexpr int BWD_ACC = 16;
+
+struct BwdArgs
+{
+    const PrimFrame* frames;
+    const PrimBox* boxes;
+    const uint64_t* row_scan;
+    uint64_t n;
+    uint64_t total_rows;
+    float vs;
+    const float* means;
+    const float* log_scales;
+    const float* quats; // raw
+    const float* logits;
+    const uint64_t* grid;
+    const float* grad_values;
+    uint64_t value_count;
+    double* acc;       // [n][16] Gaussians whose rows lie inside one chunk, away from its ends
+    double* rec;       // [chunks][2][16] first and last segment of each chunk
+    uint64_t* rec_gid; // [chunks][2] (~0: no second segment)
+    float* grad_means;
+    float* grad_logits;
+    float* grad_quats;
+    float* grad_log_scales;
+};
+
+// double-precision parameters of one Gaussian from its raw parameters (RasterUtils.cpp:59-117
+// preprocessing, gaussian_prim.slang:67-68 scale floor)
+struct PrimD
+{
+    double R[3][3];
+    double q[4]; // normalised (w, x, y, z)
+    double qn;
+    double s[3];
+    double is2[3]; // 1 / s^2
+    bool floored[3];
+    double o;
+    double mu[3];
+};
+
+__device__ __forceinline__ void load_prim_d(const BwdArgs& a, uint64_t g, PrimD* p)
+{
+    double qw = a.quats[4 * g + 0], qx = a.quats[4 * g + 1], qy = a.quats[4 * g + 2], qz = a.quats[4 * g + 3];
+    double qn = sqrt(qw * qw + qx * qx + qy * qy + qz * qz);
+    double w = qw / qn, x = qx / qn, y = qy / qn, z = qz / qn;
+    p->q[0] = w;
+    p->q[1] = x;
+    p->q[2] = y;
+    p->q[3] = z;
+    p->qn = qn;
+    p->R[0][0] = 1 - 2 * (y * y + z * z);
+    p->R[0][1] = 2 * (x * y - z * w);
+    p->R[0][2] = 2 * (x * z + y * w);
+    p->R[1][0] = 2 * (x * y + z * w);
+    p->R[1][1] = 1 - 2 * (x * x + z * z);
+    p->R[1][2] = 2 * (y * z - x * w);
+    p->R[2][0] = 2 * (x * z - y * w);
+    p->R[2][1] = 2 * (y * z + x * w);
+    p->R[2][2] = 1 - 2 * (x * x + y * y);
+    const double fl = (double)(0.125f * a.vs);
+    for (int c = 0; c < 3; c++)
+    {
+        double e = exp((double)a.log_scales[3 * g + c]);
+        p->floored[c] = !(e > fl);
+        p->s[c] = p->floored[c] ? fl : e;
+        p->is2[c] = 1.0 / (p->s[c] * p->s[c]);
+        p->mu[c] = a.means[3 * g + c];
+    }
+    p->o = 1.0 / (1.0 + exp(-(double)a.logits[g]));
+}
+
+__global__ __launch_bounds__(EMIT_THREADS) void k_density_bwd(BwdArgs a)
+{
+    __shared__ uint64_t g_range[2];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t row0 = (uint64_t)blockIdx.x * EMIT_ROWS;
+    if (threadIdx.x == 0)
+    {
+        g_range[0] = find_prim(a.row_scan, 0, a.n - 1, row0);
+        uint64_t last = row0 + EMIT_ROWS - 1;
+        if (last >= a.total_rows)
+            last = a.total_rows - 1;
+        g_range[1] = find_prim(a.row_scan, 0, a.n - 1, last);
+    }
+    __syncthreads();
+    const uint64_t glo = g_range[0], ghi = g_range[1];
+    const float vs = a.vs;
+    const double vsd = (double)vs;
+    for (uint32_t chunk = wave; chunk < EMIT_ROWS / 64u; chunk += EMIT_THREADS / 64u)
+    {
+        const uint64_t chunk_row = row0 + (uint64_t)chunk * 64u;
+        if (chunk_row >= a.total_rows)
+            break;
+        const uint64_t row = chunk_row + lane;
+        uint64_t g;
+        int32_t i, j, k0, len;
+        PrimFrame f;
+        const bool active =
+            row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, vs, &g, &f, &i, &j, &k0, &len);
+        double acc[BWD_ACC];
+#pragma unroll
+        for (int q = 0; q < BWD_ACC; q++)
+            acc[q] = 0.0;
+        PrimD p;
+        if (len > 0)
+            load_prim_d(a, g, &p);
+        const int32_t steps = wave_max(len);
+        uint64_t leaf = 0;
+        int32_t leaf_k = INT32_MIN;
+        walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float*, const float*) {
+            if (!keep)
+                return;
+            const int32_t k = k0 + st;
+            if ((k >> 3) != leaf_k)
+            {
+                leaf = n2_find_leaf(a.grid, i, j, k);
+                leaf_k = k >> 3;
+            }
+            uint64_t vi = leaf ? n2_leaf_value_index(a.grid, leaf, i, j, k) : ~0ull;
+            if (vi >= a.value_count)
+                return;
+            const double r = (double)a.grad_values[vi];
+            if (r == 0.0)
+                return;
+            double d[3] = { ((double)i + 0.5) * vsd - p.mu[0], ((double)j + 0.5) * vsd - p.mu[1],
+                            ((double)k + 0.5) * vsd - p.mu[2] };
+            double u[3], m = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+                u[c] = p.R[0][c] * d[0] + p.R[1][c] * d[1] + p.R[2][c] * d[2];
+                m += u[c] * u[c] * p.is2[c];
+            }
+            const double e = exp(-0.5 * m);
+            const double W = r * p.o * e;
+            acc[0] += r * e;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+                const double wu = W * u[c];
+                acc[1 + c] += wu;
+                acc[4 + c] += wu * u[c];
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+                    acc[7 + 3 * c + b] += wu * d[b];
+            }
+        });
+        // segmented inclusive scan over the lanes by Gaussian id (ids are non-decreasing)
+        const uint64_t key = active ? g : ~0ull;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            const uint64_t kup = __shfl_up(key, off, 64);
+            const bool take = lane >= (uint32_t)off && kup == key;
+#pragma unroll
+            for (int q = 0; q < BWD_ACC; q++)
+            {
+                double v = __shfl_up(acc[q], off, 64);
+                if (take)
+                    acc[q] += v;
+            }
+        }
+        const uint64_t knext = __shfl_down(key, 1, 64);
+        const uint64_t amask = __ballot(active);
+        const uint32_t last_lane = 63u - (uint32_t)__clzll(amask);
+        const uint64_t first_key = __shfl(key, 0, 64);
+        const uint64_t last_key = __shfl(key, (int)last_lane, 64);
+        const uint64_t c = chunk_row >> 6;
+        const bool seg_end = active && (lane == last_lane || knext != key);
+        if (lane == 0 && first_key == last_key)
+            a.rec_gid[2 * c + 1] = ~0ull;
+        if (seg_end)
+        {
+            double* dst;
+            if (key == first_key)
+            {
+                dst = a.rec + (2 * c) * BWD_ACC;
+                a.rec_gid[2 * c] = key;
+            }
+            else if (key == last_key)
+            {
+                dst = a.rec + (2 * c + 1) * BWD_ACC;
+                a.rec_gid[2 * c + 1] = key;
+            }
+            else
+            {
+                dst = a.acc + key * BWD_ACC;
+            }
+#pragma unroll
+            for (int q = 0; q < BWD_ACC; q++)
+                dst[q] = acc[q];
+        }
+    }
+}
+
+// per Gaussian: sum its partials in chunk order and apply the chain rule of the preprocessing
+__global__ __launch_bounds__(256) void k_density_bwd_finalize(BwdArgs a)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n)
+        return;
+    double A[BWD_ACC];
+#pragma unroll
+    for (int q = 0; q < BWD_ACC; q++)
+        A[q] = 0.0;
+    const uint64_t r0 = a.row_scan[g], r1 = a.row_scan[g + 1];
+    if (r1 > r0)
+    {
+        const uint64_t fc = r0 >> 6, lc = (r1 - 1) >> 6;
+        const bool direct = fc == lc && a.rec_gid[2 * fc] != g && a.rec_gid[2 * fc + 1] != g;
+        if (direct)
+        {
+#pragma unroll
+            for (int q = 0; q < BWD_ACC; q++)
+                A[q] = a.acc[g * BWD_ACC + q];
+        }
+        else
+        {
+            for (uint64_t c = fc; c <= lc; c++)
+            {
+                const double* src = nullptr;
+                if (a.rec_gid[2 * c] == g)
+                    src = a.rec + (2 * c) * BWD_ACC;
+                else if (a.rec_gid[2 * c + 1] == g)
+                    src = a.rec + (2 * c + 1) * BWD_ACC;
+                if (src)
+                {
+#pragma unroll
+                    for (int q = 0; q < BWD_ACC; q++)
+                        A[q] += src[q];
+                }
+            }
+        }
+    }
+    PrimD p;
+    load_prim_d(a, g, &p);
+    a.grad_logits[g] = (float)(A[0] * p.o * (1.0 - p.o));
+    for (int b = 0; b < 3; b++)
+    {
+        double gm = 0.0;
+        for (int c = 0; c < 3; c++)
+            gm += p.R[b][c] * A[1 + c] * p.is2[c];
+        a.grad_means[3 * g + b] = (float)gm;
+        a.grad_log_scales[3 * g + b] = p.floored[b] ? 0.f : (float)(A[4 + b] * p.is2[b]);
+    }
+    // dL/dR[b][c] = -D_cb / s_c^2, then through R(q) and q = q_raw / |q_raw|
+    double G[3][3];
+    for (int b = 0; b < 3; b++)
+        for (int c = 0; c < 3; c++)
+            G[b][c] = -A[7 + 3 * c + b] * p.is2[c];
+    const double w = p.q[0], x = p.q[1], y = p.q[2], z = p.q[3];
+    const double dRw[3][3] = { { 0, -2 * z, 2 * y }, { 2 * z, 0, -2 * x }, { -2 * y, 2 * x, 0 } };
+    const double dRx[3][3] = { { 0, 2 * y, 2 * z }, { 2 * y, -4 * x, -2 * w }, { 2 * z, 2 * w, -4 * x } };
+    const double dRy[3][3] = { { -4 * y, 2 * x, 2 * w }, { 2 * x, 0, 2 * z }, { -2 * w, 2 * z, -4 * y } };
+    const double dRz[3][3] = { { -4 * z, -2 * w, 2 * x }, { 2 * w, -4 * z, 2 * y }, { 2 * x, 2 * y, 0 } };
+    double gq[4] = { 0.0, 0.0, 0.0, 0.0 };
+    for (int b = 0; b < 3; b++)
+        for (int c = 0; c < 3; c++)
+        {
+            gq[0] += G[b][c] * dRw[b][c];
+            gq[1] += G[b][c] * dRx[b][c];
+            gq[2] += G[b][c] * dRy[b][c];
+            gq[3] += G[b][c] * dRz[b][c];
+        }
+    const double dot = gq[0] * w + gq[1] * x + gq[2] * y + gq[3] * z;
+    for (int k = 0; k < 4; k++)
+        a.grad_quats[4 * g + k] = (float)((gq[k] - dot * p.q[k]) / p.qn);
 }
 
 // ============================================================================ run heads
@@ -1981,6 +2291,117 @@ int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, 
                        gid_dev);
     VDBR_CHECK(ctx, hipGetLastError());
     VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+
+int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
+                     const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,
+                     float* ms_out)
+{
+    if (!raw || !grads || !grid)
+        return ctx->fail("density_backward: null argument");
+    const uint64_t n = raw->count;
+    if (!(vs > 0.f) || !(vs < 3.0e38f))
+        return ctx->fail("density_backward: voxel_size must be a positive finite number");
+    if (n >= (1ull << 32))
+        return ctx->fail("density_backward: more than 2^32-1 Gaussians");
+    if (grid_size < VDB_ROOT_ADDR + VDB_SIZE_ROOT)
+        return ctx->fail("density_backward: grid is not a node2 grid");
+    if (n == 0)
+        return 0;
+    if (!raw->means || !raw->opacities || !raw->quats || !raw->scales || !grad_values || !grads->means ||
+        !grads->opacities || !grads->quats || !grads->scales)
+        return ctx->fail("density_backward: null array");
+    hipStream_t st = ctx->stream;
+    hipEvent_t e0, e1;
+    VDBR_CHECK(ctx, hipEventCreate(&e0));
+    VDBR_CHECK(ctx, hipEventCreate(&e1));
+    struct EvGuard
+    {
+        hipEvent_t a, b;
+        ~EvGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } evg{ e0, e1 };
+    VDBR_CHECK(ctx, hipEventRecord(e0, st));
+    // preprocessed copies of the parameters (the forward's topology is recomputed from them)
+    float* pre = (float*)ctx->ws(WS_RAW_COPY, n * (4 + 3 + 1 + 3) * sizeof(float));
+    if (!pre)
+        return -1;
+    float* q_pre = pre;
+    float* s_pre = q_pre + 4 * n;
+    float* o_pre = s_pre + 3 * n;
+    float* c_pre = o_pre + n;
+    VDBR_CHECK(ctx, hipMemcpyAsync(q_pre, raw->quats, 4 * n * 4, hipMemcpyDeviceToDevice, st));
+    VDBR_CHECK(ctx, hipMemcpyAsync(s_pre, raw->scales, 3 * n * 4, hipMemcpyDeviceToDevice, st));
+    VDBR_CHECK(ctx, hipMemcpyAsync(o_pre, raw->opacities, n * 4, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_preprocess, dim3(blocks_for(n)), dim3(256), 0, st, n, o_pre, q_pre, s_pre, raw->means, c_pre);
+    PrimFrame* frames = (PrimFrame*)ctx->ws(WS_FRAMES, n * sizeof(PrimFrame));
+    PrimBox* boxes = (PrimBox*)ctx->ws(WS_BOXES, n * sizeof(PrimBox));
+    PrimVal* pvals = (PrimVal*)ctx->ws(WS_VALS, n * sizeof(PrimVal));
+    uint64_t* foot = (uint64_t*)ctx->ws(WS_FOOT, n * sizeof(uint64_t));
+    uint64_t* rows = (uint64_t*)ctx->ws(WS_ROWS, n * sizeof(uint64_t));
+    uint64_t* row_scan = (uint64_t*)ctx->ws(WS_ROW_SCAN, (n + 1) * sizeof(uint64_t));
+    uint8_t* misc = (uint8_t*)ctx->ws(WS_MISC, 4096);
+    if (!frames || !boxes || !pvals || !foot || !rows || !row_scan || !misc)
+        return -1;
+    int32_t* bounds = (int32_t*)misc;
+    unsigned long long* est = (unsigned long long*)(misc + 64);
+    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, bounds);
+    VDBR_CHECK(ctx, hipMemsetAsync(est, 0, 8, st));
+    SetupOut so{ frames, boxes, pvals, foot, rows, bounds, est };
+    hipLaunchKernelGGL(k_setup, dim3(blocks_for(n)), dim3(256), 0, st, n, raw->means, q_pre, s_pre, o_pre, c_pre, vs,
+                       1.f / vs, 0.f, so);
+    if (scan_u64(ctx, rows, row_scan, n, false, row_scan + n))
+        return -1;
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, row_scan + n, 8, hipMemcpyDeviceToHost, st));
+    VDBR_CHECK(ctx, hipStreamSynchronize(st));
+    const uint64_t total_rows = ctx->pinned[0];
+    const uint64_t chunks = (total_rows + 63) / 64;
+    double* acc = (double*)ctx->ws(WS_BWD_ACC, n * BWD_ACC * sizeof(double));
+    double* rec = (double*)ctx->ws(WS_BWD_REC, (chunks + 1) * 2 * BWD_ACC * sizeof(double));
+    uint64_t* rec_gid = (uint64_t*)ctx->ws(WS_BWD_RECGID, (chunks + 1) * 2 * sizeof(uint64_t));
+    if (!acc || !rec || !rec_gid)
+        return -1;
+    BwdArgs a;
+    a.frames = frames;
+    a.boxes = boxes;
+    a.row_scan = row_scan;
+    a.n = n;
+    a.total_rows = total_rows;
+    a.vs = vs;
+    a.means = raw->means;
+    a.log_scales = raw->scales;
+    a.quats = raw->quats;
+    a.logits = raw->opacities;
+    a.grid = (const uint64_t*)grid;
+    a.grad_values = grad_values;
+    a.value_count = value_count;
+    a.acc = acc;
+    a.rec = rec;
+    a.rec_gid = rec_gid;
+    a.grad_means = grads->means;
+    a.grad_logits = grads->opacities;
+    a.grad_quats = grads->quats;
+    a.grad_log_scales = grads->scales;
+    if (total_rows > 0)
+    {
+        uint64_t blocks = (total_rows + EMIT_ROWS - 1) / EMIT_ROWS;
+        if (blocks > 0x7FFFFFFFull)
+            return ctx->fail("density_backward: too many AABB rows for one launch");
+        hipLaunchKernelGGL(k_density_bwd, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, a);
+    }
+    hipLaunchKernelGGL(k_density_bwd_finalize, dim3(blocks_for(n)), dim3(256), 0, st, a);
+    VDBR_CHECK(ctx, hipGetLastError());
+    VDBR_CHECK(ctx, hipEventRecord(e1, st));
+    if (ms_out)
+    {
+        VDBR_CHECK(ctx, hipEventSynchronize(e1));
+        VDBR_CHECK(ctx, hipEventElapsedTime(ms_out, e0, e1));
+    }
     return 0;
 }
 

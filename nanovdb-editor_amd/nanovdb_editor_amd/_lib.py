@@ -19,6 +19,10 @@ class VdbrGaussians(ctypes.Structure):
                 ("count", u64)]
 
 
+class VdbrGaussianGrads(ctypes.Structure):
+    _fields_ = [("means", P), ("opacities", P), ("quats", P), ("scales", P)]
+
+
 class VdbrRasterParams(ctypes.Structure):
     _fields_ = [("voxel_size", f32), ("cull_scale", f32), ("value_mode", u32), ("flags", u32),
                 ("prim_batch_size", u32), ("ijk_batch_size", u32), ("max_node_count", u32), ("reserved", u32)]
@@ -64,6 +68,8 @@ SIGNATURES = {
                            ctypes.POINTER(P), ctypes.POINTER(u64), ctypes.POINTER(VdbrRasterStats)], ctypes.c_int),
     "vdbr_last_pairs": ([P, P, P, u64, ctypes.POINTER(u64)], ctypes.c_int),
     "vdbr_radix_sort_u64": ([P, P, P, u64, u32, u32], ctypes.c_int),
+    "vdbr_density_backward": ([P, ctypes.POINTER(VdbrGaussians), f32, P, u64, P, u64,
+                               ctypes.POINTER(VdbrGaussianGrads)], ctypes.c_int),
     "vdbr_node2_to_onindex": ([P, P, u64, ctypes.POINTER(P), ctypes.POINTER(u64), ctypes.POINTER(u64)],
                               ctypes.c_int),
     "vdbr_radix_sort_u32": ([P, P, P, u64, u32], ctypes.c_int),

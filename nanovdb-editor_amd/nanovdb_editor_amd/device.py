@@ -7,7 +7,7 @@ import ctypes
 
 import torch
 
-from ._lib import VdbrGaussians, VdbrRasterParams, VdbrRasterStats, lib
+from ._lib import VdbrGaussianGrads, VdbrGaussians, VdbrRasterParams, VdbrRasterStats, lib
 
 VALUE_RGBA8 = 0
 VALUE_DENSITY = 1
@@ -95,6 +95,28 @@ class Voxelizer:
             if size.value and self._lib.vdbr_copy(self._ctx, _ptr(out), grid_dev, size.value, 2) != 0:
                 self._err("vdbr_copy")
         return out, st.as_dict()
+
+    def density_backward(self, means, log_scales, quats, opacity_logits, voxel_size, grid, grad_values):
+        """Gradients (means, log_scales, quats, opacity_logits) of sum(grad_values * values) where
+        values is the value array of the density grid ``grid`` that ``raster_node2`` built from these
+        raw parameters (``raw=True``, ``value_mode=VALUE_DENSITY``)."""
+        n = means.numel() // 3
+        _check_f32("means", means, 3 * n)
+        _check_f32("log_scales", log_scales, 3 * n)
+        _check_f32("quats", quats, 4 * n)
+        _check_f32("opacity_logits", opacity_logits, n)
+        if grid.dtype != torch.uint8 or not grid.is_cuda or not grid.is_contiguous():
+            raise ValueError("grid must be a contiguous uint8 GPU tensor")
+        if grad_values.dtype != torch.float32 or not grad_values.is_cuda or not grad_values.is_contiguous():
+            raise ValueError("grad_values must be a contiguous float32 GPU tensor")
+        gm, gs, gq, go = (torch.empty_like(t) for t in (means, log_scales, quats, opacity_logits))
+        g = VdbrGaussians(_ptr(means), _ptr(opacity_logits), _ptr(quats), _ptr(log_scales), None, None, n)
+        out = VdbrGaussianGrads(_ptr(gm), _ptr(go), _ptr(gq), _ptr(gs))
+        self._bind_stream()
+        if self._lib.vdbr_density_backward(self._ctx, ctypes.byref(g), float(voxel_size), _ptr(grid), grid.numel(),
+                                           _ptr(grad_values), grad_values.numel(), ctypes.byref(out)) != 0:
+            self._err("vdbr_density_backward")
+        return gm, gs, gq, go
 
     def node2_to_onindex(self, grid):
         """NanoVDB OnIndex grid (uint8 GPU tensor, zero padded to cover the header's grid size) from a
