@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=6_000,
                     help="Gaussians in the CPU-oracle baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--fwd-bwd-steps", type=int, default=3,
+                    help="timed density forward+backward steps (torch.autograd path); 0 disables")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reads 1 GiB, writes 1 GiB per launch): the "
                          "reference kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -70,6 +72,45 @@ def profiled_traffic(kernel):
     except OSError:
         pass
     return None
+
+
+def fwd_bwd(cloud, raw, vox, steps):
+    """Density voxelization forward + backward on the same workload through the autograd API:
+    values = voxelize_density(...), loss = sum(w * values) with fixed random w, loss.backward()."""
+    import torch
+    from nanovdb_editor_amd.autograd import voxelize_density
+    params = [raw[k].clone().requires_grad_(True) for k in ("means", "s", "q", "op")]
+    w = None
+
+    def step():
+        nonlocal w
+        for p in params:
+            p.grad = None
+        values, grid = voxelize_density(params[0], params[1], params[2], params[3], cloud.voxel_size, vox=vox)
+        if w is None or w.numel() != values.numel():
+            gen = torch.Generator(device=values.device).manual_seed(7)
+            w = torch.randn(values.numel(), device=values.device, generator=gen)
+        (values * w).sum().backward()
+        return values.numel() - 2
+
+    step()  # warm-up
+    torch.cuda.synchronize()
+    t_f = t_fb = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            voxelize_density(params[0], params[1], params[2], params[3], cloud.voxel_size, vox=vox)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        voxels = step()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_f += t1 - t0
+        t_fb += t2 - t1
+    return {"value_mode": "density", "forward_ms": 1e3 * t_f / steps, "fwd_bwd_ms": 1e3 * t_fb / steps,
+            "voxels": voxels, "steps": steps,
+            "note": "float density grid forward (sorted ascending-gid sums) and its adjoint w.r.t. means, "
+                    "log-scales, quaternions, opacity logits via torch.autograd; loss = sum(w * values)"}
 
 
 def cpu_baseline(cloud, sample, threads):
@@ -172,6 +213,8 @@ def main():
                                         "ms_sort_passes", "ms_emit_kernel", "ms_reduce_kernel")},
         "emit_kernel_gbs": emit_bytes / (avg("ms_emit_kernel") * 1e-3) / 1e9 if avg("ms_emit_kernel") else None,
     }
+    if args.fwd_bwd_steps > 0:
+        out["differentiable"] = fwd_bwd(cloud, raw, vox, args.fwd_bwd_steps)
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(cloud, args.cpu_sample, args.cpu_threads)
     else:
