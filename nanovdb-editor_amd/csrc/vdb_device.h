@@ -248,4 +248,56 @@ VDB_DEV uint32_t lane_id()
     return __lane_id();
 }
 
+// DPP lane moves (GFX9 data-parallel primitives; no LDS traffic). Lanes whose source is outside
+// the row, or whose row is masked off, receive `old`.
+template <int CTRL, int ROW_MASK>
+VDB_DEV uint32_t dpp_u32(uint32_t old, uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)x, CTRL, ROW_MASK, 0xF, false);
+}
+
+// value of lane - 1 across the whole wave (lane 0 receives old)
+VDB_DEV uint64_t wave_shr1_u64(uint64_t old, uint64_t x)
+{
+    uint32_t lo = dpp_u32<0x138, 0xF>((uint32_t)old, (uint32_t)x);
+    uint32_t hi = dpp_u32<0x138, 0xF>((uint32_t)(old >> 32), (uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+VDB_DEV uint64_t readlane_u64(uint64_t x, int lane)
+{
+    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Inclusive segmented scan over the wave with an idempotent-or-associative op: lane L combines
+// the values of lanes [seg_start, L] (seg_start <= L is the first lane of L's run). Hillis-Steele
+// inside each row of 16 lanes (row_shr 1, 2, 4, 8), then the last lane of rows 0 / 2 into rows
+// 1 / 3 (row_bcast:15) and lane 31 into rows 2 and 3 (row_bcast:31).
+template <typename Op>
+VDB_DEV uint32_t seg_scan_u32(uint32_t x, uint32_t identity, uint32_t lane, uint32_t seg_start, Op op)
+{
+    uint32_t y;
+    y = dpp_u32<0x111, 0xF>(identity, x);
+    if (lane >= seg_start + 1u)
+        x = op(x, y);
+    y = dpp_u32<0x112, 0xF>(identity, x);
+    if (lane >= seg_start + 2u)
+        x = op(x, y);
+    y = dpp_u32<0x114, 0xF>(identity, x);
+    if (lane >= seg_start + 4u)
+        x = op(x, y);
+    y = dpp_u32<0x118, 0xF>(identity, x);
+    if (lane >= seg_start + 8u)
+        x = op(x, y);
+    y = dpp_u32<0x142, 0xA>(identity, x);
+    if ((lane & 16u) && seg_start < (lane & ~15u))
+        x = op(x, y);
+    y = dpp_u32<0x143, 0xC>(identity, x);
+    if (lane >= 32u && seg_start < 32u)
+        x = op(x, y);
+    return x;
+}
+
 #endif

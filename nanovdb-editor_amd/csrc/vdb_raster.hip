@@ -1053,11 +1053,38 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
     commit_bounds(mn, mx, a.bounds);
 }
 
-// RGBA8 with the alpha in the key: coalesced segmented reduction. Each block covers one
-// 4096-pair head tile (the tile's first voxel index comes from the head-count scan); each wave
-// reduces 64 consecutive pairs with a segmented shuffle scan (max of rgba, min of gaussian id)
-// and the last lane of every segment piece commits with one atomicMax / atomicMin (both are
-// order independent, so the result is deterministic).
+// RGBA8 with the alpha in the key: max over voxel runs of the sorted pairs (reduce by key).
+// Each lane owns RED_ITEMS consecutive pairs and walks them sequentially in registers; one DPP
+// segmented scan over the lanes joins the runs that cross lanes. The packed value
+// max(r8 | g8 << 8 | b8 << 16 | a8 << 24) is decided by the alpha first and the alpha rides in
+// the key, so colour bytes are gathered only for pairs whose alpha is a running maximum of their
+// run inside the lane (every pair of maximal alpha is one), all gathers in flight together.
+// Runs inside the wave are stored directly; runs cut by a wave boundary commit with atomicMax /
+// atomicMin (order independent: deterministic). Voxel indices come from the per-tile head counts
+// of run_heads (one tile of HEAD_TILE pairs per block) and an in-block scan of the lane counts.
+constexpr int RED_ITEMS = HEAD_ITEMS;
+
+__device__ __forceinline__ uint32_t wave_excl_add(uint32_t x, uint32_t lane, uint32_t* total)
+{
+    // inclusive add scan over the wave (DPP), then shifted to exclusive
+    uint32_t y;
+    y = dpp_u32<0x111, 0xF>(0u, x);
+    x += y;
+    y = dpp_u32<0x112, 0xF>(0u, x);
+    x += y;
+    y = dpp_u32<0x114, 0xF>(0u, x);
+    x += y;
+    y = dpp_u32<0x118, 0xF>(0u, x);
+    x += y;
+    y = dpp_u32<0x142, 0xA>(0u, x);
+    x += y;
+    y = dpp_u32<0x143, 0xC>(0u, x);
+    x += y;
+    *total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+    uint32_t ex = dpp_u32<0x138, 0xF>(0u, x);
+    return lane == 0u ? 0u : ex;
+}
+
 __global__ __launch_bounds__(HEAD_THREADS) void k_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
                                                               uint32_t vshift, uint32_t gid_bits,
                                                               const uint32_t* __restrict__ tile_offsets,
@@ -1066,67 +1093,171 @@ __global__ __launch_bounds__(HEAD_THREADS) void k_reduce_rgba(const uint64_t* __
                                                               uint32_t* __restrict__ vox_g0)
 {
     __shared__ uint32_t wave_heads[HEAD_THREADS / 64];
-    __shared__ uint32_t running;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint64_t le_mask = (lane == 63u) ? ~0ull : ((2ull << lane) - 1ull);
     const uint64_t gmask = (1ull << gid_bits) - 1ull;
-    const uint64_t base = (uint64_t)blockIdx.x * HEAD_TILE;
-    if (threadIdx.x == 0)
-        running = tile_offsets[blockIdx.x];
-    __syncthreads();
-    for (int m = 0; m < HEAD_ITEMS; m++)
+    const uint64_t pb = (uint64_t)blockIdx.x * HEAD_TILE + (uint64_t)threadIdx.x * RED_ITEMS;
+    auto vmax = [](uint32_t x, uint32_t y) { return x > y ? x : y; };
+    auto vmin = [](uint32_t x, uint32_t y) { return x < y ? x : y; };
+
+    uint64_t key[RED_ITEMS];
+    if (pb + RED_ITEMS <= n)
     {
-        const uint64_t p = base + (uint64_t)m * HEAD_THREADS + wave * 64u + lane;
-        const bool valid = p < n;
-        const uint64_t key = valid ? keys[p] : 0ull;
-        const uint64_t vk = key >> vshift;
-        uint64_t prev = __shfl_up(vk, 1, 64);
-        if (lane == 0u)
-            prev = (valid && p > 0) ? (keys[p - 1] >> vshift) : ~vk;
-        const bool head = valid && (p == 0 || vk != prev);
-        const uint64_t hmask = __ballot(head);
-        const uint64_t vmask = __ballot(valid);
-        if (lane == 0u)
-            wave_heads[wave] = (uint32_t)__popcll(hmask);
-        __syncthreads();
-        uint32_t wave_base = running;
-        for (uint32_t w = 0; w < wave; w++)
-            wave_base += wave_heads[w];
-        const uint64_t le = hmask & le_mask;
-        const uint32_t v = wave_base + (uint32_t)__popcll(le) - 1u;
-        const uint32_t seg_start = le ? (63u - (uint32_t)__clzll(le)) : 0u;
-        const uint32_t gid = (uint32_t)((key >> 8) & gmask);
-        uint32_t val = valid ? ((vals[gid].rgb_cull & 0x00FFFFFFu) | ((uint32_t)(key & 255u) << 24)) : 0u;
-        uint32_t g = valid ? gid : 0xFFFFFFFFu;
+        const ulonglong2* src = (const ulonglong2*)(keys + pb);
 #pragma unroll
-        for (uint32_t off = 1; off < 64; off <<= 1)
+        for (int i = 0; i < RED_ITEMS / 2; i++)
         {
-            uint32_t ov = __shfl_up(val, off, 64);
-            uint32_t og = __shfl_up(g, off, 64);
-            if (lane >= off && lane - off >= seg_start)
+            ulonglong2 v = src[i];
+            key[2 * i] = v.x;
+            key[2 * i + 1] = v.y;
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < RED_ITEMS; i++)
+            key[i] = pb + i < n ? keys[pb + i] : ~0ull;
+    }
+    // voxel key before the lane's first pair: the previous lane's last, or a load at lane 0
+    uint64_t prev = wave_shr1_u64(0ull, key[RED_ITEMS - 1]);
+    if (lane == 0u)
+        prev = (pb > 0 && pb < n) ? keys[pb - 1] : ~0ull;
+    prev >>= vshift;
+    uint32_t headbits = 0u;
+    {
+        uint64_t pv = prev;
+#pragma unroll
+        for (int i = 0; i < RED_ITEMS; i++)
+        {
+            const uint64_t vk = key[i] >> vshift;
+            const bool valid = pb + i < n;
+            if (valid && (pb + i == 0 || vk != pv))
+                headbits |= 1u << i;
+            pv = vk;
+        }
+    }
+    const uint32_t cnt = (uint32_t)__popc(headbits);
+    uint32_t wtot;
+    const uint32_t excl = wave_excl_add(cnt, lane, &wtot);
+    if (lane == 0u)
+        wave_heads[wave] = wtot;
+    // records of the running max alpha inside each run piece of the lane -> colour gathers
+    uint32_t need = 0u;
+    {
+        uint32_t am = 0u;
+#pragma unroll
+        for (int i = 0; i < RED_ITEMS; i++)
+        {
+            const uint32_t a8 = (uint32_t)(key[i] & 255u);
+            if ((headbits >> i) & 1u)
+                am = 0u;
+            if (pb + i < n && a8 >= am)
             {
-                val = ov > val ? ov : val;
-                g = og < g ? og : g;
+                need |= 1u << i;
+                am = a8;
             }
         }
-        const bool next_head = lane < 63u && ((hmask >> (lane + 1u)) & 1ull);
-        const bool next_valid = lane < 63u && ((vmask >> (lane + 1u)) & 1ull);
-        if (valid && (lane == 63u || next_head || !next_valid))
+    }
+    uint32_t rgba[RED_ITEMS];
+#pragma unroll
+    for (int i = 0; i < RED_ITEMS; i++)
+    {
+        const uint32_t gid = (uint32_t)((key[i] >> 8) & gmask);
+        rgba[i] = ((need >> i) & 1u) ? ((vals[gid].rgb_cull & 0x00FFFFFFu) | ((uint32_t)(key[i] & 255u) << 24)) : 0u;
+    }
+    __syncthreads();
+    uint32_t wbase = tile_offsets[blockIdx.x];
+    const uint32_t nvox = tile_offsets[gridDim.x]; // scan total: writes stay below it
+    for (uint32_t w = 0; w < wave; w++)
+        wbase += wave_heads[w];
+    const uint32_t base = wbase + excl; // voxel index of the lane's first head
+
+    // sequential runs: F = piece before the first head (continues an earlier run), complete runs
+    // between heads (stored here, as they are inside the wave), L = piece from the last head on
+    uint32_t fval = 0u, fg = 0xFFFFFFFFu;
+    uint32_t cval = 0u, cg = 0xFFFFFFFFu;
+    int32_t cur = -1; // ordinal of the head of the current run inside the lane, -1 before the first
+#pragma unroll
+    for (int i = 0; i < RED_ITEMS; i++)
+    {
+        const bool valid = pb + i < n;
+        const uint32_t gid = (uint32_t)((key[i] >> 8) & gmask);
+        if ((headbits >> i) & 1u)
         {
-            atomicMax(&vox_val[v], val);
-            atomicMin(&vox_g0[v], g);
+            if (cur < 0)
+            {
+                fval = cval;
+                fg = cg;
+            }
+            else if (base + (uint32_t)cur < nvox)
+            {
+                vox_val[base + cur] = cval;
+                vox_g0[base + cur] = cg;
+            }
+            cur++;
+            if (base + (uint32_t)cur < nvox)
+                vox_key[base + cur] = key[i] >> vshift;
+            cval = 0u;
+            cg = 0xFFFFFFFFu;
         }
-        if (head)
-            vox_key[v] = vk;
-        __syncthreads();
-        if (threadIdx.x == 0)
+        if (valid)
         {
-            uint32_t t = 0;
-            for (uint32_t w = 0; w < HEAD_THREADS / 64; w++)
-                t += wave_heads[w];
-            running += t;
+            cval = rgba[i] > cval ? rgba[i] : cval;
+            cg = gid < cg ? gid : cg;
         }
-        __syncthreads();
+    }
+    if (cur < 0)
+    {
+        // no head in the lane: the whole lane continues the incoming run
+        fval = cval;
+        fg = cg;
+    }
+    // cross-lane join. X = L for lanes with a head (starts a new segment), else the whole lane.
+    const bool has_head = cnt != 0u;
+    const uint64_t hl = __ballot(has_head);
+    const uint64_t below = lane ? (hl & ((1ull << lane) - 1ull)) : 0ull;
+    const uint32_t seg_start = has_head ? lane : (below ? 63u - (uint32_t)__clzll(below) : 0u);
+    const uint32_t sval = seg_scan_u32(has_head ? cval : fval, 0u, lane, seg_start, vmax);
+    const uint32_t sg = seg_scan_u32(has_head ? cg : fg, 0xFFFFFFFFu, lane, seg_start, vmin);
+    // incoming run of a lane with a head: previous lanes' accumulated piece + F
+    const uint32_t in_val = dpp_u32<0x138, 0xF>(0u, sval);
+    const uint32_t in_g = dpp_u32<0x138, 0xF>(0xFFFFFFFFu, sg);
+    const bool any_valid = pb < n;
+    if (has_head && any_valid)
+    {
+        // the incoming run ends at this lane's first head (F may be empty)
+        const bool started_in_wave = below != 0ull; // a head in an earlier lane of this wave
+        const uint32_t v = base - 1u;
+        const bool exists = lane > 0u || !(headbits & 1u); // lane 0 with a head at 0: nothing incoming
+        if (exists && base > 0u && v < nvox)
+        {
+            uint32_t tv = lane > 0u ? (in_val > fval ? in_val : fval) : fval;
+            uint32_t tg = lane > 0u ? (in_g < fg ? in_g : fg) : fg;
+            if (started_in_wave)
+            {
+                vox_val[v] = tv;
+                vox_g0[v] = tg;
+            }
+            else
+            {
+                atomicMax(&vox_val[v], tv);
+                atomicMin(&vox_g0[v], tg);
+            }
+        }
+    }
+    // the run open at the wave's last valid lane continues into the next wave (or ends there)
+    const uint64_t vl = __ballot(any_valid);
+    if (vl)
+    {
+        const uint32_t last = 63u - (uint32_t)__clzll(vl);
+        if (lane == last)
+        {
+            const uint32_t v = has_head ? base + cnt - 1u : base - 1u;
+            if (v < nvox && (has_head || base > 0u))
+            {
+                atomicMax(&vox_val[v], sval);
+                atomicMin(&vox_g0[v], sg);
+            }
+        }
     }
 }
 
