@@ -378,6 +378,13 @@ PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32
 PNANOVDB_API int vdbr_node2_to_onindex(vdbr_context_t* ctx, const void* node2_dev, uint64_t node2_size,
                                        void** onindex_dev, uint64_t* onindex_size, uint64_t* buffer_size);
 
+/* Self-test of the RGBA8 alpha evaluation (no counterpart in the reference): `count` hashed
+ * (opacity, radius, sample distance, pow exponent) inputs from `seed`; *mismatches receives the
+ * number of inputs where the fast decision differs from the correctly rounded chain of
+ * gaussian_frag_color.slang:197-212, *fast_decided (optional) how many the fast path decided. */
+PNANOVDB_API int vdbr_selftest_alpha8(vdbr_context_t* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches,
+                                      uint64_t* fast_decided);
+
 PNANOVDB_API int vdbr_inclusive_scan(vdbr_context_t* ctx, const void* in, void* out, uint64_t count, int op);
 
 #ifdef __cplusplus

@@ -895,6 +895,15 @@ PNANOVDB_API int vdbr_density_backward(vdbr_context_t* ctx, const vdbr_gaussians
     return density_backward(c, raw, voxel_size, grid_dev, grid_size, grad_values_dev, value_count, grads, nullptr);
 }
 
+PNANOVDB_API int vdbr_selftest_alpha8(vdbr_context_t* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches,
+                                      uint64_t* fast_decided)
+{
+    Context* c = (Context*)ctx;
+    if (!c || !mismatches)
+        return -1;
+    return selftest_alpha8(c, count, seed, mismatches, fast_decided);
+}
+
 PNANOVDB_API int vdbr_node2_to_onindex(vdbr_context_t* ctx, const void* node2_dev, uint64_t node2_size,
                                        void** onindex_dev, uint64_t* onindex_size, uint64_t* buffer_size)
 {

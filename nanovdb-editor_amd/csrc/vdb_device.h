@@ -212,6 +212,54 @@ VDB_DEV uint32_t alpha8_from_dist2(const PrimVal& pv, float d)
     return u8_of(a);
 }
 
+// Same result as alpha8_from_dist2, decided cheaply when possible. The reference chain is
+//   x = -(r d), t = exp(x), e = o t, a = max(0, e), b = 1 - a, p = pow(b, k), a2 = 1 - p,
+//   y = 255 clamp01(a2), alpha = trunc(y)
+// with exp and pow correctly rounded. Here exp and pow come from the hardware base-2 exp / log
+// (v_exp_f32, v_log_f32; 1 ulp each by the ISA, assumed 4 ulp) and every other step is the same
+// FP32 operation. |y_ref - y| is bounded by propagating the exp / pow errors and one rounding
+// difference per step (u = 2^-24 relative; dp/db = k p / b); the bound is doubled. When y lies
+// farther than the bound from an integer, trunc(y) = trunc(y_ref); otherwise (and for any
+// non-finite intermediate) the correctly rounded path decides.
+VDB_DEV bool alpha8_try_fast(const PrimVal& pv, float d, uint32_t* out)
+{
+    if (pv.rgb_cull >> 24)
+    {
+        *out = 0u;
+        return true;
+    }
+    const float u = 5.9604644775390625e-08f;
+    const float x = -(pv.local_scale * d);
+    const float t = __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+    const float e = pv.opacity * t;
+    const float a = fmaxf(0.f, e);
+    const float b = 1.f - a;
+    const float m = pv.pow_exp * __builtin_amdgcn_logf(b);
+    const float p = __builtin_amdgcn_exp2f(m);
+    const float a2 = 1.f - p;
+    const float y = 255.f * fmaxf(0.f, fminf(1.f, a2));
+    const float eps_t = (fabsf(x) + 8.f) * u;              // relative error of t
+    const float db = (eps_t + 3.f * u) * a + 2.f * u;       // |b_ref - b|
+    const float eps_p = (4.f * fabsf(m) + 8.f) * u;         // relative error of p at fixed b
+    float bound = 255.f * (pv.pow_exp * (p / b) * db + eps_p * p + 2.f * u) + 3.1e-5f;
+    bound = 2.f * bound + 1e-6f;
+    const float fl = floorf(y);
+    if (y - fl > bound && (fl + 1.f) - y > bound)
+    {
+        *out = (uint32_t)fl;
+        return true;
+    }
+    return false;
+}
+
+VDB_DEV uint32_t alpha8_fast(const PrimVal& pv, float d)
+{
+    uint32_t a8;
+    if (alpha8_try_fast(pv, d, &a8))
+        return a8;
+    return alpha8_from_dist2(pv, d);
+}
+
 VDB_DEV uint32_t pair_alpha8(const PrimVal& pv, const float vmin[3], const float vmax[3])
 {
     return alpha8_from_dist2(pv, min_sample_dist2(vmin, vmax));

@@ -149,6 +149,9 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
                      const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,
                      float* ms_out);
 
+// vdb_raster.hip: fast RGBA8 alpha decision against the correctly rounded path on hashed inputs
+int selftest_alpha8(Context* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches, uint64_t* decided);
+
 // vdb_convert.hip: node2 -> NanoVDB OnIndex (putil/Convert.h:40-313). *out_size is the OnIndex
 // layout size; *alloc_size (>= out_size and >= the grid size field copied from the node2 header,
 // a multiple of 8) is the zero-padded buffer length. The result lives in the context workspace.

@@ -329,7 +329,7 @@ __device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, uint64_t* sta
         if (a.kl.alpha_bits)
         {
             uint32_t gid = (uint32_t)((key >> a.kl.alpha_bits) & gmask);
-            key |= (uint64_t)alpha8_from_dist2(a.vals[gid], stage_d[e]);
+            key |= (uint64_t)alpha8_fast(a.vals[gid], stage_d[e]);
         }
         if (pos < a.cap)
             a.out[pos] = key;
@@ -887,6 +887,64 @@ __global__ __launch_bounds__(256) void k_density_bwd_finalize(BwdArgs a)
     const double dot = gq[0] * w + gq[1] * x + gq[2] * y + gq[3] * z;
     for (int k = 0; k < 4; k++)
         a.grad_quats[4 * g + k] = (float)((gq[k] - dot * p.q[k]) / p.qn);
+}
+
+// ============================================================================ alpha self-test
+// Compares alpha8_fast with the correctly rounded alpha8_from_dist2 on hashed inputs that cover
+// the ranges the value pass produces: opacity in (0.01, 1), r = sqrt(2 log(o / 0.01)) as in
+// setup, squared sample distance in [0, 4), pow exponent in [0.02, 48], about 1/8 culled.
+__device__ __forceinline__ uint32_t hash32(uint64_t x)
+{
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return (uint32_t)x;
+}
+
+__global__ void k_selftest_alpha8(uint64_t count, uint64_t seed, unsigned long long* mism,
+                                  unsigned long long* fast_taken)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long bad = 0ull, fast = 0ull;
+    for (; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const uint64_t h = seed * 0x9E3779B97F4A7C15ull + i;
+        const uint32_t h0 = hash32(h), h1 = hash32(h + 0x1234567ull), h2 = hash32(h + 0x89abcdefull);
+        PrimVal pv;
+        pv.opacity = 0.0100001f + 0.9899999f * ((float)(h0 >> 8) * (1.f / 16777216.f));
+        pv.local_scale = fsqrt_rn(2.f * vdb_logf(fdiv_rn(pv.opacity, 0.01f)));
+        pv.pow_exp = 0.02f + 48.f * ((float)(h1 >> 8) * (1.f / 16777216.f));
+        pv.rgb_cull = (h2 & 0x00FFFFFFu) | (((h2 >> 29) == 0u) ? (1u << 24) : 0u);
+        const float d = 4.f * ((float)(hash32(h + 0x5555ull) >> 8) * (1.f / 16777216.f));
+        const uint32_t ref = alpha8_from_dist2(pv, d);
+        uint32_t got;
+        if (alpha8_try_fast(pv, d, &got))
+        {
+            fast += 1ull;
+            bad += got != ref ? 1ull : 0ull;
+        }
+    }
+    atomicAdd(mism, bad);
+    atomicAdd(fast_taken, fast);
+}
+
+int selftest_alpha8(Context* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches, uint64_t* decided)
+{
+    unsigned long long* d = (unsigned long long*)ctx->ws(WS_MISC, 4096);
+    if (!d)
+        return -1;
+    unsigned long long* cnt = d + 32;
+    VDBR_CHECK(ctx, hipMemsetAsync(cnt, 0, 16, ctx->stream));
+    hipLaunchKernelGGL(k_selftest_alpha8, dim3(4096), dim3(256), 0, ctx->stream, count, seed, cnt, cnt + 1);
+    VDBR_CHECK(ctx, hipGetLastError());
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, cnt, 16, hipMemcpyDeviceToHost, ctx->stream));
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *mismatches = ctx->pinned[0];
+    if (decided)
+        *decided = ctx->pinned[1];
+    return 0;
 }
 
 // ============================================================================ run heads

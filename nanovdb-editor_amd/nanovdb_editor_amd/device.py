@@ -135,6 +135,15 @@ class Voxelizer:
             self._err("vdbr_copy")
         return out, size.value
 
+    def selftest_alpha8(self, count, seed=1):
+        """(mismatches, fast_decided) of the fast RGBA8 alpha against the correctly rounded chain."""
+        mism = ctypes.c_uint64()
+        fast = ctypes.c_uint64()
+        self._bind_stream()
+        if self._lib.vdbr_selftest_alpha8(self._ctx, int(count), int(seed), ctypes.byref(mism), ctypes.byref(fast)) != 0:
+            self._err("vdbr_selftest_alpha8")
+        return mism.value, fast.value
+
     def last_pairs(self):
         """Sorted pair list of the last raster call: (ijk int32 [P,3], gid int32 [P]) on the GPU."""
         cnt = ctypes.c_uint64()
