@@ -123,6 +123,9 @@ static Context* create_context(int device)
     if (!ctx)
         return nullptr;
     ctx->device = device;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->emit_blocks = (uint32_t)cus * 3u; // 3 blocks of the emission kernel fit a CU's LDS
     if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess)
     {
         delete ctx;

@@ -177,23 +177,23 @@ VDB_DEV uint32_t u8_of(float v)
 // separable sum over the 4x4x4 lattice is the sum of per-axis minima for the same reason
 // (rounded addition is monotone in each operand).
 // minimum over the 4x4x4 sample lattice of |lerp(vmin, vmax, w)|^2 (gaussian_frag_color.slang:192-199)
+// The end samples are exact: w = 0 gives 0*vmax + 1*vmin = vmin and w = fl(3 fl(1/3)) = 1 gives
+// vmax + 0*vmin = vmax for finite box bounds, so only the two interior samples are formed.
 VDB_DEV float min_sample_dist2(const float vmin[3], const float vmax[3])
 {
     const float third = 1.f / 3.f;
+    const float w1 = third * 1.f, ow1 = 1.f - w1;
+    const float w2 = third * 2.f, ow2 = 1.f - w2;
     float msq[3];
 #pragma unroll
     for (int c = 0; c < 3; c++)
     {
-        float best = 0.f;
-#pragma unroll
-        for (int s = 0; s < 4; s++)
-        {
-            float w = third * (float)s;
-            float ow = 1.f - w;
-            float lp = w * vmax[c] + ow * vmin[c];
-            float sq = lp * lp;
-            best = (s == 0) ? sq : fminf(best, sq);
-        }
+        float l1 = w1 * vmax[c] + ow1 * vmin[c];
+        float l2 = w2 * vmax[c] + ow2 * vmin[c];
+        float best = vmin[c] * vmin[c];
+        best = fminf(best, l1 * l1);
+        best = fminf(best, l2 * l2);
+        best = fminf(best, vmax[c] * vmax[c]);
         msq[c] = best;
     }
     float d = msq[0] + msq[1];
@@ -317,6 +317,28 @@ VDB_DEV uint64_t readlane_u64(uint64_t x, int lane)
     uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, lane);
     uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
     return ((uint64_t)hi << 32) | lo;
+}
+
+// exclusive add scan over the wave; *total receives the wave sum
+VDB_DEV uint32_t wave_excl_add(uint32_t x, uint32_t lane, uint32_t* total)
+{
+    // inclusive add scan over the wave (DPP), then shifted to exclusive
+    uint32_t y;
+    y = dpp_u32<0x111, 0xF>(0u, x);
+    x += y;
+    y = dpp_u32<0x112, 0xF>(0u, x);
+    x += y;
+    y = dpp_u32<0x114, 0xF>(0u, x);
+    x += y;
+    y = dpp_u32<0x118, 0xF>(0u, x);
+    x += y;
+    y = dpp_u32<0x142, 0xA>(0u, x);
+    x += y;
+    y = dpp_u32<0x143, 0xC>(0u, x);
+    x += y;
+    *total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+    uint32_t ex = dpp_u32<0x138, 0xF>(0u, x);
+    return lane == 0u ? 0u : ex;
 }
 
 // Inclusive segmented scan over the wave with an idempotent-or-associative op: lane L combines

@@ -60,6 +60,8 @@ enum WsSlot
     WS_CONV_FLAGS2,
     WS_CONV_LEAVES,
     WS_ONINDEX,
+    WS_SEGS,
+    WS_SEG_META,
     WS_BWD_ACC,
     WS_BWD_REC,
     WS_BWD_RECGID,
@@ -90,6 +92,7 @@ struct Context
     void* ws_ptr[WS_COUNT] = {};
     size_t ws_size[WS_COUNT] = {};
     uint64_t* pinned = nullptr; // small host readback area (pinned)
+    uint32_t emit_blocks = 2048; // resident blocks of the persistent emission kernel (CUs x 8)
     uint32_t sort_epoch = 0;
     uint64_t sort_status_tiles = 0;
 

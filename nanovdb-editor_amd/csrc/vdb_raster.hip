@@ -293,7 +293,20 @@ struct EmitArgs
     uint64_t* out;
     uint64_t cap;
     unsigned long long* counter;
+    const uint4* segs; // candidate segments (k_row_segments), in SEG_PARTS partitions
+    uint64_t seg_part_cap;          // capacity of one partition
+    const uint64_t* seg_part_count; // segments written per partition
+    const uint64_t* seg_chunk_scan; // SEG_PARTS + 1 exclusive scan of 64-segment chunks per partition
 };
+
+// segment buffer partitions: block b reserves in partition b % SEG_PARTS (separate counters on
+// separate cache lines, so reservations do not serialize on one address)
+constexpr uint32_t SEG_PARTS = 64;
+constexpr uint32_t SEG_COUNTER_STRIDE = 16; // u64 words between counters (128 bytes)
+
+// A candidate segment: up to EMIT_SEG consecutive voxels (i, j, k .. k + count - 1) of one
+// Gaussian's pruned row interval. x = gaussian id, y = i, z = j, w = (k + 2^17) | count << 20.
+constexpr int32_t EMIT_SEG = 8;
 
 __device__ __forceinline__ uint64_t find_prim(const uint64_t* __restrict__ row_scan, uint64_t lo, uint64_t hi, uint64_t row)
 {
@@ -535,18 +548,18 @@ __device__ __forceinline__ int32_t wave_max(int32_t v)
     return v;
 }
 
-// One block covers EMIT_ROWS consecutive AABB rows; each wave takes 64-row chunks (one row per
-// lane) and walks the rows' candidate intervals in lock step. Survivors are compacted into a
-// wave-private LDS stage with ballots and flushed by the wave alone (no block barriers).
-__global__ __launch_bounds__(EMIT_THREADS) void k_emit(EmitArgs a)
+// Row pass of the segment-driven emission: one lane per AABB row computes the pruned candidate
+// interval and writes it as segments of at most EMIT_SEG voxels (wave-compacted, one atomic
+// reservation per wave). The emission pass then walks segments, so the lanes of a wave walk at
+// most EMIT_SEG steps each instead of the longest row of their 64.
+__global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4* __restrict__ segs, uint64_t part_cap,
+                                                               unsigned long long* __restrict__ seg_counters)
 {
-    __shared__ uint64_t stage_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
-    __shared__ float stage_d_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
+    const uint32_t part = blockIdx.x % SEG_PARTS;
+    unsigned long long* seg_counter = seg_counters + part * SEG_COUNTER_STRIDE;
+    uint4* part_segs = segs + (uint64_t)part * part_cap;
     __shared__ uint64_t g_range[2];
-
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint64_t* stage = stage_all[wave];
-    float* stage_d = stage_d_all[wave];
     const uint64_t row0 = (uint64_t)blockIdx.x * EMIT_ROWS;
     if (threadIdx.x == 0)
     {
@@ -558,10 +571,6 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit(EmitArgs a)
     }
     __syncthreads();
     const uint64_t glo = g_range[0], ghi = g_range[1];
-    const float vs = a.vs;
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
-    uint32_t count = 0u; // wave-uniform stage fill
-
     for (uint32_t chunk = wave; chunk < EMIT_ROWS / 64u; chunk += EMIT_THREADS / 64u)
     {
         const uint64_t row = row0 + (uint64_t)chunk * 64u + lane;
@@ -570,14 +579,104 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit(EmitArgs a)
         uint64_t g;
         int32_t i, j, k0, len;
         PrimFrame f;
-        row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, vs, &g, &f, &i, &j, &k0, &len);
+        row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, a.vs, &g, &f, &i, &j, &k0, &len);
+        const uint32_t nseg = len > 0 ? (uint32_t)((len + EMIT_SEG - 1) / EMIT_SEG) : 0u;
+        uint32_t total;
+        const uint32_t excl = wave_excl_add(nseg, lane, &total);
+        unsigned long long base = 0ull;
+        if (lane == 0u && total)
+            base = atomicAdd(seg_counter, (unsigned long long)total);
+        base = __shfl(base, 0, 64);
+        // the wave writes its segments with consecutive lanes: segment s belongs to the last lane
+        // whose exclusive count is <= s
+        for (uint32_t s0 = 0; s0 < total; s0 += 64u)
+        {
+            const uint32_t sidx = s0 + lane;
+            uint32_t lo = 0, hi = 63;
+#pragma unroll
+            for (int step = 0; step < 6; step++)
+            {
+                const uint32_t mid = (lo + hi + 1u) >> 1;
+                const uint32_t ex_mid = (uint32_t)__shfl((int)excl, (int)mid, 64);
+                if (ex_mid <= sidx)
+                    lo = mid;
+                else
+                    hi = mid - 1u;
+            }
+            const uint32_t owner = lo;
+            const uint32_t o_excl = (uint32_t)__shfl((int)excl, (int)owner, 64);
+            const uint32_t o_g = (uint32_t)__shfl((int)(uint32_t)g, (int)owner, 64);
+            const int32_t o_i = __shfl(i, (int)owner, 64);
+            const int32_t o_j = __shfl(j, (int)owner, 64);
+            const int32_t o_k0 = __shfl(k0, (int)owner, 64);
+            const int32_t o_len = __shfl(len, (int)owner, 64);
+            const int32_t q = (int32_t)(sidx - o_excl);
+            const uint64_t pos = base + sidx;
+            if (sidx < total && pos < part_cap)
+            {
+                const int32_t kq = o_k0 + q * EMIT_SEG;
+                const int32_t cnt = min(EMIT_SEG, o_len - q * EMIT_SEG);
+                part_segs[pos] = make_uint4(o_g, (uint32_t)o_i, (uint32_t)o_j,
+                                            ((uint32_t)(kq + 131072) & 0xFFFFFu) | ((uint32_t)cnt << 20));
+            }
+        }
+    }
+}
+
+// Segment-driven emission: each lane takes one candidate segment, walks its voxels with the row
+// walker (same FP32 predicate, lattice planes shared within the segment) and stages survivors as
+// in the row-per-lane form.
+__global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
+{
+    __shared__ uint64_t stage_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
+    __shared__ float stage_d_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint64_t* stage = stage_all[wave];
+    float* stage_d = stage_d_all[wave];
+    const float vs = a.vs;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    uint32_t count = 0u;
+    const uint64_t waves_total = (uint64_t)gridDim.x * (EMIT_THREADS / 64);
+    const uint64_t chunks_total = a.seg_chunk_scan[SEG_PARTS];
+    for (uint64_t c = (uint64_t)blockIdx.x * (EMIT_THREADS / 64) + wave; c < chunks_total; c += waves_total)
+    {
+        // partition of chunk c (wave-uniform binary search over the chunk scan)
+        uint32_t lo = 0, hi = SEG_PARTS - 1;
+        while (lo < hi)
+        {
+            uint32_t mid = (lo + hi + 1) >> 1;
+            if (a.seg_chunk_scan[mid] <= c)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const uint64_t local = (c - a.seg_chunk_scan[lo]) * 64u + lane;
+        int32_t len = 0, i = 0, j = 0, k0 = 0;
+        uint64_t g = 0;
+        PrimFrame f;
+        if (local < a.seg_part_count[lo])
+        {
+            const uint4 sg = a.segs[(uint64_t)lo * a.seg_part_cap + local];
+            g = sg.x;
+            i = (int32_t)sg.y;
+            j = (int32_t)sg.z;
+            k0 = (int32_t)(sg.w & 0xFFFFFu) - 131072;
+            len = (int32_t)(sg.w >> 20);
+            f = a.frames[g];
+        }
         const int32_t steps = wave_max(len);
+        // voxel key along the segment: the (i, j) part is fixed, k moves the z leaf and slot
+        const uint32_t kshift = a.kl.gid_bits + a.kl.alpha_bits;
+        const uint64_t ij_key = encode_voxel(a.kl, i, j, a.kl.lmin[2] * 8);
+        const uint64_t g_bits = g << a.kl.alpha_bits;
         walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float* vmin, const float* vmax) {
             uint64_t key = 0ull;
             float dist = 0.f;
             if (keep)
             {
-                key = (encode_voxel(a.kl, i, j, k0 + st) << (a.kl.gid_bits + a.kl.alpha_bits)) | (g << a.kl.alpha_bits);
+                const int32_t k = k0 + st;
+                const uint64_t vkey = ij_key + ((uint64_t)(uint32_t)((k >> 3) - a.kl.lmin[2]) << 9) + (uint64_t)(k & 7);
+                key = (vkey << kshift) | g_bits;
                 if (a.kl.alpha_bits)
                     dist = min_sample_dist2(vmin, vmax);
             }
@@ -1121,27 +1220,6 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
 // atomicMin (order independent: deterministic). Voxel indices come from the per-tile head counts
 // of run_heads (one tile of HEAD_TILE pairs per block) and an in-block scan of the lane counts.
 constexpr int RED_ITEMS = HEAD_ITEMS;
-
-__device__ __forceinline__ uint32_t wave_excl_add(uint32_t x, uint32_t lane, uint32_t* total)
-{
-    // inclusive add scan over the wave (DPP), then shifted to exclusive
-    uint32_t y;
-    y = dpp_u32<0x111, 0xF>(0u, x);
-    x += y;
-    y = dpp_u32<0x112, 0xF>(0u, x);
-    x += y;
-    y = dpp_u32<0x114, 0xF>(0u, x);
-    x += y;
-    y = dpp_u32<0x118, 0xF>(0u, x);
-    x += y;
-    y = dpp_u32<0x142, 0xA>(0u, x);
-    x += y;
-    y = dpp_u32<0x143, 0xC>(0u, x);
-    x += y;
-    *total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
-    uint32_t ex = dpp_u32<0x138, 0xF>(0u, x);
-    return lane == 0u ? 0u : ex;
-}
 
 __global__ __launch_bounds__(HEAD_THREADS) void k_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
                                                               uint32_t vshift, uint32_t gid_bits,
@@ -2082,8 +2160,74 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     uint64_t pair_count = 0;
     uint64_t* pairs = nullptr;
     uint32_t attempts = 0;
+    uint4* segs = nullptr;
+    uint64_t seg_count = 0;
     if (any)
     {
+        // candidate segments of the pruned row intervals
+        uint64_t blocks_rows = (total_rows + EMIT_ROWS - 1) / EMIT_ROWS;
+        if (blocks_rows > 0x7FFFFFFFull)
+            return ctx->fail("raster_node2: too many AABB rows for one launch");
+        uint64_t seg_total = total_rows + (2 * est_pairs) / EMIT_SEG + 1024;
+        if (seg_total > total_rows + frag_total / EMIT_SEG + 1024)
+            seg_total = total_rows + frag_total / EMIT_SEG + 1024;
+        // partitions receive blocks round robin; a quarter of slack absorbs their imbalance
+        uint64_t part_cap = seg_total / SEG_PARTS + seg_total / (4 * SEG_PARTS) + 4096;
+        uint64_t* seg_meta = (uint64_t*)ctx->ws(WS_SEG_META, (SEG_PARTS * SEG_COUNTER_STRIDE + 2 * (SEG_PARTS + 1)) * 8);
+        if (!seg_meta)
+            return -1;
+        unsigned long long* seg_counters = (unsigned long long*)seg_meta;
+        uint64_t* part_count_dev = seg_meta + SEG_PARTS * SEG_COUNTER_STRIDE;
+        uint64_t* chunk_scan_dev = part_count_dev + (SEG_PARTS + 1);
+        VDBR_CHECK(ctx, hipEventRecord(ev[8], st));
+        for (uint32_t sa = 0;; sa++)
+        {
+            segs = (uint4*)ctx->ws(WS_SEGS, SEG_PARTS * part_cap * sizeof(uint4));
+            if (!segs)
+                return -1;
+            VDBR_CHECK(ctx, hipMemsetAsync(seg_counters, 0, SEG_PARTS * SEG_COUNTER_STRIDE * 8, st));
+            EmitArgs ra;
+            memset(&ra, 0, sizeof(ra));
+            ra.frames = frames;
+            ra.boxes = boxes;
+            ra.row_scan = row_scan;
+            ra.n = n;
+            ra.total_rows = total_rows;
+            ra.vs = vs;
+            hipLaunchKernelGGL(k_row_segments, dim3((uint32_t)blocks_rows), dim3(EMIT_THREADS), 0, st, ra, segs, part_cap,
+                               seg_counters);
+            VDBR_CHECK(ctx, hipGetLastError());
+            VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, seg_counters, SEG_PARTS * SEG_COUNTER_STRIDE * 8,
+                                           hipMemcpyDeviceToHost, st));
+            VDBR_CHECK(ctx, hipStreamSynchronize(st));
+            uint64_t maxc = 0;
+            seg_count = 0;
+            for (uint32_t q = 0; q < SEG_PARTS; q++)
+            {
+                uint64_t cq = ctx->pinned[q * SEG_COUNTER_STRIDE];
+                maxc = cq > maxc ? cq : maxc;
+                seg_count += cq;
+            }
+            if (maxc <= part_cap)
+                break;
+            if (sa >= 1)
+                return ctx->fail("raster_node2: segment buffer overflow after resize");
+            part_cap = maxc;
+        }
+        {
+            uint64_t host[2 * (SEG_PARTS + 1)];
+            uint64_t acc = 0;
+            for (uint32_t q = 0; q < SEG_PARTS; q++)
+            {
+                uint64_t cq = ctx->pinned[q * SEG_COUNTER_STRIDE];
+                host[q] = cq;
+                host[SEG_PARTS + 1 + q] = acc;
+                acc += (cq + 63) / 64;
+            }
+            host[SEG_PARTS] = 0;
+            host[2 * SEG_PARTS + 1] = acc;
+            VDBR_CHECK(ctx, hipMemcpy(part_count_dev, host, sizeof(host), hipMemcpyHostToDevice));
+        }
         uint64_t cap = est_pairs + est_pairs / 4 + 1024;
         if (cap > frag_total)
             cap = frag_total;
@@ -2106,11 +2250,19 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             ea.out = pairs;
             ea.cap = cap;
             ea.counter = counter;
-            uint64_t blocks = (total_rows + EMIT_ROWS - 1) / EMIT_ROWS;
-            if (blocks > 0x7FFFFFFFull)
-                return ctx->fail("raster_node2: too many AABB rows for one launch");
-            VDBR_CHECK(ctx, hipEventRecord(ev[8], st));
-            hipLaunchKernelGGL(k_emit, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, ea);
+            ea.segs = segs;
+            ea.seg_part_cap = part_cap;
+            ea.seg_part_count = part_count_dev;
+            ea.seg_chunk_scan = chunk_scan_dev;
+            // persistent waves: each walks many chunks, so its LDS stage flushes only when full
+            // (one reservation per ~EMIT_WAVE_STAGE pairs)
+            uint64_t chunks = (seg_count + 63) / 64 + SEG_PARTS;
+            uint64_t blocks = (chunks + EMIT_THREADS / 64 - 1) / (EMIT_THREADS / 64);
+            if (blocks > ctx->emit_blocks)
+                blocks = ctx->emit_blocks;
+            if (blocks == 0)
+                blocks = 1;
+            hipLaunchKernelGGL(k_emit_segments, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, ea);
             VDBR_CHECK(ctx, hipGetLastError());
             VDBR_CHECK(ctx, hipEventRecord(ev[9], st));
             VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, counter, 8, hipMemcpyDeviceToHost, st));
