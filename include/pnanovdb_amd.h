@@ -385,6 +385,30 @@ PNANOVDB_API int vdbr_node2_to_onindex(vdbr_context_t* ctx, const void* node2_de
 PNANOVDB_API int vdbr_selftest_alpha8(vdbr_context_t* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches,
                                       uint64_t* fast_decided);
 
+/* One autotuner measurement: onesweep tile shape (variant, threads per block, keys per thread),
+ * digit width in bits, and the best of two full sorts in milliseconds. */
+typedef struct vdbr_tune_result_t
+{
+    uint32_t variant;
+    uint32_t threads;
+    uint32_t keys_per_thread;
+    uint32_t digit_bits;
+    float ms;
+} vdbr_tune_result_t;
+
+/* Sort autotuner (no counterpart in the reference, whose radix sort has fixed shapes): times
+ * every tile shape x digit width on key_count hashed keys of key_bits bits, keeps the fastest
+ * for sorts of key_count keys or more on this context (every configuration produces the same
+ * stable order), and reports all measurements (up to cap) and the choice. */
+PNANOVDB_API int vdbr_autotune_sort(vdbr_context_t* ctx, uint64_t key_count, uint32_t key_bits,
+                                    vdbr_tune_result_t* results, uint32_t cap, uint32_t* count,
+                                    uint32_t* best_variant, uint32_t* best_digit_bits);
+
+/* Fixes the sort configuration for sorts of from_count keys or more (variant index as reported
+ * by vdbr_autotune_sort, digit width 6..8). */
+PNANOVDB_API int vdbr_set_sort_config(vdbr_context_t* ctx, uint64_t from_count, uint32_t variant,
+                                      uint32_t digit_bits);
+
 PNANOVDB_API int vdbr_inclusive_scan(vdbr_context_t* ctx, const void* in, void* out, uint64_t count, int op);
 
 #ifdef __cplusplus

@@ -63,12 +63,13 @@ int Context::sort_workspace(uint64_t n, bool with_vals, uint64_t tiles, bool aux
     out->keys_alt = (uint64_t*)ws(aux ? WS_AUX_ALT : WS_SORT_ALT, n * sizeof(uint64_t));
     out->vals_alt = with_vals ? (uint32_t*)ws(aux ? WS_AUX_VALS_ALT : WS_SORT_VALS_ALT, n * sizeof(uint32_t)) : nullptr;
     out->status = (uint64_t*)ws(WS_SORT_STATUS, tiles * 256 * sizeof(uint64_t));
-    uint8_t* small = (uint8_t*)ws(WS_SORT_SMALL, 8 * 256 * 8 * 2 + 64);
+    // histograms and digit starts of up to 16 passes, then 16 tile counters
+    uint8_t* small = (uint8_t*)ws(WS_SORT_SMALL, 16 * 256 * 8 * 2 + 64);
     if (!out->keys_alt || (with_vals && !out->vals_alt) || !out->status || !small)
         return -1;
     out->hist = (uint64_t*)small;
-    out->starts = (uint64_t*)(small + 8 * 256 * 8);
-    out->tile_counters = (uint32_t*)(small + 8 * 256 * 8 * 2);
+    out->starts = (uint64_t*)(small + 16 * 256 * 8);
+    out->tile_counters = (uint32_t*)(small + 16 * 256 * 8 * 2);
     sort_status_tiles = ws_size[WS_SORT_STATUS] / (256 * sizeof(uint64_t));
     return 0;
 }
@@ -905,6 +906,25 @@ PNANOVDB_API int vdbr_selftest_alpha8(vdbr_context_t* ctx, uint64_t count, uint6
     if (!c || !mismatches)
         return -1;
     return selftest_alpha8(c, count, seed, mismatches, fast_decided);
+}
+
+PNANOVDB_API int vdbr_autotune_sort(vdbr_context_t* ctx, uint64_t key_count, uint32_t key_bits,
+                                    vdbr_tune_result_t* results, uint32_t cap, uint32_t* count,
+                                    uint32_t* best_variant, uint32_t* best_digit_bits)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    return autotune_sort(c, key_count, key_bits, results, cap, count, best_variant, best_digit_bits);
+}
+
+PNANOVDB_API int vdbr_set_sort_config(vdbr_context_t* ctx, uint64_t from_count, uint32_t variant,
+                                      uint32_t digit_bits)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    return set_sort_config(c, from_count, variant, digit_bits);
 }
 
 PNANOVDB_API int vdbr_node2_to_onindex(vdbr_context_t* ctx, const void* node2_dev, uint64_t node2_size,

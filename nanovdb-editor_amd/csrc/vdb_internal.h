@@ -62,6 +62,7 @@ enum WsSlot
     WS_ONINDEX,
     WS_SEGS,
     WS_SEG_META,
+    WS_TUNE_KEYS,
     WS_BWD_ACC,
     WS_BWD_REC,
     WS_BWD_RECGID,
@@ -83,6 +84,14 @@ struct SortTiming
     float ms;
 };
 
+// onesweep configuration: tile shape variant (threads x keys per thread) and digit width
+struct SortConfig
+{
+    uint32_t variant;
+    uint32_t digit_bits;
+};
+constexpr uint32_t SORT_BUCKETS = 40; // per floor(log2(key count))
+
 struct Context
 {
     int device = 0;
@@ -92,7 +101,8 @@ struct Context
     void* ws_ptr[WS_COUNT] = {};
     size_t ws_size[WS_COUNT] = {};
     uint64_t* pinned = nullptr; // small host readback area (pinned)
-    uint32_t emit_blocks = 2048; // resident blocks of the persistent emission kernel (CUs x 8)
+    uint32_t emit_blocks = 2048; // resident blocks of the persistent emission kernel (CUs x 3)
+    SortConfig sort_tuned[SORT_BUCKETS] = {}; // autotuned sort configuration per size bucket
     uint32_t sort_epoch = 0;
     uint64_t sort_status_tiles = 0;
 
@@ -134,6 +144,10 @@ int scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, uint64_t n, bool i
 int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive);
 int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
                    SortTiming* timing);
+// sort autotuner (vdb_prims.hip)
+int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_t* results, uint32_t cap,
+                  uint32_t* count, uint32_t* best_variant, uint32_t* best_digit_bits);
+int set_sort_config(Context* ctx, uint64_t from_count, uint32_t variant, uint32_t digit_bits);
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
                           uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out);
 

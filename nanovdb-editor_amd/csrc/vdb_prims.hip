@@ -225,11 +225,14 @@ int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bo
 }
 
 // ============================================================================ radix sort
-constexpr int SORT_THREADS = 512;
-constexpr int SORT_ITEMS = 16;
-constexpr int SORT_WAVE_TILE = 64 * SORT_ITEMS;  // 1024
-constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS; // 8192
-constexpr int SORT_MAX_PASSES = 8;
+// onesweep tile shapes (threads per block x keys per thread); SORT_VARIANTS[0] is the default
+struct SortShape
+{
+    int threads, items;
+};
+constexpr SortShape SORT_VARIANTS[] = { { 512, 16 }, { 256, 16 }, { 1024, 8 }, { 256, 24 }, { 512, 12 } };
+constexpr uint32_t SORT_VARIANT_COUNT = sizeof(SORT_VARIANTS) / sizeof(SORT_VARIANTS[0]);
+constexpr int SORT_MAX_PASSES = 11; // 64 bits in digits of at least 6 bits
 
 // status word: [63:48] epoch, [47:46] flag (1 = tile aggregate, 2 = inclusive prefix), [45:0] count
 #define ST_AGG 1ull
@@ -239,26 +242,20 @@ __device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint64_t flag, uint6
     return ((uint64_t)epoch << 48) | (flag << 46) | (value & ((1ull << 46) - 1ull));
 }
 
-// digit of pass p covers key bits [bit_begin + 8p, min(bit_end, bit_begin + 8p + 8)); the same
-// mask is applied here and in k_sort_pass so the per-pass counts always agree.
+// digit of pass p covers key bits [bit_begin + W p, min(bit_end, bit_begin + W p + W)) for the
+// digit width W <= 8; the same mask is applied here and in k_sort_pass so the per-pass counts
+// always agree.
 // Each thread counts 16 consecutive keys and adds runs of equal digits at once: the leading
 // digits of neighbouring pair keys are mostly equal, so per-key LDS atomics would serialise.
 constexpr int HIST_RUN = 16;
 __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, uint64_t n, uint32_t bit_begin,
-                                                   uint32_t bit_count, unsigned long long* __restrict__ hist)
+                                                   uint32_t bit_count, uint32_t W, unsigned long long* __restrict__ hist)
 {
-    const uint32_t passes = (bit_count + 7u) / 8u;
+    const uint32_t passes = (bit_count + W - 1u) / W;
     __shared__ uint32_t h[SORT_MAX_PASSES][256];
     for (int p = 0; p < SORT_MAX_PASSES; p++)
         h[p][threadIdx.x] = 0u;
     __syncthreads();
-    uint32_t dmask[SORT_MAX_PASSES];
-#pragma unroll
-    for (int p = 0; p < SORT_MAX_PASSES; p++)
-    {
-        uint32_t rem = bit_count > 8u * p ? bit_count - 8u * p : 0u;
-        dmask[p] = rem >= 8u ? 255u : ((1u << rem) - 1u);
-    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * HIST_RUN;
     for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * HIST_RUN; c < n; c += stride)
     {
@@ -281,20 +278,19 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
             for (int r = 0; r < HIST_RUN; r++)
                 k[r] = (uint32_t)r < cnt ? keys[c + r] : 0ull;
         }
-#pragma unroll
-        for (int p = 0; p < SORT_MAX_PASSES; p++)
+        for (uint32_t p = 0; p < passes; p++)
         {
-            if ((uint32_t)p >= passes)
-                break;
-            const uint32_t shift = bit_begin + 8u * p;
-            uint32_t cur = (uint32_t)(k[0] >> shift) & dmask[p];
+            const uint32_t shift = bit_begin + W * p;
+            const uint32_t rem = bit_count - W * p;
+            const uint32_t dm = rem >= W ? ((1u << W) - 1u) : ((1u << rem) - 1u);
+            uint32_t cur = (uint32_t)(k[0] >> shift) & dm;
             uint32_t run = 1u;
 #pragma unroll
             for (int r = 1; r < HIST_RUN; r++)
             {
                 if ((uint32_t)r >= cnt)
                     break;
-                uint32_t d = (uint32_t)(k[r] >> shift) & dmask[p];
+                uint32_t d = (uint32_t)(k[r] >> shift) & dm;
                 if (d != cur)
                 {
                     atomicAdd(&h[p][cur], run);
@@ -330,7 +326,7 @@ __global__ __launch_bounds__(256) void k_sort_digit_starts(const unsigned long l
     }
 }
 
-template <bool HAS_VALS>
+template <int SORT_THREADS, int SORT_ITEMS, bool HAS_VALS>
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                             const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                             uint64_t n, uint32_t shift, uint32_t digit_bits,
@@ -339,6 +335,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
                                                             uint32_t epoch)
 {
     constexpr int WAVES = SORT_THREADS / 64;
+    constexpr int SORT_WAVE_TILE = 64 * SORT_ITEMS;
+    constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
+    static_assert(WAVES >= 4, "digit bookkeeping uses the first 256 threads");
     __shared__ uint32_t s_tile;
     __shared__ uint32_t wcnt[WAVES][256];
     __shared__ uint32_t s_hist_excl[256];
@@ -511,16 +510,70 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
     }
 }
 
+// chosen shape and digit width for a sort of n keys (autotuned per size bucket when available)
+static uint32_t size_bucket(uint64_t n)
+{
+    uint32_t b = 0;
+    while (b < 63u && (2ull << b) <= n)
+        b++;
+    return b < SORT_BUCKETS ? b : SORT_BUCKETS - 1;
+}
+
+static SortConfig sort_config(const Context* ctx, uint64_t n)
+{
+    SortConfig c = ctx->sort_tuned[size_bucket(n)];
+    if (c.variant >= SORT_VARIANT_COUNT || c.digit_bits < 6u || c.digit_bits > 8u)
+        c = SortConfig{ 0u, 8u };
+    return c;
+}
+
+static uint64_t sort_tile(uint32_t variant)
+{
+    return (uint64_t)SORT_VARIANTS[variant].threads * (uint64_t)SORT_VARIANTS[variant].items;
+}
+
+template <bool V>
+static void launch_sort_pass(uint32_t variant, uint64_t tiles, hipStream_t st, const uint64_t* kin, uint64_t* kout,
+                             const uint32_t* vin, uint32_t* vout, uint64_t n, uint32_t shift, uint32_t dbits,
+                             const uint64_t* starts, uint64_t* status, uint32_t* counter, uint32_t epoch)
+{
+    switch (variant)
+    {
+    case 1:
+        hipLaunchKernelGGL((k_sort_pass<256, 16, V>), dim3((uint32_t)tiles), dim3(256), 0, st, kin, kout, vin, vout, n,
+                           shift, dbits, starts, status, counter, epoch);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_sort_pass<1024, 8, V>), dim3((uint32_t)tiles), dim3(1024), 0, st, kin, kout, vin, vout, n,
+                           shift, dbits, starts, status, counter, epoch);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_sort_pass<256, 24, V>), dim3((uint32_t)tiles), dim3(256), 0, st, kin, kout, vin, vout, n,
+                           shift, dbits, starts, status, counter, epoch);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_sort_pass<512, 12, V>), dim3((uint32_t)tiles), dim3(512), 0, st, kin, kout, vin, vout, n,
+                           shift, dbits, starts, status, counter, epoch);
+        break;
+    default:
+        hipLaunchKernelGGL((k_sort_pass<512, 16, V>), dim3((uint32_t)tiles), dim3(512), 0, st, kin, kout, vin, vout, n,
+                           shift, dbits, starts, status, counter, epoch);
+        break;
+    }
+}
+
 int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
                    SortTiming* timing)
 {
     if (n <= 1 || bit_end <= bit_begin)
         return 0;
     uint32_t bits = bit_end - bit_begin;
-    uint32_t passes = (bits + 7u) / 8u;
+    const SortConfig cfg = sort_config(ctx, n);
+    const uint32_t W = cfg.digit_bits;
+    uint32_t passes = (bits + W - 1u) / W;
     if (passes > SORT_MAX_PASSES)
         return ctx->fail("radix sort: more than 64 key bits");
-    uint64_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    uint64_t tiles = (n + sort_tile(cfg.variant) - 1) / sort_tile(cfg.variant);
     if (tiles > 0xFFFFFFFFull)
         return ctx->fail("radix sort: too many tiles");
     SortWorkspace ws;
@@ -531,7 +584,7 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
     uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
     if (hist_blocks > 2048u)
         hist_blocks = 2048u;
-    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits,
+    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits, W,
                        (unsigned long long*)ws.hist);
     hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
                        passes, ws.starts);
@@ -541,16 +594,15 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
     uint32_t* vout = ws.vals_alt;
     for (uint32_t p = 0; p < passes; p++)
     {
-        uint32_t shift = bit_begin + 8u * p;
-        uint32_t dbits = (bits - 8u * p) < 8u ? (bits - 8u * p) : 8u;
+        uint32_t shift = bit_begin + W * p;
+        uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
         uint32_t epoch = ctx->next_sort_epoch(tiles);
         if (vals)
-            hipLaunchKernelGGL(k_sort_pass<true>, dim3((uint32_t)tiles), dim3(SORT_THREADS), 0, ctx->stream, kin, kout,
-                               vin, vout, n, shift, dbits, ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
+            launch_sort_pass<true>(cfg.variant, tiles, ctx->stream, kin, kout, vin, vout, n, shift, dbits,
+                                   ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
         else
-            hipLaunchKernelGGL(k_sort_pass<false>, dim3((uint32_t)tiles), dim3(SORT_THREADS), 0, ctx->stream, kin, kout,
-                               nullptr, nullptr, n, shift, dbits, ws.starts + p * 256, ws.status, ws.tile_counters + p,
-                               epoch);
+            launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, n, shift, dbits,
+                                    ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;
@@ -611,10 +663,14 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
     if (n <= 1 || bit_end <= bit_begin)
         return 0;
     uint32_t bits = bit_end - bit_begin;
-    uint32_t passes = (bits + 7u) / 8u;
+    const SortConfig cfg = sort_config(ctx, n);
+    const uint32_t W = cfg.digit_bits;
+    uint32_t passes = (bits + W - 1u) / W;
     if (passes > SORT_MAX_PASSES)
         return ctx->fail("radix sort: more than 64 key bits");
-    uint64_t tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    uint64_t tiles = (n + sort_tile(cfg.variant) - 1) / sort_tile(cfg.variant);
+    if (tiles > 0xFFFFFFFFull)
+        return ctx->fail("radix sort: too many tiles");
     SortWorkspace ws;
     if (ctx->sort_workspace(n, false, tiles, false, &ws))
         return -1;
@@ -623,7 +679,7 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
     uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
     if (hist_blocks > 2048u)
         hist_blocks = 2048u;
-    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits,
+    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits, W,
                        (unsigned long long*)ws.hist);
     hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
                        passes, ws.starts);
@@ -633,12 +689,11 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
         VDBR_CHECK(ctx, hipEventRecord(passes_begin, ctx->stream));
     for (uint32_t p = 0; p < passes; p++)
     {
-        uint32_t shift = bit_begin + 8u * p;
-        uint32_t dbits = (bits - 8u * p) < 8u ? (bits - 8u * p) : 8u;
+        uint32_t shift = bit_begin + W * p;
+        uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
         uint32_t epoch = ctx->next_sort_epoch(tiles);
-        hipLaunchKernelGGL(k_sort_pass<false>, dim3((uint32_t)tiles), dim3(SORT_THREADS), 0, ctx->stream, kin, kout,
-                           nullptr, nullptr, n, shift, dbits, ws.starts + p * 256, ws.status, ws.tile_counters + p,
-                           epoch);
+        launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, n, shift, dbits,
+                                ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;
@@ -649,6 +704,117 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
         *passes_out = passes;
     *result_keys = kin;
     VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+// ============================================================================ sort autotuner
+// Times every tile shape x digit width on hashed keys of the requested size and bit count and
+// records the fastest for that size bucket and all larger ones. Candidate digit widths keep the
+// pass count of 8-bit digits or add one pass; every candidate produces the same sorted order.
+__global__ void k_tune_keys(uint64_t* __restrict__ keys, uint64_t n, uint32_t key_bits, uint64_t seed)
+{
+    const uint64_t mask = key_bits >= 64u ? ~0ull : ((1ull << key_bits) - 1ull);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        uint64_t x = (i + 1) * 0x9E3779B97F4A7C15ull ^ seed;
+        x ^= x >> 31;
+        x *= 0xBF58476D1CE4E5B9ull;
+        x ^= x >> 29;
+        keys[i] = x & mask;
+    }
+}
+
+int set_sort_config(Context* ctx, uint64_t from_count, uint32_t variant, uint32_t digit_bits)
+{
+    if (variant >= SORT_VARIANT_COUNT || digit_bits < 6u || digit_bits > 8u)
+        return ctx->fail("sort config: unknown variant or digit width outside [6, 8]");
+    for (uint32_t b = size_bucket(from_count); b < SORT_BUCKETS; b++)
+        ctx->sort_tuned[b] = SortConfig{ variant, digit_bits };
+    return 0;
+}
+
+int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_t* results, uint32_t cap,
+                  uint32_t* count, uint32_t* best_variant, uint32_t* best_digit_bits)
+{
+    if (n < 2 || key_bits == 0 || key_bits > 64)
+        return ctx->fail("autotune: need at least 2 keys and 1..64 key bits");
+    uint64_t* pristine = (uint64_t*)ctx->ws(WS_TUNE_KEYS, 2 * n * sizeof(uint64_t));
+    if (!pristine)
+        return -1;
+    uint64_t* work = pristine + n;
+    hipLaunchKernelGGL(k_tune_keys, dim3(2048), dim3(256), 0, ctx->stream, pristine, n, key_bits, 0x5DEECE66Dull);
+    VDBR_CHECK(ctx, hipGetLastError());
+    hipEvent_t e0, e1;
+    VDBR_CHECK(ctx, hipEventCreate(&e0));
+    VDBR_CHECK(ctx, hipEventCreate(&e1));
+    const uint32_t bucket = size_bucket(n);
+    const SortConfig saved = ctx->sort_tuned[bucket];
+    float best = 3.0e38f;
+    SortConfig best_cfg{ 0u, 8u };
+    uint32_t k = 0;
+    int rc = 0;
+    const uint32_t widths[2] = { 8u, 7u };
+    for (uint32_t v = 0; v < SORT_VARIANT_COUNT && rc == 0; v++)
+        for (uint32_t wi = 0; wi < 2 && rc == 0; wi++)
+        {
+            ctx->sort_tuned[bucket] = SortConfig{ v, widths[wi] };
+            float ms_best = 3.0e38f;
+            for (int rep = 0; rep < 2 && rc == 0; rep++)
+            {
+                uint64_t* out = nullptr;
+                if (hipMemcpyAsync(work, pristine, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream) !=
+                        hipSuccess ||
+                    hipEventRecord(e0, ctx->stream) != hipSuccess)
+                {
+                    rc = ctx->fail("autotune: copy or event failed");
+                    break;
+                }
+                if (radix_sort_u64_noswap(ctx, work, n, 0, key_bits, &out, nullptr, nullptr, nullptr))
+                {
+                    rc = -1;
+                    break;
+                }
+                float ms = 0.f;
+                if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                {
+                    rc = ctx->fail("autotune: timing failed");
+                    break;
+                }
+                ms_best = ms < ms_best ? ms : ms_best;
+            }
+            if (rc)
+                break;
+            if (results && k < cap)
+            {
+                results[k].variant = v;
+                results[k].threads = (uint32_t)SORT_VARIANTS[v].threads;
+                results[k].keys_per_thread = (uint32_t)SORT_VARIANTS[v].items;
+                results[k].digit_bits = widths[wi];
+                results[k].ms = ms_best;
+            }
+            k++;
+            if (ms_best < best)
+            {
+                best = ms_best;
+                best_cfg = ctx->sort_tuned[bucket];
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc)
+    {
+        ctx->sort_tuned[bucket] = saved;
+        return rc;
+    }
+    for (uint32_t b = bucket; b < SORT_BUCKETS; b++)
+        ctx->sort_tuned[b] = best_cfg;
+    if (count)
+        *count = k;
+    if (best_variant)
+        *best_variant = best_cfg.variant;
+    if (best_digit_bits)
+        *best_digit_bits = best_cfg.digit_bits;
     return 0;
 }
 

@@ -23,6 +23,10 @@ class VdbrGaussianGrads(ctypes.Structure):
     _fields_ = [("means", P), ("opacities", P), ("quats", P), ("scales", P)]
 
 
+class VdbrTuneResult(ctypes.Structure):
+    _fields_ = [("variant", u32), ("threads", u32), ("keys_per_thread", u32), ("digit_bits", u32), ("ms", f32)]
+
+
 class VdbrRasterParams(ctypes.Structure):
     _fields_ = [("voxel_size", f32), ("cull_scale", f32), ("value_mode", u32), ("flags", u32),
                 ("prim_batch_size", u32), ("ijk_batch_size", u32), ("max_node_count", u32), ("reserved", u32)]
@@ -71,6 +75,9 @@ SIGNATURES = {
     "vdbr_density_backward": ([P, ctypes.POINTER(VdbrGaussians), f32, P, u64, P, u64,
                                ctypes.POINTER(VdbrGaussianGrads)], ctypes.c_int),
     "vdbr_selftest_alpha8": ([P, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
+    "vdbr_autotune_sort": ([P, u64, u32, ctypes.POINTER(VdbrTuneResult), u32, ctypes.POINTER(u32),
+                            ctypes.POINTER(u32), ctypes.POINTER(u32)], ctypes.c_int),
+    "vdbr_set_sort_config": ([P, u64, u32, u32], ctypes.c_int),
     "vdbr_node2_to_onindex": ([P, P, u64, ctypes.POINTER(P), ctypes.POINTER(u64), ctypes.POINTER(u64)],
                               ctypes.c_int),
     "vdbr_radix_sort_u32": ([P, P, P, u64, u32], ctypes.c_int),

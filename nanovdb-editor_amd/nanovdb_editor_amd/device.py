@@ -7,7 +7,7 @@ import ctypes
 
 import torch
 
-from ._lib import VdbrGaussianGrads, VdbrGaussians, VdbrRasterParams, VdbrRasterStats, lib
+from ._lib import VdbrGaussianGrads, VdbrGaussians, VdbrRasterParams, VdbrRasterStats, VdbrTuneResult, lib
 
 VALUE_RGBA8 = 0
 VALUE_DENSITY = 1
@@ -143,6 +143,24 @@ class Voxelizer:
         if self._lib.vdbr_selftest_alpha8(self._ctx, int(count), int(seed), ctypes.byref(mism), ctypes.byref(fast)) != 0:
             self._err("vdbr_selftest_alpha8")
         return mism.value, fast.value
+
+    def autotune_sort(self, key_count, key_bits):
+        """Times every onesweep tile shape x digit width on key_count hashed keys and keeps the
+        fastest for sorts of that size or larger. Returns (choice dict, list of measurement dicts)."""
+        cap = 64
+        res = (VdbrTuneResult * cap)()
+        cnt, bv, bw = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        self._bind_stream()
+        if self._lib.vdbr_autotune_sort(self._ctx, int(key_count), int(key_bits), res, cap, ctypes.byref(cnt),
+                                        ctypes.byref(bv), ctypes.byref(bw)) != 0:
+            self._err("vdbr_autotune_sort")
+        rows = [{k: getattr(res[i], k) for k, _ in VdbrTuneResult._fields_} for i in range(min(cnt.value, cap))]
+        best = next(r for r in rows if r["variant"] == bv.value and r["digit_bits"] == bw.value)
+        return best, rows
+
+    def set_sort_config(self, from_count, variant, digit_bits):
+        if self._lib.vdbr_set_sort_config(self._ctx, int(from_count), int(variant), int(digit_bits)) != 0:
+            self._err("vdbr_set_sort_config")
 
     def last_pairs(self):
         """Sorted pair list of the last raster call: (ijk int32 [P,3], gid int32 [P]) on the GPU."""

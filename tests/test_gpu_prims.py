@@ -61,3 +61,48 @@ def test_scans(vox, n):
     assert np.array_equal(got, np.cumsum(b))
     got = vox.inclusive_scan(torch.from_numpy(a).cuda(), "max").cpu().numpy()
     assert np.array_equal(got, np.maximum.accumulate(a))
+
+
+@pytest.mark.parametrize("digit_bits", [6, 7, 8])
+def test_sort_every_shape(digit_bits):
+    """Every onesweep tile shape at every digit width gives the stable order."""
+    from nanovdb_editor_amd.device import Voxelizer
+    v = Voxelizer(0)
+    try:
+        rng = np.random.default_rng(digit_bits)
+        for variant in range(5):
+            v.set_sort_config(0, variant, digit_bits)
+            for n, bits in ((70_001, 45), (1_200_007, 64)):
+                keys = rng.integers(0, 2**63, n, dtype=np.int64).astype(np.uint64)
+                keys ^= rng.integers(0, 2, n, dtype=np.int64).astype(np.uint64) << np.uint64(63)
+                keys &= np.uint64(0xFFFFFFFFFFFFFFFF >> (64 - bits))
+                keys[::7] = keys[0]                      # long runs of equal keys exercise stability
+                vals = np.arange(n, dtype=np.int32)
+                kt = torch.from_numpy(keys.view(np.int64).copy()).cuda()
+                vt = torch.from_numpy(vals.copy()).cuda()
+                v.radix_sort_u64(kt, vt, 0, bits)
+                order = np.argsort(keys, kind="stable")
+                assert np.array_equal(kt.cpu().numpy().view(np.uint64), keys[order]), (variant, n)
+                assert np.array_equal(vt.cpu().numpy(), vals[order]), (variant, n)
+    finally:
+        v.close()
+
+
+def test_autotune_sort():
+    from nanovdb_editor_amd.device import Voxelizer
+    v = Voxelizer(0)
+    try:
+        best, rows = v.autotune_sort(1 << 20, 40)
+        assert len(rows) == 10
+        assert {(r["variant"], r["digit_bits"]) for r in rows} == {(i, w) for i in range(5) for w in (8, 7)}
+        assert best["ms"] == min(r["ms"] for r in rows) and best["ms"] > 0
+        # the chosen configuration sorts correctly
+        rng = np.random.default_rng(3)
+        keys = rng.integers(0, 2**40, 1_500_000, dtype=np.int64)
+        kt = torch.from_numpy(keys.copy()).cuda()
+        v.radix_sort_u64(kt, None, 0, 40)
+        assert np.array_equal(kt.cpu().numpy(), np.sort(keys))
+        with pytest.raises(RuntimeError):
+            v.set_sort_config(0, 9, 8)
+    finally:
+        v.close()
