@@ -33,6 +33,11 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=6_000,
                     help="Gaussians in the CPU-oracle baseline sample (0 disables it)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch-steps", type=int, default=2,
+                    help="timed calls of the 256-cloud batched entry point (0 skips it)")
+    ap.add_argument("--batch-workers", type=int, default=4)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default sort shape instead of autotuning it for the workload")
     ap.add_argument("--fwd-bwd-steps", type=int, default=3,
                     help="timed density forward+backward steps (torch.autograd path); 0 disables")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -113,6 +118,45 @@ def fwd_bwd(cloud, raw, vox, steps):
                     "log-scales, quaternions, opacity logits via torch.autograd; loss = sum(w * values)"}
 
 
+def batched(vox, steps, workers):
+    """The 256-cloud batch through vdbr_raster_node2_batch: the raw arrays are preprocessed in place
+    by an untimed first call, then timed calls rerun the preprocessed clouds."""
+    import numpy as np
+    import torch
+    from nanovdb_editor_amd import synthetic as syn
+    clouds = syn.workload_batch_256()
+    dev = torch.device("cuda", 0)
+    items = []
+    for c in clouds:
+        d = {k: torch.from_numpy(np.ascontiguousarray(a, np.float32).ravel()).to(dev)
+             for k, a in (("means", c.means), ("opacities", c.opacity_logits), ("quats", c.quats),
+                          ("scales", c.log_scales), ("sh0", c.sh0))}
+        d["colors"] = torch.empty_like(d["sh0"])
+        d.update(voxel_size=c.voxel_size, raw=True)
+        items.append(d)
+    vox.raster_node2_batch(items, workers=workers, copy_out=False)
+    for d in items:
+        d["raw"] = False
+    vox.raster_node2_batch(items, workers=workers, copy_out=False)
+    torch.cuda.synchronize()
+
+    def timed(w, k):
+        t0 = time.perf_counter()
+        for _ in range(k):
+            _, st = vox.raster_node2_batch(items, workers=w, copy_out=False)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / k, st
+
+    ms, st = timed(workers, steps)
+    ms_seq, _ = timed(1, 1)
+    n = sum(c.count for c in clouds)
+    return {"workload": "batch_256: 256 clouds, sizes log-uniform in [1e3, 5e5], uniform or 64-cluster "
+                        "(coin flip), voxel size U[0.004, 0.02], log-scales N(ln(1.5 vs), 0.4^2)",
+            "clouds": len(clouds), "gaussians": n, "pairs": sum(x["pair_count"] for x in st),
+            "voxels": sum(x["voxel_count"] for x in st), "workers": workers, "ms_per_batch": ms,
+            "gaussians_per_s": n / (ms * 1e-3), "ms_per_batch_one_worker": ms_seq, "steps": steps}
+
+
 def cpu_baseline(cloud, sample, threads):
     """The oracle (CPU restatement of the reference path) on a fixed subsample of the workload."""
     import numpy as np
@@ -163,6 +207,15 @@ def main():
                                  value_mode=mode, raw=True, sh0=work["sh0"], copy_out=False)
         return st
 
+    tuned = None
+    if args.autotune:
+        # one probe step gives the pair count and sorted bit range; the sort autotuner then picks the
+        # onesweep tile shape and digit width for sorts of that size (untimed, before the warmup)
+        probe = step()
+        torch.cuda.synchronize()
+        best, rows = vox.autotune_sort(probe["pair_count"], probe["key_bits"] - probe["sort_bit_begin"])
+        tuned = {"choice": best, "candidates": len(rows), "key_count": probe["pair_count"],
+                 "key_bits": probe["key_bits"] - probe["sort_bit_begin"]}
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -208,6 +261,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_sort_pass (onesweep radix pass over 64-bit pair keys)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_pass_ms": pass_ms, "algorithmic_bytes_per_launch": pass_bytes},
+        "sort_autotune": tuned,
         "pairs_per_s": P / (ms_per_step * 1e-3),
         "stage_ms": {k: avg(k) for k in ("ms_setup", "ms_emit", "ms_sort", "ms_reduce", "ms_build", "ms_total",
                                         "ms_sort_passes", "ms_emit_kernel", "ms_reduce_kernel")},
@@ -215,6 +269,8 @@ def main():
     }
     if args.fwd_bwd_steps > 0:
         out["differentiable"] = fwd_bwd(cloud, raw, vox, args.fwd_bwd_steps)
+    if args.batch_steps > 0 and default_workload:
+        out["batched"] = batched(vox, args.batch_steps, args.batch_workers)
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(cloud, args.cpu_sample, args.cpu_threads)
     else:

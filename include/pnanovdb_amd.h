@@ -332,6 +332,8 @@ typedef struct vdbr_raster_stats_t
     float ms_sort_passes;      /* time of the onesweep pass kernels only (HIP events) */
     float ms_emit_kernel;      /* time of the last emission kernel launch (HIP events) */
     float ms_reduce_kernel;    /* time of the per-voxel reduction kernel (HIP events) */
+    uint32_t sort_bit_begin;   /* the pair sort orders key bits [sort_bit_begin, key_bits) */
+    uint32_t reserved_stats;
 } vdbr_raster_stats_t;
 
 /* Full raster3d pipeline on device inputs. The node2 grid is written to a context-owned
@@ -340,6 +342,28 @@ typedef struct vdbr_raster_stats_t
 PNANOVDB_API int vdbr_raster_node2(vdbr_context_t* ctx, const vdbr_gaussians_t* gaussians,
                                    const vdbr_raster_params_t* params, void** grid_dev, uint64_t* grid_size,
                                    vdbr_raster_stats_t* stats);
+
+/* One cloud of a batched call: inputs and parameters as vdbr_raster_node2 takes them; the call
+ * fills grid_dev/grid_size (a node2 grid in context-owned device memory, valid until the next
+ * batched call on the same context) and stats. */
+typedef struct vdbr_batch_item_t
+{
+    vdbr_gaussians_t gaussians;
+    vdbr_raster_params_t params;
+    void* grid_dev;
+    uint64_t grid_size;
+    vdbr_raster_stats_t stats;
+} vdbr_batch_item_t;
+
+/* Batched raster3d (no counterpart in the reference, whose entry point takes one cloud): runs
+ * every cloud of items[count] through the full pipeline, each result identical to a
+ * vdbr_raster_node2 call on that cloud alone. Clouds are taken largest first by `workers` worker
+ * contexts (0 -> 4), each with its own stream and workspace, so the kernels of several clouds
+ * overlap on the device. Worker streams wait for the context's stream before the first cloud and
+ * the context's stream waits for every worker at the end. On failure the message names the
+ * first failing cloud. */
+PNANOVDB_API int vdbr_raster_node2_batch(vdbr_context_t* ctx, vdbr_batch_item_t* items, uint32_t count,
+                                         uint32_t workers);
 
 /* Adjoint of the density value mode (no counterpart in the reference, which has no backward
  * pass; the forward it differentiates is vdbr_raster_node2 with VDBR_VALUE_DENSITY):

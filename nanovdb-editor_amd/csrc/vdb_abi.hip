@@ -7,8 +7,11 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "vdb_internal.h"
@@ -147,12 +150,170 @@ static void destroy_context(Context* ctx)
     if (!ctx)
         return;
     (void)hipSetDevice(ctx->device);
+    for (Context* w : ctx->batch_workers)
+        destroy_context(w);
+    ctx->batch_workers.clear();
+    for (auto& c : ctx->batch_chunks)
+        (void)hipFree(c.first);
+    ctx->batch_chunks.clear();
     ctx->release();
     if (ctx->pinned)
         (void)hipHostFree(ctx->pinned);
     if (ctx->own_stream)
         (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
+}
+
+// ----------------------------------------------------------------------------- batched launch
+constexpr size_t BATCH_CHUNK_BYTES = size_t(1) << 30; // output arena grows in chunks of at least 1 GiB
+
+// Output arena: grids of one batched call are packed into chunks that persist on the context and
+// are reused by the next call (no device allocation once the chunks cover a batch).
+struct BatchArena
+{
+    Context* owner;
+    std::mutex lock;
+    size_t chunk = 0;  // current chunk
+    size_t offset = 0; // first free byte in it
+
+    void* reserve(size_t bytes, std::string* err)
+    {
+        bytes = (bytes + 255u) & ~size_t(255u);
+        std::lock_guard<std::mutex> g(lock);
+        while (chunk < owner->batch_chunks.size() && offset + bytes > owner->batch_chunks[chunk].second)
+        {
+            chunk++;
+            offset = 0;
+        }
+        if (chunk == owner->batch_chunks.size())
+        {
+            size_t sz = std::max(bytes, BATCH_CHUNK_BYTES);
+            void* p = nullptr;
+            hipError_t e = hipMalloc(&p, sz);
+            if (e != hipSuccess)
+            {
+                *err = std::string("batch output arena: hipMalloc: ") + hipGetErrorString(e);
+                return nullptr;
+            }
+            owner->batch_chunks.emplace_back(p, sz);
+            offset = 0;
+        }
+        void* p = (uint8_t*)owner->batch_chunks[chunk].first + offset;
+        offset += bytes;
+        return p;
+    }
+};
+
+static int raster_batch(Context* ctx, vdbr_batch_item_t* items, uint32_t count, uint32_t workers)
+{
+    if (count == 0)
+        return 0;
+    if (!items)
+        return ctx->fail("raster batch: items is NULL");
+    if (workers == 0)
+        workers = 4;
+    workers = std::min<uint32_t>(std::min<uint32_t>(workers, 16u), count);
+    while (ctx->batch_workers.size() < workers)
+    {
+        Context* w = create_context(ctx->device);
+        if (!w)
+            return ctx->fail("raster batch: could not create a worker context");
+        ctx->batch_workers.push_back(w);
+    }
+    // largest clouds first (longest-processing-time order balances the workers)
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; i++)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return items[a].gaussians.count > items[b].gaussians.count;
+    });
+    hipEvent_t ready;
+    VDBR_CHECK(ctx, hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    VDBR_CHECK(ctx, hipEventRecord(ready, ctx->stream));
+    for (uint32_t w = 0; w < workers; w++)
+    {
+        Context* wc = ctx->batch_workers[w];
+        std::copy(ctx->sort_tuned, ctx->sort_tuned + SORT_BUCKETS, wc->sort_tuned);
+        wc->stream = wc->own_stream;
+        if (hipStreamWaitEvent(wc->stream, ready, 0) != hipSuccess)
+        {
+            (void)hipEventDestroy(ready);
+            return ctx->fail("raster batch: hipStreamWaitEvent failed");
+        }
+    }
+    (void)hipEventDestroy(ready);
+
+    BatchArena arena;
+    arena.owner = ctx;
+    std::atomic<uint32_t> next{ 0 };
+    std::atomic<bool> failed{ false };
+    std::mutex err_lock;
+    std::string first_error;
+    uint32_t first_failed = ~0u;
+    auto work = [&](uint32_t w) {
+        Context* wc = ctx->batch_workers[w];
+        (void)hipSetDevice(wc->device);
+        while (!failed.load())
+        {
+            uint32_t slot = next.fetch_add(1);
+            if (slot >= count)
+                break;
+            uint32_t i = order[slot];
+            vdbr_batch_item_t& it = items[i];
+            void* grid = nullptr;
+            uint64_t size = 0;
+            std::string err;
+            if (raster_node2(wc, &it.gaussians, &it.params, &grid, &size, &it.stats) != 0)
+                err = wc->error;
+            else if (size > 0)
+            {
+                void* dst = arena.reserve(size, &err);
+                if (dst)
+                {
+                    hipError_t e = hipMemcpyAsync(dst, grid, size, hipMemcpyDeviceToDevice, wc->stream);
+                    if (e != hipSuccess)
+                        err = std::string("hipMemcpyAsync: ") + hipGetErrorString(e);
+                    it.grid_dev = dst;
+                    it.grid_size = size;
+                }
+            }
+            else
+            {
+                it.grid_dev = nullptr;
+                it.grid_size = 0;
+            }
+            if (!err.empty())
+            {
+                std::lock_guard<std::mutex> g(err_lock);
+                if (i < first_failed)
+                {
+                    first_failed = i;
+                    first_error = err;
+                }
+                failed.store(true);
+            }
+        }
+    };
+    std::vector<std::thread> threads;
+    for (uint32_t w = 1; w < workers; w++)
+        threads.emplace_back(work, w);
+    work(0);
+    for (auto& t : threads)
+        t.join();
+    // the caller's stream orders after every worker's last copy
+    int rc = 0;
+    for (uint32_t w = 0; w < workers; w++)
+    {
+        hipEvent_t done;
+        Context* wc = ctx->batch_workers[w];
+        if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess ||
+            hipEventRecord(done, wc->stream) != hipSuccess || hipStreamWaitEvent(ctx->stream, done, 0) != hipSuccess)
+            rc = ctx->fail("raster batch: could not order the caller's stream after the workers");
+        (void)hipEventDestroy(done);
+    }
+    if (failed.load())
+        return ctx->fail("raster batch: cloud " + std::to_string(first_failed) + ": " + first_error);
+    return rc;
 }
 
 // ----------------------------------------------------------------------------- default context
@@ -850,6 +1011,16 @@ PNANOVDB_API int vdbr_raster_node2(vdbr_context_t* ctx, const vdbr_gaussians_t* 
         return -1;
     (void)hipSetDevice(c->device);
     return raster_node2(c, gaussians, params, grid_dev, grid_size, stats);
+}
+
+PNANOVDB_API int vdbr_raster_node2_batch(vdbr_context_t* ctx, vdbr_batch_item_t* items, uint32_t count,
+                                         uint32_t workers)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    (void)hipSetDevice(c->device);
+    return raster_batch(c, items, count, workers);
 }
 
 PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap,

@@ -6,6 +6,8 @@
 #include <stdint.h>
 
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/pnanovdb_amd.h"
 
@@ -113,6 +115,10 @@ struct Context
     uint32_t last_alpha_bits = 0;
     int32_t last_lmin[3] = { 0, 0, 0 };
     uint32_t last_bits[3] = { 0, 0, 0 };
+
+    // batched launch: worker contexts (own streams and workspaces) and the output arena chunks
+    std::vector<Context*> batch_workers;
+    std::vector<std::pair<void*, size_t>> batch_chunks;
 
     int fail(const std::string& msg)
     {

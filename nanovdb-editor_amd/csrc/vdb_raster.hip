@@ -2572,6 +2572,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         stats->leaf_count = (uint32_t)nleaf;
         stats->window_count = (uint32_t)wtot;
         stats->key_bits = key_bits;
+        stats->sort_bit_begin = sort_begin;
         stats->emit_attempts = attempts;
         float ms;
         (void)hipEventElapsedTime(&ms, ev[0], ev[1]);

@@ -37,10 +37,16 @@ class VdbrRasterStats(ctypes.Structure):
                 ("grid_size", u64), ("upper_count", u32), ("lower_count", u32), ("leaf_count", u32),
                 ("window_count", u32), ("key_bits", u32), ("emit_attempts", u32), ("ms_setup", f32),
                 ("ms_emit", f32), ("ms_sort", f32), ("ms_reduce", f32), ("ms_build", f32), ("ms_total", f32),
-                ("sort_passes", u32), ("ms_sort_passes", f32), ("ms_emit_kernel", f32), ("ms_reduce_kernel", f32)]
+                ("sort_passes", u32), ("ms_sort_passes", f32), ("ms_emit_kernel", f32), ("ms_reduce_kernel", f32),
+                ("sort_bit_begin", u32), ("reserved_stats", u32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class VdbrBatchItem(ctypes.Structure):
+    _fields_ = [("gaussians", VdbrGaussians), ("params", VdbrRasterParams), ("grid_dev", P), ("grid_size", u64),
+                ("stats", VdbrRasterStats)]
 
 
 class VdbrBuffer(ctypes.Structure):
@@ -75,6 +81,7 @@ SIGNATURES = {
     "vdbr_density_backward": ([P, ctypes.POINTER(VdbrGaussians), f32, P, u64, P, u64,
                                ctypes.POINTER(VdbrGaussianGrads)], ctypes.c_int),
     "vdbr_selftest_alpha8": ([P, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
+    "vdbr_raster_node2_batch": ([P, ctypes.POINTER(VdbrBatchItem), u32, u32], ctypes.c_int),
     "vdbr_autotune_sort": ([P, u64, u32, ctypes.POINTER(VdbrTuneResult), u32, ctypes.POINTER(u32),
                             ctypes.POINTER(u32), ctypes.POINTER(u32)], ctypes.c_int),
     "vdbr_set_sort_config": ([P, u64, u32, u32], ctypes.c_int),

@@ -7,7 +7,8 @@ import ctypes
 
 import torch
 
-from ._lib import VdbrGaussianGrads, VdbrGaussians, VdbrRasterParams, VdbrRasterStats, VdbrTuneResult, lib
+from ._lib import (VdbrBatchItem, VdbrGaussianGrads, VdbrGaussians, VdbrRasterParams, VdbrRasterStats,
+                   VdbrTuneResult, lib)
 
 VALUE_RGBA8 = 0
 VALUE_DENSITY = 1
@@ -95,6 +96,43 @@ class Voxelizer:
             if size.value and self._lib.vdbr_copy(self._ctx, _ptr(out), grid_dev, size.value, 2) != 0:
                 self._err("vdbr_copy")
         return out, st.as_dict()
+
+    def raster_node2_batch(self, clouds, *, workers=4, copy_out=True):
+        """Many clouds in one call. ``clouds`` is a sequence of dicts with the tensors of
+        ``raster_node2`` (means, quats, scales, colors, opacities, and sh0 when raw) plus
+        ``voxel_size`` and optional ``raw``, ``value_mode``, ``cull_scale``. Each result equals a
+        ``raster_node2`` call on that cloud alone. Returns ``(grids, stats)``: uint8 GPU tensors
+        (or None without ``copy_out``) and stats dicts, in input order."""
+        items = (VdbrBatchItem * max(len(clouds), 1))()
+        keep = []
+        for i, c in enumerate(clouds):
+            n = c["means"].numel() // 3
+            raw = bool(c.get("raw", False))
+            for name, k in (("means", 3), ("quats", 4), ("scales", 3), ("colors", 3), ("opacities", 1)):
+                _check_f32(f"clouds[{i}].{name}", c[name], k * n)
+            if raw:
+                _check_f32(f"clouds[{i}].sh0", c["sh0"], 3 * n)
+            items[i].gaussians = VdbrGaussians(_ptr(c["means"]), _ptr(c["opacities"]), _ptr(c["quats"]),
+                                               _ptr(c["scales"]), _ptr(c["colors"]), _ptr(c.get("sh0")), n)
+            items[i].params = VdbrRasterParams(float(c["voxel_size"]), float(c.get("cull_scale", 0.0)),
+                                               int(c.get("value_mode", VALUE_RGBA8)),
+                                               FLAG_RAW_INPUTS if raw else 0, 0, 0, 0, 0)
+            keep.append(c)
+        self._bind_stream()
+        if self._lib.vdbr_raster_node2_batch(self._ctx, items, len(clouds), int(workers)) != 0:
+            self._err("vdbr_raster_node2_batch")
+        grids, stats = [], []
+        for i in range(len(clouds)):
+            stats.append(items[i].stats.as_dict())
+            if not copy_out:
+                grids.append(None)
+                continue
+            size = items[i].grid_size
+            out = torch.empty(size, dtype=torch.uint8, device=self.device)
+            if size and self._lib.vdbr_copy(self._ctx, _ptr(out), ctypes.c_void_p(items[i].grid_dev), size, 2) != 0:
+                self._err("vdbr_copy")
+            grids.append(out)
+        return grids, stats
 
     def density_backward(self, means, log_scales, quats, opacity_logits, voxel_size, grid, grad_values):
         """Gradients (means, log_scales, quats, opacity_logits) of sum(grad_values * values) where

@@ -82,19 +82,23 @@ def workload_clustered_16m(seed=5):
     return clustered_cloud(16_000_000, 256, 0.1, 0.002, -5.5, 0.6, seed)
 
 
-def workload_batch_256(seed=4, count=256):
-    """256 clouds, sizes log-uniform in [1k, 500k], coin flip uniform/clustered, voxel size
-    uniform in [0.004, 0.02]. Clustered clouds use 64 clusters (sigma 0.1) and log-scales
-    N(-5, 0.7^2); uniform clouds use log-scales N(-4, 0.5^2)."""
+def workload_batch_256(seed=4, count=256, max_size=500_000):
+    """256 clouds, sizes log-uniform in [1k, 500k] (capped at ``max_size``), coin flip uniform /
+    clustered (64 clusters, sigma 0.1), voxel size uniform in [0.004, 0.02]. Scale law, frozen:
+    log-scales N(ln(1.5 * voxel size), 0.4^2) per axis, so every cloud is resolved at about 1.5
+    voxels per standard deviation whatever its voxel size."""
     rng = np.random.default_rng(seed)
     sizes = np.round(np.exp(rng.uniform(np.log(1e3), np.log(5e5), count))).astype(np.int64)
+    sizes = np.minimum(sizes, max_size)
     kinds = rng.integers(0, 2, count)
     vss = rng.uniform(0.004, 0.02, count)
     clouds = []
     for i in range(count):
         s = int(rng.integers(0, 2**31))
+        vs = float(vss[i])
+        mean = float(np.log(1.5 * vs))
         if kinds[i] == 0:
-            clouds.append(uniform_cloud(int(sizes[i]), float(vss[i]), -4.0, 0.5, s))
+            clouds.append(uniform_cloud(int(sizes[i]), vs, mean, 0.4, s))
         else:
-            clouds.append(clustered_cloud(int(sizes[i]), 64, 0.1, float(vss[i]), -5.0, 0.7, s))
+            clouds.append(clustered_cloud(int(sizes[i]), 64, 0.1, vs, mean, 0.4, s))
     return clouds
