@@ -201,7 +201,7 @@ struct pnanovdb_raster_t
 
 /* nanovdb_editor/putil/ParallelPrimitives.h:22-88. Buffers are vdbr_buffer_t*. Scans are
  * inclusive; sorts are stable, ascending, LSD over the low key_bit_count bits; key_count 0 is
- * a no-op. radix_sort_dual_key is NULL (its only caller is the out-of-scope Raster2D path). */
+ * a no-op. radix_sort_dual_key orders by (high key, low key), each over its own bit count. */
 typedef struct pnanovdb_parallel_primitives_t
 {
     const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
@@ -226,7 +226,13 @@ typedef struct pnanovdb_parallel_primitives_t
                                    pnanovdb_compute_buffer_t* key_inout, pnanovdb_compute_buffer_t* val_inout,
                                    pnanovdb_uint64_t key_count, pnanovdb_uint64_t buffer_key_count,
                                    pnanovdb_uint32_t key_bit_count);
-    void* radix_sort_dual_key;
+    void(PNANOVDB_ABI* radix_sort_dual_key)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                            pnanovdb_parallel_primitives_context_t* context_in,
+                                            pnanovdb_compute_buffer_t* key_low_inout,
+                                            pnanovdb_compute_buffer_t* key_high_inout,
+                                            pnanovdb_compute_buffer_t* val_inout, pnanovdb_uint64_t key_count,
+                                            pnanovdb_uint64_t buffer_key_count, pnanovdb_uint32_t key_low_bit_count,
+                                            pnanovdb_uint32_t key_high_bit_count);
     void(PNANOVDB_ABI* radix_sort_key64)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
                                          pnanovdb_parallel_primitives_context_t* context_in,
                                          pnanovdb_compute_buffer_t* key_inout, pnanovdb_compute_buffer_t* val_inout,
@@ -392,7 +398,28 @@ PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32
 /* Stable LSD radix sort of 32-bit keys with optional 32-bit payload, in place. */
 PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32_t* vals, uint64_t count,
                                      uint32_t key_bit_count);
+/* Stable sort by (key_high over its low key_high_bit_count bits, then key_low over its low
+ * key_low_bit_count bits), both keys and the optional payload permuted in place
+ * (ParallelPrimitives.cpp:527-683, radix_sort_dual_key; bit counts clamp to 32). */
+PNANOVDB_API int vdbr_radix_sort_dual_key(vdbr_context_t* ctx, uint32_t* key_low, uint32_t* key_high, uint32_t* vals,
+                                          uint64_t count, uint32_t key_low_bit_count, uint32_t key_high_bit_count);
+/* Tile binning of projected Gaussians, the scan/sort stage of Raster2D (raster/Raster2D.cpp:347-535:
+ * gaussian_count_tiles.slang, global_scan, gaussian_tile_intersections.slang,
+ * radix_sort_dual_key(32, tile-id bits), gaussian_tile_offsets.slang). Device inputs: means2d[2N]
+ * (pixels), radii[N] (pixels; <= 0 culls), depths[N]. Tiles are tile_size pixels square over a
+ * width x height image (row-major tile index). Outputs: *count intersections; *keys_low (depth
+ * bits), *keys_high (tile index) and *vals (Gaussian index) point to context-owned device arrays
+ * sorted by (tile, depth bits) in Gaussian order within ties (valid until the next call);
+ * tile_offsets (optional caller device buffer of tiles_w * tiles_h entries) receives the first
+ * record of each tile (all 0 when there is no intersection). Float-to-tile conversions saturate
+ * (NaN and negatives to 0), as the GPUs' conversion instruction does. */
+PNANOVDB_API int vdbr_tile_bin(vdbr_context_t* ctx, const float* means2d, const int32_t* radii, const float* depths,
+                               uint32_t count_in, uint32_t image_width, uint32_t image_height, uint32_t tile_size,
+                               uint32_t* tile_offsets, uint32_t** keys_low, uint32_t** keys_high, uint32_t** vals,
+                               uint64_t* count);
 /* Inclusive scans (ParallelPrimitives.cpp:151-310): op 0 = add u32, 1 = add u64, 2 = max u32. */
+PNANOVDB_API int vdbr_inclusive_scan(vdbr_context_t* ctx, const void* in, void* out, uint64_t count, int op);
+
 /* putil/Convert.h:40-313 (pnanovdb_node2_convert_to_grid_type with PNANOVDB_GRID_TYPE_ONINDEX, as
  * called by RasterUtils.cpp:187-204): converts a node2 grid in device memory into a NanoVDB
  * OnIndex grid whose only blind data is the node2 value array. *onindex_dev points into the
@@ -432,8 +459,6 @@ PNANOVDB_API int vdbr_autotune_sort(vdbr_context_t* ctx, uint64_t key_count, uin
  * by vdbr_autotune_sort, digit width 6..8). */
 PNANOVDB_API int vdbr_set_sort_config(vdbr_context_t* ctx, uint64_t from_count, uint32_t variant,
                                       uint32_t digit_bits);
-
-PNANOVDB_API int vdbr_inclusive_scan(vdbr_context_t* ctx, const void* in, void* out, uint64_t count, int op);
 
 #ifdef __cplusplus
 }

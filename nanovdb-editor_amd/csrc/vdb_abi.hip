@@ -888,6 +888,26 @@ static void PNANOVDB_ABI pp_radix_sort_key64(const pnanovdb_compute_t* compute, 
     (void)hipStreamSynchronize(ctx->stream);
 }
 
+static void PNANOVDB_ABI pp_radix_sort_dual_key(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                                pnanovdb_parallel_primitives_context_t* context,
+                                                pnanovdb_compute_buffer_t* key_low_inout,
+                                                pnanovdb_compute_buffer_t* key_high_inout,
+                                                pnanovdb_compute_buffer_t* val_inout, pnanovdb_uint64_t key_count,
+                                                pnanovdb_uint64_t buffer_key_count, pnanovdb_uint32_t key_low_bit_count,
+                                                pnanovdb_uint32_t key_high_bit_count)
+{
+    (void)compute;
+    (void)queue;
+    (void)buffer_key_count;
+    Context* ctx = (Context*)context;
+    if (!ctx || key_count == 0)
+        return;
+    if (radix_sort_dual_key(ctx, (uint32_t*)buf_ptr(key_low_inout), (uint32_t*)buf_ptr(key_high_inout),
+                            (uint32_t*)buf_ptr(val_inout), key_count, key_low_bit_count, key_high_bit_count))
+        fprintf(stderr, "radix_sort_dual_key: %s\n", ctx->error.c_str());
+    (void)hipStreamSynchronize(ctx->stream);
+}
+
 static pnanovdb_parallel_primitives_t g_pp;
 static bool g_pp_init = false;
 
@@ -930,6 +950,7 @@ PNANOVDB_API pnanovdb_parallel_primitives_t* pnanovdb_get_parallel_primitives(vo
         g_pp.global_scan_uint64 = pp_global_scan_uint64;
         g_pp.global_scan_max = pp_global_scan_max;
         g_pp.radix_sort = pp_radix_sort;
+        g_pp.radix_sort_dual_key = pp_radix_sort_dual_key;
         g_pp.radix_sort_key64 = pp_radix_sort_key64;
         g_pp.compute = compute_table();
         g_pp_init = true;
@@ -1030,6 +1051,31 @@ PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t
     if (!c)
         return -1;
     return last_pairs(c, ijk_dev, gid_dev, cap, count);
+}
+
+PNANOVDB_API int vdbr_tile_bin(vdbr_context_t* ctx, const float* means2d, const int32_t* radii, const float* depths,
+                               uint32_t count_in, uint32_t image_width, uint32_t image_height, uint32_t tile_size,
+                               uint32_t* tile_offsets, uint32_t** keys_low, uint32_t** keys_high, uint32_t** vals,
+                               uint64_t* count)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    (void)hipSetDevice(c->device);
+    return tile_bin(c, means2d, radii, depths, count_in, image_width, image_height, tile_size, tile_offsets, keys_low,
+                    keys_high, vals, count);
+}
+
+PNANOVDB_API int vdbr_radix_sort_dual_key(vdbr_context_t* ctx, uint32_t* key_low, uint32_t* key_high, uint32_t* vals,
+                                          uint64_t count, uint32_t key_low_bit_count, uint32_t key_high_bit_count)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    if (radix_sort_dual_key(c, key_low, key_high, vals, count, key_low_bit_count, key_high_bit_count))
+        return -1;
+    VDBR_CHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
 }
 
 PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32_t* vals, uint64_t count,

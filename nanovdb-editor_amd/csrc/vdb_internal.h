@@ -65,6 +65,11 @@ enum WsSlot
     WS_SEGS,
     WS_SEG_META,
     WS_TUNE_KEYS,
+    WS_DUAL_WIDE,
+    WS_TILE_SCAN,
+    WS_TILE_KEYS,
+    WS_TILE_OUT,
+    WS_DUAL_TMP,
     WS_BWD_ACC,
     WS_BWD_REC,
     WS_BWD_RECGID,
@@ -150,6 +155,12 @@ int scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, uint64_t n, bool i
 int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive);
 int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
                    SortTiming* timing);
+int radix_sort_dual_key(Context* ctx, uint32_t* key_low, uint32_t* key_high, uint32_t* vals, uint64_t n,
+                        uint32_t low_bits, uint32_t high_bits);
+// vdb_tiles.hip
+int tile_bin(Context* ctx, const float* means2d, const int32_t* radii, const float* depths, uint32_t n,
+             uint32_t width, uint32_t height, uint32_t tile_size, uint32_t* offsets, uint32_t** keys_low_out,
+             uint32_t** keys_high_out, uint32_t** vals_out, uint64_t* count_out);
 // sort autotuner (vdb_prims.hip)
 int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_t* results, uint32_t cap,
                   uint32_t* count, uint32_t* best_variant, uint32_t* best_digit_bits);

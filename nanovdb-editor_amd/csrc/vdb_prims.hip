@@ -15,6 +15,8 @@
 // Tiles take their index from an atomic counter, so every tile a block waits on has already
 // been claimed by a running block.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 
 #include "vdb_internal.h"
@@ -648,6 +650,75 @@ int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, uin
     if (radix_sort_u64(ctx, wide, vals, n, 0, key_bits, nullptr))
         return -1;
     hipLaunchKernelGGL(k_narrow_u64, dim3(blocks), dim3(256), 0, ctx->stream, wide, n, keys);
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------- dual-key sort
+// Stable ascending sort by (key_high over its low high_bits bits, key_low over its low low_bits
+// bits) with both keys and the payload permuted (the order of ParallelPrimitives.cpp:527-683,
+// which sorts twice on 32-bit keys). Here one 64-bit composite key (high << low_bits | low) is
+// sorted once with the element index as payload, then the three arrays are gathered.
+__global__ void k_dual_pack(const uint32_t* __restrict__ lo, const uint32_t* __restrict__ hi, uint64_t n,
+                            uint32_t low_bits, uint32_t high_bits, uint64_t* __restrict__ wide, uint32_t* __restrict__ idx)
+{
+    const uint32_t ml = low_bits >= 32u ? 0xFFFFFFFFu : ((1u << low_bits) - 1u);
+    const uint32_t mh = high_bits >= 32u ? 0xFFFFFFFFu : ((1u << high_bits) - 1u);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        wide[i] = ((uint64_t)(hi[i] & mh) << low_bits) | (uint64_t)(lo[i] & ml);
+        idx[i] = (uint32_t)i;
+    }
+}
+
+__global__ void k_dual_gather(const uint32_t* __restrict__ idx, uint64_t n, const uint32_t* __restrict__ lo_src,
+                              const uint32_t* __restrict__ hi_src, const uint32_t* __restrict__ val_src,
+                              uint32_t* __restrict__ lo, uint32_t* __restrict__ hi, uint32_t* __restrict__ val)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        uint32_t s = idx[i];
+        lo[i] = lo_src[s];
+        hi[i] = hi_src[s];
+        if (val)
+            val[i] = val_src[s];
+    }
+}
+
+int radix_sort_dual_key(Context* ctx, uint32_t* key_low, uint32_t* key_high, uint32_t* vals, uint64_t n,
+                        uint32_t low_bits, uint32_t high_bits)
+{
+    if (n <= 1)
+        return 0;
+    if (n >= 0xFFFFFFFFull)
+        return ctx->fail("dual-key sort: more than 2^32-1 keys");
+    if (!key_low || !key_high)
+        return ctx->fail("dual-key sort: null key array");
+    low_bits = low_bits > 32u ? 32u : low_bits;
+    high_bits = high_bits > 32u ? 32u : high_bits;
+    if (low_bits + high_bits == 0u)
+        return 0;
+    uint64_t* wide = (uint64_t*)ctx->ws(WS_DUAL_WIDE, n * sizeof(uint64_t));
+    uint32_t* tmp = (uint32_t*)ctx->ws(WS_DUAL_TMP, 4 * n * sizeof(uint32_t));
+    if (!wide || !tmp)
+        return -1;
+    uint32_t* idx = tmp;
+    uint32_t* lo_src = tmp + n;
+    uint32_t* hi_src = tmp + 2 * n;
+    uint32_t* val_src = tmp + 3 * n;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_dual_pack, dim3(blocks), dim3(256), 0, ctx->stream, key_low, key_high, n, low_bits, high_bits,
+                       wide, idx);
+    VDBR_CHECK(ctx, hipGetLastError());
+    if (radix_sort_u64(ctx, wide, idx, n, 0, low_bits + high_bits, nullptr))
+        return -1;
+    VDBR_CHECK(ctx, hipMemcpyAsync(lo_src, key_low, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    VDBR_CHECK(ctx, hipMemcpyAsync(hi_src, key_high, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    if (vals)
+        VDBR_CHECK(ctx, hipMemcpyAsync(val_src, vals, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_dual_gather, dim3(blocks), dim3(256), 0, ctx->stream, (const uint32_t*)idx, n,
+                       (const uint32_t*)lo_src, (const uint32_t*)hi_src, (const uint32_t*)val_src, key_low, key_high,
+                       vals);
     VDBR_CHECK(ctx, hipGetLastError());
     return 0;
 }

@@ -228,6 +228,44 @@ class Voxelizer:
         if self._lib.vdbr_radix_sort_u32(self._ctx, _ptr(keys), _ptr(vals), keys.numel(), key_bit_count) != 0:
             self._err("vdbr_radix_sort_u32")
 
+    def tile_bin(self, means2d, radii, depths, width, height, tile_size=16):
+        """Raster2D tile binning (raster/Raster2D.cpp:347-535) of projected Gaussians: float32
+        means2d [N, 2] pixels, int32 radii [N], float32 depths [N] on the GPU. Returns int32 tensors
+        (keys_low = depth bits, keys_high = tile index, vals = Gaussian index) sorted by (tile,
+        depth), and the int32 tile offsets [tiles_h * tiles_w]."""
+        n = radii.numel()
+        _check_f32("means2d", means2d, 2 * n)
+        _check_f32("depths", depths, n)
+        if radii.dtype != torch.int32 or not radii.is_cuda or not radii.is_contiguous():
+            raise ValueError("radii must be a contiguous int32 GPU tensor")
+        tiles = ((width + tile_size - 1) // tile_size) * ((height + tile_size - 1) // tile_size)
+        offsets = torch.empty(tiles, dtype=torch.int32, device=self.device)
+        lo, hi, va = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        cnt = ctypes.c_uint64()
+        self._bind_stream()
+        if self._lib.vdbr_tile_bin(self._ctx, _ptr(means2d), _ptr(radii), _ptr(depths), n, int(width), int(height),
+                                   int(tile_size), _ptr(offsets), ctypes.byref(lo), ctypes.byref(hi), ctypes.byref(va),
+                                   ctypes.byref(cnt)) != 0:
+            self._err("vdbr_tile_bin")
+        outs = []
+        for src in (lo, hi, va):
+            t = torch.empty(cnt.value, dtype=torch.int32, device=self.device)
+            if cnt.value and self._lib.vdbr_copy(self._ctx, _ptr(t), src, 4 * cnt.value, 2) != 0:
+                self._err("vdbr_copy")
+            outs.append(t)
+        return outs[0], outs[1], outs[2], offsets
+
+    def radix_sort_dual_key(self, key_low, key_high, vals=None, low_bits=32, high_bits=32):
+        """Stable in-place sort of int32 tensors by (key_high, key_low) over the given low bits."""
+        for name, t in (("key_low", key_low), ("key_high", key_high), ("vals", vals)):
+            if t is not None and (t.dtype != torch.int32 or not t.is_cuda or not t.is_contiguous()
+                                  or t.numel() != key_low.numel()):
+                raise ValueError(f"{name} must be a contiguous int32 GPU tensor of the key count")
+        self._bind_stream()
+        if self._lib.vdbr_radix_sort_dual_key(self._ctx, _ptr(key_low), _ptr(key_high), _ptr(vals), key_low.numel(),
+                                              int(low_bits), int(high_bits)) != 0:
+            self._err("vdbr_radix_sort_dual_key")
+
     def inclusive_scan(self, values, op="add"):
         """Inclusive scan: op 'add' (int32 as u32 or int64 as u64) or 'max' (int32 as u32)."""
         if not values.is_cuda or not values.is_contiguous():

@@ -65,6 +65,8 @@ def lib():
         L.vdbo_node2_active_voxels.argtypes = [P, u64, P, P, u64]
         L.vdbo_node2_active_voxels.restype = u64
         L.vdbo_node2_to_onindex.argtypes = [P, u64, P, u64]
+        L.vdbo_tile_bin.argtypes = [u32, P, P, P, u32, u32, u32, P, P, P, u64, P]
+        L.vdbo_tile_bin.restype = u64
         L.vdbo_node2_to_onindex.restype = u64
         _LIB = L
     return _LIB
@@ -181,3 +183,23 @@ def node2_to_onindex(grid):
     out = np.zeros(size, np.uint8)
     L.vdbo_node2_to_onindex(_p(grid), grid.size, _p(out), size)
     return out
+
+
+def tile_bin(means2d, radii, depths, width, height, tile_size):
+    """Raster2D tile binning (Raster2D.cpp:347-535): sorted (keys_low = depth bits, keys_high =
+    tile index, vals = Gaussian index) and the tile offsets."""
+    means2d = np.ascontiguousarray(means2d, np.float32)
+    radii = np.ascontiguousarray(radii, np.int32)
+    depths = np.ascontiguousarray(depths, np.float32)
+    n = radii.size
+    tiles = ((width + tile_size - 1) // tile_size) * ((height + tile_size - 1) // tile_size)
+    L = lib()
+    offsets = np.zeros(tiles, np.uint32)
+    total = L.vdbo_tile_bin(n, _p(means2d), _p(radii), _p(depths), width, height, tile_size, None, None, None, 0,
+                            None)
+    lo = np.zeros(total, np.uint32)
+    hi = np.zeros(total, np.uint32)
+    va = np.zeros(total, np.uint32)
+    L.vdbo_tile_bin(n, _p(means2d), _p(radii), _p(depths), width, height, tile_size, _p(lo), _p(hi), _p(va), total,
+                    _p(offsets))
+    return lo, hi, va, offsets

@@ -29,6 +29,13 @@
  *                          tree) is restated from the public format: parity unpinned.
  *   vdbo_density_f64       pinned: single axis-aligned Gaussian closed form.
  *   vdbo_density_grad_f64  pinned: FP64 central finite differences.
+ *   vdbo_node2_active_voxels / vdbo_node2_to_onindex
+ *                          pinned: the OnIndex grid's active voxels and value indices equal
+ *                          the node2 grid's (the Convert.h validation); OnIndex header bytes
+ *                          are restated from the public NanoVDB format: parity unpinned.
+ *   vdbo_tile_bin          pinned: intersections equal a brute-force half-open interval test
+ *                          per tile, the order equals a stable sort by (tile, depth bits), and
+ *                          the offsets equal a left binary search of every tile index.
  */
 #include "vdb_oracle.h"
 
@@ -1552,6 +1559,128 @@ uint64_t vdbo_node2_to_onindex(const uint8_t* grid, uint64_t grid_size, uint8_t*
     if (out && cap >= end)
         memcpy(out, d.w.data(), end);
     return end;
+}
+
+} // extern "C"
+
+// ============================================================================ Raster2D tile binning
+namespace
+{
+// float -> u32 as the GPUs' conversion instruction does it (NaN and negatives to 0, saturating
+// at 2^32-1); the shaders write (uint32_t)floor(x), which C leaves undefined below zero
+uint32_t sat_u32(float x)
+{
+    if (!(x > 0.f))
+        return 0u;
+    if (x >= 4294967295.f)
+        return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+struct TileBox
+{
+    uint32_t x0, y0, x1, y1;
+};
+
+// gaussian_count_tiles.slang:27-44 (the same box as gaussian_tile_intersections.slang:30-47)
+TileBox tile_box(float mx, float my, int32_t radius, uint32_t tile_size, uint32_t tiles_w, uint32_t tiles_h)
+{
+    float radiusf = (float)radius;
+    float tile_radius = radiusf / (float)tile_size;
+    float u = mx / (float)tile_size;
+    float v = my / (float)tile_size;
+    TileBox b;
+    b.x0 = std::min(sat_u32(std::floor(u - tile_radius)), tiles_w);
+    b.y0 = std::min(sat_u32(std::floor(v - tile_radius)), tiles_h);
+    b.x1 = std::min(sat_u32(std::ceil(u + tile_radius)), tiles_w);
+    b.y1 = std::min(sat_u32(std::ceil(v + tile_radius)), tiles_h);
+    return b;
+}
+} // namespace
+
+extern "C" {
+
+uint64_t vdbo_tile_bin(uint32_t n, const float* means2d, const int32_t* radii, const float* depths, uint32_t width,
+                       uint32_t height, uint32_t tile_size, uint32_t* keys_low, uint32_t* keys_high, uint32_t* vals,
+                       uint64_t cap, uint32_t* tile_offsets)
+{
+    const uint32_t tiles_w = (width + tile_size - 1u) / tile_size; // Raster2D.cpp:202-205
+    const uint32_t tiles_h = (height + tile_size - 1u) / tile_size;
+    const uint32_t num_tiles = tiles_w * tiles_h;
+    // counts and their inclusive scan (Raster2D.cpp:347-366)
+    std::vector<uint32_t> scan(n);
+    uint32_t run = 0;
+    for (uint32_t g = 0; g < n; g++)
+    {
+        uint32_t c = 0;
+        if (radii[g] > 0)
+        {
+            TileBox b = tile_box(means2d[2 * g], means2d[2 * g + 1], radii[g], tile_size, tiles_w, tiles_h);
+            c = (b.y1 - b.y0) * (b.x1 - b.x0);
+        }
+        run += c;
+        scan[g] = run;
+    }
+    const uint64_t total = n ? scan[n - 1] : 0;
+    if (tile_offsets)
+        std::fill(tile_offsets, tile_offsets + num_tiles, 0u);
+    if (total > cap || !keys_low || !keys_high || !vals)
+        return total;
+    // intersections in Gaussian-major, row-major tile order (gaussian_tile_intersections.slang:49-64)
+    std::vector<uint32_t> lo(total), hi(total), va(total);
+    for (uint32_t g = 0; g < n; g++)
+    {
+        if (radii[g] <= 0)
+            continue;
+        TileBox b = tile_box(means2d[2 * g], means2d[2 * g + 1], radii[g], tile_size, tiles_w, tiles_h);
+        uint32_t depth_bits;
+        std::memcpy(&depth_bits, &depths[g], 4);
+        uint32_t cur = g ? scan[g - 1] : 0u;
+        for (uint32_t i = b.y0; i < b.y1; i++)
+            for (uint32_t j = b.x0; j < b.x1; j++)
+            {
+                lo[cur] = depth_bits;
+                hi[cur] = i * tiles_w + j;
+                va[cur] = g;
+                cur++;
+            }
+    }
+    // radix_sort_dual_key(low 32 bits, high tile-id bits): stable by (high, low) (Raster2D.cpp:495-507)
+    uint32_t tile_bits = 0;
+    while (tile_bits < 32 && (1u << tile_bits) < num_tiles)
+        tile_bits++;
+    const uint32_t mh = tile_bits >= 32 ? 0xFFFFFFFFu : ((1u << tile_bits) - 1u);
+    std::vector<uint32_t> order(total);
+    for (uint64_t e = 0; e < total; e++)
+        order[e] = (uint32_t)e;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        uint32_t ha = hi[a] & mh, hb = hi[b] & mh;
+        return ha != hb ? ha < hb : lo[a] < lo[b];
+    });
+    for (uint64_t e = 0; e < total; e++)
+    {
+        keys_low[e] = lo[order[e]];
+        keys_high[e] = hi[order[e]];
+        vals[e] = va[order[e]];
+    }
+    // tile offsets (gaussian_tile_offsets.slang:24-56); no intersections leaves every offset 0
+    if (tile_offsets)
+    {
+        for (uint64_t e = 0; e < total; e++)
+        {
+            uint32_t t = keys_high[e];
+            if (e == 0)
+                for (uint32_t i = 0; i < t + 1u && i < num_tiles; i++)
+                    tile_offsets[i] = 0u;
+            if (e == total - 1)
+                for (uint32_t i = t + 1u; i < num_tiles; i++)
+                    tile_offsets[i] = (uint32_t)total;
+            if (e > 0 && keys_high[e - 1] != t)
+                for (uint32_t i = keys_high[e - 1] + 1u; i < t + 1u && i < num_tiles; i++)
+                    tile_offsets[i] = (uint32_t)e;
+        }
+    }
+    return total;
 }
 
 } // extern "C"

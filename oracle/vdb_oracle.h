@@ -81,6 +81,15 @@ uint64_t vdbo_node2_active_voxels(const uint8_t* grid, uint64_t grid_size, int32
  * Returns the OnIndex size in bytes; writes it to out when cap >= size. */
 uint64_t vdbo_node2_to_onindex(const uint8_t* grid, uint64_t grid_size, uint8_t* out, uint64_t cap);
 
+/* Raster2D tile binning (raster/Raster2D.cpp:347-535): per-Gaussian tile counts
+ * (gaussian_count_tiles.slang:14-46), inclusive scan, intersections (gaussian_tile_intersections.slang:18-65),
+ * radix_sort_dual_key(32, tile-id bits) (Raster2D.cpp:495-507, ParallelPrimitives.cpp:527-683) and
+ * tile offsets (gaussian_tile_offsets.slang:13-56). Returns the intersection count; fills the
+ * sorted keys/vals when cap >= count and tile_offsets[num_tiles] when not NULL. */
+uint64_t vdbo_tile_bin(uint32_t n, const float* means2d, const int32_t* radii, const float* depths, uint32_t width,
+                       uint32_t height, uint32_t tile_size, uint32_t* keys_low, uint32_t* keys_high, uint32_t* vals,
+                       uint64_t cap, uint32_t* tile_offsets);
+
 #ifdef __cplusplus
 }
 #endif

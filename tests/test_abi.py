@@ -115,6 +115,10 @@ def test_function_tables_populated():
     comp = ctypes.cast(L.pnanovdb_get_compute(), ctypes.POINTER(pnanovdb_Compute)).contents
     for f in ("create_array", "destroy_array", "map_array", "unmap_array", "duplicate_array"):
         assert ctypes.cast(getattr(comp, f), ctypes.c_void_p).value, f
+    from nanovdb_editor_amd._lib import PnanovdbParallelPrimitives
+    pp = ctypes.cast(L.pnanovdb_get_parallel_primitives(), ctypes.POINTER(PnanovdbParallelPrimitives)).contents
+    for f, _ in PnanovdbParallelPrimitives._fields_[1:]:
+        assert pp.__getattribute__(f), f
     # the getters hand out the same writable static each time
     assert L.pnanovdb_get_raster() == L.pnanovdb_get_raster()
 
