@@ -398,6 +398,18 @@ PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32
 /* Stable LSD radix sort of 32-bit keys with optional 32-bit payload, in place. */
 PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32_t* vals, uint64_t count,
                                      uint32_t key_bit_count);
+/* VoxelBVH Gaussian keys (raster/VoxelBVH.cpp:1225-1435, ijkl_from_gaussians): preprocessed device
+ * arrays (means, opacities, quats (w,x,y,z), scales; colors and sh0 unused) -> world box of the
+ * Gaussians' bounding cubes (world_bbox[6]: min xyz, max xyz), up to 8 cell keys per Gaussian
+ * ((i << 48) | (j << 32) | (k << 16) | level, unused slots ~0) sorted ascending into
+ * ijkl[8 N] with the Gaussian index (unused ~0) in prim_ids[8 N], and the [start, end) key index
+ * of every run of equal keys in ranges[2 * 8 N] (entries past the run count are 0). All output
+ * buffers are caller device memory. resolution is the voxel count along the box's longest axis
+ * (1..4096, VoxelBVH.cpp:30-33); transform (optional, 16 floats, rows of a 4x4 matrix applied as
+ * (p, 1) * M) maps Gaussian space to world space as the reference's transform_floats. */
+PNANOVDB_API int vdbr_voxelbvh_ijkl(vdbr_context_t* ctx, const vdbr_gaussians_t* gaussians, uint32_t resolution,
+                                    const float* transform, uint64_t* ijkl, uint32_t* prim_ids, uint32_t* ranges,
+                                    float* world_bbox);
 /* Stable sort by (key_high over its low key_high_bit_count bits, then key_low over its low
  * key_low_bit_count bits), both keys and the optional payload permuted in place
  * (ParallelPrimitives.cpp:527-683, radix_sort_dual_key; bit counts clamp to 32). */

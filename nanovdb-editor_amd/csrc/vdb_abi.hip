@@ -1053,6 +1053,17 @@ PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t
     return last_pairs(c, ijk_dev, gid_dev, cap, count);
 }
 
+PNANOVDB_API int vdbr_voxelbvh_ijkl(vdbr_context_t* ctx, const vdbr_gaussians_t* gaussians, uint32_t resolution,
+                                    const float* transform, uint64_t* ijkl, uint32_t* prim_ids, uint32_t* ranges,
+                                    float* world_bbox)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    (void)hipSetDevice(c->device);
+    return voxelbvh_ijkl(c, gaussians, resolution, transform, ijkl, prim_ids, ranges, world_bbox);
+}
+
 PNANOVDB_API int vdbr_tile_bin(vdbr_context_t* ctx, const float* means2d, const int32_t* radii, const float* depths,
                                uint32_t count_in, uint32_t image_width, uint32_t image_height, uint32_t tile_size,
                                uint32_t* tile_offsets, uint32_t** keys_low, uint32_t** keys_high, uint32_t** vals,

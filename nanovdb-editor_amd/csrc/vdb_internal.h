@@ -67,6 +67,8 @@ enum WsSlot
     WS_TUNE_KEYS,
     WS_DUAL_WIDE,
     WS_TILE_SCAN,
+    WS_BVH_PARTIAL,
+    WS_BVH_STARTS,
     WS_TILE_KEYS,
     WS_TILE_OUT,
     WS_DUAL_TMP,
@@ -157,6 +159,9 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
                    SortTiming* timing);
 int radix_sort_dual_key(Context* ctx, uint32_t* key_low, uint32_t* key_high, uint32_t* vals, uint64_t n,
                         uint32_t low_bits, uint32_t high_bits);
+// vdb_bvh.hip
+int voxelbvh_ijkl(Context* ctx, const vdbr_gaussians_t* gs, uint32_t resolution, const float* transform,
+                  uint64_t* ijkl, uint32_t* prim_ids, uint32_t* ranges, float* world_bbox);
 // vdb_tiles.hip
 int tile_bin(Context* ctx, const float* means2d, const int32_t* radii, const float* depths, uint32_t n,
              uint32_t width, uint32_t height, uint32_t tile_size, uint32_t* offsets, uint32_t** keys_low_out,

@@ -81,6 +81,7 @@ SIGNATURES = {
     "vdbr_density_backward": ([P, ctypes.POINTER(VdbrGaussians), f32, P, u64, P, u64,
                                ctypes.POINTER(VdbrGaussianGrads)], ctypes.c_int),
     "vdbr_selftest_alpha8": ([P, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
+    "vdbr_voxelbvh_ijkl": ([P, ctypes.POINTER(VdbrGaussians), u32, P, P, P, P, P], ctypes.c_int),
     "vdbr_tile_bin": ([P, P, P, P, u32, u32, u32, u32, P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P),
                        ctypes.POINTER(u64)], ctypes.c_int),
     "vdbr_radix_sort_dual_key": ([P, P, P, P, u64, u32, u32], ctypes.c_int),

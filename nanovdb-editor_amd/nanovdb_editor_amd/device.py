@@ -228,6 +228,30 @@ class Voxelizer:
         if self._lib.vdbr_radix_sort_u32(self._ctx, _ptr(keys), _ptr(vals), keys.numel(), key_bit_count) != 0:
             self._err("vdbr_radix_sort_u32")
 
+    def voxelbvh_ijkl(self, means, opacities, quats, scales, resolution, transform=None):
+        """VoxelBVH Gaussian keys (raster/VoxelBVH.cpp:1225-1435) of preprocessed float32 GPU
+        arrays (opacities after the sigmoid, normalised (w,x,y,z) quats, scales after exp).
+        Returns (keys int64 [8N] sorted, Gaussian ids int32 [8N], run headers int32 [8N, 2],
+        world box float32 [6]); unused key slots hold -1 (all bits set)."""
+        n = opacities.numel()
+        _check_f32("means", means, 3 * n)
+        _check_f32("opacities", opacities, n)
+        _check_f32("quats", quats, 4 * n)
+        _check_f32("scales", scales, 3 * n)
+        keys = torch.empty(8 * n, dtype=torch.int64, device=self.device)
+        ids = torch.empty(8 * n, dtype=torch.int32, device=self.device)
+        ranges = torch.empty((8 * n, 2), dtype=torch.int32, device=self.device)
+        bbox = torch.empty(6, dtype=torch.float32, device=self.device)
+        xf = None
+        if transform is not None:
+            xf = (ctypes.c_float * 16)(*[float(v) for v in torch.as_tensor(transform).reshape(-1).tolist()])
+        g = VdbrGaussians(_ptr(means), _ptr(opacities), _ptr(quats), _ptr(scales), None, None, n)
+        self._bind_stream()
+        if self._lib.vdbr_voxelbvh_ijkl(self._ctx, ctypes.byref(g), int(resolution), xf, _ptr(keys), _ptr(ids),
+                                        _ptr(ranges), _ptr(bbox)) != 0:
+            self._err("vdbr_voxelbvh_ijkl")
+        return keys, ids, ranges, bbox
+
     def tile_bin(self, means2d, radii, depths, width, height, tile_size=16):
         """Raster2D tile binning (raster/Raster2D.cpp:347-535) of projected Gaussians: float32
         means2d [N, 2] pixels, int32 radii [N], float32 depths [N] on the GPU. Returns int32 tensors

@@ -67,6 +67,9 @@ def lib():
         L.vdbo_node2_to_onindex.argtypes = [P, u64, P, u64]
         L.vdbo_tile_bin.argtypes = [u32, P, P, P, u32, u32, u32, P, P, P, u64, P]
         L.vdbo_tile_bin.restype = u64
+        L.vdbo_voxelbvh_ijkl.argtypes = [u64, P, P, P, P, u32, P, P, P, P, P]
+        L.vdbo_voxelbvh_ijkl.restype = i32
+        L.vdbo_mat_inverse.argtypes = [P, P]
         L.vdbo_node2_to_onindex.restype = u64
         _LIB = L
     return _LIB
@@ -203,3 +206,28 @@ def tile_bin(means2d, radii, depths, width, height, tile_size):
     L.vdbo_tile_bin(n, _p(means2d), _p(radii), _p(depths), width, height, tile_size, _p(lo), _p(hi), _p(va), total,
                     _p(offsets))
     return lo, hi, va, offsets
+
+
+def voxelbvh_ijkl(means, opacities, quats, scales, resolution, transform=None):
+    """VoxelBVH Gaussian keys (VoxelBVH.cpp:1225-1435) on preprocessed arrays: sorted keys [8N]
+    (uint64), Gaussian ids [8N], run headers [8N, 2] and the world box [6]."""
+    means, opacities, quats, scales = (np.ascontiguousarray(a, np.float32) for a in (means, opacities, quats, scales))
+    n = opacities.size
+    keys = np.zeros(8 * n, np.uint64)
+    ids = np.zeros(8 * n, np.uint32)
+    ranges = np.zeros(16 * n, np.uint32)
+    bbox = np.zeros(6, np.float32)
+    xf = None if transform is None else np.ascontiguousarray(transform, np.float32).ravel()
+    rc = lib().vdbo_voxelbvh_ijkl(n, _p(means), _p(opacities), _p(quats), _p(scales), resolution, _p(xf), _p(keys),
+                                  _p(ids), _p(ranges), _p(bbox))
+    if rc != 0:
+        raise ValueError("resolution must be in [1, 4096]")
+    return keys, ids, ranges.reshape(-1, 2), bbox
+
+
+def mat_inverse(m):
+    """pnanovdb_camera_mat_inverse (putil/Camera.h:316-375) in float32."""
+    m = np.ascontiguousarray(m, np.float32).ravel()
+    out = np.zeros(16, np.float32)
+    lib().vdbo_mat_inverse(_p(m), _p(out))
+    return out.reshape(4, 4)

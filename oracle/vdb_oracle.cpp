@@ -1684,3 +1684,293 @@ uint64_t vdbo_tile_bin(uint32_t n, const float* means2d, const int32_t* radii, c
 }
 
 } // extern "C"
+
+// ============================================================================ VoxelBVH Gaussian keys
+namespace
+{
+struct V3
+{
+    float x, y, z;
+};
+
+// (p, 1) * m with m given as rows (the shaders' mul(float4(p, 1), transform) under the
+// column-major constant layout of the transposed host matrix), summed left to right
+V3 row_times(const float* m, V3 p)
+{
+    V3 r;
+    r.x = p.x * m[0] + p.y * m[4] + p.z * m[8] + m[12];
+    r.y = p.x * m[1] + p.y * m[5] + p.z * m[9] + m[13];
+    r.z = p.x * m[2] + p.y * m[6] + p.z * m[10] + m[14];
+    return r;
+}
+
+struct BvhPrim
+{
+    V3 mean, scale;
+    float qx, qy, qz, qw;
+    float local_scale;
+};
+
+BvhPrim bvh_prim(const float* means, const float* opacities, const float* quats, const float* scales, uint64_t g)
+{
+    BvhPrim p;
+    p.mean = { means[3 * g], means[3 * g + 1], means[3 * g + 2] };
+    p.scale = { scales[3 * g], scales[3 * g + 1], scales[3 * g + 2] };
+    p.qw = quats[4 * g];
+    p.qx = quats[4 * g + 1];
+    p.qy = quats[4 * g + 2];
+    p.qz = quats[4 * g + 3];
+    float o = std::fmax(0.01f, opacities[g]); // voxelbvh_gaussians_to_ijkl.slang:49
+    p.local_scale = sqrtf(2.f * cr_logf(o / 0.01f));
+    return p;
+}
+
+// voxelbvh_gaussians_to_ijkl.slang:150-182 (and bbox_reduce1.slang:56-89)
+void bvh_box(const BvhPrim& p, const float* xf, V3& mn, V3& mx)
+{
+    const float x = p.qx, y = p.qy, z = p.qz, w = p.qw;
+    const float R[3][3] = { { 1.f - 2.f * (y * y + z * z), 2.f * (x * y - z * w), 2.f * (x * z + y * w) },
+                            { 2.f * (x * y + z * w), 1.f - 2.f * (x * x + z * z), 2.f * (y * z - x * w) },
+                            { 2.f * (x * z - y * w), 2.f * (y * z + x * w), 1.f - 2.f * (x * x + y * y) } };
+    const float inf = INFINITY;
+    mn = { inf, inf, inf };
+    mx = { -inf, -inf, -inf };
+    for (int v = 0; v < 8; v++)
+    {
+        float c[3] = { (v & 1) ? 1.f : -1.f, (v & 2) ? 1.f : -1.f, (v & 4) ? 1.f : -1.f };
+        c[0] *= p.scale.x * p.local_scale;
+        c[1] *= p.scale.y * p.local_scale;
+        c[2] *= p.scale.z * p.local_scale;
+        V3 q = { c[0] * R[0][0] + c[1] * R[0][1] + c[2] * R[0][2] + p.mean.x,
+                 c[0] * R[1][0] + c[1] * R[1][1] + c[2] * R[1][2] + p.mean.y,
+                 c[0] * R[2][0] + c[1] * R[2][1] + c[2] * R[2][2] + p.mean.z };
+        q = row_times(xf, q);
+        mn = { std::fmin(mn.x, q.x), std::fmin(mn.y, q.y), std::fmin(mn.z, q.z) };
+        mx = { std::fmax(mx.x, q.x), std::fmax(mx.y, q.y), std::fmax(mx.z, q.z) };
+    }
+}
+
+// voxelbvh_gaussians_to_ijkl.slang:29-105
+bool bvh_overlap(const BvhPrim& p, const float* xf_inv, const float w2v[4], int32_t i, int32_t j, int32_t k,
+                 uint32_t level)
+{
+    const float x = p.qx, y = p.qy, z = p.qz, w = p.qw;
+    const float Ri[3][3] = { { 1.f - 2.f * (y * y + z * z), 2.f * (x * y + z * w), 2.f * (x * z - y * w) },
+                             { 2.f * (x * y - z * w), 1.f - 2.f * (x * x + z * z), 2.f * (y * z + x * w) },
+                             { 2.f * (x * z + y * w), 2.f * (y * z - x * w), 1.f - 2.f * (x * x + y * y) } };
+    const float inv_ext[3] = { 1.f / (p.local_scale * p.scale.x), 1.f / (p.local_scale * p.scale.y),
+                               1.f / (p.local_scale * p.scale.z) };
+    float lo[3] = { 0, 0, 0 }, hi[3] = { 0, 0, 0 };
+    for (int v = 0; v < 8; v++)
+    {
+        float vp[3] = { (float)i, (float)j, (float)k };
+        for (int a = 0; a < 3; a++)
+            if (v & (1 << a))
+                vp[a] += (float)(1u << level);
+        V3 l = { (vp[0] - w2v[0]) / w2v[3], (vp[1] - w2v[1]) / w2v[3], (vp[2] - w2v[2]) / w2v[3] };
+        l = row_times(xf_inv, l);
+        l = { l.x - p.mean.x, l.y - p.mean.y, l.z - p.mean.z };
+        float r[3];
+        for (int a = 0; a < 3; a++)
+            r[a] = (l.x * Ri[a][0] + l.y * Ri[a][1] + l.z * Ri[a][2]) * inv_ext[a];
+        for (int a = 0; a < 3; a++)
+        {
+            lo[a] = v == 0 ? r[a] : std::fmin(lo[a], r[a]);
+            hi[a] = v == 0 ? r[a] : std::fmax(hi[a], r[a]);
+        }
+    }
+    float d2 = 1.f;
+    for (int a = 0; a < 3; a++)
+    {
+        if (0.f < lo[a])
+            d2 -= lo[a] * lo[a];
+        else if (0.f > hi[a])
+            d2 -= hi[a] * hi[a];
+    }
+    return d2 > 0.f;
+}
+
+int32_t cvt_i32(float x) // GPU float -> int conversion (NaN 0, saturating)
+{
+    if (x != x)
+        return 0;
+    if (x <= -2147483648.f)
+        return INT32_MIN;
+    if (x >= 2147483648.f)
+        return INT32_MAX;
+    return (int32_t)x;
+}
+} // namespace
+
+extern "C" {
+
+void vdbo_mat_inverse(const float* m, float* out)
+{
+    // Camera.h:318-374: 1 / (sum of 24 signed products of one entry per row), then 16 cofactors of
+    // 6 signed triple products each; products are formed left to right over rows, sums left to right
+    auto M = [&](int r, int c) { return m[4 * r + c]; };
+    static const int det_cols[24][4] = { { 0, 1, 2, 3 }, { 0, 2, 3, 1 }, { 0, 3, 1, 2 }, { 1, 0, 3, 2 }, { 1, 2, 0, 3 },
+                                         { 1, 3, 2, 0 }, { 2, 0, 1, 3 }, { 2, 1, 3, 0 }, { 2, 3, 0, 1 }, { 3, 0, 2, 1 },
+                                         { 3, 1, 0, 2 }, { 3, 2, 1, 0 }, { 0, 1, 3, 2 }, { 0, 2, 1, 3 }, { 0, 3, 2, 1 },
+                                         { 1, 0, 2, 3 }, { 1, 2, 3, 0 }, { 1, 3, 0, 2 }, { 2, 0, 3, 1 }, { 2, 1, 0, 3 },
+                                         { 2, 3, 1, 0 }, { 3, 0, 1, 2 }, { 3, 1, 2, 0 }, { 3, 2, 0, 1 } };
+    float det = 0.f;
+    for (int t = 0; t < 24; t++)
+    {
+        float a = M(0, det_cols[t][0]);
+        if (t >= 12)
+            a = -a;
+        float prod = a * M(1, det_cols[t][1]) * M(2, det_cols[t][2]) * M(3, det_cols[t][3]);
+        det = t ? det + prod : prod;
+    }
+    const float f = float(1.0) / det;
+    // entry e of the result (rows of the returned matrix): excluded row / column, and the six terms
+    // (sign, columns of the three remaining rows in ascending row order)
+    static const int ex_row[16] = { 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3, 0, 1, 2, 3 };
+    static const int terms[16][6][4] = {
+        { { 1, 1, 2, 3 }, { 1, 2, 3, 1 }, { 1, 3, 1, 2 }, { -1, 1, 3, 2 }, { -1, 2, 1, 3 }, { -1, 3, 2, 1 } },
+        { { 1, 1, 3, 2 }, { 1, 2, 1, 3 }, { 1, 3, 2, 1 }, { -1, 1, 2, 3 }, { -1, 2, 3, 1 }, { -1, 3, 1, 2 } },
+        { { 1, 1, 2, 3 }, { 1, 2, 3, 1 }, { 1, 3, 1, 2 }, { -1, 1, 3, 2 }, { -1, 2, 1, 3 }, { -1, 3, 2, 1 } },
+        { { 1, 1, 3, 2 }, { 1, 2, 1, 3 }, { 1, 3, 2, 1 }, { -1, 1, 2, 3 }, { -1, 2, 3, 1 }, { -1, 3, 1, 2 } },
+        { { 1, 0, 3, 2 }, { 1, 2, 0, 3 }, { 1, 3, 2, 0 }, { -1, 0, 2, 3 }, { -1, 2, 3, 0 }, { -1, 3, 0, 2 } },
+        { { 1, 0, 2, 3 }, { 1, 2, 3, 0 }, { 1, 3, 0, 2 }, { -1, 0, 3, 2 }, { -1, 2, 0, 3 }, { -1, 3, 2, 0 } },
+        { { 1, 0, 3, 2 }, { 1, 2, 0, 3 }, { 1, 3, 2, 0 }, { -1, 0, 2, 3 }, { -1, 2, 3, 0 }, { -1, 3, 0, 2 } },
+        { { 1, 0, 2, 3 }, { 1, 2, 3, 0 }, { 1, 3, 0, 2 }, { -1, 0, 3, 2 }, { -1, 2, 0, 3 }, { -1, 3, 2, 0 } },
+        { { 1, 0, 1, 3 }, { 1, 1, 3, 0 }, { 1, 3, 0, 1 }, { -1, 0, 3, 1 }, { -1, 1, 0, 3 }, { -1, 3, 1, 0 } },
+        { { -1, 0, 1, 3 }, { -1, 1, 3, 0 }, { -1, 3, 0, 1 }, { 1, 0, 3, 1 }, { 1, 1, 0, 3 }, { 1, 3, 1, 0 } },
+        { { 1, 0, 1, 3 }, { 1, 1, 3, 0 }, { 1, 3, 0, 1 }, { -1, 0, 3, 1 }, { -1, 1, 0, 3 }, { -1, 3, 1, 0 } },
+        { { 1, 0, 3, 1 }, { 1, 1, 0, 3 }, { 1, 3, 1, 0 }, { -1, 1, 3, 0 }, { -1, 3, 0, 1 }, { -1, 0, 1, 3 } },
+        { { 1, 0, 2, 1 }, { 1, 1, 0, 2 }, { 1, 2, 1, 0 }, { -1, 0, 1, 2 }, { -1, 1, 2, 0 }, { -1, 2, 0, 1 } },
+        { { 1, 0, 1, 2 }, { 1, 1, 2, 0 }, { 1, 2, 0, 1 }, { -1, 0, 2, 1 }, { -1, 1, 0, 2 }, { -1, 2, 1, 0 } },
+        { { 1, 0, 2, 1 }, { 1, 1, 0, 2 }, { 1, 2, 1, 0 }, { -1, 0, 1, 2 }, { -1, 1, 2, 0 }, { -1, 2, 0, 1 } },
+        { { 1, 0, 1, 2 }, { 1, 1, 2, 0 }, { 1, 2, 0, 1 }, { -1, 0, 2, 1 }, { -1, 1, 0, 2 }, { -1, 2, 1, 0 } },
+    };
+    for (int e = 0; e < 16; e++)
+    {
+        int rows[3], n = 0;
+        for (int r = 0; r < 4; r++)
+            if (r != ex_row[e])
+                rows[n++] = r;
+        float s = 0.f;
+        for (int t = 0; t < 6; t++)
+        {
+            float a = M(rows[0], terms[e][t][1]);
+            if (terms[e][t][0] < 0)
+                a = -a;
+            float prod = a * M(rows[1], terms[e][t][2]) * M(rows[2], terms[e][t][3]);
+            s = t ? s + prod : prod;
+        }
+        out[e] = s * f;
+    }
+}
+
+int vdbo_voxelbvh_ijkl(uint64_t n, const float* means, const float* opacities, const float* quats,
+                       const float* scales, uint32_t resolution, const float* transform, uint64_t* keys,
+                       uint32_t* ids, uint32_t* ranges, float* bbox)
+{
+    if (resolution < 1 || resolution > 4096) // VoxelBVH.cpp:30-33
+        return -1;
+    float xf[16], xf_inv[16];
+    for (int e = 0; e < 16; e++)
+        xf[e] = xf_inv[e] = (e % 5 == 0) ? 1.f : 0.f;
+    if (transform)
+    {
+        std::memcpy(xf, transform, sizeof(xf));
+        vdbo_mat_inverse(xf, xf_inv);
+    }
+    // world box (bbox_reduce1/2.slang): min / max over every corner of every Gaussian
+    V3 wmn = { INFINITY, INFINITY, INFINITY }, wmx = { -INFINITY, -INFINITY, -INFINITY };
+    for (uint64_t g = 0; g < n; g++)
+    {
+        V3 mn, mx;
+        bvh_box(bvh_prim(means, opacities, quats, scales, g), xf, mn, mx);
+        wmn = { std::fmin(wmn.x, mn.x), std::fmin(wmn.y, mn.y), std::fmin(wmn.z, mn.z) };
+        wmx = { std::fmax(wmx.x, mx.x), std::fmax(wmx.y, mx.y), std::fmax(wmx.z, mx.z) };
+    }
+    bbox[0] = wmn.x, bbox[1] = wmn.y, bbox[2] = wmn.z, bbox[3] = wmx.x, bbox[4] = wmx.y, bbox[5] = wmx.z;
+    if (n == 0)
+        return 0;
+    // compute_world_to_voxel (voxelbvh_gaussians_to_ijkl.slang:107-114)
+    const float dx = wmx.x - wmn.x, dy = wmx.y - wmn.y, dz = wmx.z - wmn.z;
+    const float dmax = std::fmax(dx, std::fmax(dy, dz));
+    const float sc = float(resolution - 1u) / dmax;
+    const float w2v[4] = { -wmn.x * sc + 0.5f, -wmn.y * sc + 0.5f, -wmn.z * sc + 0.5f, sc };
+    std::vector<uint64_t> k(8 * n);
+    std::vector<uint32_t> id(8 * n);
+    for (uint64_t g = 0; g < n; g++)
+    {
+        const BvhPrim p = bvh_prim(means, opacities, quats, scales, g);
+        V3 mn, mx;
+        bvh_box(p, xf, mn, mx);
+        int32_t lo[3] = { cvt_i32(std::floor(sc * mn.x + w2v[0])), cvt_i32(std::floor(sc * mn.y + w2v[1])),
+                          cvt_i32(std::floor(sc * mn.z + w2v[2])) };
+        int32_t hi[3] = { cvt_i32(std::floor(sc * mx.x + w2v[0])), cvt_i32(std::floor(sc * mx.y + w2v[1])),
+                          cvt_i32(std::floor(sc * mx.z + w2v[2])) };
+        auto overlaps = [&](uint32_t level, uint32_t cap) {
+            const int32_t msk = ~((1 << level) - 1);
+            uint32_t c = 0;
+            for (int32_t i = lo[0] & msk; i <= (hi[0] & msk); i += (1 << level))
+                for (int32_t j = lo[1] & msk; j <= (hi[1] & msk); j += (1 << level))
+                    for (int32_t kk = lo[2] & msk; kk <= (hi[2] & msk); kk += (1 << level))
+                    {
+                        if (bvh_overlap(p, xf_inv, w2v, i, j, kk, level))
+                        {
+                            if (c < cap)
+                            {
+                                k[8 * g + c] = ((uint64_t)(int64_t)i << 48) | ((uint64_t)(int64_t)j << 32) |
+                                               ((uint64_t)(int64_t)kk << 16) | (uint64_t)level;
+                                id[8 * g + c] = (uint32_t)g;
+                            }
+                            c++;
+                        }
+                    }
+            return c;
+        };
+        // voxelbvh_gaussians_to_ijkl.slang:186-221: search from level 11 towards 0
+        uint32_t level = 11, last_good = 11;
+        while (level < 12)
+        {
+            if (overlaps(level, 0) > 8)
+            {
+                level = last_good;
+                break;
+            }
+            if (level == 0)
+                break;
+            last_good = level;
+            level -= 1;
+        }
+        uint32_t written = std::min<uint32_t>(overlaps(level, 8), 8);
+        for (uint32_t s2 = written; s2 < 8; s2++)
+        {
+            k[8 * g + s2] = ~0ull;
+            id[8 * g + s2] = ~0u;
+        }
+    }
+    // stable sort of (key, id) by key (radix_sort_key64 over 64 bits)
+    std::vector<uint32_t> order(8 * n);
+    for (uint64_t e = 0; e < 8 * n; e++)
+        order[e] = (uint32_t)e;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return k[a] < k[b]; });
+    for (uint64_t e = 0; e < 8 * n; e++)
+    {
+        keys[e] = k[order[e]];
+        ids[e] = id[order[e]];
+    }
+    // run headers (find_range_starts.slang, global_scan, scatter_range_headers.slang)
+    std::fill(ranges, ranges + 16 * n, 0u);
+    uint32_t run = 0;
+    for (uint64_t e = 0; e < 8 * n; e++)
+    {
+        if (e == 0 || keys[e] != keys[e - 1])
+        {
+            ranges[2 * run] = (uint32_t)e;
+            run++;
+        }
+        if (e + 1 == 8 * n || keys[e + 1] != keys[e])
+            ranges[2 * (run - 1) + 1] = (uint32_t)(e + 1);
+    }
+    return 0;
+}
+
+} // extern "C"

@@ -90,6 +90,17 @@ uint64_t vdbo_tile_bin(uint32_t n, const float* means2d, const int32_t* radii, c
                        uint32_t height, uint32_t tile_size, uint32_t* keys_low, uint32_t* keys_high, uint32_t* vals,
                        uint64_t cap, uint32_t* tile_offsets);
 
+/* VoxelBVH Gaussian keys (raster/VoxelBVH.cpp:1225-1435): world box, up to 8 cell keys per
+ * Gaussian (voxelbvh_gaussians_to_ijkl.slang), stable sort, run headers. keys/ids hold 8 n
+ * entries, ranges 16 n, bbox 6. transform: 16 floats (rows) or NULL. Returns 0, or -1 for a
+ * resolution outside [1, 4096]. */
+int vdbo_voxelbvh_ijkl(uint64_t n, const float* means, const float* opacities, const float* quats,
+                       const float* scales, uint32_t resolution, const float* transform, uint64_t* keys,
+                       uint32_t* ids, uint32_t* ranges, float* bbox);
+
+/* pnanovdb_camera_mat_inverse (putil/Camera.h:316-375) in the reference's term order. */
+void vdbo_mat_inverse(const float* m, float* out);
+
 #ifdef __cplusplus
 }
 #endif
