@@ -152,8 +152,11 @@ struct pnanovdb_compute_t
     pnanovdb_compute_array_t*(PNANOVDB_ABI* duplicate_array)(pnanovdb_compute_array_t* array);
 };
 
-/* nanovdb_editor/putil/Raster.h:58-174 (field order is the ABI). raster_gaussian_2d,
- * raster_file and create_gaussian_data_from_desc are NULL (outside the raster3d path). */
+/* nanovdb_editor/putil/Raster.h:58-174 (field order is the ABI). raster_gaussian_2d and
+ * create_gaussian_data_from_desc are NULL (outside the raster3d path). raster_file
+ * (RasterUtils.cpp:295-395) reads a Gaussian PLY or NPZ (see pnanovdb_fileformat_t) and rasterizes
+ * it as raster_to_nanovdb_from_arrays does, or builds gaussian data when gaussian_data and
+ * raster_context are given; the sparse-voxel file layout the reference leaves unimplemented fails. */
 typedef struct pnanovdb_raster_t pnanovdb_raster_t;
 struct pnanovdb_raster_t
 {
@@ -184,7 +187,14 @@ struct pnanovdb_raster_t
         pnanovdb_compute_array_t* colors, pnanovdb_compute_array_t* sh_0, pnanovdb_compute_array_t* sh_n,
         pnanovdb_compute_array_t* opacities, pnanovdb_compute_array_t** shader_params_arrays,
         pnanovdb_profiler_report_t profiler_report, void* userdata);
-    void* raster_file;
+    pnanovdb_bool_t(PNANOVDB_ABI* raster_file)(pnanovdb_raster_t* raster, const pnanovdb_compute_t* compute,
+                                               pnanovdb_compute_queue_t* queue, const char* filename, float voxel_size,
+                                               pnanovdb_compute_array_t** nanovdb_arr,
+                                               pnanovdb_raster_gaussian_data_t** gaussian_data,
+                                               pnanovdb_raster_context_t** raster_context,
+                                               pnanovdb_compute_array_t** shader_params_arrays,
+                                               pnanovdb_raster_shader_params_t* raster_params,
+                                               pnanovdb_profiler_report_t profiler_report, void* userdata);
     pnanovdb_bool_t(PNANOVDB_ABI* raster_to_nanovdb_from_arrays)(pnanovdb_raster_t* raster,
                                                                  const pnanovdb_compute_t* compute,
                                                                  pnanovdb_compute_queue_t* queue, float voxel_size,
@@ -243,6 +253,22 @@ typedef struct pnanovdb_parallel_primitives_t
 
 /* Getters: nanovdb_editor/putil/Raster.h:197, Compute.h:883, ParallelPrimitives.h:107.
  * Each returns a writable static table that lives as long as the library. */
+/* nanovdb_editor/putil/FileFormat.h:19-35 (without the optional e57 entry). Loads Gaussian
+ * arrays by name from binary PLY (3DGS properties: x y z, opacity, rot_0..3, scale_0..2,
+ * f_dc_0..2, f_rest_*, red green blue) or NPZ (members <name>.npy; "sh" [N, K, 3] stands for sh_0
+ * and sh_n) into arrays created with the compute table's create_array (free them with its
+ * destroy_array). Names: means/positions, opacities, quaternions, scales, sh_0, sh_n, colors.
+ * can_load_file accepts .ply and .npz. */
+typedef struct pnanovdb_fileformat_t
+{
+    const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
+    pnanovdb_bool_t(PNANOVDB_ABI* can_load_file)(const char* filename);
+    pnanovdb_bool_t(PNANOVDB_ABI* load_file)(const char* filename, pnanovdb_uint32_t array_count,
+                                             const char** array_names, pnanovdb_compute_array_t** out_arrays);
+    void* module;
+} pnanovdb_fileformat_t;
+
+PNANOVDB_API pnanovdb_fileformat_t* pnanovdb_get_fileformat(void);
 PNANOVDB_API pnanovdb_raster_t* pnanovdb_get_raster(void);
 PNANOVDB_API pnanovdb_compute_t* pnanovdb_get_compute(void);
 PNANOVDB_API pnanovdb_parallel_primitives_t* pnanovdb_get_parallel_primitives(void);

@@ -14,6 +14,7 @@
 #include <thread>
 #include <vector>
 
+#include "vdb_fileformat.h"
 #include "vdb_internal.h"
 
 namespace vdbr
@@ -784,8 +785,106 @@ static pnanovdb_bool_t PNANOVDB_ABI r_create_gaussian_data_from_arrays(
     return PNANOVDB_TRUE;
 }
 
+// RasterUtils.cpp:295-395: Gaussian file -> OnIndex grid (or gaussian data)
+static pnanovdb_bool_t PNANOVDB_ABI r_raster_file(pnanovdb_raster_t* raster, const pnanovdb_compute_t* compute,
+                                                  pnanovdb_compute_queue_t* queue, const char* filename,
+                                                  float voxel_size, pnanovdb_compute_array_t** nanovdb_arr,
+                                                  pnanovdb_raster_gaussian_data_t** gaussian_data,
+                                                  pnanovdb_raster_context_t** raster_context,
+                                                  pnanovdb_compute_array_t** shader_params_arrays,
+                                                  pnanovdb_raster_shader_params_t* raster_params,
+                                                  pnanovdb_profiler_report_t profiler_report, void* userdata)
+{
+    (void)profiler_report;
+    (void)userdata;
+    const pnanovdb_compute_t* c = (compute && compute->create_array) ? compute : compute_table();
+    const char* names[6] = { "means", "opacities", "quaternions", "scales", "sh_0", "sh_n" };
+    pnanovdb_compute_array_t* arrays[6] = {};
+    std::string err;
+    FileArrayAlloc alloc = [&](size_t es, uint64_t count, const void* data) { return c->create_array(es, count, data); };
+    if (!ff_load(filename, 6, names, arrays, alloc, &err))
+    {
+        fprintf(stderr, "libvdbraster: Failed to load Gaussian data from file '%s': %s\n", filename ? filename : "",
+                err.c_str());
+        return PNANOVDB_FALSE;
+    }
+    pnanovdb_bool_t ok = PNANOVDB_TRUE;
+    for (int i = 0; i < 5; i++)
+        if (!arrays[i] || arrays[i]->element_size != 4)
+        {
+            fprintf(stderr, "libvdbraster: '%s': array '%s' is missing or not float32\n", filename, names[i]);
+            ok = PNANOVDB_FALSE;
+        }
+    if (ok)
+    {
+        if (gaussian_data && raster_context)
+            ok = r_create_gaussian_data_from_arrays(raster, compute, queue, arrays, 6, gaussian_data, raster_params,
+                                                    raster_context);
+        else if (!nanovdb_arr)
+            ok = PNANOVDB_FALSE;
+        else
+        {
+            // RasterUtils.cpp:143-207: preprocess, rasterize with the call's shader parameters, OnIndex
+            const uint64_t n = arrays[0]->element_count / 3u;
+            if (arrays[1]->element_count < n || arrays[2]->element_count < 4 * n || arrays[3]->element_count < 3 * n ||
+                arrays[4]->element_count < 3 * n)
+            {
+                fprintf(stderr, "libvdbraster: '%s': array lengths disagree\n", filename);
+                ok = PNANOVDB_FALSE;
+            }
+            else
+            {
+                std::lock_guard<std::mutex> lock(g_default_mutex);
+                Context* ctx = default_context();
+                std::vector<uint8_t> grid;
+                if (!ctx ||
+                    run_host(ctx, n, (const float*)arrays[0]->data, (const float*)arrays[2]->data,
+                             (const float*)arrays[3]->data, (const float*)arrays[4]->data, (const float*)arrays[1]->data,
+                             voxel_size, cull_scale_of(shader_params_arrays), VDBR_FLAG_RAW_INPUTS, &grid, nullptr,
+                             nullptr, nullptr, true))
+                {
+                    fprintf(stderr, "libvdbraster: %s\n", ctx ? ctx->error.c_str() : "no HIP device");
+                    ok = PNANOVDB_FALSE;
+                }
+                else
+                {
+                    *nanovdb_arr = make_output(compute, grid);
+                    ok = *nanovdb_arr ? PNANOVDB_TRUE : PNANOVDB_FALSE;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < 6; i++)
+        if (arrays[i])
+            c->destroy_array(arrays[i]);
+    return ok;
+}
+
 static pnanovdb_raster_t g_raster;
 static bool g_raster_init = false;
+
+// FileFormat.cpp:795-806
+static pnanovdb_bool_t PNANOVDB_ABI ff_can_load_file(const char* filename)
+{
+    return ff_can_load(filename) ? PNANOVDB_TRUE : PNANOVDB_FALSE;
+}
+
+static pnanovdb_bool_t PNANOVDB_ABI ff_load_file(const char* filename, pnanovdb_uint32_t array_count,
+                                                 const char** array_names, pnanovdb_compute_array_t** out_arrays)
+{
+    const pnanovdb_compute_t* c = compute_table();
+    std::string err;
+    FileArrayAlloc alloc = [&](size_t es, uint64_t count, const void* data) { return c->create_array(es, count, data); };
+    if (!ff_load(filename, array_count, array_names, out_arrays, alloc, &err))
+    {
+        fprintf(stderr, "Error: Failed to load file: %s (%s)\n", filename ? filename : "", err.c_str());
+        return PNANOVDB_FALSE;
+    }
+    return PNANOVDB_TRUE;
+}
+
+static pnanovdb_fileformat_t g_fileformat;
+static bool g_fileformat_init = false;
 
 // ============================================================================= parallel primitives
 static pnanovdb_parallel_primitives_context_t* PNANOVDB_ABI pp_create_context(const pnanovdb_compute_t* compute,
@@ -932,11 +1031,24 @@ PNANOVDB_API pnanovdb_raster_t* pnanovdb_get_raster(void)
         g_raster.destroy_gaussian_data = r_destroy_gaussian_data;
         g_raster.raster_gaussian_3d = r_raster_gaussian_3d;
         g_raster.raster_to_nanovdb = r_raster_to_nanovdb;
+        g_raster.raster_file = r_raster_file;
         g_raster.raster_to_nanovdb_from_arrays = r_raster_to_nanovdb_from_arrays;
         g_raster.create_gaussian_data_from_arrays = r_create_gaussian_data_from_arrays;
         g_raster_init = true;
     }
     return &g_raster;
+}
+
+PNANOVDB_API pnanovdb_fileformat_t* pnanovdb_get_fileformat(void)
+{
+    if (!g_fileformat_init)
+    {
+        memset(&g_fileformat, 0, sizeof(g_fileformat));
+        g_fileformat.can_load_file = ff_can_load_file;
+        g_fileformat.load_file = ff_load_file;
+        g_fileformat_init = true;
+    }
+    return &g_fileformat;
 }
 
 PNANOVDB_API pnanovdb_parallel_primitives_t* pnanovdb_get_parallel_primitives(void)

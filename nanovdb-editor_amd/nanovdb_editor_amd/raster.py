@@ -4,7 +4,7 @@
 ``Raster.raster_to_nanovdb`` and ``Raster.raster_to_nanovdb_from_arrays`` keep the reference
 argument order, validation and error behaviour.
 """
-from ctypes import CFUNCTYPE, POINTER, Structure, byref, c_float, c_uint32, c_void_p, pointer
+from ctypes import CFUNCTYPE, POINTER, Structure, byref, c_char_p, c_float, c_uint32, c_void_p, pointer
 
 from ._lib import lib
 from .compute import Compute, pnanovdb_bool_t, pnanovdb_Compute, pnanovdb_ComputeArray
@@ -30,7 +30,9 @@ class pnanovdb_Raster(Structure):
                                          c_void_p, c_void_p, c_void_p)),
         ("raster_to_nanovdb", CFUNCTYPE(_ArrP, POINTER(pnanovdb_Compute), c_void_p, c_float, _ArrP, _ArrP, _ArrP,
                                         _ArrP, _ArrP, _ArrP, _ArrP, POINTER(_ArrP), c_void_p, c_void_p)),
-        ("raster_file", c_void_p),
+        ("raster_file", CFUNCTYPE(pnanovdb_bool_t, c_void_p, POINTER(pnanovdb_Compute), c_void_p, c_char_p, c_float,
+                                  POINTER(_ArrP), POINTER(c_void_p), POINTER(c_void_p), POINTER(_ArrP), c_void_p,
+                                  c_void_p, c_void_p)),
         ("raster_to_nanovdb_from_arrays", CFUNCTYPE(pnanovdb_bool_t, c_void_p, POINTER(pnanovdb_Compute), c_void_p,
                                                     c_float, POINTER(_ArrP), c_uint32, POINTER(_ArrP))),
         ("create_gaussian_data_from_arrays", CFUNCTYPE(pnanovdb_bool_t, c_void_p, POINTER(pnanovdb_Compute),
@@ -79,4 +81,14 @@ class Raster:
                   byref(out))
         if not ok or not out:
             raise RuntimeError("Failed to rasterize Gaussian arrays to NanoVDB")
+        return out.contents
+
+    def raster_file(self, filename: str, voxel_size: float, shader_params_arrays=None) -> pnanovdb_ComputeArray:
+        """Gaussian PLY / NPZ file -> OnIndex grid array (RasterUtils.cpp:295-395)."""
+        func = self._raster.contents.raster_file
+        out = _ArrP()
+        ok = func(self._raster, self._compute.get_compute(), None, filename.encode(), c_float(voxel_size), byref(out),
+                  None, None, shader_params_arrays, None, None, None)
+        if not ok or not out:
+            raise RuntimeError(f"Failed to rasterize Gaussian file {filename}")
         return out.contents

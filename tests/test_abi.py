@@ -54,12 +54,15 @@ int main(void) {
   O(pnanovdb_compute_t, destroy_array) O(pnanovdb_compute_t, map_array) O(pnanovdb_compute_t, duplicate_array)
   S(pnanovdb_raster_t) O(pnanovdb_raster_t, raster_gaussian_3d) O(pnanovdb_raster_t, raster_to_nanovdb)
   O(pnanovdb_raster_t, raster_to_nanovdb_from_arrays) O(pnanovdb_raster_t, create_gaussian_data_from_arrays)
+  O(pnanovdb_raster_t, raster_file)
+  S(pnanovdb_fileformat_t) O(pnanovdb_fileformat_t, load_file) O(pnanovdb_fileformat_t, module)
+  S(vdbr_batch_item_t) O(vdbr_batch_item_t, stats) S(vdbr_tune_result_t)
   S(pnanovdb_parallel_primitives_t) O(pnanovdb_parallel_primitives_t, radix_sort_key64)
   O(pnanovdb_parallel_primitives_t, compute)
   S(vdbr_gaussians_t) O(vdbr_gaussians_t, count)
   S(vdbr_raster_params_t) O(vdbr_raster_params_t, max_node_count)
   S(vdbr_raster_stats_t) O(vdbr_raster_stats_t, key_bits) O(vdbr_raster_stats_t, ms_total)
-  O(vdbr_raster_stats_t, ms_reduce_kernel)
+  O(vdbr_raster_stats_t, ms_reduce_kernel) O(vdbr_raster_stats_t, sort_bit_begin)
   S(vdbr_buffer_t)
   return 0;
 }
@@ -92,7 +95,7 @@ def test_struct_layouts_match_ctypes(c_layout):
         assert c[f"pnanovdb_compute_t.{f}"] == off(pnanovdb_Compute, f), f
     assert c["pnanovdb_raster_t"] == ctypes.sizeof(pnanovdb_Raster)
     for f in ("raster_gaussian_3d", "raster_to_nanovdb", "raster_to_nanovdb_from_arrays",
-              "create_gaussian_data_from_arrays"):
+              "create_gaussian_data_from_arrays", "raster_file"):
         assert c[f"pnanovdb_raster_t.{f}"] == off(pnanovdb_Raster, f), f
     assert c["pnanovdb_parallel_primitives_t"] == ctypes.sizeof(PnanovdbParallelPrimitives)
     assert c["pnanovdb_parallel_primitives_t.radix_sort_key64"] == off(PnanovdbParallelPrimitives, "radix_sort_key64")
@@ -102,6 +105,15 @@ def test_struct_layouts_match_ctypes(c_layout):
     assert c["vdbr_raster_stats_t.key_bits"] == off(VdbrRasterStats, "key_bits")
     assert c["vdbr_raster_stats_t.ms_reduce_kernel"] == off(VdbrRasterStats, "ms_reduce_kernel")
     assert c["vdbr_buffer_t"] == ctypes.sizeof(VdbrBuffer)
+    assert c["vdbr_raster_stats_t.sort_bit_begin"] == off(VdbrRasterStats, "sort_bit_begin")
+    from nanovdb_editor_amd._lib import VdbrBatchItem, VdbrTuneResult
+    from nanovdb_editor_amd.fileformat import pnanovdb_FileFormat
+    assert c["pnanovdb_fileformat_t"] == ctypes.sizeof(pnanovdb_FileFormat)
+    assert c["pnanovdb_fileformat_t.load_file"] == off(pnanovdb_FileFormat, "load_file")
+    assert c["pnanovdb_fileformat_t.module"] == off(pnanovdb_FileFormat, "module")
+    assert c["vdbr_batch_item_t"] == ctypes.sizeof(VdbrBatchItem)
+    assert c["vdbr_batch_item_t.stats"] == off(VdbrBatchItem, "stats")
+    assert c["vdbr_tune_result_t"] == ctypes.sizeof(VdbrTuneResult)
 
 
 def test_function_tables_populated():
@@ -109,9 +121,9 @@ def test_function_tables_populated():
     raster = ctypes.cast(L.pnanovdb_get_raster(), ctypes.POINTER(pnanovdb_Raster)).contents
     for f in ("create_context", "destroy_context", "create_gaussian_data", "upload_gaussian_data",
               "destroy_gaussian_data", "raster_gaussian_3d", "raster_to_nanovdb", "raster_to_nanovdb_from_arrays",
-              "create_gaussian_data_from_arrays"):
+              "create_gaussian_data_from_arrays", "raster_file"):
         assert ctypes.cast(getattr(raster, f), ctypes.c_void_p).value, f
-    assert not raster.raster_gaussian_2d and not raster.raster_file
+    assert not raster.raster_gaussian_2d and not raster.create_gaussian_data_from_desc
     comp = ctypes.cast(L.pnanovdb_get_compute(), ctypes.POINTER(pnanovdb_Compute)).contents
     for f in ("create_array", "destroy_array", "map_array", "unmap_array", "duplicate_array"):
         assert ctypes.cast(getattr(comp, f), ctypes.c_void_p).value, f
