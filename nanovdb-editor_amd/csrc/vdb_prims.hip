@@ -231,8 +231,11 @@ int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bo
 struct SortShape
 {
     int threads, items;
+    bool with_vals; // the shape also serves sorts with a 32-bit payload (LDS room for it)
 };
-constexpr SortShape SORT_VARIANTS[] = { { 512, 16 }, { 256, 16 }, { 1024, 8 }, { 256, 24 }, { 512, 12 } };
+// the last two shapes stage 16384 / 12288 keys (128 / 96 KB of LDS, one block per CU): keys-only sorts
+constexpr SortShape SORT_VARIANTS[] = { { 512, 16, true },  { 256, 16, true },   { 1024, 8, true }, { 256, 24, true },
+                                        { 512, 12, true },  { 1024, 16, false }, { 768, 16, false } };
 constexpr uint32_t SORT_VARIANT_COUNT = sizeof(SORT_VARIANTS) / sizeof(SORT_VARIANTS[0]);
 constexpr int SORT_MAX_PASSES = 11; // 64 bits in digits of at least 6 bits
 
@@ -529,11 +532,13 @@ static uint32_t size_bucket(uint64_t n)
     return b < SORT_BUCKETS ? b : SORT_BUCKETS - 1;
 }
 
-static SortConfig sort_config(const Context* ctx, uint64_t n)
+static SortConfig sort_config(const Context* ctx, uint64_t n, bool with_vals)
 {
     SortConfig c = ctx->sort_tuned[size_bucket(n)];
     if (c.variant >= SORT_VARIANT_COUNT || c.digit_bits < 6u || c.digit_bits > 8u)
         c = SortConfig{ 0u, 8u };
+    if (with_vals && !SORT_VARIANTS[c.variant].with_vals)
+        c.variant = 0u;
     return c;
 }
 
@@ -565,6 +570,16 @@ static void launch_sort_pass(uint32_t variant, uint64_t tiles, hipStream_t st, c
         hipLaunchKernelGGL((k_sort_pass<512, 12, V>), dim3((uint32_t)tiles), dim3(512), 0, st, kin, kout, vin, vout, n,
                            shift, dbits, starts, status, counter, epoch);
         break;
+    case 5: // keys only (sort_config never selects 5 or 6 with a payload)
+        if (!V)
+            hipLaunchKernelGGL((k_sort_pass<1024, 16, false>), dim3((uint32_t)tiles), dim3(1024), 0, st, kin, kout,
+                               nullptr, nullptr, n, shift, dbits, starts, status, counter, epoch);
+        break;
+    case 6:
+        if (!V)
+            hipLaunchKernelGGL((k_sort_pass<768, 16, false>), dim3((uint32_t)tiles), dim3(768), 0, st, kin, kout,
+                               nullptr, nullptr, n, shift, dbits, starts, status, counter, epoch);
+        break;
     default:
         hipLaunchKernelGGL((k_sort_pass<512, 16, V>), dim3((uint32_t)tiles), dim3(512), 0, st, kin, kout, vin, vout, n,
                            shift, dbits, starts, status, counter, epoch);
@@ -578,7 +593,7 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
     if (n <= 1 || bit_end <= bit_begin)
         return 0;
     uint32_t bits = bit_end - bit_begin;
-    const SortConfig cfg = sort_config(ctx, n);
+    const SortConfig cfg = sort_config(ctx, n, vals != nullptr);
     const uint32_t W = cfg.digit_bits;
     uint32_t passes = (bits + W - 1u) / W;
     if (passes > SORT_MAX_PASSES)
@@ -742,7 +757,7 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
     if (n <= 1 || bit_end <= bit_begin)
         return 0;
     uint32_t bits = bit_end - bit_begin;
-    const SortConfig cfg = sort_config(ctx, n);
+    const SortConfig cfg = sort_config(ctx, n, false);
     const uint32_t W = cfg.digit_bits;
     uint32_t passes = (bits + W - 1u) / W;
     if (passes > SORT_MAX_PASSES)

@@ -70,7 +70,7 @@ def test_sort_every_shape(digit_bits):
     v = Voxelizer(0)
     try:
         rng = np.random.default_rng(digit_bits)
-        for variant in range(5):
+        for variant in range(7):
             v.set_sort_config(0, variant, digit_bits)
             for n, bits in ((70_001, 45), (1_200_007, 64)):
                 keys = rng.integers(0, 2**63, n, dtype=np.int64).astype(np.uint64)
@@ -84,6 +84,10 @@ def test_sort_every_shape(digit_bits):
                 order = np.argsort(keys, kind="stable")
                 assert np.array_equal(kt.cpu().numpy().view(np.uint64), keys[order]), (variant, n)
                 assert np.array_equal(vt.cpu().numpy(), vals[order]), (variant, n)
+                # keys only (the shapes without payload room are used only here)
+                ko = torch.from_numpy(keys.view(np.int64).copy()).cuda()
+                v.radix_sort_u64(ko, None, 0, bits)
+                assert np.array_equal(ko.cpu().numpy().view(np.uint64), keys[order]), (variant, n, "keys only")
     finally:
         v.close()
 
@@ -93,8 +97,8 @@ def test_autotune_sort():
     v = Voxelizer(0)
     try:
         best, rows = v.autotune_sort(1 << 20, 40)
-        assert len(rows) == 10
-        assert {(r["variant"], r["digit_bits"]) for r in rows} == {(i, w) for i in range(5) for w in (8, 7)}
+        assert len(rows) == 14
+        assert {(r["variant"], r["digit_bits"]) for r in rows} == {(i, w) for i in range(7) for w in (8, 7)}
         assert best["ms"] == min(r["ms"] for r in rows) and best["ms"] > 0
         # the chosen configuration sorts correctly
         rng = np.random.default_rng(3)
