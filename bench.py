@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2,
                     help="timed calls of the 256-cloud batched entry point (0 skips it)")
     ap.add_argument("--batch-workers", type=int, default=4)
+    ap.add_argument("--fit-steps", type=int, default=1,
+                    help="timed fit steps (forward + L2 + backward) of the 1M-Gaussian fit workload (0 skips it)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default sort shape instead of autotuning it for the workload")
     ap.add_argument("--fwd-bwd-steps", type=int, default=3,
@@ -116,6 +118,59 @@ def fwd_bwd(cloud, raw, vox, steps):
             "voxels": voxels, "steps": steps,
             "note": "float density grid forward (sorted ascending-gid sums) and its adjoint w.r.t. means, "
                     "log-scales, quaternions, opacity logits via torch.autograd; loss = sum(w * values)"}
+
+
+def fit_1m(vox, steps):
+    """Fit step of the 1M-Gaussian workload: density forward of the fit cloud (seed 31) with voxel
+    coordinates, the target grid (seed 32, built once) sampled at those voxels, L2 loss over the
+    fit voxels, backward to all four parameter tensors."""
+    import numpy as np
+    import torch
+    from nanovdb_editor_amd import synthetic as syn
+    from nanovdb_editor_amd.autograd import voxelize_density
+    from nanovdb_editor_amd.device import VALUE_DENSITY
+    fit, tgt = syn.workload_fit_1m(31), syn.workload_fit_1m(32)
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32).ravel()).to(dev)
+    tw = [t(a) for a in (tgt.means, tgt.quats, tgt.log_scales, tgt.sh0, tgt.opacity_logits)]
+    tgrid, tst = vox.raster_node2(tw[0], tw[1], tw[2], torch.empty_like(tw[3]), tw[4], tgt.voxel_size,
+                                  value_mode=VALUE_DENSITY, raw=True, sh0=tw[3])
+    tcount = tst["voxel_count"] + 2
+    tvals = tgrid[tgrid.numel() - 4 * tcount:].view(torch.float32)
+    params = [t(a).requires_grad_(True) for a in (fit.means, fit.log_scales, fit.quats, fit.opacity_logits)]
+    stats = {}
+
+    def step():
+        for p in params:
+            p.grad = None
+        values, grid, ijk = voxelize_density(*params, fit.voxel_size, vox=vox, with_coords=True)
+        target = vox.node2_sample(tgrid, tvals, ijk[2:].contiguous())
+        loss = ((values[2:] - target) ** 2).sum()
+        loss.backward()
+        stats["voxels"] = values.numel() - 2
+        stats["loss"] = float(loss.detach())
+
+    step()
+    torch.cuda.synchronize()
+    t_f = t_fb = 0.0
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            voxelize_density(*params, fit.voxel_size, vox=vox)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_f += t1 - t0
+        t_fb += t2 - t1
+    _, st = vox.raster_node2(*(t(a) for a in (fit.means, fit.quats, fit.log_scales)), torch.empty_like(params[0]),
+                             t(fit.opacity_logits), fit.voxel_size, value_mode=VALUE_DENSITY, raw=True,
+                             sh0=torch.zeros_like(params[0]), copy_out=False)
+    return {"workload": "fit_1m: 1M Gaussians, means U[-1,1]^3, log-scales N(-4,0.5^2), voxel size 0.005; "
+                        "fit seed 31 against the target grid of seed 32, L2 over the fit voxels",
+            "gaussians": fit.count, "pairs": st["pair_count"], "voxels": stats["voxels"], "target_voxels": tcount - 2,
+            "forward_ms": 1e3 * t_f / steps, "fwd_bwd_ms": 1e3 * t_fb / steps, "steps": steps}
 
 
 def batched(vox, steps, workers):
@@ -269,6 +324,13 @@ def main():
     }
     if args.fwd_bwd_steps > 0:
         out["differentiable"] = fwd_bwd(cloud, raw, vox, args.fwd_bwd_steps)
+    if args.fit_steps > 0 and default_workload:
+        vox.trim()  # the fit workload needs most of HBM
+        torch.cuda.empty_cache()
+        fvox = Voxelizer(0)
+        out["fit_1m"] = fit_1m(fvox, args.fit_steps)
+        fvox.close()
+        torch.cuda.empty_cache()
     if args.batch_steps > 0 and default_workload:
         out["batched"] = batched(vox, args.batch_steps, args.batch_workers)
     if args.cpu_sample > 0:

@@ -300,6 +300,9 @@ PNANOVDB_API int vdbr_synchronize(vdbr_context_t* ctx);
 PNANOVDB_API int vdbr_copy(vdbr_context_t* ctx, void* dst, const void* src, uint64_t bytes, int kind);
 /* bytes of device memory currently held by the context's workspace */
 PNANOVDB_API uint64_t vdbr_workspace_bytes(vdbr_context_t* ctx);
+/* Releases the context's workspace (and its batch workers'); device buffers handed out by earlier
+ * calls become invalid. The next call allocates again. */
+PNANOVDB_API int vdbr_context_trim(vdbr_context_t* ctx);
 PNANOVDB_API int vdbr_release_workspace(vdbr_context_t* ctx);
 
 /* Gaussian arrays on the device, float32, reference layouts: means[3N], opacities[N],
@@ -333,6 +336,7 @@ typedef struct vdbr_gaussian_grads_t
 
 #define VDBR_FLAG_RAW_INPUTS 1u
 #define VDBR_FLAG_SPATIAL_ORDER 2u /* nodes in BFS tree order instead of the windowed order */
+#define VDBR_FLAG_VOXEL_COORDS 4u  /* also record every value slot's voxel (vdbr_last_voxels) */
 
 typedef struct vdbr_raster_params_t
 {
@@ -343,7 +347,7 @@ typedef struct vdbr_raster_params_t
     uint32_t prim_batch_size;   /* node-order model, 0 -> 262144 (Raster.cpp:443) */
     uint32_t ijk_batch_size;    /* 0 -> 8388608 (Raster.cpp:444) */
     uint32_t max_node_count;    /* 0 -> 2097152 (GridBuild.cpp:222) */
-    uint32_t reserved;
+    uint32_t slab_pairs;        /* pairs per slab before the build is windowed over voxel x; 0 -> 2^31 */
 } vdbr_raster_params_t;
 
 typedef struct vdbr_raster_stats_t
@@ -365,7 +369,7 @@ typedef struct vdbr_raster_stats_t
     float ms_emit_kernel;      /* time of the last emission kernel launch (HIP events) */
     float ms_reduce_kernel;    /* time of the per-voxel reduction kernel (HIP events) */
     uint32_t sort_bit_begin;   /* the pair sort orders key bits [sort_bit_begin, key_bits) */
-    uint32_t reserved_stats;
+    uint32_t slab_count;       /* voxel-x slabs the pairs were emitted, sorted and reduced in */
 } vdbr_raster_stats_t;
 
 /* Full raster3d pipeline on device inputs. The node2 grid is written to a context-owned
@@ -416,6 +420,18 @@ PNANOVDB_API int vdbr_density_backward(vdbr_context_t* ctx, const vdbr_gaussians
  * to query it). */
 PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap,
                                  uint64_t* count);
+
+/* Voxel coordinates (int32 i, j, k per value slot, value-index order; slots 0 and 1, the
+ * background, hold INT32_MIN-like sentinels) of the last vdbr_raster_node2 call made with
+ * VDBR_FLAG_VOXEL_COORDS, copied to ijk_dev[3 * cap]; *count receives the value count. No
+ * counterpart in the reference (its consumers walk the tree); used to evaluate losses between grids. */
+PNANOVDB_API int vdbr_last_voxels(vdbr_context_t* ctx, int32_t* ijk_dev, uint64_t cap, uint64_t* count);
+
+/* Value words of a node2 grid at voxel coordinates: out[q] = values[value index of ijk[3q..3q+2]]
+ * (PNanoVDBExt.h:225-318 descent), 0 where the voxel is inactive. values is the grid's value array
+ * (value_count 32-bit words, e.g. the trailing blind data of a grid this library built). */
+PNANOVDB_API int vdbr_node2_sample(vdbr_context_t* ctx, const void* grid_dev, uint64_t grid_size, const uint32_t* values,
+                                   uint64_t value_count, const int32_t* ijk_dev, uint64_t count, uint32_t* out_dev);
 
 /* Stable LSD radix sort of 64-bit keys over bits [bit_begin, bit_end), optional 32-bit
  * payload, in place (ParallelPrimitives.cpp:685-869 semantics). */

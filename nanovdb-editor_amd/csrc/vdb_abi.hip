@@ -22,6 +22,16 @@ namespace vdbr
 
 static const size_t PINNED_BYTES = 1u << 20;
 
+void Context::ws_free(WsSlot slot)
+{
+    if (!ws_ptr[slot])
+        return;
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(ws_ptr[slot]);
+    ws_ptr[slot] = nullptr;
+    ws_size[slot] = 0;
+}
+
 void* Context::ws(WsSlot slot, size_t bytes)
 {
     if (bytes == 0)
@@ -1122,6 +1132,20 @@ PNANOVDB_API int vdbr_copy(vdbr_context_t* ctx, void* dst, const void* src, uint
     return 0;
 }
 
+PNANOVDB_API int vdbr_context_trim(vdbr_context_t* ctx)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    (void)hipSetDevice(c->device);
+    for (Context* w : c->batch_workers)
+        w->release();
+    c->release();
+    c->last_voxel_ijk = nullptr;
+    c->last_value_count = 0;
+    return 0;
+}
+
 PNANOVDB_API uint64_t vdbr_workspace_bytes(vdbr_context_t* ctx)
 {
     return ctx ? ((Context*)ctx)->workspace_bytes() : 0;
@@ -1154,6 +1178,23 @@ PNANOVDB_API int vdbr_raster_node2_batch(vdbr_context_t* ctx, vdbr_batch_item_t*
         return -1;
     (void)hipSetDevice(c->device);
     return raster_batch(c, items, count, workers);
+}
+
+PNANOVDB_API int vdbr_last_voxels(vdbr_context_t* ctx, int32_t* ijk_dev, uint64_t cap, uint64_t* count)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    return last_voxels(c, ijk_dev, cap, count);
+}
+
+PNANOVDB_API int vdbr_node2_sample(vdbr_context_t* ctx, const void* grid_dev, uint64_t grid_size, const uint32_t* values,
+                                   uint64_t value_count, const int32_t* ijk_dev, uint64_t count, uint32_t* out_dev)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    return node2_sample(c, grid_dev, grid_size, values, value_count, ijk_dev, count, out_dev);
 }
 
 PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap,

@@ -67,6 +67,12 @@ enum WsSlot
     WS_TUNE_KEYS,
     WS_DUAL_WIDE,
     WS_TILE_SCAN,
+    WS_VOX_IJK,
+    WS_VOX_G0,
+    WS_VOX_LEAF,
+    WS_VOX_GROW,
+    WS_SLAB_HIST,
+    WS_SLAB_ROWS,
     WS_BVH_PARTIAL,
     WS_BVH_STARTS,
     WS_TILE_KEYS,
@@ -122,6 +128,8 @@ struct Context
     uint32_t last_alpha_bits = 0;
     int32_t last_lmin[3] = { 0, 0, 0 };
     uint32_t last_bits[3] = { 0, 0, 0 };
+    int32_t* last_voxel_ijk = nullptr; // value-index-ordered voxel coordinates (VDBR_FLAG_VOXEL_COORDS)
+    uint64_t last_value_count = 0;
 
     // batched launch: worker contexts (own streams and workspaces) and the output arena chunks
     std::vector<Context*> batch_workers;
@@ -133,6 +141,7 @@ struct Context
         return -1;
     }
     void* ws(WsSlot slot, size_t bytes);
+    void ws_free(WsSlot slot); // releases one slot (after the stream has drained)
     void* scratch_scan(size_t bytes)
     {
         return ws(WS_SCAN_PARTIALS, bytes);
@@ -159,6 +168,9 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
                    SortTiming* timing);
 int radix_sort_dual_key(Context* ctx, uint32_t* key_low, uint32_t* key_high, uint32_t* vals, uint64_t n,
                         uint32_t low_bits, uint32_t high_bits);
+int last_voxels(Context* ctx, int32_t* ijk_dev, uint64_t cap, uint64_t* count);
+int node2_sample(Context* ctx, const void* grid, uint64_t grid_size, const uint32_t* values, uint64_t value_count,
+                 const int32_t* ijk, uint64_t n, uint32_t* out);
 // vdb_bvh.hip
 int voxelbvh_ijkl(Context* ctx, const vdbr_gaussians_t* gs, uint32_t resolution, const float* transform,
                   uint64_t* ijkl, uint32_t* prim_ids, uint32_t* ranges, float* world_bbox);

@@ -896,6 +896,7 @@ int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_
         }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    ctx->ws_free(WS_TUNE_KEYS); // the hashed keys are needed only while tuning
     if (rc)
     {
         ctx->sort_tuned[bucket] = saved;

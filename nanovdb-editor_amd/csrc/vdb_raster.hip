@@ -297,7 +297,40 @@ struct EmitArgs
     uint64_t seg_part_cap;          // capacity of one partition
     const uint64_t* seg_part_count; // segments written per partition
     const uint64_t* seg_chunk_scan; // SEG_PARTS + 1 exclusive scan of 64-segment chunks per partition
+    int32_t slab_i0;                // first voxel column of the slab (INT32_MIN: no slab)
 };
+
+// rows of every Gaussian inside the voxel columns [i0, i1) of a slab
+__global__ void k_slab_rows(const PrimBox* __restrict__ boxes, uint64_t n, int32_t i0, int32_t i1,
+                            uint64_t* __restrict__ rows)
+{
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const PrimBox b = boxes[g];
+        const int64_t lo = max((int64_t)b.mn[0], (int64_t)i0);
+        const int64_t hi = min((int64_t)b.mn[0] + (int64_t)b.rng_x, (int64_t)i1);
+        rows[g] = hi > lo ? (uint64_t)(hi - lo) * (uint64_t)b.rng_y : 0ull;
+    }
+}
+
+// candidate fragments per leaf column (x / 8 - lmin_x), each Gaussian's footprint spread evenly
+// over the columns its AABB spans: the slab planner's load estimate
+__global__ void k_slab_hist(const PrimBox* __restrict__ boxes, const uint64_t* __restrict__ foot, uint64_t n,
+                            int32_t lmin_x, uint32_t cols, unsigned long long* __restrict__ hist)
+{
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const uint64_t fp = foot[g];
+        if (!fp)
+            continue;
+        const PrimBox b = boxes[g];
+        const int32_t c0 = (b.mn[0] >> 3) - lmin_x;
+        const int32_t c1 = ((b.mn[0] + (int32_t)b.rng_x - 1) >> 3) - lmin_x;
+        const uint64_t share = fp / (uint64_t)(c1 - c0 + 1) + 1ull;
+        for (int32_t c = max(c0, 0); c <= c1 && c < (int32_t)cols; c++)
+            atomicAdd(&hist[c], (unsigned long long)share);
+    }
+}
 
 // segment buffer partitions: block b reserves in partition b % SEG_PARTS (separate counters on
 // separate cache lines, so reservations do not serialize on one address)
@@ -520,10 +553,13 @@ __device__ __forceinline__ void walk_row(const PrimFrame& f, int32_t i, int32_t 
 
 // Row -> (Gaussian, i, j, candidate interval): the Gaussian owning the row (binary search of the
 // row scan between the block's first and last Gaussian), its frame, and the pruned k interval.
+// rows of a Gaussian are its AABB's (i, j) pairs with i >= i_lo (the first voxel column of the
+// slab being emitted; INT32_MIN without slabs)
 __device__ __forceinline__ bool row_candidates(const uint64_t* __restrict__ row_scan, const PrimFrame* __restrict__ frames,
                                                const PrimBox* __restrict__ boxes, uint64_t row, uint64_t total_rows,
                                                uint64_t glo, uint64_t ghi, float vs, uint64_t* g, PrimFrame* f,
-                                               int32_t* i, int32_t* j, int32_t* k0, int32_t* len)
+                                               int32_t* i, int32_t* j, int32_t* k0, int32_t* len,
+                                               int32_t i_lo = INT32_MIN)
 {
     *g = 0;
     *i = *j = *k0 = *len = 0;
@@ -533,7 +569,7 @@ __device__ __forceinline__ bool row_candidates(const uint64_t* __restrict__ row_
     PrimBox b = boxes[*g];
     *f = frames[*g];
     uint64_t lr = row - row_scan[*g];
-    *i = b.mn[0] + (int32_t)(lr / (uint64_t)b.rng_y);
+    *i = max(b.mn[0], i_lo) + (int32_t)(lr / (uint64_t)b.rng_y);
     *j = b.mn[1] + (int32_t)(lr % (uint64_t)b.rng_y);
     *k0 = b.mn[2];
     *len = b.rng_z;
@@ -579,7 +615,8 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
         uint64_t g;
         int32_t i, j, k0, len;
         PrimFrame f;
-        row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, a.vs, &g, &f, &i, &j, &k0, &len);
+        row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, a.vs, &g, &f, &i, &j, &k0, &len,
+                       a.slab_i0);
         const uint32_t nseg = len > 0 ? (uint32_t)((len + EMIT_SEG - 1) / EMIT_SEG) : 0u;
         uint32_t total;
         const uint32_t excl = wave_excl_add(nseg, lane, &total);
@@ -1879,7 +1916,8 @@ __global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, G
                                                       const uint32_t* __restrict__ leaf_pos,
                                                       const uint32_t* __restrict__ leaf_node_idx,
                                                       const uint64_t* __restrict__ vals_scan,
-                                                      const uint32_t* __restrict__ vox_leaf)
+                                                      const uint32_t* __restrict__ vox_leaf, KeyLayout kl,
+                                                      int32_t* __restrict__ ijk_out)
 {
     uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvox)
@@ -1891,8 +1929,17 @@ __global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, G
         return;
     uint64_t idx = 2ull + vals_scan[ni] + rank;
     if (idx < gi.value_count)
+    {
         *(uint32_t*)(g + gi.values + 4ull * idx) = vox_val[v];
-    (void)vox_key;
+        if (ijk_out) // voxel coordinates in value-index order (VDBR_FLAG_VOXEL_COORDS)
+        {
+            int32_t i, j, k;
+            decode_voxel(kl, vox_key[v], &i, &j, &k);
+            ijk_out[3 * idx + 0] = i;
+            ijk_out[3 * idx + 1] = j;
+            ijk_out[3 * idx + 2] = k;
+        }
+    }
     (void)leaf_of_run_start;
 }
 
@@ -2018,13 +2065,15 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         return ctx->fail("raster_node2: null Gaussian array");
     if (prm->value_mode != VDBR_VALUE_RGBA8 && prm->value_mode != VDBR_VALUE_DENSITY)
         return ctx->fail("raster_node2: unknown value_mode");
+    ctx->last_voxel_ijk = nullptr;
+    ctx->last_value_count = 0;
     const uint32_t PB = prm->prim_batch_size ? prm->prim_batch_size : 262144u;
     const uint32_t W = prm->ijk_batch_size ? prm->ijk_batch_size : 8388608u;
     const uint32_t max_nodes = prm->max_node_count ? prm->max_node_count : 2097152u;
     const float vs_inv = 1.f / vs;
     hipStream_t st = ctx->stream;
 
-    const int NEV = 12;
+    const int NEV = 16;
     hipEvent_t ev[NEV];
     for (int e = 0; e < NEV; e++)
         VDBR_CHECK(ctx, hipEventCreate(&ev[e]));
@@ -2033,7 +2082,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         hipEvent_t* e;
         ~EvGuard()
         {
-            for (int i = 0; i < 12; i++)
+            for (int i = 0; i < 16; i++)
                 (void)hipEventDestroy(e[i]);
         }
     } evg{ ev };
@@ -2156,202 +2205,378 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     const uint32_t key_bits = kl.voxel_bits + kl.gid_bits + kl.alpha_bits;
     const uint32_t sort_begin = alpha_in_key ? kl.gid_bits + 8u : 0u;
 
-    // ---------------------------------------------------------------- emit
-    uint64_t pair_count = 0;
-    uint64_t* pairs = nullptr;
-    uint32_t attempts = 0;
-    uint4* segs = nullptr;
-    uint64_t seg_count = 0;
-    if (any)
+    // ---------------------------------------------------------------- slab plan
+    // Pairs are emitted, sorted and reduced per slab of leaf-aligned voxel columns x in [i0, i1);
+    // x is the most significant voxel-key axis, so each slab's sorted voxels form one contiguous
+    // key range and the slabs' voxel lists concatenate into the global sorted list. One slab
+    // unless the estimated pairs exceed the budget; a slab that still overflows is split.
+    const uint64_t slab_budget = prm->slab_pairs ? (uint64_t)prm->slab_pairs : (1ull << 31);
+    const uint64_t pair_limit = std::min<uint64_t>(0xFFFFFFFEull, 2 * slab_budget);
+    struct Slab
     {
-        // candidate segments of the pruned row intervals
-        uint64_t blocks_rows = (total_rows + EMIT_ROWS - 1) / EMIT_ROWS;
-        if (blocks_rows > 0x7FFFFFFFull)
-            return ctx->fail("raster_node2: too many AABB rows for one launch");
-        uint64_t seg_total = total_rows + (2 * est_pairs) / EMIT_SEG + 1024;
-        if (seg_total > total_rows + frag_total / EMIT_SEG + 1024)
-            seg_total = total_rows + frag_total / EMIT_SEG + 1024;
-        // partitions receive blocks round robin; a quarter of slack absorbs their imbalance
-        uint64_t part_cap = seg_total / SEG_PARTS + seg_total / (4 * SEG_PARTS) + 4096;
-        uint64_t* seg_meta = (uint64_t*)ctx->ws(WS_SEG_META, (SEG_PARTS * SEG_COUNTER_STRIDE + 2 * (SEG_PARTS + 1)) * 8);
-        if (!seg_meta)
+        int32_t i0, i1;
+        uint64_t est;
+    };
+    std::vector<Slab> todo; // processed from the back
+    if (!any || est_pairs <= slab_budget)
+        todo.push_back(Slab{ INT32_MIN, INT32_MAX, est_pairs });
+    else
+    {
+        const int32_t lmin_x = kl.lmin[0];
+        const uint32_t cols = (uint32_t)((hb[3] >> 3) - lmin_x + 1);
+        unsigned long long* hist = (unsigned long long*)ctx->ws(WS_SLAB_HIST, (size_t)cols * 8);
+        if (!hist)
             return -1;
-        unsigned long long* seg_counters = (unsigned long long*)seg_meta;
-        uint64_t* part_count_dev = seg_meta + SEG_PARTS * SEG_COUNTER_STRIDE;
-        uint64_t* chunk_scan_dev = part_count_dev + (SEG_PARTS + 1);
-        VDBR_CHECK(ctx, hipEventRecord(ev[8], st));
-        for (uint32_t sa = 0;; sa++)
+        VDBR_CHECK(ctx, hipMemsetAsync(hist, 0, (size_t)cols * 8, st));
+        hipLaunchKernelGGL(k_slab_hist, dim3(std::min<uint32_t>(blocks_for(n), 4096u)), dim3(256), 0, st,
+                           (const PrimBox*)boxes, (const uint64_t*)foot, n, lmin_x, cols, hist);
+        VDBR_CHECK(ctx, hipGetLastError());
+        std::vector<unsigned long long> h(cols);
+        VDBR_CHECK(ctx, hipMemcpyAsync(h.data(), hist, (size_t)cols * 8, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipStreamSynchronize(st));
+        double tot = 0.0;
+        for (uint32_t c = 0; c < cols; c++)
+            tot += (double)h[c];
+        const double scale = tot > 0.0 ? (double)est_pairs / tot : 0.0;
+        std::vector<Slab> plan;
+        uint32_t start = 0;
+        double acc = 0.0;
+        for (uint32_t c = 0; c < cols; c++)
         {
-            segs = (uint4*)ctx->ws(WS_SEGS, SEG_PARTS * part_cap * sizeof(uint4));
-            if (!segs)
-                return -1;
-            VDBR_CHECK(ctx, hipMemsetAsync(seg_counters, 0, SEG_PARTS * SEG_COUNTER_STRIDE * 8, st));
-            EmitArgs ra;
-            memset(&ra, 0, sizeof(ra));
-            ra.frames = frames;
-            ra.boxes = boxes;
-            ra.row_scan = row_scan;
-            ra.n = n;
-            ra.total_rows = total_rows;
-            ra.vs = vs;
-            hipLaunchKernelGGL(k_row_segments, dim3((uint32_t)blocks_rows), dim3(EMIT_THREADS), 0, st, ra, segs, part_cap,
-                               seg_counters);
-            VDBR_CHECK(ctx, hipGetLastError());
-            VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, seg_counters, SEG_PARTS * SEG_COUNTER_STRIDE * 8,
-                                           hipMemcpyDeviceToHost, st));
-            VDBR_CHECK(ctx, hipStreamSynchronize(st));
-            uint64_t maxc = 0;
-            seg_count = 0;
-            for (uint32_t q = 0; q < SEG_PARTS; q++)
+            const double w = (double)h[c] * scale;
+            if (c > start && acc + w > (double)slab_budget)
             {
-                uint64_t cq = ctx->pinned[q * SEG_COUNTER_STRIDE];
-                maxc = cq > maxc ? cq : maxc;
-                seg_count += cq;
+                plan.push_back(Slab{ (lmin_x + (int32_t)start) * 8, (lmin_x + (int32_t)c) * 8, (uint64_t)acc });
+                start = c;
+                acc = 0.0;
             }
-            if (maxc <= part_cap)
-                break;
-            if (sa >= 1)
-                return ctx->fail("raster_node2: segment buffer overflow after resize");
-            part_cap = maxc;
+            acc += w;
         }
-        {
-            uint64_t host[2 * (SEG_PARTS + 1)];
-            uint64_t acc = 0;
-            for (uint32_t q = 0; q < SEG_PARTS; q++)
-            {
-                uint64_t cq = ctx->pinned[q * SEG_COUNTER_STRIDE];
-                host[q] = cq;
-                host[SEG_PARTS + 1 + q] = acc;
-                acc += (cq + 63) / 64;
-            }
-            host[SEG_PARTS] = 0;
-            host[2 * SEG_PARTS + 1] = acc;
-            VDBR_CHECK(ctx, hipMemcpy(part_count_dev, host, sizeof(host), hipMemcpyHostToDevice));
-        }
-        uint64_t cap = est_pairs + est_pairs / 4 + 1024;
-        if (cap > frag_total)
-            cap = frag_total;
-        for (;;)
-        {
-            attempts++;
-            pairs = (uint64_t*)ctx->ws(WS_PAIRS, cap * sizeof(uint64_t));
-            if (!pairs)
-                return -1;
-            VDBR_CHECK(ctx, hipMemsetAsync(counter, 0, 8, st));
-            EmitArgs ea;
-            ea.frames = frames;
-            ea.boxes = boxes;
-            ea.vals = pvals;
-            ea.row_scan = row_scan;
-            ea.n = n;
-            ea.total_rows = total_rows;
-            ea.vs = vs;
-            ea.kl = kl;
-            ea.out = pairs;
-            ea.cap = cap;
-            ea.counter = counter;
-            ea.segs = segs;
-            ea.seg_part_cap = part_cap;
-            ea.seg_part_count = part_count_dev;
-            ea.seg_chunk_scan = chunk_scan_dev;
-            // persistent waves: each walks many chunks, so its LDS stage flushes only when full
-            // (one reservation per ~EMIT_WAVE_STAGE pairs)
-            uint64_t chunks = (seg_count + 63) / 64 + SEG_PARTS;
-            uint64_t blocks = (chunks + EMIT_THREADS / 64 - 1) / (EMIT_THREADS / 64);
-            if (blocks > ctx->emit_blocks)
-                blocks = ctx->emit_blocks;
-            if (blocks == 0)
-                blocks = 1;
-            hipLaunchKernelGGL(k_emit_segments, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, ea);
-            VDBR_CHECK(ctx, hipGetLastError());
-            VDBR_CHECK(ctx, hipEventRecord(ev[9], st));
-            VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, counter, 8, hipMemcpyDeviceToHost, st));
-            VDBR_CHECK(ctx, hipStreamSynchronize(st));
-            pair_count = ctx->pinned[0];
-            if (pair_count <= cap)
-                break;
-            if (attempts >= 2)
-                return ctx->fail("raster_node2: pair buffer overflow after resize");
-            cap = pair_count;
-        }
+        plan.push_back(Slab{ (lmin_x + (int32_t)start) * 8, (lmin_x + (int32_t)cols) * 8, (uint64_t)acc });
+        plan.front().i0 = INT32_MIN; // every voxel belongs to exactly one slab
+        plan.back().i1 = INT32_MAX;
+        todo.assign(plan.rbegin(), plan.rend());
     }
-    VDBR_CHECK(ctx, hipEventRecord(ev[2], st));
 
-    // ---------------------------------------------------------------- sort
-    uint64_t* sorted = pairs;
-    if (pair_count > 1)
+    // ---------------------------------------------------------------- emit + sort + reduce per slab
+    const bool rgba_wave = kl.alpha_bits != 0u;
+    uint64_t pair_count = 0; // all slabs
+    uint64_t nvox = 0;       // all slabs
+    uint64_t vcap = 0;       // capacity of the voxel arrays
+    uint64_t* vox_key = nullptr;
+    uint32_t* vox_val = nullptr;
+    uint32_t* vox_t = nullptr;
+    uint32_t* vox_g0 = nullptr;
+    uint32_t attempts = 0;
+    uint32_t slab_count = 0;
+    float acc_emit = 0.f, acc_sort = 0.f, acc_reduce = 0.f, acc_passes = 0.f, acc_emit_k = 0.f, acc_reduce_k = 0.f;
+    uint64_t* last_sorted = nullptr;
+    uint64_t last_slab_pairs = 0;
+    // grows one voxel array, keeping its first `keep` bytes
+    auto grow_keep = [&](WsSlot slot, size_t keep, size_t bytes) -> void* {
+        if (ctx->ws_size[slot] >= bytes && ctx->ws_ptr[slot])
+            return ctx->ws_ptr[slot];
+        void* tmp = nullptr;
+        if (keep)
+        {
+            tmp = ctx->ws(WS_VOX_GROW, keep);
+            if (!tmp || hipMemcpyAsync(tmp, ctx->ws_ptr[slot], keep, hipMemcpyDeviceToDevice, st) != hipSuccess)
+                return nullptr;
+        }
+        void* p = ctx->ws(slot, bytes);
+        if (p && keep && hipMemcpyAsync(p, tmp, keep, hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return nullptr;
+        return p;
+    };
+    while (any && !todo.empty())
     {
-        if (radix_sort_u64_noswap(ctx, pairs, pair_count, sort_begin, key_bits, &sorted, ev[6], ev[7], &sort_passes))
+        const Slab sl = todo.back();
+        todo.pop_back();
+        const bool whole = sl.i0 == INT32_MIN && sl.i1 == INT32_MAX;
+        VDBR_CHECK(ctx, hipEventRecord(ev[12], st));
+        // rows of this slab
+        const uint64_t* srow_scan = row_scan;
+        uint64_t srows = total_rows;
+        if (!whole)
+        {
+            uint64_t* rows_s = (uint64_t*)ctx->ws(WS_SLAB_ROWS, 2 * (nn + 1) * sizeof(uint64_t));
+            if (!rows_s)
+                return -1;
+            uint64_t* scan_s = rows_s + (nn + 1);
+            hipLaunchKernelGGL(k_slab_rows, dim3(std::min<uint32_t>(blocks_for(n), 4096u)), dim3(256), 0, st,
+                               (const PrimBox*)boxes, n, sl.i0, sl.i1, rows_s);
+            VDBR_CHECK(ctx, hipGetLastError());
+            if (scan_u64(ctx, rows_s, scan_s, n, false, scan_s + n))
+                return -1;
+            VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, scan_s + n, 8, hipMemcpyDeviceToHost, st));
+            VDBR_CHECK(ctx, hipStreamSynchronize(st));
+            srow_scan = scan_s;
+            srows = ctx->pinned[0];
+        }
+        const uint64_t sest = whole ? est_pairs : sl.est;
+        // a slab can be split when it spans more than one leaf column of the voxel bounds
+        const int64_t span_lo = whole ? (int64_t)hb[0] : std::max<int64_t>((int64_t)sl.i0, (int64_t)hb[0]);
+        const int64_t span_hi = whole ? (int64_t)hb[3] + 1 : std::min<int64_t>((int64_t)sl.i1, (int64_t)hb[3] + 1);
+        const int64_t split_at = ((span_lo + span_hi) / 2) & ~7ll;
+        const bool splittable = split_at > span_lo && split_at < span_hi;
+        uint64_t spairs = 0;
+        uint64_t* pairs = nullptr;
+        uint4* segs = nullptr;
+        uint64_t seg_count = 0;
+        if (srows > 0)
+        {
+            // candidate segments of the pruned row intervals
+            uint64_t blocks_rows = (srows + EMIT_ROWS - 1) / EMIT_ROWS;
+            if (blocks_rows > 0x7FFFFFFFull)
+                return ctx->fail("raster_node2: too many AABB rows for one launch");
+            uint64_t seg_total = srows + (2 * sest) / EMIT_SEG + 1024;
+            if (seg_total > srows + frag_total / EMIT_SEG + 1024)
+                seg_total = srows + frag_total / EMIT_SEG + 1024;
+            // partitions receive blocks round robin; a quarter of slack absorbs their imbalance
+            uint64_t part_cap = seg_total / SEG_PARTS + seg_total / (4 * SEG_PARTS) + 4096;
+            uint64_t* seg_meta =
+                (uint64_t*)ctx->ws(WS_SEG_META, (SEG_PARTS * SEG_COUNTER_STRIDE + 2 * (SEG_PARTS + 1)) * 8);
+            if (!seg_meta)
+                return -1;
+            unsigned long long* seg_counters = (unsigned long long*)seg_meta;
+            uint64_t* part_count_dev = seg_meta + SEG_PARTS * SEG_COUNTER_STRIDE;
+            uint64_t* chunk_scan_dev = part_count_dev + (SEG_PARTS + 1);
+            VDBR_CHECK(ctx, hipEventRecord(ev[8], st));
+            for (uint32_t sa = 0;; sa++)
+            {
+                segs = (uint4*)ctx->ws(WS_SEGS, SEG_PARTS * part_cap * sizeof(uint4));
+                if (!segs)
+                    return -1;
+                VDBR_CHECK(ctx, hipMemsetAsync(seg_counters, 0, SEG_PARTS * SEG_COUNTER_STRIDE * 8, st));
+                EmitArgs ra;
+                memset(&ra, 0, sizeof(ra));
+                ra.frames = frames;
+                ra.boxes = boxes;
+                ra.row_scan = srow_scan;
+                ra.n = n;
+                ra.total_rows = srows;
+                ra.vs = vs;
+                ra.slab_i0 = whole ? INT32_MIN : sl.i0;
+                hipLaunchKernelGGL(k_row_segments, dim3((uint32_t)blocks_rows), dim3(EMIT_THREADS), 0, st, ra, segs,
+                                   part_cap, seg_counters);
+                VDBR_CHECK(ctx, hipGetLastError());
+                VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, seg_counters, SEG_PARTS * SEG_COUNTER_STRIDE * 8,
+                                               hipMemcpyDeviceToHost, st));
+                VDBR_CHECK(ctx, hipStreamSynchronize(st));
+                uint64_t maxc = 0;
+                seg_count = 0;
+                for (uint32_t q = 0; q < SEG_PARTS; q++)
+                {
+                    uint64_t cq = ctx->pinned[q * SEG_COUNTER_STRIDE];
+                    maxc = cq > maxc ? cq : maxc;
+                    seg_count += cq;
+                }
+                if (maxc <= part_cap)
+                    break;
+                if (sa >= 1)
+                    return ctx->fail("raster_node2: segment buffer overflow after resize");
+                part_cap = maxc;
+            }
+            {
+                uint64_t host[2 * (SEG_PARTS + 1)];
+                uint64_t acc = 0;
+                for (uint32_t q = 0; q < SEG_PARTS; q++)
+                {
+                    uint64_t cq = ctx->pinned[q * SEG_COUNTER_STRIDE];
+                    host[q] = cq;
+                    host[SEG_PARTS + 1 + q] = acc;
+                    acc += (cq + 63) / 64;
+                }
+                host[SEG_PARTS] = 0;
+                host[2 * SEG_PARTS + 1] = acc;
+                VDBR_CHECK(ctx, hipMemcpy(part_count_dev, host, sizeof(host), hipMemcpyHostToDevice));
+            }
+            uint64_t cap = sest + sest / 4 + 1024;
+            if (cap > frag_total)
+                cap = frag_total;
+            if (cap > pair_limit + 1)
+                cap = pair_limit + 1;
+            uint32_t sattempts = 0;
+            for (;;)
+            {
+                sattempts++;
+                pairs = (uint64_t*)ctx->ws(WS_PAIRS, cap * sizeof(uint64_t));
+                if (!pairs)
+                    return -1;
+                VDBR_CHECK(ctx, hipMemsetAsync(counter, 0, 8, st));
+                EmitArgs ea;
+                memset(&ea, 0, sizeof(ea));
+                ea.frames = frames;
+                ea.boxes = boxes;
+                ea.vals = pvals;
+                ea.row_scan = srow_scan;
+                ea.n = n;
+                ea.total_rows = srows;
+                ea.vs = vs;
+                ea.kl = kl;
+                ea.out = pairs;
+                ea.cap = cap;
+                ea.counter = counter;
+                ea.segs = segs;
+                ea.seg_part_cap = part_cap;
+                ea.seg_part_count = part_count_dev;
+                ea.seg_chunk_scan = chunk_scan_dev;
+                ea.slab_i0 = whole ? INT32_MIN : sl.i0;
+                // persistent waves: each walks many chunks, so its LDS stage flushes only when full
+                // (one reservation per ~EMIT_WAVE_STAGE pairs)
+                uint64_t chunks = (seg_count + 63) / 64 + SEG_PARTS;
+                uint64_t blocks = (chunks + EMIT_THREADS / 64 - 1) / (EMIT_THREADS / 64);
+                if (blocks > ctx->emit_blocks)
+                    blocks = ctx->emit_blocks;
+                if (blocks == 0)
+                    blocks = 1;
+                hipLaunchKernelGGL(k_emit_segments, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, ea);
+                VDBR_CHECK(ctx, hipGetLastError());
+                VDBR_CHECK(ctx, hipEventRecord(ev[9], st));
+                VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, counter, 8, hipMemcpyDeviceToHost, st));
+                VDBR_CHECK(ctx, hipStreamSynchronize(st));
+                spairs = ctx->pinned[0];
+                if (spairs <= cap)
+                    break;
+                if (spairs > pair_limit && splittable)
+                    break; // the slab is split below
+                if (spairs > 0xFFFFFFFEull)
+                    return ctx->fail("raster_node2: more than 2^32-2 pairs in one leaf column");
+                if (sattempts >= 2)
+                    return ctx->fail("raster_node2: pair buffer overflow after resize");
+                cap = spairs;
+            }
+            attempts = std::max(attempts, sattempts);
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[8], ev[9]) == hipSuccess)
+                acc_emit_k += ms;
+        }
+        if (spairs > pair_limit && splittable)
+        {
+            // too many pairs for one slab: split at a leaf column boundary and redo both halves
+            todo.push_back(Slab{ (int32_t)split_at, sl.i1, spairs / 2 + 1 });
+            todo.push_back(Slab{ sl.i0, (int32_t)split_at, spairs / 2 + 1 });
+            continue;
+        }
+        slab_count++;
+        pair_count += spairs;
+        VDBR_CHECK(ctx, hipEventRecord(ev[13], st));
+
+        // sort
+        uint64_t* sorted = pairs;
+        uint32_t passes = 0;
+        if (spairs > 1)
+        {
+            if (radix_sort_u64_noswap(ctx, pairs, spairs, sort_begin, key_bits, &sorted, ev[6], ev[7], &passes))
+                return -1;
+            sort_passes = passes;
+        }
+        VDBR_CHECK(ctx, hipEventRecord(ev[14], st));
+        last_sorted = sorted;
+        last_slab_pairs = spairs;
+
+        // reduce per voxel, appended after the voxels of the previous slabs
+        uint32_t* seg = rgba_wave ? nullptr : (uint32_t*)ctx->ws(WS_SEG, (spairs + 2) * sizeof(uint32_t));
+        if (!rgba_wave && !seg)
+            return -1;
+        uint64_t svox = 0;
+        const uint32_t* tile_offsets = nullptr;
+        if (run_heads(ctx, sorted, spairs, kl.gid_bits + kl.alpha_bits, seg, &svox, &tile_offsets))
+            return -1;
+        if (nvox + svox > vcap || !vox_key)
+        {
+            const uint64_t want = whole ? std::max<uint64_t>(svox, 1) : std::max<uint64_t>(nvox + svox, vcap + vcap / 2);
+            vox_key = (uint64_t*)grow_keep(WS_VOX_KEY, nvox * sizeof(uint64_t), want * sizeof(uint64_t));
+            vox_val = (uint32_t*)grow_keep(WS_VOX_VAL, nvox * sizeof(uint32_t), want * sizeof(uint32_t));
+            vox_t = (uint32_t*)grow_keep(WS_VOX_T, nvox * sizeof(uint32_t), want * sizeof(uint32_t));
+            vox_g0 = (uint32_t*)grow_keep(WS_VOX_G0, nvox * sizeof(uint32_t), want * sizeof(uint32_t));
+            if (!vox_key || !vox_val || !vox_t || !vox_g0)
+                return -1;
+            vcap = want;
+        }
+        uint64_t* skey = vox_key + nvox;
+        uint32_t* sval = vox_val + nvox;
+        uint32_t* st_t = vox_t + nvox;
+        uint32_t* sg0 = vox_g0 + nvox;
+        if (svox > 0 && rgba_wave)
+        {
+            VDBR_CHECK(ctx, hipMemsetAsync(sval, 0, svox * sizeof(uint32_t), st));
+            VDBR_CHECK(ctx, hipMemsetAsync(sg0, 0xFF, svox * sizeof(uint32_t), st));
+            uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
+            VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
+            hipLaunchKernelGGL(k_reduce_rgba, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, st, (const uint64_t*)sorted,
+                               spairs, kl.gid_bits + kl.alpha_bits, kl.gid_bits, tile_offsets, (const PrimVal*)pvals,
+                               skey, sval, sg0);
+            VDBR_CHECK(ctx, hipGetLastError());
+            VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
+            hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(svox), 2048u)), dim3(256), 0, st, svox,
+                               kl, (const uint64_t*)skey, (const uint32_t*)sg0, (const PrimBox*)boxes,
+                               (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds);
+            VDBR_CHECK(ctx, hipGetLastError());
+        }
+        else if (svox > 0)
+        {
+            ReduceArgs ra;
+            ra.keys = sorted;
+            ra.seg = seg;
+            ra.nvox = svox;
+            ra.kl = kl;
+            ra.frames = frames;
+            ra.vals = pvals;
+            ra.boxes = boxes;
+            ra.foot_scan = foot_scan;
+            ra.win_base = win_base_dev;
+            ra.prim_batch = PB;
+            ra.ijk_batch = W;
+            ra.value_mode = prm->value_mode;
+            ra.vs = vs;
+            ra.vox_key = skey;
+            ra.vox_val = sval;
+            ra.vox_t = st_t;
+            ra.bounds = vbounds;
+            VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
+            hipLaunchKernelGGL(k_reduce, dim3(blocks_for(svox)), dim3(256), 0, st, ra);
+            VDBR_CHECK(ctx, hipGetLastError());
+            VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
+        }
+        VDBR_CHECK(ctx, hipEventRecord(ev[15], st));
+        VDBR_CHECK(ctx, hipEventSynchronize(ev[15]));
+        nvox += svox;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, ev[12], ev[13]) == hipSuccess)
+            acc_emit += ms;
+        if (hipEventElapsedTime(&ms, ev[13], ev[14]) == hipSuccess)
+            acc_sort += ms;
+        if (hipEventElapsedTime(&ms, ev[14], ev[15]) == hipSuccess)
+            acc_reduce += ms;
+        if (passes && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess)
+            acc_passes += ms;
+        if (svox && hipEventElapsedTime(&ms, ev[10], ev[11]) == hipSuccess)
+            acc_reduce_k += ms;
+    }
+    if (nvox >= 0xFFFFFFFFull)
+        return ctx->fail("raster_node2: more than 2^32-1 active voxels");
+    const uint64_t nv1 = nvox > 0 ? nvox : 1;
+    if (!vox_key)
+    {
+        vox_key = (uint64_t*)ctx->ws(WS_VOX_KEY, nv1 * sizeof(uint64_t));
+        vox_val = (uint32_t*)ctx->ws(WS_VOX_VAL, nv1 * sizeof(uint32_t));
+        vox_t = (uint32_t*)ctx->ws(WS_VOX_T, nv1 * sizeof(uint32_t));
+        if (!vox_key || !vox_val || !vox_t)
             return -1;
     }
-    VDBR_CHECK(ctx, hipEventRecord(ev[3], st));
-    ctx->last_pairs = sorted;
-    ctx->last_pair_count = pair_count;
+    uint32_t* vox_leaf = (uint32_t*)ctx->ws(WS_VOX_LEAF, nv1 * sizeof(uint32_t));
+    if (!vox_leaf)
+        return -1;
+    // the pair list of a single-slab build stays available (vdbr_last_pairs)
+    ctx->last_pairs = slab_count == 1 ? last_sorted : nullptr;
+    ctx->last_pair_count = slab_count == 1 ? last_slab_pairs : 0;
     ctx->last_gid_bits = kl.gid_bits;
     ctx->last_alpha_bits = kl.alpha_bits;
     for (int a = 0; a < 3; a++)
     {
         ctx->last_lmin[a] = kl.lmin[a];
         ctx->last_bits[a] = kl.bits[a];
-    }
-
-    // ---------------------------------------------------------------- reduce per voxel
-    if (pair_count >= 0xFFFFFFFFull)
-        return ctx->fail("raster_node2: more than 2^32-1 pairs in one build");
-    const bool rgba_wave = kl.alpha_bits != 0u;
-    uint32_t* seg = rgba_wave ? nullptr : (uint32_t*)ctx->ws(WS_SEG, (pair_count + 2) * sizeof(uint32_t));
-    uint64_t nvox = 0;
-    const uint32_t* tile_offsets = nullptr;
-    if (run_heads(ctx, sorted, pair_count, kl.gid_bits + kl.alpha_bits, seg, &nvox, &tile_offsets))
-        return -1;
-    const uint64_t nv1 = nvox > 0 ? nvox : 1;
-    uint64_t* vox_key = (uint64_t*)ctx->ws(WS_VOX_KEY, nv1 * sizeof(uint64_t));
-    uint32_t* vox_val = (uint32_t*)ctx->ws(WS_VOX_VAL, nv1 * sizeof(uint32_t));
-    uint32_t* vox_t = (uint32_t*)ctx->ws(WS_VOX_T, 3 * nv1 * sizeof(uint32_t));
-    uint32_t* vox_leaf = vox_t + nv1;
-    uint32_t* vox_g0 = vox_t + 2 * nv1;
-    if (!vox_key || !vox_val || !vox_t)
-        return -1;
-    if (nvox > 0 && rgba_wave)
-    {
-        VDBR_CHECK(ctx, hipMemsetAsync(vox_val, 0, nvox * sizeof(uint32_t), st));
-        VDBR_CHECK(ctx, hipMemsetAsync(vox_g0, 0xFF, nvox * sizeof(uint32_t), st));
-        uint64_t tiles = (pair_count + HEAD_TILE - 1) / HEAD_TILE;
-        VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
-        hipLaunchKernelGGL(k_reduce_rgba, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, st, (const uint64_t*)sorted,
-                           pair_count, kl.gid_bits + kl.alpha_bits, kl.gid_bits, tile_offsets, (const PrimVal*)pvals,
-                           vox_key, vox_val, vox_g0);
-        VDBR_CHECK(ctx, hipGetLastError());
-        VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
-        hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(nvox), 2048u)), dim3(256), 0, st, nvox, kl, (const uint64_t*)vox_key,
-                           (const uint32_t*)vox_g0, (const PrimBox*)boxes, (const uint64_t*)foot_scan,
-                           (const uint32_t*)win_base_dev, PB, W, vox_t, vbounds);
-        VDBR_CHECK(ctx, hipGetLastError());
-    }
-    else if (nvox > 0)
-    {
-        ReduceArgs ra;
-        ra.keys = sorted;
-        ra.seg = seg;
-        ra.nvox = nvox;
-        ra.kl = kl;
-        ra.frames = frames;
-        ra.vals = pvals;
-        ra.boxes = boxes;
-        ra.foot_scan = foot_scan;
-        ra.win_base = win_base_dev;
-        ra.prim_batch = PB;
-        ra.ijk_batch = W;
-        ra.value_mode = prm->value_mode;
-        ra.vs = vs;
-        ra.vox_key = vox_key;
-        ra.vox_val = vox_val;
-        ra.vox_t = vox_t;
-        ra.bounds = vbounds;
-        VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
-        hipLaunchKernelGGL(k_reduce, dim3(blocks_for(nvox)), dim3(256), 0, st, ra);
-        VDBR_CHECK(ctx, hipGetLastError());
-        VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
     }
     VDBR_CHECK(ctx, hipEventRecord(ev[4], st));
 
@@ -2548,11 +2773,24 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nleaf)), dim3(256), 0, st, grid, gi, nleaf,
                            (const uint32_t*)lvals, (const uint64_t*)leaf_mask, (const uint32_t*)leaf_node_idx,
                            (const uint64_t*)size_scan, (const uint64_t*)vals_scan);
+    int32_t* ijk_out = nullptr;
+    ctx->last_voxel_ijk = nullptr;
+    ctx->last_value_count = 0;
+    if (prm->flags & VDBR_FLAG_VOXEL_COORDS)
+    {
+        ijk_out = (int32_t*)ctx->ws(WS_VOX_IJK, 3 * (2 + nvox) * sizeof(int32_t));
+        if (!ijk_out)
+            return -1;
+        // the two background slots have no voxel
+        VDBR_CHECK(ctx, hipMemsetAsync(ijk_out, 0x80, 6 * sizeof(int32_t), st));
+        ctx->last_voxel_ijk = ijk_out;
+        ctx->last_value_count = 2 + nvox;
+    }
     if (nvox)
         hipLaunchKernelGGL(k_write_values, dim3(blocks_for(nvox)), dim3(256), 0, st, grid, gi, nvox,
                            (const uint32_t*)vox_val, (const uint64_t*)vox_key, (const uint32_t*)leaf_first,
                            (const uint32_t*)nullptr, (const uint32_t*)leaf_pos, (const uint32_t*)leaf_node_idx,
-                           (const uint64_t*)vals_scan, (const uint32_t*)vox_leaf);
+                           (const uint64_t*)vals_scan, (const uint32_t*)vox_leaf, kl, ijk_out);
     VDBR_CHECK(ctx, hipGetLastError());
     VDBR_CHECK(ctx, hipEventRecord(ev[5], st));
     VDBR_CHECK(ctx, hipEventSynchronize(ev[5]));
@@ -2574,26 +2812,22 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         stats->key_bits = key_bits;
         stats->sort_bit_begin = sort_begin;
         stats->emit_attempts = attempts;
+        stats->slab_count = slab_count;
         float ms;
         (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
         stats->ms_setup = ms;
-        (void)hipEventElapsedTime(&ms, ev[1], ev[2]);
-        stats->ms_emit = ms;
-        (void)hipEventElapsedTime(&ms, ev[2], ev[3]);
-        stats->ms_sort = ms;
-        (void)hipEventElapsedTime(&ms, ev[3], ev[4]);
-        stats->ms_reduce = ms;
         (void)hipEventElapsedTime(&ms, ev[4], ev[5]);
         stats->ms_build = ms;
         (void)hipEventElapsedTime(&ms, ev[0], ev[5]);
         stats->ms_total = ms;
+        // per-stage sums over the slabs
+        stats->ms_emit = acc_emit;
+        stats->ms_sort = acc_sort;
+        stats->ms_reduce = acc_reduce;
         stats->sort_passes = sort_passes;
-        if (sort_passes && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess)
-            stats->ms_sort_passes = ms;
-        if (pair_count && hipEventElapsedTime(&ms, ev[8], ev[9]) == hipSuccess)
-            stats->ms_emit_kernel = ms;
-        if (nvox && hipEventElapsedTime(&ms, ev[10], ev[11]) == hipSuccess)
-            stats->ms_reduce_kernel = ms;
+        stats->ms_sort_passes = acc_passes;
+        stats->ms_emit_kernel = acc_emit_k;
+        stats->ms_reduce_kernel = acc_reduce_k;
         (void)hipGetLastError();
     }
     return 0;
@@ -2636,6 +2870,57 @@ int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, 
     return 0;
 }
 
+
+int last_voxels(Context* ctx, int32_t* ijk_dev, uint64_t cap, uint64_t* count)
+{
+    if (count)
+        *count = ctx->last_value_count;
+    if (!ijk_dev || !ctx->last_voxel_ijk)
+        return 0;
+    const uint64_t n = ctx->last_value_count < cap ? ctx->last_value_count : cap;
+    if (n)
+        VDBR_CHECK(ctx, hipMemcpyAsync(ijk_dev, ctx->last_voxel_ijk, 3 * n * sizeof(int32_t), hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// value words of a node2 grid at voxel coordinates (0 where the voxel is inactive or outside)
+__global__ void k_node2_sample(const uint64_t* __restrict__ w, const uint32_t* __restrict__ values, uint64_t value_count,
+                               const int32_t* __restrict__ ijk, uint64_t n, uint32_t* __restrict__ out)
+{
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const int32_t i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
+        uint32_t v = 0u;
+        const bool in_range = i >= -65536 && i < 65536 && j >= -65536 && j < 65536 && k >= -65536 && k < 65536;
+        const uint64_t leaf = in_range ? n2_find_leaf(w, i, j, k) : 0ull;
+        if (leaf)
+        {
+            const uint64_t idx = n2_leaf_value_index(w, leaf, i, j, k);
+            if (idx < value_count)
+                v = values[idx];
+        }
+        out[q] = v;
+    }
+}
+
+int node2_sample(Context* ctx, const void* grid, uint64_t grid_size, const uint32_t* values, uint64_t value_count,
+                 const int32_t* ijk, uint64_t n, uint32_t* out)
+{
+    if (!grid || !values || (n && (!ijk || !out)))
+        return ctx->fail("node2_sample: null argument");
+    if (grid_size < VDB_ROOT_ADDR + VDB_SIZE_ROOT)
+        return ctx->fail("node2_sample: grid is not a node2 grid");
+    if (n == 0)
+        return 0;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_node2_sample, dim3(blocks), dim3(256), 0, ctx->stream, (const uint64_t*)grid, values,
+                       value_count, ijk, n, out);
+    VDBR_CHECK(ctx, hipGetLastError());
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+}
 
 int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
                      const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,

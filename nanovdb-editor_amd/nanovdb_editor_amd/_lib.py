@@ -29,7 +29,7 @@ class VdbrTuneResult(ctypes.Structure):
 
 class VdbrRasterParams(ctypes.Structure):
     _fields_ = [("voxel_size", f32), ("cull_scale", f32), ("value_mode", u32), ("flags", u32),
-                ("prim_batch_size", u32), ("ijk_batch_size", u32), ("max_node_count", u32), ("reserved", u32)]
+                ("prim_batch_size", u32), ("ijk_batch_size", u32), ("max_node_count", u32), ("slab_pairs", u32)]
 
 
 class VdbrRasterStats(ctypes.Structure):
@@ -38,7 +38,7 @@ class VdbrRasterStats(ctypes.Structure):
                 ("window_count", u32), ("key_bits", u32), ("emit_attempts", u32), ("ms_setup", f32),
                 ("ms_emit", f32), ("ms_sort", f32), ("ms_reduce", f32), ("ms_build", f32), ("ms_total", f32),
                 ("sort_passes", u32), ("ms_sort_passes", f32), ("ms_emit_kernel", f32), ("ms_reduce_kernel", f32),
-                ("sort_bit_begin", u32), ("reserved_stats", u32)]
+                ("sort_bit_begin", u32), ("slab_count", u32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -82,6 +82,9 @@ SIGNATURES = {
     "vdbr_density_backward": ([P, ctypes.POINTER(VdbrGaussians), f32, P, u64, P, u64,
                                ctypes.POINTER(VdbrGaussianGrads)], ctypes.c_int),
     "vdbr_selftest_alpha8": ([P, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
+    "vdbr_context_trim": ([P], ctypes.c_int),
+    "vdbr_last_voxels": ([P, P, u64, ctypes.POINTER(u64)], ctypes.c_int),
+    "vdbr_node2_sample": ([P, P, u64, P, u64, P, u64, P], ctypes.c_int),
     "vdbr_voxelbvh_ijkl": ([P, ctypes.POINTER(VdbrGaussians), u32, P, P, P, P, P], ctypes.c_int),
     "vdbr_tile_bin": ([P, P, P, P, u32, u32, u32, u32, P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.POINTER(P),
                        ctypes.POINTER(u64)], ctypes.c_int),

@@ -29,7 +29,7 @@ def density_value_offset(grid, value_count):
 
 class DensityVoxelize(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, means, log_scales, quats, opacity_logits, voxel_size, vox):
+    def forward(ctx, means, log_scales, quats, opacity_logits, voxel_size, vox, with_coords=False):
         m = means.detach().contiguous()
         work_q = quats.detach().contiguous().clone()
         work_s = log_scales.detach().contiguous().clone()
@@ -37,7 +37,7 @@ class DensityVoxelize(torch.autograd.Function):
         sh0 = torch.zeros_like(m)
         colors = torch.empty_like(m)
         grid, st = vox.raster_node2(m, work_q, work_s, colors, work_o, voxel_size, value_mode=VALUE_DENSITY, raw=True,
-                                    sh0=sh0)
+                                    sh0=sh0, voxel_coords=with_coords)
         count = st["voxel_count"] + 2
         off = density_value_offset(grid, count)
         values = grid[off:].view(torch.float32).clone()
@@ -45,23 +45,28 @@ class DensityVoxelize(torch.autograd.Function):
         ctx.voxel_size = float(voxel_size)
         ctx.vox = vox
         ctx.mark_non_differentiable(grid)
+        if with_coords:
+            ijk = vox.last_voxels()
+            ctx.mark_non_differentiable(ijk)
+            return values, grid, ijk
         return values, grid
 
     @staticmethod
-    def backward(ctx, grad_values, grad_grid):
+    def backward(ctx, grad_values, grad_grid, grad_ijk=None):
         means, log_scales, quats, opacity_logits, grid = ctx.saved_tensors
         if grad_values is None:
-            return None, None, None, None, None, None
+            return None, None, None, None, None, None, None
         gm, gs, gq, go = ctx.vox.density_backward(
             means.detach().contiguous(), log_scales.detach().contiguous(), quats.detach().contiguous(),
             opacity_logits.detach().contiguous(), ctx.voxel_size, grid, grad_values.contiguous().float())
-        return gm, gs, gq, go, None, None
+        return gm, gs, gq, go, None, None, None
 
 
-def voxelize_density(means, log_scales, quats, opacity_logits, voxel_size, vox=None):
-    """(values float32 [V + 2], node2 grid uint8) with values differentiable w.r.t. the inputs.
+def voxelize_density(means, log_scales, quats, opacity_logits, voxel_size, vox=None, with_coords=False):
+    """(values float32 [V + 2], node2 grid uint8) with values differentiable w.r.t. the inputs;
+    with ``with_coords=True`` also the int32 [V + 2, 3] voxel coordinates of the value slots.
 
     Inputs are flat or shaped float32 CUDA tensors: means [N, 3], log_scales [N, 3], quats [N, 4]
     (w, x, y, z, any norm), opacity_logits [N]."""
     vox = vox or _voxelizer(means.device)
-    return DensityVoxelize.apply(means, log_scales, quats, opacity_logits, voxel_size, vox)
+    return DensityVoxelize.apply(means, log_scales, quats, opacity_logits, voxel_size, vox, with_coords)

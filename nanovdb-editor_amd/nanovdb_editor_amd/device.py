@@ -14,6 +14,7 @@ VALUE_RGBA8 = 0
 VALUE_DENSITY = 1
 FLAG_RAW_INPUTS = 1
 FLAG_SPATIAL_ORDER = 2
+FLAG_VOXEL_COORDS = 4
 
 
 def _ptr(t):
@@ -60,12 +61,13 @@ class Voxelizer:
 
     def raster_node2(self, means, quats, scales, colors, opacities, voxel_size, *, value_mode=VALUE_RGBA8,
                      cull_scale=0.0, raw=False, sh0=None, prim_batch_size=0, ijk_batch_size=0, max_node_count=0,
-                     spatial_order=False, copy_out=True):
+                     spatial_order=False, copy_out=True, voxel_coords=False, slab_pairs=0):
         """Gaussians (float32 GPU tensors, reference layouts) -> node2 grid.
 
         With ``raw=True`` the arrays hold opacity logits, unnormalised (w,x,y,z) quaternions and
         log-scales and are preprocessed in place (colors receives 0.28209479*sh0 + 0.5 from
-        ``sh0``). Returns ``(grid_uint8_tensor or None, stats_dict)``.
+        ``sh0``). With ``voxel_coords=True`` the call also records every value slot's voxel
+        (``last_voxels``). Returns ``(grid_uint8_tensor or None, stats_dict)``.
         """
         n = means.numel() // 3
         _check_f32("means", means, 3 * n)
@@ -80,8 +82,9 @@ class Voxelizer:
         g = VdbrGaussians(_ptr(means), _ptr(opacities), _ptr(quats), _ptr(scales), _ptr(colors),
                           _ptr(sh0), n)
         p = VdbrRasterParams(float(voxel_size), float(cull_scale), int(value_mode),
-                             (FLAG_RAW_INPUTS if raw else 0) | (FLAG_SPATIAL_ORDER if spatial_order else 0),
-                             int(prim_batch_size), int(ijk_batch_size), int(max_node_count), 0)
+                             (FLAG_RAW_INPUTS if raw else 0) | (FLAG_SPATIAL_ORDER if spatial_order else 0)
+                             | (FLAG_VOXEL_COORDS if voxel_coords else 0),
+                             int(prim_batch_size), int(ijk_batch_size), int(max_node_count), int(slab_pairs))
         grid_dev = ctypes.c_void_p()
         size = ctypes.c_uint64()
         st = VdbrRasterStats()
@@ -200,6 +203,32 @@ class Voxelizer:
         if self._lib.vdbr_set_sort_config(self._ctx, int(from_count), int(variant), int(digit_bits)) != 0:
             self._err("vdbr_set_sort_config")
 
+    def last_voxels(self):
+        """int32 [value_count, 3] voxel coordinates of the last ``raster_node2(voxel_coords=True)``
+        call in value-index order (rows 0 and 1, the background slots, hold sentinels)."""
+        cnt = ctypes.c_uint64()
+        self._lib.vdbr_last_voxels(self._ctx, None, 0, ctypes.byref(cnt))
+        out = torch.empty((cnt.value, 3), dtype=torch.int32, device=self.device)
+        if cnt.value and self._lib.vdbr_last_voxels(self._ctx, _ptr(out), cnt.value, ctypes.byref(cnt)) != 0:
+            self._err("vdbr_last_voxels")
+        return out
+
+    def node2_sample(self, grid, values, ijk):
+        """Value words (same dtype as ``values``, 32-bit) of a node2 grid at int32 voxel coordinates
+        [M, 3]; 0 where the voxel is inactive."""
+        if grid.dtype != torch.uint8 or not grid.is_cuda or not grid.is_contiguous():
+            raise ValueError("grid must be a contiguous uint8 GPU tensor")
+        if values.element_size() != 4 or not values.is_cuda or not values.is_contiguous():
+            raise ValueError("values must be a contiguous 32-bit GPU tensor")
+        if ijk.dtype != torch.int32 or not ijk.is_cuda or not ijk.is_contiguous() or ijk.numel() % 3:
+            raise ValueError("ijk must be a contiguous int32 GPU tensor of shape [M, 3]")
+        out = torch.empty(ijk.numel() // 3, dtype=values.dtype, device=self.device)
+        self._bind_stream()
+        if self._lib.vdbr_node2_sample(self._ctx, _ptr(grid), grid.numel(), _ptr(values), values.numel(), _ptr(ijk),
+                                       ijk.numel() // 3, _ptr(out)) != 0:
+            self._err("vdbr_node2_sample")
+        return out
+
     def last_pairs(self):
         """Sorted pair list of the last raster call: (ijk int32 [P,3], gid int32 [P]) on the GPU."""
         cnt = ctypes.c_uint64()
@@ -307,6 +336,10 @@ class Voxelizer:
         if self._lib.vdbr_inclusive_scan(self._ctx, _ptr(values), _ptr(out), values.numel(), code) != 0:
             self._err("vdbr_inclusive_scan")
         return out
+
+    def trim(self):
+        """Frees the context's device workspace (results of earlier calls become invalid)."""
+        self._lib.vdbr_context_trim(self._ctx)
 
     def workspace_bytes(self):
         return int(self._lib.vdbr_workspace_bytes(self._ctx))
