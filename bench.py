@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=2,
                     help="timed calls of the 256-cloud batched entry point (0 skips it)")
     ap.add_argument("--batch-workers", type=int, default=4)
+    ap.add_argument("--big-steps", type=int, default=1,
+                    help="timed calls of the 16M-Gaussian slabbed workload (0 skips it)")
     ap.add_argument("--fit-steps", type=int, default=1,
                     help="timed fit steps (forward + L2 + backward) of the 1M-Gaussian fit workload (0 skips it)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
@@ -171,6 +173,43 @@ def fit_1m(vox, steps):
                         "fit seed 31 against the target grid of seed 32, L2 over the fit voxels",
             "gaussians": fit.count, "pairs": st["pair_count"], "voxels": stats["voxels"], "target_voxels": tcount - 2,
             "forward_ms": 1e3 * t_f / steps, "fwd_bwd_ms": 1e3 * t_fb / steps, "steps": steps}
+
+
+def clustered_16m(vox, steps):
+    """16M Gaussians (256 clusters, log-scales N(-5.5, 0.6^2), voxel size 0.002): beyond the
+    reference's 2 Mi-node and 3 GiB caps, built in voxel-x slabs of at most 2^31 pairs each."""
+    import numpy as np
+    import torch
+    from nanovdb_editor_amd import synthetic as syn
+    cloud = syn.workload_clustered_16m()
+    dev = torch.device("cuda", 0)
+    raw = {k: torch.from_numpy(np.ascontiguousarray(v, np.float32).ravel()).to(dev)
+           for k, v in (("means", cloud.means), ("op", cloud.opacity_logits), ("q", cloud.quats),
+                        ("s", cloud.log_scales), ("sh0", cloud.sh0))}
+    del cloud.means, cloud.quats, cloud.log_scales, cloud.sh0
+    work = {k: torch.empty_like(v) for k, v in raw.items()}
+    colors = torch.empty_like(raw["sh0"])
+
+    def step():
+        for k in raw:
+            work[k].copy_(raw[k])
+        _, st = vox.raster_node2(work["means"], work["q"], work["s"], colors, work["op"], 0.002, raw=True,
+                                 sh0=work["sh0"], copy_out=False, max_node_count=0xFFFFFFFF)
+        return st
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = step()
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    return {"workload": "clustered_16m: 16M Gaussians, 256 clusters (sigma 0.1), log-scales N(-5.5,0.6^2), "
+                        "voxel size 0.002, RGBA8, slabs of <= 2^31 pairs",
+            "gaussians": 16_000_000, "ms_per_step": ms, "gaussians_per_s": 16e6 / (ms * 1e-3),
+            "pairs": st["pair_count"], "voxels": st["voxel_count"], "leaves": st["leaf_count"],
+            "slabs": st["slab_count"], "grid_bytes": st["grid_size"], "steps": steps,
+            "stage_ms": {k: st[k] for k in ("ms_setup", "ms_emit", "ms_sort", "ms_reduce", "ms_build")}}
 
 
 def batched(vox, steps, workers):
@@ -330,6 +369,11 @@ def main():
         fvox = Voxelizer(0)
         out["fit_1m"] = fit_1m(fvox, args.fit_steps)
         fvox.close()
+        torch.cuda.empty_cache()
+    if args.big_steps > 0 and default_workload:
+        bvox = Voxelizer(0)
+        out["clustered_16m"] = clustered_16m(bvox, args.big_steps)
+        bvox.close()
         torch.cuda.empty_cache()
     if args.batch_steps > 0 and default_workload:
         out["batched"] = batched(vox, args.batch_steps, args.batch_workers)

@@ -314,10 +314,20 @@ __global__ void k_slab_rows(const PrimBox* __restrict__ boxes, uint64_t n, int32
 }
 
 // candidate fragments per leaf column (x / 8 - lmin_x), each Gaussian's footprint spread evenly
-// over the columns its AABB spans: the slab planner's load estimate
-__global__ void k_slab_hist(const PrimBox* __restrict__ boxes, const uint64_t* __restrict__ foot, uint64_t n,
-                            int32_t lmin_x, uint32_t cols, unsigned long long* __restrict__ hist)
+// over the columns its AABB spans: the slab planner's load estimate. Few columns receive many
+// Gaussians, so each block sums into LDS first (up to SLAB_LDS_COLS columns) and adds one value
+// per column to the global histogram.
+constexpr uint32_t SLAB_LDS_COLS = 4096;
+__global__ __launch_bounds__(256) void k_slab_hist(const PrimBox* __restrict__ boxes, const uint64_t* __restrict__ foot,
+                                                   uint64_t n, int32_t lmin_x, uint32_t cols,
+                                                   unsigned long long* __restrict__ hist)
 {
+    __shared__ unsigned long long sh[SLAB_LDS_COLS];
+    const bool local = cols <= SLAB_LDS_COLS;
+    if (local)
+        for (uint32_t c = threadIdx.x; c < cols; c += blockDim.x)
+            sh[c] = 0ull;
+    __syncthreads();
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x)
     {
         const uint64_t fp = foot[g];
@@ -328,8 +338,18 @@ __global__ void k_slab_hist(const PrimBox* __restrict__ boxes, const uint64_t* _
         const int32_t c1 = ((b.mn[0] + (int32_t)b.rng_x - 1) >> 3) - lmin_x;
         const uint64_t share = fp / (uint64_t)(c1 - c0 + 1) + 1ull;
         for (int32_t c = max(c0, 0); c <= c1 && c < (int32_t)cols; c++)
-            atomicAdd(&hist[c], (unsigned long long)share);
+        {
+            if (local)
+                atomicAdd(&sh[c], (unsigned long long)share);
+            else
+                atomicAdd(&hist[c], (unsigned long long)share);
+        }
     }
+    __syncthreads();
+    if (local)
+        for (uint32_t c = threadIdx.x; c < cols; c += blockDim.x)
+            if (sh[c])
+                atomicAdd(&hist[c], sh[c]);
 }
 
 // segment buffer partitions: block b reserves in partition b % SEG_PARTS (separate counters on
@@ -2218,7 +2238,9 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         uint64_t est;
     };
     std::vector<Slab> todo; // processed from the back
-    if (!any || est_pairs <= slab_budget)
+    // the pair estimate is approximate: one slab up to 1.5x the budget (an actual overflow of the
+    // hard limit still splits the slab)
+    if (!any || est_pairs <= slab_budget + slab_budget / 2)
         todo.push_back(Slab{ INT32_MIN, INT32_MAX, est_pairs });
     else
     {
@@ -2228,7 +2250,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         if (!hist)
             return -1;
         VDBR_CHECK(ctx, hipMemsetAsync(hist, 0, (size_t)cols * 8, st));
-        hipLaunchKernelGGL(k_slab_hist, dim3(std::min<uint32_t>(blocks_for(n), 4096u)), dim3(256), 0, st,
+        hipLaunchKernelGGL(k_slab_hist, dim3(std::min<uint32_t>(blocks_for(n), 1024u)), dim3(256), 0, st,
                            (const PrimBox*)boxes, (const uint64_t*)foot, n, lmin_x, cols, hist);
         VDBR_CHECK(ctx, hipGetLastError());
         std::vector<unsigned long long> h(cols);
