@@ -271,25 +271,6 @@ VDB_DEV uint32_t pair_rgba(const PrimVal& pv, const float vmin[3], const float v
     return (pv.rgb_cull & 0x00FFFFFFu) | (pair_alpha8(pv, vmin, vmax) << 24);
 }
 
-// density contribution o * exp(-m/2) at the voxel centre. The Mahalanobis term is formed in
-// double precision: for elongated Gaussians the short-axis coordinate is a difference of much
-// larger products, which loses ~1e-5 relative accuracy in float.
-VDB_DEV float pair_density(const PrimFrame& f, float opacity, int32_t i, int32_t j, int32_t k, double vs)
-{
-    const double d0 = ((double)i + 0.5) * vs - (double)f.mu[0];
-    const double d1 = ((double)j + 0.5) * vs - (double)f.mu[1];
-    const double d2 = ((double)k + 0.5) * vs - (double)f.mu[2];
-    double m = 0.0;
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-    {
-        double u = (double)f.rinv[3 * r] * d0 + (double)f.rinv[3 * r + 1] * d1 + (double)f.rinv[3 * r + 2] * d2;
-        double q = u * (double)f.inv_s[r];
-        m += q * q;
-    }
-    return opacity * expf((float)(-0.5 * m));
-}
-
 // ----------------------------------------------------------------------------- wave helpers
 VDB_DEV uint32_t lane_id()
 {

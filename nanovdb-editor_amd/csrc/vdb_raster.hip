@@ -8,7 +8,7 @@
 //               LDS stage and flushed in 4 KB bursts -> 64-bit keys (voxel << gid_bits | gid)
 //   sort        stable onesweep radix sort of the keys (vdb_prims.hip)
 //   reduce      one thread per voxel run: RGBA8 max (gaussian_frag_color.slang:184-247) or
-//               float density sum in ascending gaussian id; no float atomics
+//               density summed in double in ascending gaussian id and rounded once; no float atomics
 //   build       leaves/lowers/uppers from the sorted voxel keys (ballot-free: runs of key
 //               prefixes), node order of the reference's windowed allocation
 //               (grid_build_scatter_new.slang:110-174, Raster.cpp:441-535), packed prefix
@@ -1197,7 +1197,38 @@ struct ReduceArgs
     uint32_t* vox_val;
     uint32_t* vox_t;
     int32_t* bounds; // [6]
+    // preprocessed Gaussian arrays (density terms are evaluated from them in double)
+    const float* means;
+    const float* quats;
+    const float* scales;
+    const float* opacities;
 };
+
+// density term o * exp(-m / 2) of Gaussian g at the centre of voxel (i, j, k), in double from the
+// preprocessed float parameters: rotation from the (normalised) quaternion, scale floored at
+// vs / 8 (gaussian_prim.slang:67-68), Mahalanobis distance by division by the scales
+__device__ __forceinline__ double density_term(const ReduceArgs& a, uint32_t g, int32_t i, int32_t j, int32_t k)
+{
+    const double w = a.quats[4 * g + 0], x = a.quats[4 * g + 1], y = a.quats[4 * g + 2], z = a.quats[4 * g + 3];
+    const double R[3][3] = { { 1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w) },
+                             { 2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w) },
+                             { 2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y) } };
+    const double vs = (double)a.vs;
+    const double fl = (double)(0.125f * a.vs);
+    const double d[3] = { ((double)i + 0.5) * vs - (double)a.means[3 * g + 0],
+                          ((double)j + 0.5) * vs - (double)a.means[3 * g + 1],
+                          ((double)k + 0.5) * vs - (double)a.means[3 * g + 2] };
+    double m = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        const double u = R[0][c] * d[0] + R[1][c] * d[1] + R[2][c] * d[2];
+        const double sc = fmax((double)a.scales[3 * g + c], fl);
+        const double q = u / sc;
+        m += q * q;
+    }
+    return (double)a.opacities[g] * exp(-0.5 * m);
+}
 
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
 {
@@ -1215,7 +1246,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
         decode_voxel(a.kl, vkey, &i, &j, &k);
         uint32_t value = 0u;
         uint32_t g0 = 0xFFFFFFFFu;
-        float dens = 0.f;
+        double dens = 0.0; // density: summed in double in ascending gaussian id, rounded once
         if (ab)
         {
             // RGBA8 with the pair alpha in the key: max of (rgb | alpha << 24), any pair order
@@ -1234,9 +1265,9 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
             {
                 uint32_t gid = (uint32_t)(a.keys[p] & gmask);
                 g0 = gid < g0 ? gid : g0;
-                const PrimFrame f = a.frames[gid];
                 if (a.value_mode == VDBR_VALUE_RGBA8)
                 {
+                    const PrimFrame f = a.frames[gid];
                     float vmin[3], vmax[3];
                     voxel_box(f, i, j, k, a.vs, vmin, vmax);
                     uint32_t rgba = pair_rgba(a.vals[gid], vmin, vmax);
@@ -1244,12 +1275,12 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
                 }
                 else
                 {
-                    dens += pair_density(f, a.vals[gid].opacity, i, j, k, (double)a.vs);
+                    dens += density_term(a, gid, i, j, k);
                 }
             }
         }
         if (a.value_mode != VDBR_VALUE_RGBA8)
-            value = __float_as_uint(dens);
+            value = __float_as_uint((float)dens);
         // window of the pair with the lowest gaussian id: fragment slot within its prim batch
         PrimBox bx = a.boxes[g0];
         uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
@@ -2556,6 +2587,10 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             ra.vox_val = sval;
             ra.vox_t = st_t;
             ra.bounds = vbounds;
+            ra.means = gs->means;
+            ra.quats = gs->quats;
+            ra.scales = gs->scales;
+            ra.opacities = gs->opacities;
             VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
             hipLaunchKernelGGL(k_reduce, dim3(blocks_for(svox)), dim3(256), 0, st, ra);
             VDBR_CHECK(ctx, hipGetLastError());
