@@ -332,7 +332,7 @@ typedef struct vdbr_gaussian_grads_t
 } vdbr_gaussian_grads_t;
 
 #define VDBR_VALUE_RGBA8 0u   /* gaussian_frag_color.slang semantics (packed RGBA8, max) */
-#define VDBR_VALUE_DENSITY 1u /* sum_g o_g exp(-m_g/2), ascending gaussian id, float32 */
+#define VDBR_VALUE_DENSITY 1u /* sum_g o_g exp(-m_g/2) at voxel centres (exact sum of double terms), float32 */
 
 #define VDBR_FLAG_RAW_INPUTS 1u
 #define VDBR_FLAG_SPATIAL_ORDER 2u /* nodes in BFS tree order instead of the windowed order */
@@ -414,8 +414,7 @@ PNANOVDB_API int vdbr_density_backward(vdbr_context_t* ctx, const vdbr_gaussians
                                        uint64_t value_count, const vdbr_gaussian_grads_t* grads);
 
 /* The surviving pair list of the last vdbr_raster_node2 call, grouped by voxel (voxels in
- * leaf-major key order; within a voxel in ascending gaussian id for VDBR_VALUE_DENSITY, in
- * emission order for VDBR_VALUE_RGBA8): voxel ijk (int32 x3) and gaussian id per pair, copied
+ * leaf-major key order; within a voxel in emission order): voxel ijk (int32 x3) and gaussian id per pair, copied
  * to device buffers of the caller. Returns the pair count through *count (buffers may be NULL
  * to query it). */
 PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap,

@@ -8,7 +8,7 @@
 //               LDS stage and flushed in 4 KB bursts -> 64-bit keys (voxel << gid_bits | gid)
 //   sort        stable onesweep radix sort of the keys (vdb_prims.hip)
 //   reduce      one thread per voxel run: RGBA8 max (gaussian_frag_color.slang:184-247) or
-//               density summed in double in ascending gaussian id and rounded once; no float atomics
+//               density terms in double summed exactly in fixed point (any pair order); no float atomics
 //   build       leaves/lowers/uppers from the sorted voxel keys (ballot-free: runs of key
 //               prefixes), node order of the reference's windowed allocation
 //               (grid_build_scatter_new.slang:110-174, Raster.cpp:441-535), packed prefix
@@ -1204,6 +1204,56 @@ struct ReduceArgs
     const float* opacities;
 };
 
+// Exact sum of non-negative doubles below 2^63 in 192-bit fixed point with 128 fractional bits:
+// each term is rounded once to a multiple of 2^-128 (terms of the density sums exceed 1e-25, so
+// the rounding is below 1e-13 of any term) and the integer additions are exact, so the sum does
+// not depend on the order of the terms.
+struct FixedSum
+{
+    uint64_t w0 = 0, w1 = 0, w2 = 0; // little-endian words; value = (w2:w1:w0) * 2^-128
+
+    __device__ __forceinline__ void add_words(uint64_t a0, uint64_t a1, uint64_t a2)
+    {
+        const uint64_t s0 = w0 + a0;
+        const uint64_t c0 = s0 < w0 ? 1ull : 0ull;
+        const uint64_t t1 = w1 + a1;
+        const uint64_t c1 = t1 < w1 ? 1ull : 0ull;
+        const uint64_t s1 = t1 + c0;
+        const uint64_t c1b = s1 < t1 ? 1ull : 0ull;
+        w0 = s0;
+        w1 = s1;
+        w2 = w2 + a2 + c1 + c1b;
+    }
+    __device__ __forceinline__ void add(double t)
+    {
+        const uint64_t bits = (uint64_t)__double_as_longlong(t);
+        const int32_t be = (int32_t)((bits >> 52) & 0x7FFu);
+        if (be == 0 || (bits >> 63)) // zero, subnormal (below 2^-1022) or negative: nothing to add
+            return;
+        const uint64_t m = (bits & ((1ull << 52) - 1ull)) | (1ull << 52); // t = m * 2^(be - 1075)
+        const int32_t sh = be - 1075 + 128;                               // position of m's bit 0
+        if (sh < 0)
+        {
+            if (sh < -53)
+                return;
+            const int32_t r = -sh; // round half up to 2^-128
+            add_words((m + (1ull << (r - 1))) >> r, 0ull, 0ull);
+        }
+        else if (sh < 64)
+            add_words(m << sh, sh ? (m >> (64 - sh)) : 0ull, 0ull);
+        else if (sh < 128)
+            add_words(0ull, m << (sh - 64), sh > 64 ? (m >> (128 - sh)) : 0ull);
+        else if (sh < 192)
+            add_words(0ull, 0ull, m << (sh - 128));
+    }
+    // nearest float (the words are summed into a double first; deterministic for any term order)
+    __device__ __forceinline__ float to_float() const
+    {
+        const double v = (double)w2 * 0x1p0 + (double)w1 * 0x1p-64 + (double)w0 * 0x1p-128;
+        return (float)v;
+    }
+};
+
 // density term o * exp(-m / 2) of Gaussian g at the centre of voxel (i, j, k), in double from the
 // preprocessed float parameters: rotation from the (normalised) quaternion, scale floored at
 // vs / 8 (gaussian_prim.slang:67-68), Mahalanobis distance by division by the scales
@@ -1246,7 +1296,7 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
         decode_voxel(a.kl, vkey, &i, &j, &k);
         uint32_t value = 0u;
         uint32_t g0 = 0xFFFFFFFFu;
-        double dens = 0.0; // density: summed in double in ascending gaussian id, rounded once
+        FixedSum dens; // density: exact fixed-point sum of the double terms (order independent)
         if (ab)
         {
             // RGBA8 with the pair alpha in the key: max of (rgb | alpha << 24), any pair order
@@ -1275,12 +1325,12 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
                 }
                 else
                 {
-                    dens += density_term(a, gid, i, j, k);
+                    dens.add(density_term(a, gid, i, j, k));
                 }
             }
         }
         if (a.value_mode != VDBR_VALUE_RGBA8)
-            value = __float_as_uint((float)dens);
+            value = __float_as_uint(dens.to_float());
         // window of the pair with the lowest gaussian id: fragment slot within its prim batch
         PrimBox bx = a.boxes[g0];
         uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
@@ -2248,13 +2298,15 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     kl.voxel_bits = kl.bits[0] + kl.bits[1] + kl.bits[2] + 9u;
     if (kl.voxel_bits + kl.gid_bits > 64)
         return ctx->fail("raster_node2: pair keys need more than 64 bits");
-    // RGBA8: the pair alpha rides in the key and only the voxel bits are sorted (the max and
-    // the lowest gaussian id per voxel do not depend on the pair order). Density: voxel and
-    // gaussian id bits are sorted so each voxel sums its pairs in ascending gaussian id.
+    // Only the voxel bits are sorted: RGBA8 (the pair alpha rides in the key) keeps the max and
+    // the lowest gaussian id per voxel, both independent of the pair order; density sums its
+    // terms exactly in fixed point, so the sum does not depend on the pair order either.
     const bool alpha_in_key = prm->value_mode == VDBR_VALUE_RGBA8 && kl.voxel_bits + kl.gid_bits + 8u <= 64u;
     kl.alpha_bits = alpha_in_key ? 8u : 0u;
     const uint32_t key_bits = kl.voxel_bits + kl.gid_bits + kl.alpha_bits;
-    const uint32_t sort_begin = alpha_in_key ? kl.gid_bits + 8u : 0u;
+    const uint32_t sort_begin = alpha_in_key                                ? kl.gid_bits + 8u
+                                : prm->value_mode == VDBR_VALUE_DENSITY ? kl.gid_bits
+                                                                        : 0u;
 
     // ---------------------------------------------------------------- slab plan
     // Pairs are emitted, sorted and reduced per slab of leaf-aligned voxel columns x in [i0, i1);

@@ -88,11 +88,9 @@ def test_pairs_match_oracle(vox, value_mode):
     a_ijk, a_gid = sort_pairs(ijk, gid)
     b_ijk, b_gid = sort_pairs(rijk, rg)
     assert np.array_equal(a_ijk, b_ijk) and np.array_equal(a_gid, b_gid)
-    # pairs of a voxel are contiguous; in density mode they are in ascending gaussian id
+    # pairs of a voxel are contiguous
     same_voxel = np.all(ijk[1:] == ijk[:-1], axis=1)
     assert np.unique(ijk, axis=0).shape[0] == ijk.shape[0] - same_voxel.sum()
-    if value_mode == 1:
-        assert np.all(gid[1:][same_voxel] > gid[:-1][same_voxel])
 
 
 def test_density_values_within_tolerance(vox):
