@@ -81,6 +81,9 @@ enum WsSlot
     WS_BWD_ACC,
     WS_BWD_REC,
     WS_BWD_RECGID,
+    WS_BWD_LEAVES,
+    WS_BWD_LEAFGRAD,
+    WS_BWD_PRIMS,
     WS_COUNT
 };
 

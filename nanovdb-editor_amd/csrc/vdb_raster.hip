@@ -769,14 +769,40 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
 // m = |diag(1/s) R^T (x_v - mu)|^2 at the voxel centre, differentiated with respect to the raw
 // parameters (means, log-scales, unnormalised quaternions (w, x, y, z), opacity logits) in
 // double precision. The pairs are enumerated again exactly as the forward emits them (same row
-// intervals and predicate), each kept voxel finds dL/drho through the node2 tree, and every row
-// accumulates 16 sums of its Gaussian:
-//   E = sum r e,  Q1_a = sum W u_a,  Q2_a = sum W u_a^2,  D_ab = sum W u_a d_b
-// with r = dL/drho(v), e = exp(-m/2), W = r o e, d = x_v - mu, u = R^T d. Rows are reduced per
-// Gaussian inside the wave (segmented scan over the lanes, whose Gaussian ids are non-decreasing);
-// a Gaussian whose rows cross a 64-row chunk boundary leaves one partial per chunk, summed in
-// chunk order by the finalize pass. The result does not depend on scheduling.
-constexpr int BWD_ACC = 16;
+// intervals and predicate) and every Gaussian accumulates 10 sums:
+//   E = sum r e,  P_a = sum W v_a,  S_ab = sum W v_a v_b (a <= b)
+// with r = dL/drho(v), e = exp(-m/2), W = r o e, d = x_v - mu, v = diag(1/s) R^T d, m = |v|^2.
+// The finalize pass forms from them the sums of u = R^T d = diag(s) v and d = R u:
+//   Q1_a = sum W u_a = s_a P_a,  Q2_a = sum W u_a^2 = s_a^2 S_aa,
+//   D_ab = sum W u_a d_b = s_a sum_c R_bc s_c S_ac.
+//
+// dL/drho comes from a leaf table built from the grid first: every leaf is found by descending
+// the root, upper and lower child masks (PNanoVDBExt.h:225-318 run over every slot), entered in
+// an open-addressing hash on its leaf coordinates and expanded to 512 per-voxel values (0 for
+// inactive voxels), so a pair costs one load, plus one hash probe per leaf along its row, instead
+// of a tree descent and a mask lookup.
+//
+// Each wave takes 64 consecutive AABB rows and cuts their candidate intervals into segments of
+// at most BWD_SEG voxels. Per round it walks 64 segments with the FP32 predicate (its lanes step
+// through at most BWD_SEG voxels instead of the longest of 64 rows) and stages the kept voxels in
+// LDS in segment order; the double-precision terms then run over the staged pairs 64 at a time
+// with every lane busy. Segments keep row order, so Gaussian ids are non-decreasing over the
+// staged pairs: a segmented scan sums each Gaussian's pairs within a group of 64, and a group's
+// last Gaussian carries into the next group. A Gaussian owning the first or last row of a 64-row
+// chunk leaves its chunk sum in that chunk's record (the finalize pass sums records in chunk
+// order); one strictly inside a chunk stores its sum directly. Every sum runs in an order fixed
+// by the rows, so the result does not depend on scheduling.
+constexpr int BWD_ACC = 10;
+constexpr int32_t BWD_SEG = 8;
+constexpr uint32_t LT_NONE = 0xFFFFFFFFu;
+
+// double-precision parameters of one Gaussian as the pair terms use them
+struct BwdPrim
+{
+    double M[9]; // diag(1/s) R^T, row-major (floored scales)
+    double mu[3];
+    double o;
+};
 
 struct BwdArgs
 {
@@ -790,17 +816,222 @@ struct BwdArgs
     const float* log_scales;
     const float* quats; // raw
     const float* logits;
-    const uint64_t* grid;
-    const float* grad_values;
-    uint64_t value_count;
-    double* acc;       // [n][16] Gaussians whose rows lie inside one chunk, away from its ends
-    double* rec;       // [chunks][2][16] first and last segment of each chunk
-    uint64_t* rec_gid; // [chunks][2] (~0: no second segment)
+    BwdPrim* prims;
+    const uint4* lt_slots;
+    uint32_t lt_mask;
+    const float* leaf_grad;
+    double* acc;       // [n][16] Gaussians whose rows lie inside one chunk, away from its ends (zeroed first)
+    double* rec;       // [chunks][2][16] first and last Gaussian of each chunk
+    uint64_t* rec_gid; // [chunks][2] (~0: no second Gaussian)
     float* grad_means;
     float* grad_logits;
     float* grad_quats;
     float* grad_log_scales;
 };
+
+// Leaf table of a node2 grid (built by k_lt_root, k_lt_upper, k_lt_lower, k_lt_grad in order)
+struct NodeRef
+{
+    uint64_t word; // node2 word index of the node
+    int32_t o[3];  // index-space origin
+    int32_t pad;
+};
+
+struct LeafTable
+{
+    const uint64_t* grid; // node2 grid words
+    uint64_t words;       // grid size in words
+    NodeRef* uppers;
+    NodeRef* lowers;
+    uint64_t* leaves;     // node2 word index by leaf ordinal
+    uint4* slots;         // (i >> 3, j >> 3, k >> 3, ordinal + 1); w == 0: empty
+    uint32_t mask;        // slot count - 1 (a power of two, at least twice the leaf capacity)
+    uint32_t cap[3];      // upper, lower, leaf list capacities
+    unsigned int* counters; // [0..2] nodes found per level, [3] inconsistency flag
+    const float* grad_values;
+    uint64_t value_count;
+    float* leaf_grad;     // [ordinal][512] dL/drho by voxel slot n = (i & 7) << 6 | (j & 7) << 3 | (k & 7)
+};
+
+__device__ __forceinline__ uint32_t lt_hash(int32_t li, int32_t lj, int32_t lk)
+{
+    uint32_t h = (uint32_t)li * 0x9E3779B1u ^ (uint32_t)lj * 0x85EBCA77u ^ (uint32_t)lk * 0xC2B2AE3Du;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    return h;
+}
+
+// leaf ordinal of leaf coordinates (li, lj, lk), or LT_NONE
+__device__ __forceinline__ uint32_t lt_find(const uint4* __restrict__ slots, uint32_t mask, int32_t li, int32_t lj,
+                                            int32_t lk)
+{
+    uint32_t h = lt_hash(li, lj, lk) & mask;
+    for (uint32_t p = 0; p <= mask; p++)
+    {
+        const uint4 s = slots[h];
+        if (s.w == 0u)
+            return LT_NONE;
+        if (s.x == (uint32_t)li && s.y == (uint32_t)lj && s.z == (uint32_t)lk)
+            return s.w - 1u;
+        h = (h + 1u) & mask;
+    }
+    return LT_NONE;
+}
+
+// child node of a parent from its table entry (byte offset from the parent), bounded by the grid
+__device__ __forceinline__ bool lt_child(const LeafTable& t, uint64_t parent, uint64_t entry, uint64_t child_words,
+                                         uint64_t* child)
+{
+    const uint64_t rel = t.grid[entry] >> 3;
+    if (rel == 0 || rel > t.words || parent + rel + child_words > t.words)
+        return false;
+    *child = parent + rel;
+    return true;
+}
+
+__device__ __forceinline__ bool lt_push(const LeafTable& t, int level, uint32_t* slot)
+{
+    const uint32_t s = atomicAdd(&t.counters[level], 1u);
+    if (s >= t.cap[level])
+    {
+        atomicOr(&t.counters[3], 1u);
+        return false;
+    }
+    *slot = s;
+    return true;
+}
+
+// root slots -> upper nodes
+__global__ __launch_bounds__(256) void k_lt_root(LeafTable t)
+{
+    const uint32_t rn = blockIdx.x * 256u + threadIdx.x;
+    if (rn >= 32768u)
+        return;
+    const uint64_t root = VDB_ROOT_ADDR >> 3;
+    bool act = false, kact = false;
+    const uint32_t rank =
+        n2_mask_n_to_idx(t.grid, root + VDB_OFF_CMP_ROOT, VDB_OFF_CM_ROOT - VDB_OFF_CMP_ROOT, rn, &act);
+    if (!act)
+        return;
+    const uint32_t kidx =
+        n2_mask_n_to_idx(t.grid, root + VDB_OFF_KMP_ROOT, VDB_OFF_KM_ROOT - VDB_OFF_KMP_ROOT, rn, &kact);
+    if (!kact)
+        return;
+    const uint32_t child_count = (uint32_t)(t.grid[root + VDB_OFF_CMP_ROOT] & 0xFFFFu);
+    // the key holds (u >> 12) of the unsigned coordinates u: x at bit 42, y at bit 21, z at bit 0
+    const uint64_t key = t.grid[root + VDB_OFF_CHILDREN_ROOT + child_count + kidx];
+    const uint32_t ku[3] = { (uint32_t)(key >> 42) & 0xFFFFFu, (uint32_t)(key >> 21) & 0xFFFFFu,
+                             (uint32_t)key & 0xFFFFFu };
+    // a lookup reaches this slot only with coordinates whose low five root bits name the slot
+    if ((ku[0] & 31u) != (rn >> 10) || (ku[1] & 31u) != ((rn >> 5) & 31u) || (ku[2] & 31u) != (rn & 31u))
+        return;
+    uint64_t upper;
+    uint32_t s;
+    if (!lt_child(t, root, root + VDB_OFF_CHILDREN_ROOT + rank, VDB_SIZE_UPPER >> 3, &upper))
+    {
+        atomicOr(&t.counters[3], 1u);
+        return;
+    }
+    if (!lt_push(t, 0, &s))
+        return;
+    NodeRef r;
+    r.word = upper;
+    r.pad = 0;
+    for (int c = 0; c < 3; c++)
+        r.o[c] = (int32_t)(ku[c] << 12);
+    t.uppers[s] = r;
+}
+
+// (upper, slot) -> lower nodes
+__global__ __launch_bounds__(256) void k_lt_upper(LeafTable t)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t u = x >> 15;
+    const uint32_t un = (uint32_t)x & 32767u;
+    if (u >= (uint64_t)min(t.counters[0], t.cap[0]))
+        return;
+    const NodeRef p = t.uppers[u];
+    bool act = false;
+    const uint32_t rank =
+        n2_mask_n_to_idx(t.grid, p.word + VDB_OFF_CMP_UPPER, VDB_OFF_CM_UPPER - VDB_OFF_CMP_UPPER, un, &act);
+    if (!act)
+        return;
+    uint64_t lower;
+    uint32_t s;
+    if (!lt_child(t, p.word, p.word + VDB_OFF_CHILDREN_UPPER + rank, VDB_SIZE_LOWER >> 3, &lower))
+    {
+        atomicOr(&t.counters[3], 1u);
+        return;
+    }
+    if (!lt_push(t, 1, &s))
+        return;
+    NodeRef r;
+    r.word = lower;
+    r.pad = 0;
+    r.o[0] = p.o[0] + (int32_t)((un >> 10) << 7);
+    r.o[1] = p.o[1] + (int32_t)(((un >> 5) & 31u) << 7);
+    r.o[2] = p.o[2] + (int32_t)((un & 31u) << 7);
+    t.lowers[s] = r;
+}
+
+// (lower, slot) -> leaves: ordinal, hash entry
+__global__ __launch_bounds__(256) void k_lt_lower(LeafTable t)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t l = x >> 12;
+    const uint32_t ln = (uint32_t)x & 4095u;
+    if (l >= (uint64_t)min(t.counters[1], t.cap[1]))
+        return;
+    const NodeRef p = t.lowers[l];
+    bool act = false;
+    const uint32_t rank =
+        n2_mask_n_to_idx(t.grid, p.word + VDB_OFF_CMP_LOWER, VDB_OFF_CM_LOWER - VDB_OFF_CMP_LOWER, ln, &act);
+    if (!act)
+        return;
+    uint64_t leaf;
+    uint32_t s;
+    if (!lt_child(t, p.word, p.word + VDB_OFF_CHILDREN_LOWER + rank, VDB_SIZE_LEAF >> 3, &leaf))
+    {
+        atomicOr(&t.counters[3], 1u);
+        return;
+    }
+    if (!lt_push(t, 2, &s))
+        return;
+    t.leaves[s] = leaf;
+    const int32_t li = (p.o[0] >> 3) + (int32_t)(ln >> 8);
+    const int32_t lj = (p.o[1] >> 3) + (int32_t)((ln >> 4) & 15u);
+    const int32_t lk = (p.o[2] >> 3) + (int32_t)(ln & 15u);
+    uint32_t h = lt_hash(li, lj, lk) & t.mask;
+    for (uint32_t q = 0; q <= t.mask; q++)
+    {
+        if (atomicCAS(&t.slots[h].w, 0u, s + 1u) == 0u)
+        {
+            t.slots[h].x = (uint32_t)li;
+            t.slots[h].y = (uint32_t)lj;
+            t.slots[h].z = (uint32_t)lk;
+            return;
+        }
+        h = (h + 1u) & t.mask;
+    }
+    atomicOr(&t.counters[3], 1u);
+}
+
+// (leaf ordinal, voxel slot) -> dL/drho, 0 for inactive voxels
+__global__ __launch_bounds__(256) void k_lt_grad(LeafTable t)
+{
+    const uint64_t x = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint64_t l = x >> 9;
+    const uint32_t n = (uint32_t)x & 511u;
+    if (l >= (uint64_t)min(t.counters[2], t.cap[2]))
+        return;
+    const uint64_t leaf = t.leaves[l];
+    bool act = false;
+    const uint32_t local =
+        n2_mask_n_to_idx(t.grid, leaf + VDB_OFF_VMP_LEAF, VDB_OFF_VM_LEAF - VDB_OFF_VMP_LEAF, n, &act);
+    const uint64_t vi = t.grid[leaf + VDB_OFF_VALUE_IDX] + local;
+    t.leaf_grad[x] = act && vi < t.value_count ? t.grad_values[vi] : 0.f;
+}
 
 // double-precision parameters of one Gaussian from its raw parameters (RasterUtils.cpp:59-117
 // preprocessing, gaussian_prim.slang:67-68 scale floor)
@@ -847,10 +1078,58 @@ __device__ __forceinline__ void load_prim_d(const BwdArgs& a, uint64_t g, PrimD*
     p->o = 1.0 / (1.0 + exp(-(double)a.logits[g]));
 }
 
-__global__ __launch_bounds__(EMIT_THREADS) void k_density_bwd(BwdArgs a)
+__global__ __launch_bounds__(256) void k_bwd_prims(BwdArgs a)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n)
+        return;
+    PrimD p;
+    load_prim_d(a, g, &p);
+    BwdPrim q;
+    for (int c = 0; c < 3; c++)
+    {
+        for (int b = 0; b < 3; b++)
+            q.M[3 * c + b] = p.R[b][c] / p.s[c];
+        q.mu[c] = p.mu[c];
+    }
+    q.o = p.o;
+    a.prims[g] = q;
+}
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// last lane L whose (non-decreasing) exclusive count excl(L) is <= idx
+__device__ __forceinline__ uint32_t wave_owner(uint32_t excl, uint32_t idx)
+{
+    uint32_t lo = 0, hi = 63;
+#pragma unroll
+    for (int step = 0; step < 6; step++)
+    {
+        const uint32_t mid = (lo + hi + 1u) >> 1;
+        const uint32_t ex_mid = (uint32_t)__shfl((int)excl, (int)mid, 64);
+        if (ex_mid <= idx)
+            lo = mid;
+        else
+            hi = mid - 1u;
+    }
+    return lo;
+}
+
+// At least 4 waves per SIMD (128 VGPRs, a few spills): the pass waits on memory, and 4 waves beat
+// the 3 that the unconstrained allocation (144 VGPRs) allows (74.5 vs 86 ms on 2M Gaussians).
+__global__ __launch_bounds__(EMIT_THREADS, 4) void k_density_bwd(BwdArgs a)
 {
     __shared__ uint64_t g_range[2];
+    __shared__ double carry_all[EMIT_THREADS / 64][BWD_ACC];
+    __shared__ uint32_t staged_all[EMIT_THREADS / 64][64 * BWD_SEG]; // leaf ordinal << 3 | step
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    double* carry = carry_all[wave];
+    uint32_t* staged = staged_all[wave];
     const uint64_t row0 = (uint64_t)blockIdx.x * EMIT_ROWS;
     if (threadIdx.x == 0)
     {
@@ -875,99 +1154,204 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_density_bwd(BwdArgs a)
         PrimFrame f;
         const bool active =
             row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, vs, &g, &f, &i, &j, &k0, &len);
-        double acc[BWD_ACC];
-#pragma unroll
-        for (int q = 0; q < BWD_ACC; q++)
-            acc[q] = 0.0;
-        PrimD p;
-        if (len > 0)
-            load_prim_d(a, g, &p);
-        const int32_t steps = wave_max(len);
-        uint64_t leaf = 0;
-        int32_t leaf_k = INT32_MIN;
-        walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float*, const float*) {
-            if (!keep)
-                return;
-            const int32_t k = k0 + st;
-            if ((k >> 3) != leaf_k)
-            {
-                leaf = n2_find_leaf(a.grid, i, j, k);
-                leaf_k = k >> 3;
-            }
-            uint64_t vi = leaf ? n2_leaf_value_index(a.grid, leaf, i, j, k) : ~0ull;
-            if (vi >= a.value_count)
-                return;
-            const double r = (double)a.grad_values[vi];
-            if (r == 0.0)
-                return;
-            double d[3] = { ((double)i + 0.5) * vsd - p.mu[0], ((double)j + 0.5) * vsd - p.mu[1],
-                            ((double)k + 0.5) * vsd - p.mu[2] };
-            double u[3], m = 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-            {
-                u[c] = p.R[0][c] * d[0] + p.R[1][c] * d[1] + p.R[2][c] * d[2];
-                m += u[c] * u[c] * p.is2[c];
-            }
-            const double e = exp(-0.5 * m);
-            const double W = r * p.o * e;
-            acc[0] += r * e;
-#pragma unroll
-            for (int c = 0; c < 3; c++)
-            {
-                const double wu = W * u[c];
-                acc[1 + c] += wu;
-                acc[4 + c] += wu * u[c];
-#pragma unroll
-                for (int b = 0; b < 3; b++)
-                    acc[7 + 3 * c + b] += wu * d[b];
-            }
-        });
-        // segmented inclusive scan over the lanes by Gaussian id (ids are non-decreasing)
         const uint64_t key = active ? g : ~0ull;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1)
-        {
-            const uint64_t kup = __shfl_up(key, off, 64);
-            const bool take = lane >= (uint32_t)off && kup == key;
-#pragma unroll
-            for (int q = 0; q < BWD_ACC; q++)
-            {
-                double v = __shfl_up(acc[q], off, 64);
-                if (take)
-                    acc[q] += v;
-            }
-        }
-        const uint64_t knext = __shfl_down(key, 1, 64);
         const uint64_t amask = __ballot(active);
         const uint32_t last_lane = 63u - (uint32_t)__clzll(amask);
-        const uint64_t first_key = __shfl(key, 0, 64);
-        const uint64_t last_key = __shfl(key, (int)last_lane, 64);
+        const uint64_t first_key = readlane_u64(key, 0);
+        const uint64_t last_key = readlane_u64(key, (int)last_lane);
         const uint64_t c = chunk_row >> 6;
-        const bool seg_end = active && (lane == last_lane || knext != key);
-        if (lane == 0 && first_key == last_key)
-            a.rec_gid[2 * c + 1] = ~0ull;
-        if (seg_end)
+        const uint32_t nseg = len > 0 ? (uint32_t)((len + BWD_SEG - 1) / BWD_SEG) : 0u;
+        uint32_t total;
+        const uint32_t excl = wave_excl_add(nseg, lane, &total);
+        auto dest = [&](uint64_t gg) -> double* {
+            if (gg == first_key)
+                return a.rec + (2 * c) * BWD_ACC;
+            if (gg == last_key)
+                return a.rec + (2 * c + 1) * BWD_ACC;
+            return a.acc + gg * BWD_ACC;
+        };
+        bool first_done = false, last_done = false; // wave-uniform
+        uint64_t carry_g = ~0ull;                   // wave-uniform
+        for (uint32_t s0 = 0; s0 < total; s0 += 64u)
         {
-            double* dst;
-            if (key == first_key)
+            // walk: segment sidx belongs to the last lane (row) whose exclusive count is <= sidx
+            const uint32_t sidx = s0 + lane;
+            const uint32_t owner = wave_owner(excl, sidx);
+            const uint32_t o_excl = (uint32_t)__shfl((int)excl, (int)owner, 64);
+            const uint64_t o_g = __shfl(key, (int)owner, 64);
+            const int32_t si = __shfl(i, (int)owner, 64);
+            const int32_t sj = __shfl(j, (int)owner, 64);
+            const int32_t o_k0 = __shfl(k0, (int)owner, 64);
+            const int32_t o_len = __shfl(len, (int)owner, 64);
+            const bool has = sidx < total;
+            const int32_t q = (int32_t)(sidx - o_excl);
+            uint64_t sg = ~0ull;
+            int32_t sk0 = 0, slen = 0;
+            PrimFrame sf;
+            if (has)
             {
-                dst = a.rec + (2 * c) * BWD_ACC;
-                a.rec_gid[2 * c] = key;
+                sg = o_g;
+                sk0 = o_k0 + q * BWD_SEG;
+                slen = min(BWD_SEG, o_len - q * BWD_SEG);
+                sf = a.frames[sg];
             }
-            else if (key == last_key)
+            const int32_t steps = wave_max(slen);
+            uint32_t lord = LT_NONE, kept = 0u;
+            int32_t leaf_k = INT32_MIN;
+            walk_row(sf, si, sj, sk0, slen, steps, vs, [&](int32_t st, bool keep, const float*, const float*) {
+                if (!keep)
+                    return;
+                const int32_t k = sk0 + st;
+                if ((k >> 3) != leaf_k)
+                {
+                    lord = lt_find(a.lt_slots, a.lt_mask, si >> 3, sj >> 3, k >> 3);
+                    leaf_k = k >> 3;
+                }
+                if (lord != LT_NONE)
+                    staged[lane * BWD_SEG + kept++] = (lord << 3) | (uint32_t)st;
+            });
+            uint32_t kept_total;
+            const uint32_t kexcl = wave_excl_add(kept, lane, &kept_total);
+            wave_sync_lds();
+            // terms: staged pairs in segment order, 64 per group
+            for (uint32_t p0 = 0; p0 < kept_total; p0 += 64u)
             {
-                dst = a.rec + (2 * c + 1) * BWD_ACC;
-                a.rec_gid[2 * c + 1] = key;
-            }
-            else
-            {
-                dst = a.acc + key * BWD_ACC;
-            }
+                const uint32_t pidx = p0 + lane;
+                const uint32_t sl = wave_owner(kexcl, pidx);
+                const uint32_t t_in = pidx - (uint32_t)__shfl((int)kexcl, (int)sl, 64);
+                const uint64_t pg0 = __shfl(sg, (int)sl, 64);
+                const int32_t pi = __shfl(si, (int)sl, 64);
+                const int32_t pj = __shfl(sj, (int)sl, 64);
+                const int32_t pk0 = __shfl(sk0, (int)sl, 64);
+                const bool valid = pidx < kept_total;
+                uint64_t pg = ~0ull;
+                double acc[BWD_ACC];
 #pragma unroll
-            for (int q = 0; q < BWD_ACC; q++)
-                dst[q] = acc[q];
+                for (int t = 0; t < BWD_ACC; t++)
+                    acc[t] = 0.0;
+                if (valid)
+                {
+                    pg = pg0;
+                    const uint32_t ent = staged[sl * BWD_SEG + t_in];
+                    const int32_t k = pk0 + (int32_t)(ent & 7u);
+                    const uint32_t nv = (((uint32_t)pi & 7u) << 6) | (((uint32_t)pj & 7u) << 3) | ((uint32_t)k & 7u);
+                    const double r = (double)a.leaf_grad[(uint64_t)(ent >> 3) * 512u + nv];
+                    const BwdPrim& sp = a.prims[pg0];
+                    const double d[3] = { ((double)pi + 0.5) * vsd - sp.mu[0], ((double)pj + 0.5) * vsd - sp.mu[1],
+                                          ((double)k + 0.5) * vsd - sp.mu[2] };
+                    double v[3];
+#pragma unroll
+                    for (int cc = 0; cc < 3; cc++)
+                        v[cc] = sp.M[3 * cc + 0] * d[0] + sp.M[3 * cc + 1] * d[1] + sp.M[3 * cc + 2] * d[2];
+                    const double m = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                    const double e = exp(-0.5 * m);
+                    const double W = r * sp.o * e;
+                    const double w0 = W * v[0], w1 = W * v[1], w2 = W * v[2];
+                    acc[0] = r * e;
+                    acc[1] = w0;
+                    acc[2] = w1;
+                    acc[3] = w2;
+                    acc[4] = w0 * v[0];
+                    acc[5] = w1 * v[1];
+                    acc[6] = w2 * v[2];
+                    acc[7] = w0 * v[1];
+                    acc[8] = w0 * v[2];
+                    acc[9] = w1 * v[2];
+                }
+                // segmented inclusive scan over the lanes by Gaussian id (ids are non-decreasing)
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1)
+                {
+                    const uint64_t kup = __shfl_up(pg, off, 64);
+                    const bool take = lane >= (uint32_t)off && kup == pg;
+#pragma unroll
+                    for (int t = 0; t < BWD_ACC; t++)
+                    {
+                        double u = __shfl_up(acc[t], off, 64);
+                        if (take)
+                            acc[t] += u;
+                    }
+                }
+                const uint32_t nlast = min(kept_total - p0, 64u) - 1u; // last lane holding a pair
+                const uint64_t knext = __shfl_down(pg, 1, 64);
+                const bool run_end = valid && (lane == nlast || knext != pg);
+                const uint64_t rfirst = readlane_u64(pg, 0);
+                const uint64_t rlast = readlane_u64(pg, (int)nlast);
+                // the previous group's last Gaussian joins this group's first run, or is complete
+                if (carry_g != ~0ull)
+                {
+                    if (carry_g == rfirst)
+                    {
+                        if (run_end && pg == rfirst)
+                        {
+#pragma unroll
+                            for (int t = 0; t < BWD_ACC; t++)
+                                acc[t] = carry[t] + acc[t];
+                        }
+                    }
+                    else
+                    {
+                        if (lane == 0u)
+                        {
+                            double* dst = dest(carry_g);
+                            for (int t = 0; t < BWD_ACC; t++)
+                                dst[t] = carry[t];
+                        }
+                        first_done = first_done || carry_g == first_key;
+                        last_done = last_done || carry_g == last_key;
+                    }
+                }
+                wave_sync_lds();
+                // the group's last run waits for the next group; the others are complete
+                const bool to_carry = run_end && lane == nlast;
+                const bool emit = run_end && !to_carry;
+                if (to_carry)
+                {
+#pragma unroll
+                    for (int t = 0; t < BWD_ACC; t++)
+                        carry[t] = acc[t];
+                }
+                if (emit)
+                {
+                    double* dst = dest(pg);
+#pragma unroll
+                    for (int t = 0; t < BWD_ACC; t++)
+                        dst[t] = acc[t];
+                }
+                first_done = first_done || __ballot(emit && pg == first_key) != 0ull;
+                last_done = last_done || __ballot(emit && pg == last_key) != 0ull;
+                carry_g = rlast;
+                wave_sync_lds();
+            }
         }
+        if (carry_g != ~0ull)
+        {
+            if (lane == 0u)
+            {
+                double* dst = dest(carry_g);
+                for (int t = 0; t < BWD_ACC; t++)
+                    dst[t] = carry[t];
+            }
+            first_done = first_done || carry_g == first_key;
+            last_done = last_done || carry_g == last_key;
+        }
+        // chunk records: ids, and zero sums for end Gaussians without pairs in this chunk
+        if (lane == 0u)
+        {
+            a.rec_gid[2 * c] = first_key;
+            a.rec_gid[2 * c + 1] = last_key != first_key ? last_key : ~0ull;
+            if (!first_done)
+            {
+                for (int t = 0; t < BWD_ACC; t++)
+                    a.rec[(2 * c) * BWD_ACC + t] = 0.0;
+            }
+            if (last_key != first_key && !last_done)
+            {
+                for (int t = 0; t < BWD_ACC; t++)
+                    a.rec[(2 * c + 1) * BWD_ACC + t] = 0.0;
+            }
+        }
+        wave_sync_lds();
     }
 }
 
@@ -1012,20 +1396,35 @@ __global__ __launch_bounds__(256) void k_density_bwd_finalize(BwdArgs a)
     }
     PrimD p;
     load_prim_d(a, g, &p);
+    // sums of v = diag(1/s) R^T d: P_a = A[1 + a], S (symmetric) from A[4..9]
+    double S[3][3];
+    S[0][0] = A[4];
+    S[1][1] = A[5];
+    S[2][2] = A[6];
+    S[0][1] = S[1][0] = A[7];
+    S[0][2] = S[2][0] = A[8];
+    S[1][2] = S[2][1] = A[9];
     a.grad_logits[g] = (float)(A[0] * p.o * (1.0 - p.o));
+    // dL/dmu_b = sum_c R_bc Q1_c / s_c^2 = sum_c R_bc P_c / s_c; dL/dlog s_b = Q2_b / s_b^2 = S_bb
     for (int b = 0; b < 3; b++)
     {
         double gm = 0.0;
         for (int c = 0; c < 3; c++)
-            gm += p.R[b][c] * A[1 + c] * p.is2[c];
+            gm += p.R[b][c] * A[1 + c] / p.s[c];
         a.grad_means[3 * g + b] = (float)gm;
-        a.grad_log_scales[3 * g + b] = p.floored[b] ? 0.f : (float)(A[4 + b] * p.is2[b]);
+        a.grad_log_scales[3 * g + b] = p.floored[b] ? 0.f : (float)S[b][b];
     }
-    // dL/dR[b][c] = -D_cb / s_c^2, then through R(q) and q = q_raw / |q_raw|
+    // dL/dR[b][c] = -D_cb / s_c^2 with D_cb = s_c sum_e R_be s_e S_ce, then through R(q) and
+    // q = q_raw / |q_raw|
     double G[3][3];
     for (int b = 0; b < 3; b++)
         for (int c = 0; c < 3; c++)
-            G[b][c] = -A[7 + 3 * c + b] * p.is2[c];
+        {
+            double dcb = 0.0;
+            for (int e = 0; e < 3; e++)
+                dcb += p.R[b][e] * p.s[e] * S[c][e];
+            G[b][c] = -dcb / p.s[c];
+        }
     const double w = p.q[0], x = p.q[1], y = p.q[2], z = p.q[3];
     const double dRw[3][3] = { { 0, -2 * z, 2 * y }, { 2 * z, 0, -2 * x }, { -2 * y, 2 * x, 0 } };
     const double dRx[3][3] = { { 0, 2 * y, 2 * z }, { 2 * y, -4 * x, -2 * w }, { 2 * z, 2 * w, -4 * x } };
@@ -3093,15 +3492,59 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
                        1.f / vs, 0.f, so);
     if (scan_u64(ctx, rows, row_scan, n, false, row_scan + n))
         return -1;
+    // leaf table of the grid: node counts from the tree header bound the node lists
+    const uint64_t grid_words = grid_size >> 3;
     VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, row_scan + n, 8, hipMemcpyDeviceToHost, st));
+    VDBR_CHECK(ctx, hipMemcpyAsync((uint8_t*)ctx->pinned + 8, (const uint8_t*)grid + VDB_TREE_ADDR +
+                                   VDB_TREE_OFF_COUNT_LEAF, 12, hipMemcpyDeviceToHost, st));
     VDBR_CHECK(ctx, hipStreamSynchronize(st));
     const uint64_t total_rows = ctx->pinned[0];
+    const uint32_t* tree_counts = (const uint32_t*)((const uint8_t*)ctx->pinned + 8);
+    const uint64_t n_leaf = tree_counts[0], n_lower = tree_counts[1], n_upper = tree_counts[2];
+    if (n_upper > 32768u || n_lower > n_upper * 32768u || n_leaf > n_lower * 4096u ||
+        n_upper * VDB_SIZE_UPPER + n_lower * VDB_SIZE_LOWER + n_leaf * VDB_SIZE_LEAF > grid_size)
+        return ctx->fail("density_backward: grid tree node counts are inconsistent with its size");
+    uint64_t lt_cap = 2;
+    while (lt_cap < 2 * n_leaf)
+        lt_cap <<= 1;
+    if (lt_cap > (1ull << 31))
+        return ctx->fail("density_backward: more than 2^30 leaves");
+    uint8_t* lt_ws = (uint8_t*)ctx->ws(WS_BWD_LEAVES, 64 + (n_upper + n_lower) * sizeof(NodeRef) +
+                                                          n_leaf * sizeof(uint64_t) + lt_cap * sizeof(uint4));
+    float* leaf_grad = (float*)ctx->ws(WS_BWD_LEAFGRAD, (n_leaf * 512 + 1) * sizeof(float));
+    BwdPrim* prims = (BwdPrim*)ctx->ws(WS_BWD_PRIMS, n * sizeof(BwdPrim));
+    if (!lt_ws || !leaf_grad || !prims)
+        return -1;
+    LeafTable lt;
+    lt.grid = (const uint64_t*)grid;
+    lt.words = grid_words;
+    lt.counters = (unsigned int*)lt_ws;
+    lt.slots = (uint4*)(lt_ws + 64);
+    lt.uppers = (NodeRef*)(lt_ws + 64 + lt_cap * sizeof(uint4));
+    lt.lowers = lt.uppers + n_upper;
+    lt.leaves = (uint64_t*)(lt.lowers + n_lower);
+    lt.mask = (uint32_t)(lt_cap - 1);
+    lt.cap[0] = (uint32_t)n_upper;
+    lt.cap[1] = (uint32_t)n_lower;
+    lt.cap[2] = (uint32_t)n_leaf;
+    lt.grad_values = grad_values;
+    lt.value_count = value_count;
+    lt.leaf_grad = leaf_grad;
+    VDBR_CHECK(ctx, hipMemsetAsync(lt_ws, 0, 64 + lt_cap * sizeof(uint4), st));
+    hipLaunchKernelGGL(k_lt_root, dim3(32768 / 256), dim3(256), 0, st, lt);
+    if (n_upper)
+        hipLaunchKernelGGL(k_lt_upper, dim3(blocks_for(n_upper << 15)), dim3(256), 0, st, lt);
+    if (n_lower)
+        hipLaunchKernelGGL(k_lt_lower, dim3(blocks_for(n_lower << 12)), dim3(256), 0, st, lt);
+    if (n_leaf)
+        hipLaunchKernelGGL(k_lt_grad, dim3(blocks_for(n_leaf << 9)), dim3(256), 0, st, lt);
     const uint64_t chunks = (total_rows + 63) / 64;
     double* acc = (double*)ctx->ws(WS_BWD_ACC, n * BWD_ACC * sizeof(double));
     double* rec = (double*)ctx->ws(WS_BWD_REC, (chunks + 1) * 2 * BWD_ACC * sizeof(double));
     uint64_t* rec_gid = (uint64_t*)ctx->ws(WS_BWD_RECGID, (chunks + 1) * 2 * sizeof(uint64_t));
     if (!acc || !rec || !rec_gid)
         return -1;
+    VDBR_CHECK(ctx, hipMemsetAsync(acc, 0, n * BWD_ACC * sizeof(double), st));
     BwdArgs a;
     a.frames = frames;
     a.boxes = boxes;
@@ -3113,9 +3556,10 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
     a.log_scales = raw->scales;
     a.quats = raw->quats;
     a.logits = raw->opacities;
-    a.grid = (const uint64_t*)grid;
-    a.grad_values = grad_values;
-    a.value_count = value_count;
+    a.prims = prims;
+    a.lt_slots = lt.slots;
+    a.lt_mask = lt.mask;
+    a.leaf_grad = leaf_grad;
     a.acc = acc;
     a.rec = rec;
     a.rec_gid = rec_gid;
@@ -3123,6 +3567,7 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
     a.grad_logits = grads->opacities;
     a.grad_quats = grads->quats;
     a.grad_log_scales = grads->scales;
+    hipLaunchKernelGGL(k_bwd_prims, dim3(blocks_for(n)), dim3(256), 0, st, a);
     if (total_rows > 0)
     {
         uint64_t blocks = (total_rows + EMIT_ROWS - 1) / EMIT_ROWS;
@@ -3133,11 +3578,12 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
     hipLaunchKernelGGL(k_density_bwd_finalize, dim3(blocks_for(n)), dim3(256), 0, st, a);
     VDBR_CHECK(ctx, hipGetLastError());
     VDBR_CHECK(ctx, hipEventRecord(e1, st));
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, lt.counters, 16, hipMemcpyDeviceToHost, st));
+    VDBR_CHECK(ctx, hipStreamSynchronize(st));
+    if (((const uint32_t*)ctx->pinned)[3])
+        return ctx->fail("density_backward: the grid's nodes do not match its tree header");
     if (ms_out)
-    {
-        VDBR_CHECK(ctx, hipEventSynchronize(e1));
         VDBR_CHECK(ctx, hipEventElapsedTime(ms_out, e0, e1));
-    }
     return 0;
 }
 
