@@ -84,6 +84,8 @@ enum WsSlot
     WS_BWD_LEAVES,
     WS_BWD_LEAFGRAD,
     WS_BWD_PRIMS,
+    WS_DENS_PRIMS,
+    WS_VOX_FIX,
     WS_COUNT
 };
 

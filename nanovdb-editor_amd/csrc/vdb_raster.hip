@@ -1590,94 +1590,12 @@ struct ReduceArgs
     const uint32_t* win_base;  // per prim batch
     uint32_t prim_batch;
     uint32_t ijk_batch;
-    uint32_t value_mode;
     float vs;
     uint64_t* vox_key;
     uint32_t* vox_val;
     uint32_t* vox_t;
     int32_t* bounds; // [6]
-    // preprocessed Gaussian arrays (density terms are evaluated from them in double)
-    const float* means;
-    const float* quats;
-    const float* scales;
-    const float* opacities;
 };
-
-// Exact sum of non-negative doubles below 2^63 in 192-bit fixed point with 128 fractional bits:
-// each term is rounded once to a multiple of 2^-128 (terms of the density sums exceed 1e-25, so
-// the rounding is below 1e-13 of any term) and the integer additions are exact, so the sum does
-// not depend on the order of the terms.
-struct FixedSum
-{
-    uint64_t w0 = 0, w1 = 0, w2 = 0; // little-endian words; value = (w2:w1:w0) * 2^-128
-
-    __device__ __forceinline__ void add_words(uint64_t a0, uint64_t a1, uint64_t a2)
-    {
-        const uint64_t s0 = w0 + a0;
-        const uint64_t c0 = s0 < w0 ? 1ull : 0ull;
-        const uint64_t t1 = w1 + a1;
-        const uint64_t c1 = t1 < w1 ? 1ull : 0ull;
-        const uint64_t s1 = t1 + c0;
-        const uint64_t c1b = s1 < t1 ? 1ull : 0ull;
-        w0 = s0;
-        w1 = s1;
-        w2 = w2 + a2 + c1 + c1b;
-    }
-    __device__ __forceinline__ void add(double t)
-    {
-        const uint64_t bits = (uint64_t)__double_as_longlong(t);
-        const int32_t be = (int32_t)((bits >> 52) & 0x7FFu);
-        if (be == 0 || (bits >> 63)) // zero, subnormal (below 2^-1022) or negative: nothing to add
-            return;
-        const uint64_t m = (bits & ((1ull << 52) - 1ull)) | (1ull << 52); // t = m * 2^(be - 1075)
-        const int32_t sh = be - 1075 + 128;                               // position of m's bit 0
-        if (sh < 0)
-        {
-            if (sh < -53)
-                return;
-            const int32_t r = -sh; // round half up to 2^-128
-            add_words((m + (1ull << (r - 1))) >> r, 0ull, 0ull);
-        }
-        else if (sh < 64)
-            add_words(m << sh, sh ? (m >> (64 - sh)) : 0ull, 0ull);
-        else if (sh < 128)
-            add_words(0ull, m << (sh - 64), sh > 64 ? (m >> (128 - sh)) : 0ull);
-        else if (sh < 192)
-            add_words(0ull, 0ull, m << (sh - 128));
-    }
-    // nearest float (the words are summed into a double first; deterministic for any term order)
-    __device__ __forceinline__ float to_float() const
-    {
-        const double v = (double)w2 * 0x1p0 + (double)w1 * 0x1p-64 + (double)w0 * 0x1p-128;
-        return (float)v;
-    }
-};
-
-// density term o * exp(-m / 2) of Gaussian g at the centre of voxel (i, j, k), in double from the
-// preprocessed float parameters: rotation from the (normalised) quaternion, scale floored at
-// vs / 8 (gaussian_prim.slang:67-68), Mahalanobis distance by division by the scales
-__device__ __forceinline__ double density_term(const ReduceArgs& a, uint32_t g, int32_t i, int32_t j, int32_t k)
-{
-    const double w = a.quats[4 * g + 0], x = a.quats[4 * g + 1], y = a.quats[4 * g + 2], z = a.quats[4 * g + 3];
-    const double R[3][3] = { { 1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w) },
-                             { 2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w) },
-                             { 2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y) } };
-    const double vs = (double)a.vs;
-    const double fl = (double)(0.125f * a.vs);
-    const double d[3] = { ((double)i + 0.5) * vs - (double)a.means[3 * g + 0],
-                          ((double)j + 0.5) * vs - (double)a.means[3 * g + 1],
-                          ((double)k + 0.5) * vs - (double)a.means[3 * g + 2] };
-    double m = 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-    {
-        const double u = R[0][c] * d[0] + R[1][c] * d[1] + R[2][c] * d[2];
-        const double sc = fmax((double)a.scales[3 * g + c], fl);
-        const double q = u / sc;
-        m += q * q;
-    }
-    return (double)a.opacities[g] * exp(-0.5 * m);
-}
 
 __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
 {
@@ -1695,7 +1613,6 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
         decode_voxel(a.kl, vkey, &i, &j, &k);
         uint32_t value = 0u;
         uint32_t g0 = 0xFFFFFFFFu;
-        FixedSum dens; // density: exact fixed-point sum of the double terms (order independent)
         if (ab)
         {
             // RGBA8 with the pair alpha in the key: max of (rgb | alpha << 24), any pair order
@@ -1714,22 +1631,13 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
             {
                 uint32_t gid = (uint32_t)(a.keys[p] & gmask);
                 g0 = gid < g0 ? gid : g0;
-                if (a.value_mode == VDBR_VALUE_RGBA8)
-                {
-                    const PrimFrame f = a.frames[gid];
-                    float vmin[3], vmax[3];
-                    voxel_box(f, i, j, k, a.vs, vmin, vmax);
-                    uint32_t rgba = pair_rgba(a.vals[gid], vmin, vmax);
-                    value = rgba > value ? rgba : value;
-                }
-                else
-                {
-                    dens.add(density_term(a, gid, i, j, k));
-                }
+                const PrimFrame f = a.frames[gid];
+                float vmin[3], vmax[3];
+                voxel_box(f, i, j, k, a.vs, vmin, vmax);
+                uint32_t rgba = pair_rgba(a.vals[gid], vmin, vmax);
+                value = rgba > value ? rgba : value;
             }
         }
-        if (a.value_mode != VDBR_VALUE_RGBA8)
-            value = __float_as_uint(dens.to_float());
         // window of the pair with the lowest gaussian id: fragment slot within its prim batch
         PrimBox bx = a.boxes[g0];
         uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
@@ -1934,13 +1842,272 @@ __global__ __launch_bounds__(HEAD_THREADS) void k_reduce_rgba(const uint64_t* __
     }
 }
 
-// per voxel: first-touch window from the lowest gaussian id, and voxel bounds
+// ============================================================================ density reduce
+// Density values per voxel: rho(v) = sum over the voxel's pairs of o * exp(-m / 2), m the squared
+// Mahalanobis distance of the voxel centre, each term in double from the preprocessed float
+// parameters (rotation from the normalised quaternion, scales floored at vs / 8 as in
+// gaussian_prim.slang:67-68). Terms are rounded once to multiples of 2^-120 and summed as
+// integers (exact, so the sum does not depend on the pair order and only the voxel bits of the
+// keys need sorting), then rounded to float.
+//
+// Pair-parallel over the sorted pairs: one wave per tile of HEAD_TILE pairs, 64 pairs per round,
+// one pair per lane. A 128-bit prefix sum over the lanes gives every run piece of the round, a
+// run still open at the round's end carries into the next round, and run heads are ranked with
+// the tile's head offset from run_heads. Runs inside a tile are stored directly; a run that
+// crosses a tile boundary commits with 64-bit atomic adds carrying between the words (exact, so
+// still order independent) on accumulators zeroed first.
+struct DensPrim
+{
+    double M[9]; // diag(1/s) R^T, row-major
+    double o;
+    float mu[3];
+    float pad;
+    double pad2[4]; // 128 bytes: one cache line per gather
+};
+
+// per Gaussian: the double frame of its density terms
+__global__ __launch_bounds__(256) void k_dens_prims(uint64_t n, const float* __restrict__ means,
+                                                    const float* __restrict__ quats, const float* __restrict__ scales,
+                                                    const float* __restrict__ opacities, float vs,
+                                                    DensPrim* __restrict__ out)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    const double w = quats[4 * g + 0], x = quats[4 * g + 1], y = quats[4 * g + 2], z = quats[4 * g + 3];
+    const double R[3][3] = { { 1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w) },
+                             { 2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w) },
+                             { 2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y) } };
+    const double fl = (double)(0.125f * vs);
+    DensPrim p;
+    for (int c = 0; c < 3; c++)
+    {
+        const double sc = fmax((double)scales[3 * g + c], fl);
+        for (int b = 0; b < 3; b++)
+            p.M[3 * c + b] = R[b][c] / sc;
+        p.mu[c] = means[3 * g + c];
+    }
+    p.o = (double)opacities[g];
+    p.pad = 0.f;
+    for (int q = 0; q < 4; q++)
+        p.pad2[q] = 0.0;
+    out[g] = p;
+}
+
+// t in [0, 2^8) -> round(t * 2^120) (half up) as a 128-bit integer
+__device__ __forceinline__ void fix120(double t, uint64_t* lo, uint64_t* hi)
+{
+    *lo = 0ull;
+    *hi = 0ull;
+    const uint64_t bits = (uint64_t)__double_as_longlong(t);
+    const int32_t be = (int32_t)((bits >> 52) & 0x7FFu);
+    if (be == 0 || (bits >> 63)) // zero, subnormal or negative: nothing to add
+        return;
+    const uint64_t m = (bits & ((1ull << 52) - 1ull)) | (1ull << 52); // t = m * 2^(be - 1075)
+    const int32_t sh = be - 1075 + 120;                               // position of m's bit 0
+    if (sh < 0)
+    {
+        if (sh >= -53)
+        {
+            const int32_t r = -sh;
+            *lo = (m + (1ull << (r - 1))) >> r;
+        }
+    }
+    else if (sh < 64)
+    {
+        *lo = m << sh;
+        *hi = sh ? (m >> (64 - sh)) : 0ull;
+    }
+    else if (sh < 76)
+        *hi = m << (sh - 64);
+}
+
+__device__ __forceinline__ void add128(uint64_t* lo, uint64_t* hi, uint64_t blo, uint64_t bhi)
+{
+    const uint64_t s = *lo + blo;
+    *hi = *hi + bhi + (s < blo ? 1ull : 0ull);
+    *lo = s;
+}
+
+// w += (a2:a1:a0) with the carries taken from the returned old words: every wrap of a word is
+// seen by exactly one add, so the total is exact whatever the order of concurrent adds
+__device__ __forceinline__ void atomic_add192(unsigned long long* w, uint64_t a0, uint64_t a1, uint64_t a2)
+{
+    const uint64_t o0 = atomicAdd(&w[0], (unsigned long long)a0);
+    const uint64_t k0 = o0 + a0 < o0 ? 1ull : 0ull;
+    const uint64_t b1 = a1 + k0;
+    uint64_t k1 = b1 < a1 ? 1ull : 0ull;
+    const uint64_t o1 = atomicAdd(&w[1], (unsigned long long)b1);
+    k1 += o1 + b1 < o1 ? 1ull : 0ull;
+    atomicAdd(&w[2], (unsigned long long)(a2 + k1));
+}
+
+// (w2:w1:w0) * 2^-120 -> nearest float of its double approximation (deterministic)
+__device__ __forceinline__ float fix120_to_float(uint64_t w0, uint64_t w1, uint64_t w2)
+{
+    const double v = (double)w2 * 0x1p8 + (double)w1 * 0x1p-56 + (double)w0 * 0x1p-120;
+    return (float)v;
+}
+
+// voxels whose run crosses into tile t (t >= 1, its first pair continues a run): zero accumulators
+__global__ __launch_bounds__(256) void k_density_cut_init(const uint64_t* __restrict__ keys, uint64_t n,
+                                                          uint32_t vshift, const uint32_t* __restrict__ tile_offsets,
+                                                          uint64_t tiles, unsigned long long* __restrict__ vox_fix,
+                                                          uint32_t* __restrict__ vox_g0)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (t >= tiles)
+        return;
+    const uint64_t p = t * HEAD_TILE;
+    if (p >= n || (keys[p] >> vshift) != (keys[p - 1] >> vshift))
+        return;
+    const uint64_t v = tile_offsets[t] - 1u;
+    vox_fix[3 * v + 0] = 0ull;
+    vox_fix[3 * v + 1] = 0ull;
+    vox_fix[3 * v + 2] = 0ull;
+    vox_g0[v] = 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(HEAD_THREADS) void k_reduce_density(const uint64_t* __restrict__ keys, uint64_t n,
+                                                                 KeyLayout kl, float vs,
+                                                                 const uint32_t* __restrict__ tile_offsets,
+                                                                 uint64_t tiles, const DensPrim* __restrict__ prims,
+                                                                 uint64_t* __restrict__ vox_key,
+                                                                 unsigned long long* __restrict__ vox_fix,
+                                                                 uint32_t* __restrict__ vox_g0)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t tile = (uint64_t)blockIdx.x * (HEAD_THREADS / 64) + (threadIdx.x >> 6);
+    if (tile >= tiles)
+        return;
+    auto vmin = [](uint32_t x, uint32_t y) { return x < y ? x : y; };
+    const uint32_t vshift = kl.gid_bits;
+    const uint64_t gmask = (1ull << kl.gid_bits) - 1ull;
+    const double vsd = (double)vs;
+    const uint64_t t0 = tile * HEAD_TILE;
+    const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
+    const uint32_t tile_base = tile_offsets[tile];
+    // the run open at the tile's end goes on in the next tile; the tile's first run began before it
+    const bool cont_next = t1 < n && (keys[t1] >> vshift) == (keys[t1 - 1] >> vshift);
+    uint64_t prev_v = t0 > 0 ? keys[t0 - 1] >> vshift : ~0ull;
+    bool open_outside = t0 > 0 && (keys[t0] >> vshift) == prev_v;
+    uint32_t heads = 0u; // run heads of the tile before this round
+    uint64_t c0 = 0ull, c1 = 0ull, c2 = 0ull;
+    uint32_t cg = 0xFFFFFFFFu; // carried piece of the run open at the previous round's end
+    for (uint64_t r0 = t0; r0 < t1; r0 += 64u)
+    {
+        const uint64_t p = r0 + lane;
+        const bool valid = p < t1;
+        const uint64_t key = valid ? keys[p] : ~0ull;
+        const uint64_t vk = key >> vshift;
+        uint64_t pv = __shfl_up(vk, 1, 64);
+        if (lane == 0u)
+            pv = prev_v;
+        const bool head = valid && (p == 0 || vk != pv);
+        const uint64_t hm = __ballot(head);
+        const uint64_t vm = __ballot(valid);
+        const uint32_t last = 63u - (uint32_t)__clzll(vm); // last valid lane
+        const uint64_t hle = hm & (lane == 63u ? ~0ull : ((2ull << lane) - 1ull));
+        const uint32_t run_start = hle ? 63u - (uint32_t)__clzll(hle) : 0u;
+        const bool cont = hle == 0ull; // the lane's run began before this round
+        const uint32_t v = tile_base + heads + (uint32_t)__popcll(hle) - 1u;
+        uint64_t lo = 0ull, hi = 0ull;
+        uint32_t gid = 0xFFFFFFFFu;
+        if (valid)
+        {
+            gid = (uint32_t)(key & gmask);
+            int32_t i, j, k;
+            decode_voxel(kl, vk, &i, &j, &k);
+            const DensPrim& q = prims[gid];
+            const double d[3] = { ((double)i + 0.5) * vsd - (double)q.mu[0], ((double)j + 0.5) * vsd - (double)q.mu[1],
+                                  ((double)k + 0.5) * vsd - (double)q.mu[2] };
+            double m = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+            {
+                const double u = q.M[3 * c + 0] * d[0] + q.M[3 * c + 1] * d[1] + q.M[3 * c + 2] * d[2];
+                m += u * u;
+            }
+            fix120(q.o * exp(-0.5 * m), &lo, &hi);
+        }
+        // inclusive 128-bit prefix sums over the lanes
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            const uint64_t ulo = __shfl_up(lo, off, 64);
+            const uint64_t uhi = __shfl_up(hi, off, 64);
+            if (lane >= (uint32_t)off)
+                add128(&lo, &hi, ulo, uhi);
+        }
+        // the lane's run piece in this round: prefix(lane) - prefix(run_start - 1)
+        const int src = run_start > 0u ? (int)run_start - 1 : 0;
+        uint64_t blo = __shfl(lo, src, 64), bhi = __shfl(hi, src, 64);
+        if (run_start == 0u)
+            blo = bhi = 0ull;
+        uint64_t s0 = lo - blo;
+        uint64_t s1 = hi - bhi - (lo < blo ? 1ull : 0ull);
+        uint64_t s2 = 0ull;
+        uint32_t sg = seg_scan_u32(gid, 0xFFFFFFFFu, lane, run_start, vmin);
+        if (cont)
+        {
+            // add the carried piece
+            const uint64_t a0 = s0 + c0;
+            const uint64_t k0 = a0 < c0 ? 1ull : 0ull;
+            const uint64_t a1 = s1 + c1;
+            const uint64_t k1 = a1 < c1 ? 1ull : 0ull;
+            const uint64_t b1 = a1 + k0;
+            const uint64_t k1b = b1 < a1 ? 1ull : 0ull;
+            s0 = a0;
+            s1 = b1;
+            s2 = c2 + k1 + k1b;
+            sg = sg < cg ? sg : cg;
+        }
+        // the run at the last lane goes on in the next round when the next pair has its voxel
+        const bool more = r0 + 64u < t1;
+        const uint64_t last_v = readlane_u64(vk, (int)last);
+        const bool carry_out = more && (keys[r0 + 64u] >> vshift) == last_v;
+        const bool run_end = valid && (lane == last || (lane < 63u && ((hm >> (lane + 1u)) & 1ull) != 0ull));
+        if (run_end && !(lane == last && carry_out))
+        {
+            const bool atomic = (cont && open_outside) || (lane == last && !more && cont_next);
+            if (atomic)
+            {
+                atomic_add192(vox_fix + 3 * (uint64_t)v, s0, s1, s2);
+                atomicMin(&vox_g0[v], sg);
+            }
+            else
+            {
+                vox_fix[3 * (uint64_t)v + 0] = s0;
+                vox_fix[3 * (uint64_t)v + 1] = s1;
+                vox_fix[3 * (uint64_t)v + 2] = s2;
+                vox_g0[v] = sg;
+            }
+        }
+        if (head)
+            vox_key[v] = vk;
+        if (more)
+        {
+            c0 = carry_out ? readlane_u64(s0, (int)last) : 0ull;
+            c1 = carry_out ? readlane_u64(s1, (int)last) : 0ull;
+            c2 = carry_out ? readlane_u64(s2, (int)last) : 0ull;
+            cg = carry_out ? (uint32_t)__builtin_amdgcn_readlane((int)sg, (int)last) : 0xFFFFFFFFu;
+            open_outside = open_outside && hm == 0ull;
+            prev_v = last_v;
+        }
+        heads += (uint32_t)__popcll(hm);
+    }
+}
+
+// per voxel: first-touch window from the lowest gaussian id, voxel bounds and (density) the
+// float value of the fixed-point sum
 __global__ __launch_bounds__(256) void k_voxel_finish(uint64_t nvox, KeyLayout kl, const uint64_t* __restrict__ vox_key,
                                                       const uint32_t* __restrict__ vox_g0, const PrimBox* __restrict__ boxes,
                                                       const uint64_t* __restrict__ foot_scan,
                                                       const uint32_t* __restrict__ win_base, uint32_t prim_batch,
                                                       uint32_t ijk_batch, uint32_t* __restrict__ vox_t,
-                                                      int32_t* __restrict__ bounds)
+                                                      int32_t* __restrict__ bounds,
+                                                      const unsigned long long* __restrict__ vox_fix,
+                                                      uint32_t* __restrict__ vox_val)
 {
     int32_t mn[3] = { 2147483647, 2147483647, 2147483647 }, mx[3] = { -2147483647, -2147483647, -2147483647 };
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -1948,6 +2115,8 @@ __global__ __launch_bounds__(256) void k_voxel_finish(uint64_t nvox, KeyLayout k
     {
         int32_t i, j, k;
         decode_voxel(kl, vox_key[v], &i, &j, &k);
+        if (vox_fix) // density: fixed-point sum -> float value
+            vox_val[v] = __float_as_uint(fix120_to_float(vox_fix[3 * v], vox_fix[3 * v + 1], vox_fix[3 * v + 2]));
         uint32_t g0 = vox_g0[v];
         PrimBox bx = boxes[g0];
         uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
@@ -2764,6 +2933,16 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
 
     // ---------------------------------------------------------------- emit + sort + reduce per slab
     const bool rgba_wave = kl.alpha_bits != 0u;
+    const bool density = prm->value_mode == VDBR_VALUE_DENSITY;
+    DensPrim* dprims = nullptr;
+    if (density && n > 0)
+    {
+        dprims = (DensPrim*)ctx->ws(WS_DENS_PRIMS, n * sizeof(DensPrim));
+        if (!dprims)
+            return -1;
+        hipLaunchKernelGGL(k_dens_prims, dim3(blocks_for(n)), dim3(256), 0, st, n, gs->means, gs->quats, gs->scales,
+                           gs->opacities, vs, dprims);
+    }
     uint64_t pair_count = 0; // all slabs
     uint64_t nvox = 0;       // all slabs
     uint64_t vcap = 0;       // capacity of the voxel arrays
@@ -2980,8 +3159,9 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         last_slab_pairs = spairs;
 
         // reduce per voxel, appended after the voxels of the previous slabs
-        uint32_t* seg = rgba_wave ? nullptr : (uint32_t*)ctx->ws(WS_SEG, (spairs + 2) * sizeof(uint32_t));
-        if (!rgba_wave && !seg)
+        const bool by_voxel = !rgba_wave && !density; // lane-per-voxel reduce over the run starts
+        uint32_t* seg = by_voxel ? (uint32_t*)ctx->ws(WS_SEG, (spairs + 2) * sizeof(uint32_t)) : nullptr;
+        if (by_voxel && !seg)
             return -1;
         uint64_t svox = 0;
         const uint32_t* tile_offsets = nullptr;
@@ -3015,7 +3195,29 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
             hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(svox), 2048u)), dim3(256), 0, st, svox,
                                kl, (const uint64_t*)skey, (const uint32_t*)sg0, (const PrimBox*)boxes,
-                               (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds);
+                               (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds,
+                               (const unsigned long long*)nullptr, (uint32_t*)nullptr);
+            VDBR_CHECK(ctx, hipGetLastError());
+        }
+        else if (svox > 0 && density)
+        {
+            const uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
+            unsigned long long* fix = (unsigned long long*)ctx->ws(WS_VOX_FIX, svox * 3 * sizeof(uint64_t));
+            if (!fix)
+                return -1;
+            VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
+            if (tiles > 1)
+                hipLaunchKernelGGL(k_density_cut_init, dim3(blocks_for(tiles - 1)), dim3(256), 0, st,
+                                   (const uint64_t*)sorted, spairs, kl.gid_bits, tile_offsets, tiles, fix, sg0);
+            hipLaunchKernelGGL(k_reduce_density, dim3((uint32_t)((tiles + 3) / 4)), dim3(HEAD_THREADS), 0, st,
+                               (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, tiles, (const DensPrim*)dprims,
+                               skey, fix, sg0);
+            VDBR_CHECK(ctx, hipGetLastError());
+            VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
+            hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(svox), 2048u)), dim3(256), 0, st, svox,
+                               kl, (const uint64_t*)skey, (const uint32_t*)sg0, (const PrimBox*)boxes,
+                               (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds,
+                               (const unsigned long long*)fix, sval);
             VDBR_CHECK(ctx, hipGetLastError());
         }
         else if (svox > 0)
@@ -3032,16 +3234,11 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             ra.win_base = win_base_dev;
             ra.prim_batch = PB;
             ra.ijk_batch = W;
-            ra.value_mode = prm->value_mode;
             ra.vs = vs;
             ra.vox_key = skey;
             ra.vox_val = sval;
             ra.vox_t = st_t;
             ra.bounds = vbounds;
-            ra.means = gs->means;
-            ra.quats = gs->quats;
-            ra.scales = gs->scales;
-            ra.opacities = gs->opacities;
             VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
             hipLaunchKernelGGL(k_reduce, dim3(blocks_for(svox)), dim3(256), 0, st, ra);
             VDBR_CHECK(ctx, hipGetLastError());
