@@ -764,6 +764,46 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
     emit_flush_wave(a, stage, stage_d, count, lane);
 }
 
+// 2^(j / 64), j = 0..63, as the nearest doubles
+__constant__ double EXP2_J64[64] = {
+    0x1.0000000000000p+0, 0x1.02c9a3e778061p+0, 0x1.059b0d3158574p+0, 0x1.0874518759bc8p+0,
+    0x1.0b5586cf9890fp+0, 0x1.0e3ec32d3d1a2p+0, 0x1.11301d0125b51p+0, 0x1.1429aaea92de0p+0,
+    0x1.172b83c7d517bp+0, 0x1.1a35beb6fcb75p+0, 0x1.1d4873168b9aap+0, 0x1.2063b88628cd6p+0,
+    0x1.2387a6e756238p+0, 0x1.26b4565e27cddp+0, 0x1.29e9df51fdee1p+0, 0x1.2d285a6e4030bp+0,
+    0x1.306fe0a31b715p+0, 0x1.33c08b26416ffp+0, 0x1.371a7373aa9cbp+0, 0x1.3a7db34e59ff7p+0,
+    0x1.3dea64c123422p+0, 0x1.4160a21f72e2ap+0, 0x1.44e086061892dp+0, 0x1.486a2b5c13cd0p+0,
+    0x1.4bfdad5362a27p+0, 0x1.4f9b2769d2ca7p+0, 0x1.5342b569d4f82p+0, 0x1.56f4736b527dap+0,
+    0x1.5ab07dd485429p+0, 0x1.5e76f15ad2148p+0, 0x1.6247eb03a5585p+0, 0x1.6623882552225p+0,
+    0x1.6a09e667f3bcdp+0, 0x1.6dfb23c651a2fp+0, 0x1.71f75e8ec5f74p+0, 0x1.75feb564267c9p+0,
+    0x1.7a11473eb0187p+0, 0x1.7e2f336cf4e62p+0, 0x1.82589994cce13p+0, 0x1.868d99b4492edp+0,
+    0x1.8ace5422aa0dbp+0, 0x1.8f1ae99157736p+0, 0x1.93737b0cdc5e5p+0, 0x1.97d829fde4e50p+0,
+    0x1.9c49182a3f090p+0, 0x1.a0c667b5de565p+0, 0x1.a5503b23e255dp+0, 0x1.a9e6b5579fdbfp+0,
+    0x1.ae89f995ad3adp+0, 0x1.b33a2b84f15fbp+0, 0x1.b7f76f2fb5e47p+0, 0x1.bcc1e904bc1d2p+0,
+    0x1.c199bdd85529cp+0, 0x1.c67f12e57d14bp+0, 0x1.cb720dcef9069p+0, 0x1.d072d4a07897cp+0,
+    0x1.d5818dcfba487p+0, 0x1.da9e603db3285p+0, 0x1.dfc97337b9b5fp+0, 0x1.e502ee78b3ff6p+0,
+    0x1.ea4afa2a490dap+0, 0x1.efa1bee615a27p+0, 0x1.f50765b6e4540p+0, 0x1.fa7c1819e90d8p+0,
+};
+
+// exp(x) for -745 <= x <= 0 with its 2^(j/64) table in LDS (t64): x = k ln2/64 + r, |r| <= ln2/128
+// (Cody-Waite reduction, ln2/64 split into a 36-bit head, exact times k, and a tail), exp(r) by
+// the degree-5 Taylor polynomial (truncation below 4e-17 relative), scaled by 2^(k >> 6) and the
+// table entry k & 63. Within 2 ulp of exp(x). Six double constants instead of a library exp's
+// thirteen, which the register allocator otherwise keeps live across the pair loops.
+__device__ __forceinline__ double exp_neg(double x, const double* __restrict__ t64)
+{
+    const double kd = rint(x * 0x1.71547652b82fep+6); // 64 / ln2
+    const double r0 = fma(-kd, 0x1.62e42fee00000p-7, x);
+    const double r = fma(-kd, 0x1.a39ef35793c76p-39, r0);
+    double p = 1.0 / 120.0;
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    const int32_t k = (int32_t)kd;
+    return ldexp(t64[k & 63] * p, k >> 6);
+}
+
 // ============================================================================ density backward
 // Adjoint of the density value mode: rho(v) = sum over pairs (v, g) of o_g exp(-m_g(v) / 2), with
 // m = |diag(1/s) R^T (x_v - mu)|^2 at the voxel centre, differentiated with respect to the raw
@@ -785,13 +825,14 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
 // Each wave takes 64 consecutive AABB rows and cuts their candidate intervals into segments of
 // at most BWD_SEG voxels. Per round it walks 64 segments with the FP32 predicate (its lanes step
 // through at most BWD_SEG voxels instead of the longest of 64 rows) and stages the kept voxels in
-// LDS in segment order; the double-precision terms then run over the staged pairs 64 at a time
-// with every lane busy. Segments keep row order, so Gaussian ids are non-decreasing over the
-// staged pairs: a segmented scan sums each Gaussian's pairs within a group of 64, and a group's
-// last Gaussian carries into the next group. A Gaussian owning the first or last row of a 64-row
-// chunk leaves its chunk sum in that chunk's record (the finalize pass sums records in chunk
-// order); one strictly inside a chunk stores its sum directly. Every sum runs in an order fixed
-// by the rows, so the result does not depend on scheduling.
+// LDS in segment order; the double-precision terms then run over the staged pairs with every lane
+// busy, each lane summing a contiguous slice sequentially. Segments keep row order, so Gaussian ids
+// are non-decreasing over the staged pairs: runs complete inside a lane are stored directly, one
+// segmented scan over the lanes per round joins the runs that cross lanes, and the round's last
+// run carries into the next round. A Gaussian owning the first or last row of a 64-row chunk
+// leaves its chunk sum in that chunk's record (the finalize pass sums records in chunk order); one
+// strictly inside a chunk stores its sum directly. Every sum runs in an order fixed by the rows,
+// so the result does not depend on scheduling.
 constexpr int BWD_ACC = 10;
 constexpr int32_t BWD_SEG = 8;
 constexpr uint32_t LT_NONE = 0xFFFFFFFFu;
@@ -1103,6 +1144,11 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+__device__ __forceinline__ double readlane_f64(double x, int l)
+{
+    return __longlong_as_double((long long)readlane_u64((uint64_t)__double_as_longlong(x), l));
+}
+
 // last lane L whose (non-decreasing) exclusive count excl(L) is <= idx
 __device__ __forceinline__ uint32_t wave_owner(uint32_t excl, uint32_t idx)
 {
@@ -1127,9 +1173,18 @@ __global__ __launch_bounds__(EMIT_THREADS, 4) void k_density_bwd(BwdArgs a)
     __shared__ uint64_t g_range[2];
     __shared__ double carry_all[EMIT_THREADS / 64][BWD_ACC];
     __shared__ uint32_t staged_all[EMIT_THREADS / 64][64 * BWD_SEG]; // leaf ordinal << 3 | step
+    __shared__ uint32_t seg_all[EMIT_THREADS / 64][5][64];           // per segment: id, i, j, k0, pair offset
+    __shared__ double fpiece_all[EMIT_THREADS / 64][BWD_ACC * 64];    // per lane: sums before its first change
+    __shared__ double s_exp2[64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     double* carry = carry_all[wave];
     uint32_t* staged = staged_all[wave];
+    uint32_t* seg_gid = seg_all[wave][0];
+    int32_t* seg_i = (int32_t*)seg_all[wave][1];
+    int32_t* seg_j = (int32_t*)seg_all[wave][2];
+    int32_t* seg_k0 = (int32_t*)seg_all[wave][3];
+    uint32_t* seg_ex = seg_all[wave][4];
+    double* fpiece = fpiece_all[wave];
     const uint64_t row0 = (uint64_t)blockIdx.x * EMIT_ROWS;
     if (threadIdx.x == 0)
     {
@@ -1139,6 +1194,8 @@ __global__ __launch_bounds__(EMIT_THREADS, 4) void k_density_bwd(BwdArgs a)
             last = a.total_rows - 1;
         g_range[1] = find_prim(a.row_scan, 0, a.n - 1, last);
     }
+    if (threadIdx.x < 64u)
+        s_exp2[threadIdx.x] = EXP2_J64[threadIdx.x];
     __syncthreads();
     const uint64_t glo = g_range[0], ghi = g_range[1];
     const float vs = a.vs;
@@ -1212,117 +1269,145 @@ __global__ __launch_bounds__(EMIT_THREADS, 4) void k_density_bwd(BwdArgs a)
             });
             uint32_t kept_total;
             const uint32_t kexcl = wave_excl_add(kept, lane, &kept_total);
-            wave_sync_lds();
-            // terms: staged pairs in segment order, 64 per group
-            for (uint32_t p0 = 0; p0 < kept_total; p0 += 64u)
+            if (kept_total == 0u)
             {
-                const uint32_t pidx = p0 + lane;
-                const uint32_t sl = wave_owner(kexcl, pidx);
-                const uint32_t t_in = pidx - (uint32_t)__shfl((int)kexcl, (int)sl, 64);
-                const uint64_t pg0 = __shfl(sg, (int)sl, 64);
-                const int32_t pi = __shfl(si, (int)sl, 64);
-                const int32_t pj = __shfl(sj, (int)sl, 64);
-                const int32_t pk0 = __shfl(sk0, (int)sl, 64);
-                const bool valid = pidx < kept_total;
-                uint64_t pg = ~0ull;
-                double acc[BWD_ACC];
+                wave_sync_lds();
+                continue;
+            }
+            seg_gid[lane] = (uint32_t)sg;
+            seg_i[lane] = si;
+            seg_j[lane] = sj;
+            seg_k0[lane] = sk0;
+            seg_ex[lane] = kexcl;
+            wave_sync_lds();
+            // terms: lane L takes the staged pairs [L K, L K + K) in segment order and sums them
+            // sequentially; F = the piece before its first Gaussian change (it continues the run of
+            // the pair before), runs between changes are complete, A = the piece after the last change
+            const uint32_t K = (kept_total + 63u) >> 6; // at most BWD_SEG
+            const uint32_t q0 = lane * K;
+            const uint32_t q1 = min(q0 + K, kept_total);
+            uint32_t sl = wave_owner(kexcl, q0 < kept_total ? q0 : 0u);
+            const uint32_t sprev = wave_owner(kexcl, q0 > 0u ? min(q0 - 1u, kept_total - 1u) : 0u);
+            const uint32_t carry32 = carry_g == ~0ull ? 0xFFFFFFFFu : (uint32_t)carry_g;
+            const uint32_t gprev = q0 == 0u ? carry32 : seg_gid[sprev];
+            double A[BWD_ACC];
 #pragma unroll
-                for (int t = 0; t < BWD_ACC; t++)
-                    acc[t] = 0.0;
-                if (valid)
+            for (int t = 0; t < BWD_ACC; t++)
+                A[t] = 0.0;
+            uint32_t cur = gprev, heads = 0u;
+            bool hit_first = false, hit_last = false;
+#pragma unroll 1
+            for (uint32_t q = q0; q < q1; q++)
+            {
+                while (sl < 63u && q >= seg_ex[sl + 1u])
+                    sl++;
+                const uint32_t g = seg_gid[sl];
+                if (g != cur)
                 {
-                    pg = pg0;
-                    const uint32_t ent = staged[sl * BWD_SEG + t_in];
-                    const int32_t k = pk0 + (int32_t)(ent & 7u);
-                    const uint32_t nv = (((uint32_t)pi & 7u) << 6) | (((uint32_t)pj & 7u) << 3) | ((uint32_t)k & 7u);
-                    const double r = (double)a.leaf_grad[(uint64_t)(ent >> 3) * 512u + nv];
-                    const BwdPrim& sp = a.prims[pg0];
-                    const double d[3] = { ((double)pi + 0.5) * vsd - sp.mu[0], ((double)pj + 0.5) * vsd - sp.mu[1],
-                                          ((double)k + 0.5) * vsd - sp.mu[2] };
-                    double v[3];
-#pragma unroll
-                    for (int cc = 0; cc < 3; cc++)
-                        v[cc] = sp.M[3 * cc + 0] * d[0] + sp.M[3 * cc + 1] * d[1] + sp.M[3 * cc + 2] * d[2];
-                    const double m = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-                    const double e = exp(-0.5 * m);
-                    const double W = r * sp.o * e;
-                    const double w0 = W * v[0], w1 = W * v[1], w2 = W * v[2];
-                    acc[0] = r * e;
-                    acc[1] = w0;
-                    acc[2] = w1;
-                    acc[3] = w2;
-                    acc[4] = w0 * v[0];
-                    acc[5] = w1 * v[1];
-                    acc[6] = w2 * v[2];
-                    acc[7] = w0 * v[1];
-                    acc[8] = w0 * v[2];
-                    acc[9] = w1 * v[2];
-                }
-                // segmented inclusive scan over the lanes by Gaussian id (ids are non-decreasing)
-#pragma unroll
-                for (int off = 1; off < 64; off <<= 1)
-                {
-                    const uint64_t kup = __shfl_up(pg, off, 64);
-                    const bool take = lane >= (uint32_t)off && kup == pg;
-#pragma unroll
-                    for (int t = 0; t < BWD_ACC; t++)
+                    if (heads == 0u)
                     {
-                        double u = __shfl_up(acc[t], off, 64);
-                        if (take)
-                            acc[t] += u;
-                    }
-                }
-                const uint32_t nlast = min(kept_total - p0, 64u) - 1u; // last lane holding a pair
-                const uint64_t knext = __shfl_down(pg, 1, 64);
-                const bool run_end = valid && (lane == nlast || knext != pg);
-                const uint64_t rfirst = readlane_u64(pg, 0);
-                const uint64_t rlast = readlane_u64(pg, (int)nlast);
-                // the previous group's last Gaussian joins this group's first run, or is complete
-                if (carry_g != ~0ull)
-                {
-                    if (carry_g == rfirst)
-                    {
-                        if (run_end && pg == rfirst)
-                        {
-#pragma unroll
-                            for (int t = 0; t < BWD_ACC; t++)
-                                acc[t] = carry[t] + acc[t];
-                        }
+                        // F waits in LDS for the cross-lane join
+                        for (int t = 0; t < BWD_ACC; t++)
+                            fpiece[t * 64 + lane] = A[t];
                     }
                     else
                     {
-                        if (lane == 0u)
-                        {
-                            double* dst = dest(carry_g);
-                            for (int t = 0; t < BWD_ACC; t++)
-                                dst[t] = carry[t];
-                        }
-                        first_done = first_done || carry_g == first_key;
-                        last_done = last_done || carry_g == last_key;
+                        double* dst = dest(cur);
+                        for (int t = 0; t < BWD_ACC; t++)
+                            dst[t] = A[t];
+                        hit_first = hit_first || cur == first_key;
+                        hit_last = hit_last || cur == last_key;
                     }
-                }
-                wave_sync_lds();
-                // the group's last run waits for the next group; the others are complete
-                const bool to_carry = run_end && lane == nlast;
-                const bool emit = run_end && !to_carry;
-                if (to_carry)
-                {
+                    heads++;
+                    cur = g;
 #pragma unroll
                     for (int t = 0; t < BWD_ACC; t++)
-                        carry[t] = acc[t];
+                        A[t] = 0.0;
                 }
-                if (emit)
-                {
-                    double* dst = dest(pg);
+                const uint32_t ent = staged[sl * BWD_SEG + (q - seg_ex[sl])];
+                const int32_t pi = seg_i[sl], pj = seg_j[sl];
+                const int32_t k = seg_k0[sl] + (int32_t)(ent & 7u);
+                const uint32_t nv = (((uint32_t)pi & 7u) << 6) | (((uint32_t)pj & 7u) << 3) | ((uint32_t)k & 7u);
+                const double r = (double)a.leaf_grad[(uint64_t)(ent >> 3) * 512u + nv];
+                const BwdPrim& spm = a.prims[g];
+                const double d[3] = { ((double)pi + 0.5) * vsd - spm.mu[0], ((double)pj + 0.5) * vsd - spm.mu[1],
+                                      ((double)k + 0.5) * vsd - spm.mu[2] };
+                double v[3];
 #pragma unroll
-                    for (int t = 0; t < BWD_ACC; t++)
-                        dst[t] = acc[t];
-                }
-                first_done = first_done || __ballot(emit && pg == first_key) != 0ull;
-                last_done = last_done || __ballot(emit && pg == last_key) != 0ull;
-                carry_g = rlast;
-                wave_sync_lds();
+                for (int cc = 0; cc < 3; cc++)
+                    v[cc] = spm.M[3 * cc + 0] * d[0] + spm.M[3 * cc + 1] * d[1] + spm.M[3 * cc + 2] * d[2];
+                const double m = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                const double e = exp_neg(-0.5 * m, s_exp2);
+                const double W = r * spm.o * e;
+                const double w0 = W * v[0], w1 = W * v[1], w2 = W * v[2];
+                A[0] += r * e;
+                A[1] += w0;
+                A[2] += w1;
+                A[3] += w2;
+                A[4] += w0 * v[0];
+                A[5] += w1 * v[1];
+                A[6] += w2 * v[2];
+                A[7] += w0 * v[1];
+                A[8] += w0 * v[2];
+                A[9] += w1 * v[2];
             }
+            // cross-lane join: segmented inclusive scan over the lanes of A (the piece after the
+            // lane's last change, or the whole lane inside a run), segments starting at lanes with
+            // a change
+            const bool hh = heads > 0u;
+            const uint64_t hb = __ballot(hh);
+            const uint64_t hle = hb & (lane == 63u ? ~0ull : ((2ull << lane) - 1ull));
+            const uint32_t segid = hle ? 63u - (uint32_t)__clzll(hle) : 0u;
+            double* S = A;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1)
+            {
+                const uint32_t sid = (uint32_t)__shfl_up((int)segid, off, 64);
+                const bool take = lane >= (uint32_t)off && sid == segid;
+#pragma unroll
+                for (int t = 0; t < BWD_ACC; t++)
+                {
+                    const double u = __shfl_up(S[t], off, 64);
+                    if (take)
+                        S[t] += u;
+                }
+            }
+            // the run coming into a lane with a change ends at that change: the scan of the lanes
+            // before it, the lane's F and, for the round's first run, the carried piece
+            const bool round_first = (hb & ((1ull << lane) - 1ull)) == 0ull;
+            const bool add_carry = round_first && carry32 != 0xFFFFFFFFu && gprev == carry32;
+            double inc[BWD_ACC];
+#pragma unroll
+            for (int t = 0; t < BWD_ACC; t++)
+            {
+                const double u = __shfl_up(S[t], 1, 64);
+                inc[t] = (lane > 0u ? u : 0.0) + (hh ? fpiece[t * 64 + lane] : 0.0);
+            }
+            if (hh && gprev != 0xFFFFFFFFu)
+            {
+                double* dst = dest(gprev);
+                for (int t = 0; t < BWD_ACC; t++)
+                    dst[t] = add_carry ? carry[t] + inc[t] : inc[t];
+                hit_first = hit_first || gprev == first_key;
+                hit_last = hit_last || gprev == last_key;
+            }
+            first_done = first_done || __ballot(hit_first) != 0ull;
+            last_done = last_done || __ballot(hit_last) != 0ull;
+            // the run open at the round's last pair carries into the next round
+            const uint32_t lastl = (kept_total - 1u) / K;
+            const uint32_t last_g = (uint32_t)__builtin_amdgcn_readlane((int)cur, (int)lastl);
+            double nc[BWD_ACC];
+#pragma unroll
+            for (int t = 0; t < BWD_ACC; t++)
+                nc[t] = readlane_f64(S[t], (int)lastl);
+            wave_sync_lds();
+            if (lane == 0u)
+            {
+                for (int t = 0; t < BWD_ACC; t++)
+                    carry[t] = hb == 0ull ? carry[t] + nc[t] : nc[t];
+            }
+            carry_g = last_g;
+            wave_sync_lds();
         }
         if (carry_g != ~0ull)
         {
@@ -1664,16 +1749,17 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
 // Runs inside the wave are stored directly; runs cut by a wave boundary commit with atomicMax /
 // atomicMin (order independent: deterministic). Voxel indices come from the per-tile head counts
 // of run_heads (one tile of HEAD_TILE pairs per block) and an in-block scan of the lane counts.
-constexpr int RED_ITEMS = HEAD_ITEMS;
+constexpr int RED_THREADS = 256;
+constexpr int RED_ITEMS = HEAD_TILE / RED_THREADS; // one run-head tile per block
 
-__global__ __launch_bounds__(HEAD_THREADS) void k_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
                                                               uint32_t vshift, uint32_t gid_bits,
                                                               const uint32_t* __restrict__ tile_offsets,
                                                               const PrimVal* __restrict__ vals,
                                                               uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
                                                               uint32_t* __restrict__ vox_g0)
 {
-    __shared__ uint32_t wave_heads[HEAD_THREADS / 64];
+    __shared__ uint32_t wave_heads[RED_THREADS / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t gmask = (1ull << gid_bits) - 1ull;
     const uint64_t pb = (uint64_t)blockIdx.x * HEAD_TILE + (uint64_t)threadIdx.x * RED_ITEMS;
@@ -1976,6 +2062,10 @@ __global__ __launch_bounds__(HEAD_THREADS) void k_reduce_density(const uint64_t*
                                                                  unsigned long long* __restrict__ vox_fix,
                                                                  uint32_t* __restrict__ vox_g0)
 {
+    __shared__ double s_exp2[64];
+    if (threadIdx.x < 64u)
+        s_exp2[threadIdx.x] = EXP2_J64[threadIdx.x];
+    __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t tile = (uint64_t)blockIdx.x * (HEAD_THREADS / 64) + (threadIdx.x >> 6);
     if (tile >= tiles)
@@ -2028,7 +2118,7 @@ __global__ __launch_bounds__(HEAD_THREADS) void k_reduce_density(const uint64_t*
                 const double u = q.M[3 * c + 0] * d[0] + q.M[3 * c + 1] * d[1] + q.M[3 * c + 2] * d[2];
                 m += u * u;
             }
-            fix120(q.o * exp(-0.5 * m), &lo, &hi);
+            fix120(q.o * exp_neg(-0.5 * m, s_exp2), &lo, &hi);
         }
         // inclusive 128-bit prefix sums over the lanes
 #pragma unroll
@@ -3188,7 +3278,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             VDBR_CHECK(ctx, hipMemsetAsync(sg0, 0xFF, svox * sizeof(uint32_t), st));
             uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
             VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
-            hipLaunchKernelGGL(k_reduce_rgba, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, st, (const uint64_t*)sorted,
+            hipLaunchKernelGGL(k_reduce_rgba, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st, (const uint64_t*)sorted,
                                spairs, kl.gid_bits + kl.alpha_bits, kl.gid_bits, tile_offsets, (const PrimVal*)pvals,
                                skey, sval, sg0);
             VDBR_CHECK(ctx, hipGetLastError());
