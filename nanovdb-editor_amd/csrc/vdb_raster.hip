@@ -1765,6 +1765,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_rgba(const uint64_t* __r
     const uint64_t pb = (uint64_t)blockIdx.x * HEAD_TILE + (uint64_t)threadIdx.x * RED_ITEMS;
     auto vmax = [](uint32_t x, uint32_t y) { return x > y ? x : y; };
     auto vmin = [](uint32_t x, uint32_t y) { return x < y ? x : y; };
+    // the tile's voxel offset and the voxel total, loaded with the keys
+    const uint32_t tile_base = tile_offsets[blockIdx.x];
+    const uint32_t nvox = tile_offsets[gridDim.x]; // scan total: writes stay below it
 
     uint64_t key[RED_ITEMS];
     if (pb + RED_ITEMS <= n)
@@ -1832,8 +1835,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_rgba(const uint64_t* __r
         rgba[i] = ((need >> i) & 1u) ? ((vals[gid].rgb_cull & 0x00FFFFFFu) | ((uint32_t)(key[i] & 255u) << 24)) : 0u;
     }
     __syncthreads();
-    uint32_t wbase = tile_offsets[blockIdx.x];
-    const uint32_t nvox = tile_offsets[gridDim.x]; // scan total: writes stay below it
+    uint32_t wbase = tile_base;
     for (uint32_t w = 0; w < wave; w++)
         wbase += wave_heads[w];
     const uint32_t base = wbase + excl; // voxel index of the lane's first head
