@@ -1740,35 +1740,45 @@ __global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
     commit_bounds(mn, mx, a.bounds);
 }
 
-// RGBA8 with the alpha in the key: max over voxel runs of the sorted pairs (reduce by key).
-// Each lane owns RED_ITEMS consecutive pairs and walks them sequentially in registers; one DPP
-// segmented scan over the lanes joins the runs that cross lanes. The packed value
-// max(r8 | g8 << 8 | b8 << 16 | a8 << 24) is decided by the alpha first and the alpha rides in
-// the key, so colour bytes are gathered only for pairs whose alpha is a running maximum of their
-// run inside the lane (every pair of maximal alpha is one), all gathers in flight together.
-// Runs inside the wave are stored directly; runs cut by a wave boundary commit with atomicMax /
-// atomicMin (order independent: deterministic). Voxel indices come from the per-tile head counts
-// of run_heads (one tile of HEAD_TILE pairs per block) and an in-block scan of the lane counts.
+// Block shape of the tile reduces: one run-head tile of HEAD_TILE pairs per block, RED_ITEMS
+// consecutive pairs per lane.
 constexpr int RED_THREADS = 256;
 constexpr int RED_ITEMS = HEAD_TILE / RED_THREADS; // one run-head tile per block
 
-__global__ __launch_bounds__(RED_THREADS) void k_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
-                                                              uint32_t vshift, uint32_t gid_bits,
-                                                              const uint32_t* __restrict__ tile_offsets,
-                                                              const PrimVal* __restrict__ vals,
-                                                              uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
-                                                              uint32_t* __restrict__ vox_g0)
+// ============================================================================ RGBA8 reduce by leaf
+// With the RGBA8 alpha in the key, the pairs are sorted by their leaf bits only (the 9 in-leaf
+// slot bits and the Gaussian id stay unsorted): each voxel's value, the maximum packed
+// r8 | g8 << 8 | b8 << 16 | a8 << 24 over its pairs, and its lowest Gaussian id are independent
+// of the pair order. One block per tile of HEAD_TILE pairs folds its pairs into LDS slots of up to
+// LR_LEAVES leaves (pass 1: maximum alpha and lowest id per slot; pass 2: colour gathers only
+// for pairs holding their slot's maximum alpha), then merges the slots into the leaf staging
+// arrays with atomicMax / atomicMin (order independent). Leaves past LR_LEAVES in one tile go
+// straight to the staging arrays. A count pass, a scan and a write pass then list the active
+// voxels in key order.
+constexpr int LR_LEAVES = 8;
+
+__global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
+                                                                  uint32_t lshift, uint32_t gid_bits,
+                                                                  const uint32_t* __restrict__ tile_offsets,
+                                                                  const PrimVal* __restrict__ vals,
+                                                                  uint32_t* __restrict__ lval, uint32_t* __restrict__ lg0,
+                                                                  uint64_t* __restrict__ lkey)
 {
+    __shared__ uint32_t s_val[LR_LEAVES * 512];
+    __shared__ uint32_t s_g0[LR_LEAVES * 512];
     __shared__ uint32_t wave_heads[RED_THREADS / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t gmask = (1ull << gid_bits) - 1ull;
-    const uint64_t pb = (uint64_t)blockIdx.x * HEAD_TILE + (uint64_t)threadIdx.x * RED_ITEMS;
-    auto vmax = [](uint32_t x, uint32_t y) { return x > y ? x : y; };
-    auto vmin = [](uint32_t x, uint32_t y) { return x < y ? x : y; };
-    // the tile's voxel offset and the voxel total, loaded with the keys
+    const uint64_t t0 = (uint64_t)blockIdx.x * HEAD_TILE;
+    const uint64_t pb = t0 + (uint64_t)threadIdx.x * RED_ITEMS;
     const uint32_t tile_base = tile_offsets[blockIdx.x];
-    const uint32_t nvox = tile_offsets[gridDim.x]; // scan total: writes stay below it
-
+    // the tile's first pair continues a leaf of the previous tile
+    const bool first_cont = t0 > 0 && (keys[t0] >> lshift) == (keys[t0 - 1] >> lshift);
+    for (uint32_t x = threadIdx.x; x < (uint32_t)(LR_LEAVES * 512); x += RED_THREADS)
+    {
+        s_val[x] = 0u;
+        s_g0[x] = 0xFFFFFFFFu;
+    }
     uint64_t key[RED_ITEMS];
     if (pb + RED_ITEMS <= n)
     {
@@ -1787,145 +1797,156 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_rgba(const uint64_t* __r
         for (int i = 0; i < RED_ITEMS; i++)
             key[i] = pb + i < n ? keys[pb + i] : ~0ull;
     }
-    // voxel key before the lane's first pair: the previous lane's last, or a load at lane 0
+    // leaf heads: positions whose leaf differs from the previous pair's
     uint64_t prev = wave_shr1_u64(0ull, key[RED_ITEMS - 1]);
     if (lane == 0u)
         prev = (pb > 0 && pb < n) ? keys[pb - 1] : ~0ull;
-    prev >>= vshift;
+    prev >>= lshift;
     uint32_t headbits = 0u;
     {
         uint64_t pv = prev;
 #pragma unroll
         for (int i = 0; i < RED_ITEMS; i++)
         {
-            const uint64_t vk = key[i] >> vshift;
-            const bool valid = pb + i < n;
-            if (valid && (pb + i == 0 || vk != pv))
+            const uint64_t lk = key[i] >> lshift;
+            if (pb + i < n && (pb + i == 0 || lk != pv))
                 headbits |= 1u << i;
-            pv = vk;
+            pv = lk;
         }
     }
-    const uint32_t cnt = (uint32_t)__popc(headbits);
     uint32_t wtot;
-    const uint32_t excl = wave_excl_add(cnt, lane, &wtot);
+    const uint32_t excl = wave_excl_add((uint32_t)__popc(headbits), lane, &wtot);
     if (lane == 0u)
         wave_heads[wave] = wtot;
-    // records of the running max alpha inside each run piece of the lane -> colour gathers
-    uint32_t need = 0u;
-    {
-        uint32_t am = 0u;
-#pragma unroll
-        for (int i = 0; i < RED_ITEMS; i++)
-        {
-            const uint32_t a8 = (uint32_t)(key[i] & 255u);
-            if ((headbits >> i) & 1u)
-                am = 0u;
-            if (pb + i < n && a8 >= am)
-            {
-                need |= 1u << i;
-                am = a8;
-            }
-        }
-    }
-    uint32_t rgba[RED_ITEMS];
-#pragma unroll
-    for (int i = 0; i < RED_ITEMS; i++)
-    {
-        const uint32_t gid = (uint32_t)((key[i] >> 8) & gmask);
-        rgba[i] = ((need >> i) & 1u) ? ((vals[gid].rgb_cull & 0x00FFFFFFu) | ((uint32_t)(key[i] & 255u) << 24)) : 0u;
-    }
     __syncthreads();
-    uint32_t wbase = tile_base;
-    for (uint32_t w = 0; w < wave; w++)
-        wbase += wave_heads[w];
-    const uint32_t base = wbase + excl; // voxel index of the lane's first head
-
-    // sequential runs: F = piece before the first head (continues an earlier run), complete runs
-    // between heads (stored here, as they are inside the wave), L = piece from the last head on
-    uint32_t fval = 0u, fg = 0xFFFFFFFFu;
-    uint32_t cval = 0u, cg = 0xFFFFFFFFu;
-    int32_t cur = -1; // ordinal of the head of the current run inside the lane, -1 before the first
+    uint32_t before = excl, tile_heads = 0u;
+    for (uint32_t w = 0; w < RED_THREADS / 64u; w++)
+    {
+        before += w < wave ? wave_heads[w] : 0u;
+        tile_heads += wave_heads[w];
+    }
+    const uint32_t cont = first_cont ? 1u : 0u;
+    // pass 1: maximum alpha and lowest Gaussian id per slot; leaves past LR_LEAVES go to the
+    // staging arrays directly (with their colours)
+    uint32_t h = before;
 #pragma unroll
     for (int i = 0; i < RED_ITEMS; i++)
     {
-        const bool valid = pb + i < n;
+        if ((headbits >> i) & 1u)
+            h++;
+        if (pb + i >= n)
+            continue;
+        const uint32_t gl = tile_base + h - 1u; // leaf index of the pair
+        const uint32_t ord = h - 1u + cont;     // leaf ordinal within the tile
+        const uint32_t slot = (uint32_t)(key[i] >> (lshift - 9u)) & 511u;
+        const uint32_t a8 = (uint32_t)(key[i] & 255u);
         const uint32_t gid = (uint32_t)((key[i] >> 8) & gmask);
         if ((headbits >> i) & 1u)
+            lkey[gl] = key[i] >> lshift;
+        if (ord < (uint32_t)LR_LEAVES)
         {
-            if (cur < 0)
-            {
-                fval = cval;
-                fg = cg;
-            }
-            else if (base + (uint32_t)cur < nvox)
-            {
-                vox_val[base + cur] = cval;
-                vox_g0[base + cur] = cg;
-            }
-            cur++;
-            if (base + (uint32_t)cur < nvox)
-                vox_key[base + cur] = key[i] >> vshift;
-            cval = 0u;
-            cg = 0xFFFFFFFFu;
+            atomicMax(&s_val[ord * 512u + slot], a8 << 24);
+            atomicMin(&s_g0[ord * 512u + slot], gid);
         }
-        if (valid)
+        else
         {
-            cval = rgba[i] > cval ? rgba[i] : cval;
-            cg = gid < cg ? gid : cg;
+            atomicMax(&lval[(uint64_t)gl * 512u + slot], (vals[gid].rgb_cull & 0x00FFFFFFu) | (a8 << 24));
+            atomicMin(&lg0[(uint64_t)gl * 512u + slot], gid);
         }
     }
-    if (cur < 0)
+    __syncthreads();
+    // pass 2: colours of the pairs that hold their slot's maximum alpha
+    h = before;
+#pragma unroll
+    for (int i = 0; i < RED_ITEMS; i++)
     {
-        // no head in the lane: the whole lane continues the incoming run
-        fval = cval;
-        fg = cg;
-    }
-    // cross-lane join. X = L for lanes with a head (starts a new segment), else the whole lane.
-    const bool has_head = cnt != 0u;
-    const uint64_t hl = __ballot(has_head);
-    const uint64_t below = lane ? (hl & ((1ull << lane) - 1ull)) : 0ull;
-    const uint32_t seg_start = has_head ? lane : (below ? 63u - (uint32_t)__clzll(below) : 0u);
-    const uint32_t sval = seg_scan_u32(has_head ? cval : fval, 0u, lane, seg_start, vmax);
-    const uint32_t sg = seg_scan_u32(has_head ? cg : fg, 0xFFFFFFFFu, lane, seg_start, vmin);
-    // incoming run of a lane with a head: previous lanes' accumulated piece + F
-    const uint32_t in_val = dpp_u32<0x138, 0xF>(0u, sval);
-    const uint32_t in_g = dpp_u32<0x138, 0xF>(0xFFFFFFFFu, sg);
-    const bool any_valid = pb < n;
-    if (has_head && any_valid)
-    {
-        // the incoming run ends at this lane's first head (F may be empty)
-        const bool started_in_wave = below != 0ull; // a head in an earlier lane of this wave
-        const uint32_t v = base - 1u;
-        const bool exists = lane > 0u || !(headbits & 1u); // lane 0 with a head at 0: nothing incoming
-        if (exists && base > 0u && v < nvox)
+        if ((headbits >> i) & 1u)
+            h++;
+        const uint32_t ord = h - 1u + cont;
+        if (pb + i >= n || ord >= (uint32_t)LR_LEAVES)
+            continue;
+        const uint32_t slot = (uint32_t)(key[i] >> (lshift - 9u)) & 511u;
+        const uint32_t a8 = (uint32_t)(key[i] & 255u);
+        if ((s_val[ord * 512u + slot] >> 24) == a8)
         {
-            uint32_t tv = lane > 0u ? (in_val > fval ? in_val : fval) : fval;
-            uint32_t tg = lane > 0u ? (in_g < fg ? in_g : fg) : fg;
-            if (started_in_wave)
-            {
-                vox_val[v] = tv;
-                vox_g0[v] = tg;
-            }
-            else
-            {
-                atomicMax(&vox_val[v], tv);
-                atomicMin(&vox_g0[v], tg);
-            }
+            const uint32_t gid = (uint32_t)((key[i] >> 8) & gmask);
+            atomicMax(&s_val[ord * 512u + slot], (vals[gid].rgb_cull & 0x00FFFFFFu) | (a8 << 24));
         }
     }
-    // the run open at the wave's last valid lane continues into the next wave (or ends there)
-    const uint64_t vl = __ballot(any_valid);
-    if (vl)
+    __syncthreads();
+    // merge the LDS leaves into the staging arrays
+    const uint32_t leaves = tile_heads + cont;
+    const uint32_t nl = leaves < (uint32_t)LR_LEAVES ? leaves : (uint32_t)LR_LEAVES;
+    const uint32_t gbase = tile_base - cont;
+    for (uint32_t x = threadIdx.x; x < nl * 512u; x += RED_THREADS)
     {
-        const uint32_t last = 63u - (uint32_t)__clzll(vl);
-        if (lane == last)
+        const uint32_t g = s_g0[x];
+        if (g != 0xFFFFFFFFu)
         {
-            const uint32_t v = has_head ? base + cnt - 1u : base - 1u;
-            if (v < nvox && (has_head || base > 0u))
-            {
-                atomicMax(&vox_val[v], sval);
-                atomicMin(&vox_g0[v], sg);
-            }
+            const uint64_t at = (uint64_t)(gbase + x / 512u) * 512u + (x & 511u);
+            atomicMax(&lval[at], s_val[x]);
+            atomicMin(&lg0[at], g);
+        }
+    }
+}
+
+// one wave per leaf: number of voxels with a pair
+__global__ __launch_bounds__(256) void k_leaf_count(const uint32_t* __restrict__ lg0, uint64_t nleaf,
+                                                    uint32_t* __restrict__ cnt)
+{
+    const uint64_t leaf = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (leaf >= nleaf)
+        return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint4 a = *(const uint4*)(lg0 + leaf * 512u + lane * 8u);
+    const uint4 b = *(const uint4*)(lg0 + leaf * 512u + lane * 8u + 4u);
+    uint32_t c = (a.x != 0xFFFFFFFFu) + (a.y != 0xFFFFFFFFu) + (a.z != 0xFFFFFFFFu) + (a.w != 0xFFFFFFFFu) +
+                 (b.x != 0xFFFFFFFFu) + (b.y != 0xFFFFFFFFu) + (b.z != 0xFFFFFFFFu) + (b.w != 0xFFFFFFFFu);
+    for (int off = 32; off > 0; off >>= 1)
+        c += (uint32_t)__shfl_xor((int)c, off, 64);
+    if (lane == 0u)
+        cnt[leaf] = c;
+}
+
+// one wave per leaf: its voxels with a pair, in slot (= key) order, at the leaf's offset
+__global__ __launch_bounds__(256) void k_leaf_write(const uint32_t* __restrict__ lval, const uint32_t* __restrict__ lg0,
+                                                    const uint64_t* __restrict__ lkey,
+                                                    const uint32_t* __restrict__ lscan, uint64_t nleaf,
+                                                    uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
+                                                    uint32_t* __restrict__ vox_g0)
+{
+    const uint64_t leaf = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (leaf >= nleaf)
+        return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t at = leaf * 512u + lane * 8u;
+    uint32_t g[8], v[8];
+    {
+        const uint4 a = *(const uint4*)(lg0 + at);
+        const uint4 b = *(const uint4*)(lg0 + at + 4u);
+        g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
+        g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
+        const uint4 c = *(const uint4*)(lval + at);
+        const uint4 d = *(const uint4*)(lval + at + 4u);
+        v[0] = c.x; v[1] = c.y; v[2] = c.z; v[3] = c.w;
+        v[4] = d.x; v[5] = d.y; v[6] = d.z; v[7] = d.w;
+    }
+    uint32_t cnt = 0u;
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+        cnt += g[t] != 0xFFFFFFFFu ? 1u : 0u;
+    uint32_t wtot;
+    uint32_t pos = lscan[leaf] + wave_excl_add(cnt, lane, &wtot);
+    const uint64_t kbase = lkey[leaf] << 9;
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+    {
+        if (g[t] != 0xFFFFFFFFu)
+        {
+            vox_key[pos] = kbase | (uint64_t)(lane * 8u + (uint32_t)t);
+            vox_val[pos] = v[t];
+            vox_g0[pos] = g[t];
+            pos++;
         }
     }
 }
@@ -2964,7 +2985,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     const bool alpha_in_key = prm->value_mode == VDBR_VALUE_RGBA8 && kl.voxel_bits + kl.gid_bits + 8u <= 64u;
     kl.alpha_bits = alpha_in_key ? 8u : 0u;
     const uint32_t key_bits = kl.voxel_bits + kl.gid_bits + kl.alpha_bits;
-    const uint32_t sort_begin = alpha_in_key                                ? kl.gid_bits + 8u
+    // RGBA8 with the alpha in the key is reduced per leaf, so only the leaf bits are sorted
+    const uint32_t sort_begin = alpha_in_key                                ? kl.gid_bits + 8u + 9u
                                 : prm->value_mode == VDBR_VALUE_DENSITY ? kl.gid_bits
                                                                         : 0u;
 
@@ -3257,7 +3279,44 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             return -1;
         uint64_t svox = 0;
         const uint32_t* tile_offsets = nullptr;
-        if (run_heads(ctx, sorted, spairs, kl.gid_bits + kl.alpha_bits, seg, &svox, &tile_offsets))
+        uint64_t sleaves = 0;
+        uint32_t* lval = nullptr;
+        uint32_t* lg0 = nullptr;
+        uint64_t* lkey = nullptr;
+        uint32_t* lcnt = nullptr;
+        if (rgba_wave)
+        {
+            // leaf staging: per leaf 512 slots of (maximum packed value, lowest Gaussian id)
+            const uint32_t lshift = kl.gid_bits + kl.alpha_bits + 9u;
+            if (run_heads(ctx, sorted, spairs, lshift, nullptr, &sleaves, &tile_offsets))
+                return -1;
+            if (sleaves > 0)
+            {
+                lval = (uint32_t*)ctx->ws(WS_LR_VAL, sleaves * 512u * sizeof(uint32_t));
+                lg0 = (uint32_t*)ctx->ws(WS_LR_G0, sleaves * 512u * sizeof(uint32_t));
+                lkey = (uint64_t*)ctx->ws(WS_LR_KEY, sleaves * sizeof(uint64_t));
+                lcnt = (uint32_t*)ctx->ws(WS_LR_CNT, (sleaves + 1) * sizeof(uint32_t));
+                if (!lval || !lg0 || !lkey || !lcnt)
+                    return -1;
+                VDBR_CHECK(ctx, hipMemsetAsync(lval, 0, sleaves * 512u * sizeof(uint32_t), st));
+                VDBR_CHECK(ctx, hipMemsetAsync(lg0, 0xFF, sleaves * 512u * sizeof(uint32_t), st));
+                const uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
+                VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
+                hipLaunchKernelGGL(k_leaf_reduce_rgba, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
+                                   (const uint64_t*)sorted, spairs, lshift, kl.gid_bits, tile_offsets,
+                                   (const PrimVal*)pvals, lval, lg0, lkey);
+                VDBR_CHECK(ctx, hipGetLastError());
+                VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
+                const uint32_t lblocks = (uint32_t)((sleaves + 3) / 4);
+                hipLaunchKernelGGL(k_leaf_count, dim3(lblocks), dim3(256), 0, st, (const uint32_t*)lg0, sleaves, lcnt);
+                if (scan_u32(ctx, lcnt, lcnt, sleaves, false, lcnt + sleaves))
+                    return -1;
+                VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, lcnt + sleaves, 4, hipMemcpyDeviceToHost, st));
+                VDBR_CHECK(ctx, hipStreamSynchronize(st));
+                svox = *(const uint32_t*)ctx->pinned;
+            }
+        }
+        else if (run_heads(ctx, sorted, spairs, kl.gid_bits + kl.alpha_bits, seg, &svox, &tile_offsets))
             return -1;
         if (nvox + svox > vcap || !vox_key)
         {
@@ -3276,15 +3335,10 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         uint32_t* sg0 = vox_g0 + nvox;
         if (svox > 0 && rgba_wave)
         {
-            VDBR_CHECK(ctx, hipMemsetAsync(sval, 0, svox * sizeof(uint32_t), st));
-            VDBR_CHECK(ctx, hipMemsetAsync(sg0, 0xFF, svox * sizeof(uint32_t), st));
-            uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
-            VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
-            hipLaunchKernelGGL(k_reduce_rgba, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st, (const uint64_t*)sorted,
-                               spairs, kl.gid_bits + kl.alpha_bits, kl.gid_bits, tile_offsets, (const PrimVal*)pvals,
-                               skey, sval, sg0);
+            hipLaunchKernelGGL(k_leaf_write, dim3((uint32_t)((sleaves + 3) / 4)), dim3(256), 0, st,
+                               (const uint32_t*)lval, (const uint32_t*)lg0, (const uint64_t*)lkey,
+                               (const uint32_t*)lcnt, sleaves, skey, sval, sg0);
             VDBR_CHECK(ctx, hipGetLastError());
-            VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
             hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(svox), 2048u)), dim3(256), 0, st, svox,
                                kl, (const uint64_t*)skey, (const uint32_t*)sg0, (const PrimBox*)boxes,
                                (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds,
