@@ -414,8 +414,7 @@ PNANOVDB_API int vdbr_density_backward(vdbr_context_t* ctx, const vdbr_gaussians
                                        uint64_t value_count, const vdbr_gaussian_grads_t* grads);
 
 /* The surviving pair list of the last vdbr_raster_node2 call (its last slab), in the order the
- * value pass consumed it: VDBR_VALUE_DENSITY pairs grouped by voxel (voxels in key order),
- * VDBR_VALUE_RGBA8 pairs grouped by leaf (leaves in key order, pairs inside a leaf unordered);
+ * value pass consumed it: grouped by leaf (leaves in key order, pairs inside a leaf unordered);
  * voxel ijk (int32 x3) and gaussian id per pair, copied to device buffers of the caller. Returns the pair count through *count (buffers may be NULL
  * to query it). */
 PNANOVDB_API int vdbr_last_pairs(vdbr_context_t* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap,

@@ -85,7 +85,6 @@ enum WsSlot
     WS_BWD_LEAFGRAD,
     WS_BWD_PRIMS,
     WS_DENS_PRIMS,
-    WS_VOX_FIX,
     WS_LR_VAL,
     WS_LR_G0,
     WS_LR_KEY,

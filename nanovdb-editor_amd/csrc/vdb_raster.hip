@@ -1956,15 +1956,7 @@ __global__ __launch_bounds__(256) void k_leaf_write(const uint32_t* __restrict__
 // Mahalanobis distance of the voxel centre, each term in double from the preprocessed float
 // parameters (rotation from the normalised quaternion, scales floored at vs / 8 as in
 // gaussian_prim.slang:67-68). Terms are rounded once to multiples of 2^-120 and summed as
-// integers (exact, so the sum does not depend on the pair order and only the voxel bits of the
-// keys need sorting), then rounded to float.
-//
-// Pair-parallel over the sorted pairs: one wave per tile of HEAD_TILE pairs, 64 pairs per round,
-// one pair per lane. A 128-bit prefix sum over the lanes gives every run piece of the round, a
-// run still open at the round's end carries into the next round, and run heads are ranked with
-// the tile's head offset from run_heads. Runs inside a tile are stored directly; a run that
-// crosses a tile boundary commits with 64-bit atomic adds carrying between the words (exact, so
-// still order independent) on accumulators zeroed first.
+// integers (exact, so the sum does not depend on the pair order), then rounded to float.
 struct DensPrim
 {
     double M[9]; // diag(1/s) R^T, row-major
@@ -2031,13 +2023,6 @@ __device__ __forceinline__ void fix120(double t, uint64_t* lo, uint64_t* hi)
         *hi = m << (sh - 64);
 }
 
-__device__ __forceinline__ void add128(uint64_t* lo, uint64_t* hi, uint64_t blo, uint64_t bhi)
-{
-    const uint64_t s = *lo + blo;
-    *hi = *hi + bhi + (s < blo ? 1ull : 0ull);
-    *lo = s;
-}
-
 // w += (a2:a1:a0) with the carries taken from the returned old words: every wrap of a word is
 // seen by exactly one add, so the total is exact whatever the order of concurrent adds
 __device__ __forceinline__ void atomic_add192(unsigned long long* w, uint64_t a0, uint64_t a1, uint64_t a2)
@@ -2058,169 +2043,195 @@ __device__ __forceinline__ float fix120_to_float(uint64_t w0, uint64_t w1, uint6
     return (float)v;
 }
 
-// voxels whose run crosses into tile t (t >= 1, its first pair continues a run): zero accumulators
-__global__ __launch_bounds__(256) void k_density_cut_init(const uint64_t* __restrict__ keys, uint64_t n,
-                                                          uint32_t vshift, const uint32_t* __restrict__ tile_offsets,
-                                                          uint64_t tiles, unsigned long long* __restrict__ vox_fix,
-                                                          uint32_t* __restrict__ vox_g0)
+// Density values by leaf: the pairs are sorted by their leaf bits only and each voxel's sum of
+// fixed-point terms is exact integer arithmetic, so the pair order inside a leaf does not matter.
+// One block per tile of HEAD_TILE pairs adds its terms into LDS slots (192-bit accumulators, 64-bit
+// atomic adds carrying between words) of up to LRD_LEAVES leaves, then merges the touched slots
+// into the leaf staging arrays with the same carrying atomic adds; pairs of further leaves of the
+// tile add into the staging arrays directly.
+constexpr int LRD_LEAVES = 2;
+
+__device__ __forceinline__ void add192(unsigned long long* w, uint64_t a0, uint64_t a1, uint64_t a2)
 {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x + 1;
-    if (t >= tiles)
-        return;
-    const uint64_t p = t * HEAD_TILE;
-    if (p >= n || (keys[p] >> vshift) != (keys[p - 1] >> vshift))
-        return;
-    const uint64_t v = tile_offsets[t] - 1u;
-    vox_fix[3 * v + 0] = 0ull;
-    vox_fix[3 * v + 1] = 0ull;
-    vox_fix[3 * v + 2] = 0ull;
-    vox_g0[v] = 0xFFFFFFFFu;
+    atomic_add192(w, a0, a1, a2);
 }
 
-__global__ __launch_bounds__(HEAD_THREADS) void k_reduce_density(const uint64_t* __restrict__ keys, uint64_t n,
-                                                                 KeyLayout kl, float vs,
-                                                                 const uint32_t* __restrict__ tile_offsets,
-                                                                 uint64_t tiles, const DensPrim* __restrict__ prims,
-                                                                 uint64_t* __restrict__ vox_key,
-                                                                 unsigned long long* __restrict__ vox_fix,
-                                                                 uint32_t* __restrict__ vox_g0)
+__global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_density(const uint64_t* __restrict__ keys, uint64_t n,
+                                                                     KeyLayout kl, float vs,
+                                                                     const uint32_t* __restrict__ tile_offsets,
+                                                                     const DensPrim* __restrict__ prims,
+                                                                     unsigned long long* __restrict__ lw,
+                                                                     uint32_t* __restrict__ lg0,
+                                                                     uint64_t* __restrict__ lkey)
 {
+    __shared__ unsigned long long s_w[LRD_LEAVES * 512 * 3];
+    __shared__ uint32_t s_g0[LRD_LEAVES * 512];
+    __shared__ uint32_t wave_heads[RED_THREADS / 64];
     __shared__ double s_exp2[64];
-    if (threadIdx.x < 64u)
-        s_exp2[threadIdx.x] = EXP2_J64[threadIdx.x];
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t tile = (uint64_t)blockIdx.x * (HEAD_THREADS / 64) + (threadIdx.x >> 6);
-    if (tile >= tiles)
-        return;
-    auto vmin = [](uint32_t x, uint32_t y) { return x < y ? x : y; };
-    const uint32_t vshift = kl.gid_bits;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lshift = kl.gid_bits + 9u;
     const uint64_t gmask = (1ull << kl.gid_bits) - 1ull;
     const double vsd = (double)vs;
-    const uint64_t t0 = tile * HEAD_TILE;
-    const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
-    const uint32_t tile_base = tile_offsets[tile];
-    // the run open at the tile's end goes on in the next tile; the tile's first run began before it
-    const bool cont_next = t1 < n && (keys[t1] >> vshift) == (keys[t1 - 1] >> vshift);
-    uint64_t prev_v = t0 > 0 ? keys[t0 - 1] >> vshift : ~0ull;
-    bool open_outside = t0 > 0 && (keys[t0] >> vshift) == prev_v;
-    uint32_t heads = 0u; // run heads of the tile before this round
-    uint64_t c0 = 0ull, c1 = 0ull, c2 = 0ull;
-    uint32_t cg = 0xFFFFFFFFu; // carried piece of the run open at the previous round's end
-    for (uint64_t r0 = t0; r0 < t1; r0 += 64u)
+    const uint64_t t0 = (uint64_t)blockIdx.x * HEAD_TILE;
+    const uint64_t pb = t0 + (uint64_t)threadIdx.x * RED_ITEMS;
+    const uint32_t tile_base = tile_offsets[blockIdx.x];
+    const bool first_cont = t0 > 0 && (keys[t0] >> lshift) == (keys[t0 - 1] >> lshift);
+    for (uint32_t x = threadIdx.x; x < (uint32_t)(LRD_LEAVES * 512 * 3); x += RED_THREADS)
+        s_w[x] = 0ull;
+    for (uint32_t x = threadIdx.x; x < (uint32_t)(LRD_LEAVES * 512); x += RED_THREADS)
+        s_g0[x] = 0xFFFFFFFFu;
+    if (threadIdx.x < 64u)
+        s_exp2[threadIdx.x] = EXP2_J64[threadIdx.x];
+    uint64_t key[RED_ITEMS];
+    if (pb + RED_ITEMS <= n)
     {
-        const uint64_t p = r0 + lane;
-        const bool valid = p < t1;
-        const uint64_t key = valid ? keys[p] : ~0ull;
-        const uint64_t vk = key >> vshift;
-        uint64_t pv = __shfl_up(vk, 1, 64);
-        if (lane == 0u)
-            pv = prev_v;
-        const bool head = valid && (p == 0 || vk != pv);
-        const uint64_t hm = __ballot(head);
-        const uint64_t vm = __ballot(valid);
-        const uint32_t last = 63u - (uint32_t)__clzll(vm); // last valid lane
-        const uint64_t hle = hm & (lane == 63u ? ~0ull : ((2ull << lane) - 1ull));
-        const uint32_t run_start = hle ? 63u - (uint32_t)__clzll(hle) : 0u;
-        const bool cont = hle == 0ull; // the lane's run began before this round
-        const uint32_t v = tile_base + heads + (uint32_t)__popcll(hle) - 1u;
-        uint64_t lo = 0ull, hi = 0ull;
-        uint32_t gid = 0xFFFFFFFFu;
-        if (valid)
-        {
-            gid = (uint32_t)(key & gmask);
-            int32_t i, j, k;
-            decode_voxel(kl, vk, &i, &j, &k);
-            const DensPrim& q = prims[gid];
-            const double d[3] = { ((double)i + 0.5) * vsd - (double)q.mu[0], ((double)j + 0.5) * vsd - (double)q.mu[1],
-                                  ((double)k + 0.5) * vsd - (double)q.mu[2] };
-            double m = 0.0;
+        const ulonglong2* src = (const ulonglong2*)(keys + pb);
 #pragma unroll
-            for (int c = 0; c < 3; c++)
-            {
-                const double u = q.M[3 * c + 0] * d[0] + q.M[3 * c + 1] * d[1] + q.M[3 * c + 2] * d[2];
-                m += u * u;
-            }
-            fix120(q.o * exp_neg(-0.5 * m, s_exp2), &lo, &hi);
+        for (int i = 0; i < RED_ITEMS / 2; i++)
+        {
+            ulonglong2 v = src[i];
+            key[2 * i] = v.x;
+            key[2 * i + 1] = v.y;
         }
-        // inclusive 128-bit prefix sums over the lanes
+    }
+    else
+    {
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1)
+        for (int i = 0; i < RED_ITEMS; i++)
+            key[i] = pb + i < n ? keys[pb + i] : ~0ull;
+    }
+    uint64_t prev = wave_shr1_u64(0ull, key[RED_ITEMS - 1]);
+    if (lane == 0u)
+        prev = (pb > 0 && pb < n) ? keys[pb - 1] : ~0ull;
+    prev >>= lshift;
+    uint32_t headbits = 0u;
+    {
+        uint64_t pv = prev;
+#pragma unroll
+        for (int i = 0; i < RED_ITEMS; i++)
         {
-            const uint64_t ulo = __shfl_up(lo, off, 64);
-            const uint64_t uhi = __shfl_up(hi, off, 64);
-            if (lane >= (uint32_t)off)
-                add128(&lo, &hi, ulo, uhi);
+            const uint64_t lk = key[i] >> lshift;
+            if (pb + i < n && (pb + i == 0 || lk != pv))
+                headbits |= 1u << i;
+            pv = lk;
         }
-        // the lane's run piece in this round: prefix(lane) - prefix(run_start - 1)
-        const int src = run_start > 0u ? (int)run_start - 1 : 0;
-        uint64_t blo = __shfl(lo, src, 64), bhi = __shfl(hi, src, 64);
-        if (run_start == 0u)
-            blo = bhi = 0ull;
-        uint64_t s0 = lo - blo;
-        uint64_t s1 = hi - bhi - (lo < blo ? 1ull : 0ull);
-        uint64_t s2 = 0ull;
-        uint32_t sg = seg_scan_u32(gid, 0xFFFFFFFFu, lane, run_start, vmin);
-        if (cont)
+    }
+    uint32_t wtot;
+    const uint32_t excl = wave_excl_add((uint32_t)__popc(headbits), lane, &wtot);
+    if (lane == 0u)
+        wave_heads[wave] = wtot;
+    __syncthreads();
+    uint32_t before = excl, tile_heads = 0u;
+    for (uint32_t w = 0; w < RED_THREADS / 64u; w++)
+    {
+        before += w < wave ? wave_heads[w] : 0u;
+        tile_heads += wave_heads[w];
+    }
+    const uint32_t cont = first_cont ? 1u : 0u;
+    uint32_t h = before;
+    for (int i = 0; i < RED_ITEMS; i++)
+    {
+        if ((headbits >> i) & 1u)
+            h++;
+        if (pb + i >= n)
+            continue;
+        const uint32_t gl = tile_base + h - 1u;
+        const uint32_t ord = h - 1u + cont;
+        const uint64_t vk = key[i] >> kl.gid_bits;
+        const uint32_t slot = (uint32_t)vk & 511u;
+        const uint32_t gid = (uint32_t)(key[i] & gmask);
+        if ((headbits >> i) & 1u)
+            lkey[gl] = key[i] >> lshift;
+        int32_t vi, vj, vkk;
+        decode_voxel(kl, vk, &vi, &vj, &vkk);
+        const DensPrim& q = prims[gid];
+        const double d[3] = { ((double)vi + 0.5) * vsd - (double)q.mu[0], ((double)vj + 0.5) * vsd - (double)q.mu[1],
+                              ((double)vkk + 0.5) * vsd - (double)q.mu[2] };
+        double m = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++)
         {
-            // add the carried piece
-            const uint64_t a0 = s0 + c0;
-            const uint64_t k0 = a0 < c0 ? 1ull : 0ull;
-            const uint64_t a1 = s1 + c1;
-            const uint64_t k1 = a1 < c1 ? 1ull : 0ull;
-            const uint64_t b1 = a1 + k0;
-            const uint64_t k1b = b1 < a1 ? 1ull : 0ull;
-            s0 = a0;
-            s1 = b1;
-            s2 = c2 + k1 + k1b;
-            sg = sg < cg ? sg : cg;
+            const double u = q.M[3 * c + 0] * d[0] + q.M[3 * c + 1] * d[1] + q.M[3 * c + 2] * d[2];
+            m += u * u;
         }
-        // the run at the last lane goes on in the next round when the next pair has its voxel
-        const bool more = r0 + 64u < t1;
-        const uint64_t last_v = readlane_u64(vk, (int)last);
-        const bool carry_out = more && (keys[r0 + 64u] >> vshift) == last_v;
-        const bool run_end = valid && (lane == last || (lane < 63u && ((hm >> (lane + 1u)) & 1ull) != 0ull));
-        if (run_end && !(lane == last && carry_out))
+        uint64_t lo, hi;
+        fix120(q.o * exp_neg(-0.5 * m, s_exp2), &lo, &hi);
+        if (ord < (uint32_t)LRD_LEAVES)
         {
-            const bool atomic = (cont && open_outside) || (lane == last && !more && cont_next);
-            if (atomic)
-            {
-                atomic_add192(vox_fix + 3 * (uint64_t)v, s0, s1, s2);
-                atomicMin(&vox_g0[v], sg);
-            }
-            else
-            {
-                vox_fix[3 * (uint64_t)v + 0] = s0;
-                vox_fix[3 * (uint64_t)v + 1] = s1;
-                vox_fix[3 * (uint64_t)v + 2] = s2;
-                vox_g0[v] = sg;
-            }
+            add192(&s_w[(ord * 512u + slot) * 3u], lo, hi, 0ull);
+            atomicMin(&s_g0[ord * 512u + slot], gid);
         }
-        if (head)
-            vox_key[v] = vk;
-        if (more)
+        else
         {
-            c0 = carry_out ? readlane_u64(s0, (int)last) : 0ull;
-            c1 = carry_out ? readlane_u64(s1, (int)last) : 0ull;
-            c2 = carry_out ? readlane_u64(s2, (int)last) : 0ull;
-            cg = carry_out ? (uint32_t)__builtin_amdgcn_readlane((int)sg, (int)last) : 0xFFFFFFFFu;
-            open_outside = open_outside && hm == 0ull;
-            prev_v = last_v;
+            add192(&lw[((uint64_t)gl * 512u + slot) * 3u], lo, hi, 0ull);
+            atomicMin(&lg0[(uint64_t)gl * 512u + slot], gid);
         }
-        heads += (uint32_t)__popcll(hm);
+    }
+    __syncthreads();
+    const uint32_t leaves = tile_heads + cont;
+    const uint32_t nl = leaves < (uint32_t)LRD_LEAVES ? leaves : (uint32_t)LRD_LEAVES;
+    const uint32_t gbase = tile_base - cont;
+    for (uint32_t x = threadIdx.x; x < nl * 512u; x += RED_THREADS)
+    {
+        const uint32_t g = s_g0[x];
+        if (g != 0xFFFFFFFFu)
+        {
+            const uint64_t at = (uint64_t)(gbase + x / 512u) * 512u + (x & 511u);
+            add192(&lw[at * 3u], s_w[3u * x], s_w[3u * x + 1u], s_w[3u * x + 2u]);
+            atomicMin(&lg0[at], g);
+        }
     }
 }
 
-// per voxel: first-touch window from the lowest gaussian id, voxel bounds and (density) the
-// float value of the fixed-point sum
+// one wave per leaf: its voxels with a pair, in slot (= key) order, at the leaf's offset, with the
+// float of each fixed-point sum
+__global__ __launch_bounds__(256) void k_leaf_write_density(const unsigned long long* __restrict__ lw,
+                                                            const uint32_t* __restrict__ lg0,
+                                                            const uint64_t* __restrict__ lkey,
+                                                            const uint32_t* __restrict__ lscan, uint64_t nleaf,
+                                                            uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
+                                                            uint32_t* __restrict__ vox_g0)
+{
+    const uint64_t leaf = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (leaf >= nleaf)
+        return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t at = leaf * 512u + lane * 8u;
+    uint32_t g[8];
+    {
+        const uint4 a = *(const uint4*)(lg0 + at);
+        const uint4 b = *(const uint4*)(lg0 + at + 4u);
+        g[0] = a.x; g[1] = a.y; g[2] = a.z; g[3] = a.w;
+        g[4] = b.x; g[5] = b.y; g[6] = b.z; g[7] = b.w;
+    }
+    uint32_t cnt = 0u;
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+        cnt += g[t] != 0xFFFFFFFFu ? 1u : 0u;
+    uint32_t wtot;
+    uint32_t pos = lscan[leaf] + wave_excl_add(cnt, lane, &wtot);
+    const uint64_t kbase = lkey[leaf] << 9;
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+    {
+        if (g[t] != 0xFFFFFFFFu)
+        {
+            const unsigned long long* w = lw + (at + (uint64_t)t) * 3u;
+            vox_key[pos] = kbase | (uint64_t)(lane * 8u + (uint32_t)t);
+            vox_val[pos] = __float_as_uint(fix120_to_float(w[0], w[1], w[2]));
+            vox_g0[pos] = g[t];
+            pos++;
+        }
+    }
+}
+
+// per voxel: first-touch window from the lowest gaussian id, and voxel bounds
 __global__ __launch_bounds__(256) void k_voxel_finish(uint64_t nvox, KeyLayout kl, const uint64_t* __restrict__ vox_key,
                                                       const uint32_t* __restrict__ vox_g0, const PrimBox* __restrict__ boxes,
                                                       const uint64_t* __restrict__ foot_scan,
                                                       const uint32_t* __restrict__ win_base, uint32_t prim_batch,
                                                       uint32_t ijk_batch, uint32_t* __restrict__ vox_t,
-                                                      int32_t* __restrict__ bounds,
-                                                      const unsigned long long* __restrict__ vox_fix,
-                                                      uint32_t* __restrict__ vox_val)
+                                                      int32_t* __restrict__ bounds)
 {
     int32_t mn[3] = { 2147483647, 2147483647, 2147483647 }, mx[3] = { -2147483647, -2147483647, -2147483647 };
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -2228,8 +2239,6 @@ __global__ __launch_bounds__(256) void k_voxel_finish(uint64_t nvox, KeyLayout k
     {
         int32_t i, j, k;
         decode_voxel(kl, vox_key[v], &i, &j, &k);
-        if (vox_fix) // density: fixed-point sum -> float value
-            vox_val[v] = __float_as_uint(fix120_to_float(vox_fix[3 * v], vox_fix[3 * v + 1], vox_fix[3 * v + 2]));
         uint32_t g0 = vox_g0[v];
         PrimBox bx = boxes[g0];
         uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
@@ -2985,9 +2994,9 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     const bool alpha_in_key = prm->value_mode == VDBR_VALUE_RGBA8 && kl.voxel_bits + kl.gid_bits + 8u <= 64u;
     kl.alpha_bits = alpha_in_key ? 8u : 0u;
     const uint32_t key_bits = kl.voxel_bits + kl.gid_bits + kl.alpha_bits;
-    // RGBA8 with the alpha in the key is reduced per leaf, so only the leaf bits are sorted
+    // RGBA8 with the alpha in the key and density are reduced per leaf: only the leaf bits are sorted
     const uint32_t sort_begin = alpha_in_key                                ? kl.gid_bits + 8u + 9u
-                                : prm->value_mode == VDBR_VALUE_DENSITY ? kl.gid_bits
+                                : prm->value_mode == VDBR_VALUE_DENSITY ? kl.gid_bits + 9u
                                                                         : 0u;
 
     // ---------------------------------------------------------------- slab plan
@@ -3281,30 +3290,44 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         const uint32_t* tile_offsets = nullptr;
         uint64_t sleaves = 0;
         uint32_t* lval = nullptr;
+        unsigned long long* lw = nullptr;
         uint32_t* lg0 = nullptr;
         uint64_t* lkey = nullptr;
         uint32_t* lcnt = nullptr;
-        if (rgba_wave)
+        const bool by_leaf = rgba_wave || density;
+        if (by_leaf)
         {
-            // leaf staging: per leaf 512 slots of (maximum packed value, lowest Gaussian id)
+            // leaf staging: per leaf 512 slots of (maximum packed RGBA8 value or 192-bit density
+            // sum, lowest Gaussian id)
             const uint32_t lshift = kl.gid_bits + kl.alpha_bits + 9u;
             if (run_heads(ctx, sorted, spairs, lshift, nullptr, &sleaves, &tile_offsets))
                 return -1;
             if (sleaves > 0)
             {
-                lval = (uint32_t*)ctx->ws(WS_LR_VAL, sleaves * 512u * sizeof(uint32_t));
+                if (rgba_wave)
+                    lval = (uint32_t*)ctx->ws(WS_LR_VAL, sleaves * 512u * sizeof(uint32_t));
+                else
+                    lw = (unsigned long long*)ctx->ws(WS_LR_VAL, sleaves * 512u * 3u * sizeof(uint64_t));
                 lg0 = (uint32_t*)ctx->ws(WS_LR_G0, sleaves * 512u * sizeof(uint32_t));
                 lkey = (uint64_t*)ctx->ws(WS_LR_KEY, sleaves * sizeof(uint64_t));
                 lcnt = (uint32_t*)ctx->ws(WS_LR_CNT, (sleaves + 1) * sizeof(uint32_t));
-                if (!lval || !lg0 || !lkey || !lcnt)
+                if ((!lval && !lw) || !lg0 || !lkey || !lcnt)
                     return -1;
-                VDBR_CHECK(ctx, hipMemsetAsync(lval, 0, sleaves * 512u * sizeof(uint32_t), st));
+                if (rgba_wave)
+                    VDBR_CHECK(ctx, hipMemsetAsync(lval, 0, sleaves * 512u * sizeof(uint32_t), st));
+                else
+                    VDBR_CHECK(ctx, hipMemsetAsync(lw, 0, sleaves * 512u * 3u * sizeof(uint64_t), st));
                 VDBR_CHECK(ctx, hipMemsetAsync(lg0, 0xFF, sleaves * 512u * sizeof(uint32_t), st));
                 const uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
                 VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
-                hipLaunchKernelGGL(k_leaf_reduce_rgba, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
-                                   (const uint64_t*)sorted, spairs, lshift, kl.gid_bits, tile_offsets,
-                                   (const PrimVal*)pvals, lval, lg0, lkey);
+                if (rgba_wave)
+                    hipLaunchKernelGGL(k_leaf_reduce_rgba, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
+                                       (const uint64_t*)sorted, spairs, lshift, kl.gid_bits, tile_offsets,
+                                       (const PrimVal*)pvals, lval, lg0, lkey);
+                else
+                    hipLaunchKernelGGL(k_leaf_reduce_density, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
+                                       (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, (const DensPrim*)dprims,
+                                       lw, lg0, lkey);
                 VDBR_CHECK(ctx, hipGetLastError());
                 VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
                 const uint32_t lblocks = (uint32_t)((sleaves + 3) / 4);
@@ -3333,37 +3356,20 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         uint32_t* sval = vox_val + nvox;
         uint32_t* st_t = vox_t + nvox;
         uint32_t* sg0 = vox_g0 + nvox;
-        if (svox > 0 && rgba_wave)
+        if (svox > 0 && by_leaf)
         {
-            hipLaunchKernelGGL(k_leaf_write, dim3((uint32_t)((sleaves + 3) / 4)), dim3(256), 0, st,
-                               (const uint32_t*)lval, (const uint32_t*)lg0, (const uint64_t*)lkey,
-                               (const uint32_t*)lcnt, sleaves, skey, sval, sg0);
+            if (rgba_wave)
+                hipLaunchKernelGGL(k_leaf_write, dim3((uint32_t)((sleaves + 3) / 4)), dim3(256), 0, st,
+                                   (const uint32_t*)lval, (const uint32_t*)lg0, (const uint64_t*)lkey,
+                                   (const uint32_t*)lcnt, sleaves, skey, sval, sg0);
+            else
+                hipLaunchKernelGGL(k_leaf_write_density, dim3((uint32_t)((sleaves + 3) / 4)), dim3(256), 0, st,
+                                   (const unsigned long long*)lw, (const uint32_t*)lg0, (const uint64_t*)lkey,
+                                   (const uint32_t*)lcnt, sleaves, skey, sval, sg0);
             VDBR_CHECK(ctx, hipGetLastError());
             hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(svox), 2048u)), dim3(256), 0, st, svox,
                                kl, (const uint64_t*)skey, (const uint32_t*)sg0, (const PrimBox*)boxes,
-                               (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds,
-                               (const unsigned long long*)nullptr, (uint32_t*)nullptr);
-            VDBR_CHECK(ctx, hipGetLastError());
-        }
-        else if (svox > 0 && density)
-        {
-            const uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
-            unsigned long long* fix = (unsigned long long*)ctx->ws(WS_VOX_FIX, svox * 3 * sizeof(uint64_t));
-            if (!fix)
-                return -1;
-            VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
-            if (tiles > 1)
-                hipLaunchKernelGGL(k_density_cut_init, dim3(blocks_for(tiles - 1)), dim3(256), 0, st,
-                                   (const uint64_t*)sorted, spairs, kl.gid_bits, tile_offsets, tiles, fix, sg0);
-            hipLaunchKernelGGL(k_reduce_density, dim3((uint32_t)((tiles + 3) / 4)), dim3(HEAD_THREADS), 0, st,
-                               (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, tiles, (const DensPrim*)dprims,
-                               skey, fix, sg0);
-            VDBR_CHECK(ctx, hipGetLastError());
-            VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
-            hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(svox), 2048u)), dim3(256), 0, st, svox,
-                               kl, (const uint64_t*)skey, (const uint32_t*)sg0, (const PrimBox*)boxes,
-                               (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds,
-                               (const unsigned long long*)fix, sval);
+                               (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds);
             VDBR_CHECK(ctx, hipGetLastError());
         }
         else if (svox > 0)

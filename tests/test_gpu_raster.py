@@ -88,8 +88,8 @@ def test_pairs_match_oracle(vox, value_mode):
     a_ijk, a_gid = sort_pairs(ijk, gid)
     b_ijk, b_gid = sort_pairs(rijk, rg)
     assert np.array_equal(a_ijk, b_ijk) and np.array_equal(a_gid, b_gid)
-    # density pairs are grouped by voxel, RGBA8 pairs by leaf (8^3 voxels)
-    unit = ijk if value_mode == 1 else np.floor_divide(ijk, 8)
+    # pairs are grouped by leaf (8^3 voxels)
+    unit = np.floor_divide(ijk, 8)
     same = np.all(unit[1:] == unit[:-1], axis=1)
     assert np.unique(unit, axis=0).shape[0] == unit.shape[0] - same.sum()
 
