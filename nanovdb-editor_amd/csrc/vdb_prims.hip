@@ -382,14 +382,16 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
         uint64_t idx = base + (uint64_t)m * 64u + lane;
         bool valid = idx < n;
         uint32_t d = (uint32_t)(keys[m] >> shift) & dmask;
-        // peers: lanes holding the same digit. Eight fixed steps (digit bits above the width are
-        // zero on every lane and leave the mask unchanged); per step the ballot of the bit is
-        // combined as peers &= ~(ballot ^ sign-extended bit): ballot where the bit is set, ~ballot where not
+        // peers: lanes holding the same digit. One step per digit bit (a uniform exit ends the
+        // unrolled steps at the pass's width); per step the ballot of the bit is combined as
+        // peers &= ~(ballot ^ sign-extended bit): ballot where the bit is set, ~ballot where not
         const uint64_t vb = __ballot(valid);
         uint32_t plo = (uint32_t)vb, phi = (uint32_t)(vb >> 32);
 #pragma unroll
         for (uint32_t b = 0; b < 8u; b++)
         {
+            if (b >= digit_bits)
+                break;
             const int32_t sbit = (int32_t)(d << (31u - b)) >> 31; // 0 or -1
             const uint64_t bal = __ballot(sbit != 0);
             plo &= ~((uint32_t)bal ^ (uint32_t)sbit);
