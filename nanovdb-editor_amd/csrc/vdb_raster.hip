@@ -388,17 +388,44 @@ __device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, uint64_t* sta
         base = atomicAdd(a.counter, (unsigned long long)cnt);
     base = __shfl(base, 0, 64);
     const uint64_t gmask = (1ull << a.kl.gid_bits) - 1ull;
-    for (uint32_t e = lane; e < cnt; e += 64u)
+    // four pairs per lane at a time: their Gaussians' value constants are gathered together before
+    // any alpha is evaluated or key stored (a store in between would order the next gather after it)
+    constexpr uint32_t U = 4;
+    for (uint32_t e0 = 0; e0 < cnt; e0 += 64u * U)
     {
-        uint64_t pos = base + e;
-        uint64_t key = stage[e];
-        if (a.kl.alpha_bits)
+        uint64_t key[U];
+        float dd[U];
+        PrimVal pv[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++)
         {
-            uint32_t gid = (uint32_t)((key >> a.kl.alpha_bits) & gmask);
-            key |= (uint64_t)alpha8_fast(a.vals[gid], stage_d[e]);
+            const uint32_t e = e0 + u * 64u + lane;
+            key[u] = 0ull;
+            dd[u] = 0.f;
+            if (e < cnt)
+            {
+                key[u] = stage[e];
+                if (a.kl.alpha_bits)
+                {
+                    dd[u] = stage_d[e];
+                    pv[u] = a.vals[(uint32_t)((key[u] >> a.kl.alpha_bits) & gmask)];
+                }
+            }
         }
-        if (pos < a.cap)
-            a.out[pos] = key;
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++)
+        {
+            const uint32_t e = e0 + u * 64u + lane;
+            if (e < cnt)
+            {
+                uint64_t k = key[u];
+                if (a.kl.alpha_bits)
+                    k |= (uint64_t)alpha8_fast(pv[u], dd[u]);
+                const uint64_t pos = base + e;
+                if (pos < a.cap)
+                    a.out[pos] = k;
+            }
+        }
     }
 }
 
