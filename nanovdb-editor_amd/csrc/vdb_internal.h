@@ -191,7 +191,12 @@ int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_
                   uint32_t* count, uint32_t* best_variant, uint32_t* best_digit_bits);
 int set_sort_config(Context* ctx, uint64_t from_count, uint32_t variant, uint32_t digit_bits);
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
-                          uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out);
+                          uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out,
+                          bool hist_ready = false);
+// digit width the sort of n keys will use, and the buffer that holds its digit histograms
+// (passes x 256 counts, u64): a producer of the keys may fill it and pass hist_ready
+uint32_t sort_digit_bits(const Context* ctx, uint64_t n);
+unsigned long long* sort_hist_buffer(Context* ctx);
 
 // vdb_raster.hip
 int raster_node2(Context* ctx, const vdbr_gaussians_t* g, const vdbr_raster_params_t* p, void** grid_dev,

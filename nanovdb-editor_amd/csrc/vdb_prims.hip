@@ -750,8 +750,20 @@ int radix_sort_dual_key(Context* ctx, uint32_t* key_low, uint32_t* key_high, uin
 
 // sort returning the buffer that holds the result (no copy back): *result_keys is either
 // keys or the workspace alternate
+uint32_t sort_digit_bits(const Context* ctx, uint64_t n)
+{
+    return sort_config(ctx, n, false).digit_bits;
+}
+
+unsigned long long* sort_hist_buffer(Context* ctx)
+{
+    // the first part of the small sort workspace (same size as sort_workspace asks for)
+    return (unsigned long long*)ctx->ws(WS_SORT_SMALL, 16 * 256 * 8 * 2 + 64);
+}
+
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
-                          uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out)
+                          uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out,
+                          bool hist_ready)
 {
     *result_keys = keys;
     if (passes_out)
@@ -770,13 +782,16 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
     SortWorkspace ws;
     if (ctx->sort_workspace(n, false, tiles, false, &ws))
         return -1;
-    VDBR_CHECK(ctx, hipMemsetAsync(ws.hist, 0, SORT_MAX_PASSES * 256 * sizeof(uint64_t), ctx->stream));
     VDBR_CHECK(ctx, hipMemsetAsync(ws.tile_counters, 0, SORT_MAX_PASSES * sizeof(uint32_t), ctx->stream));
-    uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
-    if (hist_blocks > 2048u)
-        hist_blocks = 2048u;
-    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits, W,
-                       (unsigned long long*)ws.hist);
+    if (!hist_ready)
+    {
+        VDBR_CHECK(ctx, hipMemsetAsync(ws.hist, 0, SORT_MAX_PASSES * 256 * sizeof(uint64_t), ctx->stream));
+        uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
+        if (hist_blocks > 2048u)
+            hist_blocks = 2048u;
+        hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits, W,
+                           (unsigned long long*)ws.hist);
+    }
     hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
                        passes, ws.starts);
     uint64_t* kin = keys;

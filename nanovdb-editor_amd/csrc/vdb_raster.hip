@@ -279,6 +279,7 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t n, const float* __restri
 constexpr int EMIT_THREADS = 256;
 constexpr int EMIT_ROWS = 1024;       // rows per block
 constexpr int EMIT_WAVE_STAGE = 1024; // staged survivors per wave
+constexpr int EMIT_HIST_PASSES = 4;   // sort passes whose digit histograms the emission can count
 
 struct EmitArgs
 {
@@ -298,6 +299,10 @@ struct EmitArgs
     const uint64_t* seg_part_count; // segments written per partition
     const uint64_t* seg_chunk_scan; // SEG_PARTS + 1 exclusive scan of 64-segment chunks per partition
     int32_t slab_i0;                // first voxel column of the slab (INT32_MIN: no slab)
+    unsigned long long* hist;       // digit histograms of the emitted keys for their sort (or null)
+    uint32_t hist_shift;            // first sorted key bit
+    uint32_t hist_bits;             // sorted key bits
+    uint32_t hist_w;                // digit width
 };
 
 // rows of every Gaussian inside the voxel columns [i0, i1) of a slab
@@ -379,10 +384,39 @@ __device__ __forceinline__ uint64_t find_prim(const uint64_t* __restrict__ row_s
 // the stage holds (key without alpha, minimum sample distance) and the alpha is evaluated here
 // on the compacted pairs, so the double-precision exp/pow run with every lane busy.
 __device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, uint64_t* stage, float* stage_d, uint32_t cnt,
-                                                uint32_t lane)
+                                                uint32_t lane, uint32_t (*s_hist)[256])
 {
     if (cnt == 0u)
         return;
+    if (a.hist)
+    {
+        // digit counts of the keys for every sort pass (the alpha bits lie below the sorted bits),
+        // each lane over a contiguous slice, run-length compressed, into the block's LDS histograms
+        const uint32_t passes = (a.hist_bits + a.hist_w - 1u) / a.hist_w;
+        const uint32_t per = (cnt + 63u) >> 6;
+        const uint32_t e0 = lane * per;
+        const uint32_t e1 = min(cnt, e0 + per);
+        for (uint32_t p = 0; p < passes && e0 < e1; p++)
+        {
+            const uint32_t shift = a.hist_shift + a.hist_w * p;
+            const uint32_t rem = a.hist_bits - a.hist_w * p;
+            const uint32_t dm = rem >= a.hist_w ? ((1u << a.hist_w) - 1u) : ((1u << rem) - 1u);
+            uint32_t cur = (uint32_t)(stage[e0] >> shift) & dm;
+            uint32_t run = 1u;
+            for (uint32_t e = e0 + 1u; e < e1; e++)
+            {
+                const uint32_t d = (uint32_t)(stage[e] >> shift) & dm;
+                if (d != cur)
+                {
+                    atomicAdd(&s_hist[p][cur], run);
+                    cur = d;
+                    run = 0u;
+                }
+                run++;
+            }
+            atomicAdd(&s_hist[p][cur], run);
+        }
+    }
     unsigned long long base = 0ull;
     if (lane == 0u)
         base = atomicAdd(a.counter, (unsigned long long)cnt);
@@ -714,6 +748,10 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
 {
     __shared__ uint64_t stage_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
     __shared__ float stage_d_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
+    __shared__ uint32_t s_hist[EMIT_HIST_PASSES][256];
+    for (uint32_t x = threadIdx.x; x < (uint32_t)(EMIT_HIST_PASSES * 256); x += EMIT_THREADS)
+        (&s_hist[0][0])[x] = 0u;
+    __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint64_t* stage = stage_all[wave];
     float* stage_d = stage_d_all[wave];
@@ -777,7 +815,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                emit_flush_wave(a, stage, stage_d, count, lane);
+                emit_flush_wave(a, stage, stage_d, count, lane, s_hist);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -788,7 +826,18 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    emit_flush_wave(a, stage, stage_d, count, lane);
+    emit_flush_wave(a, stage, stage_d, count, lane, s_hist);
+    if (a.hist)
+    {
+        __syncthreads();
+        const uint32_t passes = (a.hist_bits + a.hist_w - 1u) / a.hist_w;
+        for (uint32_t x = threadIdx.x; x < passes * 256u; x += EMIT_THREADS)
+        {
+            const uint32_t c = (&s_hist[0][0])[x];
+            if (c)
+                atomicAdd(&a.hist[x], (unsigned long long)c);
+        }
+    }
 }
 
 // 2^(j / 64), j = 0..63, as the nearest doubles
@@ -3154,6 +3203,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         const bool splittable = split_at > span_lo && split_at < span_hi;
         uint64_t spairs = 0;
         uint64_t* pairs = nullptr;
+        unsigned long long* hist_buf = nullptr;
+        uint32_t hist_w = 8;
         uint4* segs = nullptr;
         uint64_t seg_count = 0;
         if (srows > 0)
@@ -3229,6 +3280,11 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 cap = frag_total;
             if (cap > pair_limit + 1)
                 cap = pair_limit + 1;
+            // the emission counts the sort's digit histograms (for the digit width of the
+            // estimated pair count; the sort recounts when the final count picks another width)
+            hist_w = sort_digit_bits(ctx, sest > 1 ? sest : 2);
+            const uint32_t hbits = key_bits - sort_begin;
+            hist_buf = ((hbits + hist_w - 1) / hist_w <= (uint32_t)EMIT_HIST_PASSES) ? sort_hist_buffer(ctx) : nullptr;
             uint32_t sattempts = 0;
             for (;;)
             {
@@ -3237,6 +3293,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 if (!pairs)
                     return -1;
                 VDBR_CHECK(ctx, hipMemsetAsync(counter, 0, 8, st));
+                if (hist_buf)
+                    VDBR_CHECK(ctx, hipMemsetAsync(hist_buf, 0, EMIT_HIST_PASSES * 256 * sizeof(uint64_t), st));
                 EmitArgs ea;
                 memset(&ea, 0, sizeof(ea));
                 ea.frames = frames;
@@ -3250,6 +3308,10 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 ea.out = pairs;
                 ea.cap = cap;
                 ea.counter = counter;
+                ea.hist = hist_buf;
+                ea.hist_shift = sort_begin;
+                ea.hist_bits = key_bits - sort_begin;
+                ea.hist_w = hist_w;
                 ea.segs = segs;
                 ea.seg_part_cap = part_cap;
                 ea.seg_part_count = part_count_dev;
@@ -3300,7 +3362,9 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         uint32_t passes = 0;
         if (spairs > 1)
         {
-            if (radix_sort_u64_noswap(ctx, pairs, spairs, sort_begin, key_bits, &sorted, ev[6], ev[7], &passes))
+            const bool hist_ready = hist_buf && sort_digit_bits(ctx, spairs) == hist_w;
+            if (radix_sort_u64_noswap(ctx, pairs, spairs, sort_begin, key_bits, &sorted, ev[6], ev[7], &passes,
+                                      hist_ready))
                 return -1;
             sort_passes = passes;
         }
