@@ -1694,6 +1694,52 @@ __global__ __launch_bounds__(HEAD_THREADS) void k_head_count(const uint64_t* __r
         tile_counts[blockIdx.x] = cnt;
 }
 
+// Same tile counts as k_head_count when key >> shift is non-decreasing (keys sorted over every
+// bit from shift up): a 64-key segment whose last value equals the value just before it holds no
+// head, so each lane reads one segment end and the wave reads in full only the segments whose ends
+// differ. One wave per HEAD_TILE tile; a sorted array of long runs costs about one key read in 64.
+constexpr int HEAD_SEG = 64;
+static_assert(HEAD_TILE == 64 * HEAD_SEG, "one segment per lane");
+__global__ __launch_bounds__(256) void k_head_count_sorted(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
+                                                           uint64_t tiles, uint32_t* __restrict__ tile_counts)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t tile = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (tile >= tiles)
+        return; // wave-uniform
+    const uint64_t t0 = tile * HEAD_TILE;
+    const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
+    const uint64_t s = t0 + (uint64_t)lane * HEAD_SEG;
+    const bool live = s < t1;
+    const uint64_t e = s + HEAD_SEG < t1 ? s + HEAD_SEG : t1;
+    const uint64_t vend = live ? keys[e - 1] >> shift : 0ull;
+    // value just before each segment: the previous lane's segment end, key[t0 - 1] for lane 0
+    uint64_t vbefore = wave_shr1_u64(0ull, vend);
+    if (lane == 0u && t0 > 0)
+        vbefore = keys[t0 - 1] >> shift;
+    const bool first = lane == 0u && t0 == 0; // position 0 is always a head
+    uint64_t todo = __ballot(live && (first || vbefore != vend));
+    uint32_t count = 0u;
+    while (todo)
+    {
+        const int l = __builtin_ctzll(todo);
+        todo &= todo - 1ull;
+        const uint64_t ss = t0 + (uint64_t)l * HEAD_SEG;
+        const uint64_t idx = ss + lane;
+        const bool valid = idx < t1;
+        const uint64_t v = valid ? keys[idx] >> shift : 0ull;
+        uint64_t pv = wave_shr1_u64(0ull, v);
+        const uint64_t before_l = readlane_u64(vbefore, l);
+        if (lane == 0u)
+            pv = before_l;
+        const bool head = valid && (idx == 0 || v != pv);
+        count += (uint32_t)__popcll(__ballot(head));
+    }
+    if (lane == 0u)
+        tile_counts[tile] = count;
+}
+
+// writes run starts in order: starts[rank] = index of the run head
 // writes run starts in order: starts[rank] = index of the run head
 __global__ __launch_bounds__(HEAD_THREADS) void k_head_write(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
                                                              const uint32_t* __restrict__ tile_offsets,
@@ -2873,9 +2919,10 @@ static inline uint32_t bits_for(uint64_t range) // bits to represent values in [
 
 // run heads of keys >> shift: writes starts[0..runs) and starts[runs] = n when starts is not
 // null; returns the run count and, through tile_offsets_out, the exclusive head counts of the
-// HEAD_TILE-sized tiles (valid until the next run_heads call)
+// HEAD_TILE-sized tiles (valid until the next run_heads call). sorted: the keys are sorted over
+// every bit from shift up, so the count reads only the segments that hold a head.
 static int run_heads(Context* ctx, const uint64_t* keys, uint64_t n, uint32_t shift, uint32_t* starts,
-                     uint64_t* runs_out, const uint32_t** tile_offsets_out = nullptr)
+                     uint64_t* runs_out, const uint32_t** tile_offsets_out = nullptr, bool sorted = false)
 {
     if (n == 0)
     {
@@ -2890,7 +2937,12 @@ static int run_heads(Context* ctx, const uint64_t* keys, uint64_t n, uint32_t sh
     uint32_t* tile_counts = (uint32_t*)ctx->ws(WS_SEG_TILES, (tiles + 1) * sizeof(uint32_t));
     if (!tile_counts)
         return -1;
-    hipLaunchKernelGGL(k_head_count, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift, tile_counts);
+    if (sorted)
+        hipLaunchKernelGGL(k_head_count_sorted, dim3((uint32_t)((tiles + 3) / 4)), dim3(256), 0, ctx->stream, keys, n,
+                           shift, tiles, tile_counts);
+    else
+        hipLaunchKernelGGL(k_head_count, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
+                           tile_counts);
     if (scan_u32(ctx, tile_counts, tile_counts, tiles, false, tile_counts + tiles))
         return -1;
     if (starts)
@@ -3391,7 +3443,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             // leaf staging: per leaf 512 slots of (maximum packed RGBA8 value or 192-bit density
             // sum, lowest Gaussian id)
             const uint32_t lshift = kl.gid_bits + kl.alpha_bits + 9u;
-            if (run_heads(ctx, sorted, spairs, lshift, nullptr, &sleaves, &tile_offsets))
+            if (run_heads(ctx, sorted, spairs, lshift, nullptr, &sleaves, &tile_offsets, true))
                 return -1;
             if (sleaves > 0)
             {
@@ -3430,7 +3482,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 svox = *(const uint32_t*)ctx->pinned;
             }
         }
-        else if (run_heads(ctx, sorted, spairs, kl.gid_bits + kl.alpha_bits, seg, &svox, &tile_offsets))
+        else if (run_heads(ctx, sorted, spairs, kl.gid_bits + kl.alpha_bits, seg, &svox, &tile_offsets, true))
             return -1;
         if (nvox + svox > vcap || !vox_key)
         {
