@@ -270,6 +270,11 @@ def cpu_baseline(cloud, sample, threads):
                       f"{st['pair_count']} pairs"}
 
 
+def _progress(what):
+    """One line on stderr as each leg of the run completes (stdout carries only the JSON line)."""
+    print(f"[bench] {what} done", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -310,6 +315,7 @@ def main():
         best, rows = vox.autotune_sort(probe["pair_count"], probe["key_bits"] - probe["sort_bit_begin"])
         tuned = {"choice": best, "candidates": len(rows), "key_count": probe["pair_count"],
                  "key_bits": probe["key_bits"] - probe["sort_bit_begin"]}
+        _progress("sort autotune")
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -320,6 +326,7 @@ def main():
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = 1e3 * dt / args.steps
+    _progress(f"{args.steps} timed steps")
     st = stats[-1]
     P = st["pair_count"]
 
@@ -363,22 +370,27 @@ def main():
     }
     if args.fwd_bwd_steps > 0:
         out["differentiable"] = fwd_bwd(cloud, raw, vox, args.fwd_bwd_steps)
+        _progress("density forward + backward")
     if args.fit_steps > 0 and default_workload:
         vox.trim()  # the fit workload needs most of HBM
         torch.cuda.empty_cache()
         fvox = Voxelizer(0)
         out["fit_1m"] = fit_1m(fvox, args.fit_steps)
+        _progress("1M fit")
         fvox.close()
         torch.cuda.empty_cache()
     if args.big_steps > 0 and default_workload:
         bvox = Voxelizer(0)
         out["clustered_16m"] = clustered_16m(bvox, args.big_steps)
+        _progress("16M clustered")
         bvox.close()
         torch.cuda.empty_cache()
     if args.batch_steps > 0 and default_workload:
         out["batched"] = batched(vox, args.batch_steps, args.batch_workers)
+        _progress("batched clouds")
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(cloud, args.cpu_sample, args.cpu_threads)
+        _progress("cpu baseline")
     else:
         out["cpu_baseline"] = None
     vox.close()
