@@ -260,17 +260,6 @@ VDB_DEV uint32_t alpha8_fast(const PrimVal& pv, float d)
     return alpha8_from_dist2(pv, d);
 }
 
-VDB_DEV uint32_t pair_alpha8(const PrimVal& pv, const float vmin[3], const float vmax[3])
-{
-    return alpha8_from_dist2(pv, min_sample_dist2(vmin, vmax));
-}
-
-// packed RGBA8 of one surviving pair (r8 | g8 << 8 | b8 << 16 | a8 << 24)
-VDB_DEV uint32_t pair_rgba(const PrimVal& pv, const float vmin[3], const float vmax[3])
-{
-    return (pv.rgb_cull & 0x00FFFFFFu) | (pair_alpha8(pv, vmin, vmax) << 24);
-}
-
 // ----------------------------------------------------------------------------- wave helpers
 VDB_DEV uint32_t lane_id()
 {

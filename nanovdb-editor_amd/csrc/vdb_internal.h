@@ -31,7 +31,6 @@ enum WsSlot
     WS_SORT_STATUS,
     WS_SORT_SMALL,
     WS_SCAN_PARTIALS,
-    WS_SEG,
     WS_SEG_TILES,
     WS_VOX_KEY,
     WS_VOX_VAL,

@@ -1783,85 +1783,6 @@ __global__ __launch_bounds__(HEAD_THREADS) void k_head_write(const uint64_t* __r
     }
 }
 
-// ============================================================================ reduce
-struct ReduceArgs
-{
-    const uint64_t* keys;
-    const uint32_t* seg; // V + 1 entries
-    uint64_t nvox;
-    KeyLayout kl;
-    const PrimFrame* frames;
-    const PrimVal* vals;
-    const PrimBox* boxes;
-    const uint64_t* foot_scan; // exclusive, n + 1
-    const uint32_t* win_base;  // per prim batch
-    uint32_t prim_batch;
-    uint32_t ijk_batch;
-    float vs;
-    uint64_t* vox_key;
-    uint32_t* vox_val;
-    uint32_t* vox_t;
-    int32_t* bounds; // [6]
-};
-
-__global__ __launch_bounds__(256) void k_reduce(ReduceArgs a)
-{
-    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = v < a.nvox;
-    int32_t mn[3] = { 2147483647, 2147483647, 2147483647 }, mx[3] = { -2147483647, -2147483647, -2147483647 };
-    if (active)
-    {
-        uint32_t b = a.seg[v], e = a.seg[v + 1];
-        const uint32_t ab = a.kl.alpha_bits;
-        const uint64_t gmask = (1ull << a.kl.gid_bits) - 1ull;
-        uint64_t k0 = a.keys[b];
-        uint64_t vkey = k0 >> (a.kl.gid_bits + ab);
-        int32_t i, j, k;
-        decode_voxel(a.kl, vkey, &i, &j, &k);
-        uint32_t value = 0u;
-        uint32_t g0 = 0xFFFFFFFFu;
-        if (ab)
-        {
-            // RGBA8 with the pair alpha in the key: max of (rgb | alpha << 24), any pair order
-            for (uint32_t p = b; p < e; p++)
-            {
-                uint64_t key = a.keys[p];
-                uint32_t gid = (uint32_t)((key >> ab) & gmask);
-                uint32_t rgba = (a.vals[gid].rgb_cull & 0x00FFFFFFu) | ((uint32_t)(key & 255u) << 24);
-                value = rgba > value ? rgba : value;
-                g0 = gid < g0 ? gid : g0;
-            }
-        }
-        else
-        {
-            for (uint32_t p = b; p < e; p++)
-            {
-                uint32_t gid = (uint32_t)(a.keys[p] & gmask);
-                g0 = gid < g0 ? gid : g0;
-                const PrimFrame f = a.frames[gid];
-                float vmin[3], vmax[3];
-                voxel_box(f, i, j, k, a.vs, vmin, vmax);
-                uint32_t rgba = pair_rgba(a.vals[gid], vmin, vmax);
-                value = rgba > value ? rgba : value;
-            }
-        }
-        // window of the pair with the lowest gaussian id: fragment slot within its prim batch
-        PrimBox bx = a.boxes[g0];
-        uint64_t ri = ((uint64_t)(i - bx.mn[0]) * (uint64_t)bx.rng_y + (uint64_t)(j - bx.mn[1])) * (uint64_t)bx.rng_z +
-                      (uint64_t)(k - bx.mn[2]);
-        uint32_t batch = g0 / a.prim_batch;
-        uint64_t slot = a.foot_scan[g0] - a.foot_scan[(uint64_t)batch * a.prim_batch] + ri;
-        uint32_t t = a.win_base[batch] + (uint32_t)(slot / a.ijk_batch);
-        a.vox_key[v] = vkey;
-        a.vox_val[v] = value;
-        a.vox_t[v] = t;
-        mn[0] = mx[0] = i;
-        mn[1] = mx[1] = j;
-        mn[2] = mx[2] = k;
-    }
-    commit_bounds(mn, mx, a.bounds);
-}
-
 // Block shape of the tile reduces: one run-head tile of HEAD_TILE pairs per block, RED_ITEMS
 // consecutive pairs per lane.
 constexpr int RED_THREADS = 256;
@@ -1879,13 +1800,18 @@ constexpr int RED_ITEMS = HEAD_TILE / RED_THREADS; // one run-head tile per bloc
 // voxels in key order.
 constexpr int LR_LEAVES = 8;
 
+template <bool ALPHA_IN_KEY>
 __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
-                                                                  uint32_t lshift, uint32_t gid_bits,
+                                                                  KeyLayout kl, float vs,
                                                                   const uint32_t* __restrict__ tile_offsets,
+                                                                  const PrimFrame* __restrict__ frames,
                                                                   const PrimVal* __restrict__ vals,
                                                                   uint32_t* __restrict__ lval, uint32_t* __restrict__ lg0,
                                                                   uint64_t* __restrict__ lkey)
 {
+    const uint32_t gid_bits = kl.gid_bits;
+    const uint32_t ab = ALPHA_IN_KEY ? 8u : 0u;
+    const uint32_t lshift = gid_bits + ab + 9u;
     __shared__ uint32_t s_val[LR_LEAVES * 512];
     __shared__ uint32_t s_g0[LR_LEAVES * 512];
     __shared__ uint32_t wave_heads[RED_THREADS / 64];
@@ -1948,8 +1874,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t
         tile_heads += wave_heads[w];
     }
     const uint32_t cont = first_cont ? 1u : 0u;
-    // pass 1: maximum alpha and lowest Gaussian id per slot; leaves past LR_LEAVES go to the
-    // staging arrays directly (with their colours)
+    // pass 1: maximum alpha (alpha in the key) or maximum packed value (evaluated here from the
+    // voxel box, as the emission evaluates it) and lowest Gaussian id per slot; leaves past
+    // LR_LEAVES go to the staging arrays directly (with their colours)
     uint32_t h = before;
 #pragma unroll
     for (int i = 0; i < RED_ITEMS; i++)
@@ -1961,26 +1888,38 @@ __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t
         const uint32_t gl = tile_base + h - 1u; // leaf index of the pair
         const uint32_t ord = h - 1u + cont;     // leaf ordinal within the tile
         const uint32_t slot = (uint32_t)(key[i] >> (lshift - 9u)) & 511u;
-        const uint32_t a8 = (uint32_t)(key[i] & 255u);
-        const uint32_t gid = (uint32_t)((key[i] >> 8) & gmask);
+        const uint32_t gid = (uint32_t)((key[i] >> ab) & gmask);
+        uint32_t a8 = 0u, rgba = 0u;
+        if (ALPHA_IN_KEY)
+            a8 = (uint32_t)(key[i] & 255u);
+        else
+        {
+            int32_t vi, vj, vk;
+            decode_voxel(kl, key[i] >> gid_bits, &vi, &vj, &vk);
+            float vmin[3], vmax[3];
+            voxel_box(frames[gid], vi, vj, vk, vs, vmin, vmax);
+            const PrimVal pv = vals[gid];
+            rgba = (pv.rgb_cull & 0x00FFFFFFu) | (alpha8_fast(pv, min_sample_dist2(vmin, vmax)) << 24);
+        }
         if ((headbits >> i) & 1u)
             lkey[gl] = key[i] >> lshift;
         if (ord < (uint32_t)LR_LEAVES)
         {
-            atomicMax(&s_val[ord * 512u + slot], a8 << 24);
+            atomicMax(&s_val[ord * 512u + slot], ALPHA_IN_KEY ? a8 << 24 : rgba);
             atomicMin(&s_g0[ord * 512u + slot], gid);
         }
         else
         {
-            atomicMax(&lval[(uint64_t)gl * 512u + slot], (vals[gid].rgb_cull & 0x00FFFFFFu) | (a8 << 24));
+            atomicMax(&lval[(uint64_t)gl * 512u + slot],
+                      ALPHA_IN_KEY ? (vals[gid].rgb_cull & 0x00FFFFFFu) | (a8 << 24) : rgba);
             atomicMin(&lg0[(uint64_t)gl * 512u + slot], gid);
         }
     }
     __syncthreads();
-    // pass 2: colours of the pairs that hold their slot's maximum alpha
+    // pass 2 (alpha in the key): colours of the pairs that hold their slot's maximum alpha
     h = before;
 #pragma unroll
-    for (int i = 0; i < RED_ITEMS; i++)
+    for (int i = 0; ALPHA_IN_KEY && i < RED_ITEMS; i++)
     {
         if ((headbits >> i) & 1u)
             h++;
@@ -3122,10 +3061,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     const bool alpha_in_key = prm->value_mode == VDBR_VALUE_RGBA8 && kl.voxel_bits + kl.gid_bits + 8u <= 64u;
     kl.alpha_bits = alpha_in_key ? 8u : 0u;
     const uint32_t key_bits = kl.voxel_bits + kl.gid_bits + kl.alpha_bits;
-    // RGBA8 with the alpha in the key and density are reduced per leaf: only the leaf bits are sorted
-    const uint32_t sort_begin = alpha_in_key                                ? kl.gid_bits + 8u + 9u
-                                : prm->value_mode == VDBR_VALUE_DENSITY ? kl.gid_bits + 9u
-                                                                        : 0u;
+    // both value modes are reduced per leaf: only the leaf bits are sorted
+    const uint32_t sort_begin = kl.gid_bits + kl.alpha_bits + 9u;
 
     // ---------------------------------------------------------------- slab plan
     // Pairs are emitted, sorted and reduced per slab of leaf-aligned voxel columns x in [i0, i1);
@@ -3424,11 +3361,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         last_sorted = sorted;
         last_slab_pairs = spairs;
 
-        // reduce per voxel, appended after the voxels of the previous slabs
-        const bool by_voxel = !rgba_wave && !density; // lane-per-voxel reduce over the run starts
-        uint32_t* seg = by_voxel ? (uint32_t*)ctx->ws(WS_SEG, (spairs + 2) * sizeof(uint32_t)) : nullptr;
-        if (by_voxel && !seg)
-            return -1;
+        // reduce per leaf, voxels appended after those of the previous slabs
         uint64_t svox = 0;
         const uint32_t* tile_offsets = nullptr;
         uint64_t sleaves = 0;
@@ -3437,8 +3370,6 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         uint32_t* lg0 = nullptr;
         uint64_t* lkey = nullptr;
         uint32_t* lcnt = nullptr;
-        const bool by_leaf = rgba_wave || density;
-        if (by_leaf)
         {
             // leaf staging: per leaf 512 slots of (maximum packed RGBA8 value or 192-bit density
             // sum, lowest Gaussian id)
@@ -3447,7 +3378,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 return -1;
             if (sleaves > 0)
             {
-                if (rgba_wave)
+                if (!density)
                     lval = (uint32_t*)ctx->ws(WS_LR_VAL, sleaves * 512u * sizeof(uint32_t));
                 else
                     lw = (unsigned long long*)ctx->ws(WS_LR_VAL, sleaves * 512u * 3u * sizeof(uint64_t));
@@ -3456,7 +3387,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 lcnt = (uint32_t*)ctx->ws(WS_LR_CNT, (sleaves + 1) * sizeof(uint32_t));
                 if ((!lval && !lw) || !lg0 || !lkey || !lcnt)
                     return -1;
-                if (rgba_wave)
+                if (!density)
                     VDBR_CHECK(ctx, hipMemsetAsync(lval, 0, sleaves * 512u * sizeof(uint32_t), st));
                 else
                     VDBR_CHECK(ctx, hipMemsetAsync(lw, 0, sleaves * 512u * 3u * sizeof(uint64_t), st));
@@ -3464,8 +3395,12 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 const uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
                 VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
                 if (rgba_wave)
-                    hipLaunchKernelGGL(k_leaf_reduce_rgba, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
-                                       (const uint64_t*)sorted, spairs, lshift, kl.gid_bits, tile_offsets,
+                    hipLaunchKernelGGL(k_leaf_reduce_rgba<true>, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
+                                       (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, (const PrimFrame*)frames,
+                                       (const PrimVal*)pvals, lval, lg0, lkey);
+                else if (!density)
+                    hipLaunchKernelGGL(k_leaf_reduce_rgba<false>, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
+                                       (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, (const PrimFrame*)frames,
                                        (const PrimVal*)pvals, lval, lg0, lkey);
                 else
                     hipLaunchKernelGGL(k_leaf_reduce_density, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
@@ -3482,8 +3417,6 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 svox = *(const uint32_t*)ctx->pinned;
             }
         }
-        else if (run_heads(ctx, sorted, spairs, kl.gid_bits + kl.alpha_bits, seg, &svox, &tile_offsets, true))
-            return -1;
         if (nvox + svox > vcap || !vox_key)
         {
             const uint64_t want = whole ? std::max<uint64_t>(svox, 1) : std::max<uint64_t>(nvox + svox, vcap + vcap / 2);
@@ -3499,9 +3432,9 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         uint32_t* sval = vox_val + nvox;
         uint32_t* st_t = vox_t + nvox;
         uint32_t* sg0 = vox_g0 + nvox;
-        if (svox > 0 && by_leaf)
+        if (svox > 0)
         {
-            if (rgba_wave)
+            if (!density)
                 hipLaunchKernelGGL(k_leaf_write, dim3((uint32_t)((sleaves + 3) / 4)), dim3(256), 0, st,
                                    (const uint32_t*)lval, (const uint32_t*)lg0, (const uint64_t*)lkey,
                                    (const uint32_t*)lcnt, sleaves, skey, sval, sg0);
@@ -3514,30 +3447,6 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                                kl, (const uint64_t*)skey, (const uint32_t*)sg0, (const PrimBox*)boxes,
                                (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds);
             VDBR_CHECK(ctx, hipGetLastError());
-        }
-        else if (svox > 0)
-        {
-            ReduceArgs ra;
-            ra.keys = sorted;
-            ra.seg = seg;
-            ra.nvox = svox;
-            ra.kl = kl;
-            ra.frames = frames;
-            ra.vals = pvals;
-            ra.boxes = boxes;
-            ra.foot_scan = foot_scan;
-            ra.win_base = win_base_dev;
-            ra.prim_batch = PB;
-            ra.ijk_batch = W;
-            ra.vs = vs;
-            ra.vox_key = skey;
-            ra.vox_val = sval;
-            ra.vox_t = st_t;
-            ra.bounds = vbounds;
-            VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
-            hipLaunchKernelGGL(k_reduce, dim3(blocks_for(svox)), dim3(256), 0, st, ra);
-            VDBR_CHECK(ctx, hipGetLastError());
-            VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
         }
         VDBR_CHECK(ctx, hipEventRecord(ev[15], st));
         VDBR_CHECK(ctx, hipEventSynchronize(ev[15]));

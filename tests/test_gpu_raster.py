@@ -140,6 +140,30 @@ def test_spatial_order_same_topology(vox):
     assert np.array_equal(a1, a2)
 
 
+@pytest.mark.parametrize("value_mode", [0, 1])
+def test_wide_domain_alpha_outside_key(vox, value_mode):
+    # two clusters 1000 units apart: 51 voxel-key bits + 11 Gaussian-id bits leave no room for the
+    # RGBA8 alpha in the 64-bit pair key, so the pair value is evaluated again in the reduce
+    c = small_uniform(1200, 0.01, 41)
+    c.means[:600] += np.float32(500.0)
+    c.means[600:] -= np.float32(500.0)
+    m, op, q, s, col = preprocessed(c)
+    got, st, _ = _run_gpu(vox, c, value_mode=value_mode)
+    print(f"wide domain mode={value_mode}", st, flush=True)
+    assert st["key_bits"] + 8 > 64
+    if value_mode == 0:
+        ref, rst = oracle.raster_node2(m, q, s, col, op, c.voxel_size)
+        assert st["pair_count"] == rst["pair_count"]
+        assert (st["upper_count"], st["lower_count"], st["leaf_count"]) == (rst["upper_count"], rst["lower_count"],
+                                                                           rst["leaf_count"])
+        assert np.array_equal(got, ref)
+    else:
+        # density: same topology as the RGBA8 build of the cloud
+        rgba, rst, _ = _run_gpu(vox, c, value_mode=0)
+        assert st["pair_count"] == rst["pair_count"] and st["voxel_count"] == rst["voxel_count"]
+        assert got.size == rgba.size
+
+
 @pytest.mark.parametrize("seed", [51, 52])
 def test_extreme_shapes_pairs(vox, seed):
     # strongly anisotropic, tiny and large Gaussians, opacities near the 0.01 cut: the row
