@@ -2318,26 +2318,46 @@ __global__ __launch_bounds__(256) void k_voxel_finish(uint64_t nvox, KeyLayout k
 }
 
 // ============================================================================ leaves
-// per leaf (run of voxels sharing key >> 9): value mask, first-touch window, tree key
+// per leaf (run of voxels sharing key >> 9): value mask, first-touch window, tree key. One wave
+// per leaf reads the leaf's voxel run coalesced; each lane ORs its voxels' slot bits into 16
+// 32-bit halves of the 512-bit mask, and shuffles join the lanes.
 __global__ __launch_bounds__(256) void k_leaf_info(const uint64_t* __restrict__ vox_key, const uint32_t* __restrict__ vox_t,
                                                    const uint32_t* __restrict__ leaf_first, uint64_t nleaf, KeyLayout kl,
                                                    uint64_t* __restrict__ leaf_mask, uint32_t* __restrict__ leaf_t,
                                                    uint64_t* __restrict__ sort_keys, uint32_t* __restrict__ sort_vals)
 {
-    uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t l = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (l >= nleaf)
-        return;
-    uint32_t b = leaf_first[l], e = leaf_first[l + 1];
-    uint64_t m[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        return; // wave-uniform
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t b = leaf_first[l], e = leaf_first[l + 1];
+    uint32_t m[16];
+#pragma unroll
+    for (int h = 0; h < 16; h++)
+        m[h] = 0u;
     uint32_t t = 0xFFFFFFFFu;
-    for (uint32_t v = b; v < e; v++)
+    for (uint32_t v = b + lane; v < e; v += 64u)
     {
-        uint32_t n = (uint32_t)(vox_key[v] & 511u);
-        m[n >> 6] |= 1ull << (n & 63u);
+        const uint32_t n = (uint32_t)(vox_key[v] & 511u);
+        const uint32_t bit = 1u << (n & 31u);
+#pragma unroll
+        for (int h = 0; h < 16; h++)
+            m[h] |= (n >> 5) == (uint32_t)h ? bit : 0u;
         t = min(t, vox_t[v]);
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+#pragma unroll
+        for (int h = 0; h < 16; h++)
+            m[h] |= (uint32_t)__shfl_xor((int)m[h], off, 64);
+        t = min(t, (uint32_t)__shfl_xor((int)t, off, 64));
+    }
+    if (lane != 0u)
+        return;
+#pragma unroll
     for (int w = 0; w < 8; w++)
-        leaf_mask[8 * l + w] = m[w];
+        leaf_mask[8 * l + w] = (uint64_t)m[2 * w] | ((uint64_t)m[2 * w + 1] << 32);
     leaf_t[l] = t;
     int32_t i, j, k;
     decode_voxel(kl, vox_key[b] & ~511ull, &i, &j, &k);
@@ -2366,16 +2386,23 @@ __global__ __launch_bounds__(256) void k_leaf_gather(const uint64_t* __restrict_
 }
 
 // min of a u32 array over runs [first[r], first[r+1])
+// out[r] = minimum of vals over run r; one wave per run
 __global__ __launch_bounds__(256) void k_run_min(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ first,
                                                  uint64_t nrun, uint32_t* __restrict__ out)
 {
-    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t r = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (r >= nrun)
-        return;
+        return; // wave-uniform
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t e = first[r + 1];
     uint32_t t = 0xFFFFFFFFu;
-    for (uint32_t x = first[r]; x < first[r + 1]; x++)
+    for (uint32_t x = first[r] + lane; x < e; x += 64u)
         t = min(t, vals[x]);
-    out[r] = t;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        t = min(t, (uint32_t)__shfl_xor((int)t, off, 64));
+    if (lane == 0u)
+        out[r] = t;
 }
 
 // key of each run head: out[r] = keys[first[r]] >> shift
@@ -2796,13 +2823,15 @@ __global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, G
 }
 
 // voxel -> leaf run index
+// member_of[x] = r for every x of run r; one wave per run (coalesced writes)
 __global__ __launch_bounds__(256) void k_run_member(const uint32_t* __restrict__ first, uint64_t nrun,
                                                     uint32_t* __restrict__ member_of)
 {
-    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t r = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (r >= nrun)
-        return;
-    for (uint32_t x = first[r]; x < first[r + 1]; x++)
+        return; // wave-uniform
+    const uint32_t e = first[r + 1];
+    for (uint32_t x = first[r] + (threadIdx.x & 63u); x < e; x += 64u)
         member_of[x] = (uint32_t)r;
 }
 
@@ -3508,9 +3537,10 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         return -1;
     if (nleaf > 0)
     {
-        hipLaunchKernelGGL(k_leaf_info, dim3(blocks_for(nleaf)), dim3(256), 0, st, vox_key, vox_t, leaf_first, nleaf, kl,
+        hipLaunchKernelGGL(k_leaf_info, dim3((uint32_t)((nleaf + 3) / 4)), dim3(256), 0, st, vox_key, vox_t, leaf_first,
+                           nleaf, kl,
                            leaf_mask, leaf_t, lkeys, lvals);
-        hipLaunchKernelGGL(k_run_member, dim3(blocks_for(nleaf)), dim3(256), 0, st, leaf_first, nleaf, vox_leaf);
+        hipLaunchKernelGGL(k_run_member, dim3((uint32_t)((nleaf + 3) / 4)), dim3(256), 0, st, leaf_first, nleaf, vox_leaf);
         if (radix_sort_u64(ctx, lkeys, lvals, nleaf, 0, 42, nullptr))
             return -1;
         hipLaunchKernelGGL(k_leaf_gather, dim3(blocks_for(nleaf)), dim3(256), 0, st, lkeys, lvals, nleaf, leaf_t, tree_t,
@@ -3530,7 +3560,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     {
         hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nlower)), dim3(256), 0, st, lkeys, lower_first, nlower, 12u,
                            lower_key);
-        hipLaunchKernelGGL(k_run_min, dim3(blocks_for(nlower)), dim3(256), 0, st, tree_t, lower_first, nlower, lower_t);
+        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nlower + 3) / 4)), dim3(256), 0, st, tree_t, lower_first, nlower, lower_t);
     }
     uint32_t* upper_first = (uint32_t*)ctx->ws(WS_UPPER, (nlo1 + 2) * sizeof(uint32_t) * 4);
     uint64_t nupper = 0;
@@ -3544,7 +3574,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     {
         hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nupper)), dim3(256), 0, st, lower_key, upper_first, nupper, 15u,
                            upper_key);
-        hipLaunchKernelGGL(k_run_min, dim3(blocks_for(nupper)), dim3(256), 0, st, lower_t, upper_first, nupper, upper_t);
+        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nupper + 3) / 4)), dim3(256), 0, st, lower_t, upper_first, nupper, upper_t);
     }
     const uint64_t node_count = 1 + nupper + nlower + nleaf;
     if (node_count > max_nodes)
