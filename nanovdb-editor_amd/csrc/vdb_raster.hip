@@ -1697,7 +1697,7 @@ __global__ __launch_bounds__(HEAD_THREADS) void k_head_count(const uint64_t* __r
 // Same tile counts as k_head_count when key >> shift is non-decreasing (keys sorted over every
 // bit from shift up): a 64-key segment whose last value equals the value just before it holds no
 // head, so each lane reads one segment end and the wave reads in full only the segments whose ends
-// differ. One wave per HEAD_TILE tile; a sorted array of long runs costs about one key read in 64.
+// differ, after a first check of the whole tile. One wave per HEAD_TILE tile.
 constexpr int HEAD_SEG = 64;
 static_assert(HEAD_TILE == 64 * HEAD_SEG, "one segment per lane");
 __global__ __launch_bounds__(256) void k_head_count_sorted(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
@@ -1709,6 +1709,13 @@ __global__ __launch_bounds__(256) void k_head_count_sorted(const uint64_t* __res
         return; // wave-uniform
     const uint64_t t0 = tile * HEAD_TILE;
     const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
+    // a tile whose last value equals the value before it holds no head (most tiles of long runs)
+    if (t0 > 0 && (keys[t0 - 1] >> shift) == (keys[t1 - 1] >> shift))
+    {
+        if (lane == 0u)
+            tile_counts[tile] = 0u;
+        return; // wave-uniform
+    }
     const uint64_t s = t0 + (uint64_t)lane * HEAD_SEG;
     const bool live = s < t1;
     const uint64_t e = s + HEAD_SEG < t1 ? s + HEAD_SEG : t1;
