@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <exception>
 #include <map>
 #include <string>
 #include <vector>
@@ -103,6 +104,17 @@ static bool load_ply(const char* filename, uint32_t count, const char** names, p
     };
     const size_t stride = vprops.size();
     std::vector<float> rec;
+    {
+        // a record is stride floats; the declared count cannot exceed what the file holds
+        const off_t here = ftello(f);
+        fseeko(f, 0, SEEK_END);
+        const off_t end = ftello(f);
+        fseeko(f, here, SEEK_SET);
+        const uint64_t avail = (here >= 0 && end > here) ? (uint64_t)(end - here) : 0;
+        const uint64_t max_records = stride > 0 ? avail / (sizeof(float) * stride) : 0;
+        if (vertex_count > max_records)
+            vertex_count = max_records; // the reference stops at a short record
+    }
     if (vertex_count > 0 && stride > 0)
     {
         rec.resize(vertex_count * stride);
@@ -225,13 +237,22 @@ static bool parse_npy(const std::vector<uint8_t>& b, NpyArray* a, std::string* e
     size_t s = h.find("'shape'");
     if (d == std::string::npos || s == std::string::npos)
         return *err = "npy header without descr/shape", false;
-    size_t q0 = h.find('\'', h.find(':', d));
-    size_t q1 = h.find('\'', q0 + 1);
+    const size_t colon = h.find(':', d);
+    const size_t q0 = colon == std::string::npos ? colon : h.find('\'', colon);
+    const size_t q1 = q0 == std::string::npos ? q0 : h.find('\'', q0 + 1);
+    if (q1 == std::string::npos)
+        return *err = "npy header with a malformed descr", false;
     std::string descr = h.substr(q0 + 1, q1 - q0 - 1);
     if (descr.size() < 3 || (descr[0] != '<' && descr[0] != '|'))
         return *err = "npy dtype " + descr + " is not little-endian", false;
-    a->word_size = (size_t)atoi(descr.c_str() + 2);
-    size_t p0 = h.find('(', s), p1 = h.find(')', p0);
+    const long wsz = strtol(descr.c_str() + 2, nullptr, 10);
+    if (wsz != 1 && wsz != 2 && wsz != 4 && wsz != 8)
+        return *err = "npy dtype " + descr + " has an unsupported element size", false;
+    a->word_size = (size_t)wsz;
+    const size_t p0 = h.find('(', s);
+    const size_t p1 = p0 == std::string::npos ? p0 : h.find(')', p0);
+    if (p1 == std::string::npos)
+        return *err = "npy header with a malformed shape", false;
     std::string tup = h.substr(p0 + 1, p1 - p0 - 1);
     a->shape.clear();
     for (size_t pos = 0; pos < tup.size();)
@@ -244,11 +265,13 @@ static bool parse_npy(const std::vector<uint8_t>& b, NpyArray* a, std::string* e
         while (pos < tup.size() && tup[pos] != ',')
             pos++;
     }
+    // element count and byte size with overflow checks: a crafted shape must not wrap around
     size_t total = 1;
     for (size_t v : a->shape)
-        total *= v;
-    size_t bytes = total * a->word_size;
-    if (hoff + hlen + bytes > b.size())
+        if (__builtin_mul_overflow(total, v, &total))
+            return *err = "npy shape overflows", false;
+    size_t bytes = 0;
+    if (__builtin_mul_overflow(total, a->word_size, &bytes) || bytes > b.size() - (hoff + hlen))
         return *err = "truncated npy data", false;
     a->data.assign(b.begin() + (long)(hoff + hlen), b.begin() + (long)(hoff + hlen + bytes));
     return true;
@@ -291,6 +314,8 @@ static bool load_npz_members(const char* filename, std::map<std::string, NpyArra
         cd_size = rd64(r + 40);
         cd_off = rd64(r + 48);
     }
+    if (cd_size > size || cd_off > size - cd_size)
+        return fclose(f), *err = "zip directory lies outside the file", false;
     std::vector<uint8_t> cd(cd_size);
     fseeko(f, (off_t)cd_off, SEEK_SET);
     if (fread(cd.data(), 1, cd_size, f) != cd_size)
@@ -305,30 +330,58 @@ static bool load_npz_members(const char* filename, std::map<std::string, NpyArra
         uint64_t csize = rd32(&cd[p + 20]), usize = rd32(&cd[p + 24]);
         const uint16_t nlen = rd16(&cd[p + 28]), xlen = rd16(&cd[p + 30]), clen = rd16(&cd[p + 32]);
         uint64_t loff = rd32(&cd[p + 42]);
+        const size_t entry_end = p + 46 + (size_t)nlen + xlen + clen;
+        if (entry_end > cd.size())
+            return fclose(f), *err = "zip directory entry runs past the directory", false;
         std::string name((const char*)&cd[p + 46], nlen);
-        for (size_t x = p + 46 + nlen; x + 4 <= p + 46 + nlen + xlen;)
+        const size_t extra_end = p + 46 + (size_t)nlen + xlen;
+        for (size_t x = p + 46 + nlen; x + 4 <= extra_end;)
         {
             const uint16_t id = rd16(&cd[x]), len = rd16(&cd[x + 2]);
+            if (x + 4 + len > extra_end)
+                return fclose(f), *err = "zip extra field runs past its entry", false;
             if (id == 0x0001) // zip64 sizes / offset, present for the fields that overflowed
             {
                 size_t q = x + 4;
+                const size_t q_end = x + 4 + len;
                 if (usize == 0xFFFFFFFFu)
+                {
+                    if (q + 8 > q_end)
+                        return fclose(f), *err = "short zip64 extra field", false;
                     usize = rd64(&cd[q]), q += 8;
+                }
                 if (csize == 0xFFFFFFFFu)
+                {
+                    if (q + 8 > q_end)
+                        return fclose(f), *err = "short zip64 extra field", false;
                     csize = rd64(&cd[q]), q += 8;
+                }
                 if (loff == 0xFFFFFFFFu)
+                {
+                    if (q + 8 > q_end)
+                        return fclose(f), *err = "short zip64 extra field", false;
                     loff = rd64(&cd[q]);
+                }
             }
             x += 4 + len;
         }
-        p += 46 + nlen + xlen + clen;
+        p = entry_end;
         if (name.size() < 4 || name.compare(name.size() - 4, 4, ".npy") != 0)
             continue;
         uint8_t lh[30];
+        if (loff > size || size - loff < 30)
+            return fclose(f), *err = "zip local header lies outside the file", false;
         fseeko(f, (off_t)loff, SEEK_SET);
         if (fread(lh, 1, 30, f) != 30 || rd32(lh) != 0x04034b50u)
             return fclose(f), *err = "bad zip local header", false;
-        fseeko(f, (off_t)(loff + 30 + rd16(lh + 26) + rd16(lh + 28)), SEEK_SET);
+        const uint64_t data_off = loff + 30 + rd16(lh + 26) + rd16(lh + 28);
+        // sizes come from the archive: the member must fit in the file, and deflate expands
+        // at most 1032:1, so larger uncompressed sizes are corrupt
+        if (data_off > size || csize > size - data_off)
+            return fclose(f), *err = "zip member " + name + " runs past the end of the file", false;
+        if ((method == 0 && usize != csize) || (method == 8 && usize > csize * 1032u + 1024u))
+            return fclose(f), *err = "zip member " + name + " has inconsistent sizes", false;
+        fseeko(f, (off_t)data_off, SEEK_SET);
         std::vector<uint8_t> comp(csize), raw;
         if (fread(comp.data(), 1, csize, f) != csize)
             return fclose(f), *err = "read error", false;
@@ -406,7 +459,10 @@ static bool load_npz(const char* filename, uint32_t count, const char** names, p
                 width = a.shape[s];
         }
         const size_t ws = a.word_size;
-        if (key == names[i] || !strcmp(key, names[i]))
+        const bool alias = !(key == names[i] || !strcmp(key, names[i]));
+        if (alias && (a.shape.size() < 2 || stride == 0 || width == 0))
+            return *err = "array 'sh' must have shape [N, K, 3] with K > 0", false;
+        if (!alias)
             out[i] = alloc(ws, total, a.data.data());
         else if (!strcmp(names[i], "sh_0"))
         {
@@ -443,8 +499,22 @@ bool ff_load(const char* filename, uint32_t count, const char** names, pnanovdb_
     for (uint32_t i = 0; i < count; i++)
         out[i] = nullptr;
     const std::string ext = file_extension(filename);
-    return ext == ".npz" ? load_npz(filename, count, names, out, alloc, err)
-                         : load_ply(filename, count, names, out, alloc, err);
+    // the archive's own size fields drive allocations; a failure is reported, never thrown
+    // through the C ABI
+    try
+    {
+        return ext == ".npz" ? load_npz(filename, count, names, out, alloc, err)
+                             : load_ply(filename, count, names, out, alloc, err);
+    }
+    catch (const std::exception& e)
+    {
+        *err = std::string("cannot load ") + filename + ": " + e.what();
+    }
+    catch (...)
+    {
+        *err = std::string("cannot load ") + filename;
+    }
+    return false;
 }
 
 } // namespace vdbr
