@@ -436,6 +436,18 @@ PNANOVDB_API int vdbr_node2_sample(vdbr_context_t* ctx, const void* grid_dev, ui
  * payload, in place (ParallelPrimitives.cpp:685-869 semantics). */
 PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32_t* vals, uint64_t count,
                                      uint32_t bit_begin, uint32_t bit_end);
+/* Allocation-free variant of vdbr_radix_sort_u64. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same sort, same result; the ping-pong buffers, look-back
+ * status words and histograms live in a file-scope device array of the library (about 49 MiB,
+ * one copy on every device the library runs on, held until the process ends), so the call makes
+ * no hipMalloc/hipFree and allocates nothing else. count is at most 4194304 (2^22) keys; larger
+ * counts fail with a message in vdbr_last_error. The work is queued on the context's stream and
+ * the call returns without synchronizing. All calls on one device share the array: calls that
+ * overlap on one device (two streams, two host threads, two contexts) race and must be ordered by
+ * the caller. */
+PNANOVDB_API int vdbr_radix_sort_u64_noalloc(vdbr_context_t* ctx, uint64_t* keys, uint32_t* vals, uint64_t count,
+                                             uint32_t bit_begin, uint32_t bit_end);
 /* Stable LSD radix sort of 32-bit keys with optional 32-bit payload, in place. */
 PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32_t* vals, uint64_t count,
                                      uint32_t key_bit_count);

@@ -1256,6 +1256,18 @@ PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32
     return 0;
 }
 
+PNANOVDB_API int vdbr_radix_sort_u64_noalloc(vdbr_context_t* ctx, uint64_t* keys, uint32_t* vals, uint64_t count,
+                                             uint32_t bit_begin, uint32_t bit_end)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    std::string err;
+    if (radix_sort_u64_noalloc(c->device, keys, vals, count, bit_begin, bit_end, c->stream, &err))
+        return c->fail(err);
+    return 0;
+}
+
 PNANOVDB_API int vdbr_radix_sort_u32(vdbr_context_t* ctx, uint32_t* keys, uint32_t* vals, uint64_t count,
                                      uint32_t key_bit_count)
 {

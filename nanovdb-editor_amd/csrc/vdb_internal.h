@@ -192,6 +192,12 @@ int set_sort_config(Context* ctx, uint64_t from_count, uint32_t variant, uint32_
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
                           uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out,
                           bool hist_ready = false);
+// allocation-free sort (vdb_prims.hip): the stable LSD sort of radix_sort_u64 over a file-scope
+// device scratch array of the library (at most noalloc_max_keys() keys); queued on st on device
+// dev without synchronizing; calls that overlap on one device race on the shared scratch
+int radix_sort_u64_noalloc(int dev, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+                           hipStream_t st, std::string* err);
+uint64_t noalloc_max_keys();
 // digit width the sort of n keys will use, and the buffer that holds its digit histograms
 // (passes x 256 counts, u64): a producer of the keys may fill it and pass hist_ready
 uint32_t sort_digit_bits(const Context* ctx, uint64_t n);

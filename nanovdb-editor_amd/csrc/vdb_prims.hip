@@ -17,6 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <utility>
 #include <stdint.h>
 
 #include "vdb_internal.h"
@@ -927,6 +930,164 @@ int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_
         *best_variant = best_cfg.variant;
     if (best_digit_bits)
         *best_digit_bits = best_cfg.digit_bits;
+    return 0;
+}
+
+// ============================================================================ allocation-free sort
+// Scratch of vdbr_radix_sort_u64_noalloc: the ping-pong keys and payload, the look-back status
+// words and the digit histograms of one sort of at most NOALLOC_MAX_KEYS keys, in one
+// file-scope device array. The loader places a copy in the memory of every device the code
+// object is loaded on; it stays there until the process ends. Calls on one device share it.
+constexpr uint64_t NOALLOC_MAX_KEYS = 1ull << 22;
+constexpr uint32_t NOALLOC_VARIANT = 0u; // 512 threads x 16 keys, 8-bit digits
+constexpr uint64_t NOALLOC_TILE = 512ull * 16ull;
+constexpr uint64_t NOALLOC_MAX_TILES = (NOALLOC_MAX_KEYS + NOALLOC_TILE - 1) / NOALLOC_TILE;
+constexpr size_t NA_KEYS_OFF = 0;
+constexpr size_t NA_VALS_OFF = NA_KEYS_OFF + NOALLOC_MAX_KEYS * sizeof(uint64_t);
+constexpr size_t NA_STATUS_OFF = NA_VALS_OFF + NOALLOC_MAX_KEYS * sizeof(uint32_t);
+constexpr size_t NA_HIST_OFF = NA_STATUS_OFF + NOALLOC_MAX_TILES * 256 * sizeof(uint64_t);
+constexpr size_t NA_STARTS_OFF = NA_HIST_OFF + SORT_MAX_PASSES * 256 * sizeof(uint64_t);
+constexpr size_t NA_COUNTERS_OFF = NA_STARTS_OFF + SORT_MAX_PASSES * 256 * sizeof(uint64_t);
+constexpr size_t NA_BYTES = NA_COUNTERS_OFF + 16 * sizeof(uint32_t);
+static_assert(NOALLOC_MAX_KEYS % NOALLOC_TILE == 0, "whole tiles");
+static_assert(NA_STATUS_OFF % 256 == 0 && NA_HIST_OFF % 8 == 0, "aligned regions");
+
+static __device__ __align__(256) uint8_t scratch[NA_BYTES];
+
+// zero fill by a kernel rather than hipMemsetAsync, so the call can sit inside captured work
+__global__ void k_fill_zero_u64(uint64_t* __restrict__ p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+namespace
+{
+constexpr int NOALLOC_DEVICES = 64;
+struct NoAllocDevice
+{
+    uint8_t* ptr = nullptr; // device address of `scratch` on this device (from hipGetSymbolAddress)
+    uint32_t epoch = 0;     // last look-back epoch used on this device's status words
+};
+std::mutex g_noalloc_mutex;
+NoAllocDevice g_noalloc[NOALLOC_DEVICES];
+} // namespace
+
+uint64_t noalloc_max_keys()
+{
+    return NOALLOC_MAX_KEYS;
+}
+
+static int noalloc_sort_on_device(int dev, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin,
+                                  uint32_t bit_end, hipStream_t st, std::string* err);
+
+int radix_sort_u64_noalloc(int dev, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+                           hipStream_t st, std::string* err)
+{
+    // run with dev current (its copy of the scratch array), then restore the caller's device
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return *err = "radix_sort_u64_noalloc: no current device", -1;
+    }
+    if (cur != dev && hipSetDevice(dev) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return *err = "radix_sort_u64_noalloc: cannot select device " + std::to_string(dev), -1;
+    }
+    const int rc = noalloc_sort_on_device(dev, keys, vals, n, bit_begin, bit_end, st, err);
+    if (cur != dev)
+        (void)hipSetDevice(cur);
+    return rc;
+}
+
+static int noalloc_sort_on_device(int dev, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin,
+                                  uint32_t bit_end, hipStream_t st, std::string* err)
+{
+    if (bit_begin > bit_end || bit_end > 64u)
+        return *err = "radix_sort_u64_noalloc: the bit range must satisfy bit_begin <= bit_end <= 64", -1;
+    if (n > NOALLOC_MAX_KEYS)
+        return *err = "radix_sort_u64_noalloc: " + std::to_string(n) + " keys exceed the scratch maximum of " +
+                      std::to_string(NOALLOC_MAX_KEYS),
+               -1;
+    if (n <= 1 || bit_end == bit_begin)
+        return 0;
+    if (!keys)
+        return *err = "radix_sort_u64_noalloc: null key array", -1;
+    if (dev < 0 || dev >= NOALLOC_DEVICES)
+        return *err = "radix_sort_u64_noalloc: device index out of range", -1;
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    NoAllocDevice& d = g_noalloc[dev];
+    const uint32_t bits = bit_end - bit_begin;
+    const uint32_t W = 8u;
+    const uint32_t passes = (bits + W - 1u) / W;
+    const uint64_t tiles = (n + NOALLOC_TILE - 1) / NOALLOC_TILE;
+    if (!d.ptr)
+    {
+        // first call on this device: its own copy of the array, then clear the status words
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+        {
+            (void)hipGetLastError();
+            return *err = std::string("radix_sort_u64_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e), -1;
+        }
+        d.ptr = (uint8_t*)p;
+        d.epoch = 0u;
+        hipLaunchKernelGGL(k_fill_zero_u64, dim3(256), dim3(256), 0, st, (uint64_t*)(d.ptr + NA_STATUS_OFF),
+                           (uint64_t)NOALLOC_MAX_TILES * 256u);
+    }
+    if (d.epoch + passes >= 65535u)
+    {
+        // epochs are 16-bit: clear the status words before they could repeat
+        hipLaunchKernelGGL(k_fill_zero_u64, dim3(256), dim3(256), 0, st, (uint64_t*)(d.ptr + NA_STATUS_OFF),
+                           (uint64_t)NOALLOC_MAX_TILES * 256u);
+        d.epoch = 0u;
+    }
+    uint64_t* keys_alt = (uint64_t*)(d.ptr + NA_KEYS_OFF);
+    uint32_t* vals_alt = (uint32_t*)(d.ptr + NA_VALS_OFF);
+    uint64_t* status = (uint64_t*)(d.ptr + NA_STATUS_OFF);
+    unsigned long long* hist = (unsigned long long*)(d.ptr + NA_HIST_OFF);
+    uint64_t* starts = (uint64_t*)(d.ptr + NA_STARTS_OFF);
+    uint32_t* counters = (uint32_t*)(d.ptr + NA_COUNTERS_OFF);
+    // histograms, digit starts and tile counters start at zero on every call
+    hipLaunchKernelGGL(k_fill_zero_u64, dim3(8), dim3(256), 0, st, (uint64_t*)hist,
+                       (uint64_t)((NA_BYTES - NA_HIST_OFF) / sizeof(uint64_t)));
+    uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
+    if (hist_blocks > 2048u)
+        hist_blocks = 2048u;
+    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, st, keys, n, bit_begin, bits, W, hist);
+    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, st, (const unsigned long long*)hist, passes, starts);
+    uint64_t* kin = keys;
+    uint64_t* kout = keys_alt;
+    uint32_t* vin = vals;
+    uint32_t* vout = vals_alt;
+    for (uint32_t p = 0; p < passes; p++)
+    {
+        const uint32_t shift = bit_begin + W * p;
+        const uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
+        const uint32_t epoch = ++d.epoch;
+        if (vals)
+            launch_sort_pass<true>(NOALLOC_VARIANT, tiles, st, kin, kout, vin, vout, n, shift, dbits, starts + p * 256,
+                                   status, counters + p, epoch);
+        else
+            launch_sort_pass<false>(NOALLOC_VARIANT, tiles, st, kin, kout, nullptr, nullptr, n, shift, dbits,
+                                    starts + p * 256, status, counters + p, epoch);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (kin != keys)
+    {
+        hipError_t e = hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && vals)
+            e = hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess)
+            return *err = std::string("radix_sort_u64_noalloc: copy back: ") + hipGetErrorString(e), -1;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return *err = std::string("radix_sort_u64_noalloc: launch: ") + hipGetErrorString(e), -1;
     return 0;
 }
 

@@ -79,6 +79,7 @@ SIGNATURES = {
                            ctypes.POINTER(P), ctypes.POINTER(u64), ctypes.POINTER(VdbrRasterStats)], ctypes.c_int),
     "vdbr_last_pairs": ([P, P, P, u64, ctypes.POINTER(u64)], ctypes.c_int),
     "vdbr_radix_sort_u64": ([P, P, P, u64, u32, u32], ctypes.c_int),
+    "vdbr_radix_sort_u64_noalloc": ([P, P, P, u64, u32, u32], ctypes.c_int),
     "vdbr_density_backward": ([P, ctypes.POINTER(VdbrGaussians), f32, P, u64, P, u64,
                                ctypes.POINTER(VdbrGaussianGrads)], ctypes.c_int),
     "vdbr_selftest_alpha8": ([P, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
