@@ -22,3 +22,16 @@ def sort_pairs(ijk, gid):
     """Canonical order: voxel (i, j, k) then gaussian id."""
     order = np.lexsort((gid, ijk[:, 2], ijk[:, 1], ijk[:, 0]))
     return ijk[order], gid[order]
+
+
+def oracle_threads():
+    """Host threads for oracle runs: the process's CPU share, at most 16 (the share of one GPU)."""
+    import os
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        cpus = min(cpus, int(env))
+    return max(1, min(cpus, 16))
