@@ -64,7 +64,16 @@ typedef struct pnanovdb_grid_build_context_t pnanovdb_grid_build_context_t;
 typedef struct pnanovdb_parallel_primitives_context_t pnanovdb_parallel_primitives_context_t;
 typedef struct pnanovdb_camera_mat_t pnanovdb_camera_mat_t;
 typedef struct pnanovdb_raster_shader_params_t pnanovdb_raster_shader_params_t;
-typedef struct pnanovdb_compute_profiler_entry_t pnanovdb_compute_profiler_entry_t;
+/* nanovdb_editor/putil/Compute.h:546-551: one timed entry of a profiler report; times in seconds.
+ * The raster entry points that take a pnanovdb_profiler_report_t call it once per raster with one
+ * entry per pipeline stage (GPU time from HIP events) and a total (host wall time and GPU time),
+ * passing "raster" as userdata as Raster.cpp:583 arranges. */
+typedef struct pnanovdb_compute_profiler_entry_t
+{
+    const char* label;
+    float cpu_delta_time;
+    float gpu_delta_time;
+} pnanovdb_compute_profiler_entry_t;
 typedef void(PNANOVDB_ABI* pnanovdb_profiler_report_t)(void* userdata, pnanovdb_uint64_t capture_id,
                                                        pnanovdb_uint32_t num_entries,
                                                        pnanovdb_compute_profiler_entry_t* entries);
@@ -251,8 +260,10 @@ typedef struct pnanovdb_parallel_primitives_t
     const pnanovdb_compute_t* compute;
 } pnanovdb_parallel_primitives_t;
 
-/* Getters: nanovdb_editor/putil/Raster.h:197, Compute.h:883, ParallelPrimitives.h:107.
- * Each returns a writable static table that lives as long as the library. */
+/* Getters: nanovdb_editor/putil/Raster.h:197, Compute.h:883, ParallelPrimitives.h:107,
+ * GridBuild.h:176, FileFormat.h. Each returns a writable static table that lives as long as the
+ * library; the compute and fileformat tables carry the library's own handle in `module`, so hosts
+ * resolve the other getters through it (putil/Raster.h:199-210, GridBuild.h:178-181). */
 /* nanovdb_editor/putil/FileFormat.h:19-35 (without the optional e57 entry). Loads Gaussian
  * arrays by name from binary PLY (3DGS properties: x y z, opacity, rot_0..3, scale_0..2,
  * f_dc_0..2, f_rest_*, red green blue) or NPZ (members <name>.npy; "sh" [N, K, 3] stands for sh_0
@@ -268,7 +279,122 @@ typedef struct pnanovdb_fileformat_t
     void* module;
 } pnanovdb_fileformat_t;
 
+/* nanovdb_editor/putil/GridBuild.h:22-29. The buffers are this library's device buffers
+ * (vdbr_buffer_t*); node_addresses_buffer holds the build's own accumulation state between
+ * grid_build_init and grid_build_destroy, the other slots stay NULL. */
+typedef struct pnanovdb_grid_build_state_t
+{
+    pnanovdb_compute_buffer_t* node_addresses_buffer;
+    pnanovdb_compute_buffer_t* node_types_buffer;
+    pnanovdb_compute_buffer_t* new_child_counts_buffer;
+    pnanovdb_compute_buffer_t* scan_new_child_counts_buffer;
+    pnanovdb_compute_buffer_t* bboxes_buffer;
+} pnanovdb_grid_build_state_t;
+
+/* nanovdb_editor/putil/GridBuild.h:31-58 (field order is the ABI). fanout_state_init allocates the
+ * buffers as vdbr_buffer_t device buffers of the reference's sizes: fragment_counts_buffer and
+ * scan_fragment_counts_buffer (u64 per prim of a batch; the caller writes each prim's footprint,
+ * 0 past the prim count), prim_bbox_buffer (6 int32 per prim, for the caller), prim_idxs_buffer and
+ * prim_raster_idxs_buffer (u32 per window slot: the slot's prim within the batch and its index in
+ * that prim's footprint, ~0 for slots past the batch's fragments), prim_raster_ijks_buffer (3 int32
+ * per slot for the caller's voxel coordinates, initialised to 0x7FFFFFFF = skip) and
+ * scatter_prim_idxs_buffer (unused here). readback_buffer is host memory holding each batch's
+ * fragment total (read back synchronously: ijk_batch_count is exact from a batch's first window). */
+typedef struct pnanovdb_grid_build_fanout_state_t
+{
+    pnanovdb_uint32_t prim_batch_size;
+    pnanovdb_uint32_t ijk_batch_size;
+    pnanovdb_uint32_t prim_count;
+    float voxel_size;
+
+    pnanovdb_uint32_t prim_batch_idx;
+    pnanovdb_uint32_t prim_batch_count;
+    pnanovdb_uint32_t ijk_batch_idx;
+    pnanovdb_uint32_t ijk_batch_count;
+    pnanovdb_uint32_t ijk_batch_count_max;
+
+    pnanovdb_uint64_t total_fragment_count;
+
+    pnanovdb_compute_buffer_t* fragment_counts_buffer;
+    pnanovdb_compute_buffer_t* scan_fragment_counts_buffer;
+    pnanovdb_compute_buffer_t* prim_bbox_buffer;
+
+    pnanovdb_compute_buffer_t* scatter_prim_idxs_buffer;
+    pnanovdb_compute_buffer_t* prim_idxs_buffer;
+    pnanovdb_compute_buffer_t* prim_raster_idxs_buffer;
+    pnanovdb_compute_buffer_t* prim_raster_ijks_buffer;
+
+    pnanovdb_compute_buffer_t* readback_buffer;
+    pnanovdb_uint64_t* readback_frames;
+    pnanovdb_uint64_t readback_frame_count;
+} pnanovdb_grid_build_fanout_state_t;
+
+/* nanovdb_editor/putil/GridBuild.h:60-152 (field order is the ABI). Semantics as GridBuild.cpp:
+ *  - grid_build_init / grid_build_destroy: the build state; grid_build_reset writes an empty node2
+ *    grid (header, root, blind metadata) into nanovdb_out and forgets every inserted point.
+ *  - grid_build: one fragment window of point_count int32 (i, j, k) points (point_ijk_in); points
+ *    whose k is 0x7FFFFFFF are skipped (grid_build_set_masks.slang:52). Coordinates must lie in
+ *    [-65536, 65535]. Each call is one window of the node allocation order (GridBuild.cpp:361-506:
+ *    window, level, parent, slot). nanovdb_out keeps the empty grid until grid_build_finalize.
+ *  - grid_build_finalize: writes the grid of every point inserted since the reset into nanovdb_out
+ *    (nanovdb_word_count 32-bit words; a larger grid is an error, nothing is written), with value
+ *    indices in node order and blind metadata (bboxes, RGBA8 values, all values 0 as the reference
+ *    leaves them for its colour pass) as GridBuild.cpp:508-650.
+ *  - fanout_*: the prim batch x fragment window loop (GridBuild.cpp:654-942): after the caller has
+ *    written the batch's fragment counts, fanout maps every slot of the current window to (prim,
+ *    footprint index). Calls are synchronous. */
+typedef struct pnanovdb_grid_build_t
+{
+    const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
+    pnanovdb_grid_build_context_t*(PNANOVDB_ABI* create_context)(const pnanovdb_compute_t* compute,
+                                                                 pnanovdb_compute_queue_t* queue);
+    void(PNANOVDB_ABI* destroy_context)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                        pnanovdb_grid_build_context_t* context);
+    void(PNANOVDB_ABI* grid_build_init)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                        pnanovdb_grid_build_context_t* context, pnanovdb_grid_build_state_t* state,
+                                        pnanovdb_compute_buffer_t* nanovdb_out, pnanovdb_uint64_t nanovdb_word_count,
+                                        float voxel_size, pnanovdb_uint32_t dispatch_count);
+    void(PNANOVDB_ABI* grid_build_reset)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                         pnanovdb_grid_build_context_t* context, pnanovdb_grid_build_state_t* state,
+                                         pnanovdb_compute_buffer_t* nanovdb_out, pnanovdb_uint64_t nanovdb_word_count,
+                                         float voxel_size, pnanovdb_uint32_t dispatch_count);
+    void(PNANOVDB_ABI* grid_build_destroy)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                           pnanovdb_grid_build_state_t* state);
+    void(PNANOVDB_ABI* grid_build)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                   pnanovdb_grid_build_context_t* context, pnanovdb_grid_build_state_t* state,
+                                   pnanovdb_compute_buffer_t* point_ijk_in, pnanovdb_compute_buffer_t* nanovdb_out,
+                                   pnanovdb_uint64_t point_count, pnanovdb_uint64_t nanovdb_word_count, float voxel_size,
+                                   pnanovdb_uint32_t dispatch_count);
+    void(PNANOVDB_ABI* grid_build_finalize)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                            pnanovdb_grid_build_context_t* context, pnanovdb_grid_build_state_t* state,
+                                            pnanovdb_compute_buffer_t* point_ijk_in, pnanovdb_compute_buffer_t* nanovdb_out,
+                                            pnanovdb_uint64_t point_count, pnanovdb_uint64_t nanovdb_word_count,
+                                            float voxel_size, pnanovdb_uint32_t dispatch_count);
+    void(PNANOVDB_ABI* fanout_state_reset)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                           pnanovdb_grid_build_context_t* context,
+                                           pnanovdb_grid_build_fanout_state_t* state);
+    pnanovdb_bool_t(PNANOVDB_ABI* fanout_state_valid)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                                      pnanovdb_grid_build_context_t* context,
+                                                      pnanovdb_grid_build_fanout_state_t* state);
+    void(PNANOVDB_ABI* fanout_state_increment)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                               pnanovdb_grid_build_context_t* context,
+                                               pnanovdb_grid_build_fanout_state_t* state);
+    void(PNANOVDB_ABI* fanout_state_init)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                          pnanovdb_grid_build_context_t* context,
+                                          pnanovdb_grid_build_fanout_state_t* state, pnanovdb_uint32_t prim_batch_size,
+                                          pnanovdb_uint32_t ijk_batch_size, pnanovdb_uint32_t ijk_batch_count_max,
+                                          pnanovdb_uint32_t prim_count, float voxel_size);
+    void(PNANOVDB_ABI* fanout_state_destroy)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                                             pnanovdb_grid_build_fanout_state_t* state);
+    void(PNANOVDB_ABI* fanout)(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
+                               pnanovdb_grid_build_context_t* context, pnanovdb_grid_build_fanout_state_t* state,
+                               pnanovdb_uint32_t dispatch_count);
+    const pnanovdb_compute_t* compute;
+} pnanovdb_grid_build_t;
+
 PNANOVDB_API pnanovdb_fileformat_t* pnanovdb_get_fileformat(void);
+/* nanovdb_editor/putil/GridBuild.h:176 */
+PNANOVDB_API pnanovdb_grid_build_t* pnanovdb_get_grid_build(void);
 PNANOVDB_API pnanovdb_raster_t* pnanovdb_get_raster(void);
 PNANOVDB_API pnanovdb_compute_t* pnanovdb_get_compute(void);
 PNANOVDB_API pnanovdb_parallel_primitives_t* pnanovdb_get_parallel_primitives(void);

@@ -1,6 +1,7 @@
 // vdb_abi.hip -- context/workspace management and the exported C ABI of libvdbraster.so:
 // the reference function tables (pnanovdb_get_raster / _compute / _parallel_primitives) and
 // the direct vdbr_* device entry points. No C++ exception crosses the ABI.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdarg.h>
@@ -9,6 +10,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -419,11 +421,27 @@ static void PNANOVDB_ABI arr_print_range(const pnanovdb_compute_t* compute, pnan
 static pnanovdb_compute_t g_compute;
 static bool g_compute_init = false;
 
-static const pnanovdb_compute_t* compute_table()
+// This library's own handle (dlopen of the file that holds this function, no new load), which the
+// tables carry as `module` so that hosts resolve the other getters through it as
+// pnanovdb_get_proc_address(compute->module, ...) does (putil/Raster.h:199-210, GridBuild.h:178-181).
+static void* own_module()
+{
+    static void* handle = nullptr;
+    if (!handle)
+    {
+        Dl_info info;
+        if (dladdr((void*)&own_module, &info) && info.dli_fname)
+            handle = dlopen(info.dli_fname, RTLD_NOW | RTLD_NOLOAD);
+    }
+    return handle;
+}
+
+const pnanovdb_compute_t* vdbr::compute_table()
 {
     if (!g_compute_init)
     {
         memset(&g_compute, 0, sizeof(g_compute));
+        g_compute.module = own_module();
         g_compute.create_array = arr_create;
         g_compute.destroy_array = arr_destroy;
         g_compute.map_array = arr_map;
@@ -436,16 +454,98 @@ static const pnanovdb_compute_t* compute_table()
 }
 
 // ============================================================================= raster table
+// Log routing as RasterUtils.cpp:163-186: a host that gives the compute table a device interface
+// (get_compute_interface / get_compute_context) receives the messages through the compute
+// interface's get_log_print; otherwise they go to stderr.
+struct ComputeInterfacePrefix // leading fields of pnanovdb_compute_interface_t (putil/Compute.h:338-349)
+{
+    const pnanovdb_reflect_data_type_t* interface_pnanovdb_reflect_data_type;
+    void* get_api;
+    void* is_feature_supported;
+    void* get_frame_info;
+    pnanovdb_compute_log_print_t(PNANOVDB_ABI* get_log_print)(void* context);
+};
+
+static pnanovdb_compute_log_print_t host_log_print(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue)
+{
+    if (!compute || !queue)
+        return nullptr;
+    typedef void*(PNANOVDB_ABI * get_iface_t)(pnanovdb_compute_queue_t*);
+    get_iface_t get_iface = (get_iface_t)compute->device_interface.get_compute_interface;
+    get_iface_t get_ctx = (get_iface_t)compute->device_interface.get_compute_context;
+    if (!get_iface || !get_ctx)
+        return nullptr;
+    ComputeInterfacePrefix* ci = (ComputeInterfacePrefix*)get_iface(queue);
+    void* context = get_ctx(queue);
+    if (!ci || !ci->get_log_print)
+        return nullptr;
+    return ci->get_log_print(context);
+}
+
+void vdbr::log_message(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue, uint32_t level,
+                       const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    pnanovdb_compute_log_print_t lp = host_log_print(compute, queue);
+    if (lp)
+        lp(level, "%s", buf);
+    else
+        fprintf(stderr, "libvdbraster: %s\n", buf);
+}
+
+// Profiler report as Raster.cpp:583 arranges it (enable_profiler with userdata "raster"): one entry
+// per pipeline stage with its GPU time (HIP events on the library's stream) in seconds, then the
+// whole call with its host wall time.
+static void report_profile(pnanovdb_profiler_report_t report, const vdbr_raster_stats_t& st, float ms_convert,
+                           double host_seconds)
+{
+    if (!report)
+        return;
+    static std::atomic<uint64_t> capture{ 0 };
+    pnanovdb_compute_profiler_entry_t entries[7] = { { "raster_preprocess_setup", 0.f, st.ms_setup * 1e-3f },
+                         { "raster_emit_pairs", 0.f, st.ms_emit * 1e-3f },
+                         { "raster_sort_pairs", 0.f, st.ms_sort * 1e-3f },
+                         { "raster_reduce_values", 0.f, st.ms_reduce * 1e-3f },
+                         { "raster_grid_build", 0.f, st.ms_build * 1e-3f },
+                         { "convert_onindex", 0.f, ms_convert * 1e-3f },
+                         { "raster_total", (float)host_seconds, (st.ms_total + ms_convert) * 1e-3f } };
+    report((void*)"raster", capture.fetch_add(1), 7u, entries);
+}
+
 struct RasterContextImpl
 {
     int device;
 };
 
+// Gaussian data (RasterContext.cpp:92-229): host copies at creation; one device allocation holding
+// means[3N] | quats[4N] | scales[3N] | colors[3N] | opacities[N], filled once by upload (or by the
+// first raster) and rasterized from there by raster_gaussian_3d without another transfer.
 struct GaussianDataImpl
 {
-    uint64_t count;
+    uint64_t count = 0;
     std::vector<float> means, quats, scales, colors, opacities;
-    float cull_scale;
+    float cull_scale = 0.f;
+    float* dev = nullptr;
+    int device = 0;
+    bool uploaded = false;
+
+    float* d_means() const { return dev; }
+    float* d_quats() const { return dev + 3 * count; }
+    float* d_scales() const { return dev + 7 * count; }
+    float* d_colors() const { return dev + 10 * count; }
+    float* d_opac() const { return dev + 13 * count; }
+    ~GaussianDataImpl()
+    {
+        if (dev)
+        {
+            (void)hipSetDevice(device);
+            (void)hipFree(dev);
+        }
+    }
 };
 
 static pnanovdb_raster_context_t* PNANOVDB_ABI r_create_context(const pnanovdb_compute_t* compute,
@@ -467,17 +567,17 @@ static void PNANOVDB_ABI r_destroy_context(const pnanovdb_compute_t* compute, pn
     delete (RasterContextImpl*)context;
 }
 
-static bool copy_floats(std::vector<float>& dst, pnanovdb_compute_array_t* a)
+// a host compute array holds at least `count` floats
+static bool floats_ok(const pnanovdb_compute_array_t* a, uint64_t count)
 {
-    if (!a)
-    {
-        dst.clear();
-        return false;
-    }
-    dst.resize(a->element_count);
-    if (a->element_count)
-        memcpy(dst.data(), a->data, a->element_count * sizeof(float));
-    return true;
+    return a && a->element_size == 4 && a->element_count >= count && (count == 0 || a->data);
+}
+
+static void copy_floats(std::vector<float>& dst, const pnanovdb_compute_array_t* a, uint64_t count)
+{
+    dst.resize(count);
+    if (count)
+        memcpy(dst.data(), a->data, count * sizeof(float));
 }
 
 // shader_params_arrays[gaussian_frag_color] holds {float cull_scale; bool use_default; ...}
@@ -485,6 +585,8 @@ static bool copy_floats(std::vector<float>& dst, pnanovdb_compute_array_t* a)
 static float cull_scale_of(pnanovdb_compute_array_t** shader_params_arrays)
 {
     if (!shader_params_arrays || !shader_params_arrays[1] || !shader_params_arrays[1]->data)
+        return 0.f;
+    if (shader_params_arrays[1]->element_size * shader_params_arrays[1]->element_count < 8)
         return 0.f;
     const uint8_t* p = (const uint8_t*)shader_params_arrays[1]->data;
     float cs;
@@ -501,41 +603,92 @@ static pnanovdb_raster_gaussian_data_t* PNANOVDB_ABI r_create_gaussian_data(
     pnanovdb_compute_array_t* opacities, pnanovdb_compute_array_t** shader_params_arrays,
     pnanovdb_raster_shader_params_t* raster_params)
 {
-    (void)compute;
-    (void)queue;
     (void)context;
     (void)sh_0;
     (void)sh_n;
     (void)raster_params;
-    if (!means || !quaternions || !scales || !colors || !opacities)
+    if (!means || !(means->element_count == 0 || means->data))
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "create_gaussian_data: missing means array");
         return nullptr;
+    }
+    const uint64_t n = means->element_count / 3u;
+    if (!floats_ok(means, 3 * n) || !floats_ok(quaternions, 4 * n) || !floats_ok(scales, 3 * n) ||
+        !floats_ok(colors, 3 * n) || !floats_ok(opacities, n))
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR,
+                    "create_gaussian_data: arrays must be float32 with 3N means, 4N quaternions, 3N scales, 3N "
+                    "colors and N opacities (N = %llu)",
+                    (unsigned long long)n);
+        return nullptr;
+    }
     GaussianDataImpl* d = new (std::nothrow) GaussianDataImpl();
     if (!d)
         return nullptr;
-    d->count = means->element_count / 3u;
-    copy_floats(d->means, means);
-    copy_floats(d->quats, quaternions);
-    copy_floats(d->scales, scales);
-    copy_floats(d->colors, colors);
-    copy_floats(d->opacities, opacities);
-    d->cull_scale = cull_scale_of(shader_params_arrays);
-    if (d->quats.size() < 4 * d->count || d->scales.size() < 3 * d->count || d->colors.size() < 3 * d->count ||
-        d->opacities.size() < d->count)
+    try
+    {
+        d->count = n;
+        copy_floats(d->means, means, 3 * n);
+        copy_floats(d->quats, quaternions, 4 * n);
+        copy_floats(d->scales, scales, 3 * n);
+        copy_floats(d->colors, colors, 3 * n);
+        copy_floats(d->opacities, opacities, n);
+    }
+    catch (...)
     {
         delete d;
         return nullptr;
     }
+    d->cull_scale = cull_scale_of(shader_params_arrays);
     return (pnanovdb_raster_gaussian_data_t*)d;
+}
+
+// one upload per data object (RasterContext.cpp:201-229, has_uploaded)
+static int upload_data(GaussianDataImpl* d, Context* ctx)
+{
+    if (d->uploaded)
+        return 0;
+    d->device = ctx->device;
+    const uint64_t n = d->count;
+    if (!d->dev)
+    {
+        void* p = nullptr;
+        if (hipMalloc(&p, (n > 0 ? n : 1) * 16 * sizeof(float)) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return ctx->fail("upload_gaussian_data: device allocation failed");
+        }
+        d->dev = (float*)p;
+    }
+    if (n > 0)
+    {
+        VDBR_CHECK(ctx, hipMemcpyAsync(d->d_means(), d->means.data(), 3 * n * 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(d->d_quats(), d->quats.data(), 4 * n * 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(d->d_scales(), d->scales.data(), 3 * n * 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(d->d_colors(), d->colors.data(), 3 * n * 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(d->d_opac(), d->opacities.data(), n * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    d->uploaded = true;
+    return 0;
 }
 
 static void PNANOVDB_ABI r_upload_gaussian_data(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
                                                 pnanovdb_raster_context_t* context, pnanovdb_raster_gaussian_data_t* data)
 {
-    // host copies are uploaded by each raster call
-    (void)compute;
-    (void)queue;
     (void)context;
-    (void)data;
+    GaussianDataImpl* d = (GaussianDataImpl*)data;
+    if (!d)
+        return;
+    std::lock_guard<std::mutex> lock(g_default_mutex);
+    Context* ctx = default_context();
+    if (!ctx)
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "upload_gaussian_data: no HIP device");
+        return;
+    }
+    if (upload_data(d, ctx))
+        log_message(compute, queue, LOG_LEVEL_ERROR, "%s", ctx->error.c_str());
 }
 
 static void PNANOVDB_ABI r_destroy_gaussian_data(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
@@ -551,7 +704,8 @@ static void PNANOVDB_ABI r_destroy_gaussian_data(const pnanovdb_compute_t* compu
 static int run_host(Context* ctx, uint64_t n, const float* means, const float* quats, const float* scales,
                     const float* colors_or_sh0, const float* opacities, float voxel_size, float cull_scale,
                     uint32_t flags, std::vector<uint8_t>* grid_out, float* quats_back, float* scales_back,
-                    float* opac_back, bool onindex = false)
+                    float* opac_back, bool onindex = false, vdbr_raster_stats_t* stats = nullptr,
+                    float* ms_convert = nullptr)
 {
     uint64_t nn = n > 0 ? n : 1;
     size_t bytes = nn * (3 + 4 + 3 + 3 + 1 + 3) * sizeof(float);
@@ -589,14 +743,30 @@ static int run_host(Context* ctx, uint64_t n, const float* means, const float* q
     p.flags = flags;
     void* grid = nullptr;
     uint64_t size = 0;
-    if (raster_node2(ctx, &g, &p, &grid, &size, nullptr))
+    vdbr_raster_stats_t local;
+    if (raster_node2(ctx, &g, &p, &grid, &size, stats ? stats : &local))
         return -1;
     if (onindex)
     {
         // RasterUtils.cpp:187-204: the node2 grid converted to OnIndex (putil/Convert.h)
         void* oni = nullptr;
         uint64_t oni_size = 0, oni_alloc = 0;
-        if (node2_to_onindex(ctx, grid, size, &oni, &oni_size, &oni_alloc))
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool timed = ms_convert && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+        if (timed)
+            (void)hipEventRecord(e0, ctx->stream);
+        const int rc = node2_to_onindex(ctx, grid, size, &oni, &oni_size, &oni_alloc);
+        if (timed)
+        {
+            (void)hipEventRecord(e1, ctx->stream);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(ms_convert, e0, e1);
+        }
+        if (e0)
+            (void)hipEventDestroy(e0);
+        if (e1)
+            (void)hipEventDestroy(e1);
+        if (rc)
             return -1;
         grid = oni;
         size = oni_alloc;
@@ -623,6 +793,11 @@ static pnanovdb_compute_array_t* make_output(const pnanovdb_compute_t* compute, 
     return out;
 }
 
+static double seconds_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // Raster.cpp:560-672: preprocessed arrays -> node2 grid array, trimmed to the grid size
 static pnanovdb_compute_array_t* PNANOVDB_ABI r_raster_to_nanovdb(
     const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue, float voxel_size,
@@ -631,63 +806,124 @@ static pnanovdb_compute_array_t* PNANOVDB_ABI r_raster_to_nanovdb(
     pnanovdb_compute_array_t* opacities, pnanovdb_compute_array_t** shader_params_arrays,
     pnanovdb_profiler_report_t profiler_report, void* userdata)
 {
-    (void)queue;
     (void)sh_0;
     (void)sh_n;
-    (void)profiler_report;
     (void)userdata;
-    if (!means || !quaternions || !scales || !colors || !opacities)
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!means)
         return nullptr;
-    uint64_t n = means->element_count / 3u;
-    if (quaternions->element_count < 4 * n || scales->element_count < 3 * n || colors->element_count < 3 * n ||
-        opacities->element_count < n)
+    const uint64_t n = means->element_count / 3u;
+    if (!floats_ok(means, 3 * n) || !floats_ok(quaternions, 4 * n) || !floats_ok(scales, 3 * n) ||
+        !floats_ok(colors, 3 * n) || !floats_ok(opacities, n))
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "raster_to_nanovdb: array lengths disagree with %llu Gaussians",
+                    (unsigned long long)n);
         return nullptr;
+    }
     std::lock_guard<std::mutex> lock(g_default_mutex);
     Context* ctx = default_context();
     if (!ctx)
-        return nullptr;
-    std::vector<uint8_t> grid;
-    if (run_host(ctx, n, (const float*)means->data, (const float*)quaternions->data, (const float*)scales->data,
-                 (const float*)colors->data, (const float*)opacities->data, voxel_size,
-                 cull_scale_of(shader_params_arrays), 0u, &grid, nullptr, nullptr, nullptr))
     {
-        fprintf(stderr, "libvdbraster: %s\n", ctx->error.c_str());
+        log_message(compute, queue, LOG_LEVEL_ERROR, "raster_to_nanovdb: no HIP device");
         return nullptr;
     }
-    return make_output(compute, grid);
+    std::vector<uint8_t> grid;
+    vdbr_raster_stats_t st;
+    memset(&st, 0, sizeof(st));
+    if (run_host(ctx, n, (const float*)means->data, (const float*)quaternions->data, (const float*)scales->data,
+                 (const float*)colors->data, (const float*)opacities->data, voxel_size,
+                 cull_scale_of(shader_params_arrays), 0u, &grid, nullptr, nullptr, nullptr, false, &st))
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "raster_to_nanovdb: %s", ctx->error.c_str());
+        return nullptr;
+    }
+    pnanovdb_compute_array_t* out = make_output(compute, grid);
+    report_profile(profiler_report, st, 0.f, seconds_since(t0));
+    return out;
 }
 
+// Raster.cpp:420-558: the Gaussian data's device arrays (uploaded once) -> node2 grid written into
+// the caller's device buffer of nanovdb_word_count words (zero-filled past the grid). A grid larger
+// than the buffer is an error: nothing is written and the message is logged.
 static void PNANOVDB_ABI r_raster_gaussian_3d(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
                                               pnanovdb_raster_context_t* context, float voxel_size,
                                               pnanovdb_raster_gaussian_data_t* data, pnanovdb_compute_buffer_t* nanovdb_out,
                                               pnanovdb_uint64_t nanovdb_word_count, void* userdata)
 {
-    // nanovdb_out is a vdbr_buffer_t* (device memory of the caller)
-    (void)compute;
-    (void)queue;
     (void)context;
     (void)userdata;
     GaussianDataImpl* d = (GaussianDataImpl*)data;
     vdbr_buffer_t* out = (vdbr_buffer_t*)nanovdb_out;
     if (!d || !out || !out->device_ptr)
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "raster_gaussian_3d: null gaussian data or output buffer");
         return;
+    }
     std::lock_guard<std::mutex> lock(g_default_mutex);
     Context* ctx = default_context();
     if (!ctx)
-        return;
-    std::vector<uint8_t> grid;
-    if (run_host(ctx, d->count, d->means.data(), d->quats.data(), d->scales.data(), d->colors.data(),
-                 d->opacities.data(), voxel_size, d->cull_scale, 0u, &grid, nullptr, nullptr, nullptr))
     {
-        fprintf(stderr, "libvdbraster: %s\n", ctx->error.c_str());
+        log_message(compute, queue, LOG_LEVEL_ERROR, "raster_gaussian_3d: no HIP device");
+        return;
+    }
+    if (upload_data(d, ctx))
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "raster_gaussian_3d: %s", ctx->error.c_str());
+        return;
+    }
+    vdbr_gaussians_t g;
+    g.means = d->d_means();
+    g.opacities = d->d_opac();
+    g.quats = d->d_quats();
+    g.scales = d->d_scales();
+    g.colors = d->d_colors();
+    g.sh0 = nullptr;
+    g.count = d->count;
+    vdbr_raster_params_t p;
+    memset(&p, 0, sizeof(p));
+    p.voxel_size = voxel_size;
+    p.cull_scale = d->cull_scale;
+    p.value_mode = VDBR_VALUE_RGBA8;
+    void* grid = nullptr;
+    uint64_t size = 0;
+    vdbr_raster_stats_t st;
+    if (raster_node2(ctx, &g, &p, &grid, &size, &st))
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "raster_gaussian_3d: %s", ctx->error.c_str());
         return;
     }
     uint64_t cap = nanovdb_word_count * 4u;
     if (out->size_bytes < cap)
         cap = out->size_bytes;
-    uint64_t copy = grid.size() < cap ? grid.size() : cap;
-    (void)hipMemset(out->device_ptr, 0, cap);
-    (void)hipMemcpy(out->device_ptr, grid.data(), copy, hipMemcpyHostToDevice);
+    if (size > cap)
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR,
+                    "raster_gaussian_3d: the grid needs %llu bytes, the output buffer holds %llu",
+                    (unsigned long long)size, (unsigned long long)cap);
+        return;
+    }
+    hipError_t e = hipMemcpyAsync(out->device_ptr, grid, size, hipMemcpyDeviceToDevice, ctx->stream);
+    if (e == hipSuccess && cap > size)
+        e = hipMemsetAsync((uint8_t*)out->device_ptr + size, 0, cap - size, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+        log_message(compute, queue, LOG_LEVEL_ERROR, "raster_gaussian_3d: %s", hipGetErrorString(e));
+}
+
+// raw Gaussian arrays (means, opacities, quats, scales, sh_0[, sh_n]) of a table call: present,
+// float32 and long enough for the Gaussian count of the means
+static bool raw_arrays_ok(pnanovdb_compute_array_t** arrays, uint32_t count, uint64_t* n_out)
+{
+    if (!arrays || count < 5u || !arrays[0])
+        return false;
+    const uint64_t n = arrays[0]->element_count / 3u;
+    const uint64_t need[5] = { 3 * n, n, 4 * n, 3 * n, 3 * n };
+    for (int i = 0; i < 5; i++)
+        if (!floats_ok(arrays[i], need[i]))
+            return false;
+    *n_out = n;
+    return true;
 }
 
 // RasterUtils.cpp:143-207, 400-415: raw arrays (means, opacities, quats, scales, sh_0, sh_n);
@@ -700,95 +936,116 @@ static pnanovdb_bool_t PNANOVDB_ABI r_raster_to_nanovdb_from_arrays(pnanovdb_ras
                                                                     pnanovdb_compute_array_t** out)
 {
     (void)raster;
-    (void)queue;
-    if (array_count < 5u || !arrays || !out)
-        return PNANOVDB_FALSE;
-    for (int i = 0; i < 5; i++)
-        if (!arrays[i] || (!arrays[i]->data && arrays[i]->element_count))
-            return PNANOVDB_FALSE;
-    uint64_t n = arrays[0]->element_count / 3u;
-    if (arrays[1]->element_count < n || arrays[2]->element_count < 4 * n || arrays[3]->element_count < 3 * n ||
-        arrays[4]->element_count < 3 * n)
+    uint64_t n = 0;
+    if (!out || !raw_arrays_ok(arrays, array_count, &n))
         return PNANOVDB_FALSE;
     std::lock_guard<std::mutex> lock(g_default_mutex);
     Context* ctx = default_context();
     if (!ctx)
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "Failed to rasterize Gaussian data: no HIP device");
         return PNANOVDB_FALSE;
+    }
     std::vector<uint8_t> grid;
     if (run_host(ctx, n, (const float*)arrays[0]->data, (const float*)arrays[2]->data, (const float*)arrays[3]->data,
                  (const float*)arrays[4]->data, (const float*)arrays[1]->data, voxel_size, 0.f, VDBR_FLAG_RAW_INPUTS,
                  &grid, (float*)arrays[2]->data, (float*)arrays[3]->data, (float*)arrays[1]->data, true))
     {
-        fprintf(stderr, "libvdbraster: %s\n", ctx->error.c_str());
+        log_message(compute, queue, LOG_LEVEL_ERROR, "Failed to rasterize Gaussian data: %s", ctx->error.c_str());
         return PNANOVDB_FALSE;
     }
     *out = make_output(compute, grid);
     return *out ? PNANOVDB_TRUE : PNANOVDB_FALSE;
 }
 
+// RasterUtils.cpp:209-290, 417-441 (process_arrays_to_create_gaussian_data): the caller's raw arrays
+// are preprocessed in place (quaternions normalised, scales exponentiated, opacities through the
+// sigmoid; colours from sh_0), then Gaussian data is created from them. The preprocessing runs on
+// the device and its results stay there as the data's uploaded arrays.
 static pnanovdb_bool_t PNANOVDB_ABI r_create_gaussian_data_from_arrays(
     pnanovdb_raster_t* raster, const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue,
     pnanovdb_compute_array_t** arrays, pnanovdb_uint32_t array_count, pnanovdb_raster_gaussian_data_t** gaussian_data,
     pnanovdb_raster_shader_params_t* raster_params, pnanovdb_raster_context_t** raster_context)
 {
     (void)raster;
-    (void)compute;
-    (void)queue;
     (void)raster_params;
-    if (array_count < 5u || !arrays || !gaussian_data)
+    if (pnanovdb_compute_log_print_t lp = host_log_print(compute, queue))
+        lp(LOG_LEVEL_WARNING, "pnanovdb_raster::create_gaussian_data_from_arrays is deprecated and will be removed in "
+                              "version 0.2.0. Please use pnanovdb_raster::create_gaussian_data_from_desc instead.");
+    uint64_t n = 0;
+    if (!gaussian_data || !raw_arrays_ok(arrays, array_count, &n))
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR,
+                    "create_gaussian_data_from_arrays: need float32 means[3N], opacities[N], quaternions[4N], "
+                    "scales[3N] and sh_0[3N]");
         return PNANOVDB_FALSE;
-    uint64_t n = arrays[0]->element_count / 3u;
+    }
     GaussianDataImpl* d = new (std::nothrow) GaussianDataImpl();
     if (!d)
         return PNANOVDB_FALSE;
     d->count = n;
-    d->cull_scale = 0.f;
-    copy_floats(d->means, arrays[0]);
-    copy_floats(d->opacities, arrays[1]);
-    copy_floats(d->quats, arrays[2]);
-    copy_floats(d->scales, arrays[3]);
-    d->colors.resize(3 * n);
-    // RasterUtils.cpp:31-136 on the host copies, same arithmetic as the device preprocessing
+    std::lock_guard<std::mutex> lock(g_default_mutex);
+    Context* ctx = default_context();
+    auto fail = [&](const char* what) {
+        log_message(compute, queue, LOG_LEVEL_ERROR, "create_gaussian_data_from_arrays: %s", what);
+        delete d;
+        return PNANOVDB_FALSE;
+    };
+    if (!ctx)
+        return fail("no HIP device");
+    try
     {
-        std::lock_guard<std::mutex> lock(g_default_mutex);
-        Context* ctx = default_context();
-        if (!ctx)
-        {
-            delete d;
-            return PNANOVDB_FALSE;
-        }
-        std::vector<uint8_t> unused;
-        // run the device preprocessing by rasterizing nothing: upload, preprocess, download
-        uint64_t nn = n > 0 ? n : 1;
-        float* dev = (float*)ctx->ws(WS_RAW_COPY, nn * 17 * sizeof(float));
-        if (!dev)
-        {
-            delete d;
-            return PNANOVDB_FALSE;
-        }
-        (void)unused;
-        float* d_quats = dev + 3 * nn;
-        float* d_scales = d_quats + 4 * nn;
-        float* d_colors = d_scales + 3 * nn;
-        float* d_opac = d_colors + 3 * nn;
-        float* d_sh0 = d_opac + nn;
-        if (n > 0)
-        {
-            (void)hipMemcpy(d_quats, d->quats.data(), 4 * n * 4, hipMemcpyHostToDevice);
-            (void)hipMemcpy(d_scales, d->scales.data(), 3 * n * 4, hipMemcpyHostToDevice);
-            (void)hipMemcpy(d_opac, d->opacities.data(), n * 4, hipMemcpyHostToDevice);
-            (void)hipMemcpy(d_sh0, arrays[4]->data, 3 * n * 4, hipMemcpyHostToDevice);
-            if (preprocess_device(ctx, n, d_opac, d_quats, d_scales, d_sh0, d_colors))
-            {
-                delete d;
-                return PNANOVDB_FALSE;
-            }
-            (void)hipMemcpy(d->quats.data(), d_quats, 4 * n * 4, hipMemcpyDeviceToHost);
-            (void)hipMemcpy(d->scales.data(), d_scales, 3 * n * 4, hipMemcpyDeviceToHost);
-            (void)hipMemcpy(d->opacities.data(), d_opac, n * 4, hipMemcpyDeviceToHost);
-            (void)hipMemcpy(d->colors.data(), d_colors, 3 * n * 4, hipMemcpyDeviceToHost);
-        }
+        copy_floats(d->means, arrays[0], 3 * n);
+        d->quats.resize(4 * n);
+        d->scales.resize(3 * n);
+        d->colors.resize(3 * n);
+        d->opacities.resize(n);
     }
+    catch (...)
+    {
+        return fail("host allocation failed");
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, (n > 0 ? n : 1) * 19 * sizeof(float)) != hipSuccess) // 16 data + 3 sh_0 floats per Gaussian
+    {
+        (void)hipGetLastError();
+        return fail("device allocation failed");
+    }
+    d->dev = (float*)p;
+    d->device = ctx->device;
+    float* d_sh0 = d->dev + 16 * n;
+    if (n > 0)
+    {
+        hipError_t e = hipMemcpyAsync(d->d_means(), arrays[0]->data, 3 * n * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d->d_opac(), arrays[1]->data, n * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d->d_quats(), arrays[2]->data, 4 * n * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d->d_scales(), arrays[3]->data, 3 * n * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_sh0, arrays[4]->data, 3 * n * 4, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess)
+            return fail(hipGetErrorString(e));
+        if (preprocess_device(ctx, n, d->d_opac(), d->d_quats(), d->d_scales(), d_sh0, d->d_colors()))
+            return fail(ctx->error.c_str());
+        e = hipMemcpyAsync(d->quats.data(), d->d_quats(), 4 * n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d->scales.data(), d->d_scales(), 3 * n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d->opacities.data(), d->d_opac(), n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d->colors.data(), d->d_colors(), 3 * n * 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess)
+            return fail(hipGetErrorString(e));
+        // RasterUtils.cpp:59-117 preprocesses the caller's arrays in place
+        memcpy(arrays[1]->data, d->opacities.data(), n * 4);
+        memcpy(arrays[2]->data, d->quats.data(), 4 * n * 4);
+        memcpy(arrays[3]->data, d->scales.data(), 3 * n * 4);
+    }
+    d->uploaded = true;
     *gaussian_data = (pnanovdb_raster_gaussian_data_t*)d;
     if (raster_context && !*raster_context)
         *raster_context = r_create_context(compute, queue);
@@ -805,8 +1062,8 @@ static pnanovdb_bool_t PNANOVDB_ABI r_raster_file(pnanovdb_raster_t* raster, con
                                                   pnanovdb_raster_shader_params_t* raster_params,
                                                   pnanovdb_profiler_report_t profiler_report, void* userdata)
 {
-    (void)profiler_report;
     (void)userdata;
+    const auto t0 = std::chrono::steady_clock::now();
     const pnanovdb_compute_t* c = (compute && compute->create_array) ? compute : compute_table();
     const char* names[6] = { "means", "opacities", "quaternions", "scales", "sh_0", "sh_n" };
     pnanovdb_compute_array_t* arrays[6] = {};
@@ -814,54 +1071,49 @@ static pnanovdb_bool_t PNANOVDB_ABI r_raster_file(pnanovdb_raster_t* raster, con
     FileArrayAlloc alloc = [&](size_t es, uint64_t count, const void* data) { return c->create_array(es, count, data); };
     if (!ff_load(filename, 6, names, arrays, alloc, &err))
     {
-        fprintf(stderr, "libvdbraster: Failed to load Gaussian data from file '%s': %s\n", filename ? filename : "",
-                err.c_str());
+        log_message(compute, queue, LOG_LEVEL_ERROR, "Failed to load Gaussian data from file '%s': %s",
+                    filename ? filename : "", err.c_str());
+        for (int i = 0; i < 6; i++)
+            if (arrays[i])
+                c->destroy_array(arrays[i]);
         return PNANOVDB_FALSE;
     }
     pnanovdb_bool_t ok = PNANOVDB_TRUE;
-    for (int i = 0; i < 5; i++)
-        if (!arrays[i] || arrays[i]->element_size != 4)
+    uint64_t n = 0;
+    if (!raw_arrays_ok(arrays, 6, &n))
+    {
+        log_message(compute, queue, LOG_LEVEL_ERROR,
+                    "'%s': Gaussian arrays missing, not float32 or of disagreeing lengths", filename);
+        ok = PNANOVDB_FALSE;
+    }
+    else if (gaussian_data && raster_context)
+        ok = r_create_gaussian_data_from_arrays(raster, compute, queue, arrays, 6, gaussian_data, raster_params,
+                                                raster_context);
+    else if (!nanovdb_arr)
+        ok = PNANOVDB_FALSE;
+    else
+    {
+        // RasterUtils.cpp:143-207: preprocess, rasterize with the call's shader parameters, OnIndex
+        std::lock_guard<std::mutex> lock(g_default_mutex);
+        Context* ctx = default_context();
+        std::vector<uint8_t> grid;
+        vdbr_raster_stats_t st;
+        memset(&st, 0, sizeof(st));
+        float ms_convert = 0.f;
+        if (!ctx || run_host(ctx, n, (const float*)arrays[0]->data, (const float*)arrays[2]->data,
+                             (const float*)arrays[3]->data, (const float*)arrays[4]->data,
+                             (const float*)arrays[1]->data, voxel_size, cull_scale_of(shader_params_arrays),
+                             VDBR_FLAG_RAW_INPUTS, &grid, nullptr, nullptr, nullptr, true, &st, &ms_convert))
         {
-            fprintf(stderr, "libvdbraster: '%s': array '%s' is missing or not float32\n", filename, names[i]);
+            log_message(compute, queue, LOG_LEVEL_ERROR, "Failed to rasterize Gaussian data: %s",
+                        ctx ? ctx->error.c_str() : "no HIP device");
             ok = PNANOVDB_FALSE;
         }
-    if (ok)
-    {
-        if (gaussian_data && raster_context)
-            ok = r_create_gaussian_data_from_arrays(raster, compute, queue, arrays, 6, gaussian_data, raster_params,
-                                                    raster_context);
-        else if (!nanovdb_arr)
-            ok = PNANOVDB_FALSE;
         else
         {
-            // RasterUtils.cpp:143-207: preprocess, rasterize with the call's shader parameters, OnIndex
-            const uint64_t n = arrays[0]->element_count / 3u;
-            if (arrays[1]->element_count < n || arrays[2]->element_count < 4 * n || arrays[3]->element_count < 3 * n ||
-                arrays[4]->element_count < 3 * n)
-            {
-                fprintf(stderr, "libvdbraster: '%s': array lengths disagree\n", filename);
-                ok = PNANOVDB_FALSE;
-            }
-            else
-            {
-                std::lock_guard<std::mutex> lock(g_default_mutex);
-                Context* ctx = default_context();
-                std::vector<uint8_t> grid;
-                if (!ctx ||
-                    run_host(ctx, n, (const float*)arrays[0]->data, (const float*)arrays[2]->data,
-                             (const float*)arrays[3]->data, (const float*)arrays[4]->data, (const float*)arrays[1]->data,
-                             voxel_size, cull_scale_of(shader_params_arrays), VDBR_FLAG_RAW_INPUTS, &grid, nullptr,
-                             nullptr, nullptr, true))
-                {
-                    fprintf(stderr, "libvdbraster: %s\n", ctx ? ctx->error.c_str() : "no HIP device");
-                    ok = PNANOVDB_FALSE;
-                }
-                else
-                {
-                    *nanovdb_arr = make_output(compute, grid);
-                    ok = *nanovdb_arr ? PNANOVDB_TRUE : PNANOVDB_FALSE;
-                }
-            }
+            *nanovdb_arr = make_output(compute, grid);
+            ok = *nanovdb_arr ? PNANOVDB_TRUE : PNANOVDB_FALSE;
+            report_profile(profiler_report, st, ms_convert, seconds_since(t0));
         }
     }
     for (int i = 0; i < 6; i++)
@@ -1056,6 +1308,7 @@ PNANOVDB_API pnanovdb_fileformat_t* pnanovdb_get_fileformat(void)
         memset(&g_fileformat, 0, sizeof(g_fileformat));
         g_fileformat.can_load_file = ff_can_load_file;
         g_fileformat.load_file = ff_load_file;
+        g_fileformat.module = own_module();
         g_fileformat_init = true;
     }
     return &g_fileformat;

@@ -11,6 +11,7 @@
 #include <stdint.h>
 
 #include "vdb_fpmath.h"
+#include "vdb_keylayout.h"
 #include "vdb_layout.h"
 
 #define VDB_DEV __device__ __forceinline__
@@ -51,15 +52,6 @@ struct __align__(16) PrimVal
 // n = (i&7)<<6 | (j&7)<<3 | (k&7). Voxels of one leaf are contiguous and in leaf order.
 // Pair key = voxel << (gid_bits + alpha_bits) | gid << alpha_bits | alpha8, alpha_bits 8 when
 // the RGBA8 alpha of the pair travels in the key (0 otherwise).
-struct KeyLayout
-{
-    int32_t lmin[3];
-    uint32_t bits[3];
-    uint32_t gid_bits;
-    uint32_t voxel_bits; // bits[0] + bits[1] + bits[2] + 9
-    uint32_t alpha_bits; // 8 or 0
-    uint32_t pad;
-};
 
 // Correctly rounded FP32 division and square root, independent of compiler flags: the exact
 // result of a float operation rounded to double (53 bits) and then to float (24 bits) equals the

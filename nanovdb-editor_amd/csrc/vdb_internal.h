@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/pnanovdb_amd.h"
+#include "vdb_keylayout.h"
 
 namespace vdbr
 {
@@ -88,6 +89,10 @@ enum WsSlot
     WS_LR_G0,
     WS_LR_KEY,
     WS_LR_CNT,
+    WS_GB_KEYS,
+    WS_GB_FLAGS,
+    WS_GB_ENC,
+    WS_GB_T,
     WS_COUNT
 };
 
@@ -203,6 +208,33 @@ uint64_t noalloc_max_keys();
 uint32_t sort_digit_bits(const Context* ctx, uint64_t n);
 unsigned long long* sort_hist_buffer(Context* ctx);
 
+// vdb_raster.hip: the GridBuild stage shared by the raster path and the grid build table.
+// vox_key holds nvox sorted unique voxel keys in the layout kl (leaf index << 9 | slot), vox_val
+// their value words, vox_t their first-touch fragment windows (T windows); vbounds (device, 6
+// int32: min xyz, max xyz) the voxel bounding box.
+struct Node2BuildInput
+{
+    KeyLayout kl;
+    uint64_t* vox_key;
+    uint32_t* vox_val;
+    uint32_t* vox_t;
+    uint64_t nvox;
+    const int32_t* vbounds;
+    uint32_t T;
+    uint32_t max_nodes;
+    bool spatial_order;
+    bool voxel_coords;
+    uint32_t value_type;
+    float voxel_size;
+};
+struct Node2BuildOutput
+{
+    uint8_t* grid; // context workspace (WS_GRID), valid until the next build on the context
+    uint64_t grid_size;
+    uint64_t upper_count, lower_count, leaf_count;
+};
+int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* out);
+
 // vdb_raster.hip
 int raster_node2(Context* ctx, const vdbr_gaussians_t* g, const vdbr_raster_params_t* p, void** grid_dev,
                  uint64_t* grid_size, vdbr_raster_stats_t* stats);
@@ -226,6 +258,14 @@ int selftest_alpha8(Context* ctx, uint64_t count, uint64_t seed, uint64_t* misma
 // a multiple of 8) is the zero-padded buffer length. The result lives in the context workspace.
 int node2_to_onindex(Context* ctx, const void* node2, uint64_t node2_size, void** out_dev, uint64_t* out_size,
                      uint64_t* alloc_size);
+
+// vdb_abi.hip: the library's compute table, and messages routed as RasterUtils.cpp:163-186 does
+// (the host compute interface's log_print when the compute table carries one, else stderr)
+constexpr uint32_t LOG_LEVEL_ERROR = 0u; // PNANOVDB_COMPUTE_LOG_LEVEL_ERROR (putil/Compute.h)
+constexpr uint32_t LOG_LEVEL_WARNING = 1u;
+const pnanovdb_compute_t* compute_table();
+void log_message(const pnanovdb_compute_t* compute, pnanovdb_compute_queue_t* queue, uint32_t level, const char* fmt,
+                 ...);
 
 } // namespace vdbr
 

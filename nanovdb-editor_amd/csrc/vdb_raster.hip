@@ -2946,6 +2946,250 @@ struct Timer
     int n = 0;
 };
 
+// Node2 grid from sorted unique voxel keys (the GridBuild stage of the raster path,
+// GridBuild.cpp:361-650): leaves, lowers and uppers from key runs, node order by first-touch
+// window (or tree order), node sizes and value indices by scans, then the grid written by one
+// pass per level. The grid lives in the WS_GRID slot of the context.
+int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* out)
+{
+    hipStream_t st = ctx->stream;
+    const KeyLayout kl = in.kl;
+    const uint64_t nvox = in.nvox;
+    uint64_t* vox_key = in.vox_key;
+    uint32_t* vox_val = in.vox_val;
+    uint32_t* vox_t = in.vox_t;
+    const int32_t* vbounds = in.vbounds;
+    const uint32_t T = in.T;
+    const uint32_t max_nodes = in.max_nodes;
+    const float vs = in.voxel_size;
+    const float vs_inv = 1.f / vs;
+    const uint64_t nv1 = nvox > 0 ? nvox : 1;
+    uint32_t* vox_leaf = (uint32_t*)ctx->ws(WS_VOX_LEAF, nv1 * sizeof(uint32_t));
+    if (!vox_leaf)
+        return -1;
+
+    // ---------------------------------------------------------------- leaves / lowers / uppers
+    uint32_t* leaf_first = (uint32_t*)ctx->ws(WS_LEAF_FIRST, (nv1 + 2) * sizeof(uint32_t));
+    uint64_t nleaf = 0;
+    if (run_heads(ctx, vox_key, nvox, 9, leaf_first, &nleaf))
+        return -1;
+    const uint64_t nl1 = nleaf > 0 ? nleaf : 1;
+    uint64_t* leaf_mask = (uint64_t*)ctx->ws(WS_LEAF_MASK, nl1 * 8 * sizeof(uint64_t));
+    uint32_t* leaf_info = (uint32_t*)ctx->ws(WS_LEAF_INFO, nl1 * 6 * sizeof(uint32_t));
+    uint32_t* leaf_t = leaf_info;               // compact order
+    uint32_t* tree_t = leaf_info + nl1;         // tree order
+    uint32_t* leaf_pos = leaf_info + 2 * nl1;   // compact -> tree position
+    uint32_t* leaf_node_idx = leaf_info + 3 * nl1;
+    uint32_t* leaf_parent = leaf_info + 4 * nl1;
+    uint64_t* lkeys = (uint64_t*)ctx->ws(WS_LEAF_SORT_KEYS, nl1 * sizeof(uint64_t));
+    uint32_t* lvals = (uint32_t*)ctx->ws(WS_LEAF_SORT_VALS, nl1 * sizeof(uint32_t));
+    if (!leaf_mask || !leaf_info || !lkeys || !lvals)
+        return -1;
+    if (nleaf > 0)
+    {
+        hipLaunchKernelGGL(k_leaf_info, dim3((uint32_t)((nleaf + 3) / 4)), dim3(256), 0, st, vox_key, vox_t, leaf_first,
+                           nleaf, kl,
+                           leaf_mask, leaf_t, lkeys, lvals);
+        hipLaunchKernelGGL(k_run_member, dim3((uint32_t)((nleaf + 3) / 4)), dim3(256), 0, st, leaf_first, nleaf, vox_leaf);
+        if (radix_sort_u64(ctx, lkeys, lvals, nleaf, 0, 42, nullptr))
+            return -1;
+        hipLaunchKernelGGL(k_leaf_gather, dim3(blocks_for(nleaf)), dim3(256), 0, st, lkeys, lvals, nleaf, leaf_t, tree_t,
+                           leaf_pos);
+    }
+    // lowers: runs of tree key >> 12 over the tree-ordered leaves
+    uint32_t* lower_first = (uint32_t*)ctx->ws(WS_LOWER, (nl1 + 2) * sizeof(uint32_t) * 4);
+    uint64_t nlower = 0;
+    if (run_heads(ctx, lkeys, nleaf, 12, lower_first, &nlower))
+        return -1;
+    const uint64_t nlo1 = nlower > 0 ? nlower : 1;
+    uint64_t* lower_key = (uint64_t*)ctx->ws(WS_LOWER_MASK, nlo1 * sizeof(uint64_t) * 2);
+    uint32_t* lower_t = lower_first + (nl1 + 2);
+    uint32_t* lower_node_idx = lower_t + nlo1;
+    uint32_t* lower_parent = lower_node_idx + nlo1;
+    if (nlower > 0)
+    {
+        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nlower)), dim3(256), 0, st, lkeys, lower_first, nlower, 12u,
+                           lower_key);
+        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nlower + 3) / 4)), dim3(256), 0, st, tree_t, lower_first, nlower, lower_t);
+    }
+    uint32_t* upper_first = (uint32_t*)ctx->ws(WS_UPPER, (nlo1 + 2) * sizeof(uint32_t) * 4);
+    uint64_t nupper = 0;
+    if (run_heads(ctx, lower_key, nlower, 15, upper_first, &nupper))
+        return -1;
+    const uint64_t nu1 = nupper > 0 ? nupper : 1;
+    uint64_t* upper_key = (uint64_t*)ctx->ws(WS_UPPER_MASK, nu1 * sizeof(uint64_t));
+    uint32_t* upper_t = upper_first + (nlo1 + 2);
+    uint32_t* upper_node_idx = upper_t + nu1;
+    if (nupper > 0)
+    {
+        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nupper)), dim3(256), 0, st, lower_key, upper_first, nupper, 15u,
+                           upper_key);
+        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nupper + 3) / 4)), dim3(256), 0, st, lower_t, upper_first, nupper, upper_t);
+    }
+    const uint64_t node_count = 1 + nupper + nlower + nleaf;
+    if (node_count > max_nodes)
+        return ctx->fail("raster_node2: node count exceeds max_node_count");
+
+    // ---------------------------------------------------------------- node order
+    bool spatial = in.spatial_order || T <= 1;
+    if (!spatial && node_count > 1)
+    {
+        // histogram of first-touch windows per level
+        uint32_t* hist = (uint32_t*)ctx->ws(WS_HIST, (size_t)(3 * T + 4 * (T + 1)) * sizeof(uint32_t));
+        uint32_t* tot = hist + 3 * T;
+        uint32_t* tot_before = tot + (T + 1);
+        uint32_t* lev_before = tot_before + (T + 1); // 2 arrays of T+1 used per level in turn
+        VDBR_CHECK(ctx, hipMemsetAsync(hist, 0, (size_t)3 * T * sizeof(uint32_t), st));
+        if (nupper)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_t, nupper, T, hist);
+        if (nlower)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_t, nlower, T, hist + T);
+        if (nleaf)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nleaf)), dim3(256), 0, st, tree_t, nleaf, T, hist + 2 * T);
+        hipLaunchKernelGGL(k_window_totals, dim3(blocks_for(T)), dim3(256), 0, st, hist, T, tot);
+        if (scan_u32(ctx, tot, tot_before, T, false, nullptr))
+            return -1;
+        uint64_t* nkeys = (uint64_t*)ctx->ws(WS_NODE_KEYS, nl1 * sizeof(uint64_t));
+        uint32_t* nvals = (uint32_t*)ctx->ws(WS_NODE_VALS, nl1 * sizeof(uint32_t));
+        struct Level
+        {
+            uint64_t count;
+            const uint32_t* t;
+            const uint32_t* parent;
+            const uint64_t* slot_key;
+            uint32_t slot_bits;
+            uint32_t* node_idx;
+        };
+        // parents: upper -> root (0); lower -> its upper; leaf -> its lower
+        Level levels[3] = { { nupper, upper_t, nullptr, upper_key, 15u, upper_node_idx },
+                            { nlower, lower_t, lower_parent, lower_key, 15u, lower_node_idx },
+                            { nleaf, tree_t, leaf_parent, lkeys, 12u, leaf_node_idx } };
+        const uint32_t parent_bits = bits_for(node_count + 1);
+        if (20 + parent_bits + 15 > 64)
+            return ctx->fail("raster_node2: node order keys exceed 64 bits");
+        for (uint32_t lv = 0; lv < 3; lv++)
+        {
+            Level& L = levels[lv];
+            if (L.count == 0)
+                continue;
+            if (lv == 1)
+                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_first, nupper,
+                                   upper_node_idx, lower_parent);
+            if (lv == 2)
+                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_first, nlower,
+                                   lower_node_idx, leaf_parent);
+            if (scan_u32(ctx, hist + lv * T, lev_before, T, false, nullptr))
+                return -1;
+            hipLaunchKernelGGL(k_node_keys, dim3(blocks_for(L.count)), dim3(256), 0, st, L.t, lv == 0 ? nullptr : L.parent,
+                               L.slot_key, L.count, L.slot_bits, parent_bits, nkeys, nvals);
+            if (radix_sort_u64(ctx, nkeys, nvals, L.count, 0, 20 + parent_bits + L.slot_bits, nullptr))
+                return -1;
+            hipLaunchKernelGGL(k_node_assign, dim3(blocks_for(L.count)), dim3(256), 0, st, nkeys, nvals, L.count,
+                               parent_bits + L.slot_bits, tot_before, lev_before, hist, T, lv, L.node_idx);
+        }
+    }
+    else
+    {
+        if (nupper)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nupper)), dim3(256), 0, st, nupper, 1u, upper_node_idx);
+        if (nlower)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nlower)), dim3(256), 0, st, nlower, (uint32_t)(1 + nupper),
+                               lower_node_idx);
+        if (nleaf)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nleaf)), dim3(256), 0, st, nleaf,
+                               (uint32_t)(1 + nupper + nlower), leaf_node_idx);
+    }
+    // sizes and value counts by node idx -> exclusive scans
+    uint64_t* byidx = (uint64_t*)ctx->ws(WS_NODE_BYIDX, (node_count + 1) * 2 * sizeof(uint64_t));
+    uint64_t* size_by_idx = byidx;
+    uint64_t* vals_by_idx = byidx + (node_count + 1);
+    uint64_t* nscan = (uint64_t*)ctx->ws(WS_NODE_SCAN, (node_count + 1) * 2 * sizeof(uint64_t));
+    uint64_t* size_scan = nscan;
+    uint64_t* vals_scan = nscan + (node_count + 1);
+    VDBR_CHECK(ctx, hipMemsetAsync(byidx, 0, (node_count + 1) * 2 * sizeof(uint64_t), st));
+    if (nupper)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_node_idx, nupper, (uint32_t)node_count, VDB_SIZE_UPPER,
+                           nullptr, nullptr, size_by_idx, vals_by_idx);
+    if (nlower)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_node_idx, nlower, (uint32_t)node_count, VDB_SIZE_LOWER,
+                           nullptr, nullptr, size_by_idx, vals_by_idx);
+    if (nleaf)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nleaf)), dim3(256), 0, st, leaf_node_idx, nleaf, (uint32_t)node_count, VDB_SIZE_LEAF,
+                           leaf_mask, lvals, size_by_idx, vals_by_idx);
+    // node idx 0 is the root: its slot holds size 0 so the scan at idx 1 starts at 0
+    if (scan_u64(ctx, size_by_idx, size_scan, node_count, false, size_scan + node_count))
+        return -1;
+    if (scan_u64(ctx, vals_by_idx, vals_scan, node_count, false, vals_scan + node_count))
+        return -1;
+
+    // ---------------------------------------------------------------- grid layout
+    GridInfo gi;
+    memset(&gi, 0, sizeof(gi));
+    gi.root_addr = VDB_ROOT_ADDR;
+    gi.nodes_begin = VDB_ROOT_ADDR + VDB_SIZE_ROOT;
+    gi.end = gi.nodes_begin + nupper * VDB_SIZE_UPPER + nlower * VDB_SIZE_LOWER + nleaf * VDB_SIZE_LEAF;
+    gi.meta0 = gi.end;
+    gi.meta1 = gi.end + VDB_BLINDMETADATA_SIZE;
+    gi.bboxes = gi.meta1 + VDB_BLINDMETADATA_SIZE;
+    gi.values = gi.bboxes + 32;
+    gi.value_count = 2 + nvox;
+    gi.grid_size = gi.values + 4 * gi.value_count;
+    gi.upper_count = (uint32_t)nupper;
+    gi.lower_count = (uint32_t)nlower;
+    gi.leaf_count = (uint32_t)nleaf;
+    gi.value_type = in.value_type;
+    gi.node_count = (uint32_t)node_count;
+    gi.voxel_size = vs;
+    gi.voxel_size_inv = vs_inv;
+    uint8_t* grid = (uint8_t*)ctx->ws(WS_GRID, (gi.grid_size + 7) & ~7ull);
+    if (!grid)
+        return -1;
+    VDBR_CHECK(ctx, hipMemsetAsync(grid, 0, (gi.grid_size + 7) & ~7ull, st));
+    hipLaunchKernelGGL(k_write_header, dim3(1), dim3(64), 0, st, grid, gi, (const int32_t*)vbounds);
+    hipLaunchKernelGGL(k_write_root, dim3(1), dim3(1024), 0, st, grid, gi, (const uint64_t*)upper_key, nupper,
+                       (const uint32_t*)upper_node_idx, (const uint64_t*)size_scan);
+    if (nupper)
+        hipLaunchKernelGGL((k_write_internal<512>), dim3((uint32_t)nupper), dim3(256), 0, st, grid, gi,
+                           (const uint32_t*)upper_first, (const uint64_t*)lower_key, 15u, (const uint32_t*)upper_node_idx,
+                           (const uint32_t*)lower_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan,
+                           VDB_OFF_CMP_UPPER, VDB_OFF_CM_UPPER, VDB_OFF_CHILDREN_UPPER);
+    if (nlower)
+        hipLaunchKernelGGL((k_write_internal<64>), dim3((uint32_t)nlower), dim3(256), 0, st, grid, gi,
+                           (const uint32_t*)lower_first, (const uint64_t*)lkeys, 12u, (const uint32_t*)lower_node_idx,
+                           (const uint32_t*)leaf_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan,
+                           VDB_OFF_CMP_LOWER, VDB_OFF_CM_LOWER, VDB_OFF_CHILDREN_LOWER);
+    if (nleaf)
+        hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nleaf)), dim3(256), 0, st, grid, gi, nleaf,
+                           (const uint32_t*)lvals, (const uint64_t*)leaf_mask, (const uint32_t*)leaf_node_idx,
+                           (const uint64_t*)size_scan, (const uint64_t*)vals_scan);
+    int32_t* ijk_out = nullptr;
+    ctx->last_voxel_ijk = nullptr;
+    ctx->last_value_count = 0;
+    if (in.voxel_coords)
+    {
+        ijk_out = (int32_t*)ctx->ws(WS_VOX_IJK, 3 * (2 + nvox) * sizeof(int32_t));
+        if (!ijk_out)
+            return -1;
+        // the two background slots have no voxel
+        VDBR_CHECK(ctx, hipMemsetAsync(ijk_out, 0x80, 6 * sizeof(int32_t), st));
+        ctx->last_voxel_ijk = ijk_out;
+        ctx->last_value_count = 2 + nvox;
+    }
+    if (nvox)
+        hipLaunchKernelGGL(k_write_values, dim3(blocks_for(nvox)), dim3(256), 0, st, grid, gi, nvox,
+                           (const uint32_t*)vox_val, (const uint64_t*)vox_key, (const uint32_t*)leaf_first,
+                           (const uint32_t*)nullptr, (const uint32_t*)leaf_pos, (const uint32_t*)leaf_node_idx,
+                           (const uint64_t*)vals_scan, (const uint32_t*)vox_leaf, kl, ijk_out);
+    VDBR_CHECK(ctx, hipGetLastError());
+
+    out->grid = grid;
+    out->grid_size = gi.grid_size;
+    out->upper_count = nupper;
+    out->lower_count = nlower;
+    out->leaf_count = nleaf;
+    return 0;
+}
+
 int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_params_t* prm, void** grid_dev,
                  uint64_t* grid_size_out, vdbr_raster_stats_t* stats)
 {
@@ -3512,9 +3756,6 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         if (!vox_key || !vox_val || !vox_t)
             return -1;
     }
-    uint32_t* vox_leaf = (uint32_t*)ctx->ws(WS_VOX_LEAF, nv1 * sizeof(uint32_t));
-    if (!vox_leaf)
-        return -1;
     // the pair list of a single-slab build stays available (vdbr_last_pairs)
     ctx->last_pairs = slab_count == 1 ? last_sorted : nullptr;
     ctx->last_pair_count = slab_count == 1 ? last_slab_pairs : 0;
@@ -3526,223 +3767,28 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         ctx->last_bits[a] = kl.bits[a];
     }
     VDBR_CHECK(ctx, hipEventRecord(ev[4], st));
-
-    // ---------------------------------------------------------------- leaves / lowers / uppers
-    uint32_t* leaf_first = (uint32_t*)ctx->ws(WS_LEAF_FIRST, (nv1 + 2) * sizeof(uint32_t));
-    uint64_t nleaf = 0;
-    if (run_heads(ctx, vox_key, nvox, 9, leaf_first, &nleaf))
+    Node2BuildInput bin;
+    bin.kl = kl;
+    bin.vox_key = vox_key;
+    bin.vox_val = vox_val;
+    bin.vox_t = vox_t;
+    bin.nvox = nvox;
+    bin.vbounds = vbounds;
+    bin.T = T;
+    bin.max_nodes = max_nodes;
+    bin.spatial_order = (prm->flags & VDBR_FLAG_SPATIAL_ORDER) != 0;
+    bin.voxel_coords = (prm->flags & VDBR_FLAG_VOXEL_COORDS) != 0;
+    bin.value_type = prm->value_mode == VDBR_VALUE_RGBA8 ? VDB_GRID_TYPE_RGBA8 : VDB_GRID_TYPE_FLOAT;
+    bin.voxel_size = vs;
+    Node2BuildOutput bout;
+    if (build_node2_grid(ctx, bin, &bout))
         return -1;
-    const uint64_t nl1 = nleaf > 0 ? nleaf : 1;
-    uint64_t* leaf_mask = (uint64_t*)ctx->ws(WS_LEAF_MASK, nl1 * 8 * sizeof(uint64_t));
-    uint32_t* leaf_info = (uint32_t*)ctx->ws(WS_LEAF_INFO, nl1 * 6 * sizeof(uint32_t));
-    uint32_t* leaf_t = leaf_info;               // compact order
-    uint32_t* tree_t = leaf_info + nl1;         // tree order
-    uint32_t* leaf_pos = leaf_info + 2 * nl1;   // compact -> tree position
-    uint32_t* leaf_node_idx = leaf_info + 3 * nl1;
-    uint32_t* leaf_parent = leaf_info + 4 * nl1;
-    uint64_t* lkeys = (uint64_t*)ctx->ws(WS_LEAF_SORT_KEYS, nl1 * sizeof(uint64_t));
-    uint32_t* lvals = (uint32_t*)ctx->ws(WS_LEAF_SORT_VALS, nl1 * sizeof(uint32_t));
-    if (!leaf_mask || !leaf_info || !lkeys || !lvals)
-        return -1;
-    if (nleaf > 0)
-    {
-        hipLaunchKernelGGL(k_leaf_info, dim3((uint32_t)((nleaf + 3) / 4)), dim3(256), 0, st, vox_key, vox_t, leaf_first,
-                           nleaf, kl,
-                           leaf_mask, leaf_t, lkeys, lvals);
-        hipLaunchKernelGGL(k_run_member, dim3((uint32_t)((nleaf + 3) / 4)), dim3(256), 0, st, leaf_first, nleaf, vox_leaf);
-        if (radix_sort_u64(ctx, lkeys, lvals, nleaf, 0, 42, nullptr))
-            return -1;
-        hipLaunchKernelGGL(k_leaf_gather, dim3(blocks_for(nleaf)), dim3(256), 0, st, lkeys, lvals, nleaf, leaf_t, tree_t,
-                           leaf_pos);
-    }
-    // lowers: runs of tree key >> 12 over the tree-ordered leaves
-    uint32_t* lower_first = (uint32_t*)ctx->ws(WS_LOWER, (nl1 + 2) * sizeof(uint32_t) * 4);
-    uint64_t nlower = 0;
-    if (run_heads(ctx, lkeys, nleaf, 12, lower_first, &nlower))
-        return -1;
-    const uint64_t nlo1 = nlower > 0 ? nlower : 1;
-    uint64_t* lower_key = (uint64_t*)ctx->ws(WS_LOWER_MASK, nlo1 * sizeof(uint64_t) * 2);
-    uint32_t* lower_t = lower_first + (nl1 + 2);
-    uint32_t* lower_node_idx = lower_t + nlo1;
-    uint32_t* lower_parent = lower_node_idx + nlo1;
-    if (nlower > 0)
-    {
-        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nlower)), dim3(256), 0, st, lkeys, lower_first, nlower, 12u,
-                           lower_key);
-        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nlower + 3) / 4)), dim3(256), 0, st, tree_t, lower_first, nlower, lower_t);
-    }
-    uint32_t* upper_first = (uint32_t*)ctx->ws(WS_UPPER, (nlo1 + 2) * sizeof(uint32_t) * 4);
-    uint64_t nupper = 0;
-    if (run_heads(ctx, lower_key, nlower, 15, upper_first, &nupper))
-        return -1;
-    const uint64_t nu1 = nupper > 0 ? nupper : 1;
-    uint64_t* upper_key = (uint64_t*)ctx->ws(WS_UPPER_MASK, nu1 * sizeof(uint64_t));
-    uint32_t* upper_t = upper_first + (nlo1 + 2);
-    uint32_t* upper_node_idx = upper_t + nu1;
-    if (nupper > 0)
-    {
-        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nupper)), dim3(256), 0, st, lower_key, upper_first, nupper, 15u,
-                           upper_key);
-        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nupper + 3) / 4)), dim3(256), 0, st, lower_t, upper_first, nupper, upper_t);
-    }
-    const uint64_t node_count = 1 + nupper + nlower + nleaf;
-    if (node_count > max_nodes)
-        return ctx->fail("raster_node2: node count exceeds max_node_count");
-
-    // ---------------------------------------------------------------- node order
-    bool spatial = (prm->flags & VDBR_FLAG_SPATIAL_ORDER) != 0 || T <= 1;
-    if (!spatial && node_count > 1)
-    {
-        // histogram of first-touch windows per level
-        uint32_t* hist = (uint32_t*)ctx->ws(WS_HIST, (size_t)(3 * T + 4 * (T + 1)) * sizeof(uint32_t));
-        uint32_t* tot = hist + 3 * T;
-        uint32_t* tot_before = tot + (T + 1);
-        uint32_t* lev_before = tot_before + (T + 1); // 2 arrays of T+1 used per level in turn
-        VDBR_CHECK(ctx, hipMemsetAsync(hist, 0, (size_t)3 * T * sizeof(uint32_t), st));
-        if (nupper)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_t, nupper, T, hist);
-        if (nlower)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_t, nlower, T, hist + T);
-        if (nleaf)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nleaf)), dim3(256), 0, st, tree_t, nleaf, T, hist + 2 * T);
-        hipLaunchKernelGGL(k_window_totals, dim3(blocks_for(T)), dim3(256), 0, st, hist, T, tot);
-        if (scan_u32(ctx, tot, tot_before, T, false, nullptr))
-            return -1;
-        uint64_t* nkeys = (uint64_t*)ctx->ws(WS_NODE_KEYS, nl1 * sizeof(uint64_t));
-        uint32_t* nvals = (uint32_t*)ctx->ws(WS_NODE_VALS, nl1 * sizeof(uint32_t));
-        struct Level
-        {
-            uint64_t count;
-            const uint32_t* t;
-            const uint32_t* parent;
-            const uint64_t* slot_key;
-            uint32_t slot_bits;
-            uint32_t* node_idx;
-        };
-        // parents: upper -> root (0); lower -> its upper; leaf -> its lower
-        Level levels[3] = { { nupper, upper_t, nullptr, upper_key, 15u, upper_node_idx },
-                            { nlower, lower_t, lower_parent, lower_key, 15u, lower_node_idx },
-                            { nleaf, tree_t, leaf_parent, lkeys, 12u, leaf_node_idx } };
-        const uint32_t parent_bits = bits_for(node_count + 1);
-        if (20 + parent_bits + 15 > 64)
-            return ctx->fail("raster_node2: node order keys exceed 64 bits");
-        for (uint32_t lv = 0; lv < 3; lv++)
-        {
-            Level& L = levels[lv];
-            if (L.count == 0)
-                continue;
-            if (lv == 1)
-                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_first, nupper,
-                                   upper_node_idx, lower_parent);
-            if (lv == 2)
-                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_first, nlower,
-                                   lower_node_idx, leaf_parent);
-            if (scan_u32(ctx, hist + lv * T, lev_before, T, false, nullptr))
-                return -1;
-            hipLaunchKernelGGL(k_node_keys, dim3(blocks_for(L.count)), dim3(256), 0, st, L.t, lv == 0 ? nullptr : L.parent,
-                               L.slot_key, L.count, L.slot_bits, parent_bits, nkeys, nvals);
-            if (radix_sort_u64(ctx, nkeys, nvals, L.count, 0, 20 + parent_bits + L.slot_bits, nullptr))
-                return -1;
-            hipLaunchKernelGGL(k_node_assign, dim3(blocks_for(L.count)), dim3(256), 0, st, nkeys, nvals, L.count,
-                               parent_bits + L.slot_bits, tot_before, lev_before, hist, T, lv, L.node_idx);
-        }
-    }
-    else
-    {
-        if (nupper)
-            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nupper)), dim3(256), 0, st, nupper, 1u, upper_node_idx);
-        if (nlower)
-            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nlower)), dim3(256), 0, st, nlower, (uint32_t)(1 + nupper),
-                               lower_node_idx);
-        if (nleaf)
-            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nleaf)), dim3(256), 0, st, nleaf,
-                               (uint32_t)(1 + nupper + nlower), leaf_node_idx);
-    }
-    // sizes and value counts by node idx -> exclusive scans
-    uint64_t* byidx = (uint64_t*)ctx->ws(WS_NODE_BYIDX, (node_count + 1) * 2 * sizeof(uint64_t));
-    uint64_t* size_by_idx = byidx;
-    uint64_t* vals_by_idx = byidx + (node_count + 1);
-    uint64_t* nscan = (uint64_t*)ctx->ws(WS_NODE_SCAN, (node_count + 1) * 2 * sizeof(uint64_t));
-    uint64_t* size_scan = nscan;
-    uint64_t* vals_scan = nscan + (node_count + 1);
-    VDBR_CHECK(ctx, hipMemsetAsync(byidx, 0, (node_count + 1) * 2 * sizeof(uint64_t), st));
-    if (nupper)
-        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_node_idx, nupper, (uint32_t)node_count, VDB_SIZE_UPPER,
-                           nullptr, nullptr, size_by_idx, vals_by_idx);
-    if (nlower)
-        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_node_idx, nlower, (uint32_t)node_count, VDB_SIZE_LOWER,
-                           nullptr, nullptr, size_by_idx, vals_by_idx);
-    if (nleaf)
-        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nleaf)), dim3(256), 0, st, leaf_node_idx, nleaf, (uint32_t)node_count, VDB_SIZE_LEAF,
-                           leaf_mask, lvals, size_by_idx, vals_by_idx);
-    // node idx 0 is the root: its slot holds size 0 so the scan at idx 1 starts at 0
-    if (scan_u64(ctx, size_by_idx, size_scan, node_count, false, size_scan + node_count))
-        return -1;
-    if (scan_u64(ctx, vals_by_idx, vals_scan, node_count, false, vals_scan + node_count))
-        return -1;
-
-    // ---------------------------------------------------------------- grid layout
-    GridInfo gi;
-    memset(&gi, 0, sizeof(gi));
-    gi.root_addr = VDB_ROOT_ADDR;
-    gi.nodes_begin = VDB_ROOT_ADDR + VDB_SIZE_ROOT;
-    gi.end = gi.nodes_begin + nupper * VDB_SIZE_UPPER + nlower * VDB_SIZE_LOWER + nleaf * VDB_SIZE_LEAF;
-    gi.meta0 = gi.end;
-    gi.meta1 = gi.end + VDB_BLINDMETADATA_SIZE;
-    gi.bboxes = gi.meta1 + VDB_BLINDMETADATA_SIZE;
-    gi.values = gi.bboxes + 32;
-    gi.value_count = 2 + nvox;
-    gi.grid_size = gi.values + 4 * gi.value_count;
-    gi.upper_count = (uint32_t)nupper;
-    gi.lower_count = (uint32_t)nlower;
-    gi.leaf_count = (uint32_t)nleaf;
-    gi.value_type = prm->value_mode == VDBR_VALUE_RGBA8 ? VDB_GRID_TYPE_RGBA8 : VDB_GRID_TYPE_FLOAT;
-    gi.node_count = (uint32_t)node_count;
-    gi.voxel_size = vs;
-    gi.voxel_size_inv = vs_inv;
-    uint8_t* grid = (uint8_t*)ctx->ws(WS_GRID, (gi.grid_size + 7) & ~7ull);
-    if (!grid)
-        return -1;
-    VDBR_CHECK(ctx, hipMemsetAsync(grid, 0, (gi.grid_size + 7) & ~7ull, st));
-    hipLaunchKernelGGL(k_write_header, dim3(1), dim3(64), 0, st, grid, gi, (const int32_t*)vbounds);
-    hipLaunchKernelGGL(k_write_root, dim3(1), dim3(1024), 0, st, grid, gi, (const uint64_t*)upper_key, nupper,
-                       (const uint32_t*)upper_node_idx, (const uint64_t*)size_scan);
-    if (nupper)
-        hipLaunchKernelGGL((k_write_internal<512>), dim3((uint32_t)nupper), dim3(256), 0, st, grid, gi,
-                           (const uint32_t*)upper_first, (const uint64_t*)lower_key, 15u, (const uint32_t*)upper_node_idx,
-                           (const uint32_t*)lower_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan,
-                           VDB_OFF_CMP_UPPER, VDB_OFF_CM_UPPER, VDB_OFF_CHILDREN_UPPER);
-    if (nlower)
-        hipLaunchKernelGGL((k_write_internal<64>), dim3((uint32_t)nlower), dim3(256), 0, st, grid, gi,
-                           (const uint32_t*)lower_first, (const uint64_t*)lkeys, 12u, (const uint32_t*)lower_node_idx,
-                           (const uint32_t*)leaf_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan,
-                           VDB_OFF_CMP_LOWER, VDB_OFF_CM_LOWER, VDB_OFF_CHILDREN_LOWER);
-    if (nleaf)
-        hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nleaf)), dim3(256), 0, st, grid, gi, nleaf,
-                           (const uint32_t*)lvals, (const uint64_t*)leaf_mask, (const uint32_t*)leaf_node_idx,
-                           (const uint64_t*)size_scan, (const uint64_t*)vals_scan);
-    int32_t* ijk_out = nullptr;
-    ctx->last_voxel_ijk = nullptr;
-    ctx->last_value_count = 0;
-    if (prm->flags & VDBR_FLAG_VOXEL_COORDS)
-    {
-        ijk_out = (int32_t*)ctx->ws(WS_VOX_IJK, 3 * (2 + nvox) * sizeof(int32_t));
-        if (!ijk_out)
-            return -1;
-        // the two background slots have no voxel
-        VDBR_CHECK(ctx, hipMemsetAsync(ijk_out, 0x80, 6 * sizeof(int32_t), st));
-        ctx->last_voxel_ijk = ijk_out;
-        ctx->last_value_count = 2 + nvox;
-    }
-    if (nvox)
-        hipLaunchKernelGGL(k_write_values, dim3(blocks_for(nvox)), dim3(256), 0, st, grid, gi, nvox,
-                           (const uint32_t*)vox_val, (const uint64_t*)vox_key, (const uint32_t*)leaf_first,
-                           (const uint32_t*)nullptr, (const uint32_t*)leaf_pos, (const uint32_t*)leaf_node_idx,
-                           (const uint64_t*)vals_scan, (const uint32_t*)vox_leaf, kl, ijk_out);
-    VDBR_CHECK(ctx, hipGetLastError());
     VDBR_CHECK(ctx, hipEventRecord(ev[5], st));
     VDBR_CHECK(ctx, hipEventSynchronize(ev[5]));
-
+    uint8_t* grid = bout.grid;
+    GridInfo gi;
+    gi.grid_size = bout.grid_size;
+    const uint64_t nupper = bout.upper_count, nlower = bout.lower_count, nleaf = bout.leaf_count;
     *grid_dev = grid;
     *grid_size_out = gi.grid_size;
     if (stats)
@@ -3834,15 +3880,19 @@ int last_voxels(Context* ctx, int32_t* ijk_dev, uint64_t cap, uint64_t* count)
 }
 
 // value words of a node2 grid at voxel coordinates (0 where the voxel is inactive or outside)
-__global__ void k_node2_sample(const uint64_t* __restrict__ w, const uint32_t* __restrict__ values, uint64_t value_count,
-                               const int32_t* __restrict__ ijk, uint64_t n, uint32_t* __restrict__ out)
+__global__ void k_node2_sample(const uint64_t* __restrict__ w, uint64_t nwords, const uint32_t* __restrict__ values,
+                               uint64_t value_count, const int32_t* __restrict__ ijk, uint64_t n,
+                               uint32_t* __restrict__ out, uint32_t* __restrict__ bad_flag)
 {
     for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (uint64_t)gridDim.x * blockDim.x)
     {
         const int32_t i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
         uint32_t v = 0u;
         const bool in_range = i >= -65536 && i < 65536 && j >= -65536 && j < 65536 && k >= -65536 && k < 65536;
-        const uint64_t leaf = in_range ? n2_find_leaf(w, i, j, k) : 0ull;
+        bool bad = false;
+        const uint64_t leaf = in_range ? n2_find_leaf(w, nwords, i, j, k, &bad) : 0ull;
+        if (bad)
+            atomicOr(bad_flag, 1u);
         if (leaf)
         {
             const uint64_t idx = n2_leaf_value_index(w, leaf, i, j, k);
@@ -3862,11 +3912,18 @@ int node2_sample(Context* ctx, const void* grid, uint64_t grid_size, const uint3
         return ctx->fail("node2_sample: grid is not a node2 grid");
     if (n == 0)
         return 0;
+    uint32_t* bad_flag = (uint32_t*)ctx->ws(WS_MISC, 4096);
+    if (!bad_flag)
+        return -1;
+    VDBR_CHECK(ctx, hipMemsetAsync(bad_flag, 0, 4, ctx->stream));
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_node2_sample, dim3(blocks), dim3(256), 0, ctx->stream, (const uint64_t*)grid, values,
-                       value_count, ijk, n, out);
+    hipLaunchKernelGGL(k_node2_sample, dim3(blocks), dim3(256), 0, ctx->stream, (const uint64_t*)grid, grid_size / 8u,
+                       values, value_count, ijk, n, out, bad_flag);
     VDBR_CHECK(ctx, hipGetLastError());
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, bad_flag, 4, hipMemcpyDeviceToHost, ctx->stream));
     VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (*(const uint32_t*)ctx->pinned)
+        return ctx->fail("node2_sample: child offsets outside the grid (corrupt or truncated node2 grid)");
     return 0;
 }
 

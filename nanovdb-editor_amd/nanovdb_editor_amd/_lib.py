@@ -66,6 +66,7 @@ SIGNATURES = {
     "pnanovdb_get_compute": ([], P),
     "pnanovdb_get_parallel_primitives": ([], P),
     "pnanovdb_get_fileformat": ([], P),
+    "pnanovdb_get_grid_build": ([], P),
     "vdbr_context_create": ([ctypes.c_int], P),
     "vdbr_context_destroy": ([P], None),
     "vdbr_last_error": ([P], ctypes.c_char_p),

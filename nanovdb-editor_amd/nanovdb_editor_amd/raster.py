@@ -4,7 +4,7 @@
 ``Raster.raster_to_nanovdb`` and ``Raster.raster_to_nanovdb_from_arrays`` keep the reference
 argument order, validation and error behaviour.
 """
-from ctypes import CFUNCTYPE, POINTER, Structure, byref, c_char_p, c_float, c_uint32, c_void_p, pointer
+from ctypes import CFUNCTYPE, POINTER, Structure, byref, c_char_p, c_float, c_uint32, c_uint64, c_void_p, pointer
 
 from ._lib import lib
 from .compute import Compute, pnanovdb_bool_t, pnanovdb_Compute, pnanovdb_ComputeArray
@@ -27,7 +27,7 @@ class pnanovdb_Raster(Structure):
         ("destroy_gaussian_data", CFUNCTYPE(None, POINTER(pnanovdb_Compute), c_void_p, c_void_p)),
         ("raster_gaussian_2d", c_void_p),
         ("raster_gaussian_3d", CFUNCTYPE(None, POINTER(pnanovdb_Compute), c_void_p, c_void_p, c_float, c_void_p,
-                                         c_void_p, c_void_p, c_void_p)),
+                                         c_void_p, c_uint64, c_void_p)),
         ("raster_to_nanovdb", CFUNCTYPE(_ArrP, POINTER(pnanovdb_Compute), c_void_p, c_float, _ArrP, _ArrP, _ArrP,
                                         _ArrP, _ArrP, _ArrP, _ArrP, POINTER(_ArrP), c_void_p, c_void_p)),
         ("raster_file", CFUNCTYPE(pnanovdb_bool_t, c_void_p, POINTER(pnanovdb_Compute), c_void_p, c_char_p, c_float,
