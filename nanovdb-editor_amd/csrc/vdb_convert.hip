@@ -55,6 +55,9 @@ struct ConvLevel
     const uint32_t* child_rank; // exclusive scan of the child flags
     uint32_t* flags;          // child flags (flag pass)
     uint64_t* children;       // out: node2 word index of each child, in allocation order
+    uint64_t nwords;          // node2 grid length in 64-bit words
+    uint64_t child_words;     // node2 size of a child node in words
+    uint32_t* bad;            // set when a child offset leaves the grid
 };
 
 // flag pass: one thread per (parent, slot)
@@ -79,9 +82,13 @@ __global__ void k_conv_children(ConvLevel L)
     uint64_t p = t / L.fanout;
     uint32_t n = (uint32_t)(t % L.fanout);
     uint64_t node = L.parents[p];
-    bool active = false;
+    bool active = false, bad = false;
     uint32_t rank = n2_mask_n_to_idx(L.src, node + L.cmp_off, L.prefix_to_mask, n, &active);
-    L.children[L.child_rank[t]] = node + (L.src[node + L.children_off + rank] >> 3u);
+    const uint64_t child = n2_child(L.src, L.nwords, node, L.children_off, rank, L.fanout, L.child_words, &bad);
+    if (bad)
+        atomicOr(L.bad, 1u);
+    // a rejected child points at the grid's first node words (in bounds); the host stops before use
+    L.children[L.child_rank[t]] = bad ? 0ull : child;
 }
 
 // table pass: child byte offsets and tile value indices of the parents (Convert.h:171-233)
@@ -146,30 +153,35 @@ __global__ void k_conv_leaves(const uint64_t* __restrict__ src, uint8_t* __restr
 }
 
 // root slots: tile flags (an active key) and upper flags (a child)
-__global__ void k_conv_root_flags(const uint64_t* __restrict__ src, uint32_t* tile_flag, uint32_t* child_flag)
+__global__ void k_conv_root_flags(const uint64_t* __restrict__ src, uint32_t* tile_flag, uint32_t* child_flag,
+                                  uint32_t* bad_flag)
 {
     uint32_t n = blockIdx.x * CONV_THREADS + threadIdx.x;
     if (n >= 32768u)
         return;
     const uint64_t root = VDB_ROOT_ADDR >> 3;
-    bool key_active = false;
+    bool key_active = false, bad = false;
     uint32_t kidx = n2_mask_n_to_idx(src, root + VDB_OFF_KMP_ROOT, VDB_OFF_KM_ROOT - VDB_OFF_KMP_ROOT, n, &key_active);
-    uint32_t child_count = (uint32_t)(src[root + VDB_OFF_CMP_ROOT] & 0xFFFFu);
-    bool has_tile = key_active && src[root + VDB_OFF_CHILDREN_ROOT + child_count + kidx] != VDB_END_KEY;
+    bool has_tile = key_active && n2_root_key(src, kidx, &bad) != VDB_END_KEY && !bad;
+    if (bad)
+        atomicOr(bad_flag, 1u);
     tile_flag[n] = has_tile ? 1u : 0u;
     child_flag[n] = (has_tile && ((src[root + VDB_OFF_CM_ROOT + (n >> 6u)] >> (n & 63u)) & 1ull)) ? 1u : 0u;
 }
 
-__global__ void k_conv_root_uppers(const uint64_t* __restrict__ src, const uint32_t* child_flag,
-                                   const uint32_t* upper_rank, uint64_t* uppers)
+__global__ void k_conv_root_uppers(const uint64_t* __restrict__ src, uint64_t nwords, const uint32_t* child_flag,
+                                   const uint32_t* upper_rank, uint64_t* uppers, uint32_t* bad_flag)
 {
     uint32_t n = blockIdx.x * CONV_THREADS + threadIdx.x;
     if (n >= 32768u || !child_flag[n])
         return;
     const uint64_t root = VDB_ROOT_ADDR >> 3;
-    bool a = false;
+    bool a = false, bad = false;
     uint32_t rank = n2_mask_n_to_idx(src, root + VDB_OFF_CMP_ROOT, VDB_OFF_CM_ROOT - VDB_OFF_CMP_ROOT, n, &a);
-    uppers[upper_rank[n]] = root + (src[root + VDB_OFF_CHILDREN_ROOT + rank] >> 3u);
+    const uint64_t upper = n2_child(src, nwords, root, VDB_OFF_CHILDREN_ROOT, rank, 32768u, N2_WORDS_UPPER, &bad);
+    if (bad)
+        atomicOr(bad_flag, 1u);
+    uppers[upper_rank[n]] = bad ? 0ull : upper;
 }
 
 // root tiles in slot order (Convert.h:76-99, 116-127, 155-157)
@@ -182,11 +194,10 @@ __global__ void k_conv_root_tiles(const uint64_t* __restrict__ src, uint8_t* __r
         return;
     const uint64_t root = VDB_ROOT_ADDR >> 3;
     const uint64_t droot = VDB_ROOT_ADDR;
-    bool key_active = false;
+    bool key_active = false, bad = false;
     uint32_t kidx = n2_mask_n_to_idx(src, root + VDB_OFF_KMP_ROOT, VDB_OFF_KM_ROOT - VDB_OFF_KMP_ROOT, n, &key_active);
-    uint32_t child_count = (uint32_t)(src[root + VDB_OFF_CMP_ROOT] & 0xFFFFu);
     uint8_t* tile = dst + droot + ONI_ROOT_SIZE + (uint64_t)tile_rank[n] * ONI_TILE_SIZE;
-    *(uint64_t*)(tile + 0) = src[root + VDB_OFF_CHILDREN_ROOT + child_count + kidx];
+    *(uint64_t*)(tile + 0) = n2_root_key(src, kidx, &bad); // checked by the flag pass
     bool value_active = false;
     uint32_t local = n2_mask_n_to_idx(src, root + VDB_OFF_VMP_ROOT, VDB_OFF_VM_ROOT - VDB_OFF_VMP_ROOT, n, &value_active);
     *(uint32_t*)(tile + 16) = value_active ? 1u : 0u;
@@ -259,16 +270,27 @@ int node2_to_onindex(Context* ctx, const void* node2, uint64_t node2_size, void*
         return ctx->fail("node2_to_onindex: blind data outside the grid");
 
     // root level: tile and upper flags, ranks, the upper list
+    const uint64_t nwords = node2_size / 8u;
     uint32_t* root_ws = (uint32_t*)ctx->ws(WS_CONV_ROOT, (4 * 32768 + 4) * sizeof(uint32_t));
     if (!root_ws)
         return -1;
+    uint32_t* bad_flag = root_ws + 4 * 32768 + 2;
+    VDBR_CHECK(ctx, hipMemsetAsync(bad_flag, 0, 4, st));
+    // a child offset outside the grid (a corrupt or truncated grid) stops the conversion before the
+    // rejected nodes are read
+    auto grid_ok = [&]() -> bool {
+        if (hipMemcpyAsync(ctx->pinned, bad_flag, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return false;
+        return *(const uint32_t*)ctx->pinned == 0u;
+    };
     uint32_t* tile_flag = root_ws;
     uint32_t* child_flag = root_ws + 32768;
     uint32_t* tile_rank = root_ws + 2 * 32768;
     uint32_t* upper_rank = root_ws + 3 * 32768;
     uint32_t* totals = root_ws + 4 * 32768;
     hipLaunchKernelGGL(k_conv_root_flags, dim3(blocks_for(32768)), dim3(CONV_THREADS), 0, st, src, tile_flag,
-                       child_flag);
+                       child_flag, bad_flag);
     if (scan_u32(ctx, tile_flag, tile_rank, 32768, false, totals) ||
         scan_u32(ctx, child_flag, upper_rank, 32768, false, totals + 1))
         return -1;
@@ -279,8 +301,10 @@ int node2_to_onindex(Context* ctx, const void* node2, uint64_t node2_size, void*
     uint64_t* uppers = (uint64_t*)ctx->ws(WS_CONV_UPPERS, (U + 1) * sizeof(uint64_t));
     if (!uppers)
         return -1;
-    hipLaunchKernelGGL(k_conv_root_uppers, dim3(blocks_for(32768)), dim3(CONV_THREADS), 0, st, src, child_flag,
-                       upper_rank, uppers);
+    hipLaunchKernelGGL(k_conv_root_uppers, dim3(blocks_for(32768)), dim3(CONV_THREADS), 0, st, src, nwords,
+                       child_flag, upper_rank, uppers, bad_flag);
+    if (!grid_ok())
+        return ctx->fail("node2_to_onindex: root keys or upper offsets outside the grid");
 
     // upper level: U x 32768 slots -> lower ranks and the lower list
     const uint64_t un = U * 32768ull;
@@ -301,6 +325,9 @@ int node2_to_onindex(Context* ctx, const void* node2, uint64_t node2_size, void*
     LU.child_size = ONI_LOWER_SIZE;
     LU.flags = uflags;
     LU.child_rank = uflags + un;
+    LU.nwords = nwords;
+    LU.child_words = N2_WORDS_LOWER;
+    LU.bad = bad_flag;
     uint64_t L = 0;
     if (U)
     {
@@ -317,6 +344,8 @@ int node2_to_onindex(Context* ctx, const void* node2, uint64_t node2_size, void*
     LU.children = lowers;
     if (U)
         hipLaunchKernelGGL(k_conv_children, dim3(blocks_for(un)), dim3(CONV_THREADS), 0, st, LU);
+    if (!grid_ok())
+        return ctx->fail("node2_to_onindex: lower node offsets outside the grid");
 
     // lower level: L x 4096 slots -> leaf ranks and the leaf list
     const uint64_t ln = L * 4096ull;
@@ -337,6 +366,9 @@ int node2_to_onindex(Context* ctx, const void* node2, uint64_t node2_size, void*
     LL.child_size = ONI_LEAF_SIZE;
     LL.flags = lflags;
     LL.child_rank = lflags + ln;
+    LL.nwords = nwords;
+    LL.child_words = N2_WORDS_LEAF;
+    LL.bad = bad_flag;
     uint64_t F = 0;
     if (L)
     {
@@ -353,6 +385,8 @@ int node2_to_onindex(Context* ctx, const void* node2, uint64_t node2_size, void*
     LL.children = leaves;
     if (L)
         hipLaunchKernelGGL(k_conv_children, dim3(blocks_for(ln)), dim3(CONV_THREADS), 0, st, LL);
+    if (!grid_ok())
+        return ctx->fail("node2_to_onindex: leaf offsets outside the grid");
 
     // output layout: header, root + tiles, uppers, lowers, leaves, blind metadata, values
     const uint64_t upper_first = VDB_ROOT_ADDR + ONI_ROOT_SIZE + (uint64_t)tiles * ONI_TILE_SIZE;

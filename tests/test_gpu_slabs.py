@@ -66,3 +66,19 @@ def test_slabbed_density_backward_matches(vox):
     assert st2["slab_count"] > 1
     for a, b in zip(g1, g2):
         assert torch.equal(a, b)
+
+
+def test_slabs_with_alpha_outside_the_key(vox):
+    """Wide domain (clusters at +-500): voxel and Gaussian-id bits leave no room for the alpha in the
+    pair key, so each slab's leaf reduce evaluates the pair values again from the voxel boxes; many
+    slabs must still give the oracle's grid."""
+    c = small_uniform(1200, 0.01, 41)
+    c.means[:600] += np.float32(500.0)
+    c.means[600:] -= np.float32(500.0)
+    got, st = _run(vox, c, 0, 20_000)
+    print(f"wide-domain slabs: key bits {st['key_bits']}, slabs {st['slab_count']}", flush=True)
+    assert st["key_bits"] + 8 > 64 and st["slab_count"] > 1
+    m, op, q, s, col = preprocessed(c)
+    exp, rst = oracle.raster_node2(m, q, s, col, op, c.voxel_size)
+    assert st["pair_count"] == rst["pair_count"]
+    assert np.array_equal(got, exp)
