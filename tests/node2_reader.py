@@ -92,7 +92,8 @@ class Node2Grid:
         root = self.root
         nodes.append(dict(addr=root, type=ROOT, origin=None, node_idx=self.r(root + OFF_NODE_IDX),
                           value_idx=self.r(root + OFF_VALUE_IDX),
-                          value_count=int(_popcount64(self.words(root + OFF_VM[ROOT], 512)).sum())))
+                          value_count=int(_popcount64(self.words(root + OFF_VM[ROOT], 512)).sum()),
+                          parent=None, slot=0))
         cm = self.words(root + OFF_CM[ROOT], 512)
         km = self.words(root + OFF_KM[ROOT], 512)
         child_count = self.r(root + OFF_CMP[ROOT]) & 0xFFFF
@@ -107,15 +108,16 @@ class Node2Grid:
             # the direct slot of the key must be the slot it is stored in (no probing in these grids)
             slot = ((origin[0] & 0x1FFFF) >> 12) << 10 | ((origin[1] & 0x1FFFF) >> 12) << 5 | ((origin[2] & 0x1FFFF) >> 12)
             assert slot == n, "root key stored away from its direct slot"
-            self._walk_node(child, UPPER, origin, nodes, vox)
+            self._walk_node(child, UPPER, origin, nodes, vox, root, int(n))
         voxels = np.array(vox, dtype=np.int64).reshape(-1, 4)
         return nodes, voxels
 
-    def _walk_node(self, node, t, origin, nodes, vox):
+    def _walk_node(self, node, t, origin, nodes, vox, parent, slot):
         nwords = FANOUT[t] // 64
         vm = self.words(node + OFF_VM[t], nwords)
         nodes.append(dict(addr=node, type=t, origin=tuple(origin), node_idx=self.r(node + OFF_NODE_IDX),
-                          value_idx=self.r(node + OFF_VALUE_IDX), value_count=int(_popcount64(vm).sum())))
+                          value_idx=self.r(node + OFF_VALUE_IDX), value_count=int(_popcount64(vm).sum()),
+                          parent=parent, slot=slot))
         fb, tb = FANOUT_BITS[t], TILE_BITS[t]
         fmask = (1 << fb) - 1
         if t == LEAF:
@@ -128,7 +130,7 @@ class Node2Grid:
             child = node + self.r(node + OFF_CHILDREN[t] + rank) // 8
             o = (origin[0] + (((n >> (2 * fb)) & fmask) << tb), origin[1] + (((n >> fb) & fmask) << tb),
                  origin[2] + ((n & fmask) << tb))
-            self._walk_node(child, t + 1, o, nodes, vox)
+            self._walk_node(child, t + 1, o, nodes, vox, node, int(n))
 
     # --- invariants
     def check_prefix(self, node, t, prefix_off, mask_off):
@@ -212,6 +214,67 @@ class Node2Grid:
             frontier = nxt
         assert len(order) == len(by_addr)
         return order
+
+
+# ------------------------------------------------------------------- windowed allocation order
+# The reference builds the tree window by window (Raster.cpp:450-500, GridBuild.cpp:361-506): the
+# kept fragments of window t insert their voxels level by level (root -> upper, upper -> lower,
+# lower -> leaf); the children a level pass finds new are allocated after every node allocated so
+# far, in (parent allocation rank, slot) order (grid_build_scatter_new.slang). A node is therefore
+# created in the first window that holds a kept voxel inside it, and the allocation order is the
+# lexicographic order of (window, level, parent rank, slot); node_idx is the allocation rank and the
+# nodes lie in the node segment in that order.
+NODE_SPAN = {UPPER: 4096, LOWER: 128, LEAF: 8}
+
+
+def fragment_windows(bbox, footprint, gid, ijk, prim_batch_size, ijk_batch_size):
+    """Window of each kept fragment (gid, ijk): fragment slots count the voxels of each prim's AABB
+    (x-major, then y, then z) in prim order within a prim batch, windows split each batch's slots
+    into runs of ijk_batch_size and are numbered across batches (batches without fragments have no
+    window)."""
+    n = footprint.size
+    PB, W = int(prim_batch_size), int(ijk_batch_size)
+    fp = footprint.astype(np.int64)
+    nb = (n + PB - 1) // PB
+    excl = np.zeros(n, np.int64)
+    win_base = np.zeros(nb + 1, np.int64)
+    for b in range(nb):
+        seg = fp[b * PB:(b + 1) * PB]
+        excl[b * PB:(b + 1) * PB] = np.cumsum(seg) - seg
+        win_base[b + 1] = win_base[b] + (int(seg.sum()) + W - 1) // W
+    g = gid.astype(np.int64)
+    mn = bbox[g, :3].astype(np.int64)
+    rng = bbox[g, 3:].astype(np.int64) - mn + 1
+    d = ijk.astype(np.int64) - mn
+    ri = (d[:, 0] * rng[:, 1] + d[:, 1]) * rng[:, 2] + d[:, 2]
+    slot = excl[g] + ri
+    return win_base[g // PB] + slot // W
+
+
+def windowed_allocation_order(nodes, voxel_ijk, voxel_window):
+    """Node addresses in the expected allocation order, from each node's first-touch window (the
+    smallest window among the kept voxels inside it)."""
+    first = {}
+    for t in (UPPER, LOWER, LEAF):
+        span = NODE_SPAN[t]
+        org = (voxel_ijk.astype(np.int64) // span) * span
+        key = [tuple(o) for o in org]
+        for k, w in zip(key, voxel_window.tolist()):
+            if first.get((t, k), w + 1) > w:
+                first[(t, k)] = w
+    root = [d for d in nodes if d["type"] == ROOT][0]
+    rank = {root["addr"]: 0}
+    order = [root["addr"]]
+    by_level = {t: [d for d in nodes if d["type"] == t] for t in (UPPER, LOWER, LEAF)}
+    win = {d["addr"]: first[(d["type"], tuple(int(c) for c in d["origin"]))] for t in by_level for d in by_level[t]}
+    for w in sorted(set(win.values())):
+        for t in (UPPER, LOWER, LEAF):
+            new = [d for d in by_level[t] if win[d["addr"]] == w]
+            new.sort(key=lambda d: (rank[d["parent"]], d["slot"]))
+            for d in new:
+                rank[d["addr"]] = len(order)
+                order.append(d["addr"])
+    return order
 
 
 # ----------------------------------------------------------------------------- NanoVDB OnIndex

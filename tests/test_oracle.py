@@ -7,6 +7,8 @@ structural invariants of the node2 format read back by an independent reader. De
 FP32 predicate are compared only where the float64 evaluation is clear of the boundary by a
 margin far above FP32 rounding.
 """
+import os
+
 import numpy as np
 import pytest
 from scipy.spatial.transform import Rotation
@@ -381,3 +383,36 @@ def test_onindex_round_trip(seed, n, vs):
     want = 736 + 96 + 32 * tiles + 270400 * st["upper_count"] + 33856 * st["lower_count"] + 96 * st["leaf_count"] \
         + 288 + 4 * count
     assert oni.size == want
+
+
+@pytest.mark.parametrize("case", ["golden_clustered_600_windowed", "uniform_small_batches", "clustered_windows"])
+def test_windowed_node_order_restated(case):
+    """The allocation order of every node (window of first touch, level, parent rank, slot), restated
+    from the fragment window model in node2_reader, equals the node order of the oracle's windowed
+    build: the node segment in address order and node_idx."""
+    from node2_reader import fragment_windows, windowed_allocation_order
+    if case.startswith("golden"):
+        d = np.load(os.path.join(os.path.dirname(__file__), "golden", "clustered_600_rgba8_windowed.npz"))
+        m, op, q, s, col = oracle.preprocess(d["means"], d["opacity_logits"], d["quats"], d["log_scales"], d["sh0"])
+        vs, PB, W = float(d["voxel_size"]), int(d["prim_batch_size"]), int(d["ijk_batch_size"])
+        grid = d["grid"]
+        assert np.array_equal(grid, oracle.raster_node2(m, q, s, col, op, vs, prim_batch_size=PB,
+                                                        ijk_batch_size=W)[0])
+    else:
+        cloud = small_uniform(500, 0.01, 23) if case == "uniform_small_batches" else \
+            syn.clustered_cloud(900, 5, 0.2, 0.005, -5.0, 0.7, seed=25)
+        m, op, q, s, col = preprocessed(cloud)
+        vs = float(np.float32(cloud.voxel_size))
+        PB, W = (61, 30_000) if case == "uniform_small_batches" else (262144, 150_000)
+        grid, _ = oracle.raster_node2(m, q, s, col, op, vs, prim_batch_size=PB, ijk_batch_size=W)
+    bbox, foot = oracle.prim_aabb(m, q, s, op, vs)
+    gid, ijk, _ = oracle.pairs(m, q, s, col, op, vs)
+    win = fragment_windows(bbox, foot, gid, ijk, PB, W)
+    g = Node2Grid(grid)
+    nodes, _ = g.check_invariants()
+    order = windowed_allocation_order(nodes, ijk, win)
+    assert len(np.unique(win)) > 1
+    assert order == sorted(d["addr"] for d in nodes)          # the node segment in allocation order
+    idx = {d["addr"]: d["node_idx"] for d in nodes}
+    assert [idx[a] for a in order] == list(range(len(order)))
+    assert order != g.bfs_node_order(nodes)                     # the windows change the order
