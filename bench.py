@@ -118,6 +118,7 @@ def fwd_bwd(cloud, raw, vox, steps):
         t_fb += t2 - t1
     return {"value_mode": "density", "forward_ms": 1e3 * t_f / steps, "fwd_bwd_ms": 1e3 * t_fb / steps,
             "voxels": voxels, "steps": steps,
+            "backward_path": "sorted pair list" if vox.last_backward_used_pairs else "pair re-enumeration",
             "note": "float density grid forward (exact fixed-point voxel sums) and its adjoint w.r.t. means, "
                     "log-scales, quaternions, opacity logits via torch.autograd; loss = sum(w * values)"}
 

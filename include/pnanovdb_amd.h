@@ -496,6 +496,8 @@ typedef struct vdbr_raster_stats_t
     float ms_reduce_kernel;    /* time of the per-voxel reduction kernel (HIP events) */
     uint32_t sort_bit_begin;   /* the pair sort orders key bits [sort_bit_begin, key_bits) */
     uint32_t slab_count;       /* voxel-x slabs the pairs were emitted, sorted and reduced in */
+    uint64_t pair_list_id;     /* density mode, one slab: id of the pair list the context keeps for
+                                  vdbr_density_backward_pairs; 0 otherwise */
 } vdbr_raster_stats_t;
 
 /* Full raster3d pipeline on device inputs. The node2 grid is written to a context-owned
@@ -538,6 +540,20 @@ PNANOVDB_API int vdbr_raster_node2_batch(vdbr_context_t* ctx, vdbr_batch_item_t*
 PNANOVDB_API int vdbr_density_backward(vdbr_context_t* ctx, const vdbr_gaussians_t* raw, float voxel_size,
                                        const void* grid_dev, uint64_t grid_size, const float* grad_values_dev,
                                        uint64_t value_count, const vdbr_gaussian_grads_t* grads);
+
+/* vdbr_density_backward from the sorted pair list that the density forward kept on this context
+ * (no counterpart in the reference): pair_list_id is the forward's stats.pair_list_id. The pair
+ * terms are evaluated in double precision, rounded to 48-bit fixed point under a power-of-two
+ * scale taken from max |grad_values| and summed as integers (bitwise reproducible, independent of
+ * the pair order); the gradients agree with vdbr_density_backward to double rounding. Returns 0
+ * when the pair list was used and 1 when it was gone (another raster call or sort autotune on the
+ * context since the forward, a forward of more than one slab, another Gaussian count or voxel
+ * size) or grad_values holds a non-finite value: then vdbr_density_backward ran instead. The
+ * grid must be the forward's. */
+PNANOVDB_API int vdbr_density_backward_pairs(vdbr_context_t* ctx, const vdbr_gaussians_t* raw, float voxel_size,
+                                             const void* grid_dev, uint64_t grid_size, const float* grad_values_dev,
+                                             uint64_t value_count, const vdbr_gaussian_grads_t* grads,
+                                             uint64_t pair_list_id);
 
 /* The surviving pair list of the last vdbr_raster_node2 call (its last slab), in the order the
  * value pass consumed it: grouped by leaf (leaves in key order, pairs inside a leaf unordered);

@@ -120,8 +120,7 @@ void Context::release()
         ws_ptr[i] = nullptr;
         ws_size[i] = 0;
     }
-    last_pairs = nullptr;
-    last_pair_count = 0;
+    drop_pairs();
 }
 
 static Context* create_context(int device)
@@ -1543,6 +1542,21 @@ PNANOVDB_API int vdbr_density_backward(vdbr_context_t* ctx, const vdbr_gaussians
     if (!c)
         return -1;
     return density_backward(c, raw, voxel_size, grid_dev, grid_size, grad_values_dev, value_count, grads, nullptr);
+}
+
+PNANOVDB_API int vdbr_density_backward_pairs(vdbr_context_t* ctx, const vdbr_gaussians_t* raw, float voxel_size,
+                                             const void* grid_dev, uint64_t grid_size, const float* grad_values_dev,
+                                             uint64_t value_count, const vdbr_gaussian_grads_t* grads,
+                                             uint64_t pair_list_id)
+{
+    Context* c = (Context*)ctx;
+    if (!c)
+        return -1;
+    int used = 0;
+    if (density_backward_pairs(c, raw, voxel_size, grid_dev, grid_size, grad_values_dev, value_count, grads,
+                               pair_list_id, nullptr, &used))
+        return -1;
+    return used ? 0 : 1;
 }
 
 PNANOVDB_API int vdbr_selftest_alpha8(vdbr_context_t* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches,

@@ -140,6 +140,12 @@ struct Context
     uint32_t last_alpha_bits = 0;
     int32_t last_lmin[3] = { 0, 0, 0 };
     uint32_t last_bits[3] = { 0, 0, 0 };
+    // the retained pair list of the last density forward (single slab): its id (0: none; ids are
+    // never reused on a context), Gaussian count and voxel size, checked by the pair-list adjoint
+    uint64_t pair_gen = 0;
+    uint64_t last_pair_id = 0;
+    uint64_t last_pair_n = 0;
+    float last_pair_vs = 0.f;
     int32_t* last_voxel_ijk = nullptr; // value-index-ordered voxel coordinates (VDBR_FLAG_VOXEL_COORDS)
     uint64_t last_value_count = 0;
 
@@ -147,6 +153,13 @@ struct Context
     std::vector<Context*> batch_workers;
     std::vector<std::pair<void*, size_t>> batch_chunks;
 
+    // forget the retained pair list (its buffers are about to be reused)
+    void drop_pairs()
+    {
+        last_pairs = nullptr;
+        last_pair_count = 0;
+        last_pair_id = 0;
+    }
     int fail(const std::string& msg)
     {
         error = msg;
@@ -249,6 +262,13 @@ int radix_sort_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t n, uin
 int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
                      const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,
                      float* ms_out);
+// the same gradients from the sorted pair list that the density forward with id pair_list_id
+// retained on this context; when that list is gone (another raster call, a multi-slab build, a
+// different Gaussian count or voxel size) or grad_values holds a non-finite value, runs
+// density_backward instead. *used_pairs tells which path ran.
+int density_backward_pairs(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
+                           const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,
+                           uint64_t pair_list_id, float* ms_out, int* used_pairs);
 
 // vdb_raster.hip: fast RGBA8 alpha decision against the correctly rounded path on hashed inputs
 int selftest_alpha8(Context* ctx, uint64_t count, uint64_t seed, uint64_t* mismatches, uint64_t* decided);

@@ -852,6 +852,7 @@ int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_
 {
     if (n < 2 || key_bits == 0 || key_bits > 64)
         return ctx->fail("autotune: need at least 2 keys and 1..64 key bits");
+    ctx->drop_pairs(); // the timed sorts reuse the pair sort's buffers
     uint64_t* pristine = (uint64_t*)ctx->ws(WS_TUNE_KEYS, 2 * n * sizeof(uint64_t));
     if (!pristine)
         return -1;

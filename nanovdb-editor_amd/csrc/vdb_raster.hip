@@ -1518,45 +1518,10 @@ __global__ __launch_bounds__(EMIT_THREADS, 4) void k_density_bwd(BwdArgs a)
     }
 }
 
-// per Gaussian: sum its partials in chunk order and apply the chain rule of the preprocessing
-__global__ __launch_bounds__(256) void k_density_bwd_finalize(BwdArgs a)
+// gradients of Gaussian g from its 10 sums A (see above) through the chain rule of the
+// preprocessing (RasterUtils.cpp:59-117, gaussian_prim.slang:67-68 scale floor)
+__device__ __forceinline__ void bwd_write_grads(const BwdArgs& a, uint64_t g, const double* A)
 {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.n)
-        return;
-    double A[BWD_ACC];
-#pragma unroll
-    for (int q = 0; q < BWD_ACC; q++)
-        A[q] = 0.0;
-    const uint64_t r0 = a.row_scan[g], r1 = a.row_scan[g + 1];
-    if (r1 > r0)
-    {
-        const uint64_t fc = r0 >> 6, lc = (r1 - 1) >> 6;
-        const bool direct = fc == lc && a.rec_gid[2 * fc] != g && a.rec_gid[2 * fc + 1] != g;
-        if (direct)
-        {
-#pragma unroll
-            for (int q = 0; q < BWD_ACC; q++)
-                A[q] = a.acc[g * BWD_ACC + q];
-        }
-        else
-        {
-            for (uint64_t c = fc; c <= lc; c++)
-            {
-                const double* src = nullptr;
-                if (a.rec_gid[2 * c] == g)
-                    src = a.rec + (2 * c) * BWD_ACC;
-                else if (a.rec_gid[2 * c + 1] == g)
-                    src = a.rec + (2 * c + 1) * BWD_ACC;
-                if (src)
-                {
-#pragma unroll
-                    for (int q = 0; q < BWD_ACC; q++)
-                        A[q] += src[q];
-                }
-            }
-        }
-    }
     PrimD p;
     load_prim_d(a, g, &p);
     // sums of v = diag(1/s) R^T d: P_a = A[1 + a], S (symmetric) from A[4..9]
@@ -1605,6 +1570,270 @@ __global__ __launch_bounds__(256) void k_density_bwd_finalize(BwdArgs a)
     const double dot = gq[0] * w + gq[1] * x + gq[2] * y + gq[3] * z;
     for (int k = 0; k < 4; k++)
         a.grad_quats[4 * g + k] = (float)((gq[k] - dot * p.q[k]) / p.qn);
+}
+
+// per Gaussian: sum its partials in chunk order and apply the chain rule of the preprocessing
+__global__ __launch_bounds__(256) void k_density_bwd_finalize(BwdArgs a)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n)
+        return;
+    double A[BWD_ACC];
+#pragma unroll
+    for (int q = 0; q < BWD_ACC; q++)
+        A[q] = 0.0;
+    const uint64_t r0 = a.row_scan[g], r1 = a.row_scan[g + 1];
+    if (r1 > r0)
+    {
+        const uint64_t fc = r0 >> 6, lc = (r1 - 1) >> 6;
+        const bool direct = fc == lc && a.rec_gid[2 * fc] != g && a.rec_gid[2 * fc + 1] != g;
+        if (direct)
+        {
+#pragma unroll
+            for (int q = 0; q < BWD_ACC; q++)
+                A[q] = a.acc[g * BWD_ACC + q];
+        }
+        else
+        {
+            for (uint64_t c = fc; c <= lc; c++)
+            {
+                const double* src = nullptr;
+                if (a.rec_gid[2 * c] == g)
+                    src = a.rec + (2 * c) * BWD_ACC;
+                else if (a.rec_gid[2 * c + 1] == g)
+                    src = a.rec + (2 * c + 1) * BWD_ACC;
+                if (src)
+                {
+#pragma unroll
+                    for (int q = 0; q < BWD_ACC; q++)
+                        A[q] += src[q];
+                }
+            }
+        }
+    }
+    bwd_write_grads(a, g, A);
+}
+
+// ============================================================================ density backward from the pair list
+// The same 10 sums per Gaussian, taken over the sorted pair list that the density forward kept
+// (grouped by leaf; inside a leaf in emission order, where neighbouring pairs mostly belong to
+// one Gaussian). Every term is bounded by |dL/drho|: e <= 1, |v_a| e <= sqrt(m) e^(-m/2) <= e^-1/2
+// and |v_a v_b| e <= m e^(-m/2) <= 2/e. With 2^scale = 2^BP_TERM_BITS / 2^ceil(log2 max|dL/drho|)
+// every scaled term rounds to nearest to an integer of magnitude at most 2^48, and the sums are
+// integer sums: exact, so their value does not depend on the order in which pairs, lanes or
+// blocks add them. Each lane takes BP_PER_LANE consecutive pairs and sums the run of one Gaussian
+// in registers; a finished run goes to the block's LDS table (BP_SLOTS Gaussians, 64-bit sums: a
+// tile holds at most 2^12 terms of magnitude 2^48), or straight to the global sums when its probe
+// finds no free slot; at the end of the tile the table goes to the global 128-bit sums.
+constexpr int BP_THREADS = 256;
+constexpr int BP_PER_LANE = 16;
+constexpr uint64_t BP_TILE = (uint64_t)BP_THREADS * BP_PER_LANE;
+constexpr uint32_t BP_SLOTS = 256;
+constexpr uint32_t BP_PROBES = 32;
+constexpr uint32_t BP_EMPTY = 0xFFFFFFFFu;
+constexpr int BP_TERM_BITS = 48;
+
+struct PairBwdArgs
+{
+    const uint64_t* keys; // sorted pair keys: voxel key << gid_bits | Gaussian id
+    uint64_t npairs;
+    KeyLayout kl;
+    double vs;
+    const BwdPrim* prims;
+    const uint4* lt_slots;
+    uint32_t lt_mask;
+    const float* leaf_grad;
+    const unsigned int* rmax_bits; // bits of max |dL/drho| (float)
+    unsigned long long* sums;      // [n][BWD_ACC] x (low word, high word) of 128-bit sums, zeroed first
+};
+
+// 2^s with max|dL/drho| * 2^s < 2^BP_TERM_BITS (s = 0 when every dL/drho is 0)
+__device__ __forceinline__ int bp_scale_exp(unsigned int rmax_bits)
+{
+    const double r = (double)__uint_as_float(rmax_bits);
+    int ex = 0;
+    if (r > 0.0)
+        (void)frexp(r, &ex); // r < 2^ex
+    return r > 0.0 ? BP_TERM_BITS - ex : 0;
+}
+
+// x rounded to the nearest integer (ties to even), |x| < 2^51: x + 1.5 * 2^52 has an ulp of 1 and
+// keeps the integer in its low mantissa bits
+__device__ __forceinline__ int64_t bp_round(double x)
+{
+    const double big = 0x1.8p52;
+    return (int64_t)((uint64_t)__double_as_longlong(x + big) - (uint64_t)__double_as_longlong(big));
+}
+
+// v added to the 128-bit two's complement sum (w[0] low word, w[1] high word). The carry out of
+// the low word is taken against the value this atomic replaced, so concurrent adds compose exactly.
+__device__ __forceinline__ void atomic_add_i128(unsigned long long* w, int64_t v)
+{
+    const unsigned long long lo = (unsigned long long)v;
+    const unsigned long long old = atomicAdd(&w[0], lo);
+    const unsigned long long hi = (v < 0 ? ~0ull : 0ull) + (old + lo < old ? 1ull : 0ull);
+    if (hi)
+        atomicAdd(&w[1], hi);
+}
+
+__device__ __forceinline__ void bp_flush(const PairBwdArgs& a, uint32_t* s_gid,
+                                         unsigned long long (*s_acc)[BP_SLOTS], uint32_t g, const int64_t* A)
+{
+    uint32_t h = (g * 0x9E3779B1u) >> 24;
+    for (uint32_t p = 0; p < BP_PROBES; p++)
+    {
+        const uint32_t old = atomicCAS(&s_gid[h], BP_EMPTY, g);
+        if (old == BP_EMPTY || old == g)
+        {
+#pragma unroll
+            for (int t = 0; t < BWD_ACC; t++)
+                if (A[t] != 0)
+                    atomicAdd(&s_acc[t][h], (unsigned long long)A[t]);
+            return;
+        }
+        h = (h + 1u) & (BP_SLOTS - 1u);
+    }
+#pragma unroll
+    for (int t = 0; t < BWD_ACC; t++)
+        if (A[t] != 0)
+            atomic_add_i128(a.sums + ((uint64_t)g * BWD_ACC + t) * 2u, A[t]);
+}
+
+__global__ __launch_bounds__(BP_THREADS) void k_density_bwd_pairs(PairBwdArgs a)
+{
+    __shared__ uint32_t s_gid[BP_SLOTS];
+    __shared__ unsigned long long s_acc[BWD_ACC][BP_SLOTS];
+    __shared__ double s_exp2[64];
+    for (uint32_t e = threadIdx.x; e < BP_SLOTS; e += BP_THREADS)
+    {
+        s_gid[e] = BP_EMPTY;
+#pragma unroll
+        for (int t = 0; t < BWD_ACC; t++)
+            s_acc[t][e] = 0ull;
+    }
+    if (threadIdx.x < 64u)
+        s_exp2[threadIdx.x] = EXP2_J64[threadIdx.x];
+    __syncthreads();
+    const double scale = ldexp(1.0, bp_scale_exp(*a.rmax_bits));
+    const uint32_t gb = a.kl.gid_bits;
+    const uint64_t gmask = (1ull << gb) - 1ull;
+    const double vs = a.vs;
+    const uint64_t q0 = (uint64_t)blockIdx.x * BP_TILE + (uint64_t)threadIdx.x * BP_PER_LANE;
+    const uint64_t q1 = q0 + BP_PER_LANE < a.npairs ? q0 + BP_PER_LANE : a.npairs;
+    int64_t A[BWD_ACC];
+#pragma unroll
+    for (int t = 0; t < BWD_ACC; t++)
+        A[t] = 0;
+    uint32_t cur = BP_EMPTY;
+    BwdPrim p;
+    uint64_t leaf_cur = ~0ull;
+    uint32_t lord = LT_NONE;
+    int32_t li = 0, lj = 0, lk = 0;
+#pragma unroll 1
+    for (uint64_t q = q0; q < q1; q++)
+    {
+        const uint64_t key = a.keys[q];
+        const uint32_t g = (uint32_t)(key & gmask);
+        const uint64_t vox = key >> gb;
+        if (g != cur)
+        {
+            if (cur != BP_EMPTY)
+            {
+                bp_flush(a, s_gid, s_acc, cur, A);
+#pragma unroll
+                for (int t = 0; t < BWD_ACC; t++)
+                    A[t] = 0;
+            }
+            cur = g;
+            p = a.prims[g];
+        }
+        const uint64_t leaf = vox >> 9;
+        if (leaf != leaf_cur)
+        {
+            int32_t i0, j0, k0;
+            decode_voxel(a.kl, leaf << 9, &i0, &j0, &k0);
+            li = i0 >> 3;
+            lj = j0 >> 3;
+            lk = k0 >> 3;
+            lord = lt_find(a.lt_slots, a.lt_mask, li, lj, lk);
+            leaf_cur = leaf;
+        }
+        const uint32_t nv = (uint32_t)vox & 511u;
+        // dL/drho pre-multiplied by the power-of-two scale (exact)
+        const double r = lord != LT_NONE ? (double)a.leaf_grad[(uint64_t)lord * 512u + nv] * scale : 0.0;
+        const double d[3] = { ((double)(li * 8 + (int32_t)(nv >> 6)) + 0.5) * vs - p.mu[0],
+                              ((double)(lj * 8 + (int32_t)((nv >> 3) & 7u)) + 0.5) * vs - p.mu[1],
+                              ((double)(lk * 8 + (int32_t)(nv & 7u)) + 0.5) * vs - p.mu[2] };
+        double v[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            v[c] = p.M[3 * c + 0] * d[0] + p.M[3 * c + 1] * d[1] + p.M[3 * c + 2] * d[2];
+        const double m = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        const double e = exp_neg(-0.5 * m, s_exp2);
+        const double W = r * p.o * e;
+        const double w0 = W * v[0], w1 = W * v[1], w2 = W * v[2];
+        A[0] += bp_round(r * e);
+        A[1] += bp_round(w0);
+        A[2] += bp_round(w1);
+        A[3] += bp_round(w2);
+        A[4] += bp_round(w0 * v[0]);
+        A[5] += bp_round(w1 * v[1]);
+        A[6] += bp_round(w2 * v[2]);
+        A[7] += bp_round(w0 * v[1]);
+        A[8] += bp_round(w0 * v[2]);
+        A[9] += bp_round(w1 * v[2]);
+    }
+    if (cur != BP_EMPTY)
+        bp_flush(a, s_gid, s_acc, cur, A);
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < BP_SLOTS * BWD_ACC; e += BP_THREADS)
+    {
+        const uint32_t slot = e & (BP_SLOTS - 1u), t = e / BP_SLOTS;
+        const uint32_t g = s_gid[slot];
+        const int64_t v = (int64_t)s_acc[t][slot];
+        if (g != BP_EMPTY && v != 0)
+            atomic_add_i128(a.sums + ((uint64_t)g * BWD_ACC + t) * 2u, v);
+    }
+}
+
+// per Gaussian: the 128-bit sums back to doubles, then the chain rule
+__global__ __launch_bounds__(256) void k_density_bwd_pairs_finalize(BwdArgs a, const unsigned long long* __restrict__ sums,
+                                                                     const unsigned int* __restrict__ rmax_bits)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n)
+        return;
+    const int s = bp_scale_exp(*rmax_bits);
+    double A[BWD_ACC];
+#pragma unroll
+    for (int t = 0; t < BWD_ACC; t++)
+    {
+        const uint64_t lo = sums[(g * BWD_ACC + t) * 2u];
+        const uint64_t hi = sums[(g * BWD_ACC + t) * 2u + 1u];
+        // |sum| < 2^80 (at most 2^32 terms of magnitude 2^48): sum >> 40 has at most 41 bits, and
+        // the double of top * 2^40 + low rounds once
+        const int64_t top = (int64_t)((hi << 24) + (lo >> 40));
+        const double d = (double)top * 0x1p40 + (double)(lo & ((1ull << 40) - 1ull));
+        A[t] = ldexp(d, -s);
+    }
+    bwd_write_grads(a, g, A);
+}
+
+// max over x[0..n) of |x| as float bits (a NaN compares above infinity)
+__global__ __launch_bounds__(256) void k_abs_max_bits(const float* __restrict__ x, uint64_t n, unsigned int* out)
+{
+    __shared__ uint32_t s_max[4];
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u)
+        m = max(m, __float_as_uint(x[i]) & 0x7FFFFFFFu);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        m = max(m, (uint32_t)__shfl_xor((int)m, off, 64));
+    if ((threadIdx.x & 63u) == 0u)
+        s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(out, max(max(s_max[0], s_max[1]), max(s_max[2], s_max[3])));
 }
 
 // ============================================================================ alpha self-test
@@ -3207,6 +3436,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         return ctx->fail("raster_node2: unknown value_mode");
     ctx->last_voxel_ijk = nullptr;
     ctx->last_value_count = 0;
+    ctx->drop_pairs();
     const uint32_t PB = prm->prim_batch_size ? prm->prim_batch_size : 262144u;
     const uint32_t W = prm->ijk_batch_size ? prm->ijk_batch_size : 8388608u;
     const uint32_t max_nodes = prm->max_node_count ? prm->max_node_count : 2097152u;
@@ -3759,6 +3989,13 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     // the pair list of a single-slab build stays available (vdbr_last_pairs)
     ctx->last_pairs = slab_count == 1 ? last_sorted : nullptr;
     ctx->last_pair_count = slab_count == 1 ? last_slab_pairs : 0;
+    if (slab_count == 1 && density)
+    {
+        // the density adjoint may reuse this list (vdbr_density_backward_pairs)
+        ctx->last_pair_id = ++ctx->pair_gen;
+        ctx->last_pair_n = n;
+        ctx->last_pair_vs = vs;
+    }
     ctx->last_gid_bits = kl.gid_bits;
     ctx->last_alpha_bits = kl.alpha_bits;
     for (int a = 0; a < 3; a++)
@@ -3807,6 +4044,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         stats->sort_bit_begin = sort_begin;
         stats->emit_attempts = attempts;
         stats->slab_count = slab_count;
+        stats->pair_list_id = ctx->last_pair_id;
         float ms;
         (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
         stats->ms_setup = ms;
@@ -3927,10 +4165,63 @@ int node2_sample(Context* ctx, const void* grid, uint64_t grid_size, const uint3
     return 0;
 }
 
-int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
-                     const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,
-                     float* ms_out)
+// leaf table of a node2 grid with dL/drho per voxel slot; the node counts of the tree header bound
+// the node lists (checked against the grid size here, against the nodes found by the caller
+// through lt->counters[3] after the stream has run)
+static int bwd_leaf_table(Context* ctx, const void* grid, uint64_t grid_size, const float* grad_values,
+                          uint64_t value_count, LeafTable* lt)
 {
+    hipStream_t st = ctx->stream;
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, (const uint8_t*)grid + VDB_TREE_ADDR + VDB_TREE_OFF_COUNT_LEAF, 12,
+                                   hipMemcpyDeviceToHost, st));
+    VDBR_CHECK(ctx, hipStreamSynchronize(st));
+    const uint32_t* tree_counts = (const uint32_t*)ctx->pinned;
+    const uint64_t n_leaf = tree_counts[0], n_lower = tree_counts[1], n_upper = tree_counts[2];
+    if (n_upper > 32768u || n_lower > n_upper * 32768u || n_leaf > n_lower * 4096u ||
+        n_upper * VDB_SIZE_UPPER + n_lower * VDB_SIZE_LOWER + n_leaf * VDB_SIZE_LEAF > grid_size)
+        return ctx->fail("density_backward: grid tree node counts are inconsistent with its size");
+    uint64_t lt_cap = 2;
+    while (lt_cap < 2 * n_leaf)
+        lt_cap <<= 1;
+    if (lt_cap > (1ull << 31))
+        return ctx->fail("density_backward: more than 2^30 leaves");
+    uint8_t* lt_ws = (uint8_t*)ctx->ws(WS_BWD_LEAVES, 64 + (n_upper + n_lower) * sizeof(NodeRef) +
+                                                          n_leaf * sizeof(uint64_t) + lt_cap * sizeof(uint4));
+    float* leaf_grad = (float*)ctx->ws(WS_BWD_LEAFGRAD, (n_leaf * 512 + 1) * sizeof(float));
+    if (!lt_ws || !leaf_grad)
+        return -1;
+    LeafTable& t = *lt;
+    t.grid = (const uint64_t*)grid;
+    t.words = grid_size >> 3;
+    t.counters = (unsigned int*)lt_ws;
+    t.slots = (uint4*)(lt_ws + 64);
+    t.uppers = (NodeRef*)(lt_ws + 64 + lt_cap * sizeof(uint4));
+    t.lowers = t.uppers + n_upper;
+    t.leaves = (uint64_t*)(t.lowers + n_lower);
+    t.mask = (uint32_t)(lt_cap - 1);
+    t.cap[0] = (uint32_t)n_upper;
+    t.cap[1] = (uint32_t)n_lower;
+    t.cap[2] = (uint32_t)n_leaf;
+    t.grad_values = grad_values;
+    t.value_count = value_count;
+    t.leaf_grad = leaf_grad;
+    VDBR_CHECK(ctx, hipMemsetAsync(lt_ws, 0, 64 + lt_cap * sizeof(uint4), st));
+    hipLaunchKernelGGL(k_lt_root, dim3(32768 / 256), dim3(256), 0, st, t);
+    if (n_upper)
+        hipLaunchKernelGGL(k_lt_upper, dim3(blocks_for(n_upper << 15)), dim3(256), 0, st, t);
+    if (n_lower)
+        hipLaunchKernelGGL(k_lt_lower, dim3(blocks_for(n_lower << 12)), dim3(256), 0, st, t);
+    if (n_leaf)
+        hipLaunchKernelGGL(k_lt_grad, dim3(blocks_for(n_leaf << 9)), dim3(256), 0, st, t);
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+// argument checks shared by both adjoints; *empty when there is nothing to compute
+static int bwd_check_args(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
+                          const float* grad_values, const vdbr_gaussian_grads_t* grads, bool* empty)
+{
+    *empty = false;
     if (!raw || !grads || !grid)
         return ctx->fail("density_backward: null argument");
     const uint64_t n = raw->count;
@@ -3941,10 +4232,26 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
     if (grid_size < VDB_ROOT_ADDR + VDB_SIZE_ROOT)
         return ctx->fail("density_backward: grid is not a node2 grid");
     if (n == 0)
+    {
+        *empty = true;
         return 0;
+    }
     if (!raw->means || !raw->opacities || !raw->quats || !raw->scales || !grad_values || !grads->means ||
         !grads->opacities || !grads->quats || !grads->scales)
         return ctx->fail("density_backward: null array");
+    return 0;
+}
+
+int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
+                     const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,
+                     float* ms_out)
+{
+    bool empty = false;
+    if (bwd_check_args(ctx, raw, vs, grid, grid_size, grad_values, grads, &empty))
+        return -1;
+    if (empty)
+        return 0;
+    const uint64_t n = raw->count;
     hipStream_t st = ctx->stream;
     hipEvent_t e0, e1;
     VDBR_CHECK(ctx, hipEventCreate(&e0));
@@ -3989,52 +4296,15 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
                        1.f / vs, 0.f, so);
     if (scan_u64(ctx, rows, row_scan, n, false, row_scan + n))
         return -1;
-    // leaf table of the grid: node counts from the tree header bound the node lists
-    const uint64_t grid_words = grid_size >> 3;
     VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, row_scan + n, 8, hipMemcpyDeviceToHost, st));
-    VDBR_CHECK(ctx, hipMemcpyAsync((uint8_t*)ctx->pinned + 8, (const uint8_t*)grid + VDB_TREE_ADDR +
-                                   VDB_TREE_OFF_COUNT_LEAF, 12, hipMemcpyDeviceToHost, st));
     VDBR_CHECK(ctx, hipStreamSynchronize(st));
     const uint64_t total_rows = ctx->pinned[0];
-    const uint32_t* tree_counts = (const uint32_t*)((const uint8_t*)ctx->pinned + 8);
-    const uint64_t n_leaf = tree_counts[0], n_lower = tree_counts[1], n_upper = tree_counts[2];
-    if (n_upper > 32768u || n_lower > n_upper * 32768u || n_leaf > n_lower * 4096u ||
-        n_upper * VDB_SIZE_UPPER + n_lower * VDB_SIZE_LOWER + n_leaf * VDB_SIZE_LEAF > grid_size)
-        return ctx->fail("density_backward: grid tree node counts are inconsistent with its size");
-    uint64_t lt_cap = 2;
-    while (lt_cap < 2 * n_leaf)
-        lt_cap <<= 1;
-    if (lt_cap > (1ull << 31))
-        return ctx->fail("density_backward: more than 2^30 leaves");
-    uint8_t* lt_ws = (uint8_t*)ctx->ws(WS_BWD_LEAVES, 64 + (n_upper + n_lower) * sizeof(NodeRef) +
-                                                          n_leaf * sizeof(uint64_t) + lt_cap * sizeof(uint4));
-    float* leaf_grad = (float*)ctx->ws(WS_BWD_LEAFGRAD, (n_leaf * 512 + 1) * sizeof(float));
-    BwdPrim* prims = (BwdPrim*)ctx->ws(WS_BWD_PRIMS, n * sizeof(BwdPrim));
-    if (!lt_ws || !leaf_grad || !prims)
-        return -1;
     LeafTable lt;
-    lt.grid = (const uint64_t*)grid;
-    lt.words = grid_words;
-    lt.counters = (unsigned int*)lt_ws;
-    lt.slots = (uint4*)(lt_ws + 64);
-    lt.uppers = (NodeRef*)(lt_ws + 64 + lt_cap * sizeof(uint4));
-    lt.lowers = lt.uppers + n_upper;
-    lt.leaves = (uint64_t*)(lt.lowers + n_lower);
-    lt.mask = (uint32_t)(lt_cap - 1);
-    lt.cap[0] = (uint32_t)n_upper;
-    lt.cap[1] = (uint32_t)n_lower;
-    lt.cap[2] = (uint32_t)n_leaf;
-    lt.grad_values = grad_values;
-    lt.value_count = value_count;
-    lt.leaf_grad = leaf_grad;
-    VDBR_CHECK(ctx, hipMemsetAsync(lt_ws, 0, 64 + lt_cap * sizeof(uint4), st));
-    hipLaunchKernelGGL(k_lt_root, dim3(32768 / 256), dim3(256), 0, st, lt);
-    if (n_upper)
-        hipLaunchKernelGGL(k_lt_upper, dim3(blocks_for(n_upper << 15)), dim3(256), 0, st, lt);
-    if (n_lower)
-        hipLaunchKernelGGL(k_lt_lower, dim3(blocks_for(n_lower << 12)), dim3(256), 0, st, lt);
-    if (n_leaf)
-        hipLaunchKernelGGL(k_lt_grad, dim3(blocks_for(n_leaf << 9)), dim3(256), 0, st, lt);
+    if (bwd_leaf_table(ctx, grid, grid_size, grad_values, value_count, &lt))
+        return -1;
+    BwdPrim* prims = (BwdPrim*)ctx->ws(WS_BWD_PRIMS, n * sizeof(BwdPrim));
+    if (!prims)
+        return -1;
     const uint64_t chunks = (total_rows + 63) / 64;
     double* acc = (double*)ctx->ws(WS_BWD_ACC, n * BWD_ACC * sizeof(double));
     double* rec = (double*)ctx->ws(WS_BWD_REC, (chunks + 1) * 2 * BWD_ACC * sizeof(double));
@@ -4056,7 +4326,7 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
     a.prims = prims;
     a.lt_slots = lt.slots;
     a.lt_mask = lt.mask;
-    a.leaf_grad = leaf_grad;
+    a.leaf_grad = lt.leaf_grad;
     a.acc = acc;
     a.rec = rec;
     a.rec_gid = rec_gid;
@@ -4081,6 +4351,112 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
         return ctx->fail("density_backward: the grid's nodes do not match its tree header");
     if (ms_out)
         VDBR_CHECK(ctx, hipEventElapsedTime(ms_out, e0, e1));
+    return 0;
+}
+
+int density_backward_pairs(Context* ctx, const vdbr_gaussians_t* raw, float vs, const void* grid, uint64_t grid_size,
+                           const float* grad_values, uint64_t value_count, const vdbr_gaussian_grads_t* grads,
+                           uint64_t pair_list_id, float* ms_out, int* used_pairs)
+{
+    *used_pairs = 0;
+    const bool have = pair_list_id != 0 && pair_list_id == ctx->last_pair_id && raw && raw->count == ctx->last_pair_n &&
+                      vs == ctx->last_pair_vs && ctx->last_alpha_bits == 0u;
+    if (!have)
+        return density_backward(ctx, raw, vs, grid, grid_size, grad_values, value_count, grads, ms_out);
+    bool empty = false;
+    if (bwd_check_args(ctx, raw, vs, grid, grid_size, grad_values, grads, &empty))
+        return -1;
+    if (empty)
+    {
+        *used_pairs = 1;
+        return 0;
+    }
+    const uint64_t n = raw->count;
+    hipStream_t st = ctx->stream;
+    hipEvent_t e0, e1;
+    VDBR_CHECK(ctx, hipEventCreate(&e0));
+    VDBR_CHECK(ctx, hipEventCreate(&e1));
+    struct EvGuard
+    {
+        hipEvent_t a, b;
+        ~EvGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } evg{ e0, e1 };
+    VDBR_CHECK(ctx, hipEventRecord(e0, st));
+    // max |dL/drho| sets the fixed-point scale; a non-finite gradient takes the re-walk, which
+    // carries it into the results
+    uint8_t* misc = (uint8_t*)ctx->ws(WS_MISC, 4096);
+    if (!misc)
+        return -1;
+    unsigned int* rmax = (unsigned int*)(misc + 256);
+    VDBR_CHECK(ctx, hipMemsetAsync(rmax, 0, 4, st));
+    if (value_count)
+        hipLaunchKernelGGL(k_abs_max_bits, dim3(std::min<uint32_t>(blocks_for(value_count), 1024u)), dim3(256), 0, st,
+                           grad_values, value_count, rmax);
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, rmax, 4, hipMemcpyDeviceToHost, st));
+    VDBR_CHECK(ctx, hipStreamSynchronize(st));
+    if (*(const uint32_t*)ctx->pinned >= 0x7F800000u)
+        return density_backward(ctx, raw, vs, grid, grid_size, grad_values, value_count, grads, ms_out);
+    LeafTable lt;
+    if (bwd_leaf_table(ctx, grid, grid_size, grad_values, value_count, &lt))
+        return -1;
+    BwdPrim* prims = (BwdPrim*)ctx->ws(WS_BWD_PRIMS, n * sizeof(BwdPrim));
+    unsigned long long* sums = (unsigned long long*)ctx->ws(WS_BWD_ACC, n * BWD_ACC * 2 * sizeof(uint64_t));
+    if (!prims || !sums)
+        return -1;
+    VDBR_CHECK(ctx, hipMemsetAsync(sums, 0, n * BWD_ACC * 2 * sizeof(uint64_t), st));
+    BwdArgs a;
+    memset(&a, 0, sizeof(a));
+    a.n = n;
+    a.vs = vs;
+    a.means = raw->means;
+    a.log_scales = raw->scales;
+    a.quats = raw->quats;
+    a.logits = raw->opacities;
+    a.prims = prims;
+    a.grad_means = grads->means;
+    a.grad_logits = grads->opacities;
+    a.grad_quats = grads->quats;
+    a.grad_log_scales = grads->scales;
+    hipLaunchKernelGGL(k_bwd_prims, dim3(blocks_for(n)), dim3(256), 0, st, a);
+    PairBwdArgs pa;
+    memset(&pa, 0, sizeof(pa));
+    pa.keys = ctx->last_pairs;
+    pa.npairs = ctx->last_pair_count;
+    for (int c = 0; c < 3; c++)
+    {
+        pa.kl.lmin[c] = ctx->last_lmin[c];
+        pa.kl.bits[c] = ctx->last_bits[c];
+    }
+    pa.kl.gid_bits = ctx->last_gid_bits;
+    pa.kl.alpha_bits = 0u;
+    pa.kl.voxel_bits = pa.kl.bits[0] + pa.kl.bits[1] + pa.kl.bits[2] + 9u;
+    pa.vs = (double)vs;
+    pa.prims = prims;
+    pa.lt_slots = lt.slots;
+    pa.lt_mask = lt.mask;
+    pa.leaf_grad = lt.leaf_grad;
+    pa.rmax_bits = rmax;
+    pa.sums = sums;
+    const uint64_t blocks = (pa.npairs + BP_TILE - 1) / BP_TILE;
+    if (blocks > 0x7FFFFFFFull)
+        return ctx->fail("density_backward: too many pairs for one launch");
+    if (blocks && pa.keys)
+        hipLaunchKernelGGL(k_density_bwd_pairs, dim3((uint32_t)blocks), dim3(BP_THREADS), 0, st, pa);
+    hipLaunchKernelGGL(k_density_bwd_pairs_finalize, dim3(blocks_for(n)), dim3(256), 0, st, a,
+                       (const unsigned long long*)sums, (const unsigned int*)rmax);
+    VDBR_CHECK(ctx, hipGetLastError());
+    VDBR_CHECK(ctx, hipEventRecord(e1, st));
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, lt.counters, 16, hipMemcpyDeviceToHost, st));
+    VDBR_CHECK(ctx, hipStreamSynchronize(st));
+    if (((const uint32_t*)ctx->pinned)[3])
+        return ctx->fail("density_backward: the grid's nodes do not match its tree header");
+    if (ms_out)
+        VDBR_CHECK(ctx, hipEventElapsedTime(ms_out, e0, e1));
+    *used_pairs = 1;
     return 0;
 }
 

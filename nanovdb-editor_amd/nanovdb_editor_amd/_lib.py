@@ -38,7 +38,7 @@ class VdbrRasterStats(ctypes.Structure):
                 ("window_count", u32), ("key_bits", u32), ("emit_attempts", u32), ("ms_setup", f32),
                 ("ms_emit", f32), ("ms_sort", f32), ("ms_reduce", f32), ("ms_build", f32), ("ms_total", f32),
                 ("sort_passes", u32), ("ms_sort_passes", f32), ("ms_emit_kernel", f32), ("ms_reduce_kernel", f32),
-                ("sort_bit_begin", u32), ("slab_count", u32)]
+                ("sort_bit_begin", u32), ("slab_count", u32), ("pair_list_id", u64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -83,6 +83,8 @@ SIGNATURES = {
     "vdbr_radix_sort_u64_noalloc": ([P, P, P, u64, u32, u32], ctypes.c_int),
     "vdbr_density_backward": ([P, ctypes.POINTER(VdbrGaussians), f32, P, u64, P, u64,
                                ctypes.POINTER(VdbrGaussianGrads)], ctypes.c_int),
+    "vdbr_density_backward_pairs": ([P, ctypes.POINTER(VdbrGaussians), f32, P, u64, P, u64,
+                                     ctypes.POINTER(VdbrGaussianGrads), u64], ctypes.c_int),
     "vdbr_selftest_alpha8": ([P, u64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)], ctypes.c_int),
     "vdbr_context_trim": ([P], ctypes.c_int),
     "vdbr_last_voxels": ([P, P, u64, ctypes.POINTER(u64)], ctypes.c_int),

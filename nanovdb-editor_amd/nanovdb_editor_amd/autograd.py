@@ -3,9 +3,10 @@
 ``voxelize_density(means, log_scales, quats, opacity_logits, voxel_size)`` returns the value
 array of the node2 density grid (float32, one value per value index; slots 0 and 1 are the
 background) and the grid itself. The values are differentiable with respect to the four raw
-parameter tensors; the backward pass is ``vdbr_density_backward`` (deterministic, double
-precision accumulation). The voxel set of the grid is a piecewise-constant function of the
-parameters and carries no gradient.
+parameter tensors; the backward pass is ``vdbr_density_backward_pairs`` over the pair list the
+forward kept on the voxelizer's context (double precision terms summed in exact fixed point,
+deterministic), or ``vdbr_density_backward`` when another call has replaced that list. The
+voxel set of the grid is a piecewise-constant function of the parameters and carries no gradient.
 """
 import torch
 
@@ -44,6 +45,7 @@ class DensityVoxelize(torch.autograd.Function):
         ctx.save_for_backward(means, log_scales, quats, opacity_logits, grid)
         ctx.voxel_size = float(voxel_size)
         ctx.vox = vox
+        ctx.pair_list_id = st.get("pair_list_id", 0)
         ctx.mark_non_differentiable(grid)
         if with_coords:
             ijk = vox.last_voxels()
@@ -58,7 +60,8 @@ class DensityVoxelize(torch.autograd.Function):
             return None, None, None, None, None, None, None
         gm, gs, gq, go = ctx.vox.density_backward(
             means.detach().contiguous(), log_scales.detach().contiguous(), quats.detach().contiguous(),
-            opacity_logits.detach().contiguous(), ctx.voxel_size, grid, grad_values.contiguous().float())
+            opacity_logits.detach().contiguous(), ctx.voxel_size, grid, grad_values.contiguous().float(),
+            pair_list_id=ctx.pair_list_id)
         return gm, gs, gq, go, None, None, None
 
 

@@ -39,6 +39,7 @@ class Voxelizer:
         if not self._ctx:
             raise RuntimeError(f"vdbr_context_create({device}) failed (no HIP device?)")
         self.device = torch.device("cuda", int(device))
+        self.last_backward_used_pairs = False
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -137,10 +138,15 @@ class Voxelizer:
             grids.append(out)
         return grids, stats
 
-    def density_backward(self, means, log_scales, quats, opacity_logits, voxel_size, grid, grad_values):
+    def density_backward(self, means, log_scales, quats, opacity_logits, voxel_size, grid, grad_values,
+                         pair_list_id=0):
         """Gradients (means, log_scales, quats, opacity_logits) of sum(grad_values * values) where
         values is the value array of the density grid ``grid`` that ``raster_node2`` built from these
-        raw parameters (``raw=True``, ``value_mode=VALUE_DENSITY``)."""
+        raw parameters (``raw=True``, ``value_mode=VALUE_DENSITY``).
+
+        With the forward's ``stats["pair_list_id"]`` the adjoint runs over the sorted pair list that
+        the forward kept on this context (``vdbr_density_backward_pairs``); when that list is gone it
+        enumerates the pairs again. ``last_backward_used_pairs`` tells which path ran."""
         n = means.numel() // 3
         _check_f32("means", means, 3 * n)
         _check_f32("log_scales", log_scales, 3 * n)
@@ -154,9 +160,19 @@ class Voxelizer:
         g = VdbrGaussians(_ptr(means), _ptr(opacity_logits), _ptr(quats), _ptr(log_scales), None, None, n)
         out = VdbrGaussianGrads(_ptr(gm), _ptr(go), _ptr(gq), _ptr(gs))
         self._bind_stream()
-        if self._lib.vdbr_density_backward(self._ctx, ctypes.byref(g), float(voxel_size), _ptr(grid), grid.numel(),
-                                           _ptr(grad_values), grad_values.numel(), ctypes.byref(out)) != 0:
-            self._err("vdbr_density_backward")
+        if pair_list_id:
+            rc = self._lib.vdbr_density_backward_pairs(self._ctx, ctypes.byref(g), float(voxel_size), _ptr(grid),
+                                                       grid.numel(), _ptr(grad_values), grad_values.numel(),
+                                                       ctypes.byref(out), int(pair_list_id))
+            if rc < 0:
+                self._err("vdbr_density_backward_pairs")
+            self.last_backward_used_pairs = rc == 0
+        else:
+            if self._lib.vdbr_density_backward(self._ctx, ctypes.byref(g), float(voxel_size), _ptr(grid),
+                                               grid.numel(), _ptr(grad_values), grad_values.numel(),
+                                               ctypes.byref(out)) != 0:
+                self._err("vdbr_density_backward")
+            self.last_backward_used_pairs = False
         return gm, gs, gq, go
 
     def node2_to_onindex(self, grid):
