@@ -21,6 +21,14 @@ sys.path.insert(0, os.path.join(ROOT, "nanovdb-editor_amd"))
 
 METRIC = "Gaussians/s voxelized + fwd+bwd ms @2M vs plain GPU path; % HBM roofline"
 HBM_PEAK_GBS = 8000.0
+# FP32 VALU operations per second without packed or fused instructions (the predicate keeps the
+# reference's unfused operation order and the build disables packed FP32): 256 CUs x 4 SIMDs x
+# 16 lanes x 2.4 GHz
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+# FP32 operations the walk spends per candidate voxel: one new lattice plane (lz: 2; per axis
+# tz: 1, four corners x (add, mul, mul): 12, min/max over the corners: 6) = 59, the box merge of
+# two planes: 6, and the sphere test: 9
+WALK_OPS_PER_CANDIDATE = 74
 
 
 def parse():
@@ -252,23 +260,47 @@ def batched(vox, steps, workers):
             "gaussians_per_s": n / (ms * 1e-3), "ms_per_batch_one_worker": ms_seq, "steps": steps}
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(cloud, sample, threads):
-    """The oracle (CPU restatement of the reference path) on a fixed subsample of the workload."""
+    """The oracle (CPU restatement of the reference path) on a fixed subsample of the workload, on
+    every core this process may run on, and on one thread over a sixteenth of that sample."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    rng = np.random.default_rng(12345)
-    idx = np.sort(rng.choice(cloud.count, size=min(sample, cloud.count), replace=False))
-    sub = cloud.subset(idx)
-    cores = threads or min(os.cpu_count() or 1, 16)
-    t0 = time.perf_counter()
-    m, op, q, s, col = oracle.preprocess(sub.means, sub.opacity_logits, sub.quats, sub.log_scales, sub.sh0)
-    grid, st = oracle.raster_node2(m, q, s, col, op, sub.voxel_size, nthreads=cores)
-    dt = time.perf_counter() - t0
-    return {"value": sub.count / dt, "unit": "Gaussians/s", "cores": cores, "kind": "port",
-            "sample": f"{sub.count} Gaussians drawn (seed 12345) from the benchmark cloud, full oracle raster "
-                      f"(preprocess, pair predicate, windowed node2 build, RGBA8 values) in {dt:.2f} s; "
-                      f"{st['pair_count']} pairs"}
+
+    def run(count, nthreads):
+        rng = np.random.default_rng(12345)
+        idx = np.sort(rng.choice(cloud.count, size=min(count, cloud.count), replace=False))
+        sub = cloud.subset(idx)
+        t0 = time.perf_counter()
+        m, op, q, s, col = oracle.preprocess(sub.means, sub.opacity_logits, sub.quats, sub.log_scales, sub.sh0)
+        _, st = oracle.raster_node2(m, q, s, col, op, sub.voxel_size, nthreads=nthreads)
+        return sub.count, time.perf_counter() - t0, st["pair_count"]
+
+    # every core of this process's share: its affinity set, capped by the thread budget the host
+    # grants one GPU's job (OMP_NUM_THREADS)
+    affinity = len(os.sched_getaffinity(0))
+    budget = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    cores = threads or (min(affinity, budget) if budget > 0 else affinity)
+    n_all, dt_all, p_all = run(sample, cores)
+    n_one, dt_one, p_one = run(max(sample // 16, 1), 1)
+    return {"value": n_all / dt_all, "unit": "Gaussians/s", "cores": cores, "kind": "port",
+            "sample": f"{n_all} Gaussians drawn (seed 12345) from the benchmark cloud, full oracle raster "
+                      f"(preprocess, pair predicate, windowed node2 build, RGBA8 values) on {cores} threads in "
+                      f"{dt_all:.2f} s; {p_all} pairs",
+            "one_thread": {"value": n_one / dt_one, "unit": "Gaussians/s", "cores": 1,
+                           "sample": f"{n_one} Gaussians (same seed), {dt_one:.2f} s, {p_one} pairs"},
+            "nproc": os.cpu_count(), "affinity_cpus": affinity, "cpu_model": _cpu_model()}
 
 
 def _progress(what):
@@ -334,14 +366,22 @@ def main():
     def avg(key):
         return sum(s[key] for s in stats) / len(stats)
 
-    # dominant kernel: the onesweep sort pass; algorithmic bytes per pass = P * (8 read + 8 write)
+    # the onesweep sort pass; algorithmic bytes per pass = P * (8 read + 8 write)
     passes = max(st["sort_passes"], 1)
     pass_ms = avg("ms_sort_passes") / passes
     pass_bytes = P * 16
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9 if pass_ms > 0 else 0.0
     default_workload = args.gaussians == 2_000_000 and args.value_mode == "rgba8"
     traffic = profiled_traffic("k_sort_pass") if default_workload else None
-    emit_bytes = st["gaussian_count"] * 112 + P * 8  # frame+box reads per Gaussian + key writes
+    # dominant kernel: the emission walk (k_emit_segments), one launch per step. Algorithmic bytes:
+    # N * 44 (parameters) + N * 32 (per-Gaussian setup) + P * 8 (key writes); it is bound by FP32
+    # issue, reported against the VALU peak from the candidate voxels it evaluates
+    N = st["gaussian_count"]
+    emit_bytes = N * 44 + N * 32 + P * 8
+    walk_ms = avg("ms_walk_kernel")
+    emit_achieved = emit_bytes / (walk_ms * 1e-3) / 1e9 if walk_ms > 0 else 0.0
+    cand = st["candidate_voxels"]
+    valu_tops = cand * WALK_OPS_PER_CANDIDATE / (walk_ms * 1e-3) / 1e12 if walk_ms > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": args.gaussians / (ms_per_step * 1e-3),
@@ -360,14 +400,22 @@ def main():
                    "gaussians": args.gaussians, "voxel_size": cloud.voxel_size, "value_mode": args.value_mode,
                    "fragments": st["fragment_count"], "pairs": P, "voxels": st["voxel_count"],
                    "leaves": st["leaf_count"], "key_bits": st["key_bits"], "grid_bytes": st["grid_size"]},
-        "roofline": {"bound": "hbm", "kernel": "k_sort_pass (onesweep radix pass over 64-bit pair keys)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "avg_pass_ms": pass_ms, "algorithmic_bytes_per_launch": pass_bytes},
+        "roofline": {"bound": "hbm", "kernel": "k_emit_segments (pair predicate walk, staging, key writes)",
+                     "achieved": emit_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": emit_achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic("k_emit_segments") if default_workload else None,
+                     "avg_launch_ms": walk_ms, "algorithmic_bytes_per_launch": emit_bytes,
+                     "valu": {"achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "T FP32 ops/s",
+                              "frac": valu_tops / VALU_PEAK_TOPS, "candidate_voxels": cand,
+                              "ops_per_candidate": WALK_OPS_PER_CANDIDATE}},
+        "sort_roofline": {"bound": "hbm", "kernel": "k_sort_pass (onesweep radix pass over 64-bit pair keys)",
+                          "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                          "traffic": traffic, "avg_launch_ms": pass_ms, "algorithmic_bytes_per_launch": pass_bytes},
         "sort_autotune": tuned,
         "pairs_per_s": P / (ms_per_step * 1e-3),
         "stage_ms": {k: avg(k) for k in ("ms_setup", "ms_emit", "ms_sort", "ms_reduce", "ms_build", "ms_total",
-                                        "ms_sort_passes", "ms_emit_kernel", "ms_reduce_kernel")},
-        "emit_kernel_gbs": emit_bytes / (avg("ms_emit_kernel") * 1e-3) / 1e9 if avg("ms_emit_kernel") else None,
+                                        "ms_sort_passes", "ms_emit_kernel", "ms_row_kernel", "ms_walk_kernel",
+                                        "ms_reduce_kernel")},
     }
     if args.fwd_bwd_steps > 0:
         out["differentiable"] = fwd_bwd(cloud, raw, vox, args.fwd_bwd_steps)

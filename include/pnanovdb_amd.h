@@ -492,12 +492,15 @@ typedef struct vdbr_raster_stats_t
     float ms_setup, ms_emit, ms_sort, ms_reduce, ms_build, ms_total;
     uint32_t sort_passes;      /* onesweep passes of the pair sort */
     float ms_sort_passes;      /* time of the onesweep pass kernels only (HIP events) */
-    float ms_emit_kernel;      /* time of the last emission kernel launch (HIP events) */
+    float ms_emit_kernel;      /* emission: row pass, segment count readback and walk (HIP events) */
     float ms_reduce_kernel;    /* time of the per-voxel reduction kernel (HIP events) */
     uint32_t sort_bit_begin;   /* the pair sort orders key bits [sort_bit_begin, key_bits) */
     uint32_t slab_count;       /* voxel-x slabs the pairs were emitted, sorted and reduced in */
     uint64_t pair_list_id;     /* density mode, one slab: id of the pair list the context keeps for
                                   vdbr_density_backward_pairs; 0 otherwise */
+    float ms_row_kernel;       /* k_row_segments alone: AABB rows -> candidate segments (HIP events) */
+    float ms_walk_kernel;      /* k_emit_segments alone: predicate walk, staging, key writes (HIP events) */
+    uint64_t candidate_voxels; /* voxels of the candidate segments the walk evaluates the predicate on */
 } vdbr_raster_stats_t;
 
 /* Full raster3d pipeline on device inputs. The node2 grid is written to a context-owned

@@ -89,6 +89,8 @@ enum WsSlot
     WS_LR_G0,
     WS_LR_KEY,
     WS_LR_CNT,
+    WS_LR_STARTS, // first pair of every leaf of the sorted pairs (density reduce)
+    WS_LR_ITEMS,  // exclusive scan of the density reduce's chunks per leaf
     WS_GB_KEYS,
     WS_GB_FLAGS,
     WS_GB_ENC,

@@ -701,11 +701,17 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
         row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, a.vs, &g, &f, &i, &j, &k0, &len,
                        a.slab_i0);
         const uint32_t nseg = len > 0 ? (uint32_t)((len + EMIT_SEG - 1) / EMIT_SEG) : 0u;
-        uint32_t total;
+        uint32_t total, cand_total;
         const uint32_t excl = wave_excl_add(nseg, lane, &total);
+        (void)wave_excl_add(len > 0 ? (uint32_t)len : 0u, lane, &cand_total);
         unsigned long long base = 0ull;
         if (lane == 0u && total)
+        {
             base = atomicAdd(seg_counter, (unsigned long long)total);
+            // candidate voxels handed to the walk, in the padding word after the partition's counter
+            // (one address per partition: a single shared counter would serialize the waves)
+            atomicAdd(seg_counter + 1, (unsigned long long)cand_total);
+        }
         base = __shfl(base, 0, 64);
         // the wave writes its segments with consecutive lanes: segment s belongs to the last lane
         // whose exclusive count is <= s
@@ -915,12 +921,13 @@ constexpr int BWD_ACC = 10;
 constexpr int32_t BWD_SEG = 8;
 constexpr uint32_t LT_NONE = 0xFFFFFFFFu;
 
-// double-precision parameters of one Gaussian as the pair terms use them
+// double-precision parameters of one Gaussian as the pair terms use them (96 bytes)
 struct BwdPrim
 {
     double M[9]; // diag(1/s) R^T, row-major (floored scales)
-    double mu[3];
     double o;
+    float mu[3]; // the float means, exact in the double arithmetic of the terms
+    float pad;
 };
 
 struct BwdArgs
@@ -1209,9 +1216,10 @@ __global__ __launch_bounds__(256) void k_bwd_prims(BwdArgs a)
     {
         for (int b = 0; b < 3; b++)
             q.M[3 * c + b] = p.R[b][c] / p.s[c];
-        q.mu[c] = p.mu[c];
+        q.mu[c] = a.means[3 * g + c];
     }
     q.o = p.o;
+    q.pad = 0.f;
     a.prims[g] = q;
 }
 
@@ -1624,7 +1632,8 @@ __global__ __launch_bounds__(256) void k_density_bwd_finalize(BwdArgs a)
 // blocks add them. Each lane takes BP_PER_LANE consecutive pairs and sums the run of one Gaussian
 // in registers; a finished run goes to the block's LDS table (BP_SLOTS Gaussians, 64-bit sums: a
 // tile holds at most 2^12 terms of magnitude 2^48), or straight to the global sums when its probe
-// finds no free slot; at the end of the tile the table goes to the global 128-bit sums.
+// finds no free slot; at the end of the tile the table goes to the global 128-bit sums, Gaussian
+// by Gaussian (the 10 sums of one Gaussian are adjacent, so a wave's atomics touch few lines).
 constexpr int BP_THREADS = 256;
 constexpr int BP_PER_LANE = 16;
 constexpr uint64_t BP_TILE = (uint64_t)BP_THREADS * BP_PER_LANE;
@@ -1788,7 +1797,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_density_bwd_pairs(PairBwdArgs a)
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < BP_SLOTS * BWD_ACC; e += BP_THREADS)
     {
-        const uint32_t slot = e & (BP_SLOTS - 1u), t = e / BP_SLOTS;
+        const uint32_t slot = e / BWD_ACC, t = e - slot * BWD_ACC;
         const uint32_t g = s_gid[slot];
         const int64_t v = (int64_t)s_acc[t][slot];
         if (g != BP_EMPTY && v != 0)
@@ -1977,7 +1986,51 @@ __global__ __launch_bounds__(256) void k_head_count_sorted(const uint64_t* __res
         tile_counts[tile] = count;
 }
 
-// writes run starts in order: starts[rank] = index of the run head
+// run starts of sorted values in order (starts[rank] = index of the run head), visiting only the
+// 64-key segments that hold a head, as k_head_count_sorted counts them; tile_offsets is the
+// exclusive scan of those counts
+__global__ __launch_bounds__(256) void k_head_write_sorted(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
+                                                           uint64_t tiles, const uint32_t* __restrict__ tile_offsets,
+                                                           uint32_t* __restrict__ starts)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t tile = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (tile >= tiles)
+        return; // wave-uniform
+    const uint64_t t0 = tile * HEAD_TILE;
+    const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
+    if (t0 > 0 && (keys[t0 - 1] >> shift) == (keys[t1 - 1] >> shift))
+        return; // wave-uniform: no head in this tile
+    const uint64_t s = t0 + (uint64_t)lane * HEAD_SEG;
+    const bool live = s < t1;
+    const uint64_t e = s + HEAD_SEG < t1 ? s + HEAD_SEG : t1;
+    const uint64_t vend = live ? keys[e - 1] >> shift : 0ull;
+    uint64_t vbefore = wave_shr1_u64(0ull, vend);
+    if (lane == 0u && t0 > 0)
+        vbefore = keys[t0 - 1] >> shift;
+    const bool first = lane == 0u && t0 == 0;
+    uint64_t todo = __ballot(live && (first || vbefore != vend));
+    uint32_t rank = tile_offsets[tile];
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    while (todo)
+    {
+        const int l = __builtin_ctzll(todo);
+        todo &= todo - 1ull;
+        const uint64_t idx = t0 + (uint64_t)l * HEAD_SEG + lane;
+        const bool valid = idx < t1;
+        const uint64_t v = valid ? keys[idx] >> shift : 0ull;
+        uint64_t pv = wave_shr1_u64(0ull, v);
+        const uint64_t before_l = readlane_u64(vbefore, l);
+        if (lane == 0u)
+            pv = before_l;
+        const bool head = valid && (idx == 0 || v != pv);
+        const uint64_t hb = __ballot(head);
+        if (head)
+            starts[rank + (uint32_t)__popcll(hb & lt_mask)] = (uint32_t)idx;
+        rank += (uint32_t)__popcll(hb);
+    }
+}
+
 // writes run starts in order: starts[rank] = index of the run head
 __global__ __launch_bounds__(HEAD_THREADS) void k_head_write(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
                                                              const uint32_t* __restrict__ tile_offsets,
@@ -2344,140 +2397,119 @@ __device__ __forceinline__ float fix120_to_float(uint64_t w0, uint64_t w1, uint6
 
 // Density values by leaf: the pairs are sorted by their leaf bits only and each voxel's sum of
 // fixed-point terms is exact integer arithmetic, so the pair order inside a leaf does not matter.
-// One block per tile of HEAD_TILE pairs adds its terms into LDS slots (192-bit accumulators, 64-bit
-// atomic adds carrying between words) of up to LRD_LEAVES leaves, then merges the touched slots
-// into the leaf staging arrays with the same carrying atomic adds; pairs of further leaves of the
-// tile add into the staging arrays directly.
-constexpr int LRD_LEAVES = 2;
+// One block per chunk of at most LRD_CHUNK pairs of one leaf adds its terms into LDS slots
+// (192-bit accumulators, 64-bit atomic adds carrying between words). A leaf of one chunk (most of
+// them) is stored with plain stores; the chunks of a longer leaf merge into the zeroed staging
+// arrays with the same carrying atomic adds.
+constexpr uint32_t LRD_CHUNK = 16384;
 
 __device__ __forceinline__ void add192(unsigned long long* w, uint64_t a0, uint64_t a1, uint64_t a2)
 {
     atomic_add192(w, a0, a1, a2);
 }
 
-__global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_density(const uint64_t* __restrict__ keys, uint64_t n,
-                                                                     KeyLayout kl, float vs,
-                                                                     const uint32_t* __restrict__ tile_offsets,
-                                                                     const DensPrim* __restrict__ prims,
+// chunks of every leaf (starts holds nleaf + 1 entries, the last one the pair count)
+__global__ __launch_bounds__(256) void k_leaf_chunks(const uint32_t* __restrict__ starts, uint64_t nleaf,
+                                                     uint32_t* __restrict__ items)
+{
+    const uint64_t l = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (l < nleaf)
+        items[l] = (starts[l + 1] - starts[l] + LRD_CHUNK - 1u) / LRD_CHUNK;
+}
+
+__global__ void k_store_u32(uint32_t* p, uint32_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *p = v;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_density(const uint64_t* __restrict__ keys, KeyLayout kl,
+                                                                     float vs, const uint32_t* __restrict__ starts,
+                                                                     const uint32_t* __restrict__ item_scan,
+                                                                     uint64_t nleaf, const DensPrim* __restrict__ prims,
                                                                      unsigned long long* __restrict__ lw,
                                                                      uint32_t* __restrict__ lg0,
                                                                      uint64_t* __restrict__ lkey)
 {
-    __shared__ unsigned long long s_w[LRD_LEAVES * 512 * 3];
-    __shared__ uint32_t s_g0[LRD_LEAVES * 512];
-    __shared__ uint32_t wave_heads[RED_THREADS / 64];
+    __shared__ unsigned long long s_w[512 * 3];
+    __shared__ uint32_t s_g0[512];
     __shared__ double s_exp2[64];
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t lshift = kl.gid_bits + 9u;
-    const uint64_t gmask = (1ull << kl.gid_bits) - 1ull;
-    const double vsd = (double)vs;
-    const uint64_t t0 = (uint64_t)blockIdx.x * HEAD_TILE;
-    const uint64_t pb = t0 + (uint64_t)threadIdx.x * RED_ITEMS;
-    const uint32_t tile_base = tile_offsets[blockIdx.x];
-    const bool first_cont = t0 > 0 && (keys[t0] >> lshift) == (keys[t0 - 1] >> lshift);
-    for (uint32_t x = threadIdx.x; x < (uint32_t)(LRD_LEAVES * 512 * 3); x += RED_THREADS)
+    __shared__ uint32_t s_leaf;
+    const uint32_t item = blockIdx.x;
+    if (item >= item_scan[nleaf])
+        return; // block-uniform: the grid is an upper bound of the chunk count
+    if (threadIdx.x == 0)
+    {
+        // the leaf of this chunk: largest l with item_scan[l] <= item
+        uint32_t lo = 0, hi = (uint32_t)nleaf - 1u;
+        while (lo < hi)
+        {
+            const uint32_t mid = (lo + hi + 1u) >> 1;
+            if (item_scan[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1u;
+        }
+        s_leaf = lo;
+    }
+    for (uint32_t x = threadIdx.x; x < 512u * 3u; x += RED_THREADS)
         s_w[x] = 0ull;
-    for (uint32_t x = threadIdx.x; x < (uint32_t)(LRD_LEAVES * 512); x += RED_THREADS)
+    for (uint32_t x = threadIdx.x; x < 512u; x += RED_THREADS)
         s_g0[x] = 0xFFFFFFFFu;
     if (threadIdx.x < 64u)
         s_exp2[threadIdx.x] = EXP2_J64[threadIdx.x];
-    uint64_t key[RED_ITEMS];
-    if (pb + RED_ITEMS <= n)
-    {
-        const ulonglong2* src = (const ulonglong2*)(keys + pb);
-#pragma unroll
-        for (int i = 0; i < RED_ITEMS / 2; i++)
-        {
-            ulonglong2 v = src[i];
-            key[2 * i] = v.x;
-            key[2 * i + 1] = v.y;
-        }
-    }
-    else
-    {
-#pragma unroll
-        for (int i = 0; i < RED_ITEMS; i++)
-            key[i] = pb + i < n ? keys[pb + i] : ~0ull;
-    }
-    uint64_t prev = wave_shr1_u64(0ull, key[RED_ITEMS - 1]);
-    if (lane == 0u)
-        prev = (pb > 0 && pb < n) ? keys[pb - 1] : ~0ull;
-    prev >>= lshift;
-    uint32_t headbits = 0u;
-    {
-        uint64_t pv = prev;
-#pragma unroll
-        for (int i = 0; i < RED_ITEMS; i++)
-        {
-            const uint64_t lk = key[i] >> lshift;
-            if (pb + i < n && (pb + i == 0 || lk != pv))
-                headbits |= 1u << i;
-            pv = lk;
-        }
-    }
-    uint32_t wtot;
-    const uint32_t excl = wave_excl_add((uint32_t)__popc(headbits), lane, &wtot);
-    if (lane == 0u)
-        wave_heads[wave] = wtot;
     __syncthreads();
-    uint32_t before = excl, tile_heads = 0u;
-    for (uint32_t w = 0; w < RED_THREADS / 64u; w++)
+    const uint32_t l = s_leaf;
+    const uint32_t chunk = item - item_scan[l];
+    const uint32_t nchunks = item_scan[l + 1] - item_scan[l];
+    const uint64_t first = starts[l];
+    const uint64_t p0 = first + (uint64_t)chunk * LRD_CHUNK;
+    const uint64_t pend = starts[l + 1];
+    const uint64_t p1 = p0 + LRD_CHUNK < pend ? p0 + LRD_CHUNK : pend;
+    const uint32_t lshift = kl.gid_bits + 9u;
+    const uint64_t gmask = (1ull << kl.gid_bits) - 1ull;
+    const uint64_t leaf_bits = keys[first] >> lshift;
+    int32_t i0, j0, k0;
+    decode_voxel(kl, leaf_bits << 9, &i0, &j0, &k0);
+    if (chunk == 0u && threadIdx.x == 0)
+        lkey[l] = leaf_bits;
+    const double vsd = (double)vs;
+    for (uint64_t q = p0 + threadIdx.x; q < p1; q += RED_THREADS)
     {
-        before += w < wave ? wave_heads[w] : 0u;
-        tile_heads += wave_heads[w];
-    }
-    const uint32_t cont = first_cont ? 1u : 0u;
-    uint32_t h = before;
-    for (int i = 0; i < RED_ITEMS; i++)
-    {
-        if ((headbits >> i) & 1u)
-            h++;
-        if (pb + i >= n)
-            continue;
-        const uint32_t gl = tile_base + h - 1u;
-        const uint32_t ord = h - 1u + cont;
-        const uint64_t vk = key[i] >> kl.gid_bits;
-        const uint32_t slot = (uint32_t)vk & 511u;
-        const uint32_t gid = (uint32_t)(key[i] & gmask);
-        if ((headbits >> i) & 1u)
-            lkey[gl] = key[i] >> lshift;
-        int32_t vi, vj, vkk;
-        decode_voxel(kl, vk, &vi, &vj, &vkk);
-        const DensPrim& q = prims[gid];
-        const double d[3] = { ((double)vi + 0.5) * vsd - (double)q.mu[0], ((double)vj + 0.5) * vsd - (double)q.mu[1],
-                              ((double)vkk + 0.5) * vsd - (double)q.mu[2] };
+        const uint64_t key = keys[q];
+        const uint32_t gid = (uint32_t)(key & gmask);
+        const uint32_t slot = (uint32_t)(key >> kl.gid_bits) & 511u;
+        const DensPrim& pr = prims[gid];
+        const double d[3] = { ((double)(i0 + (int32_t)(slot >> 6)) + 0.5) * vsd - (double)pr.mu[0],
+                              ((double)(j0 + (int32_t)((slot >> 3) & 7u)) + 0.5) * vsd - (double)pr.mu[1],
+                              ((double)(k0 + (int32_t)(slot & 7u)) + 0.5) * vsd - (double)pr.mu[2] };
         double m = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; c++)
         {
-            const double u = q.M[3 * c + 0] * d[0] + q.M[3 * c + 1] * d[1] + q.M[3 * c + 2] * d[2];
+            const double u = pr.M[3 * c + 0] * d[0] + pr.M[3 * c + 1] * d[1] + pr.M[3 * c + 2] * d[2];
             m += u * u;
         }
         uint64_t lo, hi;
-        fix120(q.o * exp_neg(-0.5 * m, s_exp2), &lo, &hi);
-        if (ord < (uint32_t)LRD_LEAVES)
-        {
-            add192(&s_w[(ord * 512u + slot) * 3u], lo, hi, 0ull);
-            atomicMin(&s_g0[ord * 512u + slot], gid);
-        }
-        else
-        {
-            add192(&lw[((uint64_t)gl * 512u + slot) * 3u], lo, hi, 0ull);
-            atomicMin(&lg0[(uint64_t)gl * 512u + slot], gid);
-        }
+        fix120(pr.o * exp_neg(-0.5 * m, s_exp2), &lo, &hi);
+        add192(&s_w[slot * 3u], lo, hi, 0ull);
+        atomicMin(&s_g0[slot], gid);
     }
     __syncthreads();
-    const uint32_t leaves = tile_heads + cont;
-    const uint32_t nl = leaves < (uint32_t)LRD_LEAVES ? leaves : (uint32_t)LRD_LEAVES;
-    const uint32_t gbase = tile_base - cont;
-    for (uint32_t x = threadIdx.x; x < nl * 512u; x += RED_THREADS)
+    for (uint32_t x = threadIdx.x; x < 512u; x += RED_THREADS)
     {
-        const uint32_t g = s_g0[x];
-        if (g != 0xFFFFFFFFu)
+        const uint64_t at = (uint64_t)l * 512u + x;
+        if (nchunks == 1u)
         {
-            const uint64_t at = (uint64_t)(gbase + x / 512u) * 512u + (x & 511u);
+            lw[at * 3u + 0u] = s_w[3u * x + 0u];
+            lw[at * 3u + 1u] = s_w[3u * x + 1u];
+            lw[at * 3u + 2u] = s_w[3u * x + 2u];
+            lg0[at] = s_g0[x];
+        }
+        else if (s_g0[x] != 0xFFFFFFFFu)
+        {
             add192(&lw[at * 3u], s_w[3u * x], s_w[3u * x + 1u], s_w[3u * x + 2u]);
-            atomicMin(&lg0[at], g);
+            atomicMin(&lg0[at], s_g0[x]);
         }
     }
 }
@@ -3151,7 +3183,10 @@ static int run_heads(Context* ctx, const uint64_t* keys, uint64_t n, uint32_t sh
                            tile_counts);
     if (scan_u32(ctx, tile_counts, tile_counts, tiles, false, tile_counts + tiles))
         return -1;
-    if (starts)
+    if (starts && sorted)
+        hipLaunchKernelGGL(k_head_write_sorted, dim3((uint32_t)((tiles + 3) / 4)), dim3(256), 0, ctx->stream, keys, n,
+                           shift, tiles, (const uint32_t*)tile_counts, starts);
+    else if (starts)
         hipLaunchKernelGGL(k_head_write, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
                            (const uint32_t*)tile_counts, starts);
     VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, tile_counts + tiles, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -3653,6 +3688,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     uint32_t attempts = 0;
     uint32_t slab_count = 0;
     float acc_emit = 0.f, acc_sort = 0.f, acc_reduce = 0.f, acc_passes = 0.f, acc_emit_k = 0.f, acc_reduce_k = 0.f;
+    float acc_row_k = 0.f, acc_walk_k = 0.f;
+    uint64_t acc_cand = 0;
     uint64_t* last_sorted = nullptr;
     uint64_t last_slab_pairs = 0;
     // grows one voxel array, keeping its first `keep` bytes
@@ -3745,6 +3782,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 hipLaunchKernelGGL(k_row_segments, dim3((uint32_t)blocks_rows), dim3(EMIT_THREADS), 0, st, ra, segs,
                                    part_cap, seg_counters);
                 VDBR_CHECK(ctx, hipGetLastError());
+                VDBR_CHECK(ctx, hipEventRecord(ev[3], st));
                 VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, seg_counters, SEG_PARTS * SEG_COUNTER_STRIDE * 8,
                                                hipMemcpyDeviceToHost, st));
                 VDBR_CHECK(ctx, hipStreamSynchronize(st));
@@ -3757,7 +3795,14 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                     seg_count += cq;
                 }
                 if (maxc <= part_cap)
+                {
+                    for (uint32_t q = 0; q < SEG_PARTS; q++)
+                        acc_cand += ctx->pinned[q * SEG_COUNTER_STRIDE + 1];
+                    float ms = 0.f;
+                    if (hipEventElapsedTime(&ms, ev[8], ev[3]) == hipSuccess)
+                        acc_row_k += ms;
                     break;
+                }
                 if (sa >= 1)
                     return ctx->fail("raster_node2: segment buffer overflow after resize");
                 part_cap = maxc;
@@ -3826,6 +3871,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                     blocks = ctx->emit_blocks;
                 if (blocks == 0)
                     blocks = 1;
+                VDBR_CHECK(ctx, hipEventRecord(ev[2], st));
                 hipLaunchKernelGGL(k_emit_segments, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, ea);
                 VDBR_CHECK(ctx, hipGetLastError());
                 VDBR_CHECK(ctx, hipEventRecord(ev[9], st));
@@ -3846,6 +3892,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, ev[8], ev[9]) == hipSuccess)
                 acc_emit_k += ms;
+            if (hipEventElapsedTime(&ms, ev[2], ev[9]) == hipSuccess)
+                acc_walk_k += ms;
         }
         if (spairs > pair_limit && splittable)
         {
@@ -3915,9 +3963,26 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                                        (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, (const PrimFrame*)frames,
                                        (const PrimVal*)pvals, lval, lg0, lkey);
                 else
-                    hipLaunchKernelGGL(k_leaf_reduce_density, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
-                                       (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, (const DensPrim*)dprims,
-                                       lw, lg0, lkey);
+                {
+                    // leaf starts (the sparse head pass over the sorted keys), chunks per leaf
+                    uint32_t* lstarts = (uint32_t*)ctx->ws(WS_LR_STARTS, (sleaves + 1) * sizeof(uint32_t));
+                    uint32_t* litems = (uint32_t*)ctx->ws(WS_LR_ITEMS, (sleaves + 1) * sizeof(uint32_t));
+                    if (!lstarts || !litems)
+                        return -1;
+                    hipLaunchKernelGGL(k_head_write_sorted, dim3((uint32_t)((tiles + 3) / 4)), dim3(256), 0, st,
+                                       (const uint64_t*)sorted, spairs, lshift, tiles, tile_offsets, lstarts);
+                    hipLaunchKernelGGL(k_store_u32, dim3(1), dim3(64), 0, st, lstarts + sleaves, (uint32_t)spairs);
+                    hipLaunchKernelGGL(k_leaf_chunks, dim3(blocks_for(sleaves)), dim3(256), 0, st,
+                                       (const uint32_t*)lstarts, sleaves, litems);
+                    if (scan_u32(ctx, litems, litems, sleaves, false, litems + sleaves))
+                        return -1;
+                    const uint64_t items_max = sleaves + spairs / LRD_CHUNK + 1;
+                    if (items_max > 0x7FFFFFFFull)
+                        return ctx->fail("raster_node2: too many density reduce chunks");
+                    hipLaunchKernelGGL(k_leaf_reduce_density, dim3((uint32_t)items_max), dim3(RED_THREADS), 0, st,
+                                       (const uint64_t*)sorted, kl, vs, (const uint32_t*)lstarts,
+                                       (const uint32_t*)litems, sleaves, (const DensPrim*)dprims, lw, lg0, lkey);
+                }
                 VDBR_CHECK(ctx, hipGetLastError());
                 VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
                 const uint32_t lblocks = (uint32_t)((sleaves + 3) / 4);
@@ -4059,6 +4124,9 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         stats->sort_passes = sort_passes;
         stats->ms_sort_passes = acc_passes;
         stats->ms_emit_kernel = acc_emit_k;
+        stats->ms_row_kernel = acc_row_k;
+        stats->ms_walk_kernel = acc_walk_k;
+        stats->candidate_voxels = acc_cand;
         stats->ms_reduce_kernel = acc_reduce_k;
         (void)hipGetLastError();
     }

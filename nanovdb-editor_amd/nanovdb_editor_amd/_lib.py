@@ -38,7 +38,8 @@ class VdbrRasterStats(ctypes.Structure):
                 ("window_count", u32), ("key_bits", u32), ("emit_attempts", u32), ("ms_setup", f32),
                 ("ms_emit", f32), ("ms_sort", f32), ("ms_reduce", f32), ("ms_build", f32), ("ms_total", f32),
                 ("sort_passes", u32), ("ms_sort_passes", f32), ("ms_emit_kernel", f32), ("ms_reduce_kernel", f32),
-                ("sort_bit_begin", u32), ("slab_count", u32), ("pair_list_id", u64)]
+                ("sort_bit_begin", u32), ("slab_count", u32), ("pair_list_id", u64),
+                ("ms_row_kernel", f32), ("ms_walk_kernel", f32), ("candidate_voxels", u64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -40,6 +40,7 @@ class Voxelizer:
             raise RuntimeError(f"vdbr_context_create({device}) failed (no HIP device?)")
         self.device = torch.device("cuda", int(device))
         self.last_backward_used_pairs = False
+        self.last_pair_list_id = 0  # stats["pair_list_id"] of the last raster_node2 call
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -94,6 +95,7 @@ class Voxelizer:
                                          ctypes.byref(size), ctypes.byref(st))
         if rc != 0:
             self._err("vdbr_raster_node2")
+        self.last_pair_list_id = st.pair_list_id
         out = None
         if copy_out:
             out = torch.empty(size.value, dtype=torch.uint8, device=self.device)

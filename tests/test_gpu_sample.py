@@ -65,7 +65,8 @@ def test_node2_sample_matches_tree_walk(vox):
 
 def test_fit_loss_gradient_through_autograd(vox):
     """L2 fit of one cloud's density to another grid sampled at the fit voxels: the gradient equals
-    density_backward with 2 (values - target) on the value slots."""
+    density_backward with 2 (values - target) on the value slots, over the same pair list (bitwise),
+    and the re-walk adjoint to float32 rounding."""
     from nanovdb_editor_amd.autograd import voxelize_density
     fit, tgt = small_uniform(800, 0.01, 63), small_uniform(800, 0.01, 64)
     d = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32).ravel()).cuda()
@@ -80,6 +81,12 @@ def test_fit_loss_gradient_through_autograd(vox):
     loss.backward()
     gv = torch.zeros_like(values)
     gv[2:] = 2 * (values[2:].detach() - target)
-    ref = vox.density_backward(*[p.detach() for p in params], fit.voxel_size, grid, gv)
+    ref = vox.density_backward(*[p.detach() for p in params], fit.voxel_size, grid, gv,
+                               pair_list_id=vox.last_pair_list_id)
+    assert vox.last_backward_used_pairs
     for p, r in zip(params, ref):
-        assert torch.allclose(p.grad, r, rtol=0, atol=0)
+        assert torch.equal(p.grad, r)
+    rewalk = vox.density_backward(*[p.detach() for p in params], fit.voxel_size, grid, gv)
+    for p, r in zip(params, rewalk):
+        scale = float(r.abs().max())
+        assert float((p.grad - r).abs().max()) <= 1e-6 * scale
