@@ -251,13 +251,19 @@ def batched(vox, steps, workers):
         return 1e3 * (time.perf_counter() - t0) / k, st
 
     ms, st = timed(workers, steps)
+    phases = dict(vox.last_batch_timing)
+    graph_clouds = vox.last_batch_graph_clouds
     ms_seq, _ = timed(1, 1)
     n = sum(c.count for c in clouds)
     return {"workload": "batch_256: 256 clouds, sizes log-uniform in [1e3, 5e5], uniform or 64-cluster "
                         "(coin flip), voxel size U[0.004, 0.02], log-scales N(ln(1.5 vs), 0.4^2)",
             "clouds": len(clouds), "gaussians": n, "pairs": sum(x["pair_count"] for x in st),
             "voxels": sum(x["voxel_count"] for x in st), "workers": workers, "ms_per_batch": ms,
-            "gaussians_per_s": n / (ms * 1e-3), "ms_per_batch_one_worker": ms_seq, "steps": steps}
+            "gaussians_per_s": n / (ms * 1e-3), "pairs_per_s": sum(x["pair_count"] for x in st) / (ms * 1e-3),
+            "graph_clouds": graph_clouds, "last_call_phases_ms": phases,
+            "ms_per_batch_one_stream": ms_seq, "steps": steps,
+            "note": "one captured HIP graph per call over `workers` streams (setup of every cloud and one read "
+                    "back first); an unchanged plan relaunches the graph captured by the first call"}
 
 
 def _cpu_model():

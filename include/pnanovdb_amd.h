@@ -424,7 +424,8 @@ PNANOVDB_API void* vdbr_context_get_stream(vdbr_context_t* ctx);
 PNANOVDB_API int vdbr_synchronize(vdbr_context_t* ctx);
 /* kind: 0 host->device, 1 device->host, 2 device->device */
 PNANOVDB_API int vdbr_copy(vdbr_context_t* ctx, void* dst, const void* src, uint64_t bytes, int kind);
-/* bytes of device memory currently held by the context's workspace */
+/* bytes of device memory currently held by the context: its workspace, its batch workers' and the
+ * batch output arena chunks */
 PNANOVDB_API uint64_t vdbr_workspace_bytes(vdbr_context_t* ctx);
 /* Releases the context's workspace (and its batch workers'); device buffers handed out by earlier
  * calls become invalid. The next call allocates again. */
@@ -524,13 +525,28 @@ typedef struct vdbr_batch_item_t
 
 /* Batched raster3d (no counterpart in the reference, whose entry point takes one cloud): runs
  * every cloud of items[count] through the full pipeline, each result identical to a
- * vdbr_raster_node2 call on that cloud alone. Clouds are taken largest first by `workers` worker
- * contexts (0 -> 4), each with its own stream and workspace, so the kernels of several clouds
- * overlap on the device. Worker streams wait for the context's stream before the first cloud and
- * the context's stream waits for every worker at the end. On failure the message names the
- * first failing cloud. */
+ * vdbr_raster_node2 call on that cloud alone. The per-Gaussian setup of every cloud runs first
+ * with one read back of its counts; from those the call bounds each cloud's buffers and captures
+ * the remaining stages of all clouds (emission, sort, reduce, grid build, the grid copied into a
+ * context-owned output arena) as ONE HIP graph, spread over `workers` streams (0 -> 4, at most
+ * 16; each with its own workspace, clouds balanced by their pair estimates), and launches it
+ * once: no count is read back between stages. Clouds the capture does not take (more than one
+ * slab, no fragments, invalid input) and clouds whose counts exceeded their bounds run after the
+ * graph through vdbr_raster_node2. The work is ordered after the context's stream and the
+ * context's stream after it. On failure the message names the first failing cloud. */
 PNANOVDB_API int vdbr_raster_node2_batch(vdbr_context_t* ctx, vdbr_batch_item_t* items, uint32_t count,
                                          uint32_t workers);
+
+/* Clouds of the last vdbr_raster_node2_batch call on ctx that ran inside its captured graph
+ * (no counterpart in the reference). */
+PNANOVDB_API uint32_t vdbr_last_batch_graph_clouds(vdbr_context_t* ctx);
+
+/* Host-clock phases of the last vdbr_raster_node2_batch call, in milliseconds (no counterpart in
+ * the reference): ms[0] set-up of every cloud and its read back, ms[1] capture and instantiation
+ * of the graph (0 when the graph of an earlier call with the same signature -- the same clouds,
+ * plans and workspaces -- was launched again), ms[2] graph launch to completion, ms[3] 1 when the
+ * graph was captured by this call, ms[4] clouds run outside the graph. */
+PNANOVDB_API int vdbr_last_batch_timing(vdbr_context_t* ctx, float* ms);
 
 /* Adjoint of the density value mode (no counterpart in the reference, which has no backward
  * pass; the forward it differentiates is vdbr_raster_node2 with VDBR_VALUE_DENSITY):

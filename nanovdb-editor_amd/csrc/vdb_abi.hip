@@ -38,8 +38,20 @@ void* Context::ws(WsSlot slot, size_t bytes)
 {
     if (bytes == 0)
         bytes = 8;
+    if (probe)
+    {
+        // sizing pass: nothing recorded now is ever launched
+        probe_size[slot] = std::max(probe_size[slot], bytes);
+        return (void*)(uintptr_t)((uint64_t)(slot + 1) << 40);
+    }
     if (bytes <= ws_size[slot] && ws_ptr[slot])
         return ws_ptr[slot];
+    if (capturing)
+    {
+        fail("workspace slot " + std::to_string((int)slot) + " needs " + std::to_string(bytes) +
+             " bytes during graph capture (it was not reserved)");
+        return nullptr;
+    }
     if (ws_ptr[slot])
     {
         // earlier kernels on the stream may still read the old allocation
@@ -96,7 +108,9 @@ uint32_t Context::next_sort_epoch(uint64_t tiles)
     sort_epoch++;
     if (sort_epoch >= 65535u)
     {
-        (void)hipMemsetAsync(ws_ptr[WS_SORT_STATUS], 0, ws_size[WS_SORT_STATUS], stream);
+        // epochs are 16-bit: clear the status words before one could repeat (a fill kernel, so
+        // captured work may contain it)
+        (void)fill_words(this, (uint64_t*)ws_ptr[WS_SORT_STATUS], host_count(ws_size[WS_SORT_STATUS] / 8), 0ull);
         sort_epoch = 1u;
     }
     return sort_epoch;
@@ -113,6 +127,11 @@ uint64_t Context::workspace_bytes() const
 void Context::release()
 {
     (void)hipStreamSynchronize(stream);
+    // a captured batch graph refers to the workspace being released
+    if (batch_exec)
+        (void)hipGraphExecDestroy((hipGraphExec_t)batch_exec);
+    batch_exec = nullptr;
+    batch_sig.clear();
     for (int i = 0; i < WS_COUNT; i++)
     {
         if (ws_ptr[i])
@@ -171,6 +190,8 @@ static void destroy_context(Context* ctx)
     ctx->release();
     if (ctx->pinned)
         (void)hipHostFree(ctx->pinned);
+    if (ctx->batch_pinned)
+        (void)hipHostFree(ctx->batch_pinned);
     if (ctx->own_stream)
         (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -216,6 +237,9 @@ struct BatchArena
     }
 };
 
+// Batched raster3d: every cloud that fits the captured pipeline runs in one HIP graph
+// (raster_batch_graph); the rest (several slabs, no fragments, invalid input, or counts past the
+// bounds of the capture) run one by one through raster_node2, their grids packed into the arena.
 static int raster_batch(Context* ctx, vdbr_batch_item_t* items, uint32_t count, uint32_t workers)
 {
     if (count == 0)
@@ -232,100 +256,40 @@ static int raster_batch(Context* ctx, vdbr_batch_item_t* items, uint32_t count, 
             return ctx->fail("raster batch: could not create a worker context");
         ctx->batch_workers.push_back(w);
     }
-    // largest clouds first (longest-processing-time order balances the workers)
-    std::vector<uint32_t> order(count);
-    for (uint32_t i = 0; i < count; i++)
-        order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        return items[a].gaussians.count > items[b].gaussians.count;
-    });
-    hipEvent_t ready;
-    VDBR_CHECK(ctx, hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    VDBR_CHECK(ctx, hipEventRecord(ready, ctx->stream));
-    for (uint32_t w = 0; w < workers; w++)
-    {
-        Context* wc = ctx->batch_workers[w];
-        std::copy(ctx->sort_tuned, ctx->sort_tuned + SORT_BUCKETS, wc->sort_tuned);
-        wc->stream = wc->own_stream;
-        if (hipStreamWaitEvent(wc->stream, ready, 0) != hipSuccess)
-        {
-            (void)hipEventDestroy(ready);
-            return ctx->fail("raster batch: hipStreamWaitEvent failed");
-        }
-    }
-    (void)hipEventDestroy(ready);
-
+    std::vector<Context*> ws(ctx->batch_workers.begin(), ctx->batch_workers.begin() + workers);
+    std::vector<uint32_t> rerun;
+    std::vector<uint8_t> setup_done;
+    ctx->last_batch_graph_clouds = 0;
+    if (raster_batch_graph(ctx, ws, items, count, &rerun, &setup_done))
+        return -1;
+    ctx->last_batch_graph_clouds = count - (uint32_t)rerun.size();
+    const auto t0 = std::chrono::steady_clock::now();
     BatchArena arena;
     arena.owner = ctx;
-    std::atomic<uint32_t> next{ 0 };
-    std::atomic<bool> failed{ false };
-    std::mutex err_lock;
-    std::string first_error;
-    uint32_t first_failed = ~0u;
-    auto work = [&](uint32_t w) {
-        Context* wc = ctx->batch_workers[w];
-        (void)hipSetDevice(wc->device);
-        while (!failed.load())
-        {
-            uint32_t slot = next.fetch_add(1);
-            if (slot >= count)
-                break;
-            uint32_t i = order[slot];
-            vdbr_batch_item_t& it = items[i];
-            void* grid = nullptr;
-            uint64_t size = 0;
-            std::string err;
-            if (raster_node2(wc, &it.gaussians, &it.params, &grid, &size, &it.stats) != 0)
-                err = wc->error;
-            else if (size > 0)
-            {
-                void* dst = arena.reserve(size, &err);
-                if (dst)
-                {
-                    hipError_t e = hipMemcpyAsync(dst, grid, size, hipMemcpyDeviceToDevice, wc->stream);
-                    if (e != hipSuccess)
-                        err = std::string("hipMemcpyAsync: ") + hipGetErrorString(e);
-                    it.grid_dev = dst;
-                    it.grid_size = size;
-                }
-            }
-            else
-            {
-                it.grid_dev = nullptr;
-                it.grid_size = 0;
-            }
-            if (!err.empty())
-            {
-                std::lock_guard<std::mutex> g(err_lock);
-                if (i < first_failed)
-                {
-                    first_failed = i;
-                    first_error = err;
-                }
-                failed.store(true);
-            }
-        }
-    };
-    std::vector<std::thread> threads;
-    for (uint32_t w = 1; w < workers; w++)
-        threads.emplace_back(work, w);
-    work(0);
-    for (auto& t : threads)
-        t.join();
-    // the caller's stream orders after every worker's last copy
-    int rc = 0;
-    for (uint32_t w = 0; w < workers; w++)
+    for (uint32_t i : rerun)
     {
-        hipEvent_t done;
-        Context* wc = ctx->batch_workers[w];
-        if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess ||
-            hipEventRecord(done, wc->stream) != hipSuccess || hipStreamWaitEvent(ctx->stream, done, 0) != hipSuccess)
-            rc = ctx->fail("raster batch: could not order the caller's stream after the workers");
-        (void)hipEventDestroy(done);
+        vdbr_batch_item_t& it = items[i];
+        vdbr_raster_params_t prm = it.params;
+        if (setup_done[i])
+            prm.flags &= ~VDBR_FLAG_RAW_INPUTS;
+        void* grid = nullptr;
+        uint64_t size = 0;
+        if (raster_node2(ctx, &it.gaussians, &prm, &grid, &size, &it.stats) != 0)
+            return ctx->fail("raster batch: cloud " + std::to_string(i) + ": " + ctx->error);
+        it.grid_dev = nullptr;
+        it.grid_size = size;
+        if (size > 0)
+        {
+            std::string err;
+            void* dst = arena.reserve(size, &err);
+            if (!dst)
+                return ctx->fail("raster batch: cloud " + std::to_string(i) + ": " + err);
+            VDBR_CHECK(ctx, hipMemcpyAsync(dst, grid, size, hipMemcpyDeviceToDevice, ctx->stream));
+            it.grid_dev = dst;
+        }
     }
-    if (failed.load())
-        return ctx->fail("raster batch: cloud " + std::to_string(first_failed) + ": " + first_error);
-    return rc;
+    ctx->batch_ms[4] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
 }
 
 // ----------------------------------------------------------------------------- default context
@@ -1400,7 +1364,16 @@ PNANOVDB_API int vdbr_context_trim(vdbr_context_t* ctx)
 
 PNANOVDB_API uint64_t vdbr_workspace_bytes(vdbr_context_t* ctx)
 {
-    return ctx ? ((Context*)ctx)->workspace_bytes() : 0;
+    if (!ctx)
+        return 0;
+    // the context's slots, its batch workers' slots and the batch output chunks
+    const Context* c = (const Context*)ctx;
+    uint64_t b = c->workspace_bytes();
+    for (const Context* w : c->batch_workers)
+        b += w->workspace_bytes();
+    for (const auto& ch : c->batch_chunks)
+        b += ch.second;
+    return b;
 }
 
 PNANOVDB_API int vdbr_release_workspace(vdbr_context_t* ctx)
@@ -1430,6 +1403,22 @@ PNANOVDB_API int vdbr_raster_node2_batch(vdbr_context_t* ctx, vdbr_batch_item_t*
         return -1;
     (void)hipSetDevice(c->device);
     return raster_batch(c, items, count, workers);
+}
+
+PNANOVDB_API uint32_t vdbr_last_batch_graph_clouds(vdbr_context_t* ctx)
+{
+    Context* c = (Context*)ctx;
+    return c ? c->last_batch_graph_clouds : 0u;
+}
+
+PNANOVDB_API int vdbr_last_batch_timing(vdbr_context_t* ctx, float* ms)
+{
+    Context* c = (Context*)ctx;
+    if (!c || !ms)
+        return -1;
+    for (int i = 0; i < 5; i++)
+        ms[i] = c->batch_ms[i];
+    return 0;
 }
 
 PNANOVDB_API int vdbr_last_voxels(vdbr_context_t* ctx, int32_t* ijk_dev, uint64_t cap, uint64_t* count)

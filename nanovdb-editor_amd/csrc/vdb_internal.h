@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/pnanovdb_amd.h"
+#include "vdb_count.h"
 #include "vdb_keylayout.h"
 
 namespace vdbr
@@ -95,6 +96,13 @@ enum WsSlot
     WS_GB_FLAGS,
     WS_GB_ENC,
     WS_GB_T,
+    WS_GB_COUNTS, // node counts of the device-count build
+    WS_GB_INFO,   // grid layout of the device-count build
+    WS_GRAPH,     // count words and flags of the device-count raster pipeline
+    WS_ROWFRAMES, // per-Gaussian row-interval constants of the row pass
+    WS_BATCH_SETUP, // per-cloud setup buffers of a graph-captured batch
+    WS_BATCH_TABLE, // per-cloud window bases and output records of a graph-captured batch
+    WS_BATCH_ARENA, // output grids of a graph-captured batch
     WS_COUNT
 };
 
@@ -130,6 +138,16 @@ struct Context
     void* ws_ptr[WS_COUNT] = {};
     size_t ws_size[WS_COUNT] = {};
     uint64_t* pinned = nullptr; // small host readback area (pinned)
+    // set while the context's stream is being captured into a graph: no allocation, no event,
+    // no synchronisation may happen (Context::ws fails instead of growing a slot)
+    bool capturing = false;
+    // sizing pass of a capture (the recorded work is discarded, never launched): Context::ws only
+    // records the largest request per slot and hands out placeholder addresses
+    bool probe = false;
+    size_t probe_size[WS_COUNT] = {};
+    // pinned read back of a captured batch's setup counts (grown as needed)
+    uint64_t* batch_pinned = nullptr;
+    size_t batch_pinned_words = 0;
     uint32_t emit_blocks = 2048; // resident blocks of the persistent emission kernel (CUs x 3)
     SortConfig sort_tuned[SORT_BUCKETS] = {}; // autotuned sort configuration per size bucket
     uint32_t sort_epoch = 0;
@@ -154,6 +172,14 @@ struct Context
     // batched launch: worker contexts (own streams and workspaces) and the output arena chunks
     std::vector<Context*> batch_workers;
     std::vector<std::pair<void*, size_t>> batch_chunks;
+    uint32_t last_batch_graph_clouds = 0; // clouds of the last batched call that ran in its graph
+    // instantiated graph of the last captured batch and the signature it was captured under
+    void* batch_exec = nullptr;
+    uint64_t batch_arena_hint = 0; // arena bytes the last batch's grids needed, with headroom
+    std::vector<uint64_t> batch_sig;
+    // host milliseconds of the last batched call: set-up and read back, capture and
+    // instantiation, graph run, rerun clouds; [3] = 1 when the graph was captured anew
+    float batch_ms[5] = { 0.f, 0.f, 0.f, 0.f, 0.f };
 
     // forget the retained pair list (its buffers are about to be reused)
     void drop_pairs()
@@ -191,8 +217,17 @@ struct Context
 int scan_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, uint32_t* total_dev);
 int scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, uint64_t n, bool inclusive, uint64_t* total_dev);
 int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive);
+// the same scans over a count that may live on the device (launches sized by its bound)
+int scan_u32(Context* ctx, const uint32_t* in, uint32_t* out, DCount n, bool inclusive, uint32_t* total_dev);
+int scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, DCount n, bool inclusive, uint64_t* total_dev);
 int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
                    SortTiming* timing);
+int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, DCount n, uint32_t bit_begin, uint32_t bit_end);
+// fills and copies by kernels (lengths may be device counts; safe inside captured work)
+int fill_words(Context* ctx, uint32_t* p, DCount n, uint32_t v);
+int fill_words(Context* ctx, uint64_t* p, DCount n, uint64_t v);
+int copy_words(Context* ctx, const uint32_t* src, uint32_t* dst, DCount n);
+int copy_words(Context* ctx, const uint64_t* src, uint64_t* dst, DCount n);
 int radix_sort_dual_key(Context* ctx, uint32_t* key_low, uint32_t* key_high, uint32_t* vals, uint64_t n,
                         uint32_t low_bits, uint32_t high_bits);
 int last_voxels(Context* ctx, int32_t* ijk_dev, uint64_t cap, uint64_t* count);
@@ -209,7 +244,7 @@ int tile_bin(Context* ctx, const float* means2d, const int32_t* radii, const flo
 int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_t* results, uint32_t cap,
                   uint32_t* count, uint32_t* best_variant, uint32_t* best_digit_bits);
 int set_sort_config(Context* ctx, uint64_t from_count, uint32_t variant, uint32_t digit_bits);
-int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount n, uint32_t bit_begin, uint32_t bit_end,
                           uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out,
                           bool hist_ready = false);
 // allocation-free sort (vdb_prims.hip): the stable LSD sort of radix_sort_u64 over a file-scope
@@ -241,18 +276,38 @@ struct Node2BuildInput
     bool voxel_coords;
     uint32_t value_type;
     float voxel_size;
+    // device-count form (graph capture): the voxel count lives in nvox_dev (nvox is its bound),
+    // node counts stay on the device and every buffer is sized by the bounds below; a count past
+    // its bound or past max_nodes sets a bit in *flags_dev and the grid is not valid
+    const uint32_t* nvox_dev = nullptr;
+    uint64_t leaf_cap = 0, lower_cap = 0, upper_cap = 0;
+    uint32_t* flags_dev = nullptr;
 };
 struct Node2BuildOutput
 {
     uint8_t* grid; // context workspace (WS_GRID), valid until the next build on the context
-    uint64_t grid_size;
-    uint64_t upper_count, lower_count, leaf_count;
+    uint64_t grid_size; // bound of the grid size in the device-count form
+    uint64_t upper_count, lower_count, leaf_count; // bounds in the device-count form
+    const uint64_t* grid_size_dev = nullptr;       // device-count form: the grid size word
+    const uint32_t* counts_dev = nullptr;          // device-count form: upper, lower, leaf counts
 };
+// flag bits of the device-count pipeline (a count exceeded the bound its buffers were sized by)
+constexpr uint32_t GF_SEGMENTS = 1u, GF_PAIRS = 2u, GF_LEAVES = 4u, GF_VOXELS = 8u, GF_NODES = 16u, GF_ARENA = 32u,
+                   GF_RANGE = 64u;
 int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* out);
 
 // vdb_raster.hip
 int raster_node2(Context* ctx, const vdbr_gaussians_t* g, const vdbr_raster_params_t* p, void** grid_dev,
                  uint64_t* grid_size, vdbr_raster_stats_t* stats);
+// Batched raster3d as one captured HIP graph: the setup of every cloud runs first with one read
+// back of its counts; the remaining stages of all clouds (emission to grid, the grids copied into
+// an output arena of the context) are captured on the worker contexts' streams and launched as
+// one graph. Clouds that need several slabs or have no fragments, and clouds whose counts exceed
+// the bounds their captured buffers were sized by, are listed in *rerun (grid_dev left null) for
+// the caller to run through raster_node2; (*setup_done)[i] tells that cloud i's raw inputs were
+// preprocessed in place already.
+int raster_batch_graph(Context* ctx, const std::vector<Context*>& workers, vdbr_batch_item_t* items, uint32_t count,
+                       std::vector<uint32_t>* rerun, std::vector<uint8_t>* setup_done);
 int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, uint64_t* count);
 int preprocess_device(Context* ctx, uint64_t n, float* opacities, float* quats, float* scales, const float* sh0,
                       float* colors);

@@ -27,6 +27,69 @@
 namespace vdbr
 {
 
+// ============================================================================ fills and copies
+// Kernels instead of hipMemsetAsync / hipMemcpyAsync where the length may be a device count, and
+// in work that a caller may capture into a graph.
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, DCount cn, T v)
+{
+    const uint64_t n = cn.get();
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+template <typename T>
+__global__ void k_copy(const T* __restrict__ src, T* __restrict__ dst, DCount cn)
+{
+    const uint64_t n = cn.get();
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+static uint32_t stride_blocks(uint64_t n)
+{
+    const uint64_t b = (n + 255) / 256;
+    return (uint32_t)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+template <typename T>
+static int fill_impl(Context* ctx, T* p, DCount n, T v)
+{
+    if (!n.dev && n.cap == 0)
+        return 0;
+    hipLaunchKernelGGL((k_fill<T>), dim3(stride_blocks(n.cap)), dim3(256), 0, ctx->stream, p, n, v);
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+int fill_words(Context* ctx, uint32_t* p, DCount n, uint32_t v)
+{
+    return fill_impl<uint32_t>(ctx, p, n, v);
+}
+int fill_words(Context* ctx, uint64_t* p, DCount n, uint64_t v)
+{
+    return fill_impl<uint64_t>(ctx, p, n, v);
+}
+
+template <typename T>
+static int copy_impl(Context* ctx, const T* src, T* dst, DCount n)
+{
+    if (!n.dev && n.cap == 0)
+        return 0;
+    hipLaunchKernelGGL((k_copy<T>), dim3(stride_blocks(n.cap)), dim3(256), 0, ctx->stream, src, dst, n);
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+int copy_words(Context* ctx, const uint32_t* src, uint32_t* dst, DCount n)
+{
+    return copy_impl<uint32_t>(ctx, src, dst, n);
+}
+int copy_words(Context* ctx, const uint64_t* src, uint64_t* dst, DCount n)
+{
+    return copy_impl<uint64_t>(ctx, src, dst, n);
+}
+
 // ============================================================================ scans
 template <typename T, int OP>
 struct ScanOp;
@@ -101,11 +164,14 @@ __device__ __forceinline__ T block_scan_inclusive(T v, T* smem_waves, T* total)
 }
 
 template <typename T, int OP>
-__global__ __launch_bounds__(SCAN_THREADS) void k_tile_reduce(const T* __restrict__ in, uint64_t n, T* __restrict__ partials)
+__global__ __launch_bounds__(SCAN_THREADS) void k_tile_reduce(const T* __restrict__ in, DCount cn, T* __restrict__ partials)
 {
     using Op = ScanOp<T, OP>;
     __shared__ T smem[SCAN_THREADS / 64];
+    const uint64_t n = cn.get();
     uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    if (base >= n)
+        return; // block-uniform: the grid may be sized by a capacity
     T acc = Op::identity();
 #pragma unroll
     for (int m = 0; m < SCAN_ITEMS; m++)
@@ -122,10 +188,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_tile_reduce(const T* __restric
 
 // exclusive scan of the tile partials in place, one block
 template <typename T, int OP>
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(T* __restrict__ partials, uint64_t count, T* __restrict__ grand_total)
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(T* __restrict__ partials, DCount cn, T* __restrict__ grand_total)
 {
     using Op = ScanOp<T, OP>;
     __shared__ T smem[SCAN_THREADS / 64];
+    const uint64_t count = (cn.get() + SCAN_TILE - 1) / SCAN_TILE; // tiles
     T carry = Op::identity();
     for (uint64_t base = 0; base < count; base += SCAN_THREADS)
     {
@@ -150,13 +217,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_partials(T* __restrict__ 
 }
 
 template <typename T, int OP>
-__global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const T* __restrict__ in, T* __restrict__ out, uint64_t n,
+__global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const T* __restrict__ in, T* __restrict__ out, DCount cn,
                                                          const T* __restrict__ partials, int inclusive)
 {
     using Op = ScanOp<T, OP>;
     __shared__ T tile[SCAN_TILE];
     __shared__ T smem[SCAN_THREADS / 64];
+    const uint64_t n = cn.get();
     uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    if (base >= n)
+        return; // block-uniform
 #pragma unroll
     for (int m = 0; m < SCAN_ITEMS; m++)
     {
@@ -197,20 +267,29 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_tile_scan(const T* __restrict_
     }
 }
 
+// A count held on the device (cn.dev) sizes the launches by its bound and always runs the three
+// kernels (the partials pass writes the total, 0 for an empty input); a host count of 0 only
+// clears the total.
 template <typename T, int OP>
-int scan_impl(Context* ctx, const T* in, T* out, uint64_t n, bool inclusive, T* total_dev)
+int scan_impl(Context* ctx, const T* in, T* out, DCount cn, bool inclusive, T* total_dev)
 {
-    if (n == 0)
+    if (!cn.dev && cn.cap == 0)
     {
         if (total_dev)
-            VDBR_CHECK(ctx, hipMemsetAsync(total_dev, 0, sizeof(T), ctx->stream));
+            return fill_words(ctx, total_dev, host_count(1), 0);
         return 0;
     }
-    uint64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    uint64_t tiles = (cn.cap + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles == 0)
+        tiles = 1;
+    if (tiles > 0xFFFFFFFFull)
+        return ctx->fail("scan: too many tiles");
     T* partials = (T*)ctx->scratch_scan(tiles * sizeof(T));
-    hipLaunchKernelGGL((k_tile_reduce<T, OP>), dim3((uint32_t)tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, n, partials);
-    hipLaunchKernelGGL((k_scan_partials<T, OP>), dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, partials, tiles, total_dev);
-    hipLaunchKernelGGL((k_tile_scan<T, OP>), dim3((uint32_t)tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, out, n,
+    if (!partials)
+        return -1;
+    hipLaunchKernelGGL((k_tile_reduce<T, OP>), dim3((uint32_t)tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, cn, partials);
+    hipLaunchKernelGGL((k_scan_partials<T, OP>), dim3(1), dim3(SCAN_THREADS), 0, ctx->stream, partials, cn, total_dev);
+    hipLaunchKernelGGL((k_tile_scan<T, OP>), dim3((uint32_t)tiles), dim3(SCAN_THREADS), 0, ctx->stream, in, out, cn,
                        (const T*)partials, inclusive ? 1 : 0);
     VDBR_CHECK(ctx, hipGetLastError());
     return 0;
@@ -218,15 +297,23 @@ int scan_impl(Context* ctx, const T* in, T* out, uint64_t n, bool inclusive, T* 
 
 int scan_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive, uint32_t* total_dev)
 {
-    return scan_impl<uint32_t, 0>(ctx, in, out, n, inclusive, total_dev);
+    return scan_impl<uint32_t, 0>(ctx, in, out, host_count(n), inclusive, total_dev);
 }
 int scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, uint64_t n, bool inclusive, uint64_t* total_dev)
+{
+    return scan_impl<uint64_t, 1>(ctx, in, out, host_count(n), inclusive, total_dev);
+}
+int scan_u32(Context* ctx, const uint32_t* in, uint32_t* out, DCount n, bool inclusive, uint32_t* total_dev)
+{
+    return scan_impl<uint32_t, 0>(ctx, in, out, n, inclusive, total_dev);
+}
+int scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, DCount n, bool inclusive, uint64_t* total_dev)
 {
     return scan_impl<uint64_t, 1>(ctx, in, out, n, inclusive, total_dev);
 }
 int scan_max_u32(Context* ctx, const uint32_t* in, uint32_t* out, uint64_t n, bool inclusive)
 {
-    return scan_impl<uint32_t, 2>(ctx, in, out, n, inclusive, nullptr);
+    return scan_impl<uint32_t, 2>(ctx, in, out, host_count(n), inclusive, nullptr);
 }
 
 // ============================================================================ radix sort
@@ -256,9 +343,10 @@ __device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint64_t flag, uint6
 // Each thread counts 16 consecutive keys and adds runs of equal digits at once: the leading
 // digits of neighbouring pair keys are mostly equal, so per-key LDS atomics would serialise.
 constexpr int HIST_RUN = 16;
-__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, uint64_t n, uint32_t bit_begin,
+__global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ keys, DCount cn, uint32_t bit_begin,
                                                    uint32_t bit_count, uint32_t W, unsigned long long* __restrict__ hist)
 {
+    const uint64_t n = cn.get();
     const uint32_t passes = (bit_count + W - 1u) / W;
     __shared__ uint32_t h[SORT_MAX_PASSES][256];
     for (int p = 0; p < SORT_MAX_PASSES; p++)
@@ -337,7 +425,7 @@ __global__ __launch_bounds__(256) void k_sort_digit_starts(const unsigned long l
 template <int SORT_THREADS, int SORT_ITEMS, bool HAS_VALS>
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                             const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
-                                                            uint64_t n, uint32_t shift, uint32_t digit_bits,
+                                                            DCount cn, uint32_t shift, uint32_t digit_bits,
                                                             const uint64_t* __restrict__ digit_start,
                                                             uint64_t* __restrict__ status, uint32_t* __restrict__ tile_counter,
                                                             uint32_t epoch)
@@ -363,6 +451,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
         (&wcnt[0][0])[x] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
+    const uint64_t n = cn.get();
+    if ((uint64_t)tile * SORT_TILE >= n)
+        return; // block-uniform: a grid sized by a capacity has tiles past the keys
     const uint64_t base = (uint64_t)tile * SORT_TILE + (uint64_t)wave * SORT_WAVE_TILE;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t dmask = (1u << digit_bits) - 1u;
@@ -554,7 +645,7 @@ static uint64_t sort_tile(uint32_t variant)
 
 template <bool V>
 static void launch_sort_pass(uint32_t variant, uint64_t tiles, hipStream_t st, const uint64_t* kin, uint64_t* kout,
-                             const uint32_t* vin, uint32_t* vout, uint64_t n, uint32_t shift, uint32_t dbits,
+                             const uint32_t* vin, uint32_t* vout, DCount n, uint32_t shift, uint32_t dbits,
                              const uint64_t* starts, uint64_t* status, uint32_t* counter, uint32_t epoch)
 {
     switch (variant)
@@ -592,11 +683,13 @@ static void launch_sort_pass(uint32_t variant, uint64_t tiles, hipStream_t st, c
     }
 }
 
-int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
-                   SortTiming* timing)
+// Stable LSD sort of keys (and a u32 payload) over [bit_begin, bit_end). With a device count the
+// launches are sized by its bound and the kernels sort the actual count; no host synchronisation.
+int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, DCount cn, uint32_t bit_begin, uint32_t bit_end)
 {
-    if (n <= 1 || bit_end <= bit_begin)
+    if ((!cn.dev && cn.cap <= 1) || bit_end <= bit_begin || cn.cap == 0)
         return 0;
+    const uint64_t n = cn.cap;
     uint32_t bits = bit_end - bit_begin;
     const SortConfig cfg = sort_config(ctx, n, vals != nullptr);
     const uint32_t W = cfg.digit_bits;
@@ -609,12 +702,13 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
     SortWorkspace ws;
     if (ctx->sort_workspace(n, vals != nullptr, tiles, true, &ws))
         return -1;
-    VDBR_CHECK(ctx, hipMemsetAsync(ws.hist, 0, SORT_MAX_PASSES * 256 * sizeof(uint64_t), ctx->stream));
-    VDBR_CHECK(ctx, hipMemsetAsync(ws.tile_counters, 0, SORT_MAX_PASSES * sizeof(uint32_t), ctx->stream));
+    if (fill_words(ctx, ws.hist, host_count(SORT_MAX_PASSES * 256), 0ull) ||
+        fill_words(ctx, ws.tile_counters, host_count(SORT_MAX_PASSES), 0u))
+        return -1;
     uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
     if (hist_blocks > 2048u)
         hist_blocks = 2048u;
-    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits, W,
+    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, cn, bit_begin, bits, W,
                        (unsigned long long*)ws.hist);
     hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
                        passes, ws.starts);
@@ -628,10 +722,10 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
         uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
         uint32_t epoch = ctx->next_sort_epoch(tiles);
         if (vals)
-            launch_sort_pass<true>(cfg.variant, tiles, ctx->stream, kin, kout, vin, vout, n, shift, dbits,
+            launch_sort_pass<true>(cfg.variant, tiles, ctx->stream, kin, kout, vin, vout, cn, shift, dbits,
                                    ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
         else
-            launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, n, shift, dbits,
+            launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, cn, shift, dbits,
                                     ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
         uint64_t* tk = kin;
         kin = kout;
@@ -642,13 +736,20 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uin
     }
     if (kin != keys)
     {
-        VDBR_CHECK(ctx, hipMemcpyAsync(keys, kin, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
-        if (vals)
-            VDBR_CHECK(ctx, hipMemcpyAsync(vals, vin, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+        if (copy_words(ctx, (const uint64_t*)kin, keys, cn))
+            return -1;
+        if (vals && copy_words(ctx, (const uint32_t*)vin, vals, cn))
+            return -1;
     }
-    (void)timing;
     VDBR_CHECK(ctx, hipGetLastError());
     return 0;
+}
+
+int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+                   SortTiming* timing)
+{
+    (void)timing;
+    return radix_sort_u64(ctx, keys, vals, host_count(n), bit_begin, bit_end);
 }
 
 __global__ void k_widen_u32(const uint32_t* __restrict__ in, uint64_t n, uint64_t* __restrict__ out)
@@ -764,15 +865,16 @@ unsigned long long* sort_hist_buffer(Context* ctx)
     return (unsigned long long*)ctx->ws(WS_SORT_SMALL, 16 * 256 * 8 * 2 + 64);
 }
 
-int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit_begin, uint32_t bit_end,
+int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_begin, uint32_t bit_end,
                           uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out,
                           bool hist_ready)
 {
     *result_keys = keys;
     if (passes_out)
         *passes_out = 0;
-    if (n <= 1 || bit_end <= bit_begin)
+    if ((!cn.dev && cn.cap <= 1) || bit_end <= bit_begin || cn.cap == 0)
         return 0;
+    const uint64_t n = cn.cap;
     uint32_t bits = bit_end - bit_begin;
     const SortConfig cfg = sort_config(ctx, n, false);
     const uint32_t W = cfg.digit_bits;
@@ -785,34 +887,36 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, uint64_t n, uint32_t bit
     SortWorkspace ws;
     if (ctx->sort_workspace(n, false, tiles, false, &ws))
         return -1;
-    VDBR_CHECK(ctx, hipMemsetAsync(ws.tile_counters, 0, SORT_MAX_PASSES * sizeof(uint32_t), ctx->stream));
+    if (fill_words(ctx, ws.tile_counters, host_count(SORT_MAX_PASSES), 0u))
+        return -1;
     if (!hist_ready)
     {
-        VDBR_CHECK(ctx, hipMemsetAsync(ws.hist, 0, SORT_MAX_PASSES * 256 * sizeof(uint64_t), ctx->stream));
+        if (fill_words(ctx, ws.hist, host_count(SORT_MAX_PASSES * 256), 0ull))
+            return -1;
         uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
         if (hist_blocks > 2048u)
             hist_blocks = 2048u;
-        hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, n, bit_begin, bits, W,
+        hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, cn, bit_begin, bits, W,
                            (unsigned long long*)ws.hist);
     }
     hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
                        passes, ws.starts);
     uint64_t* kin = keys;
     uint64_t* kout = ws.keys_alt;
-    if (passes_begin)
+    if (passes_begin && !ctx->capturing)
         VDBR_CHECK(ctx, hipEventRecord(passes_begin, ctx->stream));
     for (uint32_t p = 0; p < passes; p++)
     {
         uint32_t shift = bit_begin + W * p;
         uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
         uint32_t epoch = ctx->next_sort_epoch(tiles);
-        launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, n, shift, dbits,
+        launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, cn, shift, dbits,
                                 ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;
     }
-    if (passes_end)
+    if (passes_end && !ctx->capturing)
         VDBR_CHECK(ctx, hipEventRecord(passes_end, ctx->stream));
     if (passes_out)
         *passes_out = passes;
@@ -884,7 +988,7 @@ int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_
                     rc = ctx->fail("autotune: copy or event failed");
                     break;
                 }
-                if (radix_sort_u64_noswap(ctx, work, n, 0, key_bits, &out, nullptr, nullptr, nullptr))
+                if (radix_sort_u64_noswap(ctx, work, host_count(n), 0, key_bits, &out, nullptr, nullptr, nullptr))
                 {
                     rc = -1;
                     break;
@@ -1058,7 +1162,7 @@ static int noalloc_sort_on_device(int dev, uint64_t* keys, uint32_t* vals, uint6
     uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
     if (hist_blocks > 2048u)
         hist_blocks = 2048u;
-    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, st, keys, n, bit_begin, bits, W, hist);
+    hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, st, keys, host_count(n), bit_begin, bits, W, hist);
     hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, st, (const unsigned long long*)hist, passes, starts);
     uint64_t* kin = keys;
     uint64_t* kout = keys_alt;
@@ -1070,10 +1174,10 @@ static int noalloc_sort_on_device(int dev, uint64_t* keys, uint32_t* vals, uint6
         const uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
         const uint32_t epoch = ++d.epoch;
         if (vals)
-            launch_sort_pass<true>(NOALLOC_VARIANT, tiles, st, kin, kout, vin, vout, n, shift, dbits, starts + p * 256,
+            launch_sort_pass<true>(NOALLOC_VARIANT, tiles, st, kin, kout, vin, vout, host_count(n), shift, dbits, starts + p * 256,
                                    status, counters + p, epoch);
         else
-            launch_sort_pass<false>(NOALLOC_VARIANT, tiles, st, kin, kout, nullptr, nullptr, n, shift, dbits,
+            launch_sort_pass<false>(NOALLOC_VARIANT, tiles, st, kin, kout, nullptr, nullptr, host_count(n), shift, dbits,
                                     starts + p * 256, status, counters + p, epoch);
         std::swap(kin, kout);
         std::swap(vin, vout);

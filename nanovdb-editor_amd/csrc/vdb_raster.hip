@@ -19,6 +19,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 #include "vdb_device.h"
@@ -287,6 +288,7 @@ struct EmitArgs
     const PrimBox* boxes;
     const PrimVal* vals; // alpha of each pair computed here when kl.alpha_bits == 8
     const uint64_t* row_scan; // exclusive scan of rows, n + 1 entries
+    const struct RowFrame* rframes; // per-Gaussian row-interval constants (row pass)
     uint64_t n;
     uint64_t total_rows;
     float vs;
@@ -634,6 +636,120 @@ __device__ __forceinline__ void walk_row(const PrimFrame& f, int32_t i, int32_t 
     }
 }
 
+// Per-Gaussian constants of row_interval: with c0(i, j) = P + i Ai + j Aj the local-frame centre
+// of voxel k = 0 of row (i, j) (the terms of row_interval's C0 regrouped), B = bc the step per
+// voxel along k, R^2 the squared culling radius (1 + |H| + delta)^2, qa = |B|^2.
+struct __align__(16) RowFrame
+{
+    double P[3], Ai[3], Aj[3], bc[3];
+    double R2, qa, inv2qa, pad;
+};
+static_assert(sizeof(RowFrame) == 128, "one cache line per Gaussian");
+
+__global__ __launch_bounds__(256) void k_row_frames(uint64_t n, const PrimFrame* __restrict__ frames,
+                                                    const PrimBox* __restrict__ boxes, float vs,
+                                                    RowFrame* __restrict__ out)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n)
+        return;
+    const PrimFrame f = frames[g];
+    const PrimBox b = boxes[g];
+    const double vsd = (double)vs;
+    double A[3][3];
+    double H2 = 0.0, hmax = 0.0;
+    const double w = (double)f.inv_r;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        const double sc = w * (double)f.inv_s[c];
+        double hs = 0.0;
+#pragma unroll
+        for (int bb = 0; bb < 3; bb++)
+        {
+            A[c][bb] = (double)f.rinv[3 * c + bb] * sc;
+            hs += fabs(A[c][bb]);
+        }
+        const double h = 0.5 * vsd * hs;
+        H2 += h * h;
+        hmax = fmax(hmax, sc);
+    }
+    // the same bound as row_interval: magnitude of the corner transforms' world-space operands and
+    // the operation-count error bound of the FP32 predicate
+    const double M = fmax(fmax(fabs((double)f.mu[0]), fabs((double)f.mu[1])), fabs((double)f.mu[2])) +
+                     vsd * (double)max(max(b.rng_x, b.rng_y), b.rng_z) + vsd;
+    const double u = 5.9604644775390625e-08; // 2^-24
+    const double H = sqrt(H2);
+    const double delta = 2.0 * (hmax * 10.4 * u * M + 2.0 * u * (1.0 + 2.0 * H)) * 1.7320508075688772 + 1e-6;
+    const double R = 1.0 + H + delta;
+    RowFrame r;
+    const double dx0 = 0.5 * vsd - (double)f.mu[0];
+    const double dy0 = 0.5 * vsd - (double)f.mu[1];
+    const double dz = 0.5 * vsd - (double)f.mu[2];
+    double qa = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        r.P[c] = A[c][0] * dx0 + A[c][1] * dy0 + A[c][2] * dz;
+        r.Ai[c] = A[c][0] * vsd;
+        r.Aj[c] = A[c][1] * vsd;
+        r.bc[c] = vsd * A[c][2];
+        qa += r.bc[c] * r.bc[c];
+    }
+    r.R2 = R * R;
+    r.qa = qa;
+    r.inv2qa = qa > 0.0 ? 0.5 / qa : 0.0;
+    r.pad = 0.0;
+    out[g] = r;
+}
+
+// row_interval from the Gaussian's RowFrame: the quadratic |c0 + k B|^2 <= R^2 in k solved in
+// double, the roots widened by one voxel each side and clipped to the row. The regrouped terms
+// round differently from row_interval by a few ulp of double, far inside that widening and the
+// error bound in R, so every voxel the FP32 predicate keeps stays a candidate.
+__device__ __forceinline__ void row_interval_frame(const RowFrame& rf, int32_t i, int32_t j, int32_t* k0, int32_t* len)
+{
+    const double di = (double)i, dj = (double)j;
+    double qb = 0.0, qc = -rf.R2;
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        const double c0 = rf.P[c] + di * rf.Ai[c] + dj * rf.Aj[c];
+        qb += 2.0 * c0 * rf.bc[c];
+        qc += c0 * c0;
+    }
+    int32_t lo = *k0, hi = *k0 + *len - 1;
+    if (rf.qa > 0.0)
+    {
+        const double disc = qb * qb - 4.0 * rf.qa * qc;
+        if (disc < 0.0)
+        {
+            *len = 0;
+            return;
+        }
+        const double sq = sqrt(disc);
+        const double r1 = (-qb - sq) * rf.inv2qa;
+        const double r2 = (-qb + sq) * rf.inv2qa;
+        if (r1 > (double)hi + 1.0 || r2 < (double)lo - 1.0)
+        {
+            *len = 0;
+            return;
+        }
+        const double fl = floor(r1) - 1.0, ce = ceil(r2) + 1.0;
+        if (fl > (double)lo)
+            lo = (int32_t)fl;
+        if (ce < (double)hi)
+            hi = (int32_t)ce;
+    }
+    else if (qc > 0.0)
+    {
+        *len = 0;
+        return;
+    }
+    *k0 = lo;
+    *len = hi >= lo ? hi - lo + 1 : 0;
+}
+
 // Row -> (Gaussian, i, j, candidate interval): the Gaussian owning the row (binary search of the
 // row scan between the block's first and last Gaussian), its frame, and the pruned k interval.
 // rows of a Gaussian are its AABB's (i, j) pairs with i >= i_lo (the first voxel column of the
@@ -695,11 +811,19 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
         const uint64_t row = row0 + (uint64_t)chunk * 64u + lane;
         if (row0 + (uint64_t)chunk * 64u >= a.total_rows)
             break;
-        uint64_t g;
-        int32_t i, j, k0, len;
-        PrimFrame f;
-        row_candidates(a.row_scan, a.frames, a.boxes, row, a.total_rows, glo, ghi, a.vs, &g, &f, &i, &j, &k0, &len,
-                       a.slab_i0);
+        uint64_t g = 0;
+        int32_t i = 0, j = 0, k0 = 0, len = 0;
+        if (row < a.total_rows)
+        {
+            g = find_prim(a.row_scan, glo, ghi, row);
+            const PrimBox b = a.boxes[g];
+            const uint64_t lr = row - a.row_scan[g];
+            i = max(b.mn[0], a.slab_i0) + (int32_t)(lr / (uint64_t)b.rng_y);
+            j = b.mn[1] + (int32_t)(lr % (uint64_t)b.rng_y);
+            k0 = b.mn[2];
+            len = b.rng_z;
+            row_interval_frame(a.rframes[g], i, j, &k0, &len);
+        }
         const uint32_t nseg = len > 0 ? (uint32_t)((len + EMIT_SEG - 1) / EMIT_SEG) : 0u;
         uint32_t total, cand_total;
         const uint32_t excl = wave_excl_add(nseg, lane, &total);
@@ -1909,29 +2033,35 @@ constexpr int HEAD_ITEMS = 16;
 constexpr int HEAD_TILE = HEAD_THREADS * HEAD_ITEMS;
 
 // counts positions i where (key[i] >> shift) differs from (key[i-1] >> shift)
-__global__ __launch_bounds__(HEAD_THREADS) void k_head_count(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
+__global__ __launch_bounds__(HEAD_THREADS) void k_head_count(const uint64_t* __restrict__ keys, DCount cn, uint32_t shift,
                                                              uint32_t* __restrict__ tile_counts)
 {
+    const uint64_t n = cn.get();
+    const uint64_t tiles = (n + HEAD_TILE - 1) / HEAD_TILE;
     __shared__ uint32_t cnt;
-    if (threadIdx.x == 0)
-        cnt = 0u;
-    __syncthreads();
-    uint64_t base = (uint64_t)blockIdx.x * HEAD_TILE;
-    uint32_t local = 0u;
-    for (int m = 0; m < HEAD_ITEMS; m++)
+    // blocks stride over the tiles (a grid sized by a capacity stays bounded)
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
-        uint64_t idx = base + (uint64_t)m * HEAD_THREADS + threadIdx.x;
-        if (idx < n)
+        if (threadIdx.x == 0)
+            cnt = 0u;
+        __syncthreads();
+        const uint64_t base = tile * HEAD_TILE;
+        uint32_t local = 0u;
+        for (int m = 0; m < HEAD_ITEMS; m++)
         {
-            uint64_t v = keys[idx] >> shift;
-            bool head = idx == 0 || (keys[idx - 1] >> shift) != v;
-            local += head ? 1u : 0u;
+            uint64_t idx = base + (uint64_t)m * HEAD_THREADS + threadIdx.x;
+            if (idx < n)
+            {
+                uint64_t v = keys[idx] >> shift;
+                bool head = idx == 0 || (keys[idx - 1] >> shift) != v;
+                local += head ? 1u : 0u;
+            }
         }
+        atomicAdd(&cnt, local);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            tile_counts[tile] = cnt;
     }
-    atomicAdd(&cnt, local);
-    __syncthreads();
-    if (threadIdx.x == 0)
-        tile_counts[blockIdx.x] = cnt;
 }
 
 // Same tile counts as k_head_count when key >> shift is non-decreasing (keys sorted over every
@@ -1940,137 +2070,148 @@ __global__ __launch_bounds__(HEAD_THREADS) void k_head_count(const uint64_t* __r
 // differ, after a first check of the whole tile. One wave per HEAD_TILE tile.
 constexpr int HEAD_SEG = 64;
 static_assert(HEAD_TILE == 64 * HEAD_SEG, "one segment per lane");
-__global__ __launch_bounds__(256) void k_head_count_sorted(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
-                                                           uint64_t tiles, uint32_t* __restrict__ tile_counts)
+__global__ __launch_bounds__(256) void k_head_count_sorted(const uint64_t* __restrict__ keys, DCount cn, uint32_t shift,
+                                                           uint32_t* __restrict__ tile_counts)
 {
+    const uint64_t n = cn.get();
+    const uint64_t tiles = (n + HEAD_TILE - 1) / HEAD_TILE;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t tile = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (tile >= tiles)
-        return; // wave-uniform
-    const uint64_t t0 = tile * HEAD_TILE;
-    const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
-    // a tile whose last value equals the value before it holds no head (most tiles of long runs)
-    if (t0 > 0 && (keys[t0 - 1] >> shift) == (keys[t1 - 1] >> shift))
+    // waves stride over the tiles (a grid sized by a capacity stays bounded)
+    for (uint64_t tile = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6); tile < tiles; tile += (uint64_t)gridDim.x * 4u)
     {
+        const uint64_t t0 = tile * HEAD_TILE;
+        const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
+        // a tile whose last value equals the value before it holds no head (most tiles of long runs)
+        if (t0 > 0 && (keys[t0 - 1] >> shift) == (keys[t1 - 1] >> shift))
+        {
+            if (lane == 0u)
+                tile_counts[tile] = 0u;
+            continue; // wave-uniform
+        }
+        const uint64_t s = t0 + (uint64_t)lane * HEAD_SEG;
+        const bool live = s < t1;
+        const uint64_t e = s + HEAD_SEG < t1 ? s + HEAD_SEG : t1;
+        const uint64_t vend = live ? keys[e - 1] >> shift : 0ull;
+        // value just before each segment: the previous lane's segment end, key[t0 - 1] for lane 0
+        uint64_t vbefore = wave_shr1_u64(0ull, vend);
+        if (lane == 0u && t0 > 0)
+            vbefore = keys[t0 - 1] >> shift;
+        const bool first = lane == 0u && t0 == 0; // position 0 is always a head
+        uint64_t todo = __ballot(live && (first || vbefore != vend));
+        uint32_t count = 0u;
+        while (todo)
+        {
+            const int l = __builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            const uint64_t ss = t0 + (uint64_t)l * HEAD_SEG;
+            const uint64_t idx = ss + lane;
+            const bool valid = idx < t1;
+            const uint64_t v = valid ? keys[idx] >> shift : 0ull;
+            uint64_t pv = wave_shr1_u64(0ull, v);
+            const uint64_t before_l = readlane_u64(vbefore, l);
+            if (lane == 0u)
+                pv = before_l;
+            const bool head = valid && (idx == 0 || v != pv);
+            count += (uint32_t)__popcll(__ballot(head));
+        }
         if (lane == 0u)
-            tile_counts[tile] = 0u;
-        return; // wave-uniform
+            tile_counts[tile] = count;
     }
-    const uint64_t s = t0 + (uint64_t)lane * HEAD_SEG;
-    const bool live = s < t1;
-    const uint64_t e = s + HEAD_SEG < t1 ? s + HEAD_SEG : t1;
-    const uint64_t vend = live ? keys[e - 1] >> shift : 0ull;
-    // value just before each segment: the previous lane's segment end, key[t0 - 1] for lane 0
-    uint64_t vbefore = wave_shr1_u64(0ull, vend);
-    if (lane == 0u && t0 > 0)
-        vbefore = keys[t0 - 1] >> shift;
-    const bool first = lane == 0u && t0 == 0; // position 0 is always a head
-    uint64_t todo = __ballot(live && (first || vbefore != vend));
-    uint32_t count = 0u;
-    while (todo)
-    {
-        const int l = __builtin_ctzll(todo);
-        todo &= todo - 1ull;
-        const uint64_t ss = t0 + (uint64_t)l * HEAD_SEG;
-        const uint64_t idx = ss + lane;
-        const bool valid = idx < t1;
-        const uint64_t v = valid ? keys[idx] >> shift : 0ull;
-        uint64_t pv = wave_shr1_u64(0ull, v);
-        const uint64_t before_l = readlane_u64(vbefore, l);
-        if (lane == 0u)
-            pv = before_l;
-        const bool head = valid && (idx == 0 || v != pv);
-        count += (uint32_t)__popcll(__ballot(head));
-    }
-    if (lane == 0u)
-        tile_counts[tile] = count;
 }
 
 // run starts of sorted values in order (starts[rank] = index of the run head), visiting only the
 // 64-key segments that hold a head, as k_head_count_sorted counts them; tile_offsets is the
 // exclusive scan of those counts
-__global__ __launch_bounds__(256) void k_head_write_sorted(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
-                                                           uint64_t tiles, const uint32_t* __restrict__ tile_offsets,
+__global__ __launch_bounds__(256) void k_head_write_sorted(const uint64_t* __restrict__ keys, DCount cn, uint32_t shift,
+                                                           const uint32_t* __restrict__ tile_offsets,
                                                            uint32_t* __restrict__ starts)
 {
+    const uint64_t n = cn.get();
+    const uint64_t tiles = (n + HEAD_TILE - 1) / HEAD_TILE;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t tile = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (tile >= tiles)
-        return; // wave-uniform
-    const uint64_t t0 = tile * HEAD_TILE;
-    const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
-    if (t0 > 0 && (keys[t0 - 1] >> shift) == (keys[t1 - 1] >> shift))
-        return; // wave-uniform: no head in this tile
-    const uint64_t s = t0 + (uint64_t)lane * HEAD_SEG;
-    const bool live = s < t1;
-    const uint64_t e = s + HEAD_SEG < t1 ? s + HEAD_SEG : t1;
-    const uint64_t vend = live ? keys[e - 1] >> shift : 0ull;
-    uint64_t vbefore = wave_shr1_u64(0ull, vend);
-    if (lane == 0u && t0 > 0)
-        vbefore = keys[t0 - 1] >> shift;
-    const bool first = lane == 0u && t0 == 0;
-    uint64_t todo = __ballot(live && (first || vbefore != vend));
-    uint32_t rank = tile_offsets[tile];
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
-    while (todo)
+    // waves stride over the tiles (a grid sized by a capacity stays bounded)
+    for (uint64_t tile = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6); tile < tiles; tile += (uint64_t)gridDim.x * 4u)
     {
-        const int l = __builtin_ctzll(todo);
-        todo &= todo - 1ull;
-        const uint64_t idx = t0 + (uint64_t)l * HEAD_SEG + lane;
-        const bool valid = idx < t1;
-        const uint64_t v = valid ? keys[idx] >> shift : 0ull;
-        uint64_t pv = wave_shr1_u64(0ull, v);
-        const uint64_t before_l = readlane_u64(vbefore, l);
-        if (lane == 0u)
-            pv = before_l;
-        const bool head = valid && (idx == 0 || v != pv);
-        const uint64_t hb = __ballot(head);
-        if (head)
-            starts[rank + (uint32_t)__popcll(hb & lt_mask)] = (uint32_t)idx;
-        rank += (uint32_t)__popcll(hb);
+        const uint64_t t0 = tile * HEAD_TILE;
+        const uint64_t t1 = t0 + HEAD_TILE < n ? t0 + HEAD_TILE : n;
+        if (t0 > 0 && (keys[t0 - 1] >> shift) == (keys[t1 - 1] >> shift))
+            continue; // wave-uniform: no head in this tile
+        const uint64_t s = t0 + (uint64_t)lane * HEAD_SEG;
+        const bool live = s < t1;
+        const uint64_t e = s + HEAD_SEG < t1 ? s + HEAD_SEG : t1;
+        const uint64_t vend = live ? keys[e - 1] >> shift : 0ull;
+        uint64_t vbefore = wave_shr1_u64(0ull, vend);
+        if (lane == 0u && t0 > 0)
+            vbefore = keys[t0 - 1] >> shift;
+        const bool first = lane == 0u && t0 == 0;
+        uint64_t todo = __ballot(live && (first || vbefore != vend));
+        uint32_t rank = tile_offsets[tile];
+        const uint64_t lt_mask = (1ull << lane) - 1ull;
+        while (todo)
+        {
+            const int l = __builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            const uint64_t idx = t0 + (uint64_t)l * HEAD_SEG + lane;
+            const bool valid = idx < t1;
+            const uint64_t v = valid ? keys[idx] >> shift : 0ull;
+            uint64_t pv = wave_shr1_u64(0ull, v);
+            const uint64_t before_l = readlane_u64(vbefore, l);
+            if (lane == 0u)
+                pv = before_l;
+            const bool head = valid && (idx == 0 || v != pv);
+            const uint64_t hb = __ballot(head);
+            if (head)
+                starts[rank + (uint32_t)__popcll(hb & lt_mask)] = (uint32_t)idx;
+            rank += (uint32_t)__popcll(hb);
+        }
     }
 }
 
 // writes run starts in order: starts[rank] = index of the run head
-__global__ __launch_bounds__(HEAD_THREADS) void k_head_write(const uint64_t* __restrict__ keys, uint64_t n, uint32_t shift,
+__global__ __launch_bounds__(HEAD_THREADS) void k_head_write(const uint64_t* __restrict__ keys, DCount cn, uint32_t shift,
                                                              const uint32_t* __restrict__ tile_offsets,
                                                              uint32_t* __restrict__ starts)
 {
+    const uint64_t n = cn.get();
+    const uint64_t tiles = (n + HEAD_TILE - 1) / HEAD_TILE;
     __shared__ uint32_t wave_tot[HEAD_THREADS / 64];
     __shared__ uint32_t run_base;
-    uint64_t base = (uint64_t)blockIdx.x * HEAD_TILE;
-    if (threadIdx.x == 0)
-        run_base = tile_offsets[blockIdx.x];
-    __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    for (int m = 0; m < HEAD_ITEMS; m++)
+    for (uint64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
     {
-        uint64_t idx = base + (uint64_t)m * HEAD_THREADS + threadIdx.x;
-        bool head = false;
-        if (idx < n)
-        {
-            uint64_t v = keys[idx] >> shift;
-            head = idx == 0 || (keys[idx - 1] >> shift) != v;
-        }
-        uint64_t bal = __ballot(head);
-        if (lane == 0)
-            wave_tot[wave] = (uint32_t)__popcll(bal);
-        __syncthreads();
-        uint32_t pre = run_base;
-        for (uint32_t w = 0; w < wave; w++)
-            pre += wave_tot[w];
-        if (head)
-            starts[pre + (uint32_t)__popcll(bal & lt_mask)] = (uint32_t)idx;
-        __syncthreads();
+        const uint64_t base = tile * HEAD_TILE;
         if (threadIdx.x == 0)
-        {
-            uint32_t t = 0u;
-            for (uint32_t w = 0; w < HEAD_THREADS / 64; w++)
-                t += wave_tot[w];
-            run_base += t;
-        }
+            run_base = tile_offsets[tile];
         __syncthreads();
+        for (int m = 0; m < HEAD_ITEMS; m++)
+        {
+            uint64_t idx = base + (uint64_t)m * HEAD_THREADS + threadIdx.x;
+            bool head = false;
+            if (idx < n)
+            {
+                uint64_t v = keys[idx] >> shift;
+                head = idx == 0 || (keys[idx - 1] >> shift) != v;
+            }
+            uint64_t bal = __ballot(head);
+            if (lane == 0)
+                wave_tot[wave] = (uint32_t)__popcll(bal);
+            __syncthreads();
+            uint32_t pre = run_base;
+            for (uint32_t w = 0; w < wave; w++)
+                pre += wave_tot[w];
+            if (head)
+                starts[pre + (uint32_t)__popcll(bal & lt_mask)] = (uint32_t)idx;
+            __syncthreads();
+            if (threadIdx.x == 0)
+            {
+                uint32_t t = 0u;
+                for (uint32_t w = 0; w < HEAD_THREADS / 64; w++)
+                    t += wave_tot[w];
+                run_base += t;
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -2092,7 +2233,7 @@ constexpr int RED_ITEMS = HEAD_TILE / RED_THREADS; // one run-head tile per bloc
 constexpr int LR_LEAVES = 8;
 
 template <bool ALPHA_IN_KEY>
-__global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t* __restrict__ keys, uint64_t n,
+__global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t* __restrict__ keys, DCount cn,
                                                                   KeyLayout kl, float vs,
                                                                   const uint32_t* __restrict__ tile_offsets,
                                                                   const PrimFrame* __restrict__ frames,
@@ -2108,7 +2249,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t
     __shared__ uint32_t wave_heads[RED_THREADS / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t gmask = (1ull << gid_bits) - 1ull;
+    const uint64_t n = cn.get();
     const uint64_t t0 = (uint64_t)blockIdx.x * HEAD_TILE;
+    if (t0 >= n)
+        return; // block-uniform: the grid may be sized by a capacity
     const uint64_t pb = t0 + (uint64_t)threadIdx.x * RED_ITEMS;
     const uint32_t tile_base = tile_offsets[blockIdx.x];
     // the tile's first pair continues a leaf of the previous tile
@@ -2242,10 +2386,129 @@ __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t
     }
 }
 
+// RGBA8 by leaf chunks (the alpha in the key): one block per chunk of at most LRC_CHUNK pairs of
+// one leaf, the chunks listed from the leaf starts as for the density reduce. Pass 1 folds the
+// maximum alpha and the lowest Gaussian id of every slot into LDS; pass 2 reads the chunk's keys
+// again (from L2) and gathers colours only for pairs holding their slot's maximum alpha. A leaf of
+// one chunk (most leaves) stores its 512 slots with plain stores; the chunks of a longer leaf
+// merge into the prefilled staging arrays with atomicMax / atomicMin (order independent).
+constexpr uint32_t LRC_CHUNK = 16384;
+constexpr int LRC_UNROLL = 4;
+
+__global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba_chunks(const uint64_t* __restrict__ keys, KeyLayout kl,
+                                                                         const uint32_t* __restrict__ starts,
+                                                                         const uint32_t* __restrict__ item_scan,
+                                                                         DCount cnleaf, const PrimVal* __restrict__ vals,
+                                                                         uint32_t* __restrict__ lval,
+                                                                         uint32_t* __restrict__ lg0,
+                                                                         uint64_t* __restrict__ lkey)
+{
+    __shared__ uint32_t s_val[512];
+    __shared__ uint32_t s_g0[512];
+    __shared__ uint32_t s_leaf;
+    const uint64_t nleaf = cnleaf.get();
+    const uint32_t item = blockIdx.x;
+    if (item >= item_scan[nleaf])
+        return; // block-uniform: the grid is an upper bound of the chunk count
+    if (threadIdx.x == 0)
+    {
+        // the leaf of this chunk: largest l with item_scan[l] <= item
+        uint32_t lo = 0, hi = (uint32_t)nleaf - 1u;
+        while (lo < hi)
+        {
+            const uint32_t mid = (lo + hi + 1u) >> 1;
+            if (item_scan[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1u;
+        }
+        s_leaf = lo;
+    }
+    for (uint32_t x = threadIdx.x; x < 512u; x += RED_THREADS)
+    {
+        s_val[x] = 0u;
+        s_g0[x] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const uint32_t l = s_leaf;
+    const uint32_t chunk = item - item_scan[l];
+    const uint32_t nchunks = item_scan[l + 1] - item_scan[l];
+    const uint64_t first = starts[l];
+    const uint64_t p0 = first + (uint64_t)chunk * LRC_CHUNK;
+    const uint64_t pend = starts[l + 1];
+    const uint64_t p1 = p0 + LRC_CHUNK < pend ? p0 + LRC_CHUNK : pend;
+    const uint32_t gid_shift = 8u;
+    const uint32_t slot_shift = kl.gid_bits + 8u;
+    const uint64_t gmask = (1ull << kl.gid_bits) - 1ull;
+    if (chunk == 0u && threadIdx.x == 0)
+        lkey[l] = keys[first] >> (slot_shift + 9u);
+    // pass 1: maximum alpha and lowest id per slot (LRC_UNROLL keys in flight per thread)
+    for (uint64_t q0 = p0 + threadIdx.x; q0 < p1; q0 += (uint64_t)RED_THREADS * LRC_UNROLL)
+    {
+        uint64_t k[LRC_UNROLL];
+#pragma unroll
+        for (int u = 0; u < LRC_UNROLL; u++)
+        {
+            const uint64_t q = q0 + (uint64_t)u * RED_THREADS;
+            k[u] = q < p1 ? keys[q] : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < LRC_UNROLL; u++)
+        {
+            if (q0 + (uint64_t)u * RED_THREADS >= p1)
+                break;
+            const uint32_t slot = (uint32_t)(k[u] >> slot_shift) & 511u;
+            atomicMax(&s_val[slot], (uint32_t)(k[u] & 255u) << 24);
+            atomicMin(&s_g0[slot], (uint32_t)((k[u] >> gid_shift) & gmask));
+        }
+    }
+    __syncthreads();
+    // pass 2: colours of the pairs holding their slot's maximum alpha
+    for (uint64_t q0 = p0 + threadIdx.x; q0 < p1; q0 += (uint64_t)RED_THREADS * LRC_UNROLL)
+    {
+        uint64_t k[LRC_UNROLL];
+#pragma unroll
+        for (int u = 0; u < LRC_UNROLL; u++)
+        {
+            const uint64_t q = q0 + (uint64_t)u * RED_THREADS;
+            k[u] = q < p1 ? keys[q] : ~0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < LRC_UNROLL; u++)
+        {
+            if (q0 + (uint64_t)u * RED_THREADS >= p1)
+                break;
+            const uint32_t slot = (uint32_t)(k[u] >> slot_shift) & 511u;
+            const uint32_t a8 = (uint32_t)(k[u] & 255u);
+            if ((s_val[slot] >> 24) == a8)
+            {
+                const uint32_t gid = (uint32_t)((k[u] >> gid_shift) & gmask);
+                atomicMax(&s_val[slot], (vals[gid].rgb_cull & 0x00FFFFFFu) | (a8 << 24));
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < 512u; x += RED_THREADS)
+    {
+        const uint64_t at = (uint64_t)l * 512u + x;
+        if (nchunks == 1u)
+        {
+            lval[at] = s_val[x];
+            lg0[at] = s_g0[x];
+        }
+        else if (s_g0[x] != 0xFFFFFFFFu)
+        {
+            atomicMax(&lval[at], s_val[x]);
+            atomicMin(&lg0[at], s_g0[x]);
+        }
+    }
+}
+
 // one wave per leaf: number of voxels with a pair
-__global__ __launch_bounds__(256) void k_leaf_count(const uint32_t* __restrict__ lg0, uint64_t nleaf,
+__global__ __launch_bounds__(256) void k_leaf_count(const uint32_t* __restrict__ lg0, DCount cnleaf,
                                                     uint32_t* __restrict__ cnt)
 {
+    const uint64_t nleaf = cnleaf.get();
     const uint64_t leaf = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (leaf >= nleaf)
         return;
@@ -2263,10 +2526,11 @@ __global__ __launch_bounds__(256) void k_leaf_count(const uint32_t* __restrict__
 // one wave per leaf: its voxels with a pair, in slot (= key) order, at the leaf's offset
 __global__ __launch_bounds__(256) void k_leaf_write(const uint32_t* __restrict__ lval, const uint32_t* __restrict__ lg0,
                                                     const uint64_t* __restrict__ lkey,
-                                                    const uint32_t* __restrict__ lscan, uint64_t nleaf,
+                                                    const uint32_t* __restrict__ lscan, DCount cnleaf,
                                                     uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
                                                     uint32_t* __restrict__ vox_g0)
 {
+    const uint64_t nleaf = cnleaf.get();
     const uint64_t leaf = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (leaf >= nleaf)
         return;
@@ -2409,12 +2673,13 @@ __device__ __forceinline__ void add192(unsigned long long* w, uint64_t a0, uint6
 }
 
 // chunks of every leaf (starts holds nleaf + 1 entries, the last one the pair count)
-__global__ __launch_bounds__(256) void k_leaf_chunks(const uint32_t* __restrict__ starts, uint64_t nleaf,
-                                                     uint32_t* __restrict__ items)
+__global__ __launch_bounds__(256) void k_leaf_chunks(const uint32_t* __restrict__ starts, DCount cnleaf,
+                                                     uint32_t chunk, uint32_t* __restrict__ items)
 {
+    const uint64_t nleaf = cnleaf.get();
     const uint64_t l = (uint64_t)blockIdx.x * 256u + threadIdx.x;
     if (l < nleaf)
-        items[l] = (starts[l + 1] - starts[l] + LRD_CHUNK - 1u) / LRD_CHUNK;
+        items[l] = (starts[l + 1] - starts[l] + chunk - 1u) / chunk;
 }
 
 __global__ void k_store_u32(uint32_t* p, uint32_t v)
@@ -2426,11 +2691,12 @@ __global__ void k_store_u32(uint32_t* p, uint32_t v)
 __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_density(const uint64_t* __restrict__ keys, KeyLayout kl,
                                                                      float vs, const uint32_t* __restrict__ starts,
                                                                      const uint32_t* __restrict__ item_scan,
-                                                                     uint64_t nleaf, const DensPrim* __restrict__ prims,
+                                                                     DCount cnleaf, const DensPrim* __restrict__ prims,
                                                                      unsigned long long* __restrict__ lw,
                                                                      uint32_t* __restrict__ lg0,
                                                                      uint64_t* __restrict__ lkey)
 {
+    const uint64_t nleaf = cnleaf.get();
     __shared__ unsigned long long s_w[512 * 3];
     __shared__ uint32_t s_g0[512];
     __shared__ double s_exp2[64];
@@ -2519,10 +2785,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_density(const uint6
 __global__ __launch_bounds__(256) void k_leaf_write_density(const unsigned long long* __restrict__ lw,
                                                             const uint32_t* __restrict__ lg0,
                                                             const uint64_t* __restrict__ lkey,
-                                                            const uint32_t* __restrict__ lscan, uint64_t nleaf,
+                                                            const uint32_t* __restrict__ lscan, DCount cnleaf,
                                                             uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
                                                             uint32_t* __restrict__ vox_g0)
 {
+    const uint64_t nleaf = cnleaf.get();
     const uint64_t leaf = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (leaf >= nleaf)
         return;
@@ -2557,13 +2824,14 @@ __global__ __launch_bounds__(256) void k_leaf_write_density(const unsigned long 
 }
 
 // per voxel: first-touch window from the lowest gaussian id, and voxel bounds
-__global__ __launch_bounds__(256) void k_voxel_finish(uint64_t nvox, KeyLayout kl, const uint64_t* __restrict__ vox_key,
+__global__ __launch_bounds__(256) void k_voxel_finish(DCount cnvox, KeyLayout kl, const uint64_t* __restrict__ vox_key,
                                                       const uint32_t* __restrict__ vox_g0, const PrimBox* __restrict__ boxes,
                                                       const uint64_t* __restrict__ foot_scan,
                                                       const uint32_t* __restrict__ win_base, uint32_t prim_batch,
                                                       uint32_t ijk_batch, uint32_t* __restrict__ vox_t,
                                                       int32_t* __restrict__ bounds)
 {
+    const uint64_t nvox = cnvox.get();
     int32_t mn[3] = { 2147483647, 2147483647, 2147483647 }, mx[3] = { -2147483647, -2147483647, -2147483647 };
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += stride)
@@ -2592,10 +2860,11 @@ __global__ __launch_bounds__(256) void k_voxel_finish(uint64_t nvox, KeyLayout k
 // per leaf reads the leaf's voxel run coalesced; each lane ORs its voxels' slot bits into 16
 // 32-bit halves of the 512-bit mask, and shuffles join the lanes.
 __global__ __launch_bounds__(256) void k_leaf_info(const uint64_t* __restrict__ vox_key, const uint32_t* __restrict__ vox_t,
-                                                   const uint32_t* __restrict__ leaf_first, uint64_t nleaf, KeyLayout kl,
+                                                   const uint32_t* __restrict__ leaf_first, DCount cnleaf, KeyLayout kl,
                                                    uint64_t* __restrict__ leaf_mask, uint32_t* __restrict__ leaf_t,
                                                    uint64_t* __restrict__ sort_keys, uint32_t* __restrict__ sort_vals)
 {
+    const uint64_t nleaf = cnleaf.get();
     const uint64_t l = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (l >= nleaf)
         return; // wave-uniform
@@ -2643,10 +2912,11 @@ __global__ __launch_bounds__(256) void k_leaf_info(const uint64_t* __restrict__ 
 
 // gather per tree-ordered leaf: key, t; and inverse permutation
 __global__ __launch_bounds__(256) void k_leaf_gather(const uint64_t* __restrict__ sorted_keys,
-                                                     const uint32_t* __restrict__ sorted_idx, uint64_t nleaf,
+                                                     const uint32_t* __restrict__ sorted_idx, DCount cnleaf,
                                                      const uint32_t* __restrict__ leaf_t, uint32_t* __restrict__ tree_t,
                                                      uint32_t* __restrict__ leaf_pos)
 {
+    const uint64_t nleaf = cnleaf.get();
     uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nleaf)
         return;
@@ -2658,8 +2928,9 @@ __global__ __launch_bounds__(256) void k_leaf_gather(const uint64_t* __restrict_
 // min of a u32 array over runs [first[r], first[r+1])
 // out[r] = minimum of vals over run r; one wave per run
 __global__ __launch_bounds__(256) void k_run_min(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ first,
-                                                 uint64_t nrun, uint32_t* __restrict__ out)
+                                                 DCount cnrun, uint32_t* __restrict__ out)
 {
+    const uint64_t nrun = cnrun.get();
     const uint64_t r = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (r >= nrun)
         return; // wave-uniform
@@ -2677,8 +2948,9 @@ __global__ __launch_bounds__(256) void k_run_min(const uint32_t* __restrict__ va
 
 // key of each run head: out[r] = keys[first[r]] >> shift
 __global__ __launch_bounds__(256) void k_run_keys(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ first,
-                                                  uint64_t nrun, uint32_t shift, uint64_t* __restrict__ out)
+                                                  DCount cnrun, uint32_t shift, uint64_t* __restrict__ out)
 {
+    const uint64_t nrun = cnrun.get();
     uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrun)
         return;
@@ -2687,9 +2959,10 @@ __global__ __launch_bounds__(256) void k_run_keys(const uint64_t* __restrict__ k
 
 // ============================================================================ node order
 // window histogram per level: hist[level * T + t]
-__global__ __launch_bounds__(256) void k_window_hist(const uint32_t* __restrict__ t, uint64_t count, uint32_t T,
+__global__ __launch_bounds__(256) void k_window_hist(const uint32_t* __restrict__ t, DCount ccount, uint32_t T,
                                                      uint32_t* __restrict__ hist_level)
 {
+    const uint64_t count = ccount.get();
     uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= count)
         return;
@@ -2707,10 +2980,11 @@ __global__ __launch_bounds__(256) void k_window_totals(const uint32_t* __restric
 
 // sort key of a node: (t << tshift) | (parent node idx << nbits) | slot
 __global__ __launch_bounds__(256) void k_node_keys(const uint32_t* __restrict__ t, const uint32_t* __restrict__ parent_idx,
-                                                   const uint64_t* __restrict__ slot_key, uint64_t count, uint32_t slot_bits,
+                                                   const uint64_t* __restrict__ slot_key, DCount ccount, uint32_t slot_bits,
                                                    uint32_t parent_bits, uint64_t* __restrict__ keys,
                                                    uint32_t* __restrict__ vals)
 {
+    const uint64_t count = ccount.get();
     uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= count)
         return;
@@ -2722,12 +2996,13 @@ __global__ __launch_bounds__(256) void k_node_keys(const uint32_t* __restrict__ 
 
 // node_idx from the sorted order: 1 + tot_before[t] + level_offset_in_t + (pos - level_before[t])
 __global__ __launch_bounds__(256) void k_node_assign(const uint64_t* __restrict__ sorted_keys,
-                                                     const uint32_t* __restrict__ sorted_vals, uint64_t count,
+                                                     const uint32_t* __restrict__ sorted_vals, DCount ccount,
                                                      uint32_t tshift, const uint32_t* __restrict__ tot_before,
                                                      const uint32_t* __restrict__ level_before,
                                                      const uint32_t* __restrict__ hist, uint32_t T, uint32_t level,
                                                      uint32_t* __restrict__ node_idx)
 {
+    const uint64_t count = ccount.get();
     uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pos >= count)
         return;
@@ -2738,19 +3013,23 @@ __global__ __launch_bounds__(256) void k_node_assign(const uint64_t* __restrict_
     node_idx[sorted_vals[pos]] = 1u + tot_before[t] + within + (uint32_t)(pos - level_before[t]);
 }
 
-__global__ __launch_bounds__(256) void k_node_bfs(uint64_t count, uint32_t base, uint32_t* __restrict__ node_idx)
+// node_idx = base + (counts of the levels before) + position, in tree order
+__global__ __launch_bounds__(256) void k_node_bfs(DCount ccount, uint32_t base, DCount before1, DCount before2,
+                                                  uint32_t* __restrict__ node_idx)
 {
+    const uint64_t count = ccount.get();
     uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= count)
         return;
-    node_idx[x] = base + (uint32_t)x;
+    node_idx[x] = base + (uint32_t)before1.get() + (uint32_t)before2.get() + (uint32_t)x;
 }
 
 // parent node idx of each child run member: child c in [first[p], first[p+1]) -> parent_idx[p]
-__global__ __launch_bounds__(256) void k_parent_idx(const uint32_t* __restrict__ first, uint64_t nparent,
+__global__ __launch_bounds__(256) void k_parent_idx(const uint32_t* __restrict__ first, DCount cnparent,
                                                     const uint32_t* __restrict__ parent_node_idx,
                                                     uint32_t* __restrict__ child_parent)
 {
+    const uint64_t nparent = cnparent.get();
     uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nparent)
         return;
@@ -2760,12 +3039,13 @@ __global__ __launch_bounds__(256) void k_parent_idx(const uint32_t* __restrict__
 }
 
 // per node idx: byte size and value count
-__global__ __launch_bounds__(256) void k_node_sizes(const uint32_t* __restrict__ node_idx, uint64_t count, uint32_t nmax,
+__global__ __launch_bounds__(256) void k_node_sizes(const uint32_t* __restrict__ node_idx, DCount ccount, uint32_t nmax,
                                                     uint64_t size,
                                                     const uint64_t* __restrict__ leaf_mask,
                                                     const uint32_t* __restrict__ tree_to_compact,
                                                     uint64_t* __restrict__ size_by_idx, uint64_t* __restrict__ vals_by_idx)
 {
+    const uint64_t count = ccount.get();
     uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= count)
         return;
@@ -2829,8 +3109,9 @@ __device__ __forceinline__ uint64_t node_addr(const GridInfo& gi, const uint64_t
     return (ni == 0u || ni >= gi.node_count) ? gi.root_addr : gi.nodes_begin + size_scan[ni];
 }
 
-__global__ void k_write_header(uint8_t* __restrict__ g, GridInfo gi, const int32_t* __restrict__ bounds)
+__global__ void k_write_header(uint8_t* __restrict__ g, GridInfo gi_in, const GridInfo* __restrict__ gip, const int32_t* __restrict__ bounds)
 {
+    const GridInfo gi = gip ? *gip : gi_in;
     if (threadIdx.x != 0 || blockIdx.x != 0)
         return;
     // GridBuild.cpp:126-187
@@ -2932,11 +3213,13 @@ __device__ void write_prefix_words(const uint64_t* mask_lds, uint32_t count, uin
 }
 
 // root: child mask + key mask over upper root slots, prefixes, children offsets and keys
-__global__ __launch_bounds__(1024) void k_write_root(uint8_t* __restrict__ g, GridInfo gi,
-                                                     const uint64_t* __restrict__ upper_key, uint64_t nupper,
+__global__ __launch_bounds__(1024) void k_write_root(uint8_t* __restrict__ g, GridInfo gi_in, const GridInfo* __restrict__ gip,
+                                                     const uint64_t* __restrict__ upper_key, DCount cnupper,
                                                      const uint32_t* __restrict__ upper_node_idx,
                                                      const uint64_t* __restrict__ size_scan)
 {
+    const GridInfo gi = gip ? *gip : gi_in;
+    const uint64_t nupper = cnupper.get();
     __shared__ uint64_t mask[512];
     __shared__ uint32_t pc[513];
     for (uint32_t w = threadIdx.x; w < 512; w += blockDim.x)
@@ -2987,7 +3270,7 @@ __global__ __launch_bounds__(1024) void k_write_root(uint8_t* __restrict__ g, Gr
 
 // internal node (upper or lower): one block per node
 template <int WORDS>
-__global__ __launch_bounds__(256) void k_write_internal(uint8_t* __restrict__ g, GridInfo gi,
+__global__ __launch_bounds__(256) void k_write_internal(uint8_t* __restrict__ g, GridInfo gi_in, const GridInfo* __restrict__ gip,
                                                         const uint32_t* __restrict__ node_first, // children runs
                                                         const uint64_t* __restrict__ child_slot_key,
                                                         uint32_t slot_bits,
@@ -2995,8 +3278,12 @@ __global__ __launch_bounds__(256) void k_write_internal(uint8_t* __restrict__ g,
                                                         const uint32_t* __restrict__ child_node_idx,
                                                         const uint64_t* __restrict__ size_scan,
                                                         const uint64_t* __restrict__ vals_scan,
-                                                        uint32_t off_cmp, uint32_t off_cm, uint32_t off_children)
+                                                        uint32_t off_cmp, uint32_t off_cm, uint32_t off_children,
+                                                        DCount ccount)
 {
+    if (blockIdx.x >= ccount.get())
+        return; // block-uniform: the grid may be sized by a capacity
+    const GridInfo gi = gip ? *gip : gi_in;
     __shared__ uint64_t mask[WORDS];
     __shared__ uint32_t pc[WORDS + 1];
     const uint32_t node = blockIdx.x;
@@ -3026,13 +3313,15 @@ __global__ __launch_bounds__(256) void k_write_internal(uint8_t* __restrict__ g,
     }
 }
 
-__global__ __launch_bounds__(256) void k_write_leaves(uint8_t* __restrict__ g, GridInfo gi, uint64_t nleaf,
+__global__ __launch_bounds__(256) void k_write_leaves(uint8_t* __restrict__ g, GridInfo gi_in, const GridInfo* __restrict__ gip, DCount cnleaf,
                                                       const uint32_t* __restrict__ tree_to_compact,
                                                       const uint64_t* __restrict__ leaf_mask,
                                                       const uint32_t* __restrict__ leaf_node_idx,
                                                       const uint64_t* __restrict__ size_scan,
                                                       const uint64_t* __restrict__ vals_scan)
 {
+    const GridInfo gi = gip ? *gip : gi_in;
+    const uint64_t nleaf = cnleaf.get();
     uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nleaf)
         return;
@@ -3057,7 +3346,7 @@ __global__ __launch_bounds__(256) void k_write_leaves(uint8_t* __restrict__ g, G
                                ((uint64_t)ex[7] << 48);
 }
 
-__global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, GridInfo gi, uint64_t nvox,
+__global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, GridInfo gi_in, const GridInfo* __restrict__ gip, DCount cnvox,
                                                       const uint32_t* __restrict__ vox_val,
                                                       const uint64_t* __restrict__ vox_key,
                                                       const uint32_t* __restrict__ leaf_first,
@@ -3068,25 +3357,27 @@ __global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, G
                                                       const uint32_t* __restrict__ vox_leaf, KeyLayout kl,
                                                       int32_t* __restrict__ ijk_out)
 {
-    uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvox)
-        return;
-    uint32_t l = vox_leaf[v];
-    uint32_t rank = (uint32_t)v - leaf_first[l];
-    uint32_t ni = leaf_node_idx[leaf_pos[l]];
-    if (ni >= gi.node_count)
-        return;
-    uint64_t idx = 2ull + vals_scan[ni] + rank;
-    if (idx < gi.value_count)
+    const GridInfo gi = gip ? *gip : gi_in;
+    const uint64_t nvox = cnvox.get();
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (uint64_t)gridDim.x * blockDim.x)
     {
-        *(uint32_t*)(g + gi.values + 4ull * idx) = vox_val[v];
-        if (ijk_out) // voxel coordinates in value-index order (VDBR_FLAG_VOXEL_COORDS)
+        uint32_t l = vox_leaf[v];
+        uint32_t rank = (uint32_t)v - leaf_first[l];
+        uint32_t ni = leaf_node_idx[leaf_pos[l]];
+        if (ni >= gi.node_count)
+            continue;
+        uint64_t idx = 2ull + vals_scan[ni] + rank;
+        if (idx < gi.value_count)
         {
-            int32_t i, j, k;
-            decode_voxel(kl, vox_key[v], &i, &j, &k);
-            ijk_out[3 * idx + 0] = i;
-            ijk_out[3 * idx + 1] = j;
-            ijk_out[3 * idx + 2] = k;
+            *(uint32_t*)(g + gi.values + 4ull * idx) = vox_val[v];
+            if (ijk_out) // voxel coordinates in value-index order (VDBR_FLAG_VOXEL_COORDS)
+            {
+                int32_t i, j, k;
+                decode_voxel(kl, vox_key[v], &i, &j, &k);
+                ijk_out[3 * idx + 0] = i;
+                ijk_out[3 * idx + 1] = j;
+                ijk_out[3 * idx + 2] = k;
+            }
         }
     }
     (void)leaf_of_run_start;
@@ -3094,9 +3385,10 @@ __global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, G
 
 // voxel -> leaf run index
 // member_of[x] = r for every x of run r; one wave per run (coalesced writes)
-__global__ __launch_bounds__(256) void k_run_member(const uint32_t* __restrict__ first, uint64_t nrun,
+__global__ __launch_bounds__(256) void k_run_member(const uint32_t* __restrict__ first, DCount cnrun,
                                                     uint32_t* __restrict__ member_of)
 {
+    const uint64_t nrun = cnrun.get();
     const uint64_t r = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
     if (r >= nrun)
         return; // wave-uniform
@@ -3155,50 +3447,84 @@ static inline uint32_t bits_for(uint64_t range) // bits to represent values in [
     return b;
 }
 
-// run heads of keys >> shift: writes starts[0..runs) and starts[runs] = n when starts is not
-// null; returns the run count and, through tile_offsets_out, the exclusive head counts of the
-// HEAD_TILE-sized tiles (valid until the next run_heads call). sorted: the keys are sorted over
-// every bit from shift up, so the count reads only the segments that hold a head.
-static int run_heads(Context* ctx, const uint64_t* keys, uint64_t n, uint32_t shift, uint32_t* starts,
-                     uint64_t* runs_out, const uint32_t** tile_offsets_out = nullptr, bool sorted = false)
+// starts[runs] = n after the run starts (the end of the last run)
+__global__ void k_store_end(uint32_t* __restrict__ starts, const uint32_t* __restrict__ runs, DCount cn)
 {
-    if (n == 0)
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        starts[*runs] = (uint32_t)cn.get();
+}
+
+// Run heads of keys >> shift: writes starts[0..runs) and starts[runs] = n when starts is not
+// null; the run count goes to *runs_out (read back) and, through tile_offsets_out, the exclusive
+// head counts of the HEAD_TILE-sized tiles (valid until the next run_heads call), whose total
+// (the run count) sits in the device word *runs_dev_out. sorted: the keys are sorted over every
+// bit from shift up, so the count reads only the segments that hold a head.
+// With a device count n (graph capture) nothing is read back: *runs_out receives runs_cap, the
+// bound the caller sizes by, and the count stays in the device word.
+// runs_word, when given, receives the run count instead of the tile-count buffer (which the next
+// run_heads call reuses): the device-count form keeps counts there that later stages read.
+static int run_heads(Context* ctx, const uint64_t* keys, DCount n, uint32_t shift, uint32_t* starts,
+                     uint64_t* runs_out, const uint32_t** runs_dev_out = nullptr,
+                     const uint32_t** tile_offsets_out = nullptr, bool sorted = false, uint64_t runs_cap = 0,
+                     uint32_t* runs_word = nullptr)
+{
+    const bool dev = n.dev != nullptr;
+    if (!dev && n.cap == 0)
     {
         *runs_out = 0;
-        if (starts)
-            VDBR_CHECK(ctx, hipMemsetAsync(starts, 0, 4, ctx->stream));
+        uint32_t* tc = (uint32_t*)ctx->ws(WS_SEG_TILES, 2 * sizeof(uint32_t));
+        if (!tc)
+            return -1;
+        if (fill_words(ctx, tc, host_count(2), 0u))
+            return -1;
+        if (starts && fill_words(ctx, starts, host_count(1), 0u))
+            return -1;
+        if (runs_word && fill_words(ctx, runs_word, host_count(1), 0u))
+            return -1;
+        if (runs_dev_out)
+            *runs_dev_out = runs_word ? runs_word : tc;
+        if (tile_offsets_out)
+            *tile_offsets_out = tc + 1;
         return 0;
     }
-    if (n >= 0xFFFFFFFFull)
+    if (n.cap >= 0xFFFFFFFFull)
         return ctx->fail("run heads: more than 2^32-1 elements");
-    uint64_t tiles = (n + HEAD_TILE - 1) / HEAD_TILE;
-    uint32_t* tile_counts = (uint32_t*)ctx->ws(WS_SEG_TILES, (tiles + 1) * sizeof(uint32_t));
+    const DCount tiles = count_div(n, HEAD_TILE);
+    const uint64_t tcap = tiles.cap > 0 ? tiles.cap : 1;
+    uint32_t* tile_counts = (uint32_t*)ctx->ws(WS_SEG_TILES, (tcap + 2) * sizeof(uint32_t));
     if (!tile_counts)
         return -1;
+    uint32_t* total = runs_word ? runs_word : tile_counts + tcap + 1; // the run count
+    // one wave (sorted) or one block per tile, at most a few thousand blocks striding over the rest
+    const uint32_t wave_blocks = (uint32_t)std::min<uint64_t>((tcap + 3) / 4, 8192);
+    const uint32_t tile_blocks = (uint32_t)std::min<uint64_t>(tcap, 8192);
     if (sorted)
-        hipLaunchKernelGGL(k_head_count_sorted, dim3((uint32_t)((tiles + 3) / 4)), dim3(256), 0, ctx->stream, keys, n,
-                           shift, tiles, tile_counts);
-    else
-        hipLaunchKernelGGL(k_head_count, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
+        hipLaunchKernelGGL(k_head_count_sorted, dim3(wave_blocks), dim3(256), 0, ctx->stream, keys, n, shift,
                            tile_counts);
-    if (scan_u32(ctx, tile_counts, tile_counts, tiles, false, tile_counts + tiles))
+    else
+        hipLaunchKernelGGL(k_head_count, dim3(tile_blocks), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
+                           tile_counts);
+    if (scan_u32(ctx, tile_counts, tile_counts, tiles, false, total))
         return -1;
     if (starts && sorted)
-        hipLaunchKernelGGL(k_head_write_sorted, dim3((uint32_t)((tiles + 3) / 4)), dim3(256), 0, ctx->stream, keys, n,
-                           shift, tiles, (const uint32_t*)tile_counts, starts);
-    else if (starts)
-        hipLaunchKernelGGL(k_head_write, dim3((uint32_t)tiles), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
+        hipLaunchKernelGGL(k_head_write_sorted, dim3(wave_blocks), dim3(256), 0, ctx->stream, keys, n, shift,
                            (const uint32_t*)tile_counts, starts);
-    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, tile_counts + tiles, 4, hipMemcpyDeviceToHost, ctx->stream));
-    VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    uint32_t runs = *(uint32_t*)ctx->pinned;
+    else if (starts)
+        hipLaunchKernelGGL(k_head_write, dim3(tile_blocks), dim3(HEAD_THREADS), 0, ctx->stream, keys, n, shift,
+                           (const uint32_t*)tile_counts, starts);
     if (starts)
+        hipLaunchKernelGGL(k_store_end, dim3(1), dim3(64), 0, ctx->stream, starts, (const uint32_t*)total, n);
+    VDBR_CHECK(ctx, hipGetLastError());
+    if (dev)
+        *runs_out = runs_cap;
+    else
     {
-        ((uint32_t*)ctx->pinned)[1] = (uint32_t)n;
-        VDBR_CHECK(ctx, hipMemcpyAsync(starts + runs, (uint32_t*)ctx->pinned + 1, 4, hipMemcpyHostToDevice, ctx->stream));
+        VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, total, 4, hipMemcpyDeviceToHost, ctx->stream));
         VDBR_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        *runs_out = *(uint32_t*)ctx->pinned;
     }
-    *runs_out = runs;
+    if (runs_dev_out)
+        *runs_dev_out = total;
     if (tile_offsets_out)
         *tile_offsets_out = tile_counts;
     return 0;
@@ -3210,15 +3536,58 @@ struct Timer
     int n = 0;
 };
 
+// node count 1 + uppers + lowers + leaves (device-count form), checked against the node bound
+__global__ void k_node_total(const uint32_t* __restrict__ cnt, uint32_t max_nodes, uint64_t leaf_cap, uint64_t lower_cap,
+                             uint64_t upper_cap, uint32_t* __restrict__ node_count, uint32_t* __restrict__ flags)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const uint64_t nu = cnt[0], nl = cnt[1], nf = cnt[2];
+    const uint64_t total = 1 + nu + nl + nf;
+    *node_count = (uint32_t)(total < 0xFFFFFFFFull ? total : 0xFFFFFFFFull);
+    if (nf > leaf_cap || nl > lower_cap || nu > upper_cap)
+        atomicOr(flags, GF_LEAVES);
+    if (total > max_nodes)
+        atomicOr(flags, GF_NODES);
+}
+
+// grid layout from the node and voxel counts (device-count form of the host layout below)
+__global__ void k_grid_info(GridInfo base, const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ nvox,
+                            uint64_t nvox_cap, GridInfo* __restrict__ gi)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    GridInfo g = base;
+    const uint64_t nu = cnt[0], nl = cnt[1], nf = cnt[2];
+    const uint64_t nv = *nvox < nvox_cap ? *nvox : nvox_cap;
+    g.end = g.nodes_begin + nu * VDB_SIZE_UPPER + nl * VDB_SIZE_LOWER + nf * VDB_SIZE_LEAF;
+    g.meta0 = g.end;
+    g.meta1 = g.end + VDB_BLINDMETADATA_SIZE;
+    g.bboxes = g.meta1 + VDB_BLINDMETADATA_SIZE;
+    g.values = g.bboxes + 32;
+    g.value_count = 2 + nv;
+    g.grid_size = g.values + 4 * g.value_count;
+    g.upper_count = (uint32_t)nu;
+    g.lower_count = (uint32_t)nl;
+    g.leaf_count = (uint32_t)nf;
+    g.node_count = (uint32_t)(1 + nu + nl + nf);
+    *gi = g;
+}
+
 // Node2 grid from sorted unique voxel keys (the GridBuild stage of the raster path,
 // GridBuild.cpp:361-650): leaves, lowers and uppers from key runs, node order by first-touch
 // window (or tree order), node sizes and value indices by scans, then the grid written by one
 // pass per level. The grid lives in the WS_GRID slot of the context.
+// Host form: every count is read back and the buffers fit them exactly. Device-count form
+// (in.nvox_dev set, for graph capture): no count leaves the device; buffers and launches are
+// sized by the bounds of the input, kernels read the counts, and the layout is computed on the
+// device (out->grid_size_dev).
 int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* out)
 {
     hipStream_t st = ctx->stream;
+    const bool devc = in.nvox_dev != nullptr;
     const KeyLayout kl = in.kl;
-    const uint64_t nvox = in.nvox;
+    const DCount nvox = devc ? dev_count(in.nvox_dev, in.nvox) : host_count(in.nvox);
     uint64_t* vox_key = in.vox_key;
     uint32_t* vox_val = in.vox_val;
     uint32_t* vox_t = in.vox_t;
@@ -3227,17 +3596,24 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
     const uint32_t max_nodes = in.max_nodes;
     const float vs = in.voxel_size;
     const float vs_inv = 1.f / vs;
-    const uint64_t nv1 = nvox > 0 ? nvox : 1;
+    const uint64_t nv1 = nvox.cap > 0 ? nvox.cap : 1;
     uint32_t* vox_leaf = (uint32_t*)ctx->ws(WS_VOX_LEAF, nv1 * sizeof(uint32_t));
-    if (!vox_leaf)
+    // device-count form: [0..2] upper, lower, leaf counts, [3] node count
+    uint32_t* cnt = devc ? (uint32_t*)ctx->ws(WS_GB_COUNTS, 64) : nullptr;
+    if (!vox_leaf || (devc && !cnt))
         return -1;
 
     // ---------------------------------------------------------------- leaves / lowers / uppers
     uint32_t* leaf_first = (uint32_t*)ctx->ws(WS_LEAF_FIRST, (nv1 + 2) * sizeof(uint32_t));
-    uint64_t nleaf = 0;
-    if (run_heads(ctx, vox_key, nvox, 9, leaf_first, &nleaf))
+    if (!leaf_first)
         return -1;
-    const uint64_t nl1 = nleaf > 0 ? nleaf : 1;
+    uint64_t nleaf_c = 0;
+    const uint32_t* nleaf_dev = nullptr;
+    if (run_heads(ctx, vox_key, nvox, 9, leaf_first, &nleaf_c, &nleaf_dev, nullptr, false, in.leaf_cap,
+                  devc ? cnt + 2 : nullptr))
+        return -1;
+    const DCount nleaf = devc ? dev_count(nleaf_dev, nleaf_c) : host_count(nleaf_c);
+    const uint64_t nl1 = nleaf.cap > 0 ? nleaf.cap : 1;
     uint64_t* leaf_mask = (uint64_t*)ctx->ws(WS_LEAF_MASK, nl1 * 8 * sizeof(uint64_t));
     uint32_t* leaf_info = (uint32_t*)ctx->ws(WS_LEAF_INFO, nl1 * 6 * sizeof(uint32_t));
     uint32_t* leaf_t = leaf_info;               // compact order
@@ -3249,50 +3625,74 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
     uint32_t* lvals = (uint32_t*)ctx->ws(WS_LEAF_SORT_VALS, nl1 * sizeof(uint32_t));
     if (!leaf_mask || !leaf_info || !lkeys || !lvals)
         return -1;
-    if (nleaf > 0)
+    if (nleaf.cap > 0)
     {
-        hipLaunchKernelGGL(k_leaf_info, dim3((uint32_t)((nleaf + 3) / 4)), dim3(256), 0, st, vox_key, vox_t, leaf_first,
-                           nleaf, kl,
-                           leaf_mask, leaf_t, lkeys, lvals);
-        hipLaunchKernelGGL(k_run_member, dim3((uint32_t)((nleaf + 3) / 4)), dim3(256), 0, st, leaf_first, nleaf, vox_leaf);
-        if (radix_sort_u64(ctx, lkeys, lvals, nleaf, 0, 42, nullptr))
+        hipLaunchKernelGGL(k_leaf_info, dim3((uint32_t)((nleaf.cap + 3) / 4)), dim3(256), 0, st, vox_key, vox_t,
+                           leaf_first, nleaf, kl, leaf_mask, leaf_t, lkeys, lvals);
+        hipLaunchKernelGGL(k_run_member, dim3((uint32_t)((nleaf.cap + 3) / 4)), dim3(256), 0, st, leaf_first, nleaf,
+                           vox_leaf);
+        if (radix_sort_u64(ctx, lkeys, lvals, nleaf, 0, 42))
             return -1;
-        hipLaunchKernelGGL(k_leaf_gather, dim3(blocks_for(nleaf)), dim3(256), 0, st, lkeys, lvals, nleaf, leaf_t, tree_t,
-                           leaf_pos);
+        hipLaunchKernelGGL(k_leaf_gather, dim3(blocks_for(nleaf.cap)), dim3(256), 0, st, lkeys, lvals, nleaf, leaf_t,
+                           tree_t, leaf_pos);
     }
     // lowers: runs of tree key >> 12 over the tree-ordered leaves
     uint32_t* lower_first = (uint32_t*)ctx->ws(WS_LOWER, (nl1 + 2) * sizeof(uint32_t) * 4);
-    uint64_t nlower = 0;
-    if (run_heads(ctx, lkeys, nleaf, 12, lower_first, &nlower))
+    if (!lower_first)
         return -1;
-    const uint64_t nlo1 = nlower > 0 ? nlower : 1;
+    uint64_t nlower_c = 0;
+    const uint32_t* nlower_dev = nullptr;
+    if (run_heads(ctx, lkeys, nleaf, 12, lower_first, &nlower_c, &nlower_dev, nullptr, false, in.lower_cap,
+                  devc ? cnt + 1 : nullptr))
+        return -1;
+    const DCount nlower = devc ? dev_count(nlower_dev, nlower_c) : host_count(nlower_c);
+    const uint64_t nlo1 = nlower.cap > 0 ? nlower.cap : 1;
     uint64_t* lower_key = (uint64_t*)ctx->ws(WS_LOWER_MASK, nlo1 * sizeof(uint64_t) * 2);
     uint32_t* lower_t = lower_first + (nl1 + 2);
     uint32_t* lower_node_idx = lower_t + nlo1;
     uint32_t* lower_parent = lower_node_idx + nlo1;
-    if (nlower > 0)
+    if (!lower_key)
+        return -1;
+    if (nlower.cap > 0)
     {
-        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nlower)), dim3(256), 0, st, lkeys, lower_first, nlower, 12u,
+        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nlower.cap)), dim3(256), 0, st, lkeys, lower_first, nlower, 12u,
                            lower_key);
-        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nlower + 3) / 4)), dim3(256), 0, st, tree_t, lower_first, nlower, lower_t);
+        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nlower.cap + 3) / 4)), dim3(256), 0, st, tree_t, lower_first,
+                           nlower, lower_t);
     }
     uint32_t* upper_first = (uint32_t*)ctx->ws(WS_UPPER, (nlo1 + 2) * sizeof(uint32_t) * 4);
-    uint64_t nupper = 0;
-    if (run_heads(ctx, lower_key, nlower, 15, upper_first, &nupper))
+    if (!upper_first)
         return -1;
-    const uint64_t nu1 = nupper > 0 ? nupper : 1;
+    uint64_t nupper_c = 0;
+    const uint32_t* nupper_dev = nullptr;
+    if (run_heads(ctx, lower_key, nlower, 15, upper_first, &nupper_c, &nupper_dev, nullptr, false, in.upper_cap,
+                  devc ? cnt : nullptr))
+        return -1;
+    const DCount nupper = devc ? dev_count(nupper_dev, nupper_c) : host_count(nupper_c);
+    const uint64_t nu1 = nupper.cap > 0 ? nupper.cap : 1;
     uint64_t* upper_key = (uint64_t*)ctx->ws(WS_UPPER_MASK, nu1 * sizeof(uint64_t));
     uint32_t* upper_t = upper_first + (nlo1 + 2);
     uint32_t* upper_node_idx = upper_t + nu1;
-    if (nupper > 0)
+    if (!upper_key)
+        return -1;
+    if (nupper.cap > 0)
     {
-        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nupper)), dim3(256), 0, st, lower_key, upper_first, nupper, 15u,
-                           upper_key);
-        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nupper + 3) / 4)), dim3(256), 0, st, lower_t, upper_first, nupper, upper_t);
+        hipLaunchKernelGGL(k_run_keys, dim3(blocks_for(nupper.cap)), dim3(256), 0, st, lower_key, upper_first, nupper,
+                           15u, upper_key);
+        hipLaunchKernelGGL(k_run_min, dim3((uint32_t)((nupper.cap + 3) / 4)), dim3(256), 0, st, lower_t, upper_first,
+                           nupper, upper_t);
     }
-    const uint64_t node_count = 1 + nupper + nlower + nleaf;
-    if (node_count > max_nodes)
+    const uint64_t node_count = 1 + nupper.cap + nlower.cap + nleaf.cap; // exact, or the bound
+    if (!devc && node_count > max_nodes)
         return ctx->fail("raster_node2: node count exceeds max_node_count");
+    DCount nodes = host_count(node_count);
+    if (devc)
+    {
+        // the three run counts sit side by side in cnt: the node total and its checks
+        hipLaunchKernelGGL(k_node_total, dim3(1), dim3(64), 0, st, (const uint32_t*)cnt, max_nodes, nleaf.cap,
+                           nlower.cap, nupper.cap, cnt + 3, in.flags_dev);
+        nodes = dev_count((const uint32_t*)(cnt + 3), node_count);
+    }
 
     // ---------------------------------------------------------------- node order
     bool spatial = in.spatial_order || T <= 1;
@@ -3300,24 +3700,31 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
     {
         // histogram of first-touch windows per level
         uint32_t* hist = (uint32_t*)ctx->ws(WS_HIST, (size_t)(3 * T + 4 * (T + 1)) * sizeof(uint32_t));
+        if (!hist)
+            return -1;
         uint32_t* tot = hist + 3 * T;
         uint32_t* tot_before = tot + (T + 1);
         uint32_t* lev_before = tot_before + (T + 1); // 2 arrays of T+1 used per level in turn
-        VDBR_CHECK(ctx, hipMemsetAsync(hist, 0, (size_t)3 * T * sizeof(uint32_t), st));
-        if (nupper)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_t, nupper, T, hist);
-        if (nlower)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_t, nlower, T, hist + T);
-        if (nleaf)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nleaf)), dim3(256), 0, st, tree_t, nleaf, T, hist + 2 * T);
+        if (fill_words(ctx, hist, host_count((uint64_t)3 * T), 0u))
+            return -1;
+        if (nupper.cap)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nupper.cap)), dim3(256), 0, st, upper_t, nupper, T, hist);
+        if (nlower.cap)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nlower.cap)), dim3(256), 0, st, lower_t, nlower, T,
+                               hist + T);
+        if (nleaf.cap)
+            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nleaf.cap)), dim3(256), 0, st, tree_t, nleaf, T,
+                               hist + 2 * T);
         hipLaunchKernelGGL(k_window_totals, dim3(blocks_for(T)), dim3(256), 0, st, hist, T, tot);
-        if (scan_u32(ctx, tot, tot_before, T, false, nullptr))
+        if (scan_u32(ctx, tot, tot_before, (uint64_t)T, false, nullptr))
             return -1;
         uint64_t* nkeys = (uint64_t*)ctx->ws(WS_NODE_KEYS, nl1 * sizeof(uint64_t));
         uint32_t* nvals = (uint32_t*)ctx->ws(WS_NODE_VALS, nl1 * sizeof(uint32_t));
+        if (!nkeys || !nvals)
+            return -1;
         struct Level
         {
-            uint64_t count;
+            DCount count;
             const uint32_t* t;
             const uint32_t* parent;
             const uint64_t* slot_key;
@@ -3334,56 +3741,62 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
         for (uint32_t lv = 0; lv < 3; lv++)
         {
             Level& L = levels[lv];
-            if (L.count == 0)
+            if (L.count.cap == 0)
                 continue;
             if (lv == 1)
-                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_first, nupper,
+                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nupper.cap)), dim3(256), 0, st, upper_first, nupper,
                                    upper_node_idx, lower_parent);
             if (lv == 2)
-                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_first, nlower,
+                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nlower.cap)), dim3(256), 0, st, lower_first, nlower,
                                    lower_node_idx, leaf_parent);
-            if (scan_u32(ctx, hist + lv * T, lev_before, T, false, nullptr))
+            if (scan_u32(ctx, hist + lv * T, lev_before, (uint64_t)T, false, nullptr))
                 return -1;
-            hipLaunchKernelGGL(k_node_keys, dim3(blocks_for(L.count)), dim3(256), 0, st, L.t, lv == 0 ? nullptr : L.parent,
-                               L.slot_key, L.count, L.slot_bits, parent_bits, nkeys, nvals);
-            if (radix_sort_u64(ctx, nkeys, nvals, L.count, 0, 20 + parent_bits + L.slot_bits, nullptr))
+            hipLaunchKernelGGL(k_node_keys, dim3(blocks_for(L.count.cap)), dim3(256), 0, st, L.t,
+                               lv == 0 ? nullptr : L.parent, L.slot_key, L.count, L.slot_bits, parent_bits, nkeys,
+                               nvals);
+            if (radix_sort_u64(ctx, nkeys, nvals, L.count, 0, 20 + parent_bits + L.slot_bits))
                 return -1;
-            hipLaunchKernelGGL(k_node_assign, dim3(blocks_for(L.count)), dim3(256), 0, st, nkeys, nvals, L.count,
+            hipLaunchKernelGGL(k_node_assign, dim3(blocks_for(L.count.cap)), dim3(256), 0, st, nkeys, nvals, L.count,
                                parent_bits + L.slot_bits, tot_before, lev_before, hist, T, lv, L.node_idx);
         }
     }
     else
     {
-        if (nupper)
-            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nupper)), dim3(256), 0, st, nupper, 1u, upper_node_idx);
-        if (nlower)
-            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nlower)), dim3(256), 0, st, nlower, (uint32_t)(1 + nupper),
+        const DCount none = host_count(0);
+        if (nupper.cap)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nupper.cap)), dim3(256), 0, st, nupper, 1u, none, none,
+                               upper_node_idx);
+        if (nlower.cap)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nlower.cap)), dim3(256), 0, st, nlower, 1u, nupper, none,
                                lower_node_idx);
-        if (nleaf)
-            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nleaf)), dim3(256), 0, st, nleaf,
-                               (uint32_t)(1 + nupper + nlower), leaf_node_idx);
+        if (nleaf.cap)
+            hipLaunchKernelGGL(k_node_bfs, dim3(blocks_for(nleaf.cap)), dim3(256), 0, st, nleaf, 1u, nupper, nlower,
+                               leaf_node_idx);
     }
     // sizes and value counts by node idx -> exclusive scans
     uint64_t* byidx = (uint64_t*)ctx->ws(WS_NODE_BYIDX, (node_count + 1) * 2 * sizeof(uint64_t));
+    uint64_t* nscan = (uint64_t*)ctx->ws(WS_NODE_SCAN, (node_count + 1) * 2 * sizeof(uint64_t));
+    if (!byidx || !nscan)
+        return -1;
     uint64_t* size_by_idx = byidx;
     uint64_t* vals_by_idx = byidx + (node_count + 1);
-    uint64_t* nscan = (uint64_t*)ctx->ws(WS_NODE_SCAN, (node_count + 1) * 2 * sizeof(uint64_t));
     uint64_t* size_scan = nscan;
     uint64_t* vals_scan = nscan + (node_count + 1);
-    VDBR_CHECK(ctx, hipMemsetAsync(byidx, 0, (node_count + 1) * 2 * sizeof(uint64_t), st));
-    if (nupper)
-        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nupper)), dim3(256), 0, st, upper_node_idx, nupper, (uint32_t)node_count, VDB_SIZE_UPPER,
-                           nullptr, nullptr, size_by_idx, vals_by_idx);
-    if (nlower)
-        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nlower)), dim3(256), 0, st, lower_node_idx, nlower, (uint32_t)node_count, VDB_SIZE_LOWER,
-                           nullptr, nullptr, size_by_idx, vals_by_idx);
-    if (nleaf)
-        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nleaf)), dim3(256), 0, st, leaf_node_idx, nleaf, (uint32_t)node_count, VDB_SIZE_LEAF,
-                           leaf_mask, lvals, size_by_idx, vals_by_idx);
-    // node idx 0 is the root: its slot holds size 0 so the scan at idx 1 starts at 0
-    if (scan_u64(ctx, size_by_idx, size_scan, node_count, false, size_scan + node_count))
+    if (fill_words(ctx, byidx, host_count((node_count + 1) * 2), 0ull))
         return -1;
-    if (scan_u64(ctx, vals_by_idx, vals_scan, node_count, false, vals_scan + node_count))
+    if (nupper.cap)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nupper.cap)), dim3(256), 0, st, upper_node_idx, nupper,
+                           (uint32_t)node_count, VDB_SIZE_UPPER, nullptr, nullptr, size_by_idx, vals_by_idx);
+    if (nlower.cap)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nlower.cap)), dim3(256), 0, st, lower_node_idx, nlower,
+                           (uint32_t)node_count, VDB_SIZE_LOWER, nullptr, nullptr, size_by_idx, vals_by_idx);
+    if (nleaf.cap)
+        hipLaunchKernelGGL(k_node_sizes, dim3(blocks_for(nleaf.cap)), dim3(256), 0, st, leaf_node_idx, nleaf,
+                           (uint32_t)node_count, VDB_SIZE_LEAF, leaf_mask, lvals, size_by_idx, vals_by_idx);
+    // node idx 0 is the root: its slot holds size 0 so the scan at idx 1 starts at 0
+    if (scan_u64(ctx, size_by_idx, size_scan, nodes, false, size_scan + node_count))
+        return -1;
+    if (scan_u64(ctx, vals_by_idx, vals_scan, nodes, false, vals_scan + node_count))
         return -1;
 
     // ---------------------------------------------------------------- grid layout
@@ -3391,66 +3804,334 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
     memset(&gi, 0, sizeof(gi));
     gi.root_addr = VDB_ROOT_ADDR;
     gi.nodes_begin = VDB_ROOT_ADDR + VDB_SIZE_ROOT;
-    gi.end = gi.nodes_begin + nupper * VDB_SIZE_UPPER + nlower * VDB_SIZE_LOWER + nleaf * VDB_SIZE_LEAF;
+    gi.end = gi.nodes_begin + nupper.cap * VDB_SIZE_UPPER + nlower.cap * VDB_SIZE_LOWER + nleaf.cap * VDB_SIZE_LEAF;
     gi.meta0 = gi.end;
     gi.meta1 = gi.end + VDB_BLINDMETADATA_SIZE;
     gi.bboxes = gi.meta1 + VDB_BLINDMETADATA_SIZE;
     gi.values = gi.bboxes + 32;
-    gi.value_count = 2 + nvox;
-    gi.grid_size = gi.values + 4 * gi.value_count;
-    gi.upper_count = (uint32_t)nupper;
-    gi.lower_count = (uint32_t)nlower;
-    gi.leaf_count = (uint32_t)nleaf;
+    gi.value_count = 2 + nvox.cap;
+    gi.grid_size = gi.values + 4 * gi.value_count; // exact, or the bound
+    gi.upper_count = (uint32_t)nupper.cap;
+    gi.lower_count = (uint32_t)nlower.cap;
+    gi.leaf_count = (uint32_t)nleaf.cap;
     gi.value_type = in.value_type;
     gi.node_count = (uint32_t)node_count;
     gi.voxel_size = vs;
     gi.voxel_size_inv = vs_inv;
+    GridInfo* gi_dev = nullptr;
+    if (devc)
+    {
+        gi_dev = (GridInfo*)ctx->ws(WS_GB_INFO, sizeof(GridInfo));
+        if (!gi_dev)
+            return -1;
+        hipLaunchKernelGGL(k_grid_info, dim3(1), dim3(64), 0, st, gi, (const uint32_t*)cnt, in.nvox_dev, nvox.cap, gi_dev);
+    }
     uint8_t* grid = (uint8_t*)ctx->ws(WS_GRID, (gi.grid_size + 7) & ~7ull);
     if (!grid)
         return -1;
-    VDBR_CHECK(ctx, hipMemsetAsync(grid, 0, (gi.grid_size + 7) & ~7ull, st));
-    hipLaunchKernelGGL(k_write_header, dim3(1), dim3(64), 0, st, grid, gi, (const int32_t*)vbounds);
-    hipLaunchKernelGGL(k_write_root, dim3(1), dim3(1024), 0, st, grid, gi, (const uint64_t*)upper_key, nupper,
-                       (const uint32_t*)upper_node_idx, (const uint64_t*)size_scan);
-    if (nupper)
-        hipLaunchKernelGGL((k_write_internal<512>), dim3((uint32_t)nupper), dim3(256), 0, st, grid, gi,
-                           (const uint32_t*)upper_first, (const uint64_t*)lower_key, 15u, (const uint32_t*)upper_node_idx,
-                           (const uint32_t*)lower_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan,
-                           VDB_OFF_CMP_UPPER, VDB_OFF_CM_UPPER, VDB_OFF_CHILDREN_UPPER);
-    if (nlower)
-        hipLaunchKernelGGL((k_write_internal<64>), dim3((uint32_t)nlower), dim3(256), 0, st, grid, gi,
-                           (const uint32_t*)lower_first, (const uint64_t*)lkeys, 12u, (const uint32_t*)lower_node_idx,
-                           (const uint32_t*)leaf_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan,
-                           VDB_OFF_CMP_LOWER, VDB_OFF_CM_LOWER, VDB_OFF_CHILDREN_LOWER);
-    if (nleaf)
-        hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nleaf)), dim3(256), 0, st, grid, gi, nleaf,
-                           (const uint32_t*)lvals, (const uint64_t*)leaf_mask, (const uint32_t*)leaf_node_idx,
-                           (const uint64_t*)size_scan, (const uint64_t*)vals_scan);
+    if (devc)
+    {
+        DCount gw = count_div(dev_count(&gi_dev->grid_size, gi.grid_size), 8u);
+        if (fill_words(ctx, (uint64_t*)grid, gw, 0ull))
+            return -1;
+    }
+    else if (fill_words(ctx, (uint64_t*)grid, host_count((gi.grid_size + 7) / 8), 0ull))
+        return -1;
+    hipLaunchKernelGGL(k_write_header, dim3(1), dim3(64), 0, st, grid, gi, (const GridInfo*)gi_dev,
+                       (const int32_t*)vbounds);
+    hipLaunchKernelGGL(k_write_root, dim3(1), dim3(1024), 0, st, grid, gi, (const GridInfo*)gi_dev,
+                       (const uint64_t*)upper_key, nupper, (const uint32_t*)upper_node_idx, (const uint64_t*)size_scan);
+    if (nupper.cap)
+        hipLaunchKernelGGL((k_write_internal<512>), dim3((uint32_t)nupper.cap), dim3(256), 0, st, grid, gi,
+                           (const GridInfo*)gi_dev, (const uint32_t*)upper_first, (const uint64_t*)lower_key, 15u,
+                           (const uint32_t*)upper_node_idx, (const uint32_t*)lower_node_idx,
+                           (const uint64_t*)size_scan, (const uint64_t*)vals_scan, VDB_OFF_CMP_UPPER, VDB_OFF_CM_UPPER,
+                           VDB_OFF_CHILDREN_UPPER, nupper);
+    if (nlower.cap)
+        hipLaunchKernelGGL((k_write_internal<64>), dim3((uint32_t)nlower.cap), dim3(256), 0, st, grid, gi,
+                           (const GridInfo*)gi_dev, (const uint32_t*)lower_first, (const uint64_t*)lkeys, 12u,
+                           (const uint32_t*)lower_node_idx, (const uint32_t*)leaf_node_idx, (const uint64_t*)size_scan,
+                           (const uint64_t*)vals_scan, VDB_OFF_CMP_LOWER, VDB_OFF_CM_LOWER, VDB_OFF_CHILDREN_LOWER,
+                           nlower);
+    if (nleaf.cap)
+        hipLaunchKernelGGL(k_write_leaves, dim3(blocks_for(nleaf.cap)), dim3(256), 0, st, grid, gi,
+                           (const GridInfo*)gi_dev, nleaf, (const uint32_t*)lvals, (const uint64_t*)leaf_mask,
+                           (const uint32_t*)leaf_node_idx, (const uint64_t*)size_scan, (const uint64_t*)vals_scan);
     int32_t* ijk_out = nullptr;
     ctx->last_voxel_ijk = nullptr;
     ctx->last_value_count = 0;
     if (in.voxel_coords)
     {
-        ijk_out = (int32_t*)ctx->ws(WS_VOX_IJK, 3 * (2 + nvox) * sizeof(int32_t));
+        if (devc)
+            return ctx->fail("raster_node2: voxel coordinates are not available in the device-count form");
+        ijk_out = (int32_t*)ctx->ws(WS_VOX_IJK, 3 * (2 + nvox.cap) * sizeof(int32_t));
         if (!ijk_out)
             return -1;
         // the two background slots have no voxel
         VDBR_CHECK(ctx, hipMemsetAsync(ijk_out, 0x80, 6 * sizeof(int32_t), st));
         ctx->last_voxel_ijk = ijk_out;
-        ctx->last_value_count = 2 + nvox;
+        ctx->last_value_count = 2 + nvox.cap;
     }
-    if (nvox)
-        hipLaunchKernelGGL(k_write_values, dim3(blocks_for(nvox)), dim3(256), 0, st, grid, gi, nvox,
-                           (const uint32_t*)vox_val, (const uint64_t*)vox_key, (const uint32_t*)leaf_first,
-                           (const uint32_t*)nullptr, (const uint32_t*)leaf_pos, (const uint32_t*)leaf_node_idx,
-                           (const uint64_t*)vals_scan, (const uint32_t*)vox_leaf, kl, ijk_out);
+    if (nvox.cap)
+        hipLaunchKernelGGL(k_write_values, dim3(std::min<uint32_t>(blocks_for(nvox.cap), 16384u)), dim3(256), 0, st, grid, gi,
+                           (const GridInfo*)gi_dev, nvox, (const uint32_t*)vox_val, (const uint64_t*)vox_key,
+                           (const uint32_t*)leaf_first, (const uint32_t*)nullptr, (const uint32_t*)leaf_pos,
+                           (const uint32_t*)leaf_node_idx, (const uint64_t*)vals_scan, (const uint32_t*)vox_leaf, kl,
+                           ijk_out);
     VDBR_CHECK(ctx, hipGetLastError());
 
     out->grid = grid;
     out->grid_size = gi.grid_size;
-    out->upper_count = nupper;
-    out->lower_count = nlower;
-    out->leaf_count = nleaf;
+    out->upper_count = nupper.cap;
+    out->lower_count = nlower.cap;
+    out->leaf_count = nleaf.cap;
+    out->grid_size_dev = devc ? &gi_dev->grid_size : nullptr;
+    out->counts_dev = devc ? cnt : nullptr;
+    return 0;
+}
+
+// ============================================================================ setup stage
+__global__ void k_init_setup(int32_t* __restrict__ bounds, int32_t* __restrict__ vbounds, unsigned long long* __restrict__ est)
+{
+    if (threadIdx.x < 3)
+    {
+        bounds[threadIdx.x] = 2147483647;
+        vbounds[threadIdx.x] = 2147483647;
+    }
+    else if (threadIdx.x < 6)
+    {
+        bounds[threadIdx.x] = -2147483647;
+        vbounds[threadIdx.x] = -2147483647;
+    }
+    else if (threadIdx.x < 8)
+        est[threadIdx.x - 6] = 0ull;
+}
+
+// Per-Gaussian buffers of one raster call and the words its setup reduces into.
+struct SetupBufs
+{
+    PrimFrame* frames;
+    PrimBox* boxes;
+    PrimVal* pvals;
+    uint64_t* foot;
+    uint64_t* foot_scan; // n + 1 entries (exclusive scan, total last)
+    uint64_t* rows;
+    uint64_t* row_scan; // n + 1 entries
+    int32_t* bounds;    // AABB voxel bounds (min xyz, max xyz)
+    int32_t* vbounds;   // active-voxel bounds, filled by the reduce
+    unsigned long long* est; // [0] pair estimate, [1] cleared
+};
+
+// optional in-place preprocessing (RasterUtils.cpp:59-117), prim setup (gaussian_prim.slang),
+// footprint and row scans; no host synchronisation
+static int launch_setup(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_params_t* prm, const SetupBufs& b,
+                        bool preprocess)
+{
+    hipStream_t st = ctx->stream;
+    const uint64_t n = gs->count;
+    const float vs = prm->voxel_size;
+    if (preprocess && n > 0)
+        hipLaunchKernelGGL(k_preprocess, dim3(blocks_for(n)), dim3(256), 0, st, n, gs->opacities, gs->quats, gs->scales,
+                           gs->sh0, gs->colors);
+    hipLaunchKernelGGL(k_init_setup, dim3(1), dim3(64), 0, st, b.bounds, b.vbounds, b.est);
+    if (n > 0)
+    {
+        SetupOut so{ b.frames, b.boxes, b.pvals, b.foot, b.rows, b.bounds, b.est };
+        hipLaunchKernelGGL(k_setup, dim3(blocks_for(n)), dim3(256), 0, st, n, gs->means, gs->quats, gs->scales,
+                           gs->opacities, gs->colors, vs, 1.f / vs, prm->cull_scale, so);
+        if (scan_u64(ctx, b.foot, b.foot_scan, n, false, b.foot_scan + n))
+            return -1;
+        if (scan_u64(ctx, b.rows, b.row_scan, n, false, b.row_scan + n))
+            return -1;
+    }
+    else if (fill_words(ctx, b.foot_scan, host_count(1), 0ull) || fill_words(ctx, b.row_scan, host_count(1), 0ull))
+        return -1;
+    VDBR_CHECK(ctx, hipGetLastError());
+    return 0;
+}
+
+// ============================================================================ reduce stage
+// A device count past the bound its buffers were sized by sets a flag bit and zeroes the counts
+// that later kernels of the stage derive write positions from (kill32: a u32 count, kill64: a
+// u64 count), so the rest of the captured work runs empty instead of writing out of bounds.
+__global__ void k_check_bound(uint32_t* __restrict__ word, uint64_t cap, uint32_t bit, uint32_t* __restrict__ flags,
+                              uint32_t* __restrict__ kill32, uint64_t* __restrict__ kill64)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && (uint64_t)*word > cap)
+    {
+        atomicOr(flags, bit);
+        *word = 0u;
+        if (kill32)
+            *kill32 = 0u;
+        if (kill64)
+            *kill64 = 0ull;
+    }
+}
+
+// item_scan[*index] = item_scan[at] (moves a scan total to the position a device count names)
+__global__ void k_move_total(uint32_t* __restrict__ item_scan, const uint32_t* __restrict__ index, uint64_t at)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        item_scan[*index] = item_scan[at];
+}
+
+// Per-leaf reduce of one slab's sorted pairs (the pairs are sorted over their leaf bits): leaf
+// runs, staging arrays of 512 slots per leaf (maximum packed RGBA8 value or 192-bit density sum,
+// lowest Gaussian id), the tile or chunk reduce, and the active-voxel count per leaf with its
+// scan. Host form (pairs a host count): the leaf and voxel counts are read back and the buffers
+// fit them. Device form: the counts stay on the device (leaves, svox) and buffers are sized by
+// leaf_cap and vox_cap; a count past its bound sets a bit in *flags.
+struct LeafReduce
+{
+    // in
+    const uint64_t* sorted = nullptr;
+    DCount pairs = host_count(0);
+    KeyLayout kl{};
+    float vs = 0.f;
+    bool rgba_wave = false, density = false;
+    const PrimFrame* frames = nullptr;
+    const PrimVal* pvals = nullptr;
+    const DensPrim* dprims = nullptr;
+    uint64_t leaf_cap = 0, vox_cap = 0; // device form
+    uint32_t* flags = nullptr;          // device form
+    uint32_t* leaves_word = nullptr;    // device form: where the leaf count is kept
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr; // host form: reduce kernel timing
+    // out
+    DCount leaves = host_count(0);
+    DCount svox = host_count(0);
+    uint32_t* lval = nullptr;
+    unsigned long long* lw = nullptr;
+    uint32_t* lg0 = nullptr;
+    uint64_t* lkey = nullptr;
+    uint32_t* lcnt = nullptr;
+};
+
+static int reduce_leaves(Context* ctx, LeafReduce* r)
+{
+    hipStream_t st = ctx->stream;
+    const bool devc = r->pairs.dev != nullptr;
+    const KeyLayout kl = r->kl;
+    const uint32_t lshift = kl.gid_bits + kl.alpha_bits + 9u;
+    const uint32_t* tile_offsets = nullptr;
+    const uint32_t* leaves_dev = nullptr;
+    uint64_t sleaves = 0;
+    if (run_heads(ctx, r->sorted, r->pairs, lshift, nullptr, &sleaves, &leaves_dev, &tile_offsets, true, r->leaf_cap,
+                  r->leaves_word))
+        return -1;
+    r->leaves = devc ? dev_count(leaves_dev, sleaves) : host_count(sleaves);
+    r->svox = host_count(0);
+    if (sleaves == 0)
+        return 0;
+    if (devc)
+        hipLaunchKernelGGL(k_check_bound, dim3(1), dim3(64), 0, st, (uint32_t*)leaves_dev, sleaves, GF_LEAVES, r->flags,
+                           (uint32_t*)nullptr, (uint64_t*)r->pairs.dev);
+    const DCount leaves = r->leaves;
+    if (!r->density)
+        r->lval = (uint32_t*)ctx->ws(WS_LR_VAL, sleaves * 512u * sizeof(uint32_t));
+    else
+        r->lw = (unsigned long long*)ctx->ws(WS_LR_VAL, sleaves * 512u * 3u * sizeof(uint64_t));
+    r->lg0 = (uint32_t*)ctx->ws(WS_LR_G0, sleaves * 512u * sizeof(uint32_t));
+    r->lkey = (uint64_t*)ctx->ws(WS_LR_KEY, sleaves * sizeof(uint64_t));
+    r->lcnt = (uint32_t*)ctx->ws(WS_LR_CNT, (sleaves + 1) * sizeof(uint32_t));
+    if ((!r->lval && !r->lw) || !r->lg0 || !r->lkey || !r->lcnt)
+        return -1;
+    if (!r->density)
+    {
+        if (fill_words(ctx, r->lval, count_mul(leaves, 512u), 0u))
+            return -1;
+    }
+    else if (fill_words(ctx, (uint64_t*)r->lw, count_mul(leaves, 512u * 3u), 0ull))
+        return -1;
+    if (fill_words(ctx, r->lg0, count_mul(leaves, 512u), 0xFFFFFFFFu))
+        return -1;
+    const uint64_t tiles = (r->pairs.cap + HEAD_TILE - 1) / HEAD_TILE;
+    if (r->ev_begin && !ctx->capturing)
+        VDBR_CHECK(ctx, hipEventRecord(r->ev_begin, st));
+    if (!r->rgba_wave && !r->density)
+        hipLaunchKernelGGL(k_leaf_reduce_rgba<false>, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st, r->sorted,
+                           r->pairs, kl, r->vs, tile_offsets, r->frames, r->pvals, r->lval, r->lg0, r->lkey);
+    else
+    {
+        // leaf starts (the sparse head pass over the sorted keys), chunks per leaf, one block per
+        // chunk (RGBA8 with the alpha in the key, or density)
+        const uint32_t chunk = r->density ? LRD_CHUNK : LRC_CHUNK;
+        uint32_t* lstarts = (uint32_t*)ctx->ws(WS_LR_STARTS, (sleaves + 1) * sizeof(uint32_t));
+        uint32_t* litems = (uint32_t*)ctx->ws(WS_LR_ITEMS, (sleaves + 1) * sizeof(uint32_t));
+        if (!lstarts || !litems)
+            return -1;
+        hipLaunchKernelGGL(k_head_write_sorted, dim3((uint32_t)std::min<uint64_t>((tiles + 3) / 4, 8192)), dim3(256), 0, st,
+                           r->sorted, r->pairs, lshift, tile_offsets, lstarts);
+        if (devc)
+            hipLaunchKernelGGL(k_store_end, dim3(1), dim3(64), 0, st, lstarts, leaves_dev, r->pairs);
+        else
+            hipLaunchKernelGGL(k_store_u32, dim3(1), dim3(64), 0, st, lstarts + sleaves, (uint32_t)r->pairs.cap);
+        hipLaunchKernelGGL(k_leaf_chunks, dim3(blocks_for(sleaves)), dim3(256), 0, st, (const uint32_t*)lstarts, leaves,
+                           chunk, litems);
+        // the item scan's total goes after the last leaf (the chunk reduces read item_scan[nleaf])
+        if (scan_u32(ctx, litems, litems, leaves, false, litems + sleaves))
+            return -1;
+        if (devc)
+            hipLaunchKernelGGL(k_move_total, dim3(1), dim3(64), 0, st, litems, leaves_dev, sleaves);
+        const uint64_t items_max = sleaves + r->pairs.cap / chunk + 1;
+        if (items_max > 0x7FFFFFFFull)
+            return ctx->fail("raster_node2: too many reduce chunks");
+        if (r->density)
+            hipLaunchKernelGGL(k_leaf_reduce_density, dim3((uint32_t)items_max), dim3(RED_THREADS), 0, st, r->sorted, kl,
+                               r->vs, (const uint32_t*)lstarts, (const uint32_t*)litems, leaves, r->dprims, r->lw,
+                               r->lg0, r->lkey);
+        else
+            hipLaunchKernelGGL(k_leaf_reduce_rgba_chunks, dim3((uint32_t)items_max), dim3(RED_THREADS), 0, st, r->sorted,
+                               kl, (const uint32_t*)lstarts, (const uint32_t*)litems, leaves, r->pvals, r->lval, r->lg0,
+                               r->lkey);
+    }
+    VDBR_CHECK(ctx, hipGetLastError());
+    if (r->ev_end && !ctx->capturing)
+        VDBR_CHECK(ctx, hipEventRecord(r->ev_end, st));
+    const uint32_t lblocks = (uint32_t)((sleaves + 3) / 4);
+    hipLaunchKernelGGL(k_leaf_count, dim3(lblocks), dim3(256), 0, st, (const uint32_t*)r->lg0, leaves, r->lcnt);
+    if (scan_u32(ctx, r->lcnt, r->lcnt, leaves, false, r->lcnt + sleaves))
+        return -1;
+    if (devc)
+    {
+        hipLaunchKernelGGL(k_check_bound, dim3(1), dim3(64), 0, st, r->lcnt + sleaves, r->vox_cap, GF_VOXELS, r->flags,
+                           (uint32_t*)leaves_dev, (uint64_t*)nullptr);
+        r->svox = dev_count((const uint32_t*)(r->lcnt + sleaves), r->vox_cap);
+    }
+    else
+    {
+        VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, r->lcnt + sleaves, 4, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipStreamSynchronize(st));
+        r->svox = host_count(*(const uint32_t*)ctx->pinned);
+    }
+    return 0;
+}
+
+struct VoxelLists
+{
+    uint64_t* key;
+    uint32_t* val;
+    uint32_t* t;
+    uint32_t* g0;
+};
+
+// the active voxels of the reduced leaves in key order, their first-touch windows and bounds
+static int write_voxels(Context* ctx, const LeafReduce& r, const VoxelLists& out, const PrimBox* boxes,
+                        const uint64_t* foot_scan, const uint32_t* win_base_dev, uint32_t PB, uint32_t W, int32_t* vbounds)
+{
+    hipStream_t st = ctx->stream;
+    const uint32_t lblocks = (uint32_t)((r.leaves.cap + 3) / 4);
+    if (!r.density)
+        hipLaunchKernelGGL(k_leaf_write, dim3(lblocks), dim3(256), 0, st, (const uint32_t*)r.lval, (const uint32_t*)r.lg0,
+                           (const uint64_t*)r.lkey, (const uint32_t*)r.lcnt, r.leaves, out.key, out.val, out.g0);
+    else
+        hipLaunchKernelGGL(k_leaf_write_density, dim3(lblocks), dim3(256), 0, st, (const unsigned long long*)r.lw,
+                           (const uint32_t*)r.lg0, (const uint64_t*)r.lkey, (const uint32_t*)r.lcnt, r.leaves, out.key,
+                           out.val, out.g0);
+    VDBR_CHECK(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(r.svox.cap), 2048u)), dim3(256), 0, st, r.svox,
+                       r.kl, (const uint64_t*)out.key, (const uint32_t*)out.g0, boxes, foot_scan, win_base_dev, PB, W,
+                       out.t, vbounds);
+    VDBR_CHECK(ctx, hipGetLastError());
     return 0;
 }
 
@@ -3475,7 +4156,6 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     const uint32_t PB = prm->prim_batch_size ? prm->prim_batch_size : 262144u;
     const uint32_t W = prm->ijk_batch_size ? prm->ijk_batch_size : 8388608u;
     const uint32_t max_nodes = prm->max_node_count ? prm->max_node_count : 2097152u;
-    const float vs_inv = 1.f / vs;
     hipStream_t st = ctx->stream;
 
     const int NEV = 16;
@@ -3495,46 +4175,35 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     VDBR_CHECK(ctx, hipEventRecord(ev[0], st));
 
     // ---------------------------------------------------------------- preprocess + setup
-    if ((prm->flags & VDBR_FLAG_RAW_INPUTS) && n > 0)
-    {
-        if (!gs->sh0)
-            return ctx->fail("raster_node2: raw inputs need sh0");
-        hipLaunchKernelGGL(k_preprocess, dim3(blocks_for(n)), dim3(256), 0, st, n, gs->opacities, gs->quats, gs->scales,
-                           gs->sh0, gs->colors);
-    }
+    if ((prm->flags & VDBR_FLAG_RAW_INPUTS) && n > 0 && !gs->sh0)
+        return ctx->fail("raster_node2: raw inputs need sh0");
     const uint64_t nn = n > 0 ? n : 1;
-    PrimFrame* frames = (PrimFrame*)ctx->ws(WS_FRAMES, nn * sizeof(PrimFrame));
-    PrimBox* boxes = (PrimBox*)ctx->ws(WS_BOXES, nn * sizeof(PrimBox));
-    PrimVal* pvals = (PrimVal*)ctx->ws(WS_VALS, nn * sizeof(PrimVal));
-    uint64_t* foot = (uint64_t*)ctx->ws(WS_FOOT, nn * sizeof(uint64_t));
-    uint64_t* foot_scan = (uint64_t*)ctx->ws(WS_FOOT_SCAN, (nn + 1) * sizeof(uint64_t));
-    uint64_t* rows = (uint64_t*)ctx->ws(WS_ROWS, nn * sizeof(uint64_t));
-    uint64_t* row_scan = (uint64_t*)ctx->ws(WS_ROW_SCAN, (nn + 1) * sizeof(uint64_t));
+    SetupBufs sb;
+    sb.frames = (PrimFrame*)ctx->ws(WS_FRAMES, nn * sizeof(PrimFrame));
+    sb.boxes = (PrimBox*)ctx->ws(WS_BOXES, nn * sizeof(PrimBox));
+    sb.pvals = (PrimVal*)ctx->ws(WS_VALS, nn * sizeof(PrimVal));
+    sb.foot = (uint64_t*)ctx->ws(WS_FOOT, nn * sizeof(uint64_t));
+    sb.foot_scan = (uint64_t*)ctx->ws(WS_FOOT_SCAN, (nn + 1) * sizeof(uint64_t));
+    sb.rows = (uint64_t*)ctx->ws(WS_ROWS, nn * sizeof(uint64_t));
+    sb.row_scan = (uint64_t*)ctx->ws(WS_ROW_SCAN, (nn + 1) * sizeof(uint64_t));
     uint8_t* misc = (uint8_t*)ctx->ws(WS_MISC, 4096);
-    int32_t* bounds = (int32_t*)misc;                               // 6 ints: AABB bounds
-    int32_t* vbounds = (int32_t*)(misc + 32);                      // 6 ints: voxel bounds
-    unsigned long long* est = (unsigned long long*)(misc + 64);    // pair estimate
-    unsigned long long* counter = (unsigned long long*)(misc + 72); // emitted pairs
-    if (!frames || !boxes || !pvals || !foot || !foot_scan || !rows || !row_scan || !misc)
+    if (!sb.frames || !sb.boxes || !sb.pvals || !sb.foot || !sb.foot_scan || !sb.rows || !sb.row_scan || !misc)
         return -1;
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, bounds);
-    hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, vbounds);
-    VDBR_CHECK(ctx, hipMemsetAsync(est, 0, 16, st));
-    if (n > 0)
-    {
-        SetupOut so{ frames, boxes, pvals, foot, rows, bounds, est };
-        hipLaunchKernelGGL(k_setup, dim3(blocks_for(n)), dim3(256), 0, st, n, gs->means, gs->quats, gs->scales,
-                           gs->opacities, gs->colors, vs, vs_inv, prm->cull_scale, so);
-        if (scan_u64(ctx, foot, foot_scan, n, false, foot_scan + n))
-            return -1;
-        if (scan_u64(ctx, rows, row_scan, n, false, row_scan + n))
-            return -1;
-    }
-    else
-    {
-        VDBR_CHECK(ctx, hipMemsetAsync(foot_scan, 0, 8, st));
-        VDBR_CHECK(ctx, hipMemsetAsync(row_scan, 0, 8, st));
-    }
+    sb.bounds = (int32_t*)misc;                                     // 6 ints: AABB bounds
+    sb.vbounds = (int32_t*)(misc + 32);                             // 6 ints: voxel bounds
+    sb.est = (unsigned long long*)(misc + 64);                      // pair estimate
+    unsigned long long* counter = (unsigned long long*)(misc + 72); // emitted pairs
+    if (launch_setup(ctx, gs, prm, sb, (prm->flags & VDBR_FLAG_RAW_INPUTS) != 0))
+        return -1;
+    PrimFrame* frames = sb.frames;
+    PrimBox* boxes = sb.boxes;
+    PrimVal* pvals = sb.pvals;
+    uint64_t* foot = sb.foot;
+    uint64_t* foot_scan = sb.foot_scan;
+    uint64_t* row_scan = sb.row_scan;
+    int32_t* bounds = sb.bounds;
+    int32_t* vbounds = sb.vbounds;
+    unsigned long long* est = sb.est;
     // batch starts of the footprint scan (window model)
     const uint64_t nbatch = n > 0 ? (n + PB - 1) / PB : 0;
     std::vector<uint64_t> batch_start(nbatch + 1, 0);
@@ -3669,6 +4338,16 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     // ---------------------------------------------------------------- emit + sort + reduce per slab
     const bool rgba_wave = kl.alpha_bits != 0u;
     const bool density = prm->value_mode == VDBR_VALUE_DENSITY;
+    // row-interval constants of every Gaussian (the row pass of every slab reads them)
+    RowFrame* rframes = nullptr;
+    if (any && n > 0)
+    {
+        rframes = (RowFrame*)ctx->ws(WS_ROWFRAMES, n * sizeof(RowFrame));
+        if (!rframes)
+            return -1;
+        hipLaunchKernelGGL(k_row_frames, dim3(blocks_for(n)), dim3(256), 0, st, n, (const PrimFrame*)frames,
+                           (const PrimBox*)boxes, vs, rframes);
+    }
     DensPrim* dprims = nullptr;
     if (density && n > 0)
     {
@@ -3779,6 +4458,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 ra.total_rows = srows;
                 ra.vs = vs;
                 ra.slab_i0 = whole ? INT32_MIN : sl.i0;
+                ra.rframes = rframes;
                 hipLaunchKernelGGL(k_row_segments, dim3((uint32_t)blocks_rows), dim3(EMIT_THREADS), 0, st, ra, segs,
                                    part_cap, seg_counters);
                 VDBR_CHECK(ctx, hipGetLastError());
@@ -3912,7 +4592,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         if (spairs > 1)
         {
             const bool hist_ready = hist_buf && sort_digit_bits(ctx, spairs) == hist_w;
-            if (radix_sort_u64_noswap(ctx, pairs, spairs, sort_begin, key_bits, &sorted, ev[6], ev[7], &passes,
+            if (radix_sort_u64_noswap(ctx, pairs, host_count(spairs), sort_begin, key_bits, &sorted, ev[6], ev[7], &passes,
                                       hist_ready))
                 return -1;
             sort_passes = passes;
@@ -3922,78 +4602,22 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         last_slab_pairs = spairs;
 
         // reduce per leaf, voxels appended after those of the previous slabs
-        uint64_t svox = 0;
-        const uint32_t* tile_offsets = nullptr;
-        uint64_t sleaves = 0;
-        uint32_t* lval = nullptr;
-        unsigned long long* lw = nullptr;
-        uint32_t* lg0 = nullptr;
-        uint64_t* lkey = nullptr;
-        uint32_t* lcnt = nullptr;
-        {
-            // leaf staging: per leaf 512 slots of (maximum packed RGBA8 value or 192-bit density
-            // sum, lowest Gaussian id)
-            const uint32_t lshift = kl.gid_bits + kl.alpha_bits + 9u;
-            if (run_heads(ctx, sorted, spairs, lshift, nullptr, &sleaves, &tile_offsets, true))
-                return -1;
-            if (sleaves > 0)
-            {
-                if (!density)
-                    lval = (uint32_t*)ctx->ws(WS_LR_VAL, sleaves * 512u * sizeof(uint32_t));
-                else
-                    lw = (unsigned long long*)ctx->ws(WS_LR_VAL, sleaves * 512u * 3u * sizeof(uint64_t));
-                lg0 = (uint32_t*)ctx->ws(WS_LR_G0, sleaves * 512u * sizeof(uint32_t));
-                lkey = (uint64_t*)ctx->ws(WS_LR_KEY, sleaves * sizeof(uint64_t));
-                lcnt = (uint32_t*)ctx->ws(WS_LR_CNT, (sleaves + 1) * sizeof(uint32_t));
-                if ((!lval && !lw) || !lg0 || !lkey || !lcnt)
-                    return -1;
-                if (!density)
-                    VDBR_CHECK(ctx, hipMemsetAsync(lval, 0, sleaves * 512u * sizeof(uint32_t), st));
-                else
-                    VDBR_CHECK(ctx, hipMemsetAsync(lw, 0, sleaves * 512u * 3u * sizeof(uint64_t), st));
-                VDBR_CHECK(ctx, hipMemsetAsync(lg0, 0xFF, sleaves * 512u * sizeof(uint32_t), st));
-                const uint64_t tiles = (spairs + HEAD_TILE - 1) / HEAD_TILE;
-                VDBR_CHECK(ctx, hipEventRecord(ev[10], st));
-                if (rgba_wave)
-                    hipLaunchKernelGGL(k_leaf_reduce_rgba<true>, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
-                                       (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, (const PrimFrame*)frames,
-                                       (const PrimVal*)pvals, lval, lg0, lkey);
-                else if (!density)
-                    hipLaunchKernelGGL(k_leaf_reduce_rgba<false>, dim3((uint32_t)tiles), dim3(RED_THREADS), 0, st,
-                                       (const uint64_t*)sorted, spairs, kl, vs, tile_offsets, (const PrimFrame*)frames,
-                                       (const PrimVal*)pvals, lval, lg0, lkey);
-                else
-                {
-                    // leaf starts (the sparse head pass over the sorted keys), chunks per leaf
-                    uint32_t* lstarts = (uint32_t*)ctx->ws(WS_LR_STARTS, (sleaves + 1) * sizeof(uint32_t));
-                    uint32_t* litems = (uint32_t*)ctx->ws(WS_LR_ITEMS, (sleaves + 1) * sizeof(uint32_t));
-                    if (!lstarts || !litems)
-                        return -1;
-                    hipLaunchKernelGGL(k_head_write_sorted, dim3((uint32_t)((tiles + 3) / 4)), dim3(256), 0, st,
-                                       (const uint64_t*)sorted, spairs, lshift, tiles, tile_offsets, lstarts);
-                    hipLaunchKernelGGL(k_store_u32, dim3(1), dim3(64), 0, st, lstarts + sleaves, (uint32_t)spairs);
-                    hipLaunchKernelGGL(k_leaf_chunks, dim3(blocks_for(sleaves)), dim3(256), 0, st,
-                                       (const uint32_t*)lstarts, sleaves, litems);
-                    if (scan_u32(ctx, litems, litems, sleaves, false, litems + sleaves))
-                        return -1;
-                    const uint64_t items_max = sleaves + spairs / LRD_CHUNK + 1;
-                    if (items_max > 0x7FFFFFFFull)
-                        return ctx->fail("raster_node2: too many density reduce chunks");
-                    hipLaunchKernelGGL(k_leaf_reduce_density, dim3((uint32_t)items_max), dim3(RED_THREADS), 0, st,
-                                       (const uint64_t*)sorted, kl, vs, (const uint32_t*)lstarts,
-                                       (const uint32_t*)litems, sleaves, (const DensPrim*)dprims, lw, lg0, lkey);
-                }
-                VDBR_CHECK(ctx, hipGetLastError());
-                VDBR_CHECK(ctx, hipEventRecord(ev[11], st));
-                const uint32_t lblocks = (uint32_t)((sleaves + 3) / 4);
-                hipLaunchKernelGGL(k_leaf_count, dim3(lblocks), dim3(256), 0, st, (const uint32_t*)lg0, sleaves, lcnt);
-                if (scan_u32(ctx, lcnt, lcnt, sleaves, false, lcnt + sleaves))
-                    return -1;
-                VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, lcnt + sleaves, 4, hipMemcpyDeviceToHost, st));
-                VDBR_CHECK(ctx, hipStreamSynchronize(st));
-                svox = *(const uint32_t*)ctx->pinned;
-            }
-        }
+        LeafReduce lr;
+        lr.sorted = sorted;
+        lr.pairs = host_count(spairs);
+        lr.kl = kl;
+        lr.vs = vs;
+        lr.rgba_wave = rgba_wave;
+        lr.density = density;
+        lr.frames = frames;
+        lr.pvals = pvals;
+        lr.dprims = dprims;
+        lr.ev_begin = ev[10];
+        lr.ev_end = ev[11];
+        if (reduce_leaves(ctx, &lr))
+            return -1;
+        const uint64_t svox = lr.svox.cap;
+        const uint64_t sleaves = lr.leaves.cap;
         if (nvox + svox > vcap || !vox_key)
         {
             const uint64_t want = whole ? std::max<uint64_t>(svox, 1) : std::max<uint64_t>(nvox + svox, vcap + vcap / 2);
@@ -4005,25 +4629,11 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 return -1;
             vcap = want;
         }
-        uint64_t* skey = vox_key + nvox;
-        uint32_t* sval = vox_val + nvox;
-        uint32_t* st_t = vox_t + nvox;
-        uint32_t* sg0 = vox_g0 + nvox;
         if (svox > 0)
         {
-            if (!density)
-                hipLaunchKernelGGL(k_leaf_write, dim3((uint32_t)((sleaves + 3) / 4)), dim3(256), 0, st,
-                                   (const uint32_t*)lval, (const uint32_t*)lg0, (const uint64_t*)lkey,
-                                   (const uint32_t*)lcnt, sleaves, skey, sval, sg0);
-            else
-                hipLaunchKernelGGL(k_leaf_write_density, dim3((uint32_t)((sleaves + 3) / 4)), dim3(256), 0, st,
-                                   (const unsigned long long*)lw, (const uint32_t*)lg0, (const uint64_t*)lkey,
-                                   (const uint32_t*)lcnt, sleaves, skey, sval, sg0);
-            VDBR_CHECK(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k_voxel_finish, dim3(std::min<uint32_t>(blocks_for(svox), 2048u)), dim3(256), 0, st, svox,
-                               kl, (const uint64_t*)skey, (const uint32_t*)sg0, (const PrimBox*)boxes,
-                               (const uint64_t*)foot_scan, (const uint32_t*)win_base_dev, PB, W, st_t, vbounds);
-            VDBR_CHECK(ctx, hipGetLastError());
+            VoxelLists vl{ vox_key + nvox, vox_val + nvox, vox_t + nvox, vox_g0 + nvox };
+            if (write_voxels(ctx, lr, vl, boxes, foot_scan, win_base_dev, PB, W, vbounds))
+                return -1;
         }
         VDBR_CHECK(ctx, hipEventRecord(ev[15], st));
         VDBR_CHECK(ctx, hipEventSynchronize(ev[15]));
@@ -4037,7 +4647,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             acc_reduce += ms;
         if (passes && hipEventElapsedTime(&ms, ev[6], ev[7]) == hipSuccess)
             acc_passes += ms;
-        if (svox && hipEventElapsedTime(&ms, ev[10], ev[11]) == hipSuccess)
+        if (sleaves && hipEventElapsedTime(&ms, ev[10], ev[11]) == hipSuccess)
             acc_reduce_k += ms;
     }
     if (nvox >= 0xFFFFFFFFull)
@@ -4525,6 +5135,798 @@ int density_backward_pairs(Context* ctx, const vdbr_gaussians_t* raw, float vs, 
     if (ms_out)
         VDBR_CHECK(ctx, hipEventElapsedTime(ms_out, e0, e1));
     *used_pairs = 1;
+    return 0;
+}
+
+
+// ============================================================================ graph-captured batch
+// Segment counts per partition -> part counts and the exclusive scan of 64-segment chunks that
+// k_emit_segments walks (the host form computes them from a read back); a partition past its
+// capacity sets GF_SEGMENTS and keeps the segments that fit.
+__global__ void k_seg_plan(const unsigned long long* __restrict__ counters, uint64_t part_cap,
+                           uint64_t* __restrict__ part_count, uint64_t* __restrict__ chunk_scan,
+                           uint32_t* __restrict__ flags)
+{
+    if (blockIdx.x != 0 || threadIdx.x >= 64u)
+        return;
+    const uint32_t q = threadIdx.x;
+    uint64_t c = counters[q * SEG_COUNTER_STRIDE];
+    if (c > part_cap)
+    {
+        atomicOr(flags, GF_SEGMENTS);
+        c = part_cap;
+    }
+    part_count[q] = c;
+    uint64_t chunks = (c + 63) / 64, incl = chunks;
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const uint64_t o = __shfl_up(incl, off, 64);
+        if (q >= (uint32_t)off)
+            incl += o;
+    }
+    chunk_scan[q] = incl - chunks;
+    if (q == 63u)
+    {
+        part_count[SEG_PARTS] = 0;
+        chunk_scan[SEG_PARTS] = incl;
+    }
+}
+
+// emitted pairs -> the sorted count; more than the buffer holds sets GF_PAIRS and empties the rest
+__global__ void k_pairs_final(const unsigned long long* __restrict__ counter, uint64_t cap, uint64_t* __restrict__ npairs,
+                              uint32_t* __restrict__ flags)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const uint64_t c = *counter;
+    if (c > cap)
+    {
+        atomicOr(flags, GF_PAIRS);
+        *npairs = 0;
+    }
+    else
+        *npairs = c;
+}
+
+// Output record of one cloud of a captured batch (device memory, read back after the launch).
+struct CloudRecord
+{
+    uint64_t offset;   // byte offset of the grid in the arena (~0: none)
+    uint64_t size;     // grid bytes
+    uint64_t pairs;    // emitted pairs
+    uint64_t voxels;   // active voxels
+    uint32_t flags;    // GF_* bits: a bound was exceeded, the cloud must run again uncaptured
+    uint32_t upper, lower, leaf;
+};
+
+// reserves the cloud's grid in the arena (a running offset shared by the batch) and records its
+// counts; a flagged cloud or an arena too small gets no grid
+__global__ void k_out_reserve(const uint64_t* __restrict__ grid_size, const uint32_t* __restrict__ node_counts,
+                              const uint32_t* __restrict__ nvox, const unsigned long long* __restrict__ pairs,
+                              unsigned long long* __restrict__ cursor, uint64_t arena_cap, CloudRecord* __restrict__ rec)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0)
+        return;
+    const uint64_t size = *grid_size;
+    rec->pairs = *pairs;
+    rec->voxels = *nvox;
+    rec->upper = node_counts[0];
+    rec->lower = node_counts[1];
+    rec->leaf = node_counts[2];
+    rec->size = size;
+    rec->offset = ~0ull;
+    if (rec->flags != 0u)
+        return;
+    const uint64_t at = atomicAdd(cursor, (unsigned long long)((size + 255) & ~255ull));
+    if (at + size > arena_cap)
+    {
+        rec->flags |= GF_ARENA;
+        return;
+    }
+    rec->offset = at;
+}
+
+__global__ void k_out_copy(const uint64_t* __restrict__ grid, uint8_t* __restrict__ arena,
+                           const CloudRecord* __restrict__ rec)
+{
+    const uint64_t off = rec->offset;
+    if (off == ~0ull)
+        return;
+    const uint64_t words = (rec->size + 7) / 8;
+    uint64_t* dst = (uint64_t*)(arena + off);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = grid[i];
+}
+
+// One cloud of a captured batch: the setup counts read back before capture and the bounds the
+// captured buffers are sized by.
+struct CloudPlan
+{
+    uint32_t item = 0;
+    vdbr_gaussians_t gs{};
+    vdbr_raster_params_t prm{};
+    SetupBufs sb{};
+    uint64_t* batch_starts_dev = nullptr; // foot_scan at every prim-batch start (gathered before capture)
+    bool graph = false;
+    // read back
+    int32_t hb[6] = { 0, 0, 0, 0, 0, 0 };
+    uint64_t est = 0, frag_total = 0, total_rows = 0;
+    // derived
+    uint32_t PB = 0, W = 0, T = 1, max_nodes = 0;
+    uint32_t win_off = 0; // first entry of the cloud's window bases in the batch table
+    KeyLayout kl{};
+    uint32_t key_bits = 0, sort_begin = 0;
+    bool rgba_wave = false, density = false;
+    uint64_t part_cap = 0, pair_cap = 0, leaf_cap = 0, vox_cap = 0, lower_cap = 0, upper_cap = 0;
+};
+
+// the captured stages of one cloud on worker context w (emission to the grid copied into the arena)
+static int capture_cloud(Context* w, const CloudPlan& p, const uint32_t* win_base_dev, CloudRecord* rec,
+                         uint8_t* arena, uint64_t arena_cap, unsigned long long* cursor)
+{
+    hipStream_t st = w->stream;
+    uint32_t* flags = &rec->flags;
+    const uint64_t n = p.gs.count;
+    const float vs = p.prm.voxel_size;
+    if (fill_words(w, flags, host_count(1), 0u))
+        return -1;
+    // counter (u64) and the sorted pair count (u64)
+    uint64_t* words = (uint64_t*)w->ws(WS_GRAPH, 64);
+    if (!words)
+        return -1;
+    unsigned long long* counter = (unsigned long long*)words;
+    uint64_t* npairs = words + 1;
+    DensPrim* dprims = nullptr;
+    if (p.density)
+    {
+        dprims = (DensPrim*)w->ws(WS_DENS_PRIMS, n * sizeof(DensPrim));
+        if (!dprims)
+            return -1;
+        hipLaunchKernelGGL(k_dens_prims, dim3(blocks_for(n)), dim3(256), 0, st, n, p.gs.means, p.gs.quats,
+                           p.gs.scales, p.gs.opacities, vs, dprims);
+    }
+    // ---- emission: candidate segments, then the persistent predicate walk
+    uint64_t* seg_meta = (uint64_t*)w->ws(WS_SEG_META, (SEG_PARTS * SEG_COUNTER_STRIDE + 2 * (SEG_PARTS + 1)) * 8);
+    uint4* segs = (uint4*)w->ws(WS_SEGS, SEG_PARTS * p.part_cap * sizeof(uint4));
+    uint64_t* pairs = (uint64_t*)w->ws(WS_PAIRS, p.pair_cap * sizeof(uint64_t));
+    if (!seg_meta || !segs || !pairs)
+        return -1;
+    unsigned long long* seg_counters = (unsigned long long*)seg_meta;
+    uint64_t* part_count_dev = seg_meta + SEG_PARTS * SEG_COUNTER_STRIDE;
+    uint64_t* chunk_scan_dev = part_count_dev + (SEG_PARTS + 1);
+    if (fill_words(w, seg_meta, host_count(SEG_PARTS * SEG_COUNTER_STRIDE), 0ull))
+        return -1;
+    EmitArgs ra;
+    memset(&ra, 0, sizeof(ra));
+    ra.frames = p.sb.frames;
+    ra.boxes = p.sb.boxes;
+    ra.row_scan = p.sb.row_scan;
+    ra.n = n;
+    ra.total_rows = p.total_rows;
+    ra.vs = vs;
+    ra.slab_i0 = INT32_MIN;
+    RowFrame* rframes = (RowFrame*)w->ws(WS_ROWFRAMES, n * sizeof(RowFrame));
+    if (!rframes)
+        return -1;
+    hipLaunchKernelGGL(k_row_frames, dim3(blocks_for(n)), dim3(256), 0, st, n, (const PrimFrame*)p.sb.frames,
+                       (const PrimBox*)p.sb.boxes, vs, rframes);
+    ra.rframes = rframes;
+    const uint64_t blocks_rows = (p.total_rows + EMIT_ROWS - 1) / EMIT_ROWS;
+    hipLaunchKernelGGL(k_row_segments, dim3((uint32_t)blocks_rows), dim3(EMIT_THREADS), 0, st, ra, segs, p.part_cap,
+                       seg_counters);
+    hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(64), 0, st, (const unsigned long long*)seg_counters, p.part_cap,
+                       part_count_dev, chunk_scan_dev, flags);
+    const uint32_t hist_w = sort_digit_bits(w, p.pair_cap);
+    const uint32_t hbits = p.key_bits - p.sort_begin;
+    unsigned long long* hist_buf =
+        ((hbits + hist_w - 1) / hist_w <= (uint32_t)EMIT_HIST_PASSES) ? sort_hist_buffer(w) : nullptr;
+    if (fill_words(w, words, host_count(2), 0ull))
+        return -1;
+    if (hist_buf && fill_words(w, (uint64_t*)hist_buf, host_count(EMIT_HIST_PASSES * 256), 0ull))
+        return -1;
+    EmitArgs ea = ra;
+    ea.vals = p.sb.pvals;
+    ea.kl = p.kl;
+    ea.out = pairs;
+    ea.cap = p.pair_cap;
+    ea.counter = counter;
+    ea.hist = hist_buf;
+    ea.hist_shift = p.sort_begin;
+    ea.hist_bits = hbits;
+    ea.hist_w = hist_w;
+    ea.segs = segs;
+    ea.seg_part_cap = p.part_cap;
+    ea.seg_part_count = part_count_dev;
+    ea.seg_chunk_scan = chunk_scan_dev;
+    hipLaunchKernelGGL(k_emit_segments, dim3(w->emit_blocks), dim3(EMIT_THREADS), 0, st, ea);
+    hipLaunchKernelGGL(k_pairs_final, dim3(1), dim3(64), 0, st, (const unsigned long long*)counter, p.pair_cap, npairs,
+                       flags);
+    VDBR_CHECK(w, hipGetLastError());
+    // ---- sort over the leaf bits
+    const DCount pc = dev_count((const uint64_t*)npairs, p.pair_cap);
+    uint64_t* sorted = pairs;
+    uint32_t passes = 0;
+    if (radix_sort_u64_noswap(w, pairs, pc, p.sort_begin, p.key_bits, &sorted, nullptr, nullptr, &passes,
+                              hist_buf != nullptr))
+        return -1;
+    // ---- reduce by leaf, voxel lists
+    LeafReduce lr;
+    lr.sorted = sorted;
+    lr.pairs = pc;
+    lr.kl = p.kl;
+    lr.vs = vs;
+    lr.rgba_wave = p.rgba_wave;
+    lr.density = p.density;
+    lr.frames = p.sb.frames;
+    lr.pvals = p.sb.pvals;
+    lr.dprims = dprims;
+    lr.leaf_cap = p.leaf_cap;
+    lr.vox_cap = p.vox_cap;
+    lr.flags = flags;
+    lr.leaves_word = (uint32_t*)(words + 3);
+    if (reduce_leaves(w, &lr))
+        return -1;
+    const uint64_t vc = p.vox_cap > 0 ? p.vox_cap : 1;
+    VoxelLists vl;
+    vl.key = (uint64_t*)w->ws(WS_VOX_KEY, vc * sizeof(uint64_t));
+    vl.val = (uint32_t*)w->ws(WS_VOX_VAL, vc * sizeof(uint32_t));
+    vl.t = (uint32_t*)w->ws(WS_VOX_T, vc * sizeof(uint32_t));
+    vl.g0 = (uint32_t*)w->ws(WS_VOX_G0, vc * sizeof(uint32_t));
+    if (!vl.key || !vl.val || !vl.t || !vl.g0)
+        return -1;
+    // no leaves: the voxel count word is the host 0 of reduce_leaves; an empty device word stands in
+    const uint32_t* nvox_dev = (const uint32_t*)lr.svox.dev;
+    if (!nvox_dev)
+    {
+        nvox_dev = (const uint32_t*)(words + 2);
+        if (fill_words(w, (uint32_t*)(words + 2), host_count(1), 0u))
+            return -1;
+    }
+    else if (write_voxels(w, lr, vl, p.sb.boxes, p.sb.foot_scan, win_base_dev, p.PB, p.W, p.sb.vbounds))
+        return -1;
+    // ---- grid build
+    Node2BuildInput bin;
+    bin.kl = p.kl;
+    bin.vox_key = vl.key;
+    bin.vox_val = vl.val;
+    bin.vox_t = vl.t;
+    bin.nvox = p.vox_cap;
+    bin.vbounds = p.sb.vbounds;
+    bin.T = p.T;
+    bin.max_nodes = p.max_nodes;
+    bin.spatial_order = (p.prm.flags & VDBR_FLAG_SPATIAL_ORDER) != 0;
+    bin.voxel_coords = false;
+    bin.value_type = p.density ? VDB_GRID_TYPE_FLOAT : VDB_GRID_TYPE_RGBA8;
+    bin.voxel_size = vs;
+    bin.nvox_dev = nvox_dev;
+    bin.leaf_cap = p.leaf_cap;
+    bin.lower_cap = p.lower_cap;
+    bin.upper_cap = p.upper_cap;
+    bin.flags_dev = flags;
+    Node2BuildOutput bout;
+    if (build_node2_grid(w, bin, &bout))
+        return -1;
+    // ---- into the arena
+    hipLaunchKernelGGL(k_out_reserve, dim3(1), dim3(64), 0, st, bout.grid_size_dev, bout.counts_dev, nvox_dev,
+                       (const unsigned long long*)counter, cursor, arena_cap, rec);
+    const uint64_t gw = (bout.grid_size + 7) / 8;
+    hipLaunchKernelGGL(k_out_copy, dim3(std::min<uint64_t>((gw + 255) / 256, 2048)), dim3(256), 0, st,
+                       (const uint64_t*)bout.grid, arena, (const CloudRecord*)rec);
+    VDBR_CHECK(w, hipGetLastError());
+    return 0;
+}
+
+// the per-cloud part of raster_node2's set-up decisions (key layout, windows, bounds); false when
+// the cloud is left to raster_node2
+static bool plan_cloud(Context* ctx, CloudPlan& p, const uint64_t* host_starts)
+{
+    const uint64_t n = p.gs.count;
+    p.PB = p.prm.prim_batch_size ? p.prm.prim_batch_size : 262144u;
+    p.W = p.prm.ijk_batch_size ? p.prm.ijk_batch_size : 8388608u;
+    p.max_nodes = p.prm.max_node_count ? p.prm.max_node_count : 2097152u;
+    if (p.frag_total == 0 || (p.prm.flags & VDBR_FLAG_VOXEL_COORDS))
+        return false;
+    for (int a = 0; a < 3; a++)
+        if (p.hb[a] < -65536 || p.hb[3 + a] > 65535)
+            return false; // raster_node2 reports the error
+    // windows of the node-order model (host_starts: foot_scan at every prim-batch start, then the total)
+    const uint64_t nbatch = (n + p.PB - 1) / p.PB;
+    uint64_t wtot = 0;
+    for (uint64_t b = 0; b < nbatch; b++)
+        wtot += (host_starts[b + 1] - host_starts[b] + p.W - 1) / p.W;
+    if (wtot >= (1ull << 20))
+        return false;
+    p.T = (uint32_t)(wtot > 0 ? wtot : 1);
+    KeyLayout& kl = p.kl;
+    memset(&kl, 0, sizeof(kl));
+    uint64_t leaves_bb = 1, lowers_bb = 1, uppers_bb = 1;
+    for (int a = 0; a < 3; a++)
+    {
+        kl.lmin[a] = p.hb[a] >> 3;
+        const int32_t lmax = p.hb[3 + a] >> 3;
+        kl.bits[a] = bits_for((uint64_t)(lmax - kl.lmin[a] + 1));
+        leaves_bb *= (uint64_t)(lmax - kl.lmin[a] + 1);
+        lowers_bb *= (uint64_t)((p.hb[3 + a] >> 7) - (p.hb[a] >> 7) + 1);
+        uppers_bb *= (uint64_t)((p.hb[3 + a] >> 12) - (p.hb[a] >> 12) + 1);
+    }
+    kl.gid_bits = bits_for(n > 1 ? n : 2);
+    kl.voxel_bits = kl.bits[0] + kl.bits[1] + kl.bits[2] + 9u;
+    if (kl.voxel_bits + kl.gid_bits > 64)
+        return false;
+    p.density = p.prm.value_mode == VDBR_VALUE_DENSITY;
+    const bool alpha_in_key = !p.density && kl.voxel_bits + kl.gid_bits + 8u <= 64u;
+    kl.alpha_bits = alpha_in_key ? 8u : 0u;
+    p.rgba_wave = alpha_in_key;
+    p.key_bits = kl.voxel_bits + kl.gid_bits + kl.alpha_bits;
+    p.sort_begin = kl.gid_bits + kl.alpha_bits + 9u;
+    // one slab only (raster_node2 splits larger clouds)
+    const uint64_t slab_budget = p.prm.slab_pairs ? (uint64_t)p.prm.slab_pairs : (1ull << 31);
+    if (p.est > slab_budget + slab_budget / 2)
+        return false;
+    // the bounds of raster_node2's first attempt
+    uint64_t seg_total = p.total_rows + (2 * p.est) / EMIT_SEG + 1024;
+    if (seg_total > p.total_rows + p.frag_total / EMIT_SEG + 1024)
+        seg_total = p.total_rows + p.frag_total / EMIT_SEG + 1024;
+    p.part_cap = seg_total / SEG_PARTS + seg_total / (4 * SEG_PARTS) + 4096;
+    uint64_t cap = p.est + p.est / 4 + 1024;
+    if (cap > p.frag_total)
+        cap = p.frag_total;
+    const uint64_t pair_limit = std::min<uint64_t>(0xFFFFFFFEull, 2 * slab_budget);
+    if (cap > pair_limit)
+        cap = pair_limit;
+    p.pair_cap = cap;
+    p.leaf_cap = std::min<uint64_t>(leaves_bb, cap);
+    p.vox_cap = std::min<uint64_t>(cap, p.leaf_cap * 512u);
+    p.lower_cap = std::min<uint64_t>(lowers_bb, p.leaf_cap);
+    p.upper_cap = std::min<uint64_t>(uppers_bb, p.lower_cap);
+    if (1 + p.upper_cap + p.lower_cap + p.leaf_cap > 0xFFFFFFF0ull)
+        return false;
+    (void)ctx;
+    return true;
+}
+
+// bound of a cloud's grid bytes (the layout of build_node2_grid at the plan's bounds)
+static uint64_t grid_bound(const CloudPlan& p)
+{
+    return VDB_ROOT_ADDR + VDB_SIZE_ROOT + p.upper_cap * VDB_SIZE_UPPER + p.lower_cap * VDB_SIZE_LOWER +
+           p.leaf_cap * VDB_SIZE_LEAF + 2 * VDB_BLINDMETADATA_SIZE + 32 + 4 * (2 + p.vox_cap) + 256;
+}
+
+static size_t align256(size_t x)
+{
+    return (x + 255) & ~(size_t)255;
+}
+
+// the batch runs on the context's own stream (a capturable stream), ordered after the caller's
+struct OwnStream
+{
+    Context* ctx;
+    hipStream_t caller;
+    bool ok = true;
+    OwnStream(Context* c) : ctx(c), caller(c->stream)
+    {
+        if (caller == c->own_stream)
+            return;
+        hipEvent_t e;
+        ok = hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess && hipEventRecord(e, caller) == hipSuccess &&
+             hipStreamWaitEvent(c->own_stream, e, 0) == hipSuccess;
+        (void)hipEventDestroy(e);
+        c->stream = c->own_stream;
+    }
+    ~OwnStream()
+    {
+        if (caller == ctx->own_stream)
+            return;
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess)
+        {
+            (void)hipEventRecord(e, ctx->own_stream);
+            (void)hipStreamWaitEvent(caller, e, 0);
+            (void)hipEventDestroy(e);
+        }
+        ctx->stream = caller;
+    }
+};
+
+int raster_batch_graph(Context* ctx, const std::vector<Context*>& workers, vdbr_batch_item_t* items, uint32_t count,
+                       std::vector<uint32_t>* rerun, std::vector<uint8_t>* setup_done)
+{
+    rerun->clear();
+    setup_done->assign(count, 0);
+    if (count == 0)
+        return 0;
+    if (workers.empty())
+        return ctx->fail("raster batch: no worker contexts");
+    const auto t_begin = std::chrono::steady_clock::now();
+    OwnStream own(ctx);
+    if (!own.ok)
+        return ctx->fail("raster batch: could not order the context stream after the caller's");
+    hipStream_t st = ctx->stream;
+    // ---- validation (raster_node2's checks; invalid clouds fail there with its message)
+    std::vector<CloudPlan> plans;
+    plans.reserve(count);
+    for (uint32_t i = 0; i < count; i++)
+    {
+        const vdbr_gaussians_t& g = items[i].gaussians;
+        const vdbr_raster_params_t& prm = items[i].params;
+        items[i].grid_dev = nullptr;
+        items[i].grid_size = 0;
+        memset(&items[i].stats, 0, sizeof(items[i].stats));
+        const bool ok = prm.voxel_size > 0.f && prm.voxel_size < 3.0e38f && g.count > 0 && g.count < (1ull << 32) &&
+                        g.means && g.opacities && g.quats && g.scales && g.colors &&
+                        (prm.value_mode == VDBR_VALUE_RGBA8 || prm.value_mode == VDBR_VALUE_DENSITY) &&
+                        (!(prm.flags & VDBR_FLAG_RAW_INPUTS) || g.sh0);
+        if (!ok)
+        {
+            rerun->push_back(i);
+            continue;
+        }
+        CloudPlan p;
+        p.item = i;
+        p.gs = g;
+        p.prm = prm;
+        plans.push_back(p);
+    }
+    // ---- setup of every cloud into its own buffers, one read back for all
+    std::vector<size_t> off(plans.size() + 1, 0);
+    std::vector<size_t> starts_at(plans.size() + 1, 0); // u64 entries of the pinned read back
+    for (size_t c = 0; c < plans.size(); c++)
+    {
+        const uint64_t n = plans[c].gs.count;
+        const uint64_t PB = plans[c].prm.prim_batch_size ? plans[c].prm.prim_batch_size : 262144u;
+        const uint64_t nbatch = (n + PB - 1) / PB;
+        off[c + 1] = off[c] + align256(n * sizeof(PrimFrame)) + align256(n * sizeof(PrimBox)) +
+                     align256(n * sizeof(PrimVal)) + 2 * align256(n * 8) + 2 * align256((n + 1) * 8) + 256 +
+                     align256((nbatch + 1) * 8);
+        starts_at[c + 1] = starts_at[c] + 8 + nbatch + 1;
+    }
+    uint8_t* setup = (uint8_t*)ctx->ws(WS_BATCH_SETUP, off[plans.size()] + 256);
+    if (!setup)
+        return -1;
+    if (ctx->batch_pinned_words < starts_at[plans.size()])
+    {
+        if (ctx->batch_pinned)
+            (void)hipHostFree(ctx->batch_pinned);
+        ctx->batch_pinned = nullptr;
+        ctx->batch_pinned_words = 0;
+        if (hipHostMalloc((void**)&ctx->batch_pinned, starts_at[plans.size()] * 8, hipHostMallocDefault) != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return ctx->fail("raster batch: pinned read-back buffer");
+        }
+        ctx->batch_pinned_words = starts_at[plans.size()];
+    }
+    for (size_t c = 0; c < plans.size(); c++)
+    {
+        CloudPlan& p = plans[c];
+        const uint64_t n = p.gs.count;
+        uint8_t* b = setup + off[c];
+        auto take = [&](size_t bytes) {
+            uint8_t* r = b;
+            b += align256(bytes);
+            return r;
+        };
+        p.sb.frames = (PrimFrame*)take(n * sizeof(PrimFrame));
+        p.sb.boxes = (PrimBox*)take(n * sizeof(PrimBox));
+        p.sb.pvals = (PrimVal*)take(n * sizeof(PrimVal));
+        p.sb.foot = (uint64_t*)take(n * 8);
+        p.sb.rows = (uint64_t*)take(n * 8);
+        p.sb.foot_scan = (uint64_t*)take((n + 1) * 8);
+        p.sb.row_scan = (uint64_t*)take((n + 1) * 8);
+        uint8_t* wordsp = take(256);
+        p.sb.bounds = (int32_t*)wordsp;
+        p.sb.vbounds = (int32_t*)(wordsp + 32);
+        p.sb.est = (unsigned long long*)(wordsp + 64);
+        const uint64_t PB = p.prm.prim_batch_size ? p.prm.prim_batch_size : 262144u;
+        const uint64_t nbatch = (n + PB - 1) / PB;
+        p.batch_starts_dev = (uint64_t*)take((nbatch + 1) * 8);
+        if (launch_setup(ctx, &p.gs, &p.prm, p.sb, (p.prm.flags & VDBR_FLAG_RAW_INPUTS) != 0))
+            return -1;
+        (*setup_done)[p.item] = 1; // raw inputs are preprocessed in place from here on
+        hipLaunchKernelGGL(k_gather_u64, dim3(blocks_for(nbatch + 1)), dim3(256), 0, st, p.sb.foot_scan, PB,
+                           nbatch, p.batch_starts_dev);
+        uint64_t* h = ctx->batch_pinned + starts_at[c];
+        VDBR_CHECK(ctx, hipMemcpyAsync(h, p.sb.bounds, 24, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipMemcpyAsync(h + 3, p.sb.est, 8, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipMemcpyAsync(h + 4, p.sb.foot_scan + n, 8, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipMemcpyAsync(h + 5, p.sb.row_scan + n, 8, hipMemcpyDeviceToHost, st));
+        VDBR_CHECK(ctx, hipMemcpyAsync(h + 8, p.batch_starts_dev, nbatch * 8, hipMemcpyDeviceToHost, st));
+    }
+    VDBR_CHECK(ctx, hipStreamSynchronize(st));
+    // ---- plans, window tables, output bounds
+    std::vector<uint32_t> win_table;
+    uint64_t arena_need = 0;
+    std::vector<size_t> graph_clouds;
+    for (size_t c = 0; c < plans.size(); c++)
+    {
+        CloudPlan& p = plans[c];
+        const uint64_t* h = ctx->batch_pinned + starts_at[c];
+        memcpy(p.hb, h, 24);
+        p.est = h[3];
+        p.frag_total = h[4];
+        p.total_rows = h[5];
+        const uint64_t n = p.gs.count;
+        const uint64_t PB = p.prm.prim_batch_size ? p.prm.prim_batch_size : 262144u;
+        const uint64_t nbatch = (n + PB - 1) / PB;
+        std::vector<uint64_t> bs(nbatch + 1);
+        for (uint64_t b = 0; b < nbatch; b++)
+            bs[b] = h[8 + b];
+        bs[nbatch] = p.frag_total;
+        p.graph = plan_cloud(ctx, p, bs.data());
+        if (!p.graph)
+        {
+            rerun->push_back(p.item);
+            continue;
+        }
+        p.win_off = (uint32_t)win_table.size();
+        const uint64_t W = p.W;
+        uint64_t wtot = 0;
+        for (uint64_t b = 0; b < nbatch; b++)
+        {
+            win_table.push_back((uint32_t)wtot);
+            wtot += (bs[b + 1] - bs[b] + W - 1) / W;
+        }
+        win_table.push_back((uint32_t)wtot);
+        arena_need += align256(grid_bound(p));
+        graph_clouds.push_back(c);
+    }
+    if (graph_clouds.empty())
+        return 0;
+    // ---- per-cloud records and window tables (uploaded before capture)
+    const size_t rec_bytes = align256(graph_clouds.size() * sizeof(CloudRecord));
+    uint8_t* table = (uint8_t*)ctx->ws(WS_BATCH_TABLE, rec_bytes + 256 + align256(win_table.size() * 4));
+    if (!table)
+        return -1;
+    CloudRecord* recs = (CloudRecord*)table;
+    unsigned long long* cursor = (unsigned long long*)(table + rec_bytes);
+    uint32_t* win_dev = (uint32_t*)(table + rec_bytes + 256);
+    VDBR_CHECK(ctx, hipMemcpyAsync(win_dev, win_table.data(), win_table.size() * 4, hipMemcpyHostToDevice, st));
+    // The arena: every grid at its bound would often be ten times the grids' real size, so it is
+    // sized from the grids of the last batch on this context (8 GiB before the first) and at most a
+    // quarter of the free device memory; a grid that does not fit is flagged and its cloud runs
+    // again outside the graph.
+    uint64_t arena_want = std::min<uint64_t>(arena_need, ctx->batch_arena_hint ? ctx->batch_arena_hint : (8ull << 30));
+    {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            arena_want = std::min<uint64_t>(arena_want, std::max<uint64_t>((uint64_t)free_b / 4 + ctx->ws_size[WS_BATCH_ARENA],
+                                                                          256ull << 20));
+        (void)hipGetLastError();
+    }
+    uint8_t* arena = (uint8_t*)ctx->ws(WS_BATCH_ARENA, std::max<uint64_t>(arena_want, 256));
+    if (!arena)
+        return -1;
+    const uint64_t arena_cap = ctx->ws_size[WS_BATCH_ARENA];
+    // ---- clouds to worker streams, largest estimate first onto the least loaded
+    const size_t K = std::min<size_t>(workers.size(), graph_clouds.size());
+    std::vector<std::vector<size_t>> assign(K);
+    {
+        std::vector<size_t> order = graph_clouds;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return plans[a].est > plans[b].est; });
+        std::vector<uint64_t> load(K, 0);
+        for (size_t c : order)
+        {
+            size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            assign[k].push_back(c);
+            load[k] += plans[c].est + plans[c].gs.count * 64;
+        }
+    }
+    std::vector<size_t> rec_of(plans.size(), 0);
+    for (size_t r = 0; r < graph_clouds.size(); r++)
+        rec_of[graph_clouds[r]] = r;
+    for (size_t k = 0; k < K; k++)
+        std::copy(ctx->sort_tuned, ctx->sort_tuned + SORT_BUCKETS, workers[k]->sort_tuned);
+
+    // ---- every host value the capture depends on: pointers and sizes of the workspaces, the
+    // clouds and their plans, the split over workers. An unchanged signature relaunches the graph
+    // captured by an earlier call instead of capturing again.
+    auto signature = [&]() {
+        std::vector<uint64_t> sig;
+        auto put = [&](const void* p, size_t bytes) {
+            const uint8_t* b = (const uint8_t*)p;
+            for (size_t x = 0; x < bytes; x += 8)
+            {
+                uint64_t v = 0;
+                memcpy(&v, b + x, std::min<size_t>(8, bytes - x));
+                sig.push_back(v);
+            }
+        };
+        sig.push_back(K);
+        for (WsSlot sl : { WS_BATCH_SETUP, WS_BATCH_TABLE, WS_BATCH_ARENA })
+        {
+            sig.push_back((uint64_t)(uintptr_t)ctx->ws_ptr[sl]);
+            sig.push_back(ctx->ws_size[sl]);
+        }
+        for (size_t k = 0; k < K; k++)
+        {
+            const Context* w = workers[k];
+            put(w->ws_ptr, sizeof(w->ws_ptr));
+            put(w->ws_size, sizeof(w->ws_size));
+            put(w->sort_tuned, sizeof(w->sort_tuned));
+            sig.push_back(w->emit_blocks);
+            sig.push_back((uint64_t)(uintptr_t)w->stream);
+            sig.push_back(assign[k].size());
+            for (size_t c : assign[k])
+            {
+                const CloudPlan& p = plans[c];
+                sig.push_back(p.item);
+                put(&p.gs, sizeof(p.gs));
+                put(&p.prm, sizeof(p.prm));
+                put(&p.sb, sizeof(p.sb));
+                put(p.hb, sizeof(p.hb));
+                sig.push_back(p.est);
+                sig.push_back(p.frag_total);
+                sig.push_back(p.total_rows);
+                sig.push_back(p.T);
+                sig.push_back(p.win_off);
+            }
+        }
+        return sig;
+    };
+    const auto t_plan = std::chrono::steady_clock::now();
+    hipGraphExec_t exec = nullptr;
+    bool captured = false;
+    if (ctx->batch_exec && ctx->batch_sig == signature())
+        exec = (hipGraphExec_t)ctx->batch_exec;
+    else
+    {
+        if (ctx->batch_exec)
+            (void)hipGraphExecDestroy((hipGraphExec_t)ctx->batch_exec);
+        ctx->batch_exec = nullptr;
+        ctx->batch_sig.clear();
+        // a sizing pass (discarded), the workspaces grown to it, then the real capture
+        hipGraph_t graph = nullptr;
+        for (int pass = 0; pass < 2; pass++)
+        {
+            const bool probe = pass == 0;
+            if (!probe)
+            {
+                for (size_t k = 0; k < K; k++)
+                {
+                    Context* w = workers[k];
+                    for (int sl = 0; sl < WS_COUNT; sl++)
+                        if (w->probe_size[sl] && !w->ws((WsSlot)sl, w->probe_size[sl]))
+                            return ctx->fail("raster batch: worker workspace: " + w->error);
+                    // the captured sorts use look-back epochs from 1 (the status words are cleared
+                    // before every launch of the graph)
+                    w->sort_epoch = 0;
+                    VDBR_CHECK(ctx, hipStreamSynchronize(w->stream));
+                }
+            }
+            hipEvent_t fork = nullptr;
+            std::vector<hipEvent_t> joins(K, nullptr);
+            VDBR_CHECK(ctx, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+            for (size_t k = 0; k < K; k++)
+                VDBR_CHECK(ctx, hipEventCreateWithFlags(&joins[k], hipEventDisableTiming));
+            struct EvG
+            {
+                hipEvent_t f;
+                std::vector<hipEvent_t>& j;
+                ~EvG()
+                {
+                    (void)hipEventDestroy(f);
+                    for (auto e : j)
+                        if (e)
+                            (void)hipEventDestroy(e);
+                }
+            } evg{ fork, joins };
+            VDBR_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int rc = 0;
+            std::string err;
+            if (hipEventRecord(fork, st) != hipSuccess)
+                rc = -1, err = "event record";
+            for (size_t k = 0; k < K && rc == 0; k++)
+            {
+                Context* w = workers[k];
+                w->probe = probe;
+                w->capturing = true;
+                if (probe)
+                    std::fill(w->probe_size, w->probe_size + WS_COUNT, (size_t)0);
+                if (hipStreamWaitEvent(w->stream, fork, 0) != hipSuccess)
+                    rc = -1, err = "stream wait";
+                for (size_t c : assign[k])
+                {
+                    if (rc)
+                        break;
+                    const CloudPlan& p = plans[c];
+                    if (capture_cloud(w, p, win_dev + p.win_off, recs + rec_of[c], arena, arena_cap, cursor))
+                        rc = -1, err = w->error;
+                }
+                if (rc == 0 && (hipEventRecord(joins[k], w->stream) != hipSuccess ||
+                                hipStreamWaitEvent(st, joins[k], 0) != hipSuccess))
+                    rc = -1, err = "join";
+                w->probe = false;
+                w->capturing = false;
+            }
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc || e != hipSuccess)
+            {
+                if (g)
+                    (void)hipGraphDestroy(g);
+                (void)hipGetLastError();
+                return ctx->fail("raster batch: capture failed: " +
+                                 (err.empty() ? std::string(hipGetErrorString(e)) : err));
+            }
+            if (probe)
+                (void)hipGraphDestroy(g);
+            else
+                graph = g;
+        }
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess)
+        {
+            (void)hipGetLastError();
+            return ctx->fail(std::string("raster batch: graph instantiation: ") + hipGetErrorString(e));
+        }
+        ctx->batch_exec = exec;
+        ctx->batch_sig = signature();
+        captured = true;
+    }
+    const auto t_graph = std::chrono::steady_clock::now();
+    // before every launch: the output cursor and the workers' look-back status words cleared
+    if (fill_words(ctx, (uint64_t*)cursor, host_count(1), 0ull))
+        return -1;
+    for (size_t k = 0; k < K; k++)
+    {
+        Context* w = workers[k];
+        if (w->ws_ptr[WS_SORT_STATUS] &&
+            fill_words(ctx, (uint64_t*)w->ws_ptr[WS_SORT_STATUS], host_count(w->ws_size[WS_SORT_STATUS] / 8), 0ull))
+            return -1;
+    }
+    hipError_t e = hipGraphLaunch(exec, st);
+    std::vector<CloudRecord> hrec(graph_clouds.size());
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(hrec.data(), recs, graph_clouds.size() * sizeof(CloudRecord), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return ctx->fail(std::string("raster batch: graph launch: ") + hipGetErrorString(e));
+    }
+    const auto t_run = std::chrono::steady_clock::now();
+    auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return (float)std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    ctx->batch_ms[0] = ms(t_begin, t_plan);
+    ctx->batch_ms[1] = ms(t_plan, t_graph);
+    ctx->batch_ms[2] = ms(t_graph, t_run);
+    ctx->batch_ms[3] = captured ? 1.f : 0.f;
+    // the next batch sizes its arena from these grids (with a quarter of headroom)
+    uint64_t grids_total = 0;
+    for (const CloudRecord& cr : hrec)
+        grids_total += (cr.size + 255) & ~255ull;
+    ctx->batch_arena_hint = std::max<uint64_t>(grids_total + grids_total / 4, 256ull << 20);
+    for (size_t r = 0; r < graph_clouds.size(); r++)
+    {
+        const CloudPlan& p = plans[graph_clouds[r]];
+        const CloudRecord& cr = hrec[r];
+        vdbr_batch_item_t& it = items[p.item];
+        if (cr.flags != 0u || cr.offset == ~0ull)
+        {
+            rerun->push_back(p.item);
+            continue;
+        }
+        it.grid_dev = arena + cr.offset;
+        it.grid_size = cr.size;
+        vdbr_raster_stats_t& s = it.stats;
+        s.gaussian_count = p.gs.count;
+        s.fragment_count = p.frag_total;
+        s.pair_count = cr.pairs;
+        s.voxel_count = cr.voxels;
+        s.grid_size = cr.size;
+        s.upper_count = cr.upper;
+        s.lower_count = cr.lower;
+        s.leaf_count = cr.leaf;
+        s.window_count = p.T;
+        s.key_bits = p.key_bits;
+        s.sort_bit_begin = p.sort_begin;
+        s.emit_attempts = 1;
+        s.slab_count = 1;
+    }
+    std::sort(rerun->begin(), rerun->end());
     return 0;
 }
 

@@ -95,6 +95,8 @@ SIGNATURES = {
                        ctypes.POINTER(u64)], ctypes.c_int),
     "vdbr_radix_sort_dual_key": ([P, P, P, P, u64, u32, u32], ctypes.c_int),
     "vdbr_raster_node2_batch": ([P, ctypes.POINTER(VdbrBatchItem), u32, u32], ctypes.c_int),
+    "vdbr_last_batch_graph_clouds": ([P], u32),
+    "vdbr_last_batch_timing": ([P, ctypes.POINTER(ctypes.c_float)], ctypes.c_int),
     "vdbr_autotune_sort": ([P, u64, u32, ctypes.POINTER(VdbrTuneResult), u32, ctypes.POINTER(u32),
                             ctypes.POINTER(u32), ctypes.POINTER(u32)], ctypes.c_int),
     "vdbr_set_sort_config": ([P, u64, u32, u32], ctypes.c_int),

@@ -41,6 +41,8 @@ class Voxelizer:
         self.device = torch.device("cuda", int(device))
         self.last_backward_used_pairs = False
         self.last_pair_list_id = 0  # stats["pair_list_id"] of the last raster_node2 call
+        self.last_batch_graph_clouds = 0  # clouds of the last batched call run inside its graph
+        self.last_batch_timing = {}  # host-clock phases of the last batched call
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -107,8 +109,11 @@ class Voxelizer:
         """Many clouds in one call. ``clouds`` is a sequence of dicts with the tensors of
         ``raster_node2`` (means, quats, scales, colors, opacities, and sh0 when raw) plus
         ``voxel_size`` and optional ``raw``, ``value_mode``, ``cull_scale``. Each result equals a
-        ``raster_node2`` call on that cloud alone. Returns ``(grids, stats)``: uint8 GPU tensors
-        (or None without ``copy_out``) and stats dicts, in input order."""
+        ``raster_node2`` call on that cloud alone. After the setup of every cloud, the stages of
+        all clouds run as one captured HIP graph over ``workers`` streams;
+        ``last_batch_graph_clouds`` counts the clouds that ran inside it. Returns
+        ``(grids, stats)``: uint8 GPU tensors (or None without ``copy_out``) and stats dicts, in
+        input order."""
         items = (VdbrBatchItem * max(len(clouds), 1))()
         keep = []
         for i, c in enumerate(clouds):
@@ -127,6 +132,11 @@ class Voxelizer:
         self._bind_stream()
         if self._lib.vdbr_raster_node2_batch(self._ctx, items, len(clouds), int(workers)) != 0:
             self._err("vdbr_raster_node2_batch")
+        self.last_batch_graph_clouds = int(self._lib.vdbr_last_batch_graph_clouds(self._ctx))
+        ms = (ctypes.c_float * 5)()
+        self._lib.vdbr_last_batch_timing(self._ctx, ms)
+        self.last_batch_timing = {"setup_ms": ms[0], "capture_ms": ms[1], "graph_ms": ms[2],
+                                  "captured": bool(ms[3]), "rerun_ms": ms[4]}
         grids, stats = [], []
         for i in range(len(clouds)):
             stats.append(items[i].stats.as_dict())

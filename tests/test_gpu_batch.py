@@ -1,4 +1,5 @@
-"""Batched raster3d: every cloud of a batch equals a single call on that cloud (and the oracle)."""
+"""Batched raster3d (one captured HIP graph for the clouds it takes): every cloud of a batch
+equals a single call on that cloud (and the oracle)."""
 import numpy as np
 import pytest
 
@@ -55,6 +56,9 @@ def test_batch_equals_single_calls(vox, workers):
     batch = [_tensors(c, 1 if density else 0) for c, density in clouds]
     print(f"batch of {len(batch)} clouds, workers={workers}", flush=True)
     grids, stats = vox.raster_node2_batch(batch, workers=workers)
+    # every cloud with fragments runs inside the captured graph (the empty cloud does not)
+    with_pairs = sum(1 for _, rst in expect if rst["fragment_count"] > 0)
+    assert vox.last_batch_graph_clouds == with_pairs
     for i, ((ref, rst), got, st) in enumerate(zip(expect, grids, stats)):
         assert st["pair_count"] == rst["pair_count"], i
         assert st["voxel_count"] == rst["voxel_count"], i
@@ -66,6 +70,7 @@ def test_batch_equals_single_calls(vox, workers):
 def test_batch_matches_oracle(vox):
     clouds = [small_uniform(900, 0.01, 31), small_clustered(2500, 0.005, 32), small_uniform(400, 0.02, 33)]
     grids, _ = vox.raster_node2_batch([_tensors(c) for c in clouds], workers=2)
+    assert vox.last_batch_graph_clouds == len(clouds)
     for c, g in zip(clouds, grids):
         m, op, q, s, col = preprocessed(c)
         ref, _ = oracle.raster_node2(m, q, s, col, op, c.voxel_size)
@@ -79,8 +84,31 @@ def test_batch_reuses_arena_and_reports_failing_cloud(vox):
     again, _ = vox.raster_node2_batch([_tensors(c) for c in clouds], workers=2)
     for a, b in zip(first, again):
         assert np.array_equal(a, b.cpu().numpy())
+    assert vox.last_batch_graph_clouds == len(clouds)
     bad = [_tensors(c) for c in clouds]
     bad[1]["voxel_size"] = -1.0
     with pytest.raises(RuntimeError, match="cloud 1"):
         vox.raster_node2_batch(bad, workers=2)
     assert vox.raster_node2_batch([], workers=2) == ([], [])
+
+
+def test_batch_relaunches_captured_graph(vox):
+    """A second call with the same clouds, plans and workspaces launches the graph captured by the
+    first call again (status words and output cursor cleared per launch): same grids."""
+    clouds = [small_uniform(800, 0.01, 51), small_clustered(3000, 0.005, 52), small_uniform(300, 0.02, 53)]
+    items = [_tensors(c) for c in clouds]
+    first, _ = vox.raster_node2_batch(items, workers=2)  # preprocesses the raw arrays in place
+    for d in items:
+        d["raw"] = False
+    second, _ = vox.raster_node2_batch(items, workers=2)
+    assert vox.last_batch_timing["captured"]
+    third, st3 = vox.raster_node2_batch(items, workers=2)
+    assert not vox.last_batch_timing["captured"], vox.last_batch_timing
+    assert vox.last_batch_graph_clouds == len(clouds)
+    for c, a, b, g in zip(clouds, first, second, third):
+        ref = a.cpu().numpy()
+        assert np.array_equal(b.cpu().numpy(), ref)
+        assert np.array_equal(g.cpu().numpy(), ref)
+        m, op, q, s, col = preprocessed(c)
+        exp, _ = oracle.raster_node2(m, q, s, col, op, c.voxel_size)
+        assert np.array_equal(ref, exp)

@@ -241,7 +241,8 @@ def test_cfg4_batch_256(vox):
         items.append(d)
     grids, stats = vox.raster_node2_batch(items, workers=4)
     _say(f"cfg4: {len(clouds)} clouds, {sum(c.count for c in clouds)} Gaussians, "
-         f"{sum(s['pair_count'] for s in stats)} pairs")
+         f"{sum(s['pair_count'] for s in stats)} pairs, {vox.last_batch_graph_clouds} in the captured graph")
+    assert vox.last_batch_graph_clouds == len(clouds)
     for i, c in enumerate(clouds):
         single, st = _gpu_grid(vox, c)
         assert st["pair_count"] == stats[i]["pair_count"]
@@ -256,6 +257,7 @@ def test_cfg4_batch_256(vox):
 
 # ----------------------------------------------------------------------------------------- cfg 5
 def test_cfg5_clustered_16m(vox):
+    vox.trim()  # the workspace of the earlier configurations (batch workers and arena included)
     cloud = syn.workload_clustered_16m()
     t0 = time.perf_counter()
     grid, st = _gpu_grid(vox, cloud, max_node_count=0xFFFFFFFF)
