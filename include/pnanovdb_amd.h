@@ -528,24 +528,25 @@ typedef struct vdbr_batch_item_t
  * vdbr_raster_node2 call on that cloud alone. The per-Gaussian setup of every cloud runs first
  * with one read back of its counts; from those the call bounds each cloud's buffers and captures
  * the remaining stages of all clouds (emission, sort, reduce, grid build, the grid copied into a
- * context-owned output arena) as ONE HIP graph, spread over `workers` streams (0 -> 4, at most
- * 16; each with its own workspace, clouds balanced by their pair estimates), and launches it
- * once: no count is read back between stages. Clouds the capture does not take (more than one
- * slab, no fragments, invalid input) and clouds whose counts exceeded their bounds run after the
- * graph through vdbr_raster_node2. The work is ordered after the context's stream and the
- * context's stream after it. On failure the message names the first failing cloud. */
+ * context-owned output arena) as HIP graphs, one per worker stream (`workers`: 0 -> 4, at most
+ * 16; each with its own workspace, clouds balanced by their pair estimates), launched together:
+ * no count is read back between stages. A later call with the same clouds, plans and workspaces
+ * relaunches the captured graphs. Clouds the capture does not take (more than one slab, no
+ * fragments, invalid input) and clouds whose counts exceeded their bounds run after the graphs
+ * through vdbr_raster_node2. The work is ordered after the context's stream and the context's
+ * stream after it. On failure the message names the first failing cloud. */
 PNANOVDB_API int vdbr_raster_node2_batch(vdbr_context_t* ctx, vdbr_batch_item_t* items, uint32_t count,
                                          uint32_t workers);
 
-/* Clouds of the last vdbr_raster_node2_batch call on ctx that ran inside its captured graph
+/* Clouds of the last vdbr_raster_node2_batch call on ctx that ran inside its captured graphs
  * (no counterpart in the reference). */
 PNANOVDB_API uint32_t vdbr_last_batch_graph_clouds(vdbr_context_t* ctx);
 
 /* Host-clock phases of the last vdbr_raster_node2_batch call, in milliseconds (no counterpart in
  * the reference): ms[0] set-up of every cloud and its read back, ms[1] capture and instantiation
- * of the graph (0 when the graph of an earlier call with the same signature -- the same clouds,
- * plans and workspaces -- was launched again), ms[2] graph launch to completion, ms[3] 1 when the
- * graph was captured by this call, ms[4] clouds run outside the graph. */
+ * of the graphs (close to 0 when the graphs of an earlier call with the same signature -- the
+ * same clouds, plans and workspaces -- were launched again), ms[2] graph launches to completion,
+ * ms[3] 1 when the graphs were captured by this call, ms[4] clouds run outside the graphs. */
 PNANOVDB_API int vdbr_last_batch_timing(vdbr_context_t* ctx, float* ms);
 
 /* Adjoint of the density value mode (no counterpart in the reference, which has no backward

@@ -24,6 +24,14 @@ namespace vdbr
 
 static const size_t PINNED_BYTES = 1u << 20;
 
+void Context::drop_batch_graphs()
+{
+    for (hipGraphExec_t x : batch_execs)
+        (void)hipGraphExecDestroy(x);
+    batch_execs.clear();
+    batch_sig.clear();
+}
+
 void Context::ws_free(WsSlot slot)
 {
     if (!ws_ptr[slot])
@@ -128,10 +136,7 @@ void Context::release()
 {
     (void)hipStreamSynchronize(stream);
     // a captured batch graph refers to the workspace being released
-    if (batch_exec)
-        (void)hipGraphExecDestroy((hipGraphExec_t)batch_exec);
-    batch_exec = nullptr;
-    batch_sig.clear();
+    drop_batch_graphs();
     for (int i = 0; i < WS_COUNT; i++)
     {
         if (ws_ptr[i])

@@ -173,8 +173,9 @@ struct Context
     std::vector<Context*> batch_workers;
     std::vector<std::pair<void*, size_t>> batch_chunks;
     uint32_t last_batch_graph_clouds = 0; // clouds of the last batched call that ran in its graph
-    // instantiated graph of the last captured batch and the signature it was captured under
-    void* batch_exec = nullptr;
+    // instantiated graphs (one per worker stream) of the last captured batch and the signature
+    // they were captured under
+    std::vector<hipGraphExec_t> batch_execs;
     uint64_t batch_arena_hint = 0; // arena bytes the last batch's grids needed, with headroom
     std::vector<uint64_t> batch_sig;
     // host milliseconds of the last batched call: set-up and read back, capture and
@@ -193,6 +194,7 @@ struct Context
         error = msg;
         return -1;
     }
+    void drop_batch_graphs();
     void* ws(WsSlot slot, size_t bytes);
     void ws_free(WsSlot slot); // releases one slot (after the stream has drained)
     void* scratch_scan(size_t bytes)

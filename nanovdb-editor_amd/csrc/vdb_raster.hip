@@ -551,14 +551,20 @@ __device__ __forceinline__ void row_interval(const PrimFrame& f, const PrimBox& 
 // The candidate voxels (i, j, k0 + st), st < len, of one AABB row, walked in lock step by the
 // wave: steps is the wave maximum of len, and on_step(st, keep, vmin, vmax) runs on every lane
 // for every st < steps (keep false past the lane's own interval). Adjacent voxels share the
-// lattice plane between them: each step transforms one new plane of 4 corner columns, and the
-// per-corner operation order is that of gaussian_frag_alloc.slang:79-121.
+// lattice plane between them, and the per-corner operation order is that of
+// gaussian_frag_alloc.slang:79-121: corner (col, kz) of axis c is
+// q = ((part[c][col] + tz(kz)) * inv_r) * inv_s[c] with part the row-constant (x, y) terms.
+// For fixed tz that rounded expression is monotone non-decreasing in part (a rounded sum with a
+// fixed operand, then products with the positive finite factors inv_r and inv_s[c]), so the
+// minimum and maximum over a plane's 4 corner columns are the corners of the columns with the
+// smallest and largest part: each step evaluates 2 corners per axis instead of 4 and obtains the
+// same bits for the box the predicate tests.
 template <typename OnStep>
 __device__ __forceinline__ void walk_row(const PrimFrame& f, int32_t i, int32_t j, int32_t k0, int32_t len,
                                          int32_t steps, float vs, OnStep&& on_step)
 {
-    // row-constant partial dot products of the 4 corner columns (dx, dy)
-    float part[3][4];
+    // row-constant partial dot products of the 4 corner columns (dx, dy): their extremes per axis
+    float part_lo[3] = { 0.f, 0.f, 0.f }, part_hi[3] = { 0.f, 0.f, 0.f };
     if (len > 0)
     {
 #pragma unroll
@@ -579,7 +585,8 @@ __device__ __forceinline__ void walk_row(const PrimFrame& f, int32_t i, int32_t 
             {
                 float t = lx * f.rinv[3 * c + 0];
                 t = t + ly * f.rinv[3 * c + 1];
-                part[c][col] = t;
+                part_lo[c] = col == 0 ? t : fminf(part_lo[c], t);
+                part_hi[c] = col == 0 ? t : fmaxf(part_hi[c], t);
             }
         }
     }
@@ -592,23 +599,14 @@ __device__ __forceinline__ void walk_row(const PrimFrame& f, int32_t i, int32_t 
         for (int c = 0; c < 3; c++)
         {
             float tz = lz * f.rinv[3 * c + 2];
-#pragma unroll
-            for (int col = 0; col < 4; col++)
-            {
-                float q = part[c][col] + tz;
-                q = q * f.inv_r;
-                q = q * f.inv_s[c];
-                if (col == 0)
-                {
-                    pmin[c] = q;
-                    pmax[c] = q;
-                }
-                else
-                {
-                    pmin[c] = fminf(pmin[c], q);
-                    pmax[c] = fmaxf(pmax[c], q);
-                }
-            }
+            float qmin = part_lo[c] + tz;
+            qmin = qmin * f.inv_r;
+            qmin = qmin * f.inv_s[c];
+            float qmax = part_hi[c] + tz;
+            qmax = qmax * f.inv_r;
+            qmax = qmax * f.inv_s[c];
+            pmin[c] = qmin;
+            pmax[c] = qmax;
         }
     };
     float lo_min[3] = { 0.f, 0.f, 0.f }, lo_max[3] = { 0.f, 0.f, 0.f };
@@ -5764,103 +5762,79 @@ int raster_batch_graph(Context* ctx, const std::vector<Context*>& workers, vdbr_
         return sig;
     };
     const auto t_plan = std::chrono::steady_clock::now();
-    hipGraphExec_t exec = nullptr;
+    // One graph per worker stream, launched together: a single graph holding the workers as
+    // parallel branches measured 964 ms per 256-cloud batch with 4 workers against 877 ms for
+    // per-stream graphs (the graph executor overlaps a graph's branches less than separate
+    // streams overlap).
+    std::vector<hipGraphExec_t>& execs = ctx->batch_execs;
     bool captured = false;
-    if (ctx->batch_exec && ctx->batch_sig == signature())
-        exec = (hipGraphExec_t)ctx->batch_exec;
-    else
+    if (execs.size() != K || ctx->batch_sig != signature())
     {
-        if (ctx->batch_exec)
-            (void)hipGraphExecDestroy((hipGraphExec_t)ctx->batch_exec);
-        ctx->batch_exec = nullptr;
-        ctx->batch_sig.clear();
-        // a sizing pass (discarded), the workspaces grown to it, then the real capture
-        hipGraph_t graph = nullptr;
+        ctx->drop_batch_graphs();
+        // a sizing pass per worker (discarded), the workspaces grown to it, then the real capture
         for (int pass = 0; pass < 2; pass++)
         {
             const bool probe = pass == 0;
-            if (!probe)
+            for (size_t k = 0; k < K; k++)
             {
-                for (size_t k = 0; k < K; k++)
+                Context* w = workers[k];
+                if (!probe)
                 {
-                    Context* w = workers[k];
                     for (int sl = 0; sl < WS_COUNT; sl++)
                         if (w->probe_size[sl] && !w->ws((WsSlot)sl, w->probe_size[sl]))
                             return ctx->fail("raster batch: worker workspace: " + w->error);
                     // the captured sorts use look-back epochs from 1 (the status words are cleared
-                    // before every launch of the graph)
+                    // before every launch)
                     w->sort_epoch = 0;
                     VDBR_CHECK(ctx, hipStreamSynchronize(w->stream));
                 }
-            }
-            hipEvent_t fork = nullptr;
-            std::vector<hipEvent_t> joins(K, nullptr);
-            VDBR_CHECK(ctx, hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-            for (size_t k = 0; k < K; k++)
-                VDBR_CHECK(ctx, hipEventCreateWithFlags(&joins[k], hipEventDisableTiming));
-            struct EvG
-            {
-                hipEvent_t f;
-                std::vector<hipEvent_t>& j;
-                ~EvG()
-                {
-                    (void)hipEventDestroy(f);
-                    for (auto e : j)
-                        if (e)
-                            (void)hipEventDestroy(e);
-                }
-            } evg{ fork, joins };
-            VDBR_CHECK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            int rc = 0;
-            std::string err;
-            if (hipEventRecord(fork, st) != hipSuccess)
-                rc = -1, err = "event record";
-            for (size_t k = 0; k < K && rc == 0; k++)
-            {
-                Context* w = workers[k];
+                VDBR_CHECK(ctx, hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
                 w->probe = probe;
                 w->capturing = true;
                 if (probe)
                     std::fill(w->probe_size, w->probe_size + WS_COUNT, (size_t)0);
-                if (hipStreamWaitEvent(w->stream, fork, 0) != hipSuccess)
-                    rc = -1, err = "stream wait";
+                int rc = 0;
+                std::string err;
                 for (size_t c : assign[k])
                 {
-                    if (rc)
-                        break;
                     const CloudPlan& p = plans[c];
                     if (capture_cloud(w, p, win_dev + p.win_off, recs + rec_of[c], arena, arena_cap, cursor))
-                        rc = -1, err = w->error;
+                    {
+                        rc = -1;
+                        err = w->error;
+                        break;
+                    }
                 }
-                if (rc == 0 && (hipEventRecord(joins[k], w->stream) != hipSuccess ||
-                                hipStreamWaitEvent(st, joins[k], 0) != hipSuccess))
-                    rc = -1, err = "join";
                 w->probe = false;
                 w->capturing = false;
-            }
-            hipGraph_t g = nullptr;
-            hipError_t e = hipStreamEndCapture(st, &g);
-            if (rc || e != hipSuccess)
-            {
-                if (g)
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamEndCapture(w->stream, &g);
+                if (rc || e != hipSuccess)
+                {
+                    if (g)
+                        (void)hipGraphDestroy(g);
+                    (void)hipGetLastError();
+                    ctx->drop_batch_graphs();
+                    return ctx->fail("raster batch: capture failed: " +
+                                     (err.empty() ? std::string(hipGetErrorString(e)) : err));
+                }
+                if (probe)
+                {
                     (void)hipGraphDestroy(g);
-                (void)hipGetLastError();
-                return ctx->fail("raster batch: capture failed: " +
-                                 (err.empty() ? std::string(hipGetErrorString(e)) : err));
-            }
-            if (probe)
+                    continue;
+                }
+                hipGraphExec_t x = nullptr;
+                e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
-            else
-                graph = g;
+                if (e != hipSuccess)
+                {
+                    (void)hipGetLastError();
+                    ctx->drop_batch_graphs();
+                    return ctx->fail(std::string("raster batch: graph instantiation: ") + hipGetErrorString(e));
+                }
+                execs.push_back(x);
+            }
         }
-        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess)
-        {
-            (void)hipGetLastError();
-            return ctx->fail(std::string("raster batch: graph instantiation: ") + hipGetErrorString(e));
-        }
-        ctx->batch_exec = exec;
         ctx->batch_sig = signature();
         captured = true;
     }
@@ -5875,12 +5849,33 @@ int raster_batch_graph(Context* ctx, const std::vector<Context*>& workers, vdbr_
             fill_words(ctx, (uint64_t*)w->ws_ptr[WS_SORT_STATUS], host_count(w->ws_size[WS_SORT_STATUS] / 8), 0ull))
             return -1;
     }
-    hipError_t e = hipGraphLaunch(exec, st);
+    // the worker streams wait for the context's stream, run their graphs, and the context's stream
+    // waits for all of them
+    hipEvent_t go = nullptr;
+    VDBR_CHECK(ctx, hipEventCreateWithFlags(&go, hipEventDisableTiming));
+    std::vector<hipEvent_t> done(K, nullptr);
+    hipError_t e = hipEventRecord(go, st);
+    for (size_t k = 0; k < K && e == hipSuccess; k++)
+    {
+        e = hipEventCreateWithFlags(&done[k], hipEventDisableTiming);
+        if (e == hipSuccess)
+            e = hipStreamWaitEvent(workers[k]->stream, go, 0);
+        if (e == hipSuccess)
+            e = hipGraphLaunch(execs[k], workers[k]->stream);
+        if (e == hipSuccess)
+            e = hipEventRecord(done[k], workers[k]->stream);
+        if (e == hipSuccess)
+            e = hipStreamWaitEvent(st, done[k], 0);
+    }
     std::vector<CloudRecord> hrec(graph_clouds.size());
     if (e == hipSuccess)
         e = hipMemcpyAsync(hrec.data(), recs, graph_clouds.size() * sizeof(CloudRecord), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess)
         e = hipStreamSynchronize(st);
+    (void)hipEventDestroy(go);
+    for (hipEvent_t d : done)
+        if (d)
+            (void)hipEventDestroy(d);
     if (e != hipSuccess)
     {
         (void)hipGetLastError();
