@@ -281,6 +281,7 @@ constexpr int EMIT_THREADS = 256;
 constexpr int EMIT_ROWS = 1024;       // rows per block
 constexpr int EMIT_WAVE_STAGE = 1024; // staged survivors per wave
 constexpr int EMIT_HIST_PASSES = 4;   // sort passes whose digit histograms the emission can count
+constexpr int EMIT_RING = 128;        // voxel boxes of staged survivors whose alpha is pending, per wave
 
 struct EmitArgs
 {
@@ -384,11 +385,30 @@ __device__ __forceinline__ uint64_t find_prim(const uint64_t* __restrict__ row_s
     return lo;
 }
 
-// Writes a wave's staged pairs at an atomically reserved position. With the alpha in the key,
-// the stage holds (key without alpha, minimum sample distance) and the alpha is evaluated here
-// on the compacted pairs, so the double-precision exp/pow run with every lane busy.
-__device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, uint64_t* stage, float* stage_d, uint32_t cnt,
-                                                uint32_t lane, uint32_t (*s_hist)[256])
+// Alpha of the staged survivors [s0, s0 + n), n <= 64, one per lane: the minimum sample distance
+// of the voxel box held in the wave's ring, then the 8-bit alpha ORed into the staged key. The
+// walk defers this work so that it runs on 64 survivors at once; evaluated inside the walk it
+// issues for the whole wave on nearly every step while only the lanes that keep their voxel
+// (about one in seven) use it.
+__device__ __forceinline__ void emit_alpha_batch(const EmitArgs& a, uint64_t* stage, const float (*ring)[EMIT_RING],
+                                                 uint32_t s0, uint32_t n, uint32_t lane)
+{
+    if (lane < n)
+    {
+        const uint32_t s = s0 + lane;
+        const uint32_t r = s & (uint32_t)(EMIT_RING - 1);
+        const uint64_t key = stage[s];
+        const uint64_t gmask = (1ull << a.kl.gid_bits) - 1ull;
+        const PrimVal pv = a.vals[(uint32_t)((key >> a.kl.alpha_bits) & gmask)];
+        const float vmin[3] = { ring[0][r], ring[1][r], ring[2][r] };
+        const float vmax[3] = { ring[3][r], ring[4][r], ring[5][r] };
+        stage[s] = key | (uint64_t)alpha8_fast(pv, min_sample_dist2(vmin, vmax));
+    }
+}
+
+// Writes a wave's staged pairs (complete keys) at an atomically reserved position.
+__device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, const uint64_t* stage, uint32_t cnt, uint32_t lane,
+                                                uint32_t (*s_hist)[256])
 {
     if (cnt == 0u)
         return;
@@ -425,45 +445,11 @@ __device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, uint64_t* sta
     if (lane == 0u)
         base = atomicAdd(a.counter, (unsigned long long)cnt);
     base = __shfl(base, 0, 64);
-    const uint64_t gmask = (1ull << a.kl.gid_bits) - 1ull;
-    // four pairs per lane at a time: their Gaussians' value constants are gathered together before
-    // any alpha is evaluated or key stored (a store in between would order the next gather after it)
-    constexpr uint32_t U = 4;
-    for (uint32_t e0 = 0; e0 < cnt; e0 += 64u * U)
+    for (uint32_t e = lane; e < cnt; e += 64u)
     {
-        uint64_t key[U];
-        float dd[U];
-        PrimVal pv[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++)
-        {
-            const uint32_t e = e0 + u * 64u + lane;
-            key[u] = 0ull;
-            dd[u] = 0.f;
-            if (e < cnt)
-            {
-                key[u] = stage[e];
-                if (a.kl.alpha_bits)
-                {
-                    dd[u] = stage_d[e];
-                    pv[u] = a.vals[(uint32_t)((key[u] >> a.kl.alpha_bits) & gmask)];
-                }
-            }
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; u++)
-        {
-            const uint32_t e = e0 + u * 64u + lane;
-            if (e < cnt)
-            {
-                uint64_t k = key[u];
-                if (a.kl.alpha_bits)
-                    k |= (uint64_t)alpha8_fast(pv[u], dd[u]);
-                const uint64_t pos = base + e;
-                if (pos < a.cap)
-                    a.out[pos] = k;
-            }
-        }
+        const uint64_t pos = base + e;
+        if (pos < a.cap)
+            a.out[pos] = stage[e];
     }
 }
 
@@ -877,17 +863,19 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
 __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
 {
     __shared__ uint64_t stage_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
-    __shared__ float stage_d_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
+    __shared__ float ring_all[EMIT_THREADS / 64][6][EMIT_RING];
     __shared__ uint32_t s_hist[EMIT_HIST_PASSES][256];
     for (uint32_t x = threadIdx.x; x < (uint32_t)(EMIT_HIST_PASSES * 256); x += EMIT_THREADS)
         (&s_hist[0][0])[x] = 0u;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint64_t* stage = stage_all[wave];
-    float* stage_d = stage_d_all[wave];
+    float(*ring)[EMIT_RING] = ring_all[wave];
+    const bool with_alpha = a.kl.alpha_bits != 0u;
     const float vs = a.vs;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    uint32_t count = 0u;
+    uint32_t count = 0u; // staged survivors
+    uint32_t done = 0u;  // staged survivors whose key holds its alpha: [done, count) wait in the ring
     const uint64_t waves_total = (uint64_t)gridDim.x * (EMIT_THREADS / 64);
     const uint64_t chunks_total = a.seg_chunk_scan[SEG_PARTS];
     for (uint64_t c = (uint64_t)blockIdx.x * (EMIT_THREADS / 64) + wave; c < chunks_total; c += waves_total)
@@ -923,40 +911,73 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
         const uint64_t g_bits = g << a.kl.alpha_bits;
         walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float* vmin, const float* vmax) {
             uint64_t key = 0ull;
-            float dist = 0.f;
             if (keep)
             {
                 const int32_t k = k0 + st;
                 const uint64_t vkey = ij_key + ((uint64_t)(uint32_t)((k >> 3) - a.kl.lmin[2]) << 9) + (uint64_t)(k & 7);
                 key = (vkey << kshift) | g_bits;
-                if (a.kl.alpha_bits)
-                    dist = min_sample_dist2(vmin, vmax);
             }
             const uint64_t m = __ballot(keep);
             if (keep)
             {
-                uint32_t slot = count + (uint32_t)__popcll(m & lt_mask);
+                const uint32_t slot = count + (uint32_t)__popcll(m & lt_mask);
                 stage[slot] = key;
-                stage_d[slot] = dist;
+                if (with_alpha)
+                {
+                    // the box for the alpha; [done, count) spans fewer than 64 + 64 slots
+                    const uint32_t r = slot & (uint32_t)(EMIT_RING - 1);
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                    {
+                        ring[c][r] = vmin[c];
+                        ring[3 + c][r] = vmax[c];
+                    }
+                }
             }
             count += (uint32_t)__popcll(m);
+            if (with_alpha && count - done >= 64u)
+            {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                emit_alpha_batch(a, stage, ring, done, 64u, lane);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                done += 64u;
+            }
             if (count > (uint32_t)(EMIT_WAVE_STAGE - 64))
             {
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                emit_flush_wave(a, stage, stage_d, count, lane, s_hist);
+                if (with_alpha)
+                {
+                    emit_alpha_batch(a, stage, ring, done, count - done, lane);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+                emit_flush_wave(a, stage, count, lane, s_hist);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 count = 0u;
+                done = 0u;
             }
         });
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    emit_flush_wave(a, stage, stage_d, count, lane, s_hist);
+    if (with_alpha)
+    {
+        emit_alpha_batch(a, stage, ring, done, count - done, lane);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    emit_flush_wave(a, stage, count, lane, s_hist);
     if (a.hist)
     {
         __syncthreads();
