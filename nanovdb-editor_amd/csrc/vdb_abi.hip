@@ -165,7 +165,10 @@ static Context* create_context(int device)
     ctx->device = device;
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
-        ctx->emit_blocks = (uint32_t)cus * 3u; // 3 blocks of the emission kernel fit a CU's LDS
+    {
+        const uint32_t per_cu = emit_blocks_per_cu();
+        ctx->emit_blocks = (uint32_t)cus * (per_cu ? per_cu : 3u);
+    }
     if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess)
     {
         delete ctx;

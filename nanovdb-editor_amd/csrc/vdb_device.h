@@ -180,13 +180,12 @@ VDB_DEV float min_sample_dist2(const float vmin[3], const float vmax[3])
 #pragma unroll
     for (int c = 0; c < 3; c++)
     {
+        // the reference's minimum of the 4 rounded squares: fl(s * s) depends on |s| only and is
+        // monotone in it, so that minimum is the rounded square of the sample of least magnitude
         float l1 = w1 * vmax[c] + ow1 * vmin[c];
         float l2 = w2 * vmax[c] + ow2 * vmin[c];
-        float best = vmin[c] * vmin[c];
-        best = fminf(best, l1 * l1);
-        best = fminf(best, l2 * l2);
-        best = fminf(best, vmax[c] * vmax[c]);
-        msq[c] = best;
+        const float m = fminf(fminf(fabsf(vmin[c]), fabsf(l1)), fminf(fabsf(l2), fabsf(vmax[c])));
+        msq[c] = m * m;
     }
     float d = msq[0] + msq[1];
     d = d + msq[2];
@@ -233,7 +232,10 @@ VDB_DEV bool alpha8_try_fast(const PrimVal& pv, float d, uint32_t* out)
     const float eps_t = (fabsf(x) + 8.f) * u;              // relative error of t
     const float db = (eps_t + 3.f * u) * a + 2.f * u;       // |b_ref - b|
     const float eps_p = (4.f * fabsf(m) + 8.f) * u;         // relative error of p at fixed b
-    float bound = 255.f * (pv.pow_exp * (p / b) * db + eps_p * p + 2.f * u) + 3.1e-5f;
+    // p / b through the hardware reciprocal (1 ulp): a relative error of ~2^-22 in this term is far
+    // inside the factor 2 applied below; b = 0 or a denormal b gives a non-finite or huge bound,
+    // which leaves the decision to the correctly rounded path as the exact quotient would
+    float bound = 255.f * (pv.pow_exp * (p * __builtin_amdgcn_rcpf(b)) * db + eps_p * p + 2.f * u) + 3.1e-5f;
     bound = 2.f * bound + 1e-6f;
     const float fl = floorf(y);
     if (y - fl > bound && (fl + 1.f) - y > bound)

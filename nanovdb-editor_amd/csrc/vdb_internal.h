@@ -148,7 +148,7 @@ struct Context
     // pinned read back of a captured batch's setup counts (grown as needed)
     uint64_t* batch_pinned = nullptr;
     size_t batch_pinned_words = 0;
-    uint32_t emit_blocks = 2048; // resident blocks of the persistent emission kernel (CUs x 3)
+    uint32_t emit_blocks = 2048; // resident blocks of the persistent emission kernel (CUs x blocks per CU)
     SortConfig sort_tuned[SORT_BUCKETS] = {}; // autotuned sort configuration per size bucket
     uint32_t sort_epoch = 0;
     uint64_t sort_status_tiles = 0;
@@ -297,6 +297,8 @@ struct Node2BuildOutput
 constexpr uint32_t GF_SEGMENTS = 1u, GF_PAIRS = 2u, GF_LEAVES = 4u, GF_VOXELS = 8u, GF_NODES = 16u, GF_ARENA = 32u,
                    GF_RANGE = 64u;
 int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* out);
+// workgroups of the persistent emission kernel resident on one CU (occupancy query; 0 on failure)
+uint32_t emit_blocks_per_cu();
 
 // vdb_raster.hip
 int raster_node2(Context* ctx, const vdbr_gaussians_t* g, const vdbr_raster_params_t* p, void** grid_dev,

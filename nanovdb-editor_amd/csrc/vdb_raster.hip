@@ -279,9 +279,10 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t n, const float* __restri
 // ============================================================================ emit
 constexpr int EMIT_THREADS = 256;
 constexpr int EMIT_ROWS = 1024;       // rows per block
-constexpr int EMIT_WAVE_STAGE = 1024; // staged survivors per wave
+constexpr int EMIT_WAVE_STAGE = 1792; // staged survivors per wave
 constexpr int EMIT_HIST_PASSES = 4;   // sort passes whose digit histograms the emission can count
-constexpr int EMIT_RING = 128;        // voxel boxes of staged survivors whose alpha is pending, per wave
+constexpr int EMIT_RING = 128;        // staged survivors whose alpha is pending, per wave
+constexpr int EMIT_RING_WORDS = 10;   // per survivor: the box extremes (6), its Gaussian's PrimVal (4)
 
 struct EmitArgs
 {
@@ -385,23 +386,44 @@ __device__ __forceinline__ uint64_t find_prim(const uint64_t* __restrict__ row_s
     return lo;
 }
 
+// orders a wave's LDS accesses before and after it (lanes exchange data through LDS)
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Alpha of the staged survivors [s0, s0 + n), n <= 64, one per lane: the minimum sample distance
-// of the voxel box held in the wave's ring, then the 8-bit alpha ORed into the staged key. The
-// walk defers this work so that it runs on 64 survivors at once; evaluated inside the walk it
-// issues for the whole wave on nearly every step while only the lanes that keep their voxel
-// (about one in seven) use it.
-__device__ __forceinline__ void emit_alpha_batch(const EmitArgs& a, uint64_t* stage, const float (*ring)[EMIT_RING],
-                                                 uint32_t s0, uint32_t n, uint32_t lane)
+// of the voxel box held in the wave's ring (slot s at s mod EMIT_RING, with the value constants
+// of its Gaussian), then the 8-bit alpha ORed into the staged key. The walk defers this work so
+// that it runs on 64 survivors at once; evaluated inside the walk it issues for the whole wave on
+// nearly every step while only the lanes that keep their voxel (about one in seven) use it. It
+// reads nothing but LDS, so it never waits on the wave's outstanding global loads and stores.
+__device__ __forceinline__ void emit_alpha_batch(uint64_t* stage, const uint32_t (*ring)[EMIT_RING], uint32_t s0,
+                                                 uint32_t n, uint32_t lane)
 {
     if (lane < n)
     {
         const uint32_t s = s0 + lane;
         const uint32_t r = s & (uint32_t)(EMIT_RING - 1);
         const uint64_t key = stage[s];
-        const uint64_t gmask = (1ull << a.kl.gid_bits) - 1ull;
-        const PrimVal pv = a.vals[(uint32_t)((key >> a.kl.alpha_bits) & gmask)];
-        const float vmin[3] = { ring[0][r], ring[1][r], ring[2][r] };
-        const float vmax[3] = { ring[3][r], ring[4][r], ring[5][r] };
+        PrimVal pv;
+        pv.opacity = __uint_as_float(ring[6][r]);
+        pv.local_scale = __uint_as_float(ring[7][r]);
+        pv.pow_exp = __uint_as_float(ring[8][r]);
+        pv.rgb_cull = ring[9][r];
+        // the ring holds each axis' two box extremes in either order; they are finite, so the
+        // plain min / max instructions order them (fminf / fmaxf would first quiet signalling
+        // NaNs of values loaded from memory, two more instructions per pair)
+        float vmin[3], vmax[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+        {
+            const float ea = __uint_as_float(ring[c][r]), eb = __uint_as_float(ring[3 + c][r]);
+            asm("v_min_f32 %0, %1, %2" : "=v"(vmin[c]) : "v"(ea), "v"(eb));
+            asm("v_max_f32 %0, %1, %2" : "=v"(vmax[c]) : "v"(ea), "v"(eb));
+        }
         stage[s] = key | (uint64_t)alpha8_fast(pv, min_sample_dist2(vmin, vmax));
     }
 }
@@ -415,41 +437,78 @@ __device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, const uint64_
     if (a.hist)
     {
         // digit counts of the keys for every sort pass (the alpha bits lie below the sorted bits),
-        // each lane over a contiguous slice, run-length compressed, into the block's LDS histograms
+        // each lane over a contiguous slice, run-length compressed, into the block's LDS histograms.
+        // The slice is read 8 keys at a time (the reads overlap instead of each waiting for the
+        // previous one) and every key read once for all passes.
         const uint32_t passes = (a.hist_bits + a.hist_w - 1u) / a.hist_w;
         const uint32_t per = (cnt + 63u) >> 6;
         const uint32_t e0 = lane * per;
         const uint32_t e1 = min(cnt, e0 + per);
-        for (uint32_t p = 0; p < passes && e0 < e1; p++)
+        uint32_t shift[EMIT_HIST_PASSES], dm[EMIT_HIST_PASSES], cur[EMIT_HIST_PASSES], run[EMIT_HIST_PASSES];
+        const uint64_t k0 = e0 < e1 ? stage[e0] : 0ull;
+#pragma unroll
+        for (uint32_t p = 0; p < (uint32_t)EMIT_HIST_PASSES; p++)
         {
-            const uint32_t shift = a.hist_shift + a.hist_w * p;
-            const uint32_t rem = a.hist_bits - a.hist_w * p;
-            const uint32_t dm = rem >= a.hist_w ? ((1u << a.hist_w) - 1u) : ((1u << rem) - 1u);
-            uint32_t cur = (uint32_t)(stage[e0] >> shift) & dm;
-            uint32_t run = 1u;
-            for (uint32_t e = e0 + 1u; e < e1; e++)
+            shift[p] = a.hist_shift + a.hist_w * p;
+            const uint32_t rem = p < passes ? a.hist_bits - a.hist_w * p : 0u;
+            dm[p] = rem >= a.hist_w ? ((1u << a.hist_w) - 1u) : ((1u << rem) - 1u);
+            cur[p] = (uint32_t)(k0 >> shift[p]) & dm[p];
+            run[p] = 0u;
+        }
+        constexpr uint32_t U = 8;
+        for (uint32_t e = e0; e < e1; e += U)
+        {
+            uint64_t k[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
+                k[u] = e + u < e1 ? stage[e + u] : 0ull;
+#pragma unroll
+            for (uint32_t u = 0; u < U; u++)
             {
-                const uint32_t d = (uint32_t)(stage[e] >> shift) & dm;
-                if (d != cur)
+                if (e + u < e1)
                 {
-                    atomicAdd(&s_hist[p][cur], run);
-                    cur = d;
-                    run = 0u;
+#pragma unroll
+                    for (uint32_t p = 0; p < (uint32_t)EMIT_HIST_PASSES; p++)
+                    {
+                        const uint32_t d = (uint32_t)(k[u] >> shift[p]) & dm[p];
+                        if (p < passes && d != cur[p])
+                        {
+                            atomicAdd(&s_hist[p][cur[p]], run[p]);
+                            cur[p] = d;
+                            run[p] = 0u;
+                        }
+                        run[p]++;
+                    }
                 }
-                run++;
             }
-            atomicAdd(&s_hist[p][cur], run);
+        }
+        if (e0 < e1)
+        {
+#pragma unroll
+            for (uint32_t p = 0; p < (uint32_t)EMIT_HIST_PASSES; p++)
+                if (p < passes)
+                    atomicAdd(&s_hist[p][cur[p]], run[p]);
         }
     }
     unsigned long long base = 0ull;
     if (lane == 0u)
         base = atomicAdd(a.counter, (unsigned long long)cnt);
     base = __shfl(base, 0, 64);
-    for (uint32_t e = lane; e < cnt; e += 64u)
+    // 4 keys per lane at a time: their LDS reads are issued together before the stores
+    constexpr uint32_t V = 4;
+    for (uint32_t e = lane; e < cnt; e += 64u * V)
     {
-        const uint64_t pos = base + e;
-        if (pos < a.cap)
-            a.out[pos] = stage[e];
+        uint64_t k[V];
+#pragma unroll
+        for (uint32_t u = 0; u < V; u++)
+            k[u] = e + 64u * u < cnt ? stage[e + 64u * u] : 0ull;
+#pragma unroll
+        for (uint32_t u = 0; u < V; u++)
+        {
+            const uint64_t pos = base + e + 64u * u;
+            if (e + 64u * u < cnt && pos < a.cap)
+                a.out[pos] = k[u];
+        }
     }
 }
 
@@ -535,16 +594,21 @@ __device__ __forceinline__ void row_interval(const PrimFrame& f, const PrimBox& 
 }
 
 // The candidate voxels (i, j, k0 + st), st < len, of one AABB row, walked in lock step by the
-// wave: steps is the wave maximum of len, and on_step(st, keep, vmin, vmax) runs on every lane
-// for every st < steps (keep false past the lane's own interval). Adjacent voxels share the
-// lattice plane between them, and the per-corner operation order is that of
-// gaussian_frag_alloc.slang:79-121: corner (col, kz) of axis c is
-// q = ((part[c][col] + tz(kz)) * inv_r) * inv_s[c] with part the row-constant (x, y) terms.
-// For fixed tz that rounded expression is monotone non-decreasing in part (a rounded sum with a
-// fixed operand, then products with the positive finite factors inv_r and inv_s[c]), so the
-// minimum and maximum over a plane's 4 corner columns are the corners of the columns with the
-// smallest and largest part: each step evaluates 2 corners per axis instead of 4 and obtains the
-// same bits for the box the predicate tests.
+// wave: steps is the wave maximum of len, and on_step(st, keep, ext_a, ext_b) runs on every lane
+// for every st < steps (keep false past the lane's own interval). The per-corner operation order
+// is that of gaussian_frag_alloc.slang:79-121: corner (col, kz) of axis c is
+// q = ((part[c][col] + tz(kz)) * inv_r) * inv_s[c], with part the row-constant (x, y) terms and
+// tz(kz) = ((kz * vs) - mu_z) * rinv[c][2]. Every step of that expression is a correctly rounded
+// operation that is monotone non-decreasing in part (a sum with a fixed operand, then products
+// with the positive finite factors inv_r and inv_s[c]) and in tz, and tz is monotone in kz with
+// the sign of rinv[c][2]. So the minimum of axis c over a voxel's 8 corners is the corner of the
+// column with the smallest part on the plane where tz is smaller, and the maximum is the corner of
+// the column with the largest part on the other plane: per axis, one value is carried from the
+// voxel's lower plane and one is new on its upper plane, and they are the box's extremes with the
+// same bits as a min / max over all 8 corners (up to the sign of a zero, which neither the
+// predicate nor the sample distances see). ext_a[c] and ext_b[c] are those two extremes in either
+// order (fminf / fmaxf order them); the predicate uses med3(a, 0, b), which is the value that
+// gaussian_frag_alloc.slang:139-162 subtracts squared (vmin if 0 < vmin, vmax if vmax < 0, else 0).
 template <typename OnStep>
 __device__ __forceinline__ void walk_row(const PrimFrame& f, int32_t i, int32_t j, int32_t k0, int32_t len,
                                          int32_t steps, float vs, OnStep&& on_step)
@@ -576,47 +640,56 @@ __device__ __forceinline__ void walk_row(const PrimFrame& f, int32_t i, int32_t 
             }
         }
     }
-    // lattice plane z -> per-axis min/max over the 4 columns
-    auto plane = [&](int32_t kz, float pmin[3], float pmax[3]) {
-        float pz = (float)kz;
-        float lz = pz * vs;
-        lz = lz - f.mu[2];
+    // per axis: the part whose corner is carried from a voxel's lower plane (pc) and the part whose
+    // corner is taken on its upper plane (pn)
+    float pc[3], pn[3], rz[3], sc[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+    {
+        rz[c] = f.rinv[3 * c + 2];
+        sc[c] = f.inv_s[c];
+        const bool up = !(rz[c] < 0.f);
+        pc[c] = up ? part_lo[c] : part_hi[c];
+        pn[c] = up ? part_hi[c] : part_lo[c];
+    }
+    const float inv_r = f.inv_r, mu_z = f.mu[2];
+    float carry[3];
+    {
+        float lz = (float)k0 * vs;
+        lz = lz - mu_z;
 #pragma unroll
         for (int c = 0; c < 3; c++)
         {
-            float tz = lz * f.rinv[3 * c + 2];
-            float qmin = part_lo[c] + tz;
-            qmin = qmin * f.inv_r;
-            qmin = qmin * f.inv_s[c];
-            float qmax = part_hi[c] + tz;
-            qmax = qmax * f.inv_r;
-            qmax = qmax * f.inv_s[c];
-            pmin[c] = qmin;
-            pmax[c] = qmax;
+            const float tz = lz * rz[c];
+            float q = pc[c] + tz;
+            q = q * inv_r;
+            carry[c] = q * sc[c];
         }
-    };
-    float lo_min[3] = { 0.f, 0.f, 0.f }, lo_max[3] = { 0.f, 0.f, 0.f };
-    if (len > 0)
-        plane(k0, lo_min, lo_max);
+    }
     for (int32_t st = 0; st < steps; st++)
     {
-        bool keep = false;
-        float vmin[3] = { 0.f, 0.f, 0.f }, vmax[3] = { 0.f, 0.f, 0.f };
-        if (st < len)
-        {
-            float hi_min[3], hi_max[3];
-            plane(k0 + st + 1, hi_min, hi_max);
+        float lz = (float)(k0 + st + 1) * vs;
+        lz = lz - mu_z;
+        float ea[3], eb[3];
+        float d2 = 1.f;
 #pragma unroll
-            for (int c = 0; c < 3; c++)
-            {
-                vmin[c] = fminf(lo_min[c], hi_min[c]);
-                vmax[c] = fmaxf(lo_max[c], hi_max[c]);
-                lo_min[c] = hi_min[c];
-                lo_max[c] = hi_max[c];
-            }
-            keep = box_kept(vmin, vmax);
+        for (int c = 0; c < 3; c++)
+        {
+            const float tz = lz * rz[c];
+            float qn = pn[c] + tz;
+            qn = qn * inv_r;
+            qn = qn * sc[c];
+            float qc = pc[c] + tz;
+            qc = qc * inv_r;
+            qc = qc * sc[c];
+            ea[c] = carry[c];
+            eb[c] = qn;
+            carry[c] = qc;
+            const float t = __builtin_amdgcn_fmed3f(ea[c], 0.f, eb[c]);
+            d2 = d2 - t * t;
         }
-        on_step(st, keep, vmin, vmax);
+        const bool keep = st < len && !(d2 <= 0.f);
+        on_step(st, keep, ea, eb);
     }
 }
 
@@ -767,6 +840,27 @@ __device__ __forceinline__ int32_t wave_max(int32_t v)
     return v;
 }
 
+// wave maximum of v in [0, 2^bits) by ballots (no cross-lane data movement)
+__device__ __forceinline__ int32_t wave_max_small(int32_t v, int bits)
+{
+    int32_t m = 0;
+    for (int b = bits - 1; b >= 0; b--)
+    {
+        const int32_t t = m | (1 << b);
+        if (__ballot(v >= t))
+            m = t;
+    }
+    return m;
+}
+
+// value of lane src (wave-uniform) of a 64-bit register
+__device__ __forceinline__ uint64_t read_lane_u64(uint64_t v, uint32_t src)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Row pass of the segment-driven emission: one lane per AABB row computes the pruned candidate
 // interval and writes it as segments of at most EMIT_SEG voxels (wave-compacted, one atomic
 // reservation per wave). The emission pass then walks segments, so the lanes of a wave walk at
@@ -860,123 +954,162 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
 // Segment-driven emission: each lane takes one candidate segment, walks its voxels with the row
 // walker (same FP32 predicate, lattice planes shared within the segment) and stages survivors as
 // in the row-per-lane form.
-__global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
+// 2 workgroups (80 KB of LDS each) per CU. Every flush reserves its output range with one atomic
+// on the single pair counter, and those same-address atomics are served at a fixed rate (about
+// 78 M/s measured), so the stage is as large as the LDS allows: at 1.67 G pairs, 896-survivor
+// flushes bound the kernel at ~25 ms, 1728-survivor flushes at ~12 ms.
+__global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_emit_segments(EmitArgs a)
 {
     __shared__ uint64_t stage_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
-    __shared__ float ring_all[EMIT_THREADS / 64][6][EMIT_RING];
+    __shared__ uint32_t ring_all[EMIT_THREADS / 64][EMIT_RING_WORDS][EMIT_RING];
     __shared__ uint32_t s_hist[EMIT_HIST_PASSES][256];
     for (uint32_t x = threadIdx.x; x < (uint32_t)(EMIT_HIST_PASSES * 256); x += EMIT_THREADS)
         (&s_hist[0][0])[x] = 0u;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint64_t* stage = stage_all[wave];
-    float(*ring)[EMIT_RING] = ring_all[wave];
+    uint32_t(*ring)[EMIT_RING] = ring_all[wave];
     const bool with_alpha = a.kl.alpha_bits != 0u;
     const float vs = a.vs;
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
     uint32_t count = 0u; // staged survivors
     uint32_t done = 0u;  // staged survivors whose key holds its alpha: [done, count) wait in the ring
+    // the alpha of every staged survivor (before a flush)
+    auto finish_alpha = [&]() {
+        if (!with_alpha)
+            return;
+        emit_alpha_batch(stage, ring, done, count - done, lane);
+        wave_sync_lds();
+        done = count;
+    };
     const uint64_t waves_total = (uint64_t)gridDim.x * (EMIT_THREADS / 64);
     const uint64_t chunks_total = a.seg_chunk_scan[SEG_PARTS];
-    for (uint64_t c = (uint64_t)blockIdx.x * (EMIT_THREADS / 64) + wave; c < chunks_total; c += waves_total)
+    // partition table in registers (lane p: partition p's first chunk, end chunk, segment count),
+    // so that a chunk's partition is one ballot and no memory access
+    static_assert(SEG_PARTS == 64, "one partition per lane");
+    const uint64_t p_first = a.seg_chunk_scan[lane];
+    const uint64_t p_end = a.seg_chunk_scan[lane + 1u];
+    const uint64_t p_count = a.seg_part_count[lane];
+    // wait for them here: left pending into the chunk loop, their first use there would also wait
+    // for the prefetches issued after them (vmcnt: vector memory counter 0, others unchanged)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    // this lane's segment of chunk c (false past the chunk's partition count or the last chunk)
+    auto chunk_segment = [&](uint64_t c, uint4& sg) -> bool {
+        if (c >= chunks_total)
+            return false;
+        const uint32_t part = (uint32_t)__popcll(__ballot(p_end <= c));
+        const uint64_t first = read_lane_u64(p_first, part);
+        const uint64_t cnt = read_lane_u64(p_count, part);
+        const uint64_t local = (c - first) * 64u + lane;
+        if (local >= cnt)
+            return false;
+        sg = a.segs[(uint64_t)part * a.seg_part_cap + local];
+        return true;
+    };
+    // software pipeline over this wave's chunks c, c + W, c + 2W, ...: while chunk c is walked,
+    // the frames of chunk c + W and the segments of chunk c + 2W are in flight
+    const uint64_t c_first =
+        (uint64_t)blockIdx.x * (EMIT_THREADS / 64) + (uint64_t)__builtin_amdgcn_readfirstlane(wave);
+    uint4 sg_cur = make_uint4(0u, 0u, 0u, 0u), sg_next = make_uint4(0u, 0u, 0u, 0u);
+    bool has_cur = chunk_segment(c_first, sg_cur);
+    bool has_next = chunk_segment(c_first + waves_total, sg_next);
+    // lanes without a segment load Gaussian 0's frame (sg.x = 0; there are Gaussians whenever
+    // there are chunks) and walk no step
+    PrimFrame f_cur = {};
+    PrimVal v_cur = {};
+    if (c_first < chunks_total)
     {
-        // partition of chunk c (wave-uniform binary search over the chunk scan)
-        uint32_t lo = 0, hi = SEG_PARTS - 1;
-        while (lo < hi)
-        {
-            uint32_t mid = (lo + hi + 1) >> 1;
-            if (a.seg_chunk_scan[mid] <= c)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const uint64_t local = (c - a.seg_chunk_scan[lo]) * 64u + lane;
+        f_cur = a.frames[sg_cur.x];
+        if (with_alpha)
+            v_cur = a.vals[sg_cur.x];
+    }
+    for (uint64_t c = c_first; c < chunks_total; c += waves_total)
+    {
+        const PrimFrame f_next = a.frames[sg_next.x];
+        PrimVal v_next = {};
+        if (with_alpha)
+            v_next = a.vals[sg_next.x];
+        uint4 sg_after = make_uint4(0u, 0u, 0u, 0u);
+        const bool has_after = chunk_segment(c + 2u * waves_total, sg_after);
+
         int32_t len = 0, i = 0, j = 0, k0 = 0;
         uint64_t g = 0;
-        PrimFrame f;
-        if (local < a.seg_part_count[lo])
+        const PrimFrame f = f_cur;
+        const PrimVal pv = v_cur;
+        if (has_cur)
         {
-            const uint4 sg = a.segs[(uint64_t)lo * a.seg_part_cap + local];
-            g = sg.x;
-            i = (int32_t)sg.y;
-            j = (int32_t)sg.z;
-            k0 = (int32_t)(sg.w & 0xFFFFFu) - 131072;
-            len = (int32_t)(sg.w >> 20);
-            f = a.frames[g];
+            g = sg_cur.x;
+            i = (int32_t)sg_cur.y;
+            j = (int32_t)sg_cur.z;
+            k0 = (int32_t)(sg_cur.w & 0xFFFFFu) - 131072;
+            len = (int32_t)(sg_cur.w >> 20);
         }
-        const int32_t steps = wave_max(len);
-        // voxel key along the segment: the (i, j) part is fixed, k moves the z leaf and slot
+        sg_cur = sg_next;
+        has_cur = has_next;
+        f_cur = f_next;
+        v_cur = v_next;
+        sg_next = sg_after;
+        has_next = has_after;
+        static_assert(EMIT_SEG < (1 << 5), "segment lengths fit 5 bits");
+        const int32_t steps = wave_max_small(len, 5);
+        // key of voxel (i, j, k0 + st) along the segment: the (i, j) part is fixed, so from one k
+        // to the next the key moves by one slot, or past the z leaf's last slot (k = 7 mod 8) to
+        // slot 0 of the next leaf (+ 2^9 - 7 in the voxel field)
         const uint32_t kshift = a.kl.gid_bits + a.kl.alpha_bits;
-        const uint64_t ij_key = encode_voxel(a.kl, i, j, a.kl.lmin[2] * 8);
-        const uint64_t g_bits = g << a.kl.alpha_bits;
-        walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float* vmin, const float* vmax) {
-            uint64_t key = 0ull;
-            if (keep)
-            {
-                const int32_t k = k0 + st;
-                const uint64_t vkey = ij_key + ((uint64_t)(uint32_t)((k >> 3) - a.kl.lmin[2]) << 9) + (uint64_t)(k & 7);
-                key = (vkey << kshift) | g_bits;
-            }
+        const uint64_t key_step = 1ull << kshift;
+        const uint64_t key_leaf = (512ull - 7ull) << kshift;
+        uint64_t key = 0ull;
+        if (len > 0)
+        {
+            const uint64_t vkey = encode_voxel(a.kl, i, j, a.kl.lmin[2] * 8) +
+                                  ((uint64_t)(uint32_t)((k0 >> 3) - a.kl.lmin[2]) << 9) + (uint64_t)(k0 & 7);
+            key = (vkey << kshift) | (g << a.kl.alpha_bits);
+        }
+        walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float* ea, const float* eb) {
             const uint64_t m = __ballot(keep);
             if (keep)
             {
-                const uint32_t slot = count + (uint32_t)__popcll(m & lt_mask);
+                const uint32_t slot =
+                    count + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 stage[slot] = key;
                 if (with_alpha)
                 {
-                    // the box for the alpha; [done, count) spans fewer than 64 + 64 slots
+                    // the box and value constants for the alpha; [done, count) spans fewer than
+                    // 64 + 64 slots
                     const uint32_t r = slot & (uint32_t)(EMIT_RING - 1);
 #pragma unroll
                     for (int c = 0; c < 3; c++)
                     {
-                        ring[c][r] = vmin[c];
-                        ring[3 + c][r] = vmax[c];
+                        ring[c][r] = __float_as_uint(ea[c]);
+                        ring[3 + c][r] = __float_as_uint(eb[c]);
                     }
+                    ring[6][r] = __float_as_uint(pv.opacity);
+                    ring[7][r] = __float_as_uint(pv.local_scale);
+                    ring[8][r] = __float_as_uint(pv.pow_exp);
+                    ring[9][r] = pv.rgb_cull;
                 }
             }
+            key += ((k0 + st) & 7) == 7 ? key_leaf : key_step;
             count += (uint32_t)__popcll(m);
             if (with_alpha && count - done >= 64u)
             {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                emit_alpha_batch(a, stage, ring, done, 64u, lane);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                wave_sync_lds();
+                emit_alpha_batch(stage, ring, done, 64u, lane);
+                wave_sync_lds();
                 done += 64u;
             }
             if (count > (uint32_t)(EMIT_WAVE_STAGE - 64))
             {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (with_alpha)
-                {
-                    emit_alpha_batch(a, stage, ring, done, count - done, lane);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                }
+                wave_sync_lds();
+                finish_alpha();
                 emit_flush_wave(a, stage, count, lane, s_hist);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                wave_sync_lds();
                 count = 0u;
                 done = 0u;
             }
         });
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (with_alpha)
-    {
-        emit_alpha_batch(a, stage, ring, done, count - done, lane);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+    wave_sync_lds();
+    finish_alpha();
     emit_flush_wave(a, stage, count, lane, s_hist);
     if (a.hist)
     {
@@ -989,6 +1122,17 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_emit_segments(EmitArgs a)
                 atomicAdd(&a.hist[x], (unsigned long long)c);
         }
     }
+}
+
+uint32_t emit_blocks_per_cu()
+{
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_emit_segments, EMIT_THREADS, 0) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return 0u;
+    }
+    return blocks > 0 ? (uint32_t)blocks : 0u;
 }
 
 // 2^(j / 64), j = 0..63, as the nearest doubles
@@ -1364,13 +1508,6 @@ __global__ __launch_bounds__(256) void k_bwd_prims(BwdArgs a)
     q.o = p.o;
     q.pad = 0.f;
     a.prims[g] = q;
-}
-
-__device__ __forceinline__ void wave_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ double readlane_f64(double x, int l)
