@@ -683,7 +683,7 @@ PNANOVDB_API int vdbr_autotune_sort(vdbr_context_t* ctx, uint64_t key_count, uin
                                     uint32_t* best_variant, uint32_t* best_digit_bits);
 
 /* Fixes the sort configuration for sorts of from_count keys or more (variant index as reported
- * by vdbr_autotune_sort, digit width 6..8). */
+ * by vdbr_autotune_sort, digit width 6..9; 9-bit digits need a shape of at least 512 threads). */
 PNANOVDB_API int vdbr_set_sort_config(vdbr_context_t* ctx, uint64_t from_count, uint32_t variant,
                                       uint32_t digit_bits);
 

@@ -98,15 +98,15 @@ int Context::sort_workspace(uint64_t n, bool with_vals, uint64_t tiles, bool aux
 {
     out->keys_alt = (uint64_t*)ws(aux ? WS_AUX_ALT : WS_SORT_ALT, n * sizeof(uint64_t));
     out->vals_alt = with_vals ? (uint32_t*)ws(aux ? WS_AUX_VALS_ALT : WS_SORT_VALS_ALT, n * sizeof(uint32_t)) : nullptr;
-    out->status = (uint64_t*)ws(WS_SORT_STATUS, tiles * 256 * sizeof(uint64_t));
+    out->status = (uint64_t*)ws(WS_SORT_STATUS, tiles * SORT_BINS * sizeof(uint64_t));
     // histograms and digit starts of up to 16 passes, then 16 tile counters
-    uint8_t* small = (uint8_t*)ws(WS_SORT_SMALL, 16 * 256 * 8 * 2 + 64);
+    uint8_t* small = (uint8_t*)ws(WS_SORT_SMALL, 16 * SORT_BINS * 8 * 2 + 64);
     if (!out->keys_alt || (with_vals && !out->vals_alt) || !out->status || !small)
         return -1;
     out->hist = (uint64_t*)small;
-    out->starts = (uint64_t*)(small + 16 * 256 * 8);
-    out->tile_counters = (uint32_t*)(small + 16 * 256 * 8 * 2);
-    sort_status_tiles = ws_size[WS_SORT_STATUS] / (256 * sizeof(uint64_t));
+    out->starts = (uint64_t*)(small + 16 * SORT_BINS * 8);
+    out->tile_counters = (uint32_t*)(small + 16 * SORT_BINS * 8 * 2);
+    sort_status_tiles = ws_size[WS_SORT_STATUS] / (SORT_BINS * sizeof(uint64_t));
     return 0;
 }
 

@@ -128,6 +128,11 @@ struct SortConfig
     uint32_t digit_bits;
 };
 constexpr uint32_t SORT_BUCKETS = 40; // per floor(log2(key count))
+// radix digits of 6 to 9 bits; histograms, digit starts and look-back status words are laid out
+// with SORT_BINS entries per pass (per tile)
+constexpr uint32_t SORT_MIN_DIGIT = 6;
+constexpr uint32_t SORT_MAX_DIGIT = 9;
+constexpr uint32_t SORT_BINS = 1u << SORT_MAX_DIGIT;
 
 struct Context
 {

@@ -338,7 +338,7 @@ __device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint64_t flag, uint6
 }
 
 // digit of pass p covers key bits [bit_begin + W p, min(bit_end, bit_begin + W p + W)) for the
-// digit width W <= 8; the same mask is applied here and in k_sort_pass so the per-pass counts
+// digit width W <= SORT_MAX_DIGIT; the same mask is applied here and in k_sort_pass so the per-pass counts
 // always agree.
 // Each thread counts 16 consecutive keys and adds runs of equal digits at once: the leading
 // digits of neighbouring pair keys are mostly equal, so per-key LDS atomics would serialise.
@@ -348,9 +348,9 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
 {
     const uint64_t n = cn.get();
     const uint32_t passes = (bit_count + W - 1u) / W;
-    __shared__ uint32_t h[SORT_MAX_PASSES][256];
-    for (int p = 0; p < SORT_MAX_PASSES; p++)
-        h[p][threadIdx.x] = 0u;
+    __shared__ uint32_t h[SORT_MAX_PASSES][SORT_BINS];
+    for (uint32_t x = threadIdx.x; x < SORT_MAX_PASSES * SORT_BINS; x += 256u)
+        (&h[0][0])[x] = 0u;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * HIST_RUN;
     for (uint64_t c = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * HIST_RUN; c < n; c += stride)
@@ -399,31 +399,47 @@ __global__ __launch_bounds__(256) void k_sort_hist(const uint64_t* __restrict__ 
         }
     }
     __syncthreads();
-    for (uint32_t p = 0; p < passes; p++)
+    for (uint32_t x = threadIdx.x; x < passes * SORT_BINS; x += 256u)
     {
-        uint32_t c = h[p][threadIdx.x];
+        uint32_t c = (&h[0][0])[x];
         if (c)
-            atomicAdd(&hist[p * 256 + threadIdx.x], (unsigned long long)c);
+            atomicAdd(&hist[x], (unsigned long long)c);
     }
 }
 
-// exclusive scan of each pass's 256 digit counts (one block of 256 threads)
-__global__ __launch_bounds__(256) void k_sort_digit_starts(const unsigned long long* __restrict__ hist, uint32_t passes,
-                                                           uint64_t* __restrict__ starts)
+// exclusive scan of each pass's SORT_BINS digit counts (one block, a thread per digit)
+__global__ __launch_bounds__(SORT_BINS) void k_sort_digit_starts(const unsigned long long* __restrict__ hist,
+                                                                 uint32_t passes, uint64_t* __restrict__ starts)
 {
-    __shared__ uint64_t smem[4];
+    __shared__ uint64_t wsum[SORT_BINS / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t p = 0; p < passes; p++)
     {
-        uint64_t c = hist[p * 256 + threadIdx.x];
-        uint64_t total;
-        uint64_t incl = block_scan_inclusive<uint64_t, 1>(c, smem, &total);
-        starts[p * 256 + threadIdx.x] = incl - c;
+        const uint64_t c = hist[p * SORT_BINS + threadIdx.x];
+        uint64_t v = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            const uint64_t o = __shfl_up(v, off, 64);
+            if (lane >= (uint32_t)off)
+                v += o;
+        }
+        if (lane == 63u)
+            wsum[wave] = v;
+        __syncthreads();
+        uint64_t pre = 0ull;
+        for (uint32_t w = 0; w < wave; w++)
+            pre += wsum[w];
+        starts[p * SORT_BINS + threadIdx.x] = pre + v - c;
         __syncthreads();
     }
 }
 
+// keys only, tiles of up to 8192 keys: at most 128 VGPRs (4 waves per SIMD), so that two
+// 512-thread tiles share a CU; the other shapes are held to one or two tiles per CU by their LDS
 template <int SORT_THREADS, int SORT_ITEMS, bool HAS_VALS>
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
+__global__ __launch_bounds__(SORT_THREADS) __attribute__((
+    amdgpu_waves_per_eu(!HAS_VALS && SORT_ITEMS <= 16 && SORT_THREADS * SORT_ITEMS <= 8192 ? 4 : 1))) void k_sort_pass(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                             const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
                                                             DCount cn, uint32_t shift, uint32_t digit_bits,
                                                             const uint64_t* __restrict__ digit_start,
@@ -433,12 +449,15 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
     constexpr int WAVES = SORT_THREADS / 64;
     constexpr int SORT_WAVE_TILE = 64 * SORT_ITEMS;
     constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;
-    static_assert(WAVES >= 4, "digit bookkeeping uses the first 256 threads");
+    static_assert(WAVES >= 4, "digit bookkeeping: a thread per digit of up to 8 bits");
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t wcnt[WAVES][256];
-    __shared__ uint32_t s_hist_excl[256];
-    __shared__ uint64_t s_base[256];
-    __shared__ uint32_t s_wsum[4];
+    // per wave and digit: 16-bit counts (at most 64 x SORT_ITEMS), two digits per word, later the
+    // wave's exclusive offset within the tile's digit
+    __shared__ uint32_t wcnt[WAVES][SORT_BINS / 2];
+    __shared__ uint32_t s_hist_excl[SORT_BINS];
+    __shared__ uint64_t s_base[SORT_BINS];
+    __shared__ uint32_t s_wsum[SORT_BINS / 64];
+    static_assert(64 * SORT_ITEMS < 65536, "16-bit wave counts");
     __shared__ uint64_t s_keys[SORT_TILE];
     __shared__ uint32_t s_vals[HAS_VALS ? SORT_TILE : 1];
 
@@ -447,7 +466,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
     const uint32_t lane = tid & 63u;
     if (tid == 0)
         s_tile = atomicAdd(tile_counter, 1u);
-    for (uint32_t x = tid; x < WAVES * 256u; x += SORT_THREADS)
+    for (uint32_t x = tid; x < WAVES * (SORT_BINS / 2); x += SORT_THREADS)
         (&wcnt[0][0])[x] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -470,59 +489,82 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
         if (HAS_VALS)
             vals[m] = valid ? vin[idx] : 0u;
     }
+    // ranks in groups of G keys: the lowest lane of each peer group (lanes holding the same digit)
+    // adds the group's size to the wave's count of the digit and gets the count before it. The
+    // LDS unit applies one wave's atomics in issue order, so a group's G additions are issued
+    // back to back without waiting for each other's result; the leaders' old counts are then
+    // broadcast to their peers.
+    constexpr int G = 4;
+    static_assert(SORT_ITEMS % G == 0, "whole groups");
 #pragma unroll
-    for (int m = 0; m < SORT_ITEMS; m++)
+    for (int m0 = 0; m0 < SORT_ITEMS; m0 += G)
     {
-        uint64_t idx = base + (uint64_t)m * 64u + lane;
-        bool valid = idx < n;
-        uint32_t d = (uint32_t)(keys[m] >> shift) & dmask;
-        // peers: lanes holding the same digit. One step per digit bit (a uniform exit ends the
-        // unrolled steps at the pass's width); per step the ballot of the bit is combined as
-        // peers &= ~(ballot ^ sign-extended bit): ballot where the bit is set, ~ballot where not
-        const uint64_t vb = __ballot(valid);
-        uint32_t plo = (uint32_t)vb, phi = (uint32_t)(vb >> 32);
+        uint32_t lead[G]; // old count (at the leader, < 2^26) and the leader's lane
 #pragma unroll
-        for (uint32_t b = 0; b < 8u; b++)
+        for (int q = 0; q < G; q++)
         {
-            if (b >= digit_bits)
-                break;
-            const int32_t sbit = (int32_t)(d << (31u - b)) >> 31; // 0 or -1
-            const uint64_t bal = __ballot(sbit != 0);
-            plo &= ~((uint32_t)bal ^ (uint32_t)sbit);
-            phi &= ~((uint32_t)(bal >> 32) ^ (uint32_t)sbit);
+            const int m = m0 + q;
+            uint64_t idx = base + (uint64_t)m * 64u + lane;
+            bool valid = idx < n;
+            uint32_t d = (uint32_t)(keys[m] >> shift) & dmask;
+            // peers: one step per digit bit (a uniform exit ends the unrolled steps at the pass's
+            // width); per step the ballot of the bit is combined as
+            // peers &= ~(ballot ^ sign-extended bit): ballot where the bit is set, ~ballot where not
+            const uint64_t vb = __ballot(valid);
+            uint32_t plo = (uint32_t)vb, phi = (uint32_t)(vb >> 32);
+#pragma unroll
+            for (uint32_t b = 0; b < SORT_MAX_DIGIT; b++)
+            {
+                if (b >= digit_bits)
+                    break;
+                const int32_t sbit = (int32_t)(d << (31u - b)) >> 31; // 0 or -1
+                const uint64_t bal = __ballot(sbit != 0);
+                plo &= ~((uint32_t)bal ^ (uint32_t)sbit);
+                phi &= ~((uint32_t)(bal >> 32) ^ (uint32_t)sbit);
+            }
+            const uint64_t peers = ((uint64_t)phi << 32) | plo;
+            const uint32_t below = (uint32_t)__popcll(peers & lt_mask);
+            uint32_t old = 0u;
+            if (valid && below == 0u)
+            {
+                const uint32_t hs = (d & 1u) << 4;
+                old = (atomicAdd(&wcnt[wave][d >> 1], (uint32_t)__popcll(peers) << hs) >> hs) & 0xFFFFu;
+            }
+            rank[m] = below;
+            lead[q] = (old << 6) | (valid ? (uint32_t)__builtin_ctzll(peers) : lane);
         }
-        const uint64_t peers = ((uint64_t)phi << 32) | plo;
-        uint32_t old = valid ? wcnt[wave][d] : 0u;
-        rank[m] = old + (uint32_t)__popcll(peers & lt_mask);
-        if (valid && (peers & lt_mask) == 0ull)
-            wcnt[wave][d] = old + (uint32_t)__popcll(peers);
-        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < G; q++)
+            rank[m0 + q] += (uint32_t)__shfl((int)(lead[q] >> 6), (int)(lead[q] & 63u), 64);
     }
     __syncthreads();
 
-    // per digit (thread tid < 256 handles digit tid): wave offsets, tile count, look-back
+    // per digit (thread tid < 2^digit_bits handles digit tid): wave offsets, tile count, look-back
+    const uint32_t nbins = 1u << digit_bits;
+    uint16_t* woff = (uint16_t*)&wcnt[0][0];
     uint64_t excl = 0ull;
     uint32_t total = 0u;
     const uint32_t d = tid;
-    if (tid < 256u)
+    if (tid < nbins)
     {
         uint32_t c[WAVES];
 #pragma unroll
         for (int w = 0; w < WAVES; w++)
-            c[w] = wcnt[w][d];
+            c[w] = (wcnt[w][d >> 1] >> ((d & 1u) << 4)) & 0xFFFFu;
+        // each thread rewrites only its own 16-bit half: the wave's exclusive offset
 #pragma unroll
         for (int w = 0; w < WAVES; w++)
         {
-            wcnt[w][d] = total;
+            woff[w * SORT_BINS + d] = (uint16_t)total;
             total += c[w];
         }
-        uint64_t* my_status = status + (uint64_t)tile * 256u + d;
+        uint64_t* my_status = status + (uint64_t)tile * SORT_BINS + d;
         // publish the tile aggregate first (look-back chains never wait on a later tile)
         __hip_atomic_store(my_status, st_pack(epoch, tile == 0 ? ST_INCL : ST_AGG, total), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    // exclusive scan of the tile digit counts (threads 0..255)
-    if (tid < 256u)
+    // exclusive scan of the tile digit counts (threads 0 .. nbins - 1)
+    if (tid < nbins)
     {
         uint32_t v = total;
 #pragma unroll
@@ -536,7 +578,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
             s_wsum[wave] = v;
     }
     __syncthreads();
-    if (tid < 256u)
+    if (tid < nbins)
     {
         uint32_t pre = 0u;
         for (uint32_t w = 0; w < wave; w++)
@@ -560,8 +602,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
                 uint64_t st4[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++)
-                    st4[q] = (t - q >= 0) ? __hip_atomic_load(status + (uint64_t)(t - q) * 256u + d, __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_AGENT)
+                    st4[q] = (t - q >= 0) ? __hip_atomic_load(status + (uint64_t)(t - q) * SORT_BINS + d,
+                                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                           : 0ull;
                 int consumed = 0;
                 bool done = false;
@@ -586,7 +628,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
                 if (consumed == 0)
                     __builtin_amdgcn_s_sleep(1);
             }
-            __hip_atomic_store(status + (uint64_t)tile * 256u + d, st_pack(epoch, ST_INCL, excl + total),
+            __hip_atomic_store(status + (uint64_t)tile * SORT_BINS + d, st_pack(epoch, ST_INCL, excl + total),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         s_base[d] = digit_start[d] + excl - (uint64_t)s_hist_excl[d];
@@ -599,7 +641,7 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
         if (idx < n)
         {
             uint32_t dd = (uint32_t)(keys[m] >> shift) & dmask;
-            uint32_t pos = s_hist_excl[dd] + wcnt[wave][dd] + rank[m];
+            uint32_t pos = s_hist_excl[dd] + (uint32_t)woff[wave * SORT_BINS + dd] + rank[m];
             s_keys[pos] = keys[m];
             if (HAS_VALS)
                 s_vals[pos] = vals[m];
@@ -608,14 +650,33 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pass(const uint64_t* __re
     __syncthreads();
     uint64_t tile_begin = (uint64_t)tile * SORT_TILE;
     uint32_t tile_n = (uint32_t)((n - tile_begin) < (uint64_t)SORT_TILE ? (n - tile_begin) : (uint64_t)SORT_TILE);
-    for (uint32_t p = tid; p < tile_n; p += SORT_THREADS)
+    // 8 keys per thread at a time: their LDS reads (key, then its digit's base) are issued
+    // together instead of each store waiting for two dependent reads of its own
+    constexpr uint32_t U = 8;
+    for (uint32_t p0 = tid; p0 < tile_n; p0 += SORT_THREADS * U)
     {
-        uint64_t k = s_keys[p];
-        uint32_t dd = (uint32_t)(k >> shift) & dmask;
-        uint64_t g = s_base[dd] + p;
-        kout[g] = k;
-        if (HAS_VALS)
-            vout[g] = s_vals[p];
+        uint64_t k[U];
+        uint64_t g[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++)
+        {
+            const uint32_t p = p0 + u * SORT_THREADS;
+            k[u] = p < tile_n ? s_keys[p] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++)
+            g[u] = s_base[(uint32_t)(k[u] >> shift) & dmask] + p0 + u * SORT_THREADS;
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++)
+        {
+            const uint32_t p = p0 + u * SORT_THREADS;
+            if (p < tile_n)
+            {
+                kout[g[u]] = k[u];
+                if (HAS_VALS)
+                    vout[g[u]] = s_vals[p];
+            }
+        }
     }
 }
 
@@ -631,10 +692,13 @@ static uint32_t size_bucket(uint64_t n)
 static SortConfig sort_config(const Context* ctx, uint64_t n, bool with_vals)
 {
     SortConfig c = ctx->sort_tuned[size_bucket(n)];
-    if (c.variant >= SORT_VARIANT_COUNT || c.digit_bits < 6u || c.digit_bits > 8u)
+    if (c.variant >= SORT_VARIANT_COUNT || c.digit_bits < SORT_MIN_DIGIT || c.digit_bits > SORT_MAX_DIGIT)
         c = SortConfig{ 0u, 8u };
     if (with_vals && !SORT_VARIANTS[c.variant].with_vals)
         c.variant = 0u;
+    // 9-bit digits need a thread per digit
+    if (c.digit_bits > 8u && SORT_VARIANTS[c.variant].threads < (int)SORT_BINS)
+        c.digit_bits = 8u;
     return c;
 }
 
@@ -702,7 +766,7 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, DCount cn, uint
     SortWorkspace ws;
     if (ctx->sort_workspace(n, vals != nullptr, tiles, true, &ws))
         return -1;
-    if (fill_words(ctx, ws.hist, host_count(SORT_MAX_PASSES * 256), 0ull) ||
+    if (fill_words(ctx, ws.hist, host_count(SORT_MAX_PASSES * SORT_BINS), 0ull) ||
         fill_words(ctx, ws.tile_counters, host_count(SORT_MAX_PASSES), 0u))
         return -1;
     uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
@@ -710,8 +774,8 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, DCount cn, uint
         hist_blocks = 2048u;
     hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, cn, bit_begin, bits, W,
                        (unsigned long long*)ws.hist);
-    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
-                       passes, ws.starts);
+    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(SORT_BINS), 0, ctx->stream,
+                       (const unsigned long long*)ws.hist, passes, ws.starts);
     uint64_t* kin = keys;
     uint64_t* kout = ws.keys_alt;
     uint32_t* vin = vals;
@@ -723,10 +787,10 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, DCount cn, uint
         uint32_t epoch = ctx->next_sort_epoch(tiles);
         if (vals)
             launch_sort_pass<true>(cfg.variant, tiles, ctx->stream, kin, kout, vin, vout, cn, shift, dbits,
-                                   ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
+                                   ws.starts + p * SORT_BINS, ws.status, ws.tile_counters + p, epoch);
         else
             launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, cn, shift, dbits,
-                                    ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
+                                    ws.starts + p * SORT_BINS, ws.status, ws.tile_counters + p, epoch);
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;
@@ -862,7 +926,7 @@ uint32_t sort_digit_bits(const Context* ctx, uint64_t n)
 unsigned long long* sort_hist_buffer(Context* ctx)
 {
     // the first part of the small sort workspace (same size as sort_workspace asks for)
-    return (unsigned long long*)ctx->ws(WS_SORT_SMALL, 16 * 256 * 8 * 2 + 64);
+    return (unsigned long long*)ctx->ws(WS_SORT_SMALL, 16 * SORT_BINS * 8 * 2 + 64);
 }
 
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_begin, uint32_t bit_end,
@@ -891,7 +955,7 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_
         return -1;
     if (!hist_ready)
     {
-        if (fill_words(ctx, ws.hist, host_count(SORT_MAX_PASSES * 256), 0ull))
+        if (fill_words(ctx, ws.hist, host_count(SORT_MAX_PASSES * SORT_BINS), 0ull))
             return -1;
         uint32_t hist_blocks = (uint32_t)((n + 256 * HIST_RUN - 1) / (256 * HIST_RUN));
         if (hist_blocks > 2048u)
@@ -899,8 +963,8 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_
         hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, ctx->stream, keys, cn, bit_begin, bits, W,
                            (unsigned long long*)ws.hist);
     }
-    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, ctx->stream, (const unsigned long long*)ws.hist,
-                       passes, ws.starts);
+    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(SORT_BINS), 0, ctx->stream,
+                       (const unsigned long long*)ws.hist, passes, ws.starts);
     uint64_t* kin = keys;
     uint64_t* kout = ws.keys_alt;
     if (passes_begin && !ctx->capturing)
@@ -911,7 +975,7 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_
         uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
         uint32_t epoch = ctx->next_sort_epoch(tiles);
         launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, cn, shift, dbits,
-                                ws.starts + p * 256, ws.status, ws.tile_counters + p, epoch);
+                                ws.starts + p * SORT_BINS, ws.status, ws.tile_counters + p, epoch);
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;
@@ -926,9 +990,9 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_
 }
 
 // ============================================================================ sort autotuner
-// Times every tile shape x digit width on hashed keys of the requested size and bit count and
-// records the fastest for that size bucket and all larger ones. Candidate digit widths keep the
-// pass count of 8-bit digits or add one pass; every candidate produces the same sorted order.
+// Times every tile shape x digit width (9, 8 and 7 bits; 9 on the shapes of at least 512
+// threads) on hashed keys of the requested size and bit count and records the fastest for that
+// size bucket and all larger ones; every candidate produces the same sorted order.
 __global__ void k_tune_keys(uint64_t* __restrict__ keys, uint64_t n, uint32_t key_bits, uint64_t seed)
 {
     const uint64_t mask = key_bits >= 64u ? ~0ull : ((1ull << key_bits) - 1ull);
@@ -944,8 +1008,10 @@ __global__ void k_tune_keys(uint64_t* __restrict__ keys, uint64_t n, uint32_t ke
 
 int set_sort_config(Context* ctx, uint64_t from_count, uint32_t variant, uint32_t digit_bits)
 {
-    if (variant >= SORT_VARIANT_COUNT || digit_bits < 6u || digit_bits > 8u)
-        return ctx->fail("sort config: unknown variant or digit width outside [6, 8]");
+    if (variant >= SORT_VARIANT_COUNT || digit_bits < SORT_MIN_DIGIT || digit_bits > SORT_MAX_DIGIT)
+        return ctx->fail("sort config: unknown variant or digit width outside [6, 9]");
+    if (digit_bits > 8u && SORT_VARIANTS[variant].threads < (int)SORT_BINS)
+        return ctx->fail("sort config: 9-bit digits need a shape of at least 512 threads");
     for (uint32_t b = size_bucket(from_count); b < SORT_BUCKETS; b++)
         ctx->sort_tuned[b] = SortConfig{ variant, digit_bits };
     return 0;
@@ -972,10 +1038,12 @@ int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_
     SortConfig best_cfg{ 0u, 8u };
     uint32_t k = 0;
     int rc = 0;
-    const uint32_t widths[2] = { 8u, 7u };
+    const uint32_t widths[3] = { 9u, 8u, 7u };
     for (uint32_t v = 0; v < SORT_VARIANT_COUNT && rc == 0; v++)
-        for (uint32_t wi = 0; wi < 2 && rc == 0; wi++)
+        for (uint32_t wi = 0; wi < 3 && rc == 0; wi++)
         {
+            if (widths[wi] > 8u && SORT_VARIANTS[v].threads < (int)SORT_BINS)
+                continue; // 9-bit digits need a thread per digit
             ctx->sort_tuned[bucket] = SortConfig{ v, widths[wi] };
             float ms_best = 3.0e38f;
             for (int rep = 0; rep < 2 && rc == 0; rep++)
@@ -1050,9 +1118,9 @@ constexpr uint64_t NOALLOC_MAX_TILES = (NOALLOC_MAX_KEYS + NOALLOC_TILE - 1) / N
 constexpr size_t NA_KEYS_OFF = 0;
 constexpr size_t NA_VALS_OFF = NA_KEYS_OFF + NOALLOC_MAX_KEYS * sizeof(uint64_t);
 constexpr size_t NA_STATUS_OFF = NA_VALS_OFF + NOALLOC_MAX_KEYS * sizeof(uint32_t);
-constexpr size_t NA_HIST_OFF = NA_STATUS_OFF + NOALLOC_MAX_TILES * 256 * sizeof(uint64_t);
-constexpr size_t NA_STARTS_OFF = NA_HIST_OFF + SORT_MAX_PASSES * 256 * sizeof(uint64_t);
-constexpr size_t NA_COUNTERS_OFF = NA_STARTS_OFF + SORT_MAX_PASSES * 256 * sizeof(uint64_t);
+constexpr size_t NA_HIST_OFF = NA_STATUS_OFF + NOALLOC_MAX_TILES * SORT_BINS * sizeof(uint64_t);
+constexpr size_t NA_STARTS_OFF = NA_HIST_OFF + SORT_MAX_PASSES * SORT_BINS * sizeof(uint64_t);
+constexpr size_t NA_COUNTERS_OFF = NA_STARTS_OFF + SORT_MAX_PASSES * SORT_BINS * sizeof(uint64_t);
 constexpr size_t NA_BYTES = NA_COUNTERS_OFF + 16 * sizeof(uint32_t);
 static_assert(NOALLOC_MAX_KEYS % NOALLOC_TILE == 0, "whole tiles");
 static_assert(NA_STATUS_OFF % 256 == 0 && NA_HIST_OFF % 8 == 0, "aligned regions");
@@ -1141,13 +1209,13 @@ static int noalloc_sort_on_device(int dev, uint64_t* keys, uint32_t* vals, uint6
         d.ptr = (uint8_t*)p;
         d.epoch = 0u;
         hipLaunchKernelGGL(k_fill_zero_u64, dim3(256), dim3(256), 0, st, (uint64_t*)(d.ptr + NA_STATUS_OFF),
-                           (uint64_t)NOALLOC_MAX_TILES * 256u);
+                           (uint64_t)NOALLOC_MAX_TILES * SORT_BINS);
     }
     if (d.epoch + passes >= 65535u)
     {
         // epochs are 16-bit: clear the status words before they could repeat
         hipLaunchKernelGGL(k_fill_zero_u64, dim3(256), dim3(256), 0, st, (uint64_t*)(d.ptr + NA_STATUS_OFF),
-                           (uint64_t)NOALLOC_MAX_TILES * 256u);
+                           (uint64_t)NOALLOC_MAX_TILES * SORT_BINS);
         d.epoch = 0u;
     }
     uint64_t* keys_alt = (uint64_t*)(d.ptr + NA_KEYS_OFF);
@@ -1163,7 +1231,8 @@ static int noalloc_sort_on_device(int dev, uint64_t* keys, uint32_t* vals, uint6
     if (hist_blocks > 2048u)
         hist_blocks = 2048u;
     hipLaunchKernelGGL(k_sort_hist, dim3(hist_blocks), dim3(256), 0, st, keys, host_count(n), bit_begin, bits, W, hist);
-    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(256), 0, st, (const unsigned long long*)hist, passes, starts);
+    hipLaunchKernelGGL(k_sort_digit_starts, dim3(1), dim3(SORT_BINS), 0, st, (const unsigned long long*)hist, passes,
+                       starts);
     uint64_t* kin = keys;
     uint64_t* kout = keys_alt;
     uint32_t* vin = vals;
@@ -1174,11 +1243,11 @@ static int noalloc_sort_on_device(int dev, uint64_t* keys, uint32_t* vals, uint6
         const uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
         const uint32_t epoch = ++d.epoch;
         if (vals)
-            launch_sort_pass<true>(NOALLOC_VARIANT, tiles, st, kin, kout, vin, vout, host_count(n), shift, dbits, starts + p * 256,
-                                   status, counters + p, epoch);
+            launch_sort_pass<true>(NOALLOC_VARIANT, tiles, st, kin, kout, vin, vout, host_count(n), shift, dbits,
+                                   starts + p * SORT_BINS, status, counters + p, epoch);
         else
             launch_sort_pass<false>(NOALLOC_VARIANT, tiles, st, kin, kout, nullptr, nullptr, host_count(n), shift, dbits,
-                                    starts + p * 256, status, counters + p, epoch);
+                                    starts + p * SORT_BINS, status, counters + p, epoch);
         std::swap(kin, kout);
         std::swap(vin, vout);
     }

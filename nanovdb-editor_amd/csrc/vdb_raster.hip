@@ -281,6 +281,7 @@ constexpr int EMIT_THREADS = 256;
 constexpr int EMIT_ROWS = 1024;       // rows per block
 constexpr int EMIT_WAVE_STAGE = 1792; // staged survivors per wave
 constexpr int EMIT_HIST_PASSES = 4;   // sort passes whose digit histograms the emission can count
+constexpr int EMIT_HIST_WORDS = 1024; // LDS counters: passes x 2^digit width of them (pass-major)
 constexpr int EMIT_RING = 128;        // staged survivors whose alpha is pending, per wave
 constexpr int EMIT_RING_WORDS = 10;   // per survivor: the box extremes (6), its Gaussian's PrimVal (4)
 
@@ -308,6 +309,14 @@ struct EmitArgs
     uint32_t hist_bits;             // sorted key bits
     uint32_t hist_w;                // digit width
 };
+
+// whether the emission can count the digit histograms of a sort of `bits` key bits in digits of
+// `w` bits (its LDS counters hold passes x 2^w of them)
+static inline bool emit_counts_hist(uint32_t bits, uint32_t w)
+{
+    const uint32_t passes = (bits + w - 1u) / w;
+    return passes <= (uint32_t)EMIT_HIST_PASSES && (passes << w) <= (uint32_t)EMIT_HIST_WORDS;
+}
 
 // rows of every Gaussian inside the voxel columns [i0, i1) of a slab
 __global__ void k_slab_rows(const PrimBox* __restrict__ boxes, uint64_t n, int32_t i0, int32_t i1,
@@ -430,7 +439,7 @@ __device__ __forceinline__ void emit_alpha_batch(uint64_t* stage, const uint32_t
 
 // Writes a wave's staged pairs (complete keys) at an atomically reserved position.
 __device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, const uint64_t* stage, uint32_t cnt, uint32_t lane,
-                                                uint32_t (*s_hist)[256])
+                                                uint32_t* s_hist)
 {
     if (cnt == 0u)
         return;
@@ -473,7 +482,7 @@ __device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, const uint64_
                         const uint32_t d = (uint32_t)(k[u] >> shift[p]) & dm[p];
                         if (p < passes && d != cur[p])
                         {
-                            atomicAdd(&s_hist[p][cur[p]], run[p]);
+                            atomicAdd(&s_hist[(p << a.hist_w) + cur[p]], run[p]);
                             cur[p] = d;
                             run[p] = 0u;
                         }
@@ -487,7 +496,7 @@ __device__ __forceinline__ void emit_flush_wave(const EmitArgs& a, const uint64_
 #pragma unroll
             for (uint32_t p = 0; p < (uint32_t)EMIT_HIST_PASSES; p++)
                 if (p < passes)
-                    atomicAdd(&s_hist[p][cur[p]], run[p]);
+                    atomicAdd(&s_hist[(p << a.hist_w) + cur[p]], run[p]);
         }
     }
     unsigned long long base = 0ull;
@@ -962,9 +971,9 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
 {
     __shared__ uint64_t stage_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
     __shared__ uint32_t ring_all[EMIT_THREADS / 64][EMIT_RING_WORDS][EMIT_RING];
-    __shared__ uint32_t s_hist[EMIT_HIST_PASSES][256];
-    for (uint32_t x = threadIdx.x; x < (uint32_t)(EMIT_HIST_PASSES * 256); x += EMIT_THREADS)
-        (&s_hist[0][0])[x] = 0u;
+    __shared__ uint32_t s_hist[EMIT_HIST_WORDS];
+    for (uint32_t x = threadIdx.x; x < (uint32_t)EMIT_HIST_WORDS; x += EMIT_THREADS)
+        s_hist[x] = 0u;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint64_t* stage = stage_all[wave];
@@ -1115,11 +1124,12 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
     {
         __syncthreads();
         const uint32_t passes = (a.hist_bits + a.hist_w - 1u) / a.hist_w;
-        for (uint32_t x = threadIdx.x; x < passes * 256u; x += EMIT_THREADS)
+        for (uint32_t x = threadIdx.x; x < (passes << a.hist_w); x += EMIT_THREADS)
         {
-            const uint32_t c = (&s_hist[0][0])[x];
+            const uint32_t c = s_hist[x];
             if (c)
-                atomicAdd(&a.hist[x], (unsigned long long)c);
+                atomicAdd(&a.hist[(x >> a.hist_w) * SORT_BINS + (x & ((1u << a.hist_w) - 1u))],
+                          (unsigned long long)c);
         }
     }
 }
@@ -4666,7 +4676,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             // estimated pair count; the sort recounts when the final count picks another width)
             hist_w = sort_digit_bits(ctx, sest > 1 ? sest : 2);
             const uint32_t hbits = key_bits - sort_begin;
-            hist_buf = ((hbits + hist_w - 1) / hist_w <= (uint32_t)EMIT_HIST_PASSES) ? sort_hist_buffer(ctx) : nullptr;
+            hist_buf = emit_counts_hist(hbits, hist_w) ? sort_hist_buffer(ctx) : nullptr;
             uint32_t sattempts = 0;
             for (;;)
             {
@@ -4676,7 +4686,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                     return -1;
                 VDBR_CHECK(ctx, hipMemsetAsync(counter, 0, 8, st));
                 if (hist_buf)
-                    VDBR_CHECK(ctx, hipMemsetAsync(hist_buf, 0, EMIT_HIST_PASSES * 256 * sizeof(uint64_t), st));
+                    VDBR_CHECK(ctx, hipMemsetAsync(hist_buf, 0, EMIT_HIST_PASSES * SORT_BINS * sizeof(uint64_t), st));
                 EmitArgs ea;
                 memset(&ea, 0, sizeof(ea));
                 ea.frames = frames;
@@ -5475,10 +5485,10 @@ static int capture_cloud(Context* w, const CloudPlan& p, const uint32_t* win_bas
     const uint32_t hist_w = sort_digit_bits(w, p.pair_cap);
     const uint32_t hbits = p.key_bits - p.sort_begin;
     unsigned long long* hist_buf =
-        ((hbits + hist_w - 1) / hist_w <= (uint32_t)EMIT_HIST_PASSES) ? sort_hist_buffer(w) : nullptr;
+        emit_counts_hist(hbits, hist_w) ? sort_hist_buffer(w) : nullptr;
     if (fill_words(w, words, host_count(2), 0ull))
         return -1;
-    if (hist_buf && fill_words(w, (uint64_t*)hist_buf, host_count(EMIT_HIST_PASSES * 256), 0ull))
+    if (hist_buf && fill_words(w, (uint64_t*)hist_buf, host_count(EMIT_HIST_PASSES * SORT_BINS), 0ull))
         return -1;
     EmitArgs ea = ra;
     ea.vals = p.sb.pvals;

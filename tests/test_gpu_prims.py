@@ -63,14 +63,22 @@ def test_scans(vox, n):
     assert np.array_equal(got, np.maximum.accumulate(a))
 
 
-@pytest.mark.parametrize("digit_bits", [6, 7, 8])
+SORT_SHAPE_THREADS = (512, 256, 1024, 256, 512, 1024, 768)
+
+
+@pytest.mark.parametrize("digit_bits", [6, 7, 8, 9])
 def test_sort_every_shape(digit_bits):
-    """Every onesweep tile shape at every digit width gives the stable order."""
+    """Every onesweep tile shape at every digit width gives the stable order (9-bit digits need
+    a thread per digit: the 256-thread shapes refuse them)."""
     from nanovdb_editor_amd.device import Voxelizer
     v = Voxelizer(0)
     try:
         rng = np.random.default_rng(digit_bits)
         for variant in range(7):
+            if digit_bits == 9 and SORT_SHAPE_THREADS[variant] < 512:
+                with pytest.raises(RuntimeError):
+                    v.set_sort_config(0, variant, digit_bits)
+                continue
             v.set_sort_config(0, variant, digit_bits)
             for n, bits in ((70_001, 45), (1_200_007, 64)):
                 keys = rng.integers(0, 2**63, n, dtype=np.int64).astype(np.uint64)
@@ -97,8 +105,9 @@ def test_autotune_sort():
     v = Voxelizer(0)
     try:
         best, rows = v.autotune_sort(1 << 20, 40)
-        assert len(rows) == 14
-        assert {(r["variant"], r["digit_bits"]) for r in rows} == {(i, w) for i in range(7) for w in (8, 7)}
+        expect = {(i, w) for i in range(7) for w in (8, 7)} | {(i, 9) for i in range(7) if SORT_SHAPE_THREADS[i] >= 512}
+        assert len(rows) == len(expect) == 19
+        assert {(r["variant"], r["digit_bits"]) for r in rows} == expect
         assert best["ms"] == min(r["ms"] for r in rows) and best["ms"] > 0
         # the chosen configuration sorts correctly
         rng = np.random.default_rng(3)
