@@ -25,10 +25,13 @@ HBM_PEAK_GBS = 8000.0
 # reference's unfused operation order and the build disables packed FP32): 256 CUs x 4 SIMDs x
 # 16 lanes x 2.4 GHz
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
-# FP32 operations the walk spends per candidate voxel: one new lattice plane (lz: 2; per axis
-# tz: 1, four corners x (add, mul, mul): 12, min/max over the corners: 6) = 59, the box merge of
-# two planes: 6, and the sphere test: 9
-WALK_OPS_PER_CANDIDATE = 74
+# FP32 operations of the walk (counted in the kernel's instruction stream): per candidate voxel,
+# the new lattice plane's z term (convert, scale, offset: 3), per axis the plane term (1), the
+# carried and the new corner (add, mul, mul: 3 each), the median with 0, its square and the
+# subtraction (3), and the sign test (1) = 34; per kept pair, ordering the box extremes (6), the
+# minimum sample distance (32) and the 8-bit alpha with its error bound (37) = 75
+WALK_OPS_PER_CANDIDATE = 34
+ALPHA_OPS_PER_PAIR = 75
 
 
 def parse():
@@ -387,7 +390,8 @@ def main():
     walk_ms = avg("ms_walk_kernel")
     emit_achieved = emit_bytes / (walk_ms * 1e-3) / 1e9 if walk_ms > 0 else 0.0
     cand = st["candidate_voxels"]
-    valu_tops = cand * WALK_OPS_PER_CANDIDATE / (walk_ms * 1e-3) / 1e12 if walk_ms > 0 else 0.0
+    walk_ops = cand * WALK_OPS_PER_CANDIDATE + (P * ALPHA_OPS_PER_PAIR if args.value_mode == "rgba8" else 0)
+    valu_tops = walk_ops / (walk_ms * 1e-3) / 1e12 if walk_ms > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": args.gaussians / (ms_per_step * 1e-3),
@@ -413,7 +417,8 @@ def main():
                      "avg_launch_ms": walk_ms, "algorithmic_bytes_per_launch": emit_bytes,
                      "valu": {"achieved": valu_tops, "peak": VALU_PEAK_TOPS, "unit": "T FP32 ops/s",
                               "frac": valu_tops / VALU_PEAK_TOPS, "candidate_voxels": cand,
-                              "ops_per_candidate": WALK_OPS_PER_CANDIDATE}},
+                              "ops_per_candidate": WALK_OPS_PER_CANDIDATE,
+                              "ops_per_pair": ALPHA_OPS_PER_PAIR if args.value_mode == "rgba8" else 0}},
         "sort_roofline": {"bound": "hbm", "kernel": "k_sort_pass (onesweep radix pass over 64-bit pair keys)",
                           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                           "traffic": traffic, "avg_launch_ms": pass_ms, "algorithmic_bytes_per_launch": pass_bytes},
