@@ -601,7 +601,7 @@ PNANOVDB_API int vdbr_radix_sort_u64(vdbr_context_t* ctx, uint64_t* keys, uint32
 /* Allocation-free variant of vdbr_radix_sort_u64. No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc). Same sort, same result; the ping-pong buffers, look-back
- * status words and histograms live in a file-scope device array of the library (about 49 MiB,
+ * status words and histograms live in a file-scope device array of the library (about 50 MiB,
  * one copy on every device the library runs on, held until the process ends), so the call makes
  * no hipMalloc/hipFree and allocates nothing else. count is at most 4194304 (2^22) keys; larger
  * counts fail with a message in vdbr_last_error. The work is queued on the context's stream and
