@@ -96,6 +96,18 @@ VDB_DEV void decode_voxel(const KeyLayout& kl, uint64_t v, int32_t* i, int32_t* 
     *k = ((int32_t)lz + kl.lmin[2]) * 8 + (int32_t)(n & 7u);
 }
 
+// leaf index of a pair key's leaf field
+VDB_DEV uint64_t leaf_of_field(const KeyLayout& kl, uint64_t field)
+{
+    return kl.rank_bits ? (uint64_t)kl.leaf_of_rank[field] : field;
+}
+
+// voxel coordinates of a pair key's voxel part (leaf field << 9 | slot)
+VDB_DEV void decode_pair_voxel(const KeyLayout& kl, uint64_t v, int32_t* i, int32_t* j, int32_t* k)
+{
+    decode_voxel(kl, (leaf_of_field(kl, v >> 9) << 9) | (v & 511u), i, j, k);
+}
+
 // gaussian_frag_alloc.slang:79-121 -- AABB of the voxel's 8 corners in the Gaussian frame,
 // each corner computed independently in the reference operation order.
 VDB_DEV void voxel_box(const PrimFrame& f, int32_t i, int32_t j, int32_t k, float vs, float vmin[3], float vmax[3])

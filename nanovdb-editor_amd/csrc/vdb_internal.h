@@ -103,6 +103,9 @@ enum WsSlot
     WS_BATCH_SETUP, // per-cloud setup buffers of a graph-captured batch
     WS_BATCH_TABLE, // per-cloud window bases and output records of a graph-captured batch
     WS_BATCH_ARENA, // output grids of a graph-captured batch
+    WS_LEAF_MAP,    // candidate-leaf bitmap of the last raster call (rank-keyed pairs)
+    WS_LEAF_PREFIX, // candidate leaves before each bitmap word (and the per-word counts)
+    WS_LEAF_RANK,   // leaf index of each candidate-leaf rank
     WS_COUNT
 };
 
@@ -161,10 +164,8 @@ struct Context
     // state of the last raster call (for vdbr_last_pairs)
     uint64_t last_pair_count = 0;
     uint64_t* last_pairs = nullptr;
-    uint32_t last_gid_bits = 0;
-    uint32_t last_alpha_bits = 0;
-    int32_t last_lmin[3] = { 0, 0, 0 };
-    uint32_t last_bits[3] = { 0, 0, 0 };
+    // pair-key layout of the last raster call (its rank tables live in the WS_LEAF_* slots)
+    KeyLayout last_kl = {};
     // the retained pair list of the last density forward (single slab): its id (0: none; ids are
     // never reused on a context), Gaussian count and voxel size, checked by the pair-list adjoint
     uint64_t pair_gen = 0;
@@ -193,6 +194,7 @@ struct Context
         last_pairs = nullptr;
         last_pair_count = 0;
         last_pair_id = 0;
+        last_kl = KeyLayout{};
     }
     int fail(const std::string& msg)
     {
@@ -262,7 +264,8 @@ int radix_sort_u64_noalloc(int dev, uint64_t* keys, uint32_t* vals, uint64_t n, 
 uint64_t noalloc_max_keys();
 // digit width the sort of n keys will use, and the buffer that holds its digit histograms
 // (passes x 256 counts, u64): a producer of the keys may fill it and pass hist_ready
-uint32_t sort_digit_bits(const Context* ctx, uint64_t n);
+// digit width the sort of n keys over `bits` key bits uses
+uint32_t sort_digit_bits(const Context* ctx, uint64_t n, uint32_t bits);
 unsigned long long* sort_hist_buffer(Context* ctx);
 
 // vdb_raster.hip: the GridBuild stage shared by the raster path and the grid build table.

@@ -689,11 +689,18 @@ static uint32_t size_bucket(uint64_t n)
     return b < SORT_BUCKETS ? b : SORT_BUCKETS - 1;
 }
 
-static SortConfig sort_config(const Context* ctx, uint64_t n, bool with_vals)
+// Untuned (or an unusable entry): the default shape with the fewest passes of digits up to
+// SORT_MAX_DIGIT bits, split evenly (21 bits: 7 + 7 + 7; 18 bits: 9 + 9; 64 bits: 8 x 8).
+static SortConfig sort_config(const Context* ctx, uint64_t n, bool with_vals, uint32_t bits)
 {
     SortConfig c = ctx->sort_tuned[size_bucket(n)];
     if (c.variant >= SORT_VARIANT_COUNT || c.digit_bits < SORT_MIN_DIGIT || c.digit_bits > SORT_MAX_DIGIT)
-        c = SortConfig{ 0u, 8u };
+    {
+        const uint32_t b = bits ? bits : 1u;
+        const uint32_t passes = (b + SORT_MAX_DIGIT - 1u) / SORT_MAX_DIGIT;
+        const uint32_t w = (b + passes - 1u) / passes;
+        c = SortConfig{ 0u, std::max<uint32_t>(SORT_MIN_DIGIT, w) };
+    }
     if (with_vals && !SORT_VARIANTS[c.variant].with_vals)
         c.variant = 0u;
     // 9-bit digits need a thread per digit
@@ -755,7 +762,7 @@ int radix_sort_u64(Context* ctx, uint64_t* keys, uint32_t* vals, DCount cn, uint
         return 0;
     const uint64_t n = cn.cap;
     uint32_t bits = bit_end - bit_begin;
-    const SortConfig cfg = sort_config(ctx, n, vals != nullptr);
+    const SortConfig cfg = sort_config(ctx, n, vals != nullptr, bits);
     const uint32_t W = cfg.digit_bits;
     uint32_t passes = (bits + W - 1u) / W;
     if (passes > SORT_MAX_PASSES)
@@ -918,9 +925,9 @@ int radix_sort_dual_key(Context* ctx, uint32_t* key_low, uint32_t* key_high, uin
 
 // sort returning the buffer that holds the result (no copy back): *result_keys is either
 // keys or the workspace alternate
-uint32_t sort_digit_bits(const Context* ctx, uint64_t n)
+uint32_t sort_digit_bits(const Context* ctx, uint64_t n, uint32_t bits)
 {
-    return sort_config(ctx, n, false).digit_bits;
+    return sort_config(ctx, n, false, bits).digit_bits;
 }
 
 unsigned long long* sort_hist_buffer(Context* ctx)
@@ -940,7 +947,7 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_
         return 0;
     const uint64_t n = cn.cap;
     uint32_t bits = bit_end - bit_begin;
-    const SortConfig cfg = sort_config(ctx, n, false);
+    const SortConfig cfg = sort_config(ctx, n, false, bits);
     const uint32_t W = cfg.digit_bits;
     uint32_t passes = (bits + W - 1u) / W;
     if (passes > SORT_MAX_PASSES)

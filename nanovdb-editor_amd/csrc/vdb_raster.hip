@@ -1021,15 +1021,33 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
     uint4 sg_cur = make_uint4(0u, 0u, 0u, 0u), sg_next = make_uint4(0u, 0u, 0u, 0u);
     bool has_cur = chunk_segment(c_first, sg_cur);
     bool has_next = chunk_segment(c_first + waves_total, sg_next);
+    // with candidate-leaf ranks, the bitmap word and rank prefix of the segment's first leaf
+    const bool ranked = a.kl.rank_bits != 0u;
+    auto rank_fetch = [&](bool has, const uint4& sg, uint64_t& word, uint32_t& pre, uint32_t& bit) {
+        word = 0ull;
+        pre = 0u;
+        bit = 0u;
+        if (ranked && has)
+        {
+            const int32_t i = (int32_t)sg.y, j = (int32_t)sg.z, k = (int32_t)(sg.w & 0xFFFFFu) - 131072;
+            const uint64_t leaf = encode_voxel(a.kl, i, j, k) >> 9;
+            word = a.kl.leaf_map[leaf >> 6];
+            pre = a.kl.leaf_prefix[leaf >> 6];
+            bit = (uint32_t)(leaf & 63u);
+        }
+    };
     // lanes without a segment load Gaussian 0's frame (sg.x = 0; there are Gaussians whenever
     // there are chunks) and walk no step
     PrimFrame f_cur = {};
     PrimVal v_cur = {};
+    uint64_t rw_cur = 0ull;
+    uint32_t rp_cur = 0u, rb_cur = 0u;
     if (c_first < chunks_total)
     {
         f_cur = a.frames[sg_cur.x];
         if (with_alpha)
             v_cur = a.vals[sg_cur.x];
+        rank_fetch(has_cur, sg_cur, rw_cur, rp_cur, rb_cur);
     }
     for (uint64_t c = c_first; c < chunks_total; c += waves_total)
     {
@@ -1037,6 +1055,9 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
         PrimVal v_next = {};
         if (with_alpha)
             v_next = a.vals[sg_next.x];
+        uint64_t rw_next;
+        uint32_t rp_next, rb_next;
+        rank_fetch(has_next, sg_next, rw_next, rp_next, rb_next);
         uint4 sg_after = make_uint4(0u, 0u, 0u, 0u);
         const bool has_after = chunk_segment(c + 2u * waves_total, sg_after);
 
@@ -1044,6 +1065,8 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
         uint64_t g = 0;
         const PrimFrame f = f_cur;
         const PrimVal pv = v_cur;
+        const uint64_t rw = rw_cur;
+        const uint32_t rp = rp_cur, rb = rb_cur;
         if (has_cur)
         {
             g = sg_cur.x;
@@ -1056,6 +1079,9 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
         has_cur = has_next;
         f_cur = f_next;
         v_cur = v_next;
+        rw_cur = rw_next;
+        rp_cur = rp_next;
+        rb_cur = rb_next;
         sg_next = sg_after;
         has_next = has_after;
         static_assert(EMIT_SEG < (1 << 5), "segment lengths fit 5 bits");
@@ -1069,8 +1095,17 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
         uint64_t key = 0ull;
         if (len > 0)
         {
-            const uint64_t vkey = encode_voxel(a.kl, i, j, a.kl.lmin[2] * 8) +
-                                  ((uint64_t)(uint32_t)((k0 >> 3) - a.kl.lmin[2]) << 9) + (uint64_t)(k0 & 7);
+            uint64_t vkey;
+            if (ranked)
+            {
+                // rank of the first leaf; the next leaf along z is the next candidate (the
+                // segment lies in the Gaussian's AABB, whose leaves are all marked)
+                const uint64_t rank = (uint64_t)rp + (uint64_t)__popcll(rw & ((1ull << rb) - 1ull));
+                vkey = (rank << 9) | (encode_voxel(a.kl, i, j, k0) & 511u);
+            }
+            else
+                vkey = encode_voxel(a.kl, i, j, a.kl.lmin[2] * 8) +
+                       ((uint64_t)(uint32_t)((k0 >> 3) - a.kl.lmin[2]) << 9) + (uint64_t)(k0 & 7);
             key = (vkey << kshift) | (g << a.kl.alpha_bits);
         }
         walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float* ea, const float* eb) {
@@ -2050,7 +2085,7 @@ __global__ __launch_bounds__(BP_THREADS) void k_density_bwd_pairs(PairBwdArgs a)
         if (leaf != leaf_cur)
         {
             int32_t i0, j0, k0;
-            decode_voxel(a.kl, leaf << 9, &i0, &j0, &k0);
+            decode_voxel(a.kl, leaf_of_field(a.kl, leaf) << 9, &i0, &j0, &k0);
             li = i0 >> 3;
             lj = j0 >> 3;
             lk = k0 >> 3;
@@ -2496,14 +2531,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba(const uint64_t
         else
         {
             int32_t vi, vj, vk;
-            decode_voxel(kl, key[i] >> gid_bits, &vi, &vj, &vk);
+            decode_pair_voxel(kl, key[i] >> gid_bits, &vi, &vj, &vk);
             float vmin[3], vmax[3];
             voxel_box(frames[gid], vi, vj, vk, vs, vmin, vmax);
             const PrimVal pv = vals[gid];
             rgba = (pv.rgb_cull & 0x00FFFFFFu) | (alpha8_fast(pv, min_sample_dist2(vmin, vmax)) << 24);
         }
         if ((headbits >> i) & 1u)
-            lkey[gl] = key[i] >> lshift;
+            lkey[gl] = leaf_of_field(kl, key[i] >> lshift);
         if (ord < (uint32_t)LR_LEAVES)
         {
             atomicMax(&s_val[ord * 512u + slot], ALPHA_IN_KEY ? a8 << 24 : rgba);
@@ -2607,7 +2642,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_rgba_chunks(const u
     const uint32_t slot_shift = kl.gid_bits + 8u;
     const uint64_t gmask = (1ull << kl.gid_bits) - 1ull;
     if (chunk == 0u && threadIdx.x == 0)
-        lkey[l] = keys[first] >> (slot_shift + 9u);
+        lkey[l] = leaf_of_field(kl, keys[first] >> (slot_shift + 9u));
     // pass 1: maximum alpha and lowest id per slot (LRC_UNROLL keys in flight per thread)
     for (uint64_t q0 = p0 + threadIdx.x; q0 < p1; q0 += (uint64_t)RED_THREADS * LRC_UNROLL)
     {
@@ -2900,7 +2935,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_leaf_reduce_density(const uint6
     const uint64_t p1 = p0 + LRD_CHUNK < pend ? p0 + LRD_CHUNK : pend;
     const uint32_t lshift = kl.gid_bits + 9u;
     const uint64_t gmask = (1ull << kl.gid_bits) - 1ull;
-    const uint64_t leaf_bits = keys[first] >> lshift;
+    const uint64_t leaf_bits = leaf_of_field(kl, keys[first] >> lshift);
     int32_t i0, j0, k0;
     decode_voxel(kl, leaf_bits << 9, &i0, &j0, &k0);
     if (chunk == 0u && threadIdx.x == 0)
@@ -3592,7 +3627,7 @@ __global__ void k_pairs_decode(const uint64_t* __restrict__ keys, uint64_t n, Ke
     if (ijk)
     {
         int32_t i, j, kk;
-        decode_voxel(kl, k >> (kl.gid_bits + kl.alpha_bits), &i, &j, &kk);
+        decode_pair_voxel(kl, k >> (kl.gid_bits + kl.alpha_bits), &i, &j, &kk);
         ijk[3 * x + 0] = i;
         ijk[3 * x + 1] = j;
         ijk[3 * x + 2] = kk;
@@ -3603,6 +3638,66 @@ __global__ void k_pairs_decode(const uint64_t* __restrict__ keys, uint64_t n, Ke
 static inline uint32_t blocks_for(uint64_t n, uint32_t t = 256)
 {
     return (uint32_t)((n + t - 1) / t);
+}
+
+// ---------------------------------------------------------------- candidate-leaf ranks
+// Marks the leaves of every Gaussian's voxel AABB in a bitmap over the leaf index (x, y, z leaf
+// coordinates relative to lmin, z least significant): a leaf column (x, y) of the AABB is one run
+// of consecutive bits. A word already holding the run's bits is not written again.
+__global__ __launch_bounds__(256) void k_leaf_mark(const PrimBox* __restrict__ boxes, uint64_t n, KeyLayout kl,
+                                                   unsigned long long* __restrict__ map)
+{
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const PrimBox b = boxes[g];
+        if (b.rng_x <= 0 || b.rng_y <= 0 || b.rng_z <= 0)
+            continue;
+        const int32_t x0 = (b.mn[0] >> 3) - kl.lmin[0], x1 = ((b.mn[0] + (int32_t)b.rng_x - 1) >> 3) - kl.lmin[0];
+        const int32_t y0 = (b.mn[1] >> 3) - kl.lmin[1], y1 = ((b.mn[1] + (int32_t)b.rng_y - 1) >> 3) - kl.lmin[1];
+        const uint64_t z0 = (uint64_t)((b.mn[2] >> 3) - kl.lmin[2]);
+        const uint64_t z1 = (uint64_t)(((b.mn[2] + (int32_t)b.rng_z - 1) >> 3) - kl.lmin[2]);
+        for (int32_t x = x0; x <= x1; x++)
+            for (int32_t y = y0; y <= y1; y++)
+            {
+                const uint64_t col = ((uint64_t)x << (kl.bits[1] + kl.bits[2])) | ((uint64_t)y << kl.bits[2]);
+                uint64_t lo = col + z0;
+                const uint64_t hi = col + z1; // inclusive
+                while (lo <= hi)
+                {
+                    const uint64_t w = lo >> 6;
+                    const uint32_t b0 = (uint32_t)(lo & 63u);
+                    const uint32_t b1 = (hi >> 6) == w ? (uint32_t)(hi & 63u) : 63u;
+                    const unsigned long long m = (~0ull >> (63u - (b1 - b0))) << b0;
+                    if ((map[w] & m) != m)
+                        atomicOr(&map[w], m);
+                    lo = (w + 1) << 6;
+                }
+            }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_leaf_popc(const unsigned long long* __restrict__ map, uint64_t words,
+                                                   uint32_t* __restrict__ cnt)
+{
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
+        cnt[w] = (uint32_t)__popcll(map[w]);
+}
+
+// rank -> leaf index for every marked leaf
+__global__ __launch_bounds__(256) void k_leaf_unrank(const unsigned long long* __restrict__ map, uint64_t words,
+                                                     const uint32_t* __restrict__ prefix, uint32_t* __restrict__ leaf_of_rank)
+{
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
+    {
+        unsigned long long m = map[w];
+        uint32_t r = prefix[w];
+        while (m)
+        {
+            const uint32_t bit = (uint32_t)__builtin_ctzll(m);
+            leaf_of_rank[r++] = (uint32_t)((w << 6) + bit);
+            m &= m - 1ull;
+        }
+    }
 }
 
 static inline uint32_t bits_for(uint64_t range) // bits to represent values in [0, range)
@@ -4301,6 +4396,54 @@ static int write_voxels(Context* ctx, const LeafReduce& r, const VoxelLists& out
     return 0;
 }
 
+// Candidate-leaf ranks for the pair keys: the leaves of every Gaussian's voxel AABB are marked in
+// a bitmap over the leaf index, and a pair's leaf field becomes the leaf's rank among them (ranks
+// follow the leaf index order, so the sort, the reduce and the voxel lists see the same order).
+// Used when the ranks need fewer bits than the leaf index (at 2M clustered Gaussians 191,382
+// candidates: 18 bits instead of 21, two 9-bit sort passes instead of three) and the bitmap stays
+// within 2^28 bits. On return kl holds the rank layout, or is unchanged.
+static int leaf_ranks(Context* ctx, hipStream_t st, const PrimBox* boxes, uint64_t n, KeyLayout* kl)
+{
+    const uint32_t index_bits = kl->bits[0] + kl->bits[1] + kl->bits[2];
+    if (index_bits < 10u || index_bits > 28u)
+        return 0;
+    const uint64_t words = ((1ull << index_bits) + 63u) / 64u;
+    unsigned long long* map = (unsigned long long*)ctx->ws(WS_LEAF_MAP, words * sizeof(uint64_t));
+    // prefix (words + 1), per-word counts (words), total (1)
+    uint32_t* prefix = (uint32_t*)ctx->ws(WS_LEAF_PREFIX, (2 * words + 2) * sizeof(uint32_t));
+    if (!map || !prefix)
+        return -1;
+    uint32_t* counts = prefix + words + 1;
+    uint32_t* total = counts + words;
+    if (fill_words(ctx, (uint64_t*)map, host_count(words), 0ull))
+        return -1;
+    hipLaunchKernelGGL(k_leaf_mark, dim3((uint32_t)std::min<uint64_t>(blocks_for(n), 4096u)), dim3(256), 0, st, boxes,
+                       n, *kl, map);
+    hipLaunchKernelGGL(k_leaf_popc, dim3((uint32_t)std::min<uint64_t>(blocks_for(words), 4096u)), dim3(256), 0, st,
+                       (const unsigned long long*)map, words, counts);
+    VDBR_CHECK(ctx, hipGetLastError());
+    if (scan_u32(ctx, counts, prefix, words, false, total))
+        return -1;
+    VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, total, 4, hipMemcpyDeviceToHost, st));
+    VDBR_CHECK(ctx, hipStreamSynchronize(st));
+    const uint32_t cand = (uint32_t)ctx->pinned[0];
+    const uint32_t rank_bits = std::max<uint32_t>(1u, bits_for(cand));
+    if (cand == 0u || rank_bits >= index_bits)
+        return 0;
+    uint32_t* of_rank = (uint32_t*)ctx->ws(WS_LEAF_RANK, (uint64_t)cand * sizeof(uint32_t));
+    if (!of_rank)
+        return -1;
+    hipLaunchKernelGGL(k_leaf_unrank, dim3((uint32_t)std::min<uint64_t>(blocks_for(words), 4096u)), dim3(256), 0, st,
+                       (const unsigned long long*)map, words, (const uint32_t*)prefix, of_rank);
+    VDBR_CHECK(ctx, hipGetLastError());
+    kl->rank_bits = rank_bits;
+    kl->leaf_map = (const uint64_t*)map;
+    kl->leaf_prefix = prefix;
+    kl->leaf_of_rank = of_rank;
+    kl->voxel_bits = rank_bits + 9u;
+    return 0;
+}
+
 int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_params_t* prm, void** grid_dev,
                  uint64_t* grid_size_out, vdbr_raster_stats_t* stats)
 {
@@ -4435,6 +4578,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
     }
     kl.gid_bits = bits_for(n > 1 ? n : 2);
     kl.voxel_bits = kl.bits[0] + kl.bits[1] + kl.bits[2] + 9u;
+    if (any && leaf_ranks(ctx, st, (const PrimBox*)boxes, n, &kl))
+        return -1;
     if (kl.voxel_bits + kl.gid_bits > 64)
         return ctx->fail("raster_node2: pair keys need more than 64 bits");
     // Only the voxel bits are sorted: RGBA8 (the pair alpha rides in the key) keeps the max and
@@ -4674,8 +4819,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 cap = pair_limit + 1;
             // the emission counts the sort's digit histograms (for the digit width of the
             // estimated pair count; the sort recounts when the final count picks another width)
-            hist_w = sort_digit_bits(ctx, sest > 1 ? sest : 2);
             const uint32_t hbits = key_bits - sort_begin;
+            hist_w = sort_digit_bits(ctx, sest > 1 ? sest : 2, hbits);
             hist_buf = emit_counts_hist(hbits, hist_w) ? sort_hist_buffer(ctx) : nullptr;
             uint32_t sattempts = 0;
             for (;;)
@@ -4757,7 +4902,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         uint32_t passes = 0;
         if (spairs > 1)
         {
-            const bool hist_ready = hist_buf && sort_digit_bits(ctx, spairs) == hist_w;
+            const bool hist_ready = hist_buf && sort_digit_bits(ctx, spairs, key_bits - sort_begin) == hist_w;
             if (radix_sort_u64_noswap(ctx, pairs, host_count(spairs), sort_begin, key_bits, &sorted, ev[6], ev[7], &passes,
                                       hist_ready))
                 return -1;
@@ -4837,13 +4982,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         ctx->last_pair_n = n;
         ctx->last_pair_vs = vs;
     }
-    ctx->last_gid_bits = kl.gid_bits;
-    ctx->last_alpha_bits = kl.alpha_bits;
-    for (int a = 0; a < 3; a++)
-    {
-        ctx->last_lmin[a] = kl.lmin[a];
-        ctx->last_bits[a] = kl.bits[a];
-    }
+    ctx->last_kl = kl;
     VDBR_CHECK(ctx, hipEventRecord(ev[4], st));
     Node2BuildInput bin;
     bin.kl = kl;
@@ -4930,15 +5069,7 @@ int last_pairs(Context* ctx, int32_t* ijk_dev, uint32_t* gid_dev, uint64_t cap, 
     uint64_t n = ctx->last_pair_count < cap ? ctx->last_pair_count : cap;
     if (n == 0)
         return 0;
-    KeyLayout kl;
-    memset(&kl, 0, sizeof(kl));
-    for (int a = 0; a < 3; a++)
-    {
-        kl.lmin[a] = ctx->last_lmin[a];
-        kl.bits[a] = ctx->last_bits[a];
-    }
-    kl.gid_bits = ctx->last_gid_bits;
-    kl.alpha_bits = ctx->last_alpha_bits;
+    const KeyLayout kl = ctx->last_kl;
     hipLaunchKernelGGL(k_pairs_decode, dim3(blocks_for(n)), dim3(256), 0, ctx->stream, ctx->last_pairs, n, kl, ijk_dev,
                        gid_dev);
     VDBR_CHECK(ctx, hipGetLastError());
@@ -5204,7 +5335,7 @@ int density_backward_pairs(Context* ctx, const vdbr_gaussians_t* raw, float vs, 
 {
     *used_pairs = 0;
     const bool have = pair_list_id != 0 && pair_list_id == ctx->last_pair_id && raw && raw->count == ctx->last_pair_n &&
-                      vs == ctx->last_pair_vs && ctx->last_alpha_bits == 0u;
+                      vs == ctx->last_pair_vs && ctx->last_kl.alpha_bits == 0u;
     if (!have)
         return density_backward(ctx, raw, vs, grid, grid_size, grad_values, value_count, grads, ms_out);
     bool empty = false;
@@ -5270,14 +5401,7 @@ int density_backward_pairs(Context* ctx, const vdbr_gaussians_t* raw, float vs, 
     memset(&pa, 0, sizeof(pa));
     pa.keys = ctx->last_pairs;
     pa.npairs = ctx->last_pair_count;
-    for (int c = 0; c < 3; c++)
-    {
-        pa.kl.lmin[c] = ctx->last_lmin[c];
-        pa.kl.bits[c] = ctx->last_bits[c];
-    }
-    pa.kl.gid_bits = ctx->last_gid_bits;
-    pa.kl.alpha_bits = 0u;
-    pa.kl.voxel_bits = pa.kl.bits[0] + pa.kl.bits[1] + pa.kl.bits[2] + 9u;
+    pa.kl = ctx->last_kl; // density pair keys: no alpha bits
     pa.vs = (double)vs;
     pa.prims = prims;
     pa.lt_slots = lt.slots;
@@ -5482,8 +5606,8 @@ static int capture_cloud(Context* w, const CloudPlan& p, const uint32_t* win_bas
                        seg_counters);
     hipLaunchKernelGGL(k_seg_plan, dim3(1), dim3(64), 0, st, (const unsigned long long*)seg_counters, p.part_cap,
                        part_count_dev, chunk_scan_dev, flags);
-    const uint32_t hist_w = sort_digit_bits(w, p.pair_cap);
     const uint32_t hbits = p.key_bits - p.sort_begin;
+    const uint32_t hist_w = sort_digit_bits(w, p.pair_cap, hbits);
     unsigned long long* hist_buf =
         emit_counts_hist(hbits, hist_w) ? sort_hist_buffer(w) : nullptr;
     if (fill_words(w, words, host_count(2), 0ull))
