@@ -421,7 +421,9 @@ def main():
                               "ops_per_pair": ALPHA_OPS_PER_PAIR if args.value_mode == "rgba8" else 0}},
         "sort_roofline": {"bound": "hbm", "kernel": "k_sort_pass (onesweep radix pass over 64-bit pair keys)",
                           "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                          "traffic": traffic, "avg_launch_ms": pass_ms, "algorithmic_bytes_per_launch": pass_bytes},
+                          "traffic": traffic, "avg_launch_ms": pass_ms, "algorithmic_bytes_per_launch": pass_bytes,
+                          "passes": passes, "sorted_bits": st["key_bits"] - st["sort_bit_begin"],
+                          "all_passes_ms": avg("ms_sort_passes")},
         "sort_autotune": tuned,
         "pairs_per_s": P / (ms_per_step * 1e-3),
         "stage_ms": {k: avg(k) for k in ("ms_setup", "ms_emit", "ms_sort", "ms_reduce", "ms_build", "ms_total",
