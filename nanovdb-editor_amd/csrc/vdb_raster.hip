@@ -395,6 +395,34 @@ __device__ __forceinline__ uint64_t find_prim(const uint64_t* __restrict__ row_s
     return lo;
 }
 
+// find_prim by the whole wave (target wave-uniform): every round probes 64 evenly spaced entries of
+// [lo, hi] and keeps the gap where row_scan passes the target, so 2M Gaussians take 4 rounds of
+// one load per lane instead of 21 dependent loads
+__device__ __forceinline__ uint64_t wave_find_prim(const uint64_t* __restrict__ row_scan, uint64_t n, uint64_t row,
+                                                   uint32_t lane)
+{
+    uint64_t lo = 0, hi = n - 1; // row_scan[lo] <= row; the answer lies in [lo, hi]
+    while (hi > lo)
+    {
+        const uint64_t span = hi - lo;
+        auto probe = [&](uint64_t l) { return lo + (span * (l + 1u) + 63u) / 64u; }; // in (lo, hi]
+        const bool le = row_scan[probe(lane)] <= row;
+        // the probes are non-decreasing in the lane and row_scan is non-decreasing: a prefix holds
+        const uint32_t c = (uint32_t)__popcll(__ballot(le));
+        if (c == 0u)
+            hi = probe(0) - 1u;
+        else if (c == 64u)
+            lo = hi;
+        else
+        {
+            const uint64_t nlo = probe(c - 1u), nhi = probe(c) - 1u;
+            lo = nlo;
+            hi = nhi;
+        }
+    }
+    return lo;
+}
+
 // orders a wave's LDS accesses before and after it (lanes exchange data through LDS)
 __device__ __forceinline__ void wave_sync_lds()
 {
@@ -881,18 +909,59 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
     unsigned long long* seg_counter = seg_counters + part * SEG_COUNTER_STRIDE;
     uint4* part_segs = segs + (uint64_t)part * part_cap;
     __shared__ uint64_t g_range[2];
+    // Gaussian of each of the block's rows, relative to the first one: the inclusive prefix count
+    // of the Gaussians in (glo, ghi] starting at or before the row
+    __shared__ uint32_t s_owner[EMIT_ROWS];
+    __shared__ uint32_t s_wtot[EMIT_THREADS / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint64_t row0 = (uint64_t)blockIdx.x * EMIT_ROWS;
-    if (threadIdx.x == 0)
+    if (wave < 2u)
     {
-        g_range[0] = find_prim(a.row_scan, 0, a.n - 1, row0);
-        uint64_t last = row0 + EMIT_ROWS - 1;
-        if (last >= a.total_rows)
-            last = a.total_rows - 1;
-        g_range[1] = find_prim(a.row_scan, 0, a.n - 1, last);
+        uint64_t target = row0;
+        if (wave == 1u)
+        {
+            target = row0 + EMIT_ROWS - 1;
+            if (target >= a.total_rows)
+                target = a.total_rows - 1;
+        }
+        const uint64_t g = wave_find_prim(a.row_scan, a.n, target, lane);
+        if (lane == 0u)
+            g_range[wave] = g;
     }
+    for (uint32_t x = threadIdx.x; x < (uint32_t)EMIT_ROWS; x += EMIT_THREADS)
+        s_owner[x] = 0u;
     __syncthreads();
     const uint64_t glo = g_range[0], ghi = g_range[1];
+    // row_scan[glo] <= row0 < row_scan[g] <= last row for g in (glo, ghi]
+    for (uint64_t g = glo + 1u + threadIdx.x; g <= ghi; g += EMIT_THREADS)
+        atomicAdd(&s_owner[(uint32_t)(a.row_scan[g] - row0)], 1u);
+    __syncthreads();
+    {
+        static_assert(EMIT_ROWS == 4 * EMIT_THREADS, "four rows per thread in the owner scan");
+        const uint32_t t4 = threadIdx.x * 4u;
+        const uint32_t v0 = s_owner[t4], v1 = s_owner[t4 + 1u], v2 = s_owner[t4 + 2u], v3 = s_owner[t4 + 3u];
+        const uint32_t sum = v0 + v1 + v2 + v3;
+        uint32_t x = sum;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
+            if (lane >= (uint32_t)off)
+                x += y;
+        }
+        if (lane == 63u)
+            s_wtot[wave] = x;
+        __syncthreads();
+        uint32_t pre = 0u;
+        for (uint32_t w = 0; w < wave; w++)
+            pre += s_wtot[w];
+        const uint32_t e = pre + x - sum;
+        s_owner[t4] = e + v0;
+        s_owner[t4 + 1u] = e + v0 + v1;
+        s_owner[t4 + 2u] = e + v0 + v1 + v2;
+        s_owner[t4 + 3u] = e + sum;
+    }
+    __syncthreads();
     for (uint32_t chunk = wave; chunk < EMIT_ROWS / 64u; chunk += EMIT_THREADS / 64u)
     {
         const uint64_t row = row0 + (uint64_t)chunk * 64u + lane;
@@ -902,7 +971,7 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
         int32_t i = 0, j = 0, k0 = 0, len = 0;
         if (row < a.total_rows)
         {
-            g = find_prim(a.row_scan, glo, ghi, row);
+            g = glo + s_owner[(uint32_t)(row - row0)];
             const PrimBox b = a.boxes[g];
             const uint64_t lr = row - a.row_scan[g];
             i = max(b.mn[0], a.slab_i0) + (int32_t)(lr / (uint64_t)b.rng_y);
@@ -924,36 +993,16 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
             atomicAdd(seg_counter + 1, (unsigned long long)cand_total);
         }
         base = __shfl(base, 0, 64);
-        // the wave writes its segments with consecutive lanes: segment s belongs to the last lane
-        // whose exclusive count is <= s
-        for (uint32_t s0 = 0; s0 < total; s0 += 64u)
+        // each lane writes its own segments at base + excl + q: in the first round the lanes with a
+        // segment write neighbouring records, later rounds only the lanes of long rows
+        for (uint32_t q = 0; q < nseg; q++)
         {
-            const uint32_t sidx = s0 + lane;
-            uint32_t lo = 0, hi = 63;
-#pragma unroll
-            for (int step = 0; step < 6; step++)
+            const uint64_t pos = base + excl + q;
+            if (pos < part_cap)
             {
-                const uint32_t mid = (lo + hi + 1u) >> 1;
-                const uint32_t ex_mid = (uint32_t)__shfl((int)excl, (int)mid, 64);
-                if (ex_mid <= sidx)
-                    lo = mid;
-                else
-                    hi = mid - 1u;
-            }
-            const uint32_t owner = lo;
-            const uint32_t o_excl = (uint32_t)__shfl((int)excl, (int)owner, 64);
-            const uint32_t o_g = (uint32_t)__shfl((int)(uint32_t)g, (int)owner, 64);
-            const int32_t o_i = __shfl(i, (int)owner, 64);
-            const int32_t o_j = __shfl(j, (int)owner, 64);
-            const int32_t o_k0 = __shfl(k0, (int)owner, 64);
-            const int32_t o_len = __shfl(len, (int)owner, 64);
-            const int32_t q = (int32_t)(sidx - o_excl);
-            const uint64_t pos = base + sidx;
-            if (sidx < total && pos < part_cap)
-            {
-                const int32_t kq = o_k0 + q * EMIT_SEG;
-                const int32_t cnt = min(EMIT_SEG, o_len - q * EMIT_SEG);
-                part_segs[pos] = make_uint4(o_g, (uint32_t)o_i, (uint32_t)o_j,
+                const int32_t kq = k0 + (int32_t)q * EMIT_SEG;
+                const int32_t cnt = min(EMIT_SEG, len - (int32_t)q * EMIT_SEG);
+                part_segs[pos] = make_uint4((uint32_t)g, (uint32_t)i, (uint32_t)j,
                                             ((uint32_t)(kq + 131072) & 0xFFFFFu) | ((uint32_t)cnt << 20));
             }
         }
