@@ -253,9 +253,11 @@ int tile_bin(Context* ctx, const float* means2d, const int32_t* radii, const flo
 int autotune_sort(Context* ctx, uint64_t n, uint32_t key_bits, vdbr_tune_result_t* results, uint32_t cap,
                   uint32_t* count, uint32_t* best_variant, uint32_t* best_digit_bits);
 int set_sort_config(Context* ctx, uint64_t from_count, uint32_t variant, uint32_t digit_bits);
+// free_order: keys equal in every sorted bit may end in any order (the first pass then ranks
+// with per-lane LDS atomics instead of stable peer ranking)
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount n, uint32_t bit_begin, uint32_t bit_end,
                           uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out,
-                          bool hist_ready = false);
+                          bool hist_ready = false, bool free_order = false);
 // allocation-free sort (vdb_prims.hip): the stable LSD sort of radix_sort_u64 over a file-scope
 // device scratch array of the library (at most noalloc_max_keys() keys); queued on st on device
 // dev without synchronizing; calls that overlap on one device race on the shared scratch

@@ -437,7 +437,9 @@ __global__ __launch_bounds__(SORT_BINS) void k_sort_digit_starts(const unsigned 
 
 // keys only, tiles of up to 8192 keys: at most 128 VGPRs (4 waves per SIMD), so that two
 // 512-thread tiles share a CU; the other shapes are held to one or two tiles per CU by their LDS
-template <int SORT_THREADS, int SORT_ITEMS, bool HAS_VALS>
+// STABLE = false: keys of one digit may leave the tile in any order (each lane takes its rank from
+// an LDS atomic; no peer masks)
+template <int SORT_THREADS, int SORT_ITEMS, bool HAS_VALS, bool STABLE = true>
 __global__ __launch_bounds__(SORT_THREADS) __attribute__((
     amdgpu_waves_per_eu(!HAS_VALS && SORT_ITEMS <= 16 && SORT_THREADS * SORT_ITEMS <= 8192 ? 4 : 1))) void k_sort_pass(const uint64_t* __restrict__ kin, uint64_t* __restrict__ kout,
                                                             const uint32_t* __restrict__ vin, uint32_t* __restrict__ vout,
@@ -496,8 +498,20 @@ __global__ __launch_bounds__(SORT_THREADS) __attribute__((
     // broadcast to their peers.
     constexpr int G = 4;
     static_assert(SORT_ITEMS % G == 0, "whole groups");
+    if (!STABLE)
+    {
+        // every lane adds itself to its digit's count; the returned counts are distinct
 #pragma unroll
-    for (int m0 = 0; m0 < SORT_ITEMS; m0 += G)
+        for (int m = 0; m < SORT_ITEMS; m++)
+        {
+            const uint64_t idx = base + (uint64_t)m * 64u + lane;
+            const uint32_t d = (uint32_t)(keys[m] >> shift) & dmask;
+            const uint32_t hs = (d & 1u) << 4;
+            rank[m] = idx < n ? (atomicAdd(&wcnt[wave][d >> 1], 1u << hs) >> hs) & 0xFFFFu : 0u;
+        }
+    }
+#pragma unroll
+    for (int m0 = 0; m0 < SORT_ITEMS && STABLE; m0 += G)
     {
         uint32_t lead[G]; // old count (at the leader, < 2^26) and the leader's lane
 #pragma unroll
@@ -717,8 +731,15 @@ static uint64_t sort_tile(uint32_t variant)
 template <bool V>
 static void launch_sort_pass(uint32_t variant, uint64_t tiles, hipStream_t st, const uint64_t* kin, uint64_t* kout,
                              const uint32_t* vin, uint32_t* vout, DCount n, uint32_t shift, uint32_t dbits,
-                             const uint64_t* starts, uint64_t* status, uint32_t* counter, uint32_t epoch)
+                             const uint64_t* starts, uint64_t* status, uint32_t* counter, uint32_t epoch,
+                             bool stable = true)
 {
+    if (!stable && !V && variant == 0u)
+    {
+        hipLaunchKernelGGL((k_sort_pass<512, 16, false, false>), dim3((uint32_t)tiles), dim3(512), 0, st, kin, kout,
+                           nullptr, nullptr, n, shift, dbits, starts, status, counter, epoch);
+        return;
+    }
     switch (variant)
     {
     case 1:
@@ -938,7 +959,7 @@ unsigned long long* sort_hist_buffer(Context* ctx)
 
 int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_begin, uint32_t bit_end,
                           uint64_t** result_keys, hipEvent_t passes_begin, hipEvent_t passes_end, uint32_t* passes_out,
-                          bool hist_ready)
+                          bool hist_ready, bool free_order)
 {
     *result_keys = keys;
     if (passes_out)
@@ -981,8 +1002,11 @@ int radix_sort_u64_noswap(Context* ctx, uint64_t* keys, DCount cn, uint32_t bit_
         uint32_t shift = bit_begin + W * p;
         uint32_t dbits = (bits - W * p) < W ? (bits - W * p) : W;
         uint32_t epoch = ctx->next_sort_epoch(tiles);
+        // LSD: only the first pass may scramble keys of one digit (the later stable passes keep
+        // its order only where every sorted bit is equal)
         launch_sort_pass<false>(cfg.variant, tiles, ctx->stream, kin, kout, nullptr, nullptr, cn, shift, dbits,
-                                ws.starts + p * SORT_BINS, ws.status, ws.tile_counters + p, epoch);
+                                ws.starts + p * SORT_BINS, ws.status, ws.tile_counters + p, epoch,
+                                !(free_order && p == 0u));
         uint64_t* tk = kin;
         kin = kout;
         kout = tk;

@@ -4952,8 +4952,9 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         if (spairs > 1)
         {
             const bool hist_ready = hist_buf && sort_digit_bits(ctx, spairs, key_bits - sort_begin) == hist_w;
+            // the reduces do not depend on the order of the pairs inside a leaf
             if (radix_sort_u64_noswap(ctx, pairs, host_count(spairs), sort_begin, key_bits, &sorted, ev[6], ev[7], &passes,
-                                      hist_ready))
+                                      hist_ready, true))
                 return -1;
             sort_passes = passes;
         }
@@ -5686,7 +5687,7 @@ static int capture_cloud(Context* w, const CloudPlan& p, const uint32_t* win_bas
     uint64_t* sorted = pairs;
     uint32_t passes = 0;
     if (radix_sort_u64_noswap(w, pairs, pc, p.sort_begin, p.key_bits, &sorted, nullptr, nullptr, &passes,
-                              hist_buf != nullptr))
+                              hist_buf != nullptr, true))
         return -1;
     // ---- reduce by leaf, voxel lists
     LeafReduce lr;
