@@ -464,6 +464,7 @@ typedef struct vdbr_gaussian_grads_t
 #define VDBR_FLAG_RAW_INPUTS 1u
 #define VDBR_FLAG_SPATIAL_ORDER 2u /* nodes in BFS tree order instead of the windowed order */
 #define VDBR_FLAG_VOXEL_COORDS 4u  /* also record every value slot's voxel (vdbr_last_voxels) */
+#define VDBR_FLAG_WIDE_SEGMENTS 8u /* 16-byte candidate-segment records even where 8-byte ones fit */
 
 typedef struct vdbr_raster_params_t
 {

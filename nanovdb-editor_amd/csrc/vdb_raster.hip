@@ -299,7 +299,8 @@ struct EmitArgs
     uint64_t* out;
     uint64_t cap;
     unsigned long long* counter;
-    const uint4* segs; // candidate segments (k_row_segments), in SEG_PARTS partitions
+    const void* segs;  // candidate segments (k_row_segments), in SEG_PARTS partitions
+    uint32_t seg_compact; // 8-byte segment records (SegCompact) instead of 16-byte ones (SegWide)
     uint64_t seg_part_cap;          // capacity of one partition
     const uint64_t* seg_part_count; // segments written per partition
     const uint64_t* seg_chunk_scan; // SEG_PARTS + 1 exclusive scan of 64-segment chunks per partition
@@ -376,10 +377,64 @@ constexpr uint32_t SEG_PARTS = 64;
 constexpr uint32_t SEG_COUNTER_STRIDE = 16; // u64 words between counters (128 bytes)
 
 // A candidate segment: up to EMIT_SEG consecutive voxels (i, j, k .. k + count - 1) of one
-// Gaussian's pruned row interval. x = gaussian id, y = i, z = j, w = (k + 2^17) | count << 20.
+// Gaussian's pruned row interval.
 // (measured at 2M clustered: 6 -> 36.7 ms emission, 8 -> 35.1, 10 -> 33.9, 14 -> 33.5, 16 -> 33.3,
 // 20 -> 33.1; longer caps let a long row hold its wave)
 constexpr int32_t EMIT_SEG = 16;
+
+// Segment records. SegCompact, when the voxel key of the segment's first voxel (leaf index and
+// slot), the Gaussian id and the count fit 64 bits: voxel key << (gid_bits + 4) | gaussian id << 4
+// | (count - 1). SegWide otherwise: x = gaussian id, y = i, z = j, w = (k + 2^17) | count << 20.
+// The walk keeps records in flight as they are; gid() and leaf() are a few bit operations.
+struct SegCompact
+{
+    using Rec = uint64_t;
+    static __device__ __forceinline__ Rec make(const KeyLayout& kl, uint64_t g, int32_t i, int32_t j, int32_t k,
+                                               int32_t cnt)
+    {
+        return (encode_voxel(kl, i, j, k) << (kl.gid_bits + 4u)) | (g << 4) | (uint64_t)(cnt - 1);
+    }
+    static __device__ __forceinline__ uint64_t gid(const KeyLayout& kl, Rec r)
+    {
+        return (r >> 4) & ((1ull << kl.gid_bits) - 1ull);
+    }
+    static __device__ __forceinline__ uint64_t vkey(const KeyLayout& kl, Rec r)
+    {
+        return r >> (kl.gid_bits + 4u);
+    }
+    static __device__ __forceinline__ void decode(const KeyLayout& kl, Rec r, int32_t* i, int32_t* j, int32_t* k,
+                                                  int32_t* len)
+    {
+        decode_voxel(kl, vkey(kl, r), i, j, k);
+        *len = (int32_t)(r & 15u) + 1;
+    }
+};
+struct SegWide
+{
+    using Rec = uint4;
+    static __device__ __forceinline__ Rec make(const KeyLayout&, uint64_t g, int32_t i, int32_t j, int32_t k,
+                                               int32_t cnt)
+    {
+        return make_uint4((uint32_t)g, (uint32_t)i, (uint32_t)j, ((uint32_t)(k + 131072) & 0xFFFFFu) | ((uint32_t)cnt << 20));
+    }
+    static __device__ __forceinline__ uint64_t gid(const KeyLayout&, Rec r)
+    {
+        return r.x;
+    }
+    static __device__ __forceinline__ uint64_t vkey(const KeyLayout& kl, Rec r)
+    {
+        return encode_voxel(kl, (int32_t)r.y, (int32_t)r.z, (int32_t)(r.w & 0xFFFFFu) - 131072);
+    }
+    static __device__ __forceinline__ void decode(const KeyLayout&, Rec r, int32_t* i, int32_t* j, int32_t* k,
+                                                  int32_t* len)
+    {
+        *i = (int32_t)r.y;
+        *j = (int32_t)r.z;
+        *k = (int32_t)(r.w & 0xFFFFFu) - 131072;
+        *len = (int32_t)(r.w >> 20);
+    }
+};
+static_assert(EMIT_SEG <= 16, "SegCompact holds count - 1 in 4 bits");
 
 __device__ __forceinline__ uint64_t find_prim(const uint64_t* __restrict__ row_scan, uint64_t lo, uint64_t hi, uint64_t row)
 {
@@ -904,12 +959,13 @@ __device__ __forceinline__ uint64_t read_lane_u64(uint64_t v, uint32_t src)
 // interval and writes it as segments of at most EMIT_SEG voxels (wave-compacted, one atomic
 // reservation per wave). The emission pass then walks segments, so the lanes of a wave walk at
 // most EMIT_SEG steps each instead of the longest row of their 64.
-__global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4* __restrict__ segs, uint64_t part_cap,
+__global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, void* __restrict__ segs, uint64_t part_cap,
                                                                unsigned long long* __restrict__ seg_counters)
 {
     const uint32_t part = blockIdx.x % SEG_PARTS;
     unsigned long long* seg_counter = seg_counters + part * SEG_COUNTER_STRIDE;
-    uint4* part_segs = segs + (uint64_t)part * part_cap;
+    SegCompact::Rec* part_compact = (SegCompact::Rec*)segs + (uint64_t)part * part_cap;
+    SegWide::Rec* part_wide = (SegWide::Rec*)segs + (uint64_t)part * part_cap;
     __shared__ uint64_t g_range[2];
     // Gaussian of each of the block's rows, relative to the first one: the inclusive prefix count
     // of the Gaussians in (glo, ghi] starting at or before the row
@@ -996,16 +1052,22 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
         }
         base = __shfl(base, 0, 64);
         // each lane writes its own segments at base + excl + q: in the first round the lanes with a
-        // segment write neighbouring records, later rounds only the lanes of long rows
+        // segment write neighbouring records, later rounds only the lanes of long rows. A compact
+        // record is the first segment's (count 1) plus q times two z leaves and the count - 1: the
+        // segments of a row start 16 voxels apart, in the same slot of the leaf two further along z
+        static_assert(EMIT_SEG == 16, "two z leaves per segment");
+        const uint64_t rec0 = a.seg_compact && nseg ? SegCompact::make(a.kl, g, i, j, k0, 1) : 0ull;
+        const uint32_t leaf_shift = a.kl.gid_bits + 4u + 9u;
         for (uint32_t q = 0; q < nseg; q++)
         {
             const uint64_t pos = base + excl + q;
             if (pos < part_cap)
             {
-                const int32_t kq = k0 + (int32_t)q * EMIT_SEG;
                 const int32_t cnt = min(EMIT_SEG, len - (int32_t)q * EMIT_SEG);
-                part_segs[pos] = make_uint4((uint32_t)g, (uint32_t)i, (uint32_t)j,
-                                            ((uint32_t)(kq + 131072) & 0xFFFFFu) | ((uint32_t)cnt << 20));
+                if (a.seg_compact)
+                    part_compact[pos] = rec0 + ((uint64_t)(2u * q) << leaf_shift) + (uint64_t)(cnt - 1);
+                else
+                    part_wide[pos] = SegWide::make(a.kl, g, i, j, k0 + (int32_t)q * EMIT_SEG, cnt);
             }
         }
     }
@@ -1018,8 +1080,11 @@ __global__ __launch_bounds__(EMIT_THREADS) void k_row_segments(EmitArgs a, uint4
 // on the single pair counter, and those same-address atomics are served at a fixed rate (about
 // 78 M/s measured), so the stage is as large as the LDS allows: at 1.67 G pairs, 896-survivor
 // flushes bound the kernel at ~25 ms, 1728-survivor flushes at ~12 ms.
+template <typename SEG>
 __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_emit_segments(EmitArgs a)
 {
+    using Rec = typename SEG::Rec;
+    const Rec* __restrict__ seg_recs = (const Rec*)a.segs;
     __shared__ uint64_t stage_all[EMIT_THREADS / 64][EMIT_WAVE_STAGE];
     __shared__ uint32_t ring_all[EMIT_THREADS / 64][EMIT_RING_WORDS][EMIT_RING];
     __shared__ uint32_t s_hist[EMIT_HIST_WORDS];
@@ -1053,7 +1118,7 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
     // for the prefetches issued after them (vmcnt: vector memory counter 0, others unchanged)
     __builtin_amdgcn_s_waitcnt(0x0F70);
     // this lane's segment of chunk c (false past the chunk's partition count or the last chunk)
-    auto chunk_segment = [&](uint64_t c, uint4& sg) -> bool {
+    auto chunk_segment = [&](uint64_t c, Rec& sg) -> bool {
         if (c >= chunks_total)
             return false;
         const uint32_t part = (uint32_t)__popcll(__ballot(p_end <= c));
@@ -1062,54 +1127,55 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
         const uint64_t local = (c - first) * 64u + lane;
         if (local >= cnt)
             return false;
-        sg = a.segs[(uint64_t)part * a.seg_part_cap + local];
+        sg = seg_recs[(uint64_t)part * a.seg_part_cap + local];
         return true;
     };
     // software pipeline over this wave's chunks c, c + W, c + 2W, ...: while chunk c is walked,
     // the frames of chunk c + W and the segments of chunk c + 2W are in flight
     const uint64_t c_first =
         (uint64_t)blockIdx.x * (EMIT_THREADS / 64) + (uint64_t)__builtin_amdgcn_readfirstlane(wave);
-    uint4 sg_cur = make_uint4(0u, 0u, 0u, 0u), sg_next = make_uint4(0u, 0u, 0u, 0u);
+    Rec sg_cur = {}, sg_next = {};
     bool has_cur = chunk_segment(c_first, sg_cur);
     bool has_next = chunk_segment(c_first + waves_total, sg_next);
     // with candidate-leaf ranks, the bitmap word and rank prefix of the segment's first leaf
     const bool ranked = a.kl.rank_bits != 0u;
-    auto rank_fetch = [&](bool has, const uint4& sg, uint64_t& word, uint32_t& pre, uint32_t& bit) {
+    auto rank_fetch = [&](bool has, const Rec& sg, uint64_t& word, uint32_t& pre, uint32_t& bit) {
         word = 0ull;
         pre = 0u;
         bit = 0u;
         if (ranked && has)
         {
-            const int32_t i = (int32_t)sg.y, j = (int32_t)sg.z, k = (int32_t)(sg.w & 0xFFFFFu) - 131072;
-            const uint64_t leaf = encode_voxel(a.kl, i, j, k) >> 9;
+            const uint64_t leaf = SEG::vkey(a.kl, sg) >> 9;
             word = a.kl.leaf_map[leaf >> 6];
             pre = a.kl.leaf_prefix[leaf >> 6];
             bit = (uint32_t)(leaf & 63u);
         }
     };
-    // lanes without a segment load Gaussian 0's frame (sg.x = 0; there are Gaussians whenever
-    // there are chunks) and walk no step
+    // lanes without a segment load Gaussian 0's frame (a zero record; there are Gaussians
+    // whenever there are chunks) and walk no step
     PrimFrame f_cur = {};
     PrimVal v_cur = {};
     uint64_t rw_cur = 0ull;
     uint32_t rp_cur = 0u, rb_cur = 0u;
     if (c_first < chunks_total)
     {
-        f_cur = a.frames[sg_cur.x];
+        const uint64_t g0 = SEG::gid(a.kl, sg_cur);
+        f_cur = a.frames[g0];
         if (with_alpha)
-            v_cur = a.vals[sg_cur.x];
+            v_cur = a.vals[g0];
         rank_fetch(has_cur, sg_cur, rw_cur, rp_cur, rb_cur);
     }
     for (uint64_t c = c_first; c < chunks_total; c += waves_total)
     {
-        const PrimFrame f_next = a.frames[sg_next.x];
+        const uint64_t g_next = SEG::gid(a.kl, sg_next);
+        const PrimFrame f_next = a.frames[g_next];
         PrimVal v_next = {};
         if (with_alpha)
-            v_next = a.vals[sg_next.x];
+            v_next = a.vals[g_next];
         uint64_t rw_next;
         uint32_t rp_next, rb_next;
         rank_fetch(has_next, sg_next, rw_next, rp_next, rb_next);
-        uint4 sg_after = make_uint4(0u, 0u, 0u, 0u);
+        Rec sg_after = {};
         const bool has_after = chunk_segment(c + 2u * waves_total, sg_after);
 
         int32_t len = 0, i = 0, j = 0, k0 = 0;
@@ -1118,13 +1184,12 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
         const PrimVal pv = v_cur;
         const uint64_t rw = rw_cur;
         const uint32_t rp = rp_cur, rb = rb_cur;
+        uint64_t vkey0 = 0ull; // voxel key of the first voxel (leaf index and slot)
         if (has_cur)
         {
-            g = sg_cur.x;
-            i = (int32_t)sg_cur.y;
-            j = (int32_t)sg_cur.z;
-            k0 = (int32_t)(sg_cur.w & 0xFFFFFu) - 131072;
-            len = (int32_t)(sg_cur.w >> 20);
+            g = SEG::gid(a.kl, sg_cur);
+            SEG::decode(a.kl, sg_cur, &i, &j, &k0, &len);
+            vkey0 = SEG::vkey(a.kl, sg_cur);
         }
         sg_cur = sg_next;
         has_cur = has_next;
@@ -1152,11 +1217,10 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
                 // rank of the first leaf; the next leaf along z is the next candidate (the
                 // segment lies in the Gaussian's AABB, whose leaves are all marked)
                 const uint64_t rank = (uint64_t)rp + (uint64_t)__popcll(rw & ((1ull << rb) - 1ull));
-                vkey = (rank << 9) | (encode_voxel(a.kl, i, j, k0) & 511u);
+                vkey = (rank << 9) | (vkey0 & 511u);
             }
             else
-                vkey = encode_voxel(a.kl, i, j, a.kl.lmin[2] * 8) +
-                       ((uint64_t)(uint32_t)((k0 >> 3) - a.kl.lmin[2]) << 9) + (uint64_t)(k0 & 7);
+                vkey = vkey0;
             key = (vkey << kshift) | (g << a.kl.alpha_bits);
         }
         walk_row(f, i, j, k0, len, steps, vs, [&](int32_t st, bool keep, const float* ea, const float* eb) {
@@ -1223,12 +1287,26 @@ __global__ __launch_bounds__(EMIT_THREADS) __attribute__((amdgpu_waves_per_eu(2)
 uint32_t emit_blocks_per_cu()
 {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_emit_segments, EMIT_THREADS, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_emit_segments<SegCompact>, EMIT_THREADS, 0) != hipSuccess)
     {
         (void)hipGetLastError();
         return 0u;
     }
     return blocks > 0 ? (uint32_t)blocks : 0u;
+}
+
+// whether the segments of a grid with this key layout fit SegCompact records
+static bool seg_compact_fits(const KeyLayout& kl)
+{
+    return kl.bits[0] + kl.bits[1] + kl.bits[2] + 9u + kl.gid_bits + 4u <= 64u;
+}
+
+static void launch_emit_segments(const EmitArgs& ea, uint32_t blocks, hipStream_t st)
+{
+    if (ea.seg_compact)
+        hipLaunchKernelGGL(k_emit_segments<SegCompact>, dim3(blocks), dim3(EMIT_THREADS), 0, st, ea);
+    else
+        hipLaunchKernelGGL(k_emit_segments<SegWide>, dim3(blocks), dim3(EMIT_THREADS), 0, st, ea);
 }
 
 // 2^(j / 64), j = 0..63, as the nearest doubles
@@ -4771,7 +4849,9 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
         uint64_t* pairs = nullptr;
         unsigned long long* hist_buf = nullptr;
         uint32_t hist_w = 8;
-        uint4* segs = nullptr;
+        void* segs = nullptr;
+        const bool seg_compact = seg_compact_fits(kl) && !(prm->flags & VDBR_FLAG_WIDE_SEGMENTS);
+        const size_t seg_rec_bytes = seg_compact ? sizeof(SegCompact::Rec) : sizeof(SegWide::Rec);
         uint64_t seg_count = 0;
         if (srows > 0)
         {
@@ -4794,7 +4874,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
             VDBR_CHECK(ctx, hipEventRecord(ev[8], st));
             for (uint32_t sa = 0;; sa++)
             {
-                segs = (uint4*)ctx->ws(WS_SEGS, SEG_PARTS * part_cap * sizeof(uint4));
+                segs = ctx->ws(WS_SEGS, SEG_PARTS * part_cap * seg_rec_bytes);
                 if (!segs)
                     return -1;
                 VDBR_CHECK(ctx, hipMemsetAsync(seg_counters, 0, SEG_PARTS * SEG_COUNTER_STRIDE * 8, st));
@@ -4808,6 +4888,8 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 ra.vs = vs;
                 ra.slab_i0 = whole ? INT32_MIN : sl.i0;
                 ra.rframes = rframes;
+                ra.kl = kl;
+                ra.seg_compact = seg_compact ? 1u : 0u;
                 hipLaunchKernelGGL(k_row_segments, dim3((uint32_t)blocks_rows), dim3(EMIT_THREADS), 0, st, ra, segs,
                                    part_cap, seg_counters);
                 VDBR_CHECK(ctx, hipGetLastError());
@@ -4888,6 +4970,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 ea.hist_bits = key_bits - sort_begin;
                 ea.hist_w = hist_w;
                 ea.segs = segs;
+                ea.seg_compact = seg_compact ? 1u : 0u;
                 ea.seg_part_cap = part_cap;
                 ea.seg_part_count = part_count_dev;
                 ea.seg_chunk_scan = chunk_scan_dev;
@@ -4901,7 +4984,7 @@ int raster_node2(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_raster_par
                 if (blocks == 0)
                     blocks = 1;
                 VDBR_CHECK(ctx, hipEventRecord(ev[2], st));
-                hipLaunchKernelGGL(k_emit_segments, dim3((uint32_t)blocks), dim3(EMIT_THREADS), 0, st, ea);
+                launch_emit_segments(ea, (uint32_t)blocks, st);
                 VDBR_CHECK(ctx, hipGetLastError());
                 VDBR_CHECK(ctx, hipEventRecord(ev[9], st));
                 VDBR_CHECK(ctx, hipMemcpyAsync(ctx->pinned, counter, 8, hipMemcpyDeviceToHost, st));
@@ -5616,7 +5699,8 @@ static int capture_cloud(Context* w, const CloudPlan& p, const uint32_t* win_bas
     }
     // ---- emission: candidate segments, then the persistent predicate walk
     uint64_t* seg_meta = (uint64_t*)w->ws(WS_SEG_META, (SEG_PARTS * SEG_COUNTER_STRIDE + 2 * (SEG_PARTS + 1)) * 8);
-    uint4* segs = (uint4*)w->ws(WS_SEGS, SEG_PARTS * p.part_cap * sizeof(uint4));
+    const bool seg_compact = seg_compact_fits(p.kl);
+    void* segs = w->ws(WS_SEGS, SEG_PARTS * p.part_cap * (seg_compact ? sizeof(SegCompact::Rec) : sizeof(SegWide::Rec)));
     uint64_t* pairs = (uint64_t*)w->ws(WS_PAIRS, p.pair_cap * sizeof(uint64_t));
     if (!seg_meta || !segs || !pairs)
         return -1;
@@ -5634,6 +5718,8 @@ static int capture_cloud(Context* w, const CloudPlan& p, const uint32_t* win_bas
     ra.total_rows = p.total_rows;
     ra.vs = vs;
     ra.slab_i0 = INT32_MIN;
+    ra.kl = p.kl;
+    ra.seg_compact = seg_compact ? 1u : 0u;
     RowFrame* rframes = (RowFrame*)w->ws(WS_ROWFRAMES, n * sizeof(RowFrame));
     if (!rframes)
         return -1;
@@ -5667,7 +5753,7 @@ static int capture_cloud(Context* w, const CloudPlan& p, const uint32_t* win_bas
     ea.seg_part_cap = p.part_cap;
     ea.seg_part_count = part_count_dev;
     ea.seg_chunk_scan = chunk_scan_dev;
-    hipLaunchKernelGGL(k_emit_segments, dim3(w->emit_blocks), dim3(EMIT_THREADS), 0, st, ea);
+    launch_emit_segments(ea, (uint32_t)w->emit_blocks, st);
     hipLaunchKernelGGL(k_pairs_final, dim3(1), dim3(64), 0, st, (const unsigned long long*)counter, p.pair_cap, npairs,
                        flags);
     VDBR_CHECK(w, hipGetLastError());

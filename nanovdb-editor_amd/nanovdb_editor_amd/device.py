@@ -15,6 +15,7 @@ VALUE_DENSITY = 1
 FLAG_RAW_INPUTS = 1
 FLAG_SPATIAL_ORDER = 2
 FLAG_VOXEL_COORDS = 4
+FLAG_WIDE_SEGMENTS = 8
 
 
 def _ptr(t):
@@ -65,13 +66,14 @@ class Voxelizer:
 
     def raster_node2(self, means, quats, scales, colors, opacities, voxel_size, *, value_mode=VALUE_RGBA8,
                      cull_scale=0.0, raw=False, sh0=None, prim_batch_size=0, ijk_batch_size=0, max_node_count=0,
-                     spatial_order=False, copy_out=True, voxel_coords=False, slab_pairs=0):
+                     spatial_order=False, copy_out=True, voxel_coords=False, slab_pairs=0, wide_segments=False):
         """Gaussians (float32 GPU tensors, reference layouts) -> node2 grid.
 
         With ``raw=True`` the arrays hold opacity logits, unnormalised (w,x,y,z) quaternions and
         log-scales and are preprocessed in place (colors receives 0.28209479*sh0 + 0.5 from
         ``sh0``). With ``voxel_coords=True`` the call also records every value slot's voxel
-        (``last_voxels``). Returns ``(grid_uint8_tensor or None, stats_dict)``.
+        (``last_voxels``). ``wide_segments=True`` keeps the emission's candidate segments in 16-byte
+        records even where 8-byte ones fit (same grid). Returns ``(grid_uint8_tensor or None, stats_dict)``.
         """
         n = means.numel() // 3
         _check_f32("means", means, 3 * n)
@@ -87,7 +89,7 @@ class Voxelizer:
                           _ptr(sh0), n)
         p = VdbrRasterParams(float(voxel_size), float(cull_scale), int(value_mode),
                              (FLAG_RAW_INPUTS if raw else 0) | (FLAG_SPATIAL_ORDER if spatial_order else 0)
-                             | (FLAG_VOXEL_COORDS if voxel_coords else 0),
+                             | (FLAG_VOXEL_COORDS if voxel_coords else 0) | (FLAG_WIDE_SEGMENTS if wide_segments else 0),
                              int(prim_batch_size), int(ijk_batch_size), int(max_node_count), int(slab_pairs))
         grid_dev = ctypes.c_void_p()
         size = ctypes.c_uint64()

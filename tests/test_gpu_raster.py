@@ -58,6 +58,24 @@ def test_node2_bytes_match_oracle_clustered(vox):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("value_mode", [0, 1])
+def test_segment_record_forms_agree(vox, value_mode):
+    # the emission's 8-byte candidate-segment records (the default where the voxel key, the
+    # Gaussian id and the count fit 64 bits) and the 16-byte form give the same grid bytes
+    # (RGBA8 maxima and exact density sums do not depend on the pair order); RGBA8 equals the oracle
+    c = small_clustered(4000, 0.005, 13)
+    compact, st, _ = _run_gpu(vox, c, value_mode=value_mode)
+    wide, wst, _ = _run_gpu(vox, c, value_mode=value_mode, wide_segments=True)
+    print(f"segment records mode={value_mode}", st, wst, flush=True)
+    assert st["pair_count"] == wst["pair_count"] and st["voxel_count"] == wst["voxel_count"]
+    assert np.array_equal(compact, wide)
+    if value_mode == 0:
+        m, op, q, s, col = preprocessed(c)
+        ref, rst = oracle.raster_node2(m, q, s, col, op, c.voxel_size)
+        assert st["pair_count"] == rst["pair_count"]
+        assert np.array_equal(compact, ref)
+
+
 def test_small_prim_batches_and_windows(vox):
     # many prim batches and windows: exercises the windowed node order across batches
     c = small_uniform(1200, 0.01, 21)
