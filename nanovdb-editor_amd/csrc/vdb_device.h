@@ -196,7 +196,11 @@ VDB_DEV float min_sample_dist2(const float vmin[3], const float vmax[3])
         // monotone in it, so that minimum is the rounded square of the sample of least magnitude
         float l1 = w1 * vmax[c] + ow1 * vmin[c];
         float l2 = w2 * vmax[c] + ow2 * vmin[c];
-        const float m = fminf(fminf(fabsf(vmin[c]), fabsf(l1)), fminf(fabsf(l2), fabsf(vmax[c])));
+        // all four are finite: the plain min instructions with |.| operands give the least magnitude
+        // (fminf would first quiet signalling NaNs, one more instruction per operand from memory)
+        float m;
+        asm("v_min3_f32 %0, |%1|, |%2|, |%3|" : "=v"(m) : "v"(vmin[c]), "v"(l1), "v"(l2));
+        asm("v_min_f32_e64 %0, %1, |%2|" : "=v"(m) : "v"(m), "v"(vmax[c]));
         msq[c] = m * m;
     }
     float d = msq[0] + msq[1];
@@ -239,18 +243,21 @@ VDB_DEV bool alpha8_try_fast(const PrimVal& pv, float d, uint32_t* out)
     const float b = 1.f - a;
     const float m = pv.pow_exp * __builtin_amdgcn_logf(b);
     const float p = __builtin_amdgcn_exp2f(m);
-    const float a2 = 1.f - p;
-    const float y = 255.f * fmaxf(0.f, fminf(1.f, a2));
-    const float eps_t = (fabsf(x) + 8.f) * u;              // relative error of t
-    const float db = (eps_t + 3.f * u) * a + 2.f * u;       // |b_ref - b|
-    const float eps_p = (4.f * fabsf(m) + 8.f) * u;         // relative error of p at fixed b
-    // p / b through the hardware reciprocal (1 ulp): a relative error of ~2^-22 in this term is far
-    // inside the factor 2 applied below; b = 0 or a denormal b gives a non-finite or huge bound,
-    // which leaves the decision to the correctly rounded path as the exact quotient would
-    float bound = 255.f * (pv.pow_exp * (p * __builtin_amdgcn_rcpf(b)) * db + eps_p * p + 2.f * u) + 3.1e-5f;
-    bound = 2.f * bound + 1e-6f;
+    const float a2 = 1.f - p; // <= 1 (p >= 0): only the lower clamp can act
+    const float y = 255.f * fmaxf(0.f, a2);
+    // In units of u: eps_t = (|x| + 8) u is the relative error of t, db = (eps_t + 3u) a + 2u =
+    // u ((|x| + 11) a + 2) bounds |b_ref - b|, eps_p = (4 |m| + 8) u the relative error of p at
+    // fixed b, and the bound on |y_ref - y| is 255 (k (p / b) db + eps_p p + 2u) + 3.1e-5, doubled.
+    // The bound's own roundings (fused here) are relative errors of a few u, far inside the
+    // factor 2. p / b through the hardware reciprocal (1 ulp) likewise; b = 0 or a denormal b
+    // gives a non-finite or huge bound, which leaves the decision to the correctly rounded path as
+    // the exact quotient would.
+    const float dbu = __builtin_fmaf(fabsf(x) + 11.f, a, 2.f);
+    const float pbu = __builtin_fmaf(__builtin_fmaf(4.f, fabsf(m), 8.f), p, 2.f);
+    const float inner = __builtin_fmaf(pv.pow_exp * (p * __builtin_amdgcn_rcpf(b)), dbu, pbu);
+    const float bound = __builtin_fmaf(inner, 2.f * 255.f * u, 2.f * 3.1e-5f + 1e-6f);
     const float fl = floorf(y);
-    if (y - fl > bound && (fl + 1.f) - y > bound)
+    if ((y - fl > bound) & ((fl + 1.f) - y > bound)) // both sides in one branch
     {
         *out = (uint32_t)fl;
         return true;
