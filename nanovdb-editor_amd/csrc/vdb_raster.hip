@@ -3275,15 +3275,15 @@ __global__ __launch_bounds__(256) void k_run_keys(const uint64_t* __restrict__ k
 }
 
 // ============================================================================ node order
-// window histogram per level: hist[level * T + t]
+// window histogram per level: hist[level * T + t]. Grid-stride over the device count: the grid is
+// bounded (NODE_GRID_BLOCKS), not sized by the count's capacity
+constexpr uint32_t NODE_GRID_BLOCKS = 2048;
 __global__ __launch_bounds__(256) void k_window_hist(const uint32_t* __restrict__ t, DCount ccount, uint32_t T,
                                                      uint32_t* __restrict__ hist_level)
 {
     const uint64_t count = ccount.get();
-    uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= count)
-        return;
-    atomicAdd(&hist_level[t[x]], 1u);
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < count; x += (uint64_t)gridDim.x * blockDim.x)
+        atomicAdd(&hist_level[t[x]], 1u);
 }
 
 __global__ __launch_bounds__(256) void k_window_totals(const uint32_t* __restrict__ hist, uint32_t T,
@@ -3346,13 +3346,17 @@ __global__ __launch_bounds__(256) void k_parent_idx(const uint32_t* __restrict__
                                                     const uint32_t* __restrict__ parent_node_idx,
                                                     uint32_t* __restrict__ child_parent)
 {
+    // one wave per parent (its children written 64 at a time), grid-stride over the device count
     const uint64_t nparent = cnparent.get();
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nparent)
-        return;
-    uint32_t idx = parent_node_idx ? parent_node_idx[p] : 0u;
-    for (uint32_t c = first[p]; c < first[p + 1]; c++)
-        child_parent[c] = idx;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64u);
+    for (uint64_t p = (uint64_t)blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); p < nparent; p += waves)
+    {
+        const uint32_t idx = parent_node_idx ? parent_node_idx[p] : 0u;
+        const uint32_t c1 = first[p + 1];
+        for (uint32_t c = first[p] + lane; c < c1; c += 64u)
+            child_parent[c] = idx;
+    }
 }
 
 // per node idx: byte size and value count
@@ -4085,12 +4089,15 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
         if (fill_words(ctx, hist, host_count((uint64_t)3 * T), 0u))
             return -1;
         if (nupper.cap)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nupper.cap)), dim3(256), 0, st, upper_t, nupper, T, hist);
+            hipLaunchKernelGGL(k_window_hist, dim3(std::min(blocks_for(nupper.cap), NODE_GRID_BLOCKS)), dim3(256), 0, st,
+                               upper_t, nupper, T, hist);
         if (nlower.cap)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nlower.cap)), dim3(256), 0, st, lower_t, nlower, T,
+            hipLaunchKernelGGL(k_window_hist, dim3(std::min(blocks_for(nlower.cap), NODE_GRID_BLOCKS)), dim3(256), 0, st,
+                               lower_t, nlower, T,
                                hist + T);
         if (nleaf.cap)
-            hipLaunchKernelGGL(k_window_hist, dim3(blocks_for(nleaf.cap)), dim3(256), 0, st, tree_t, nleaf, T,
+            hipLaunchKernelGGL(k_window_hist, dim3(std::min(blocks_for(nleaf.cap), NODE_GRID_BLOCKS)), dim3(256), 0, st,
+                               tree_t, nleaf, T,
                                hist + 2 * T);
         hipLaunchKernelGGL(k_window_totals, dim3(blocks_for(T)), dim3(256), 0, st, hist, T, tot);
         if (scan_u32(ctx, tot, tot_before, (uint64_t)T, false, nullptr))
@@ -4112,8 +4119,11 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
         Level levels[3] = { { nupper, upper_t, nullptr, upper_key, 15u, upper_node_idx },
                             { nlower, lower_t, lower_parent, lower_key, 15u, lower_node_idx },
                             { nleaf, tree_t, leaf_parent, lkeys, 12u, leaf_node_idx } };
+        // key fields as wide as their values: windows t < T, parents < node_count + 1 (none for the
+        // uppers, whose parent is the root), so the sorts take no pass over bits that are all zero
+        const uint32_t tbits = bits_for(T);
         const uint32_t parent_bits = bits_for(node_count + 1);
-        if (20 + parent_bits + 15 > 64)
+        if (tbits + parent_bits + 15 > 64)
             return ctx->fail("raster_node2: node order keys exceed 64 bits");
         for (uint32_t lv = 0; lv < 3; lv++)
         {
@@ -4121,20 +4131,22 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
             if (L.count.cap == 0)
                 continue;
             if (lv == 1)
-                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nupper.cap)), dim3(256), 0, st, upper_first, nupper,
+                hipLaunchKernelGGL(k_parent_idx, dim3(std::min(blocks_for(nupper.cap, 4u), NODE_GRID_BLOCKS)), dim3(256), 0,
+                                   st, upper_first, nupper,
                                    upper_node_idx, lower_parent);
             if (lv == 2)
-                hipLaunchKernelGGL(k_parent_idx, dim3(blocks_for(nlower.cap)), dim3(256), 0, st, lower_first, nlower,
+                hipLaunchKernelGGL(k_parent_idx, dim3(std::min(blocks_for(nlower.cap, 4u), NODE_GRID_BLOCKS)), dim3(256), 0,
+                                   st, lower_first, nlower,
                                    lower_node_idx, leaf_parent);
             if (scan_u32(ctx, hist + lv * T, lev_before, (uint64_t)T, false, nullptr))
                 return -1;
+            const uint32_t pbits = lv == 0 ? 0u : parent_bits;
             hipLaunchKernelGGL(k_node_keys, dim3(blocks_for(L.count.cap)), dim3(256), 0, st, L.t,
-                               lv == 0 ? nullptr : L.parent, L.slot_key, L.count, L.slot_bits, parent_bits, nkeys,
-                               nvals);
-            if (radix_sort_u64(ctx, nkeys, nvals, L.count, 0, 20 + parent_bits + L.slot_bits))
+                               lv == 0 ? nullptr : L.parent, L.slot_key, L.count, L.slot_bits, pbits, nkeys, nvals);
+            if (radix_sort_u64(ctx, nkeys, nvals, L.count, 0, tbits + pbits + L.slot_bits))
                 return -1;
             hipLaunchKernelGGL(k_node_assign, dim3(blocks_for(L.count.cap)), dim3(256), 0, st, nkeys, nvals, L.count,
-                               parent_bits + L.slot_bits, tot_before, lev_before, hist, T, lv, L.node_idx);
+                               pbits + L.slot_bits, tot_before, lev_before, hist, T, lv, L.node_idx);
         }
     }
     else
