@@ -2841,16 +2841,49 @@ __global__ __launch_bounds__(256) void k_leaf_count(const uint32_t* __restrict__
 }
 
 // one wave per leaf: its voxels with a pair, in slot (= key) order, at the leaf's offset
+// Writes one leaf's active voxels (slots holding a lowest Gaussian id) at base in slot order. Each
+// lane stages its up to 8 slots in the wave's LDS rows at its prefix position lpos, then the wave
+// stores 64 consecutive entries per instruction: written from the lanes' own runs, every store
+// would spread over many partly written lines.
+__device__ __forceinline__ void leaf_write_staged(uint32_t lane, const uint32_t g[8], const uint32_t v[8], uint64_t kbase,
+                                                  uint64_t base, uint32_t lpos, uint32_t wtot, uint64_t* s_key,
+                                                  uint32_t* s_val, uint32_t* s_g0, uint64_t* __restrict__ vox_key,
+                                                  uint32_t* __restrict__ vox_val, uint32_t* __restrict__ vox_g0)
+{
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+    {
+        if (g[t] != 0xFFFFFFFFu)
+        {
+            s_key[lpos] = kbase | (uint64_t)(lane * 8u + (uint32_t)t);
+            s_val[lpos] = v[t];
+            s_g0[lpos] = g[t];
+            lpos++;
+        }
+    }
+    wave_sync_lds();
+    for (uint32_t i = lane; i < wtot; i += 64u)
+    {
+        vox_key[base + i] = s_key[i];
+        vox_val[base + i] = s_val[i];
+        vox_g0[base + i] = s_g0[i];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_leaf_write(const uint32_t* __restrict__ lval, const uint32_t* __restrict__ lg0,
                                                     const uint64_t* __restrict__ lkey,
                                                     const uint32_t* __restrict__ lscan, DCount cnleaf,
                                                     uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
                                                     uint32_t* __restrict__ vox_g0)
 {
+    __shared__ uint64_t s_key[4][512];
+    __shared__ uint32_t s_val[4][512];
+    __shared__ uint32_t s_g0[4][512];
     const uint64_t nleaf = cnleaf.get();
-    const uint64_t leaf = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t leaf = (uint64_t)blockIdx.x * 4u + wave;
     if (leaf >= nleaf)
-        return;
+        return; // wave-uniform; the kernel has no block barrier
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t at = leaf * 512u + lane * 8u;
     uint32_t g[8], v[8];
@@ -2869,19 +2902,9 @@ __global__ __launch_bounds__(256) void k_leaf_write(const uint32_t* __restrict__
     for (int t = 0; t < 8; t++)
         cnt += g[t] != 0xFFFFFFFFu ? 1u : 0u;
     uint32_t wtot;
-    uint32_t pos = lscan[leaf] + wave_excl_add(cnt, lane, &wtot);
-    const uint64_t kbase = lkey[leaf] << 9;
-#pragma unroll
-    for (int t = 0; t < 8; t++)
-    {
-        if (g[t] != 0xFFFFFFFFu)
-        {
-            vox_key[pos] = kbase | (uint64_t)(lane * 8u + (uint32_t)t);
-            vox_val[pos] = v[t];
-            vox_g0[pos] = g[t];
-            pos++;
-        }
-    }
+    const uint32_t lpos = wave_excl_add(cnt, lane, &wtot);
+    leaf_write_staged(lane, g, v, lkey[leaf] << 9, lscan[leaf], lpos, wtot, s_key[wave], s_val[wave], s_g0[wave],
+                      vox_key, vox_val, vox_g0);
 }
 
 // ============================================================================ density reduce
@@ -3106,10 +3129,14 @@ __global__ __launch_bounds__(256) void k_leaf_write_density(const unsigned long 
                                                             uint64_t* __restrict__ vox_key, uint32_t* __restrict__ vox_val,
                                                             uint32_t* __restrict__ vox_g0)
 {
+    __shared__ uint64_t s_key[4][512];
+    __shared__ uint32_t s_val[4][512];
+    __shared__ uint32_t s_g0[4][512];
     const uint64_t nleaf = cnleaf.get();
-    const uint64_t leaf = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t leaf = (uint64_t)blockIdx.x * 4u + wave;
     if (leaf >= nleaf)
-        return;
+        return; // wave-uniform; the kernel has no block barrier
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t at = leaf * 512u + lane * 8u;
     uint32_t g[8];
@@ -3124,20 +3151,20 @@ __global__ __launch_bounds__(256) void k_leaf_write_density(const unsigned long 
     for (int t = 0; t < 8; t++)
         cnt += g[t] != 0xFFFFFFFFu ? 1u : 0u;
     uint32_t wtot;
-    uint32_t pos = lscan[leaf] + wave_excl_add(cnt, lane, &wtot);
-    const uint64_t kbase = lkey[leaf] << 9;
+    const uint32_t lpos = wave_excl_add(cnt, lane, &wtot);
+    uint32_t v[8];
 #pragma unroll
     for (int t = 0; t < 8; t++)
     {
+        v[t] = 0u;
         if (g[t] != 0xFFFFFFFFu)
         {
             const unsigned long long* w = lw + (at + (uint64_t)t) * 3u;
-            vox_key[pos] = kbase | (uint64_t)(lane * 8u + (uint32_t)t);
-            vox_val[pos] = __float_as_uint(fix120_to_float(w[0], w[1], w[2]));
-            vox_g0[pos] = g[t];
-            pos++;
+            v[t] = __float_as_uint(fix120_to_float(w[0], w[1], w[2]));
         }
     }
+    leaf_write_staged(lane, g, v, lkey[leaf] << 9, lscan[leaf], lpos, wtot, s_key[wave], s_val[wave], s_g0[wave],
+                      vox_key, vox_val, vox_g0);
 }
 
 // per voxel: first-touch window from the lowest gaussian id, and voxel bounds
@@ -3764,35 +3791,97 @@ static inline uint32_t blocks_for(uint64_t n, uint32_t t = 256)
 // Marks the leaves of every Gaussian's voxel AABB in a bitmap over the leaf index (x, y, z leaf
 // coordinates relative to lmin, z least significant): a leaf column (x, y) of the AABB is one run
 // of consecutive bits. A word already holding the run's bits is not written again.
+// marks the leaves z0..z1 (inclusive) of one leaf column in the candidate bitmap
+__device__ __forceinline__ void mark_leaf_run(unsigned long long* __restrict__ map, uint64_t col, uint32_t z0, uint32_t z1)
+{
+    uint64_t lo = col + z0;
+    const uint64_t hi = col + z1; // inclusive
+    while (lo <= hi)
+    {
+        const uint64_t w = lo >> 6;
+        const uint32_t b0 = (uint32_t)(lo & 63u);
+        const uint32_t b1 = (hi >> 6) == w ? (uint32_t)(hi & 63u) : 63u;
+        const unsigned long long m = (~0ull >> (63u - (b1 - b0))) << b0;
+        if ((map[w] & m) != m)
+            atomicOr(&map[w], m);
+        lo = (w + 1) << 6;
+    }
+}
+
+// columns of one Gaussian above which it marks its own columns (keeps a wave's column total
+// below 2^26)
+constexpr uint64_t LEAF_MARK_SPREAD_MAX = 1ull << 20;
+
 __global__ __launch_bounds__(256) void k_leaf_mark(const PrimBox* __restrict__ boxes, uint64_t n, KeyLayout kl,
                                                    unsigned long long* __restrict__ map)
 {
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x)
+    // a wave takes 64 Gaussians at a time and spreads their leaf columns (x, y) over its lanes:
+    // column t of the group belongs to the lane whose exclusive column prefix is the largest one
+    // <= t (an LDS search), so a large AABB does not hold the other 63 lanes
+    __shared__ uint32_t s_pre[4][64], s_ny[4][64], s_z0[4][64], s_z1[4][64];
+    __shared__ int32_t s_x0[4][64], s_y0[4][64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint64_t waves_total = (uint64_t)gridDim.x * 4u;
+    for (uint64_t g0 = ((uint64_t)blockIdx.x * 4u + wave) * 64u; g0 < n; g0 += waves_total * 64u)
     {
-        const PrimBox b = boxes[g];
-        if (b.rng_x <= 0 || b.rng_y <= 0 || b.rng_z <= 0)
-            continue;
-        const int32_t x0 = (b.mn[0] >> 3) - kl.lmin[0], x1 = ((b.mn[0] + (int32_t)b.rng_x - 1) >> 3) - kl.lmin[0];
-        const int32_t y0 = (b.mn[1] >> 3) - kl.lmin[1], y1 = ((b.mn[1] + (int32_t)b.rng_y - 1) >> 3) - kl.lmin[1];
-        const uint64_t z0 = (uint64_t)((b.mn[2] >> 3) - kl.lmin[2]);
-        const uint64_t z1 = (uint64_t)(((b.mn[2] + (int32_t)b.rng_z - 1) >> 3) - kl.lmin[2]);
-        for (int32_t x = x0; x <= x1; x++)
-            for (int32_t y = y0; y <= y1; y++)
+        const uint64_t g = g0 + lane;
+        uint32_t ncol = 0u, ny = 1u, z0 = 0u, z1 = 0u;
+        int32_t x0 = 0, y0 = 0;
+        uint64_t own_nx = 0u; // a Gaussian of more than LEAF_MARK_SPREAD_MAX columns: marked by its lane
+        if (g < n)
+        {
+            const PrimBox b = boxes[g];
+            if (b.rng_x > 0 && b.rng_y > 0 && b.rng_z > 0)
             {
-                const uint64_t col = ((uint64_t)x << (kl.bits[1] + kl.bits[2])) | ((uint64_t)y << kl.bits[2]);
-                uint64_t lo = col + z0;
-                const uint64_t hi = col + z1; // inclusive
-                while (lo <= hi)
-                {
-                    const uint64_t w = lo >> 6;
-                    const uint32_t b0 = (uint32_t)(lo & 63u);
-                    const uint32_t b1 = (hi >> 6) == w ? (uint32_t)(hi & 63u) : 63u;
-                    const unsigned long long m = (~0ull >> (63u - (b1 - b0))) << b0;
-                    if ((map[w] & m) != m)
-                        atomicOr(&map[w], m);
-                    lo = (w + 1) << 6;
-                }
+                x0 = (b.mn[0] >> 3) - kl.lmin[0];
+                y0 = (b.mn[1] >> 3) - kl.lmin[1];
+                const int32_t x1 = ((b.mn[0] + (int32_t)b.rng_x - 1) >> 3) - kl.lmin[0];
+                const int32_t y1 = ((b.mn[1] + (int32_t)b.rng_y - 1) >> 3) - kl.lmin[1];
+                z0 = (uint32_t)((b.mn[2] >> 3) - kl.lmin[2]);
+                z1 = (uint32_t)(((b.mn[2] + (int32_t)b.rng_z - 1) >> 3) - kl.lmin[2]);
+                const uint64_t nx = (uint64_t)(x1 - x0 + 1);
+                ny = (uint32_t)(y1 - y0 + 1);
+                if (nx * ny > LEAF_MARK_SPREAD_MAX)
+                    own_nx = nx;
+                else
+                    ncol = (uint32_t)(nx * ny);
             }
+        }
+        uint32_t total;
+        const uint32_t pre = wave_excl_add(ncol, lane, &total);
+        s_pre[wave][lane] = pre;
+        s_ny[wave][lane] = ny;
+        s_z0[wave][lane] = z0;
+        s_z1[wave][lane] = z1;
+        s_x0[wave][lane] = x0;
+        s_y0[wave][lane] = y0;
+        wave_sync_lds();
+        for (uint32_t t0 = 0; t0 < total; t0 += 64u)
+        {
+            const uint32_t t = t0 + lane;
+            if (t < total)
+            {
+                uint32_t o = 0u; // largest lane with s_pre <= t (s_pre[0] = 0)
+#pragma unroll
+                for (uint32_t step = 32u; step > 0u; step >>= 1)
+                    if (s_pre[wave][o + step] <= t)
+                        o += step;
+                const uint32_t local = t - s_pre[wave][o];
+                const uint32_t cy = s_ny[wave][o];
+                const uint32_t dx = local / cy;
+                const uint64_t x = (uint64_t)(uint32_t)(s_x0[wave][o] + (int32_t)dx);
+                const uint64_t y = (uint64_t)(uint32_t)(s_y0[wave][o] + (int32_t)(local - dx * cy));
+                mark_leaf_run(map, (x << (kl.bits[1] + kl.bits[2])) | (y << kl.bits[2]), s_z0[wave][o],
+                              s_z1[wave][o]);
+            }
+        }
+        for (uint64_t dx = 0; dx < own_nx; dx++)
+            for (uint32_t dy = 0; dy < ny; dy++)
+            {
+                const uint64_t x = (uint64_t)(uint32_t)(x0 + (int32_t)dx), y = (uint64_t)(uint32_t)(y0 + (int32_t)dy);
+                mark_leaf_run(map, (x << (kl.bits[1] + kl.bits[2])) | (y << kl.bits[2]), z0, z1);
+            }
+        wave_sync_lds(); // the next group overwrites the wave's rows
     }
 }
 
