@@ -239,8 +239,8 @@ __device__ __forceinline__ void setup_one(uint64_t g, const float* __restrict__ 
     {
         for (int c = 0; c < 3; c++)
         {
-            bmn[c] = mn[c];
-            bmx[c] = mx[c];
+            bmn[c] = min(bmn[c], mn[c]);
+            bmx[c] = max(bmx[c], mx[c]);
         }
         // bound on survivors: voxel centres of kept voxels satisfy |A(c - mu)| <= 1 + |h|, h the
         // local half extent of a voxel; volume (4/3) pi (1+|h|)^3 r^3 s_x s_y s_z / vs^3
@@ -258,19 +258,21 @@ __device__ __forceinline__ void setup_one(uint64_t g, const float* __restrict__ 
         double foot = (double)o.foot[g];
         if (est > foot)
             est = foot;
-        *est_sum = (unsigned long long)est;
+        *est_sum += (unsigned long long)est;
     }
 }
 
+constexpr uint32_t SETUP_BLOCKS = 2048;
 __global__ __launch_bounds__(256) void k_setup(uint64_t n, const float* __restrict__ means, const float* __restrict__ quats,
                                                const float* __restrict__ scales, const float* __restrict__ opac,
                                                const float* __restrict__ colors, float vs, float vs_inv,
                                                float cull_scale, SetupOut o)
 {
-    uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // grid-stride on a bounded grid (SETUP_BLOCKS): every block adds its bounds and estimate with
+    // atomics on one cache line, which serialize, so fewer blocks each take more Gaussians
     int32_t bmn[3] = { 2147483647, 2147483647, 2147483647 }, bmx[3] = { -2147483647, -2147483647, -2147483647 };
     unsigned long long est_sum = 0ull;
-    if (g < n)
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x)
         setup_one(g, means, quats, scales, opac, colors, vs, vs_inv, cull_scale, o, bmn, bmx, &est_sum);
     commit_bounds(bmn, bmx, o.bounds);
     commit_sum(est_sum, o.est);
@@ -4415,7 +4417,8 @@ static int launch_setup(Context* ctx, const vdbr_gaussians_t* gs, const vdbr_ras
     if (n > 0)
     {
         SetupOut so{ b.frames, b.boxes, b.pvals, b.foot, b.rows, b.bounds, b.est };
-        hipLaunchKernelGGL(k_setup, dim3(blocks_for(n)), dim3(256), 0, st, n, gs->means, gs->quats, gs->scales,
+        hipLaunchKernelGGL(k_setup, dim3(std::min(blocks_for(n), SETUP_BLOCKS)), dim3(256), 0, st, n, gs->means,
+                           gs->quats, gs->scales,
                            gs->opacities, gs->colors, vs, 1.f / vs, prm->cull_scale, so);
         if (scan_u64(ctx, b.foot, b.foot_scan, n, false, b.foot_scan + n))
             return -1;
@@ -5490,7 +5493,8 @@ int density_backward(Context* ctx, const vdbr_gaussians_t* raw, float vs, const 
     hipLaunchKernelGGL(k_init_bounds, dim3(1), dim3(64), 0, st, bounds);
     VDBR_CHECK(ctx, hipMemsetAsync(est, 0, 8, st));
     SetupOut so{ frames, boxes, pvals, foot, rows, bounds, est };
-    hipLaunchKernelGGL(k_setup, dim3(blocks_for(n)), dim3(256), 0, st, n, raw->means, q_pre, s_pre, o_pre, c_pre, vs,
+    hipLaunchKernelGGL(k_setup, dim3(std::min(blocks_for(n), SETUP_BLOCKS)), dim3(256), 0, st, n, raw->means, q_pre,
+                       s_pre, o_pre, c_pre, vs,
                        1.f / vs, 0.f, so);
     if (scan_u64(ctx, rows, row_scan, n, false, row_scan + n))
         return -1;
