@@ -70,7 +70,6 @@ enum WsSlot
     WS_TILE_SCAN,
     WS_VOX_IJK,
     WS_VOX_G0,
-    WS_VOX_LEAF,
     WS_VOX_GROW,
     WS_SLAB_HIST,
     WS_SLAB_ROWS,

@@ -3310,9 +3310,27 @@ constexpr uint32_t NODE_GRID_BLOCKS = 2048;
 __global__ __launch_bounds__(256) void k_window_hist(const uint32_t* __restrict__ t, DCount ccount, uint32_t T,
                                                      uint32_t* __restrict__ hist_level)
 {
+    // nodes of one window cluster (few windows, many leaves): each wave adds one count per distinct
+    // window among its 64 nodes, so same-address atomics do not serialize per node
     const uint64_t count = ccount.get();
-    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < count; x += (uint64_t)gridDim.x * blockDim.x)
-        atomicAdd(&hist_level[t[x]], 1u);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < count; base += stride)
+    {
+        const uint64_t x = base + lane;
+        const bool valid = x < count;
+        const uint32_t tv = valid ? t[x] : 0u;
+        uint64_t pending = __ballot(valid); // wave-uniform
+        while (pending)
+        {
+            const uint32_t leader = (uint32_t)__builtin_ctzll(pending);
+            const uint32_t tl = (uint32_t)__shfl((int)tv, (int)leader, 64);
+            const uint64_t same = __ballot(valid && tv == tl);
+            if (lane == leader)
+                atomicAdd(&hist_level[tl], (uint32_t)__popcll(same));
+            pending &= ~same;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_window_totals(const uint32_t* __restrict__ hist, uint32_t T,
@@ -3696,57 +3714,49 @@ __global__ __launch_bounds__(256) void k_write_leaves(uint8_t* __restrict__ g, G
                                ((uint64_t)ex[7] << 48);
 }
 
-__global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, GridInfo gi_in, const GridInfo* __restrict__ gip, DCount cnvox,
-                                                      const uint32_t* __restrict__ vox_val,
+__global__ __launch_bounds__(256) void k_write_values(uint8_t* __restrict__ g, GridInfo gi_in, const GridInfo* __restrict__ gip,
+                                                      DCount cnleaf, const uint32_t* __restrict__ vox_val,
                                                       const uint64_t* __restrict__ vox_key,
                                                       const uint32_t* __restrict__ leaf_first,
-                                                      const uint32_t* __restrict__ leaf_of_run_start, // unused
                                                       const uint32_t* __restrict__ leaf_pos,
                                                       const uint32_t* __restrict__ leaf_node_idx,
-                                                      const uint64_t* __restrict__ vals_scan,
-                                                      const uint32_t* __restrict__ vox_leaf, KeyLayout kl,
+                                                      const uint64_t* __restrict__ vals_scan, KeyLayout kl,
                                                       int32_t* __restrict__ ijk_out)
 {
+    // one wave per leaf (grid-stride): its node index and first value index are read once, and its
+    // voxels [leaf_first[l], leaf_first[l + 1]) go to consecutive value slots
     const GridInfo gi = gip ? *gip : gi_in;
-    const uint64_t nvox = cnvox.get();
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (uint64_t)gridDim.x * blockDim.x)
+    const uint64_t nleaf = cnleaf.get();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t waves = (uint64_t)gridDim.x * (blockDim.x / 64u);
+    for (uint64_t l = (uint64_t)blockIdx.x * (blockDim.x / 64u) + (threadIdx.x >> 6); l < nleaf; l += waves)
     {
-        uint32_t l = vox_leaf[v];
-        uint32_t rank = (uint32_t)v - leaf_first[l];
-        uint32_t ni = leaf_node_idx[leaf_pos[l]];
+        const uint32_t b = leaf_first[l], e = leaf_first[l + 1];
+        const uint32_t ni = leaf_node_idx[leaf_pos[l]];
         if (ni >= gi.node_count)
             continue;
-        uint64_t idx = 2ull + vals_scan[ni] + rank;
-        if (idx < gi.value_count)
+        const uint64_t base = 2ull + vals_scan[ni];
+        for (uint32_t v = b + lane; v < e; v += 64u)
         {
-            *(uint32_t*)(g + gi.values + 4ull * idx) = vox_val[v];
-            if (ijk_out) // voxel coordinates in value-index order (VDBR_FLAG_VOXEL_COORDS)
+            const uint64_t idx = base + (v - b);
+            if (idx < gi.value_count)
             {
-                int32_t i, j, k;
-                decode_voxel(kl, vox_key[v], &i, &j, &k);
-                ijk_out[3 * idx + 0] = i;
-                ijk_out[3 * idx + 1] = j;
-                ijk_out[3 * idx + 2] = k;
+                *(uint32_t*)(g + gi.values + 4ull * idx) = vox_val[v];
+                if (ijk_out) // voxel coordinates in value-index order (VDBR_FLAG_VOXEL_COORDS)
+                {
+                    int32_t i, j, k;
+                    decode_voxel(kl, vox_key[v], &i, &j, &k);
+                    ijk_out[3 * idx + 0] = i;
+                    ijk_out[3 * idx + 1] = j;
+                    ijk_out[3 * idx + 2] = k;
+                }
             }
         }
     }
-    (void)leaf_of_run_start;
 }
 
 // voxel -> leaf run index
 // member_of[x] = r for every x of run r; one wave per run (coalesced writes)
-__global__ __launch_bounds__(256) void k_run_member(const uint32_t* __restrict__ first, DCount cnrun,
-                                                    uint32_t* __restrict__ member_of)
-{
-    const uint64_t nrun = cnrun.get();
-    const uint64_t r = (uint64_t)blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (r >= nrun)
-        return; // wave-uniform
-    const uint32_t e = first[r + 1];
-    for (uint32_t x = first[r] + (threadIdx.x & 63u); x < e; x += 64u)
-        member_of[x] = (uint32_t)r;
-}
-
 __global__ void k_gather_u64(const uint64_t* __restrict__ src, uint64_t stride, uint64_t count,
                              uint64_t* __restrict__ dst)
 {
@@ -4069,10 +4079,9 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
     const float vs = in.voxel_size;
     const float vs_inv = 1.f / vs;
     const uint64_t nv1 = nvox.cap > 0 ? nvox.cap : 1;
-    uint32_t* vox_leaf = (uint32_t*)ctx->ws(WS_VOX_LEAF, nv1 * sizeof(uint32_t));
     // device-count form: [0..2] upper, lower, leaf counts, [3] node count
     uint32_t* cnt = devc ? (uint32_t*)ctx->ws(WS_GB_COUNTS, 64) : nullptr;
-    if (!vox_leaf || (devc && !cnt))
+    if (devc && !cnt)
         return -1;
 
     // ---------------------------------------------------------------- leaves / lowers / uppers
@@ -4101,8 +4110,6 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
     {
         hipLaunchKernelGGL(k_leaf_info, dim3((uint32_t)((nleaf.cap + 3) / 4)), dim3(256), 0, st, vox_key, vox_t,
                            leaf_first, nleaf, kl, leaf_mask, leaf_t, lkeys, lvals);
-        hipLaunchKernelGGL(k_run_member, dim3((uint32_t)((nleaf.cap + 3) / 4)), dim3(256), 0, st, leaf_first, nleaf,
-                           vox_leaf);
         if (radix_sort_u64(ctx, lkeys, lvals, nleaf, 0, 42))
             return -1;
         hipLaunchKernelGGL(k_leaf_gather, dim3(blocks_for(nleaf.cap)), dim3(256), 0, st, lkeys, lvals, nleaf, leaf_t,
@@ -4352,12 +4359,11 @@ int build_node2_grid(Context* ctx, const Node2BuildInput& in, Node2BuildOutput* 
         ctx->last_voxel_ijk = ijk_out;
         ctx->last_value_count = 2 + nvox.cap;
     }
-    if (nvox.cap)
-        hipLaunchKernelGGL(k_write_values, dim3(std::min<uint32_t>(blocks_for(nvox.cap), 16384u)), dim3(256), 0, st, grid, gi,
-                           (const GridInfo*)gi_dev, nvox, (const uint32_t*)vox_val, (const uint64_t*)vox_key,
-                           (const uint32_t*)leaf_first, (const uint32_t*)nullptr, (const uint32_t*)leaf_pos,
-                           (const uint32_t*)leaf_node_idx, (const uint64_t*)vals_scan, (const uint32_t*)vox_leaf, kl,
-                           ijk_out);
+    if (nvox.cap && nleaf.cap)
+        hipLaunchKernelGGL(k_write_values, dim3(std::min<uint32_t>(blocks_for(nleaf.cap, 4u), 16384u)), dim3(256), 0, st,
+                           grid, gi, (const GridInfo*)gi_dev, nleaf, (const uint32_t*)vox_val, (const uint64_t*)vox_key,
+                           (const uint32_t*)leaf_first, (const uint32_t*)leaf_pos, (const uint32_t*)leaf_node_idx,
+                           (const uint64_t*)vals_scan, kl, ijk_out);
     VDBR_CHECK(ctx, hipGetLastError());
 
     out->grid = grid;
