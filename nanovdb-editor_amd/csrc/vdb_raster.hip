@@ -3169,6 +3169,21 @@ __global__ __launch_bounds__(256) void k_leaf_write_density(const unsigned long 
                       vox_key, vox_val, vox_g0);
 }
 
+// floor(x / d) for d >= 1. Below 2^52, x and d are exact doubles and x * fl(1/d) is within a
+// relative 2^-52 of x / d, so less than 1 from it: its floor is the quotient or one of its
+// neighbours, and one integer check each way settles it. Larger x take the integer division.
+__device__ __forceinline__ uint64_t div_u64_by_u32(uint64_t x, uint32_t d, double inv_d)
+{
+    if (x >= (1ull << 52))
+        return x / d;
+    uint64_t q = (uint64_t)floor((double)x * inv_d);
+    if (q * d > x)
+        q--;
+    else if ((q + 1u) * d <= x)
+        q++;
+    return q;
+}
+
 // per voxel: first-touch window from the lowest gaussian id, and voxel bounds
 __global__ __launch_bounds__(256) void k_voxel_finish(DCount cnvox, KeyLayout kl, const uint64_t* __restrict__ vox_key,
                                                       const uint32_t* __restrict__ vox_g0, const PrimBox* __restrict__ boxes,
@@ -3178,6 +3193,7 @@ __global__ __launch_bounds__(256) void k_voxel_finish(DCount cnvox, KeyLayout kl
                                                       int32_t* __restrict__ bounds)
 {
     const uint64_t nvox = cnvox.get();
+    const double inv_ijk = 1.0 / (double)ijk_batch; // ijk_batch >= 1
     int32_t mn[3] = { 2147483647, 2147483647, 2147483647 }, mx[3] = { -2147483647, -2147483647, -2147483647 };
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += stride)
@@ -3190,7 +3206,7 @@ __global__ __launch_bounds__(256) void k_voxel_finish(DCount cnvox, KeyLayout kl
                       (uint64_t)(k - bx.mn[2]);
         uint32_t batch = g0 / prim_batch;
         uint64_t slot = foot_scan[g0] - foot_scan[(uint64_t)batch * prim_batch] + ri;
-        vox_t[v] = win_base[batch] + (uint32_t)(slot / ijk_batch);
+        vox_t[v] = win_base[batch] + (uint32_t)div_u64_by_u32(slot, ijk_batch, inv_ijk);
         mn[0] = min(mn[0], i);
         mn[1] = min(mn[1], j);
         mn[2] = min(mn[2], k);
